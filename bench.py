@@ -1,0 +1,171 @@
+#!/usr/bin/env python3
+"""Seconds to 50 PCs of a synthetic cells x 2000-gene matrix on one MI355X.
+
+One step = one full IRLBA solve (k=50, subspace 100, tol 1e-6, seed 0) of
+the device-resident CSR of the named workload (default: 10M cells x 2000
+genes, 100 cell types, ~1.4B nonzeros), inputs generated on the device and
+resident in HBM before the timed region.
+
+The JSON line carries the solve time, the roofline of the dominant kernel
+(algorithmic bytes per launch / its average launch time, from HIP events
+recorded on the kernels' own stream during the timed steps) and a CPU
+baseline: the C++/OpenMP restatement of the reference's parallel IRLBA run
+on the host cores on a bounded row sample of the same matrix, scaled to the
+full matrix by nonzero count.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+
+def algorithmic_bytes(nnz: int, n: int, g: int) -> int:
+    """One sparse product X.v or X^T.u: read each operand once, write each
+    result once: 8 z (values + column indices) + 4 (n+1) (row offsets) +
+    4 n (cell vector) + 8 g (gene vector, read and written)."""
+    return 8 * nnz + 4 * (n + 1) + 4 * n + 8 * g
+
+
+def load_traffic(workload: str, kernel: str):
+    path = os.path.join(ROOT, 'profiles', 'traffic.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        table = json.load(f)
+    entry = table.get(workload, {}).get(kernel)
+    return None if entry is None else entry.get('bytes_per_launch')
+
+
+def cpu_baseline(data, indices, indptr, n_full, nnz_full, g, k, rows):
+    import oracle
+    rows = min(rows, len(indptr) - 1)
+    e = int(indptr[rows])
+    d = np.ascontiguousarray(data[:e])
+    ix = np.ascontiguousarray(indices[:e])
+    ip = np.ascontiguousarray(indptr[:rows + 1]).astype(np.int64)
+    threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
+    threads = max(1, min(threads, os.cpu_count() or 1))
+    t0 = time.perf_counter()
+    _, conv, st = oracle.irlba_port(d, ix, ip, rows, g, k, num_threads=threads)
+    dt = time.perf_counter() - t0
+    scale = nnz_full / max(e, 1)
+    return {'value': round(dt * scale, 3), 'unit': 's (scaled to the full matrix by nnz)',
+            'cores': threads, 'kind': 'port',
+            'sample': f'first {rows} rows ({e} nnz) of the same matrix, full '
+                      f'solve k={k}: {dt:.2f} s, converged={conv}, '
+                      f'restarts={st["restarts"]}, x{scale:.1f} by nnz'}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--gpus', type=int, default=1)
+    ap.add_argument('--steps', type=int, default=3)
+    ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--workload', default='cells10m')
+    ap.add_argument('--k', type=int, default=50)
+    ap.add_argument('--cpu-sample-rows', type=int, default=100_000)
+    ap.add_argument('--no-cpu-baseline', action='store_true')
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f'bench.py runs on one GPU only (asked for {args.gpus}); the '
+              f'multi-GPU sharding of this path is described in DESIGN.md and '
+              f'not built', file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from brisc_amd import Context, irlba_device, synth
+
+    spec = synth.WORKLOADS[args.workload]
+    w = synth.make_workload(**spec)
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    data, indices, indptr = synth.generate_device(w, context=ctx)
+    gen_s = time.perf_counter() - t0
+    n, g, nnz = w.num_cells, w.num_genes, int(data.numel())
+    print(f'# workload {args.workload}: n={n} g={g} nnz={nnz} '
+          f'(generated on device in {gen_s:.2f} s)', file=sys.stderr, flush=True)
+    PCs = torch.empty((n, args.k), dtype=torch.float32, device='cuda')
+
+    for i in range(args.warmup):
+        _, sv, st = irlba_device(data, indices, indptr, n, g, args.k,
+                                 PCs=PCs, context=ctx)
+        print(f'# warmup {i}: {st["seconds"]:.3f} s restarts={st["restarts"]} '
+              f'converged={st["converged"]}', file=sys.stderr, flush=True)
+    ctx.set_profiling(True)
+    ctx.reset_kernel_times()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    stats = []
+    for i in range(args.steps):
+        _, sv, st = irlba_device(data, indices, indptr, n, g, args.k,
+                                 PCs=PCs, context=ctx)
+        stats.append(st)
+        print(f'# step {i}: {st["seconds"]:.3f} s restarts={st["restarts"]} '
+              f'matvecs={st["matvecs"]}', file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    ctx.set_profiling(False)
+    ktimes = ctx.kernel_times()
+
+    sec = elapsed / args.steps
+    bytes_per = algorithmic_bytes(nnz, n, g)
+    kern = {}
+    for name in ('spmv', 'spmv_t'):
+        ms, cnt = ktimes[name]
+        avg = ms / max(cnt, 1)
+        kern[name] = {'avg_ms': round(avg, 4), 'launches': cnt,
+                      'gbs': round(bytes_per / (avg * 1e-3) / 1e9, 1)}
+    dominant = max(('spmv', 'spmv_t'), key=lambda k_: ktimes[k_][0])
+    achieved = kern[dominant]['gbs']
+    roofline = {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
+                'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
+                'traffic': load_traffic(args.workload, dominant),
+                'kernel': dominant, 'bytes_per_launch': bytes_per}
+    total_ms = sum(v[0] for v in ktimes.values())
+    breakdown = {k_: {'ms_per_step': round(v[0] / args.steps, 2),
+                      'share': round(v[0] / total_ms, 4) if total_ms else 0}
+                 for k_, v in ktimes.items()}
+    spmv_ms = ktimes['spmv'][0] + ktimes['spmv_t'][0]
+    spmv_launches = ktimes['spmv'][1] + ktimes['spmv_t'][1]
+
+    baseline = None
+    if not args.no_cpu_baseline:
+        rows = min(args.cpu_sample_rows, n)
+        e = int(indptr[rows].item())
+        baseline = cpu_baseline(data[:e].cpu().numpy(), indices[:e].cpu().numpy(),
+                                indptr[:rows + 1].cpu().numpy(), n, nnz, g,
+                                args.k, rows)
+
+    line = {
+        'metric': 'sec to 50 PCs @10M cells (HBM GB/s % peak); batched PCA solves/s; p50/p99 ms',
+        'value': round(sec, 4), 'unit': 's', 'n_gpus': 1, 'steps': args.steps,
+        'warmup': args.warmup, 'ms_per_step': round(sec * 1e3, 2),
+        'higher_is_better': False, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32 spmv / f64 lanczos', 'data': 'synthetic',
+        'config': {'workload': args.workload, 'num_cells': n, 'num_genes': g,
+                   'nnz': nnz, 'num_types': spec['num_types'], 'k': args.k,
+                   'subspace_size': 100, 'tolerance': 1e-6, 'seed': 0},
+        'roofline': roofline,
+        'cpu_baseline': baseline,
+        'solve': {'restarts': stats[-1]['restarts'],
+                  'matvecs': stats[-1]['matvecs'],
+                  'converged': bool(stats[-1]['converged']),
+                  'svd_seconds': round(stats[-1]['svd_seconds'], 4),
+                  'top_singular_values': [round(float(x), 4) for x in sv[:5]]},
+        'spmv_pair_gbs': round(bytes_per * spmv_launches / (spmv_ms * 1e-3) / 1e9, 1) if spmv_ms else None,
+        'kernels': kern, 'breakdown': breakdown,
+        'generate_seconds': round(gen_s, 3),
+    }
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,120 @@
+"""ctypes binding of libbrisc_pca.so (the C ABI in include/bpca.h).
+
+The library is built in-tree (``brisc_amd/libbrisc_pca.so``).  There is no
+fallback: if it is missing, loading raises ``ImportError`` and every compute
+entry point fails loudly.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, 'libbrisc_pca.so')
+
+BPCA_OK = 0
+BPCA_ERR_INVALID_ARGUMENT = 1
+BPCA_ERR_OUT_OF_MEMORY = 2
+BPCA_ERR_DEVICE = 3
+BPCA_ERR_NONCANONICAL = 4
+BPCA_ERR_NUMERICAL = 5
+BPCA_ERR_CANCELLED = 6
+BPCA_ERR_BUDGET = 7
+BPCA_ERR_NOT_FOUND = 8
+
+
+class BpcaError(RuntimeError):
+    """A device-side failure reported by the library."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(message)
+        self.code = code
+
+
+class BudgetError(BpcaError, MemoryError):
+    """A request does not fit the context's HBM budget."""
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [('converged', ctypes.c_int32),
+                ('restarts', ctypes.c_int32),
+                ('lanczos_steps', ctypes.c_int64),
+                ('matvecs', ctypes.c_int64),
+                ('seconds', ctypes.c_double),
+                ('svd_seconds', ctypes.c_double),
+                ('smax', ctypes.c_double)]
+
+    def as_dict(self) -> dict:
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+KERNEL_CLASSES = ('spmv', 'spmv_t', 'reorth', 'vside', 'restart', 'final',
+                  'setup')
+
+_lib = None
+
+
+def lib() -> ctypes.CDLL:
+    """Load libbrisc_pca.so once and declare the signatures."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f'{LIB_PATH} is not built; build it with '
+            f'`make -C brisc_amd/csrc` (or __graft_entry__.build())')
+    L = ctypes.CDLL(LIB_PATH)
+    p = ctypes.c_void_p
+    i32, u32, u64, i64 = ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64
+    f32 = ctypes.c_float
+    sig = {
+        'bpca_last_error': (ctypes.c_char_p, []),
+        'bpca_version': (i32, []),
+        'bpca_context_create': (i32, [i32, u64, ctypes.POINTER(p)]),
+        'bpca_context_destroy': (i32, [p]),
+        'bpca_context_set_stream': (i32, [p, p]),
+        'bpca_workspace_bytes': (u64, [u64, u32, u32, u32]),
+        'bpca_context_cancel': (i32, [p]),
+        'bpca_context_set_profiling': (i32, [p, i32]),
+        'bpca_context_kernel_times': (i32, [p, p, p]),
+        'bpca_context_reset_kernel_times': (i32, [p]),
+        'bpca_irlba_f32': (i32, [p, p, p, p, p, p, i32, u64, u32, u32, u32,
+                                 f32, u32, u64, i32, i32, u32, u64, p,
+                                 ctypes.POINTER(i32)]),
+        'bpca_irlba_device': (i32, [p, p, p, p, i32, u64, u32, u32, u32, f32,
+                                    u32, u64, i32, p, p,
+                                    ctypes.POINTER(SolveStats)]),
+        'bpca_synth_row_nnz': (i32, [p, u64, u64, u64, u32, p, p, p, p]),
+        'bpca_synth_fill': (i32, [p, u64, u64, u64, u32, p, p, p, p, p, p]),
+        'bpca_normalize_log1ppf': (i32, [p, p, p, u64, p]),
+        'bpca_test_bidiag_svd': (i32, [i32, p, p, p, p]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def last_error() -> str:
+    msg = lib().bpca_last_error()
+    return msg.decode() if msg else ''
+
+
+def check(rc: int) -> None:
+    """Raise the Python exception the reference would raise for ``rc``."""
+    if rc == BPCA_OK:
+        return
+    msg = last_error()
+    if rc == BPCA_ERR_INVALID_ARGUMENT:
+        raise ValueError(msg)
+    if rc == BPCA_ERR_NONCANONICAL:
+        raise ValueError(msg)
+    if rc == BPCA_ERR_OUT_OF_MEMORY:
+        raise MemoryError(msg)
+    if rc == BPCA_ERR_CANCELLED:
+        raise KeyboardInterrupt(msg)
+    if rc == BPCA_ERR_BUDGET:
+        raise BudgetError(rc, msg)
+    raise BpcaError(rc, msg)

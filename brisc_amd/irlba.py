@@ -1,0 +1,205 @@
+"""Python entry points of the IRLBA path.
+
+``irlba`` is the drop-in for ``brisc.pca.irlba`` (``brisc/pca.pyx:661-678``):
+same keyword arguments, same in-place fill of the caller's ``PCs`` array, same
+``bool`` return (converged).  ``SingleCell.pca`` calls it by keyword
+(``brisc/single_cell.py:12082-12093``), so binding this function to the name
+``irlba`` in ``brisc.single_cell`` routes the reference's ``pca()`` through
+the GPU.
+
+``irlba_device`` is the device-resident variant for CSR already in HBM (torch
+tensors on the GPU); it returns the PCs as a device tensor, the singular
+values and the per-solve statistics.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+
+def _as_host(a, dtype=None, name='array'):
+    arr = np.asarray(a)
+    if dtype is not None and arr.dtype != dtype:
+        raise TypeError(f'{name} must have dtype {np.dtype(dtype).name}, got '
+                        f'{arr.dtype.name}')
+    if arr.ndim != 1:
+        raise ValueError(f'{name} must be one-dimensional')
+    return np.ascontiguousarray(arr)
+
+
+def irlba(data_matvec, indices_matvec, indptr_matvec, data_rmatvec,
+          indices_rmatvec, indptr_rmatvec, num_cells, num_genes, k,
+          subspace_size, tolerance, max_iterations, seed, match_parallel,
+          is_csr, num_threads, chunk_size, PCs) -> bool:
+    """Top-k PCs of the implicitly centred and scaled matrix, on the GPU.
+
+    Mirrors ``brisc.pca.irlba`` (``brisc/pca.pyx:661-678``).  The matvec
+    operand is the CSR of X; the rmatvec (CSC) operand is accepted and not
+    used, because ``X^T u`` is computed from the CSR.  ``PCs`` (float32,
+    C-contiguous, ``(num_cells, k)``) is filled in place with ``U_k S_k``.
+    Returns whether all ``k`` singular values converged.
+    """
+    data = _as_host(data_matvec, np.float32, 'data_matvec')
+    indices = _as_host(indices_matvec, name='indices_matvec')
+    indptr = _as_host(indptr_matvec, name='indptr_matvec')
+    if indices.dtype != indptr.dtype or indices.dtype not in (np.int32,
+                                                                np.int64):
+        raise TypeError('indices and indptr must both be int32 or both int64')
+    for name, arr in (('indices_rmatvec', indices_rmatvec),
+                      ('indptr_rmatvec', indptr_rmatvec)):
+        if arr is not None and np.asarray(arr).dtype != indices.dtype:
+            raise TypeError(f'{name} must have the same dtype as '
+                            f'indices_matvec')
+    if not isinstance(PCs, np.ndarray) or PCs.dtype != np.float32 or \
+            not PCs.flags.c_contiguous or PCs.shape != (num_cells, k):
+        raise TypeError('PCs must be a C-contiguous float32 array of shape '
+                        '(num_cells, k)')
+    if len(indptr) != num_cells + 1:
+        raise ValueError('indptr_matvec must have num_cells + 1 entries')
+    if int(indptr[-1]) > len(data) or len(indices) < int(indptr[-1]):
+        raise ValueError('indptr_matvec[-1] exceeds the number of stored '
+                         'elements')
+    conv = ctypes.c_int(0)
+    rc = _lib.lib().bpca_irlba_f32(
+        data.ctypes.data, indices.ctypes.data, indptr.ctypes.data, None, None,
+        None, indices.dtype.itemsize, int(num_cells), int(num_genes), int(k),
+        int(subspace_size), float(tolerance), int(max_iterations),
+        int(seed) & 0xFFFFFFFFFFFFFFFF, int(bool(match_parallel)),
+        int(bool(is_csr)), int(num_threads), int(chunk_size),
+        PCs.ctypes.data, ctypes.byref(conv))
+    _lib.check(rc)
+    return bool(conv.value)
+
+
+class Context:
+    """A device, stream and HBM budget for device-resident solves.
+
+    The context queues its work on the caller's current torch stream so that
+    tensors produced by torch are ready when the kernels read them.
+    """
+
+    def __init__(self, device: int = 0, hbm_budget_bytes: int = 0,
+                 use_torch_stream: bool = True):
+        L = _lib.lib()
+        handle = ctypes.c_void_p()
+        _lib.check(L.bpca_context_create(int(device), int(hbm_budget_bytes),
+                                         ctypes.byref(handle)))
+        self._h = handle
+        self.device = device
+        self.hbm_budget_bytes = hbm_budget_bytes
+        self._use_torch_stream = use_torch_stream
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        if self._h is None:
+            raise RuntimeError('context is closed')
+        if self._use_torch_stream:
+            import torch
+            stream = torch.cuda.current_stream(self.device).cuda_stream
+            _lib.check(_lib.lib().bpca_context_set_stream(self._h, stream))
+        return self._h
+
+    def set_profiling(self, enable: bool = True) -> None:
+        """Record HIP events around each kernel class of later solves."""
+        _lib.check(_lib.lib().bpca_context_set_profiling(self._h,
+                                                         int(enable)))
+
+    def reset_kernel_times(self) -> None:
+        _lib.check(_lib.lib().bpca_context_reset_kernel_times(self._h))
+
+    def kernel_times(self) -> dict:
+        """{class: (total milliseconds, launches)} since the last reset."""
+        n = len(_lib.KERNEL_CLASSES)
+        ms = np.zeros(n, dtype=np.float64)
+        cnt = np.zeros(n, dtype=np.int64)
+        _lib.check(_lib.lib().bpca_context_kernel_times(
+            self._h, ms.ctypes.data, cnt.ctypes.data))
+        return {name: (float(ms[i]), int(cnt[i]))
+                for i, name in enumerate(_lib.KERNEL_CLASSES)}
+
+    def cancel(self) -> None:
+        """Ask a running solve on this context to stop at its next step."""
+        _lib.check(_lib.lib().bpca_context_cancel(self._h))
+
+    def close(self) -> None:
+        if self._h is not None:
+            _lib.lib().bpca_context_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+_default_ctx = {}
+
+
+def default_context(device: int = 0) -> Context:
+    ctx = _default_ctx.get(device)
+    if ctx is None:
+        ctx = _default_ctx[device] = Context(device)
+    return ctx
+
+
+def _check_device_csr(data, indices, indptr, num_cells):
+    import torch
+    if not (data.is_cuda and indices.is_cuda and indptr.is_cuda):
+        raise ValueError('data, indices and indptr must be device tensors')
+    if data.dtype != torch.float32:
+        raise TypeError('data must be float32')
+    if indices.dtype != torch.int32:
+        raise TypeError('indices must be int32')
+    if indptr.dtype not in (torch.int32, torch.int64):
+        raise TypeError('indptr must be int32 or int64')
+    for t, name in ((data, 'data'), (indices, 'indices'), (indptr, 'indptr')):
+        if t.dim() != 1 or not t.is_contiguous():
+            raise ValueError(f'{name} must be a contiguous 1-D tensor')
+    if indptr.numel() != num_cells + 1:
+        raise ValueError('indptr must have num_cells + 1 entries')
+
+
+def irlba_device(data, indices, indptr, num_cells: int, num_genes: int,
+                 k: int = 50, subspace_size: int = 100,
+                 tolerance: float = 1e-6, max_iterations: int = 100,
+                 seed: int = 0, validate: bool = True, PCs=None,
+                 context: Context | None = None):
+    """IRLBA on a device-resident CSR (torch tensors on the GPU).
+
+    Returns ``(PCs, singular_values, stats)``: a ``(num_cells, k)`` float32
+    device tensor of ``U_k S_k``, a float64 NumPy array of the ``k`` singular
+    values and a dict of solve statistics (converged, restarts, matvecs, ...).
+    ``subspace_size`` is clipped to ``min(num_cells, num_genes)`` as
+    ``SingleCell.pca`` does (``single_cell.py:11957-11958``).
+    """
+    import torch
+    _check_device_csr(data, indices, indptr, num_cells)
+    subspace_size = min(int(subspace_size), int(num_cells), int(num_genes))
+    ctx = context or default_context(data.device.index or 0)
+    if PCs is None:
+        PCs = torch.empty((num_cells, k), dtype=torch.float32,
+                          device=data.device)
+    elif PCs.shape != (num_cells, k) or PCs.dtype != torch.float32 or \
+            not PCs.is_contiguous():
+        raise ValueError('PCs must be a contiguous float32 (num_cells, k) '
+                         'tensor')
+    sv = np.zeros(k, dtype=np.float64)
+    st = _lib.SolveStats()
+    rc = _lib.lib().bpca_irlba_device(
+        ctx.handle, data.data_ptr(), indices.data_ptr(), indptr.data_ptr(),
+        indptr.element_size(), int(num_cells), int(num_genes), int(k),
+        int(subspace_size), float(tolerance), int(max_iterations),
+        int(seed) & 0xFFFFFFFFFFFFFFFF, int(bool(validate)), PCs.data_ptr(),
+        sv.ctypes.data, ctypes.byref(st))
+    _lib.check(rc)
+    return PCs, sv, st.as_dict()

@@ -1,0 +1,310 @@
+"""``SingleCell.pca()`` on the GPU: the caller side of the IRLBA path.
+
+``SingleCell`` here is a minimal container (X, obs, var, obsm, uns) whose
+``pca()`` reproduces ``brisc/single_cell.py:11625-12137``: the same signature
+and defaults, the same argument validation and error messages, the QC x HVG
+subset, joint PCA over several datasets, the ``subspace_size`` clip, and the
+output contract -- ``obsm[PC_key]`` is a float32 C-contiguous
+``(len(obs), num_PCs)`` array of scores ``U_k S_k`` with NaN rows for cells
+that fail QC.  ``pca()`` writes nothing to ``varm``, as the reference.
+
+``obs`` and ``var`` are pandas DataFrames or dicts of 1-D arrays (the
+reference uses polars, which this package does not depend on).  QC and HVG
+columns may be a column name, a 1-D boolean array, or a function of the
+dataset returning one, as in the reference.
+"""
+from __future__ import annotations
+
+import os
+from typing import Any
+
+import numpy as np
+import scipy.sparse as sp
+
+from .irlba import irlba
+
+
+def check_type(variable: Any, variable_name: str, expected_types,
+               expected_type_name: str) -> None:
+    """Type check with the reference's message (``brisc/utils.py:326-358``)."""
+    if isinstance(variable, expected_types):
+        return
+    if not isinstance(expected_types, tuple):
+        expected_types = expected_types,
+    for t in expected_types:
+        if t is int and isinstance(variable, np.integer):
+            return
+        if t is float and isinstance(variable, np.floating):
+            return
+        if t is bool and isinstance(variable, np.bool_):
+            return
+    raise TypeError(f'{variable_name} must be {expected_type_name}, but has '
+                    f'type {type(variable).__name__!r}')
+
+
+def check_bounds(variable, variable_name: str, lower_bound=None,
+                 upper_bound=None, *, left_open: bool = False,
+                 right_open: bool = False) -> None:
+    """Bound check with the reference's message (``brisc/utils.py:197-234``)."""
+    if lower_bound is not None and (variable <= lower_bound if left_open
+                                    else variable < lower_bound) or \
+            upper_bound is not None and (variable >= upper_bound if right_open
+                                         else variable > upper_bound):
+        msg = f'{variable_name} is {variable:,}, but must be'
+        if lower_bound is not None:
+            msg += f' {">" if left_open else "≥"} {lower_bound:,}'
+            if upper_bound is not None:
+                msg += ' and'
+        if upper_bound is not None:
+            msg += f' {"<" if right_open else "≤"} {upper_bound:,}'
+        raise ValueError(msg)
+
+
+def _columns(frame) -> dict:
+    if frame is None:
+        return {}
+    if hasattr(frame, 'columns') and hasattr(frame, '__getitem__'):
+        return {c: np.asarray(frame[c]) for c in frame.columns}
+    return {k: np.asarray(v) for k, v in dict(frame).items()}
+
+
+class SingleCell:
+    """Cells x genes counts with obs/var annotations and obsm results."""
+
+    def __init__(self, X=None, obs=None, var=None, obsm=None, varm=None,
+                 uns=None, num_threads: int = 1):
+        if X is not None and not isinstance(X, (sp.csr_array, sp.csc_array,
+                                                sp.csr_matrix, sp.csc_matrix)):
+            raise TypeError('X must be a CSR or CSC sparse array')
+        self._X = X
+        n = X.shape[0] if X is not None else 0
+        g = X.shape[1] if X is not None else 0
+        self._obs = obs if obs is not None else {'cell': np.arange(n)}
+        self._var = var if var is not None else {
+            'gene': np.array([f'gene{i}' for i in range(g)])}
+        self._obsm = dict(obsm or {})
+        self._varm = dict(varm or {})
+        self._uns = dict(uns or {})
+        self._uns.setdefault('normalized', False)
+        self._num_threads = os.cpu_count() if num_threads == -1 else num_threads
+        for key, value in self._obsm.items():
+            if len(value) != n:
+                raise ValueError(f'obsm[{key!r}] has length {len(value)}, '
+                                 f'but there are {n:,} cells')
+
+    X = property(lambda self: self._X)
+    obs = property(lambda self: self._obs)
+    var = property(lambda self: self._var)
+    obsm = property(lambda self: self._obsm)
+    varm = property(lambda self: self._varm)
+    uns = property(lambda self: self._uns)
+    num_threads = property(lambda self: self._num_threads)
+
+    def __len__(self) -> int:
+        return self._X.shape[0] if self._X is not None else \
+            len(next(iter(_columns(self._obs).values()), ()))
+
+    @property
+    def var_names(self) -> np.ndarray:
+        cols = _columns(self._var)
+        return np.asarray(next(iter(cols.values())))
+
+    def _process_num_threads(self, num_threads) -> int:
+        """``single_cell.py:1787-1817``."""
+        if num_threads is None:
+            return self._num_threads
+        check_type(num_threads, 'num_threads', int,
+                   'a positive integer, -1, or None')
+        if num_threads == -1:
+            return os.cpu_count()
+        num_threads = int(num_threads)
+        if num_threads <= 0:
+            raise ValueError(f'num_threads is {num_threads:,}, but must be a '
+                             f'positive integer, -1, or None')
+        return num_threads
+
+    def _get_column(self, where: str, column, name: str, allow_missing: bool,
+                    custom_error: str | None, length: int):
+        if column is None:
+            return None
+        if callable(column):
+            column = column(self)
+        if isinstance(column, str):
+            cols = _columns(self._obs if where == 'obs' else self._var)
+            if column not in cols:
+                if allow_missing:
+                    return None
+                if custom_error is not None:
+                    raise ValueError(custom_error.format(repr(column)))
+                raise ValueError(f'{name} {column!r} is not a column of '
+                                 f'{where}')
+            column = cols[column]
+        column = np.asarray(column)
+        if column.dtype != np.bool_:
+            raise TypeError(f'{name} must be Boolean, but has data type '
+                            f'{str(column.dtype)!r}')
+        if column.ndim != 1 or len(column) != length:
+            raise ValueError(f'{name} has length {len(column):,}, but must '
+                             f'have length {length:,}')
+        return column
+
+    def pca(self, *others: 'SingleCell', QC_column='passed_QC',
+            hvg_column='highly_variable', PC_key: str = 'pca',
+            num_PCs: int = 50, subspace_size: int = 100,
+            tolerance: float = 1e-6, max_iterations: int = 100,
+            chunk_size: int = 1024, seed: int = 0,
+            match_parallel: bool = False, overwrite: bool = False,
+            verbose: bool = False, num_threads: int | None = None):
+        """Top ``num_PCs`` principal components across cells, on the GPU.
+
+        Same contract as ``brisc/single_cell.py:11625-12137``; see the module
+        docstring.  Returns a new ``SingleCell`` (or a tuple, one per dataset,
+        when ``others`` is given) with ``obsm[PC_key]`` added.
+        """
+        if others:
+            for o in others:
+                if not isinstance(o, SingleCell):
+                    raise TypeError('all elements of others must be SingleCell '
+                                    'datasets')
+        datasets = [self] + list(others)
+        check_type(PC_key, 'PC_key', str, 'a string')
+        check_type(overwrite, 'overwrite', bool, 'Boolean')
+        suffix = ' for at least one dataset' if others else ''
+        for dataset in datasets:
+            if not overwrite and PC_key in dataset._obsm:
+                raise ValueError(
+                    f'PC_key {PC_key!r} is already a key of obsm{suffix}; did '
+                    f'you already run PCA()? Set overwrite=True to overwrite.')
+        if any(dataset._X is None for dataset in datasets):
+            raise ValueError(f'X is None{suffix}, so finding principal '
+                             f'components is not possible')
+        n_sets = len(datasets)
+
+        def per_dataset(column):
+            if isinstance(column, (list, tuple)) and not isinstance(column, str):
+                if len(column) != n_sets:
+                    raise ValueError(f'column sequence must have length '
+                                     f'{n_sets}')
+                return list(column)
+            return [column] * n_sets
+
+        QC_columns = [d._get_column('obs', c, 'QC_column', True, None, len(d))
+                      for d, c in zip(datasets, per_dataset(QC_column))]
+        if any(c is not None and not c.any() for c in QC_columns):
+            raise ValueError(f'no cells pass QC{suffix}: QC_column is '
+                             f'all-False')
+        hvg_error = (f'hvg_column {{}} is not a column of var{suffix}; did '
+                     f'you forget to run hvg() (and possibly normalize()) '
+                     f'before PCA()?')
+        hvg_columns = [d._get_column('var', c, 'hvg_column', False, hvg_error,
+                                     d._X.shape[1])
+                       for d, c in zip(datasets, per_dataset(hvg_column))]
+        if not all(dataset._uns['normalized'] for dataset in datasets):
+            raise ValueError(
+                f"PCA() requires normalized counts but uns['normalized'] is "
+                f"False{suffix}; did you forget to run normalize() before "
+                f"PCA()?")
+        for dataset in datasets:
+            if dataset._X.dtype != np.float32:
+                raise TypeError(
+                    f'PCA() requires normalized counts with data type '
+                    f'float32, but X has data type '
+                    f'{str(dataset._X.dtype)!r}{suffix}; did you forget to '
+                    f'run normalize() before PCA()?')
+        check_type(num_PCs, 'num_PCs', int, 'a positive integer')
+        check_bounds(num_PCs, 'num_PCs', 1)
+        check_type(subspace_size, 'subspace_size', int, 'a positive integer')
+        if subspace_size < num_PCs:
+            raise ValueError(f'subspace_size is {subspace_size:,}, but must be '
+                             f'≥ num_PCs ({num_PCs:,})')
+        check_type(tolerance, 'tolerance', (int, float), 'a positive number')
+        check_bounds(tolerance, 'tolerance', 0, left_open=True)
+        check_type(max_iterations, 'max_iterations', int,
+                   'a positive integer')
+        check_bounds(max_iterations, 'max_iterations', 1)
+        check_type(chunk_size, 'chunk_size', int, 'a positive integer')
+        check_bounds(chunk_size, 'chunk_size', 1)
+        check_type(seed, 'seed', int, 'an integer')
+        check_type(verbose, 'verbose', bool, 'Boolean')
+        num_threads = self._process_num_threads(num_threads)
+        check_type(match_parallel, 'match_parallel', bool, 'Boolean')
+        if match_parallel and num_threads != 1:
+            raise ValueError(
+                'match_parallel must be False unless num_threads is 1')
+
+        # QC x HVG subset; several datasets are aligned on shared gene names.
+        if others:
+            names = [d.var_names[h] if h is not None else d.var_names
+                     for d, h in zip(datasets, hvg_columns)]
+            common = sorted(set(names[0]).intersection(*names[1:]))
+            gene_indices = []
+            for d, h in zip(datasets, hvg_columns):
+                pos = {name: i for i, name in enumerate(d.var_names)}
+                idx = np.array([pos[name] for name in common], dtype=np.int64)
+                gene_indices.append(idx)
+        else:
+            gene_indices = [np.flatnonzero(h) if h is not None else None
+                            for h in hvg_columns]
+        Xs = []
+        for d, q, genes in zip(datasets, QC_columns, gene_indices):
+            X = sp.csr_array(d._X)
+            if q is not None:
+                X = X[np.flatnonzero(q)]
+            if genes is not None:
+                X = X[:, genes]
+            Xs.append(X)
+        num_cells_per_dataset = np.array([X.shape[0] for X in Xs])
+        X = Xs[0] if len(Xs) == 1 else sp.vstack(Xs, format='csr')
+        del Xs
+        num_cells, num_genes = X.shape
+        if num_cells < num_PCs:
+            raise ValueError(f'num_PCs is {num_PCs:,}, but must be ≤ the '
+                             f'number of cells ({num_cells:,})')
+        if num_genes < num_PCs:
+            raise ValueError(f'num_PCs is {num_PCs:,}, but must be ≤ the '
+                             f'number of genes ({num_genes:,})')
+        if num_cells == 1:
+            raise ValueError('there is only one cell, so principal components '
+                             'cannot be calculated')
+        if num_genes == 1:
+            raise ValueError('there is only one gene, so principal components '
+                             'cannot be calculated')
+        subspace_size = min(subspace_size, num_cells, num_genes)
+
+        # The GPU path takes the CSR of X for both products; canonical rows.
+        X = sp.csr_array(X)
+        X.sum_duplicates()
+        X.sort_indices()
+        index_dtype = np.int64 if X.nnz > np.iinfo(np.int32).max else np.int32
+        PCs = np.empty((num_cells, num_PCs), dtype=np.float32)
+        converged = irlba(
+            data_matvec=np.ascontiguousarray(X.data, dtype=np.float32),
+            indices_matvec=X.indices.astype(index_dtype, copy=False),
+            indptr_matvec=X.indptr.astype(index_dtype, copy=False),
+            data_rmatvec=None, indices_rmatvec=None, indptr_rmatvec=None,
+            num_cells=num_cells, num_genes=num_genes, k=num_PCs,
+            subspace_size=subspace_size, tolerance=tolerance,
+            max_iterations=max_iterations, seed=seed,
+            match_parallel=match_parallel, is_csr=True,
+            num_threads=num_threads, chunk_size=chunk_size, PCs=PCs)
+        if not converged and verbose:
+            print(f'PCA did not converge to a tolerance of {tolerance:.2g} '
+                  f'after {max_iterations:,} iterations; consider increasing '
+                  f'max_iterations or tolerance')
+
+        out = []
+        start = 0
+        for d, q, n in zip(datasets, QC_columns, num_cells_per_dataset):
+            part = PCs[start:start + n]
+            start += n
+            if q is not None:
+                full = np.full((len(d), num_PCs), np.nan, dtype=np.float32)
+                full[q] = part
+                part = full
+            elif len(datasets) > 1:
+                part = np.ascontiguousarray(part)
+            out.append(SingleCell(X=d._X, obs=d._obs, var=d._var,
+                                  obsm=d._obsm | {PC_key: part},
+                                  varm=d._varm, uns=d._uns,
+                                  num_threads=d._num_threads))
+        return out[0] if not others else tuple(out)

@@ -1,0 +1,168 @@
+/*
+ * bpca.h -- C ABI of the MI355X sparse-PCA (IRLBA) library, libbrisc_pca.so.
+ *
+ * The library computes the top-k principal components of the implicitly
+ * centred and scaled cells x genes matrix Z = (X - 1 mu^T) diag(1/sd), where X
+ * is a CSR float32 matrix of log-normalised counts, by the implicitly
+ * restarted Lanczos bidiagonalisation the reference implements in
+ * brisc/pca.pyx:661-1257.  X.v and X^T.u stream FP32 values with FP32
+ * accumulation; reorthogonalisation, restart and the small SVD run in FP64.
+ * Every reduction has a fixed order keyed to data blocks, so results are
+ * bit-identical from run to run and across launch shapes.
+ *
+ * Conventions: every entry point returns a status code (BPCA_OK on success)
+ * and never throws across the ABI; bpca_last_error() returns a message for
+ * the calling thread's last failure.  Pointers named d_* are device pointers
+ * on the context's device; all other pointers are host pointers.  Calls are
+ * synchronous: when they return, outputs are complete.
+ */
+#ifndef BPCA_H
+#define BPCA_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum bpca_status {
+    BPCA_OK = 0,
+    BPCA_ERR_INVALID_ARGUMENT = 1, /* ValueError/TypeError in pca() */
+    BPCA_ERR_OUT_OF_MEMORY = 2,    /* MemoryError (bad_alloc via except +) */
+    BPCA_ERR_DEVICE = 3,           /* HIP runtime failure */
+    BPCA_ERR_NONCANONICAL = 4,     /* CSR row indices not strictly increasing
+                                      or out of range */
+    BPCA_ERR_NUMERICAL = 5,        /* small SVD did not converge */
+    BPCA_ERR_CANCELLED = 6,        /* KeyboardInterrupt (PyErr_CheckSignals) */
+    BPCA_ERR_BUDGET = 7,           /* request larger than the HBM budget */
+    BPCA_ERR_NOT_FOUND = 8         /* unknown request id */
+};
+
+/* Message describing the calling thread's most recent failure. */
+const char *bpca_last_error(void);
+
+/* Library version, major*10000 + minor*100 + patch. */
+int bpca_version(void);
+
+/* ---------------------------------------------------------------- context */
+
+/* Device, stream, HBM budget and reusable workspace.  No counterpart in the
+ * reference, which has no device; it carries the GPU-specific knobs that are
+ * kept out of pca()'s signature. */
+typedef struct bpca_context bpca_context;
+
+int bpca_context_create(int device, uint64_t hbm_budget_bytes,
+                        bpca_context **out);
+int bpca_context_destroy(bpca_context *ctx);
+/* Queue all work of this context on `hip_stream` (a hipStream_t; NULL = the
+ * context's own stream). */
+int bpca_context_set_stream(bpca_context *ctx, void *hip_stream);
+/* Request that the solve running on this context stop; it returns
+ * BPCA_ERR_CANCELLED at its next Lanczos step (the role of the reference's
+ * PyErr_CheckSignals per step, pca.pyx:1173-1174). */
+int bpca_context_cancel(bpca_context *ctx);
+
+/* Kernel timing with HIP events recorded on the context's stream around each
+ * class of launches.  Classes: */
+enum bpca_kernel_class {
+    BPCA_K_SPMV = 0,     /* u = X v'                 */
+    BPCA_K_SPMV_T = 1,   /* partials of X^T u        */
+    BPCA_K_REORTH = 2,   /* U-side CGS, mean, alpha  */
+    BPCA_K_VSIDE = 3,    /* group sums, V-side CGS   */
+    BPCA_K_RESTART = 4,  /* U, V restart products    */
+    BPCA_K_FINAL = 5,    /* PCs = U (Ub S)           */
+    BPCA_K_SETUP = 6,    /* validation, column stats */
+    BPCA_NUM_KERNEL_CLASSES = 7
+};
+int bpca_context_set_profiling(bpca_context *ctx, int enable);
+/* Accumulated milliseconds and launch counts per class since the last reset
+ * (arrays of BPCA_NUM_KERNEL_CLASSES entries). */
+int bpca_context_kernel_times(bpca_context *ctx, double *total_ms,
+                              int64_t *launches);
+int bpca_context_reset_kernel_times(bpca_context *ctx);
+
+/* Bytes of device workspace a solve of this shape allocates. */
+uint64_t bpca_workspace_bytes(uint64_t num_cells, uint32_t num_genes,
+                              uint32_t k, uint32_t subspace_size);
+
+/* Per-solve statistics. */
+typedef struct bpca_solve_stats {
+    int32_t converged;      /* all k Ritz values met the tolerance */
+    int32_t restarts;       /* IRLBA iterations run (1 = no restart) */
+    int64_t lanczos_steps;  /* Lanczos steps over all iterations */
+    int64_t matvecs;        /* X.v plus X^T.u products (2 per step) */
+    double seconds;         /* wall time of the device-resident solve */
+    double svd_seconds;     /* host time in the small FP64 SVDs */
+    double smax;            /* largest Ritz value seen (convergence scale) */
+} bpca_solve_stats;
+
+/* ------------------------------------------------------- single solve */
+
+/* Drop-in for brisc.pca.irlba (brisc/pca.pyx:661-678).
+ *
+ * Same arguments in the same order and with the same meaning; the index
+ * arrays are all int32 (index_width 4) or all int64 (index_width 8), as the
+ * reference's fused type `signed_integer` requires (cyutils.pxd:25-27).
+ * The matvec operand must be the CSR of X (as SingleCell.pca passes it for
+ * the parallel path, single_cell.py:12037-12069); the rmatvec (CSC) operand
+ * may be NULL because X^T.u is computed from the CSR.  match_parallel,
+ * is_csr, num_threads and chunk_size are validated and otherwise do not
+ * change the result.  PCs (num_cells x k, float32, C order, caller-owned)
+ * receives U_k S_k; *converged receives the return value of the reference
+ * function.  Runs on the process-wide default context (current device). */
+int bpca_irlba_f32(const float *data_matvec, const void *indices_matvec,
+                   const void *indptr_matvec, const float *data_rmatvec,
+                   const void *indices_rmatvec, const void *indptr_rmatvec,
+                   int index_width, uint64_t num_cells, uint32_t num_genes,
+                   uint32_t k, uint32_t subspace_size, float tolerance,
+                   uint32_t max_iterations, uint64_t seed, int match_parallel,
+                   int is_csr, uint32_t num_threads, uint64_t chunk_size,
+                   float *PCs, int *converged);
+
+/* Device-resident variant of bpca_irlba_f32 for data already in HBM.
+ * d_indices are int32 column indices, d_indptr int32 or int64 (indptr_width
+ * 4 or 8) row offsets with d_indptr[0] == 0.  d_PCs: num_cells x k float32,
+ * C order.  singular_values (host, k doubles) and stats may be NULL.
+ * validate != 0 checks on the device that each row's column indices are
+ * strictly increasing and below num_genes (BPCA_ERR_NONCANONICAL if not). */
+int bpca_irlba_device(bpca_context *ctx, const float *d_data,
+                      const int32_t *d_indices, const void *d_indptr,
+                      int indptr_width, uint64_t num_cells, uint32_t num_genes,
+                      uint32_t k, uint32_t subspace_size, float tolerance,
+                      uint32_t max_iterations, uint64_t seed, int validate,
+                      float *d_PCs, double *singular_values,
+                      bpca_solve_stats *stats);
+
+/* ------------------------------------------------ synthetic workloads */
+
+/* Device generator of the synthetic NB count workloads (no counterpart in the
+ * reference; benchmark input).  Rows [row0, row0+num_rows) of the workload
+ * with per-type gene means type_gene_mean (num_types x num_genes, float64),
+ * per-row cell_type (int32) and cell_factor (float64), all device arrays
+ * indexed from row0.  synth_row_nnz writes the nonzero count of each row;
+ * synth_fill writes int32 column indices and int32 counts at the offsets
+ * d_indptr (int64, num_rows+1). */
+int bpca_synth_row_nnz(bpca_context *ctx, uint64_t seed, uint64_t row0,
+                       uint64_t num_rows, uint32_t num_genes,
+                       const double *d_type_gene_mean,
+                       const int32_t *d_cell_type, const double *d_cell_factor,
+                       int64_t *d_row_nnz);
+int bpca_synth_fill(bpca_context *ctx, uint64_t seed, uint64_t row0,
+                    uint64_t num_rows, uint32_t num_genes,
+                    const double *d_type_gene_mean, const int32_t *d_cell_type,
+                    const double *d_cell_factor, const int64_t *d_indptr,
+                    int32_t *d_indices, int32_t *d_counts);
+
+/* log1pPF normalisation of integer counts on the device
+ * (brisc/normalize.pyx:9-128, method 'log1pPF', no QC column):
+ * x' = log1p(x * (mean_row_sum / row_sum)) in float32. */
+int bpca_normalize_log1ppf(bpca_context *ctx, const int32_t *d_counts,
+                           const int64_t *d_indptr, uint64_t num_cells,
+                           float *d_out);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* BPCA_H */

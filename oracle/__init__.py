@@ -1,0 +1,221 @@
+"""ORACLE -- test infrastructure only.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this package, and only as the checker (or the timed
+CPU baseline).  The product library never links or calls it.
+
+Contents:
+
+* ``generate_csr`` -- CPU generator of the synthetic NB count workloads
+  (``gen.c``), bit-identical in its integer counts to the device generator,
+  followed by the log1pPF restatement of ``brisc/normalize.pyx:9-128``.
+* ``irlba_port`` -- FP32 restatement of the reference's parallel IRLBA path
+  (``brisc/pca.pyx:973-1254``) calling the same SciPy LAPACK ``sgesdd`` and
+  BLAS ``sgemm`` the reference calls.
+* ``fp64_pca`` -- the FP64 oracle: exact Gram ``Z^T Z`` of the implicitly
+  centred and scaled matrix, diagonalised with ``numpy.linalg.eigh``; gives
+  singular values, right singular vectors and PCs (``Z V_k``).
+* ``clipped_stddev`` -- closed form of ``clipped_stddev_csc``
+  (``brisc/pca.pyx:51-103``) in FP64.
+
+Pinning (what fixes each function independently of its own formula):
+golden vectors written by the reference itself (``tests/golden``), a dense
+FP64 SVD on tiny inputs, SVD identities, SciPy sparse products in FP64, and
+exact integer checks of the generator (row sums, SHA-256 of the arrays).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = None
+
+
+def build() -> None:
+    """Compile liboracle.so with the committed Makefile."""
+    import subprocess
+    subprocess.run(['make', '-s', '-C', _HERE], check=True)
+
+
+def lib() -> ctypes.CDLL:
+    global _LIB
+    if _LIB is None:
+        path = os.path.join(_HERE, 'liboracle.so')
+        if not os.path.exists(path):
+            build()
+        L = ctypes.CDLL(path)
+        p = ctypes.c_void_p
+        u64, u32, i64 = ctypes.c_uint64, ctypes.c_uint32, ctypes.c_int64
+        L.gen_row_nnz.argtypes = [u64, u64, u64, u32, p, p, p, p]
+        L.gen_fill.argtypes = [u64, u64, u64, u32, p, p, p, p, p, p]
+        L.normalize_log1ppf.argtypes = [p, p, u64, p, p]
+        L.gram_f64.argtypes = [p, p, p, i64, ctypes.c_int32, p, p]
+        L.gram_f64.restype = ctypes.c_int
+        L.irlba_port.argtypes = [p, p, p, u64, u32, u32, u32, ctypes.c_float,
+                                 u32, u64, u32, u64, p, p, p, p, p]
+        L.irlba_port.restype = ctypes.c_int
+        _LIB = L
+    return _LIB
+
+
+def _ptr(a: np.ndarray) -> int:
+    return a.ctypes.data
+
+
+def generate_counts(w, row0: int = 0, num_rows: int | None = None):
+    """Integer counts of rows [row0, row0+num_rows) of workload ``w``.
+
+    Returns ``(indptr int64, indices int32, counts int32)``.
+    """
+    L = lib()
+    if num_rows is None:
+        num_rows = w.num_cells - row0
+    ct = np.ascontiguousarray(w.cell_type[row0:row0 + num_rows])
+    cf = np.ascontiguousarray(w.cell_factor[row0:row0 + num_rows])
+    tgm = np.ascontiguousarray(w.type_gene_mean)
+    row_nnz = np.empty(num_rows, dtype=np.int64)
+    L.gen_row_nnz(w.seed, row0, num_rows, w.num_genes, _ptr(tgm), _ptr(ct),
+                  _ptr(cf), _ptr(row_nnz))
+    indptr = np.zeros(num_rows + 1, dtype=np.int64)
+    np.cumsum(row_nnz, out=indptr[1:])
+    nnz = int(indptr[-1])
+    indices = np.empty(nnz, dtype=np.int32)
+    counts = np.empty(nnz, dtype=np.int32)
+    L.gen_fill(w.seed, row0, num_rows, w.num_genes, _ptr(tgm), _ptr(ct),
+               _ptr(cf), _ptr(indptr), _ptr(indices), _ptr(counts))
+    return indptr, indices, counts
+
+
+def log1ppf(counts: np.ndarray, indptr: np.ndarray) -> np.ndarray:
+    """log1pPF of integer CSR counts (brisc/normalize.pyx:9-128)."""
+    n = len(indptr) - 1
+    out = np.empty(len(counts), dtype=np.float32)
+    row_sums = np.empty(n, dtype=np.uint64)
+    lib().normalize_log1ppf(_ptr(np.ascontiguousarray(counts, np.int32)),
+                            _ptr(np.ascontiguousarray(indptr, np.int64)), n,
+                            _ptr(out), _ptr(row_sums))
+    return out
+
+
+def generate_csr(w):
+    """Normalised CSR (data float32, indices int32, indptr int64) of ``w``."""
+    indptr, indices, counts = generate_counts(w)
+    return log1ppf(counts, indptr), indices, indptr
+
+
+def _capsule_pointer(capsule) -> int:
+    get_name = ctypes.pythonapi.PyCapsule_GetName
+    get_name.restype = ctypes.c_char_p
+    get_name.argtypes = [ctypes.py_object]
+    get_ptr = ctypes.pythonapi.PyCapsule_GetPointer
+    get_ptr.restype = ctypes.c_void_p
+    get_ptr.argtypes = [ctypes.py_object, ctypes.c_char_p]
+    return get_ptr(capsule, get_name(capsule))
+
+
+def scipy_blas_lapack():
+    """Addresses of SciPy's Fortran sgesdd and sgemm (as the reference uses)."""
+    from scipy.linalg import cython_blas, cython_lapack
+    return (_capsule_pointer(cython_lapack.__pyx_capi__['sgesdd']),
+            _capsule_pointer(cython_blas.__pyx_capi__['sgemm']))
+
+
+def irlba_port(data, indices, indptr, num_cells, num_genes, k,
+               subspace_size=100, tolerance=1e-6, max_iterations=100, seed=0,
+               num_threads=None, chunk_size=1024):
+    """Run the FP32 restatement of the reference parallel IRLBA.
+
+    Returns ``(PCs, converged, stats)`` with ``stats`` = dict(restarts,
+    matvecs, singular_values).
+    """
+    if num_threads is None:
+        num_threads = os.cpu_count() or 1
+    subspace_size = min(subspace_size, num_cells, num_genes)
+    data = np.ascontiguousarray(data, np.float32)
+    indices = np.ascontiguousarray(indices, np.int32)
+    indptr = np.ascontiguousarray(indptr, np.int64)
+    PCs = np.empty((num_cells, k), dtype=np.float32)
+    stats = np.zeros(3, dtype=np.int64)
+    sv = np.zeros(k, dtype=np.float32)
+    sgesdd, sgemm = scipy_blas_lapack()
+    conv = lib().irlba_port(_ptr(data), _ptr(indices), _ptr(indptr),
+                            num_cells, num_genes, k, subspace_size,
+                            tolerance, max_iterations, seed, num_threads,
+                            chunk_size, _ptr(PCs), sgesdd, sgemm, _ptr(stats),
+                            _ptr(sv))
+    return PCs, bool(conv), dict(restarts=int(stats[0]),
+                                 matvecs=int(stats[1]), singular_values=sv)
+
+
+def clipped_stddev(data, indices, indptr, num_cells, num_genes,
+                   clip=1e-8):
+    """FP64 per-gene sample std (ddof=1) clipped below at ``clip``."""
+    s = np.bincount(indices, weights=data.astype(np.float64),
+                    minlength=num_genes)
+    ss = np.bincount(indices, weights=data.astype(np.float64) ** 2,
+                     minlength=num_genes)
+    var = (ss - s * s / num_cells) / (num_cells - 1)
+    return np.maximum(np.sqrt(np.maximum(var, 0.0)), clip)
+
+
+def gram(data, indices, indptr, num_genes):
+    """FP64 (X^T X, X^T 1) of a CSR float32 matrix."""
+    n = len(indptr) - 1
+    G = np.empty((num_genes, num_genes), dtype=np.float64)
+    cs = np.empty(num_genes, dtype=np.float64)
+    rc = lib().gram_f64(_ptr(np.ascontiguousarray(data, np.float32)),
+                        _ptr(np.ascontiguousarray(indices, np.int32)),
+                        _ptr(np.ascontiguousarray(indptr, np.int64)), n,
+                        num_genes, _ptr(G), _ptr(cs))
+    if rc != 0:
+        raise MemoryError('gram_f64 allocation failed')
+    return G, cs
+
+
+def fp64_pca(data, indices, indptr, num_cells, num_genes, k,
+             want_pcs=True):
+    """FP64 oracle of the top-k PCA of Z = (X - 1 mu^T) diag(1/sd).
+
+    Returns dict(singular_values (k,), right (g,k), PCs (n,k) or None, sd).
+    Singular values are sqrt of the top eigenvalues of the exact FP64 Gram
+    Z^T Z = D^-1 (X^T X - n mu mu^T) D^-1.  PCs = Z V_k (= U_k S_k).
+    """
+    G, cs = gram(data, indices, indptr, num_genes)
+    sd = clipped_stddev(data, indices, indptr, num_cells, num_genes)
+    mu = cs / num_cells
+    C = (G - num_cells * np.outer(mu, mu)) / np.outer(sd, sd)
+    evals, evecs = np.linalg.eigh(C)
+    order = np.argsort(evals)[::-1][:k]
+    lam = np.maximum(evals[order], 0.0)
+    Vk = evecs[:, order]
+    out = dict(singular_values=np.sqrt(lam), right=Vk, sd=sd, PCs=None)
+    if want_pcs:
+        import scipy.sparse as sp
+        X = sp.csr_array((data.astype(np.float64), indices, indptr),
+                         shape=(num_cells, num_genes))
+        W = Vk / sd[:, None]
+        out['PCs'] = X @ W - (mu @ W)[None, :]
+    return out
+
+
+def dense_z(data, indices, indptr, num_cells, num_genes):
+    """Dense FP64 Z = (X - 1 mu^T) diag(1/sd) for tiny brute-force checks."""
+    X = np.zeros((num_cells, num_genes), dtype=np.float64)
+    for i in range(num_cells):
+        for e in range(indptr[i], indptr[i + 1]):
+            X[i, indices[e]] += data[e]
+    sd = clipped_stddev(data, indices, indptr, num_cells, num_genes)
+    return (X - X.mean(0)) / sd
+
+
+def compare_up_to_sign(A: np.ndarray, B: np.ndarray) -> np.ndarray:
+    """Per-column relative error of A vs B after aligning each column's sign."""
+    A = np.asarray(A, np.float64)
+    B = np.asarray(B, np.float64)
+    sign = np.sign(np.sum(A * B, axis=0))
+    sign[sign == 0] = 1
+    diff = np.linalg.norm(A * sign - B, axis=0)
+    return diff / np.maximum(np.linalg.norm(B, axis=0), 1e-300)

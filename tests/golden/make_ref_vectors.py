@@ -1,0 +1,186 @@
+"""Write the reference's own IRLBA / log1pPF outputs as small fixtures.
+
+Run in the development container only (it reads the reference sources):
+
+    python tests/golden/make_ref_vectors.py [--reference /root/reference]
+
+It cythonizes ``brisc/cyutils.pyx``, ``brisc/pca.pyx`` and
+``brisc/normalize.pyx`` from the reference tree into the git-ignored scratch
+directory ``tests/golden/_refbuild`` (never shipped anywhere), compiles them
+with the reference's Linux flags (``setup.py:93-98``) and compiler directives
+(``setup.py:148-151``), imports them under the package name ``brisc``, and
+records what only the reference itself can pin:
+
+* the exact FP32 output of ``irlba`` (parallel path, ``pca.pyx:973-1254``) --
+  full PCs for tiny inputs, singular values (column norms of the PCs) and 64
+  PC rows for the 50k-cell workload, plus the converged flag;
+* the exact bytes of ``normalize_csr`` (log1pPF, ``normalize.pyx:9-128``) as a
+  SHA-256 of the normalised data.
+
+The inputs are regenerated deterministically by the tests from the seeds
+stored beside the outputs, so no reference code or large array is committed.
+"""
+from __future__ import annotations
+
+import argparse
+import importlib
+import json
+import os
+import subprocess
+import sys
+import sysconfig
+import time
+import types
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+
+from brisc_amd import synth  # noqa: E402
+import oracle  # noqa: E402
+
+BUILD = os.path.join(HERE, '_refbuild')
+MODULES = ('cyutils', 'pca', 'normalize')
+
+# Tiny inputs: (name, seed, cells, genes, types, density, k, subspace, chunk)
+TINY = [
+    ('tiny_a', 11, 300, 40, 4, 0.30, 5, 12, 1024),
+    ('tiny_b', 12, 1000, 120, 6, 0.15, 10, 25, 1024),
+    ('tiny_c', 13, 257, 64, 5, 0.20, 8, 20, 7),
+    ('tiny_d', 14, 2500, 200, 8, 0.10, 20, 45, 100),
+]
+LARGE = ('cells50k', 50)
+ROW_SAMPLE = 64
+
+
+def build_reference(reference: str) -> None:
+    os.makedirs(os.path.join(BUILD, 'brisc'), exist_ok=True)
+    suffix = sysconfig.get_config_var('EXT_SUFFIX')
+    np_inc = np.get_include()
+    py_inc = sysconfig.get_paths()['include']
+    procs = []
+    for m in MODULES:
+        src = os.path.join(reference, 'brisc', f'{m}.pyx')
+        cpp = os.path.join(BUILD, f'{m}.cpp')
+        so = os.path.join(BUILD, 'brisc', f'{m}{suffix}')
+        if os.path.exists(so) and os.path.getmtime(so) > os.path.getmtime(src):
+            continue
+        subprocess.run([sys.executable, '-m', 'cython', '-3', '--cplus',
+                        '-X', 'boundscheck=False', '-X', 'wraparound=False',
+                        '-X', 'cdivision=True', '-X', 'initializedcheck=False',
+                        src, '-o', cpp], check=True)
+        procs.append(subprocess.Popen(
+            ['g++', '-Ofast', '-funroll-loops', '-std=c++17', '-g0',
+             '-fopenmp', '-fPIC', '-shared', '-march=native',
+             f'-I{py_inc}', f'-I{np_inc}',
+             '-DNPY_NO_DEPRECATED_API=NPY_1_17_API_VERSION', cpp, '-o', so]))
+    for p in procs:
+        if p.wait() != 0:
+            raise RuntimeError('compiling the reference module failed')
+
+
+def import_reference():
+    pkg = types.ModuleType('brisc')
+    pkg.__path__ = [os.path.join(BUILD, 'brisc')]
+    sys.modules['brisc'] = pkg
+    return (importlib.import_module('brisc.pca'),
+            importlib.import_module('brisc.normalize'))
+
+
+def ref_normalize(norm_mod, counts, indices, indptr, num_threads=8):
+    n = len(indptr) - 1
+    out = np.empty(len(counts), np.float32)
+    row_sums = np.empty(n, np.uint64)
+    norm_mod.normalize_csr(counts.astype(np.int32), indices.astype(np.int32),
+                           indptr.astype(np.int32), np.empty(0, np.int8), out,
+                           row_sums, n, 1, num_threads)
+    return out
+
+
+def ref_irlba(pca_mod, data, indices, indptr, n, g, k, m, chunk,
+              num_threads=8, seed=0, tolerance=1e-6, max_iterations=100):
+    import scipy.sparse as sp
+    X = sp.csr_array((data, indices, indptr), shape=(n, g))
+    Xc = X.tocsc()
+    Xc.sort_indices()
+    PCs = np.empty((n, k), np.float32)
+    m = min(m, n, g)
+    t0 = time.perf_counter()
+    conv = pca_mod.irlba(
+        data_matvec=data, indices_matvec=indices.astype(np.int32),
+        indptr_matvec=indptr.astype(np.int32), data_rmatvec=Xc.data,
+        indices_rmatvec=Xc.indices.astype(np.int32),
+        indptr_rmatvec=Xc.indptr.astype(np.int32), num_cells=n, num_genes=g,
+        k=k, subspace_size=m, tolerance=tolerance,
+        max_iterations=max_iterations, seed=seed,
+        match_parallel=num_threads == 1, is_csr=True,
+        num_threads=num_threads, chunk_size=chunk, PCs=PCs)
+    return PCs, bool(conv), time.perf_counter() - t0
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument('--reference', default='/root/reference')
+    args = ap.parse_args()
+    build_reference(args.reference)
+    pca_mod, norm_mod = import_reference()
+
+    tiny = {}
+    meta = {'tiny': [], 'large': None,
+            'note': 'outputs of the reference brisc irlba/normalize_csr'}
+    for (name, seed, n, g, T, dens, k, m, chunk) in TINY:
+        w = synth.make_workload(seed, n, num_types=T, num_genes=g,
+                                target_density=dens)
+        indptr, indices, counts = oracle.generate_counts(w)
+        data = ref_normalize(norm_mod, counts, indices, indptr)
+        PCs, conv, _ = ref_irlba(pca_mod, data, indices, indptr, n, g, k, m,
+                                 chunk)
+        PCs1, conv1, _ = ref_irlba(pca_mod, data, indices, indptr, n, g, k, m,
+                                   chunk, num_threads=1)
+        assert np.array_equal(PCs, PCs1), 'reference not thread-invariant'
+        tiny[f'{name}_data'] = data
+        tiny[f'{name}_indices'] = indices
+        tiny[f'{name}_indptr'] = indptr
+        tiny[f'{name}_pcs'] = PCs
+        meta['tiny'].append(dict(name=name, seed=seed, num_cells=n,
+                                 num_genes=g, num_types=T, density=dens, k=k,
+                                 subspace_size=m, chunk_size=chunk,
+                                 converged=conv,
+                                 csr_sha256=synth.csr_sha256(data, indices,
+                                                             indptr)))
+        print(name, 'converged', conv, flush=True)
+    np.savez_compressed(os.path.join(HERE, 'ref_irlba_tiny.npz'), **tiny)
+
+    wname, k = LARGE
+    w = synth.make_workload(**synth.WORKLOADS[wname])
+    indptr, indices, counts = oracle.generate_counts(w)
+    data = ref_normalize(norm_mod, counts, indices, indptr)
+    PCs, conv, t8 = ref_irlba(pca_mod, data, indices, indptr, w.num_cells,
+                              w.num_genes, k, 100, 1024, num_threads=8)
+    PCs1, conv1, t1 = ref_irlba(pca_mod, data, indices, indptr, w.num_cells,
+                                w.num_genes, k, 100, 1024, num_threads=1)
+    rows = np.random.Generator(np.random.PCG64(1)).choice(
+        w.num_cells, ROW_SAMPLE, replace=False)
+    rows.sort()
+    np.savez_compressed(
+        os.path.join(HERE, f'ref_irlba_{wname}.npz'),
+        singular_values=np.linalg.norm(PCs.astype(np.float64), axis=0),
+        rows=rows, pcs_rows=PCs[rows])
+    meta['large'] = dict(
+        workload=wname, k=k, subspace_size=100, chunk_size=1024,
+        tolerance=1e-6, max_iterations=100, seed=0, converged=conv,
+        thread_invariant=bool(np.array_equal(PCs, PCs1)),
+        nnz=int(len(data)), scale=w.scale,
+        csr_sha256=synth.csr_sha256(data, indices, indptr),
+        counts_sha256=synth.csr_sha256(counts, indices, indptr),
+        seconds_8_threads=round(t8, 3), seconds_1_thread=round(t1, 3),
+        host_cpus=os.cpu_count())
+    with open(os.path.join(HERE, 'ref_vectors.json'), 'w') as f:
+        json.dump(meta, f, indent=1)
+    print(json.dumps(meta['large'], indent=1))
+
+
+if __name__ == '__main__':
+    main()

@@ -1,0 +1,59 @@
+"""Comparison helpers shared by the parity tests."""
+import numpy as np
+
+
+def singular_values(PCs):
+    """Column norms of the scores U_k S_k, i.e. the singular values."""
+    return np.linalg.norm(np.asarray(PCs, np.float64), axis=0)
+
+
+def relative_sv_error(PCs, reference_sv):
+    sv = singular_values(PCs)
+    ref = np.asarray(reference_sv, np.float64)
+    return np.abs(sv - ref) / ref
+
+
+def well_separated(sv, rel_gap):
+    """Indices whose singular value is at least rel_gap from its neighbours."""
+    sv = np.asarray(sv, np.float64)
+    out = []
+    for i in range(len(sv)):
+        lo = abs(sv[i] - sv[i - 1]) if i > 0 else np.inf
+        hi = abs(sv[i] - sv[i + 1]) if i + 1 < len(sv) else np.inf
+        if min(lo, hi) > rel_gap * sv[i]:
+            out.append(i)
+    return np.array(out, dtype=int)
+
+
+def column_errors_up_to_sign(A, B):
+    A = np.asarray(A, np.float64)
+    B = np.asarray(B, np.float64)
+    sign = np.sign(np.sum(A * B, axis=0))
+    sign[sign == 0] = 1
+    return np.linalg.norm(A * sign - B, axis=0) / np.linalg.norm(B, axis=0)
+
+
+def cluster_projection_deficit(A, B, sv_ref, rel_window):
+    """1 - |projection| of each normalised column of A onto the span of the
+    columns of B whose singular values lie within rel_window of its own.
+
+    Columns whose cluster touches the last column are skipped (the cluster
+    may continue past k).  Returns the deficits of the checked columns.
+    """
+    A = np.asarray(A, np.float64)
+    B = np.asarray(B, np.float64)
+    A = A / np.linalg.norm(A, axis=0)
+    Q, _ = np.linalg.qr(B / np.linalg.norm(B, axis=0))
+    sv_ref = np.asarray(sv_ref, np.float64)
+    k = len(sv_ref)
+    out = []
+    for i in range(k):
+        members = np.flatnonzero(np.abs(sv_ref - sv_ref[i]) <= rel_window * sv_ref[i])
+        if members.max() == k - 1 and i != k - 1 and len(members) > 1:
+            continue
+        if members.max() == k - 1 and len(members) > 1:
+            continue
+        Qm, _ = np.linalg.qr(B[:, members])
+        proj = np.linalg.norm(Qm.T @ A[:, i])
+        out.append(1.0 - proj)
+    return np.array(out)

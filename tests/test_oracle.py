@@ -1,0 +1,176 @@
+"""The oracle, pinned against the reference's own outputs and the mathematics.
+
+Every function of ``oracle/`` is checked by something other than its own
+formula: the reference's golden vectors (written by the compiled reference
+``irlba`` / ``normalize_csr``, ``tests/golden``), a dense FP64 SVD on tiny
+inputs, closed forms and exact integer identities.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from brisc_amd import synth
+
+import helpers
+
+SV_RTOL = 1e-4          # singular values, relative
+COL_RTOL = 1e-2         # individual PCs with a >=1% singular-value gap
+CLUSTER_TOL = 1e-3      # projection deficit onto near-degenerate clusters
+
+
+def _tiny_case(golden, name):
+    meta, tiny, _ = golden
+    case = next(c for c in meta['tiny'] if c['name'] == name)
+    return case, (tiny[f'{name}_data'], tiny[f'{name}_indices'],
+                  tiny[f'{name}_indptr'], tiny[f'{name}_pcs'])
+
+
+TINY = ['tiny_a', 'tiny_b', 'tiny_c', 'tiny_d']
+
+
+# ----------------------------------------------------------- generator
+
+def test_generator_matches_reference_normalisation_bytes(golden, cells50k):
+    """CPU generator + log1pPF restatement reproduce, byte for byte, the
+    arrays the reference's normalize_csr produced for the 50k workload."""
+    meta, _, _ = golden
+    w, data, indices, indptr = cells50k
+    assert w.scale == pytest.approx(meta['large']['scale'], rel=0, abs=0)
+    assert synth.csr_sha256(data, indices, indptr) == meta['large']['csr_sha256']
+    assert len(data) == meta['large']['nnz']
+
+
+@pytest.mark.parametrize('name', TINY)
+def test_tiny_inputs_regenerate_exactly(golden, name):
+    case, (data, indices, indptr, _) = _tiny_case(golden, name)
+    w = synth.make_workload(case['seed'], case['num_cells'],
+                            num_types=case['num_types'],
+                            num_genes=case['num_genes'],
+                            target_density=case['density'])
+    d2, i2, p2 = oracle.generate_csr(w)
+    assert synth.csr_sha256(d2, i2, p2) == case['csr_sha256']
+    assert np.array_equal(d2.view(np.uint32), data.view(np.uint32))
+
+
+def test_generator_row_ranges_are_independent():
+    """Counts of a row range do not depend on where generation starts."""
+    w = synth.make_workload(5, 3000, num_types=7)
+    full = oracle.generate_counts(w)
+    part = oracle.generate_counts(w, row0=1234, num_rows=777)
+    s, e = full[0][1234], full[0][1234 + 777]
+    assert np.array_equal(full[1][s:e], part[1])
+    assert np.array_equal(full[2][s:e], part[2])
+    assert np.array_equal(full[0][1234:1234 + 778] - s, part[0])
+
+
+def test_generator_statistics():
+    """Density near the calibrated 7%, sorted unique columns, positive counts,
+    and the NB mean/variance relation on a high-mean gene."""
+    w = synth.make_workload(21, 20000)
+    indptr, indices, counts = oracle.generate_counts(w)
+    density = len(counts) / (w.num_cells * w.num_genes)
+    assert abs(density - w.expected_density) < 0.003
+    assert (counts > 0).all()
+    rows = np.repeat(np.arange(w.num_cells), np.diff(indptr))
+    key = rows.astype(np.int64) * w.num_genes + indices
+    assert (np.diff(key) > 0).all()
+    # mean of one gene within one type vs its NB mean, over cells
+    gene = int(np.argmax(w.type_gene_mean[0]))
+    sel = (w.cell_type == 0)
+    dense = np.zeros(w.num_cells)
+    hit = indices == gene
+    dense[rows[hit]] = counts[hit]
+    m = w.type_gene_mean[0, gene] * w.cell_factor[sel]
+    assert abs(dense[sel].mean() - m.mean()) < 4 * np.sqrt((m + 0.2 * m * m).mean() / sel.sum())
+
+
+def test_log1ppf_closed_form():
+    """log1pPF equals log1p(x * mean_row_sum / row_sum) to FP32 rounding."""
+    w = synth.make_workload(8, 500, num_types=3, num_genes=200, target_density=0.2)
+    indptr, indices, counts = oracle.generate_counts(w)
+    out = oracle.log1ppf(counts, indptr)
+    rs = np.add.reduceat(counts.astype(np.float64), indptr[:-1])
+    rs[np.diff(indptr) == 0] = 0
+    mean = rs.sum() / w.num_cells
+    ref = np.log1p(counts * np.repeat(mean / np.where(rs > 0, rs, 1), np.diff(indptr)))
+    assert np.allclose(out, ref, rtol=3e-7, atol=0)
+
+
+# ----------------------------------------------------------- FP64 oracle
+
+@pytest.mark.parametrize('name', TINY)
+def test_fp64_oracle_matches_dense_svd(golden, name):
+    case, (data, indices, indptr, _) = _tiny_case(golden, name)
+    n, g, k = case['num_cells'], case['num_genes'], case['k']
+    out = oracle.fp64_pca(data, indices, indptr, n, g, k)
+    Z = oracle.dense_z(data, indices, indptr, n, g)
+    U, S, Vt = np.linalg.svd(Z, full_matrices=False)
+    assert np.allclose(out['singular_values'], S[:k], rtol=1e-10, atol=0)
+    ws = helpers.well_separated(S[:k], 1e-3)
+    errs = helpers.column_errors_up_to_sign(out['PCs'][:, ws], (U * S)[:, ws])
+    assert errs.max() < 1e-8
+
+
+def test_stddev_closed_form():
+    rng = np.random.default_rng(0)
+    X = (rng.random((60, 9)) < 0.3) * rng.random((60, 9)).astype(np.float32)
+    X[:, 4] = 0.0                      # zero-variance gene -> clipped
+    X[:, 7] = 2.5                      # constant gene -> clipped
+    import scipy.sparse as sp
+    A = sp.csr_array(X.astype(np.float32))
+    sd = oracle.clipped_stddev(A.data, A.indices, A.indptr, 60, 9)
+    ref = np.maximum(X.astype(np.float64).std(axis=0, ddof=1), 1e-8)
+    assert np.allclose(sd, ref, rtol=1e-12, atol=1e-15)
+    assert sd[4] == 1e-8 and sd[7] == 1e-8
+
+
+def test_planted_centering_mistake_is_caught(golden):
+    """Dropping the centring term from the oracle's Gram (a plausible
+    mistake) moves the singular values far outside the tolerance."""
+    case, (data, indices, indptr, _) = _tiny_case(golden, 'tiny_b')
+    n, g, k = case['num_cells'], case['num_genes'], case['k']
+    G, cs = oracle.gram(data, indices, indptr, g)
+    sd = oracle.clipped_stddev(data, indices, indptr, n, g)
+    wrong = np.sqrt(np.sort(np.linalg.eigvalsh(G / np.outer(sd, sd)))[::-1][:k])
+    right = oracle.fp64_pca(data, indices, indptr, n, g, k)['singular_values']
+    assert np.max(np.abs(wrong - right) / right) > 100 * SV_RTOL
+
+
+# ----------------------------------------------------------- FP32 port
+
+@pytest.mark.parametrize('name', TINY)
+def test_port_and_reference_agree_with_fp64_oracle(golden, name):
+    case, (data, indices, indptr, ref_pcs) = _tiny_case(golden, name)
+    n, g, k = case['num_cells'], case['num_genes'], case['k']
+    out = oracle.fp64_pca(data, indices, indptr, n, g, k)
+    P, conv, _ = oracle.irlba_port(data, indices, indptr, n, g, k,
+                                   subspace_size=case['subspace_size'],
+                                   chunk_size=case['chunk_size'],
+                                   num_threads=2)
+    assert conv and case['converged']
+    sv = out['singular_values']
+    assert helpers.relative_sv_error(ref_pcs, sv).max() < SV_RTOL
+    assert helpers.relative_sv_error(P, sv).max() < SV_RTOL
+    ws = helpers.well_separated(sv, 1e-2)
+    if len(ws):
+        assert helpers.column_errors_up_to_sign(P[:, ws], ref_pcs[:, ws]).max() < COL_RTOL
+        assert helpers.column_errors_up_to_sign(ref_pcs[:, ws], out['PCs'][:, ws]).max() < COL_RTOL
+    assert helpers.cluster_projection_deficit(P, out['PCs'], sv, 1e-3).max() < CLUSTER_TOL
+
+
+@pytest.mark.slow
+def test_reference_port_oracle_agree_at_50k(golden, cells50k):
+    meta, _, large = golden
+    w, data, indices, indptr = cells50k
+    k = meta['large']['k']
+    out = oracle.fp64_pca(data, indices, indptr, w.num_cells, w.num_genes, k)
+    ref_sv = large['singular_values']
+    assert np.max(np.abs(ref_sv - out['singular_values']) / out['singular_values']) < SV_RTOL
+    P, conv, st = oracle.irlba_port(data, indices, indptr, w.num_cells,
+                                    w.num_genes, k, num_threads=8)
+    assert conv
+    assert helpers.relative_sv_error(P, ref_sv).max() < SV_RTOL
+    rows = large['rows']
+    ws = helpers.well_separated(ref_sv, 1e-2)
+    errs = helpers.column_errors_up_to_sign(P[rows][:, ws], large['pcs_rows'][:, ws])
+    assert errs.max() < COL_RTOL
