@@ -158,7 +158,7 @@ def main():
         'solve': {'restarts': stats[-1]['restarts'],
                   'matvecs': stats[-1]['matvecs'],
                   'converged': bool(stats[-1]['converged']),
-                  'svd_seconds': round(stats[-1]['svd_seconds'], 4),
+                  'svd_sweeps_last': stats[-1]['svd_sweeps'],
                   'top_singular_values': [round(float(x), 4) for x in sv[:5]]},
         'spmv_pair_gbs': round(bytes_per * spmv_launches / (spmv_ms * 1e-3) / 1e9, 1) if spmv_ms else None,
         'kernels': kern, 'breakdown': breakdown,

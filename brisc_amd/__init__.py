@@ -12,8 +12,9 @@ ctypes.
 * :mod:`brisc_amd.synth` -- synthetic NB count workloads (host parameters and
   the device generator).
 """
-from .irlba import Context, default_context, irlba, irlba_device
+from .irlba import (Context, Engine, default_context, irlba, irlba_batched,
+                    irlba_device, small_svd)
 from .single_cell import SingleCell
 
-__all__ = ['Context', 'SingleCell', 'default_context', 'irlba',
-           'irlba_device']
+__all__ = ['Context', 'Engine', 'SingleCell', 'default_context', 'irlba',
+           'irlba_batched', 'irlba_device', 'small_svd']

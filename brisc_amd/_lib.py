@@ -35,13 +35,32 @@ class BudgetError(BpcaError, MemoryError):
     """A request does not fit the context's HBM budget."""
 
 
+class Problem(ctypes.Structure):
+    """struct bpca_problem (include/bpca.h)."""
+    _fields_ = [('data', ctypes.c_void_p),
+                ('indices', ctypes.c_void_p),
+                ('indptr', ctypes.c_void_p),
+                ('indptr_width', ctypes.c_int32),
+                ('validate', ctypes.c_int32),
+                ('num_cells', ctypes.c_uint64),
+                ('num_genes', ctypes.c_uint32),
+                ('k', ctypes.c_uint32),
+                ('subspace_size', ctypes.c_uint32),
+                ('max_iterations', ctypes.c_uint32),
+                ('tolerance', ctypes.c_float),
+                ('pad', ctypes.c_uint32),
+                ('seed', ctypes.c_uint64),
+                ('PCs', ctypes.c_void_p)]
+
+
 class SolveStats(ctypes.Structure):
     _fields_ = [('converged', ctypes.c_int32),
                 ('restarts', ctypes.c_int32),
                 ('lanczos_steps', ctypes.c_int64),
                 ('matvecs', ctypes.c_int64),
                 ('seconds', ctypes.c_double),
-                ('svd_seconds', ctypes.c_double),
+                ('svd_sweeps', ctypes.c_int32),
+                ('slot', ctypes.c_int32),
                 ('smax', ctypes.c_double)]
 
     def as_dict(self) -> dict:
@@ -63,6 +82,14 @@ def lib() -> ctypes.CDLL:
         raise ImportError(
             f'{LIB_PATH} is not built; build it with '
             f'`make -C brisc_amd/csrc` (or __graft_entry__.build())')
+    # One HIP runtime per process.  PyTorch-ROCm bundles its own
+    # libamdhip64.so (SONAME libamdhip64.so.7); loading it first makes this
+    # library bind to that same runtime instead of pulling in a second copy
+    # from /opt/rocm, which would leave torch without a device.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = ctypes.CDLL(LIB_PATH)
     p = ctypes.c_void_p
     i32, u32, u64, i64 = ctypes.c_int, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int64
@@ -87,7 +114,17 @@ def lib() -> ctypes.CDLL:
         'bpca_synth_row_nnz': (i32, [p, u64, u64, u64, u32, p, p, p, p]),
         'bpca_synth_fill': (i32, [p, u64, u64, u64, u32, p, p, p, p, p, p]),
         'bpca_normalize_log1ppf': (i32, [p, p, p, u64, p]),
-        'bpca_test_bidiag_svd': (i32, [i32, p, p, p, p]),
+        'bpca_test_small_svd': (i32, [p, i32, p, p, p, p, p]),
+        'bpca_irlba_batched': (i32, [p, i32, ctypes.POINTER(Problem),
+                                     ctypes.POINTER(SolveStats), p, i32]),
+        'bpca_engine_create': (i32, [p, ctypes.POINTER(p)]),
+        'bpca_engine_destroy': (i32, [p]),
+        'bpca_engine_add': (i32, [p, ctypes.POINTER(Problem),
+                                  ctypes.POINTER(ctypes.c_int32)]),
+        'bpca_engine_step': (i32, [p, i32, ctypes.POINTER(ctypes.c_int32)]),
+        'bpca_engine_collect': (i32, [p, i32, p, ctypes.POINTER(SolveStats),
+                                      p, i32, ctypes.POINTER(ctypes.c_int32)]),
+        'bpca_engine_active': (i32, [p, ctypes.POINTER(ctypes.c_int32)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(L, name)

@@ -1,18 +1,24 @@
-// Device kernels of the IRLBA path for gfx950 (CDNA4, wave64).
+// Device kernels of the IRLBA engine for gfx950 (CDNA4, wave64).
 //
-// Reference loops they replace (brisc/pca.pyx):
-//   spmv_kernel            matvec_csr_parallel_raw       440-475
-//   reorth_* / alpha       fused passes 1 and 2, alpha   1006-1090
-//   spmv_t_kernel          rmatvec_csc_parallel_raw      478-511 (from CSR)
-//   colstats_kernel        clipped_stddev_csc            51-103
-//   vside_kernel           V update, CGS, beta           1103-1171
-//   tall_gemm_*            matmul_nn_f_out / _c_out      578-658, 1211-1254
+// Reference code each kernel replaces (brisc/pca.pyx unless noted):
+//   k_colstats, k_setup_finish  clipped_stddev_csc                51-103
+//   k_spmv                      matvec_csr_parallel_raw           440-475
+//   k_reorth_*, k_alpha         fused passes 1, 2 and alpha       1006-1090
+//   k_spmv_t                    rmatvec_csc_parallel_raw          478-511
+//                               (computed from the CSR, no CSC copy)
+//   k_vside                     V update, CGS, beta / remainder   1103-1171
+//   k_svd                       sgesdd + convergence + k_current  514-545,
+//                                                                 1178-1202
+//   k_tall_gemm<false>,         matmul_nn_f_out + V sgemm + B     578-616,
+//   k_restart_v                                                   1204-1231
+//   k_tall_gemm<true>           matmul_nn_c_out with Ub*Sb        619-658,
+//                                                                 1233-1254
 //
-// Determinism: no floating-point atomics anywhere.  Every sum has a fixed
-// order defined by the data (row position, fixed row blocks of kRowBlock
-// rows, fixed sub-blocks of kSubRows rows, fixed groups of kGroup blocks),
-// never by grid size or scheduling, so results are bit-identical from run to
-// run, for any grid size, and when problems share the device.
+// Determinism: no floating-point atomics.  Every sum has a fixed order set by
+// the data -- the position inside a row, fixed data blocks of kRowBlock rows,
+// fixed sub-blocks of kSubRows rows, fixed groups of kGroup blocks -- never by
+// grid size, scheduling or the other problems sharing a launch.  A problem's
+// results are therefore bit-identical run to run, alone or in a batch.
 #include "kernels.h"
 
 namespace bpca {
@@ -31,8 +37,8 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
-// Sum over a block of NW waves: xor-tree inside each wave, then a pairwise
-// tree over the wave sums in wave order.  `red` holds NW doubles.
+// Sum over a block of NW waves: xor butterfly inside each wave, then a
+// pairwise tree over the wave sums in wave order.  `red` holds NW doubles.
 template <int NW>
 __device__ inline double block_sum(double v, double *red) {
     v = wave_sum(v);
@@ -54,130 +60,219 @@ __device__ inline int wave_id() {
     return __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
 }
 
+// Rows [first, first + count) of data block `lb` owned by wave w when the
+// block is split into 4 sub-blocks of kSubRows rows.
+__device__ inline void sub_block(int64_t n, int64_t lb, int w, int64_t &first,
+                                 int64_t &count) {
+    first = lb * kRowBlock + (int64_t)w * kSubRows;
+    count = n - first;
+    count = count < 0 ? 0 : (count > kSubRows ? kSubRows : count);
+}
+
 }  // namespace
 
-// ------------------------------------------------------------ validation
+// ================================================================ set-up
 
-template <typename IP>
-__global__ __launch_bounds__(256) void validate_kernel(
-    const int32_t *__restrict__ indices, const IP *__restrict__ indptr,
-    int64_t n, int g, int *flag) {
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t row = (int64_t)blockIdx.x * 4 + wave_id();
-    if (row >= n) return;
-    const int64_t b = indptr[row], e = indptr[row + 1];
-    int bad = 0;
-    if (row == 0 && b != 0) bad = 1;
-    if (e < b) bad = 1;
-    for (int64_t p = b + lane; p < e && !bad; p += kWave) {
-        int32_t c = indices[p];
-        int32_t prev = p > b ? indices[p - 1] : -1;
-        if (c < 0 || c >= g || c <= prev) bad = 1;
-    }
-    if (bad) atomicOr(flag, 1);
-}
-
-hipError_t launch_validate(const Csr &X, int *d_flag, hipStream_t s) {
-    dim3 grid((unsigned)ceil_div(X.n, 4));
-    if (X.indptr_width == 8)
-        validate_kernel<int64_t><<<grid, 256, 0, s>>>(
-            X.indices, (const int64_t *)X.indptr, X.n, X.g, d_flag);
-    else
-        validate_kernel<int32_t><<<grid, 256, 0, s>>>(
-            X.indices, (const int32_t *)X.indptr, X.n, X.g, d_flag);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(256) void narrow_indices_kernel(
-    const int64_t *__restrict__ in, int64_t nnz, int g,
-    int32_t *__restrict__ out, int *flag) {
-    const int64_t stride = (int64_t)gridDim.x * 256;
-    int bad = 0;
-    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nnz; i += stride) {
-        const int64_t v = in[i];
-        if (v < 0 || v >= g) bad = 1;
-        out[i] = (int32_t)v;
-    }
-    if (bad) atomicOr(flag, 1);
-}
-
-hipError_t launch_narrow_indices(const int64_t *in, int64_t nnz, int g,
-                                 int32_t *out, int *d_flag, hipStream_t s) {
-    if (nnz <= 0) return hipSuccess;
-    const int grid = (int)std::min<int64_t>(ceil_div(nnz, 256), 65536);
-    narrow_indices_kernel<<<grid, 256, 0, s>>>(in, nnz, g, out, d_flag);
-    return hipGetLastError();
-}
-
-// ------------------------------------------------------------ u = X vs
-//
-// One wave per row (grid-stride over rows).  Canonical order of a row's sum:
-// lane l accumulates the row's elements at positions l, l+64, l+128, ...
-// (relative to the row start) with FP32 fma in that order, then the 64 lane
-// sums are combined by the xor butterfly 32,16,8,4,2,1.
-
-template <typename IP>
-__global__ __launch_bounds__(256) void spmv_kernel(
-    const float *__restrict__ data, const int32_t *__restrict__ indices,
-    const IP *__restrict__ indptr, const float *__restrict__ vs, int g,
-    int64_t n, float *__restrict__ u) {
-    extern __shared__ float lds_v[];
-    for (int i = threadIdx.x; i < g; i += blockDim.x) lds_v[i] = vs[i];
-    __syncthreads();
-    const int lane = threadIdx.x & (kWave - 1);
-    const int64_t stride = (int64_t)gridDim.x * 4;
-    for (int64_t row = (int64_t)blockIdx.x * 4 + wave_id(); row < n;
-         row += stride) {
-        const int64_t b = indptr[row], e = indptr[row + 1];
-        float acc = 0.0f;
-        for (int64_t base = b; base < e; base += 4 * kWave) {
-            float d[4];
-            int ix[4];
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int64_t p = base + lane + t * kWave;
-                const bool ok = p < e;
-                d[t] = ok ? data[p] : 0.0f;
-                ix[t] = ok ? indices[p] : 0;
-            }
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const bool ok = base + lane + t * kWave < e;
-                if (ok) acc = fmaf(d[t], lds_v[ix[t]], acc);
-            }
-        }
-        acc = wave_sum(acc);
-        if (lane == 0) u[row] = acc;
-    }
-}
-
-hipError_t launch_spmv(const Csr &X, const float *vs, float *u32, int grid,
-                       hipStream_t s) {
-    if (grid <= 0) grid = (int)std::min<int64_t>(ceil_div(X.n, 4), 8192);
-    size_t lds = sizeof(float) * (size_t)X.g;
-    if (X.indptr_width == 8)
-        spmv_kernel<int64_t><<<grid, 256, lds, s>>>(
-            X.data, X.indices, (const int64_t *)X.indptr, vs, X.g, X.n, u32);
-    else
-        spmv_kernel<int32_t><<<grid, 256, lds, s>>>(
-            X.data, X.indices, (const int32_t *)X.indptr, vs, X.g, X.n, u32);
-    return hipGetLastError();
-}
-
-// ------------------------------------------- reorthogonalisation (FP64)
-//
-// Block b covers rows [b*1024, b*1024+1024).  Within a block, lane l of a
-// wave handles rows l, l+64, ..., l+960 in that order, then xor butterfly.
-
-__global__ __launch_bounds__(256) void reorth_partials_kernel(
-    const float *__restrict__ u32, const double *__restrict__ U, int64_t n,
-    int j, double *__restrict__ P, int64_t nb) {
-    constexpr int R = kRowBlock / kWave;  // 16 rows per lane
+// Column sums of x (kSquare = false) or x^2 (true) for one data block:
+// wave w accumulates its sub-block in a private LDS vector (each row's
+// column indices are unique, so the 64 lanes touch distinct entries); the
+// per-gene sum is the FP32 chain over the sub-block's rows in order, and
+// the four sub-blocks combine as (w0 + w1) + (w2 + w3).
+template <bool kSquare>
+__global__ __launch_bounds__(256) void k_colstats(Tables T) {
+    extern __shared__ float lds[];
+    const int2 bp = T.blocks[blockIdx.x];
+    Prob &pr = T.probs[bp.x];
+    if (pr.phase != PH_SETUP) return;
+    const int g = pr.g;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = wave_id();
-    const int64_t b = blockIdx.x;
-    const int64_t r0 = b * kRowBlock;
-    const int64_t nrows = std::min<int64_t>(kRowBlock, n - r0);
+    float *acc = lds + (size_t)w * g;
+    for (int i = lane; i < g; i += kWave) acc[i] = 0.0f;
+    int64_t r0, nrows;
+    sub_block(pr.n, bp.y, w, r0, nrows);
+    const int64_t *indptr = pr.indptr;
+    const int32_t *indices = pr.indices;
+    const float *data = pr.data;
+    for (int64_t rr = 0; rr < nrows; rr++) {
+        const int64_t b = indptr[r0 + rr], e = indptr[r0 + rr + 1];
+        for (int64_t p = b + lane; p < e; p += kWave) {
+            const int c = indices[p];
+            const float x = data[p];
+            acc[c] = kSquare ? fmaf(x, x, acc[c]) : acc[c] + x;
+        }
+    }
+    __syncthreads();
+    float *out = pr.tpart + (int64_t)bp.y * 2 * g + (kSquare ? g : 0);
+    for (int i = threadIdx.x; i < g; i += 256)
+        out[i] = (lds[i] + lds[g + i]) + (lds[2 * g + i] + lds[3 * g + i]);
+}
+
+// grp[G*2g + i] = FP64 sum over the data blocks of group G, in block order.
+__global__ __launch_bounds__(256) void k_reduce_groups(Tables T, int setup) {
+    const int2 gp = T.groups[blockIdx.y];
+    Prob &pr = T.probs[gp.x];
+    if (pr.phase != (setup ? PH_SETUP : PH_LANCZOS)) return;
+    const int g = pr.g;
+    const int width = setup ? 2 * g : g;
+    const int i = blockIdx.x * 256 + threadIdx.x;
+    if (i >= width) return;
+    const int64_t b0 = (int64_t)gp.y * kGroup;
+    const int64_t b1 = b0 + kGroup < pr.nb ? b0 + kGroup : pr.nb;
+    const float *src = pr.tpart + i;
+    double acc = 0.0;
+    for (int64_t b = b0; b < b1; b++) acc += (double)src[b * 2 * g];
+    pr.grp[(int64_t)gp.y * 2 * g + i] = acc;
+}
+
+// sd = max(sqrt((sum x^2 - (sum x)^2/n)/(n-1)), 1e-8) in FP64; vs = V0/sd;
+// B = 0; state reset; phase -> Lanczos.
+__global__ __launch_bounds__(256) void k_setup_finish(Tables T) {
+    Prob &pr = T.probs[blockIdx.x];
+    if (pr.phase != PH_SETUP) return;
+    const int g = pr.g, m = pr.m;
+    const double n = (double)pr.n;
+    const double *V0 = pr.V[0];
+    for (int i = threadIdx.x; i < g; i += 256) {
+        double s = 0.0, ss = 0.0;
+        for (int64_t G = 0; G < pr.ngrp; G++) {
+            s += pr.grp[G * 2 * g + i];
+            ss += pr.grp[G * 2 * g + g + i];
+        }
+        const double var = (ss - s * s / n) / (n - 1.0);
+        double sd = sqrt(var > 0.0 ? var : 0.0);
+        sd = sd < 1e-8 ? 1e-8 : sd;
+        pr.sd[i] = sd;
+        pr.vs[i] = (float)(V0[i] / sd);
+    }
+    for (int i = threadIdx.x; i < m * (m + 1); i += 256) pr.B[i] = 0.0;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        pr.j = 0;
+        pr.kc = 0;
+        pr.iter = 0;
+        pr.ubuf = 0;
+        pr.vbuf = 0;
+        pr.converged = 0;
+        pr.nconv = 0;
+        pr.svd_sweeps = 0;
+        pr.smax = 2.4221817809573368e-05;  // FP32 eps^(2/3), pca.pyx:688
+        pr.steps = 0;
+        pr.phase = PH_LANCZOS;
+    }
+}
+
+// ================================================================ u = X vs
+//
+// One workgroup per data block; wave w walks its 256-row sub-block two rows
+// at a time (loads of both rows in flight together).  Canonical order of a
+// row's sum: lane l accumulates the row's elements at positions l, l+64,
+// l+128, ... (relative to the row start) with FP32 fma in that order, then
+// the 64 lane sums meet in the xor butterfly 32,16,8,4,2,1.
+
+namespace {
+
+struct RowChunk {
+    float d[4];
+    int ix[4];
+};
+
+__device__ inline void load_chunk(const float *__restrict__ data,
+                                  const int32_t *__restrict__ indices,
+                                  int64_t base, int64_t e, int lane,
+                                  RowChunk &c) {
+#pragma unroll
+    for (int t = 0; t < 4; t++) {
+        const int64_t p = base + lane + t * kWave;
+        const bool ok = p < e;
+        c.d[t] = ok ? data[p] : 0.0f;
+        c.ix[t] = ok ? indices[p] : 0;
+    }
+}
+
+__device__ inline float dot_chunk(const RowChunk &c, const float *vs,
+                                  int64_t base, int64_t e, int lane,
+                                  float acc) {
+#pragma unroll
+    for (int t = 0; t < 4; t++)
+        if (base + lane + t * kWave < e) acc = fmaf(c.d[t], vs[c.ix[t]], acc);
+    return acc;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(256) void k_spmv(Tables T) {
+    extern __shared__ float lds_v[];
+    const int2 bp = T.blocks[blockIdx.x];
+    Prob &pr = T.probs[bp.x];
+    if (pr.phase != PH_LANCZOS) return;
+    const int g = pr.g;
+    for (int i = threadIdx.x; i < g; i += 256) lds_v[i] = pr.vs[i];
+    __syncthreads();
+    const int lane = threadIdx.x & (kWave - 1);
+    int64_t r0, nrows;
+    sub_block(pr.n, bp.y, wave_id(), r0, nrows);
+    const int64_t *__restrict__ indptr = pr.indptr;
+    const int32_t *__restrict__ indices = pr.indices;
+    const float *__restrict__ data = pr.data;
+    float *__restrict__ u = pr.u32;
+    int64_t rr = 0;
+    for (; rr + 1 < nrows; rr += 2) {
+        const int64_t ra = r0 + rr;
+        const int64_t ba = indptr[ra], ea = indptr[ra + 1], eb = indptr[ra + 2];
+        const int64_t bb = ea;
+        RowChunk ca, cb;
+        load_chunk(data, indices, ba, ea, lane, ca);
+        load_chunk(data, indices, bb, eb, lane, cb);
+        float acca = dot_chunk(ca, lds_v, ba, ea, lane, 0.0f);
+        float accb = dot_chunk(cb, lds_v, bb, eb, lane, 0.0f);
+        for (int64_t base = ba + 4 * kWave; base < ea; base += 4 * kWave) {
+            load_chunk(data, indices, base, ea, lane, ca);
+            acca = dot_chunk(ca, lds_v, base, ea, lane, acca);
+        }
+        for (int64_t base = bb + 4 * kWave; base < eb; base += 4 * kWave) {
+            load_chunk(data, indices, base, eb, lane, cb);
+            accb = dot_chunk(cb, lds_v, base, eb, lane, accb);
+        }
+        acca = wave_sum(acca);
+        accb = wave_sum(accb);
+        if (lane == 0) {
+            u[ra] = acca;
+            u[ra + 1] = accb;
+        }
+    }
+    if (rr < nrows) {
+        const int64_t ra = r0 + rr;
+        const int64_t ba = indptr[ra], ea = indptr[ra + 1];
+        RowChunk ca;
+        float acca = 0.0f;
+        for (int64_t base = ba; base < ea; base += 4 * kWave) {
+            load_chunk(data, indices, base, ea, lane, ca);
+            acca = dot_chunk(ca, lds_v, base, ea, lane, acca);
+        }
+        acca = wave_sum(acca);
+        if (lane == 0) u[ra] = acca;
+    }
+}
+
+// ============================================ reorthogonalisation (FP64)
+//
+// Block partials: lane l of a wave handles rows l, l+64, ..., l+960 of the
+// data block in that order, then the xor butterfly.  P[c*nb + b]: c = 0 the
+// block sum of u, c >= 1 the dot of U[:, c-1] with u.
+
+__global__ __launch_bounds__(256) void k_reorth_partials(Tables T) {
+    constexpr int R = kRowBlock / kWave;  // 16 rows per lane
+    const int2 bp = T.blocks[blockIdx.x];
+    Prob &pr = T.probs[bp.x];
+    if (pr.phase != PH_LANCZOS) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_id();
+    const int64_t n = pr.n, b = bp.y, r0 = b * kRowBlock;
+    const int64_t nrows = n - r0 < kRowBlock ? n - r0 : kRowBlock;
+    const int j = pr.j;
+    const float *u32 = pr.u32;
+    const double *U = pr.U[pr.ubuf];
     double uv[R];
 #pragma unroll
     for (int t = 0; t < R; t++) {
@@ -199,47 +294,44 @@ __global__ __launch_bounds__(256) void reorth_partials_kernel(
             }
         }
         acc = wave_sum(acc);
-        if (lane == 0) P[(int64_t)c * nb + b] = acc;
+        if (lane == 0) pr.P[(int64_t)c * pr.nb + b] = acc;
     }
 }
 
-hipError_t launch_reorth_partials(const float *u32, const double *U,
-                                  int64_t n, int j, double *P, hipStream_t s) {
-    const int64_t nb = ceil_div(n, kRowBlock);
-    reorth_partials_kernel<<<(unsigned)nb, 256, 0, s>>>(u32, U, n, j, P, nb);
-    return hipGetLastError();
-}
-
-// Thread t sums blocks t, t+256, ... in order; then block_sum<4>.
-__global__ __launch_bounds__(256) void reduce_columns_kernel(
-    const double *__restrict__ P, int64_t nb, double *__restrict__ red) {
+// red[c] = sum over blocks of P[c*nb + b]: thread t takes blocks t, t+256,
+// ... in order, then block_sum<4>.
+__global__ __launch_bounds__(256) void k_reduce_columns(Tables T) {
     __shared__ double tmp[4];
-    const double *p = P + (int64_t)blockIdx.x * nb;
+    const int cols = T.max_m + 1;
+    Prob &pr = T.probs[blockIdx.x / cols];
+    const int c = blockIdx.x % cols;
+    if (pr.phase != PH_LANCZOS || c > pr.j) return;
+    const double *p = pr.P + (int64_t)c * pr.nb;
     double acc = 0.0;
-    for (int64_t b = threadIdx.x; b < nb; b += 256) acc += p[b];
+    for (int64_t b = threadIdx.x; b < pr.nb; b += 256) acc += p[b];
     acc = block_sum<4>(acc, tmp);
-    if (threadIdx.x == 0) red[blockIdx.x] = acc;
+    if (threadIdx.x == 0) pr.red[c] = acc;
 }
 
-hipError_t launch_reduce_columns(const double *P, int64_t nb, int ncols,
-                                 double *red, hipStream_t s) {
-    reduce_columns_kernel<<<ncols, 256, 0, s>>>(P, nb, red);
-    return hipGetLastError();
-}
-
-// Thread t handles rows r0+t, r0+t+256, r0+t+512, r0+t+768 in order; the
-// correction is mean + sum_c U[r,c] coef[c] accumulated over c ascending.
-__global__ __launch_bounds__(256) void reorth_apply_kernel(
-    const float *__restrict__ u32, double *__restrict__ U, int64_t n, int j,
-    const double *__restrict__ red, double *__restrict__ sq) {
-    extern __shared__ double coef[];  // j+1 values
+// U[:, j] = u - (mean + sum_c U[:, c] red[c+1]) with the correction summed
+// over c ascending; thread t handles rows r0+t, r0+t+256, ... in order and
+// the block's sum of squares is block_sum<4> of the thread sums.
+__global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
+    extern __shared__ double coef[];
     __shared__ double tmp[4];
-    for (int i = threadIdx.x; i <= j; i += 256) coef[i] = red[i];
+    const int2 bp = T.blocks[blockIdx.x];
+    Prob &pr = T.probs[bp.x];
+    if (pr.phase != PH_LANCZOS) return;
+    const int j = pr.j;
+    const int64_t n = pr.n;
+    for (int i = threadIdx.x; i <= j; i += 256) coef[i] = pr.red[i];
     __syncthreads();
     const double mean = coef[0] / (double)n;
-    const int64_t r0 = (int64_t)blockIdx.x * kRowBlock;
-    double acc = 0.0;
+    const int64_t r0 = (int64_t)bp.y * kRowBlock;
+    double *U = pr.U[pr.ubuf];
     double *Uj = U + (int64_t)j * n;
+    const float *u32 = pr.u32;
+    double acc = 0.0;
 #pragma unroll
     for (int t = 0; t < kRowBlock / 256; t++) {
         const int64_t r = r0 + threadIdx.x + t * 256;
@@ -252,234 +344,119 @@ __global__ __launch_bounds__(256) void reorth_apply_kernel(
         }
     }
     acc = block_sum<4>(acc, tmp);
-    if (threadIdx.x == 0) sq[blockIdx.x] = acc;
+    if (threadIdx.x == 0) pr.sq[bp.y] = acc;
 }
 
-hipError_t launch_reorth_apply(const float *u32, double *U, int64_t n, int j,
-                               const double *red, double *sq, hipStream_t s) {
-    const int64_t nb = ceil_div(n, kRowBlock);
-    reorth_apply_kernel<<<(unsigned)nb, 256, sizeof(double) * (j + 1), s>>>(
-        u32, U, n, j, red, sq);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(256) void alpha_kernel(
-    const double *__restrict__ sq, int64_t nb, double *__restrict__ scal,
-    double *__restrict__ B, int m, int j) {
+// alpha = max(sqrt(sum of block squares), 1e-8); B[j, j] = alpha.
+__global__ __launch_bounds__(256) void k_alpha(Tables T) {
     __shared__ double tmp[4];
+    Prob &pr = T.probs[blockIdx.x];
+    if (pr.phase != PH_LANCZOS) return;
     double acc = 0.0;
-    for (int64_t b = threadIdx.x; b < nb; b += 256) acc += sq[b];
+    for (int64_t b = threadIdx.x; b < pr.nb; b += 256) acc += pr.sq[b];
     acc = block_sum<4>(acc, tmp);
     if (threadIdx.x == 0) {
         double alpha = sqrt(acc);
         if (alpha < 1e-8) alpha = 1e-8;
-        scal[0] = alpha;
-        scal[1] = 1.0 / alpha;
-        B[(int64_t)j + (int64_t)j * m] = alpha;
+        pr.alpha = alpha;
+        pr.inv_alpha = 1.0 / alpha;
+        pr.B[(int64_t)pr.j + (int64_t)pr.j * pr.m] = alpha;
     }
 }
 
-hipError_t launch_alpha(const double *sq, int64_t nb, double *scal, double *B,
-                        int m, int j, hipStream_t s) {
-    alpha_kernel<<<1, 256, 0, s>>>(sq, nb, scal, B, m, j);
-    return hipGetLastError();
-}
-
-// ------------------------------------------------------------ X^T u
+// ================================================================ X^T u
 //
-// Block b (kRowBlock rows) is processed by one workgroup of 4 waves; wave w
-// owns sub-block rows [b*1024 + 256w, +256) and a private LDS accumulator of
-// g floats.  Each row's column indices are unique (validated), so the 64
-// lanes of a wave update distinct accumulator entries and the per-gene sum of
-// a sub-block is the FP32 fma chain over its rows in ascending order.  The
-// four sub-block accumulators are combined as (w0 + w1) + (w2 + w3).
-// Loads of the next row are issued before the current row's LDS updates.
+// One workgroup per data block; wave w owns a 256-row sub-block and a
+// private LDS accumulator of g floats.  u = U[:, j] / alpha is written back
+// normalised.  Per gene the sub-block sum is the FP32 fma chain over its rows
+// in order (row indices are unique); sub-blocks combine as (w0+w1)+(w2+w3).
 
-template <typename IP>
-__global__ __launch_bounds__(256) void spmv_t_kernel(
-    const float *__restrict__ data, const int32_t *__restrict__ indices,
-    const IP *__restrict__ indptr, double *__restrict__ Ucol,
-    const double *__restrict__ scal, int64_t n, int g,
-    float *__restrict__ partial) {
+__global__ __launch_bounds__(256) void k_spmv_t(Tables T) {
     extern __shared__ float lds[];
+    const int2 bp = T.blocks[blockIdx.x];
+    Prob &pr = T.probs[bp.x];
+    if (pr.phase != PH_LANCZOS) return;
+    const int g = pr.g;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = wave_id();
     float *acc = lds + (size_t)w * g;
     float *uval = lds + (size_t)4 * g + (size_t)w * kSubRows;
     for (int i = lane; i < g; i += kWave) acc[i] = 0.0f;
-    const int64_t rs = (int64_t)blockIdx.x * kRowBlock + (int64_t)w * kSubRows;
-    const int64_t nrows = std::max<int64_t>(0, std::min<int64_t>(kSubRows, n - rs));
-    const double inv_alpha = scal[1];
+    int64_t r0, nrows;
+    sub_block(pr.n, bp.y, w, r0, nrows);
+    double *Uj = pr.U[pr.ubuf] + (int64_t)pr.j * pr.n;
+    const double inv_alpha = pr.inv_alpha;
     for (int t = lane; t < nrows; t += kWave) {
-        const double x = Ucol[rs + t] * inv_alpha;
-        Ucol[rs + t] = x;
+        const double x = Uj[r0 + t] * inv_alpha;
+        Uj[r0 + t] = x;
         uval[t] = (float)x;
     }
     __syncthreads();
-    for (int64_t rr = 0; rr < nrows; rr++) {
-        const int64_t r = rs + rr;
-        const int64_t b = indptr[r], e = indptr[r + 1];
-        const float ur = uval[rr];
-        for (int64_t base = b; base < e; base += 4 * kWave) {
-            float d[4];
-            int ix[4];
-#pragma unroll
-            for (int t = 0; t < 4; t++) {
-                const int64_t p = base + lane + t * kWave;
-                const bool ok = p < e;
-                d[t] = ok ? data[p] : 0.0f;
-                ix[t] = ok ? indices[p] : 0;
+    const int64_t *__restrict__ indptr = pr.indptr;
+    const int32_t *__restrict__ indices = pr.indices;
+    const float *__restrict__ data = pr.data;
+    if (nrows > 0) {
+        int64_t b = indptr[r0], e = indptr[r0 + 1];
+        RowChunk cur;
+        load_chunk(data, indices, b, e, lane, cur);
+        for (int64_t rr = 0; rr < nrows; rr++) {
+            const float ur = uval[rr];
+            // Prefetch the first chunk of the next row before the LDS
+            // updates of this one.
+            int64_t nb_ = 0, ne_ = 0;
+            RowChunk nxt;
+            if (rr + 1 < nrows) {
+                nb_ = e;
+                ne_ = indptr[r0 + rr + 2];
+                load_chunk(data, indices, nb_, ne_, lane, nxt);
             }
 #pragma unroll
-            for (int t = 0; t < 4; t++) {
-                if (base + lane + t * kWave < e) acc[ix[t]] = fmaf(d[t], ur, acc[ix[t]]);
+            for (int t = 0; t < 4; t++)
+                if (b + lane + t * kWave < e) acc[cur.ix[t]] = fmaf(cur.d[t], ur, acc[cur.ix[t]]);
+            for (int64_t base = b + 4 * kWave; base < e; base += 4 * kWave) {
+                RowChunk more;
+                load_chunk(data, indices, base, e, lane, more);
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    if (base + lane + t * kWave < e) acc[more.ix[t]] = fmaf(more.d[t], ur, acc[more.ix[t]]);
+            }
+            if (rr + 1 < nrows) {
+                b = nb_;
+                e = ne_;
+                cur = nxt;
             }
         }
     }
     __syncthreads();
-    float *out = partial + (int64_t)blockIdx.x * g;
+    float *out = pr.tpart + (int64_t)bp.y * 2 * g;
     for (int i = threadIdx.x; i < g; i += 256)
         out[i] = (lds[i] + lds[g + i]) + (lds[2 * g + i] + lds[3 * g + i]);
 }
 
-hipError_t launch_spmv_t(const Csr &X, double *Ucol, const double *scal,
-                         float *partial, hipStream_t s) {
-    const int64_t nb = ceil_div(X.n, kRowBlock);
-    size_t lds = sizeof(float) * ((size_t)4 * X.g + 4 * kSubRows);
-    if (X.indptr_width == 8)
-        spmv_t_kernel<int64_t><<<(unsigned)nb, 256, lds, s>>>(
-            X.data, X.indices, (const int64_t *)X.indptr, Ucol, scal, X.n, X.g,
-            partial);
-    else
-        spmv_t_kernel<int32_t><<<(unsigned)nb, 256, lds, s>>>(
-            X.data, X.indices, (const int32_t *)X.indptr, Ucol, scal, X.n, X.g,
-            partial);
-    return hipGetLastError();
-}
-
-// Column sums of x (kSquare = false) or x^2 (true) with the same block /
-// sub-block structure; written to partial[b*2g + (kSquare ? g : 0) + gene].
-template <typename IP, bool kSquare>
-__global__ __launch_bounds__(256) void colstats_kernel(
-    const float *__restrict__ data, const int32_t *__restrict__ indices,
-    const IP *__restrict__ indptr, int64_t n, int g,
-    float *__restrict__ partial) {
-    extern __shared__ float lds[];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int w = wave_id();
-    float *acc = lds + (size_t)w * g;
-    for (int i = lane; i < g; i += kWave) acc[i] = 0.0f;
-    __syncthreads();
-    const int64_t rs = (int64_t)blockIdx.x * kRowBlock + (int64_t)w * kSubRows;
-    const int64_t nrows = std::max<int64_t>(0, std::min<int64_t>(kSubRows, n - rs));
-    for (int64_t rr = 0; rr < nrows; rr++) {
-        const int64_t r = rs + rr;
-        const int64_t b = indptr[r], e = indptr[r + 1];
-        for (int64_t p = b + lane; p < e; p += kWave) {
-            const int c = indices[p];
-            const float x = data[p];
-            acc[c] = kSquare ? fmaf(x, x, acc[c]) : acc[c] + x;
-        }
-    }
-    __syncthreads();
-    float *out = partial + (int64_t)blockIdx.x * 2 * g + (kSquare ? g : 0);
-    for (int i = threadIdx.x; i < g; i += 256)
-        out[i] = (lds[i] + lds[g + i]) + (lds[2 * g + i] + lds[3 * g + i]);
-}
-
-template <bool kSquare>
-static hipError_t colstats_pass(const Csr &X, float *partial, hipStream_t s) {
-    const int64_t nb = ceil_div(X.n, kRowBlock);
-    size_t lds = sizeof(float) * (size_t)4 * X.g;
-    if (X.indptr_width == 8)
-        colstats_kernel<int64_t, kSquare><<<(unsigned)nb, 256, lds, s>>>(
-            X.data, X.indices, (const int64_t *)X.indptr, X.n, X.g, partial);
-    else
-        colstats_kernel<int32_t, kSquare><<<(unsigned)nb, 256, lds, s>>>(
-            X.data, X.indices, (const int32_t *)X.indptr, X.n, X.g, partial);
-    return hipGetLastError();
-}
-
-hipError_t launch_colstats(const Csr &X, float *partial, hipStream_t s) {
-    hipError_t e = colstats_pass<false>(X, partial, s);
-    if (e != hipSuccess) return e;
-    return colstats_pass<true>(X, partial, s);
-}
-
-size_t lds_bytes_needed(int g, int m) {
-    size_t need = sizeof(float) * ((size_t)4 * g + 4 * kSubRows);    // spmv_t
-    need = std::max(need, sizeof(double) * ((size_t)g + m));          // vside
-    need = std::max(need, sizeof(double) * (size_t)m * 32);           // gemm
-    return need;
-}
-
-// grp[G*width + i] = sum over blocks G*64 .. G*64+63 (ascending) of
-// partial[b*width + i], accumulated in FP64.
-__global__ __launch_bounds__(256) void reduce_groups_kernel(
-    const float *__restrict__ partial, int64_t nblocks, int width,
-    double *__restrict__ grp) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= width) return;
-    const int64_t G = blockIdx.y;
-    const int64_t b0 = G * kGroup, b1 = std::min<int64_t>(b0 + kGroup, nblocks);
-    double acc = 0.0;
-    for (int64_t b = b0; b < b1; b++) acc += (double)partial[b * width + i];
-    grp[G * width + i] = acc;
-}
-
-hipError_t launch_reduce_groups(const float *partial, int64_t nblocks,
-                                int width, double *grp, hipStream_t s) {
-    dim3 grid((unsigned)ceil_div(width, 256), (unsigned)ceil_div(nblocks, kGroup));
-    reduce_groups_kernel<<<grid, 256, 0, s>>>(partial, nblocks, width, grp);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(256) void finish_stddev_kernel(
-    const double *__restrict__ grp, int64_t ngroups, int g, int64_t n,
-    double *__restrict__ sd) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i >= g) return;
-    double s = 0.0, ss = 0.0;
-    for (int64_t G = 0; G < ngroups; G++) {
-        s += grp[G * 2 * g + i];
-        ss += grp[G * 2 * g + g + i];
-    }
-    double var = (ss - s * s / (double)n) / (double)(n - 1);
-    double v = sqrt(var > 0.0 ? var : 0.0);
-    sd[i] = v < 1e-8 ? 1e-8 : v;
-}
-
-hipError_t launch_finish_stddev(const double *grp, int64_t ngroups, int g,
-                                int64_t n, double *sd, hipStream_t s) {
-    finish_stddev_kernel<<<(unsigned)ceil_div(g, 256), 256, 0, s>>>(grp, ngroups,
-                                                                    g, n, sd);
-    return hipGetLastError();
-}
-
-// ------------------------------------------------------------ V side
+// ================================================================ V side
 //
-// One workgroup of 16 waves.  w[gene] sums the group partials in group order
-// (FP64); dots with V columns: lane-strided over genes then xor butterfly,
-// wave c%16 owning column c; corrections accumulate over columns ascending.
+// One workgroup of 16 waves per problem.  w[gene] sums the group partials in
+// group order (FP64); w = w/sd - alpha V[:, j]; dots with the V columns are
+// lane-strided over genes then xor butterfly (wave c%16 owns column c); the
+// correction accumulates over columns ascending; beta = sqrt(block_sum<16>).
 
-__global__ __launch_bounds__(1024) void vside_kernel(
-    const double *__restrict__ grp, int64_t ngroups, int g,
-    const double *__restrict__ sd, double *__restrict__ V, int j, int m,
-    double *__restrict__ B, const double *__restrict__ scal,
-    double *__restrict__ rem, double *__restrict__ rscal,
-    float *__restrict__ vs) {
+__global__ __launch_bounds__(1024) void k_vside(Tables T) {
     extern __shared__ double sh[];
-    double *w = sh;            // g
-    double *work = sh + g;     // m
     __shared__ double tmp[16];
+    Prob &pr = T.probs[blockIdx.x];
+    if (pr.phase != PH_LANCZOS) return;
+    const int g = pr.g, m = pr.m, j = pr.j;
+    double *w = sh;           // g
+    double *work = sh + g;    // m
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = wave_id();
-    const double alpha = scal[0];
+    double *V = pr.V[pr.vbuf];
+    const double alpha = pr.alpha;
     const double *Vj = V + (int64_t)j * g;
     for (int i = threadIdx.x; i < g; i += 1024) {
         double acc = 0.0;
-        for (int64_t G = 0; G < ngroups; G++) acc += grp[G * g + i];
-        w[i] = acc / sd[i] - alpha * Vj[i];
+        for (int64_t G = 0; G < pr.ngrp; G++) acc += pr.grp[G * 2 * g + i];
+        w[i] = acc / pr.sd[i] - alpha * Vj[i];
     }
     __syncthreads();
     const bool last = (j == m - 1);
@@ -509,125 +486,452 @@ __global__ __launch_bounds__(1024) void vside_kernel(
         for (int i = threadIdx.x; i < g; i += 1024) {
             const double x = w[i] * inv;
             Vn[i] = x;
-            vs[i] = (float)(x / sd[i]);
+            pr.vs[i] = (float)(x / pr.sd[i]);
         }
-        if (threadIdx.x == 0) B[(int64_t)j + (int64_t)(j + 1) * m] = bclip;
+        if (threadIdx.x == 0) {
+            pr.B[(int64_t)j + (int64_t)(j + 1) * m] = bclip;
+            pr.j = j + 1;
+            pr.steps += 1;
+        }
     } else {
         const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
-        for (int i = threadIdx.x; i < g; i += 1024) rem[i] = w[i] * inv;
-        if (threadIdx.x == 0) rscal[0] = beta;
+        for (int i = threadIdx.x; i < g; i += 1024) pr.rem[i] = w[i] * inv;
+        if (threadIdx.x == 0) {
+            pr.rnorm = beta;
+            pr.steps += 1;
+            pr.phase = PH_RESTART;
+        }
     }
 }
 
-hipError_t launch_vside(const double *grp, int64_t ngroups, int g,
-                        const double *sd, double *V, int j, int m, double *B,
-                        const double *scal, double *rem, double *rscal,
-                        float *vs, hipStream_t s) {
-    size_t lds = sizeof(double) * ((size_t)g + m);
-    vside_kernel<<<1, 1024, lds, s>>>(grp, ngroups, g, sd, V, j, m, B, scal,
-                                     rem, rscal, vs);
-    return hipGetLastError();
-}
-
-__global__ void init_vs_kernel(const double *__restrict__ V,
-                               const double *__restrict__ sd, int g,
-                               float *__restrict__ vs) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    if (i < g) vs[i] = (float)(V[i] / sd[i]);
-}
-
-hipError_t launch_init_vs(const double *V, const double *sd, int g, float *vs,
-                          hipStream_t s) {
-    init_vs_kernel<<<(unsigned)ceil_div(g, 256), 256, 0, s>>>(V, sd, g, vs);
-    return hipGetLastError();
-}
-
-// ------------------------------------------------------------ restart GEMMs
+// ================================================================ small SVD
 //
-// out[r, c] = sum_l A[r, l] coef[l, c], thread per row, 32 output columns per
-// pass with the coefficient slab in LDS; the sum runs over l ascending.
+// One-sided (Hestenes) Jacobi on the columns of C = B[:, :m] (B's last column
+// is zero): rotate column pairs until all are orthogonal to working
+// precision (|w_p . w_q| <= sqrt(m) eps |w_p| |w_q|); then s_c = |W[:, c]|,
+// Ub = W / s, right vectors = accumulated rotations.  Pairs follow the round-robin (circle) schedule, m/2 disjoint
+// pairs per round, one wave per pair, so every rotation is applied in a fixed
+// order.  Sweeps stop when a sweep performs no rotation.
 
-template <bool kRowMajorF32>
-__global__ __launch_bounds__(256) void tall_gemm_kernel(
-    const double *__restrict__ A, int64_t lda, int64_t n, int kin,
-    const double *__restrict__ coef, int kout, void *__restrict__ out,
-    int64_t ldo) {
-    constexpr int CW = 32;
-    extern __shared__ double cs[];  // kin * CW
-    const int64_t r = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    for (int c0 = 0; c0 < kout; c0 += CW) {
-        const int nc = std::min(CW, kout - c0);
+namespace {
+
+__device__ inline int rr_player(int pos, int round, int mm) {
+    return pos == 0 ? 0 : ((pos - 1 + round) % (mm - 1)) + 1;
+}
+
+// W (m x mm) and Jv (mm x mm) column-major; m rows of W are live.
+__device__ void jacobi_svd(double *W, double *Jv, int m, int mm, int *sweeps,
+                           int *rot_count) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    const double tol = 2.220446049250313e-16 * sqrt((double)m);
+    int sweep = 0;
+    for (; sweep < 60; sweep++) {
+        if (threadIdx.x == 0) *rot_count = 0;
         __syncthreads();
-        for (int i = threadIdx.x; i < kin * CW; i += 256) {
-            const int l = i / CW, cc = i % CW;
-            cs[i] = cc < nc ? coef[l + (int64_t)(c0 + cc) * kin] : 0.0;
+        for (int round = 0; round < mm - 1; round++) {
+            for (int slot = wv; slot < mm / 2; slot += nw) {
+                int p = rr_player(slot, round, mm);
+                int q = rr_player(mm - 1 - slot, round, mm);
+                if (p > q) {
+                    int t = p;
+                    p = q;
+                    q = t;
+                }
+                double *wp = W + (int64_t)p * m, *wq = W + (int64_t)q * m;
+                double a = 0.0, b = 0.0, c = 0.0;
+                for (int i = lane; i < m; i += kWave) {
+                    const double x = wp[i], y = wq[i];
+                    a = fma(x, x, a);
+                    b = fma(y, y, b);
+                    c = fma(x, y, c);
+                }
+                a = wave_sum(a);
+                b = wave_sum(b);
+                c = wave_sum(c);
+                if (c != 0.0 && fabs(c) > tol * sqrt(a * b)) {
+                    const double zeta = (b - a) / (2.0 * c);
+                    const double t = (zeta >= 0.0 ? 1.0 : -1.0) /
+                                     (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double cs = 1.0 / sqrt(1.0 + t * t);
+                    const double sn = cs * t;
+                    for (int i = lane; i < m; i += kWave) {
+                        const double x = wp[i], y = wq[i];
+                        wp[i] = cs * x - sn * y;
+                        wq[i] = sn * x + cs * y;
+                    }
+                    double *jp = Jv + (int64_t)p * mm, *jq = Jv + (int64_t)q * mm;
+                    for (int i = lane; i < mm; i += kWave) {
+                        const double x = jp[i], y = jq[i];
+                        jp[i] = cs * x - sn * y;
+                        jq[i] = sn * x + cs * y;
+                    }
+                    if (lane == 0) atomicAdd(rot_count, 1);
+                }
+            }
+            __syncthreads();
+        }
+        const int rc = *rot_count;
+        __syncthreads();
+        if (rc == 0) break;
+    }
+    if (threadIdx.x == 0) *sweeps = sweep + 1;
+}
+
+// Sort columns by norm (descending, ties by index) and write Sb, Ub = W/s
+// and the right vectors Vb (m x m).
+__device__ void svd_sort_outputs(const double *W, const double *Jv, int m,
+                                 int mm, double *s, int *perm, double *Sb,
+                                 double *Ub, double *Vb) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = threadIdx.x / kWave;
+    const int nw = blockDim.x / kWave;
+    for (int c = wv; c < m; c += nw) {
+        const double *wc = W + (int64_t)c * m;
+        double a = 0.0;
+        for (int i = lane; i < m; i += kWave) a = fma(wc[i], wc[i], a);
+        a = wave_sum(a);
+        if (lane == 0) s[c] = sqrt(a);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int c = 0; c < m; c++) perm[c] = c;
+        for (int a = 1; a < m; a++) {
+            const int v = perm[a];
+            int b = a - 1;
+            while (b >= 0 && (s[perm[b]] < s[v])) {
+                perm[b + 1] = perm[b];
+                b--;
+            }
+            perm[b + 1] = v;
+        }
+    }
+    __syncthreads();
+    for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
+        const int t = idx / m, i = idx % m;
+        const int src = perm[t];
+        const double sv = s[src];
+        Ub[idx] = sv > 0.0 ? W[(int64_t)src * m + i] / sv : 0.0;
+        Vb[idx] = Jv[(int64_t)src * mm + i];
+    }
+    for (int t = threadIdx.x; t < m; t += blockDim.x) Sb[t] = s[perm[t]];
+    (void)mm;
+}
+
+}  // namespace
+
+__global__ __launch_bounds__(1024) void k_svd(Tables T) {
+    __shared__ int sh_int[4];
+    extern __shared__ double sh[];  // s[m] then perm[m] (as int)
+    Prob &pr = T.probs[blockIdx.x];
+    if (pr.phase != PH_RESTART) return;
+    const int m = pr.m, mm = m + (m & 1), k = pr.k;
+    double *W = pr.W, *Jv = pr.Jv;
+    for (int idx = threadIdx.x; idx < m * mm; idx += blockDim.x) {
+        const int c = idx / m;
+        W[idx] = c < m ? pr.B[idx] : 0.0;  // B column-major with ld m
+    }
+    for (int idx = threadIdx.x; idx < mm * mm; idx += blockDim.x)
+        Jv[idx] = (idx / mm == idx % mm) ? 1.0 : 0.0;
+    __syncthreads();
+    jacobi_svd(W, Jv, m, mm, &sh_int[0], &sh_int[1]);
+    __syncthreads();
+    double *s = sh;
+    int *perm = (int *)(sh + m);
+    // Vb (right vectors, m x m) is stored in B's memory after the SVD reads
+    // B; B is rebuilt below for the restart.
+    double *Vb = pr.Jv + (int64_t)mm * mm;
+    svd_sort_outputs(W, Jv, m, mm, s, perm, pr.Sb, pr.Ub, Vb);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const double rnorm = pr.rnorm;
+        double smax = pr.smax;
+        smax = pr.Sb[0] > smax ? pr.Sb[0] : smax;
+        pr.smax = smax;
+        int nconv = 0;
+        for (int i = 0; i < k; i++) {
+            const double res = rnorm * pr.Ub[(m - 1) + (int64_t)i * m];
+            nconv += fabs(res) < (double)pr.tol * smax;
+        }
+        pr.nconv = nconv;
+        pr.svd_sweeps = sh_int[0];
+        int kc = pr.kc;
+        if (nconv == k) {
+            pr.converged = 1;
+            pr.phase = PH_FINAL;
+        } else if (pr.iter == pr.maxit - 1) {
+            pr.phase = PH_FINAL;
+        } else {
+            kc = kc > k + nconv ? kc : k + nconv;
+            kc = kc < m - 1 ? kc : m - 1;
+            pr.kc = kc;
+            pr.iter = pr.iter + 1;
+            pr.phase = PH_RESTARTING;
+        }
+        sh_int[2] = pr.phase;
+        sh_int[3] = kc;
+    }
+    __syncthreads();
+    if (sh_int[2] == PH_RESTARTING) {
+        const int kc = sh_int[3];
+        const double rnorm = pr.rnorm;
+        for (int idx = threadIdx.x; idx < m * (m + 1); idx += blockDim.x) {
+            const int i = idx % m, c = idx / m;
+            double v = 0.0;
+            if (i < kc && c == i) v = pr.Sb[i];
+            if (i < kc && c == kc) v = rnorm * pr.Ub[(m - 1) + (int64_t)i * m];
+            pr.B[idx] = v;
+        }
+    }
+}
+
+// ================================================================ restart
+//
+// out[r, c] = sum_l U[r, l] coef[l, c] with the sum over l ascending; one
+// thread per row, kGemmCols output columns per pass, the coefficient slab in
+// LDS.  Used for U <- U Ub[:, :kc] and for PCs = U (Ub[:, :k] Sb[:k]).
+
+template <bool kFinal>
+__global__ __launch_bounds__(256) void k_tall_gemm(Tables T) {
+    extern __shared__ double cs[];  // m * kGemmCols
+    const int2 bp = T.blocks[blockIdx.x];
+    Prob &pr = T.probs[bp.x];
+    if (pr.phase != (kFinal ? PH_FINAL : PH_RESTARTING)) return;
+    const int m = pr.m;
+    const int kout = kFinal ? pr.k : pr.kc;
+    const int64_t n = pr.n;
+    const double *A = pr.U[pr.ubuf];
+    double *out = pr.U[pr.ubuf ^ 1];
+    const int64_t r0 = (int64_t)bp.y * kRowBlock;
+    for (int c0 = 0; c0 < kout; c0 += kGemmCols) {
+        const int nc = kout - c0 < kGemmCols ? kout - c0 : kGemmCols;
+        __syncthreads();
+        for (int i = threadIdx.x; i < m * kGemmCols; i += 256) {
+            const int l = i / kGemmCols, cc = i % kGemmCols;
+            double v = 0.0;
+            if (cc < nc) {
+                v = pr.Ub[l + (int64_t)(c0 + cc) * m];
+                if (kFinal) v *= pr.Sb[c0 + cc];
+            }
+            cs[i] = v;
         }
         __syncthreads();
-        if (r < n) {
-            double acc[CW];
+        for (int t = 0; t < kRowBlock / 256; t++) {
+            const int64_t r = r0 + threadIdx.x + t * 256;
+            if (r >= n) break;
+            double acc[kGemmCols];
 #pragma unroll
-            for (int cc = 0; cc < CW; cc++) acc[cc] = 0.0;
-            for (int l = 0; l < kin; l++) {
-                const double x = A[(int64_t)l * lda + r];
+            for (int cc = 0; cc < kGemmCols; cc++) acc[cc] = 0.0;
+            for (int l = 0; l < m; l++) {
+                const double x = A[(int64_t)l * n + r];
 #pragma unroll
-                for (int cc = 0; cc < CW; cc++) acc[cc] = fma(x, cs[l * CW + cc], acc[cc]);
+                for (int cc = 0; cc < kGemmCols; cc++)
+                    acc[cc] = fma(x, cs[l * kGemmCols + cc], acc[cc]);
             }
-            if (kRowMajorF32) {
-                float *o = (float *)out + r * kout + c0;
+            if (kFinal) {
+                float *o = pr.PCs + r * pr.k + c0;
 #pragma unroll
-                for (int cc = 0; cc < CW; cc++)
+                for (int cc = 0; cc < kGemmCols; cc++)
                     if (cc < nc) o[cc] = (float)acc[cc];
             } else {
-                double *o = (double *)out;
 #pragma unroll
-                for (int cc = 0; cc < CW; cc++)
-                    if (cc < nc) o[(int64_t)(c0 + cc) * ldo + r] = acc[cc];
+                for (int cc = 0; cc < kGemmCols; cc++)
+                    if (cc < nc) out[(int64_t)(c0 + cc) * n + r] = acc[cc];
             }
         }
     }
 }
 
-hipError_t launch_tall_gemm_f64(const double *A, int64_t lda, int64_t n,
-                                int kin, const double *coef, int kout,
-                                double *out, int64_t ldo, hipStream_t s) {
-    if (kout <= 0) return hipSuccess;
-    tall_gemm_kernel<false><<<(unsigned)ceil_div(n, 256), 256,
-                              sizeof(double) * kin * 32, s>>>(
-        A, lda, n, kin, coef, kout, out, ldo);
-    return hipGetLastError();
-}
-
-hipError_t launch_tall_gemm_rowmajor_f32(const double *A, int64_t lda,
-                                         int64_t n, int kin,
-                                         const double *coef, int kout,
-                                         float *out, hipStream_t s) {
-    tall_gemm_kernel<true><<<(unsigned)ceil_div(n, 256), 256,
-                             sizeof(double) * kin * 32, s>>>(
-        A, lda, n, kin, coef, kout, out, 0);
-    return hipGetLastError();
-}
-
-__global__ __launch_bounds__(256) void v_restart_kernel(
-    const double *__restrict__ V, int g, int m, const double *__restrict__ Vbt,
-    int kc, const double *__restrict__ rem, double *__restrict__ V2) {
-    const int i = blockIdx.x * 256 + threadIdx.x;
-    const int c = blockIdx.y;
-    if (i >= g) return;
-    if (c == kc) {
-        V2[(int64_t)kc * g + i] = rem[i];
-        return;
+// V <- V Vb[:, :kc], V[:, kc] = remainder, vs = remainder / sd; flip the U and
+// V buffers; continue the Lanczos sweep at j = kc.
+__global__ __launch_bounds__(1024) void k_restart_v(Tables T) {
+    Prob &pr = T.probs[blockIdx.x];
+    if (pr.phase != PH_RESTARTING) return;
+    const int g = pr.g, m = pr.m, kc = pr.kc, mm = m + (m & 1);
+    const double *V = pr.V[pr.vbuf];
+    double *V2 = pr.V[pr.vbuf ^ 1];
+    const double *Vb = pr.Jv + (int64_t)mm * mm;
+    for (int idx = threadIdx.x; idx < g * kc; idx += 1024) {
+        const int c = idx / g, i = idx % g;
+        double acc = 0.0;
+        for (int l = 0; l < m; l++) acc = fma(V[(int64_t)l * g + i], Vb[l + (int64_t)c * m], acc);
+        V2[(int64_t)c * g + i] = acc;
     }
-    double acc = 0.0;
-    for (int l = 0; l < m; l++)
-        acc = fma(V[(int64_t)l * g + i], Vbt[c + (int64_t)l * m], acc);
-    V2[(int64_t)c * g + i] = acc;
+    for (int i = threadIdx.x; i < g; i += 1024) {
+        V2[(int64_t)kc * g + i] = pr.rem[i];
+        pr.vs[i] = (float)(pr.rem[i] / pr.sd[i]);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        pr.ubuf ^= 1;
+        pr.vbuf ^= 1;
+        pr.j = kc;
+        pr.phase = PH_LANCZOS;
+    }
 }
 
-hipError_t launch_v_restart(const double *V, int g, int m, const double *Vbt,
-                            int kc, const double *rem, double *V2,
-                            hipStream_t s) {
-    dim3 grid((unsigned)ceil_div(g, 256), (unsigned)(kc + 1));
-    v_restart_kernel<<<grid, 256, 0, s>>>(V, g, m, Vbt, kc, rem, V2);
+__global__ void k_finish(Tables T, int *done_counter) {
+    Prob &pr = T.probs[blockIdx.x];
+    if (threadIdx.x != 0 || pr.phase != PH_FINAL) return;
+    pr.phase = PH_DONE;
+    atomicAdd(done_counter, 1);
+}
+
+// ================================================================ launchers
+
+size_t lds_bytes_needed(int g, int m) {
+    size_t need = sizeof(float) * ((size_t)4 * g + 4 * kSubRows);   // spmv_t
+    need = std::max(need, sizeof(double) * ((size_t)g + m));         // vside
+    need = std::max(need, sizeof(double) * (size_t)m * kGemmCols);   // gemm
+    need = std::max(need, sizeof(double) * (size_t)m + sizeof(int) * (size_t)m);
+    return need;
+}
+
+hipError_t launch_setup(const Tables &T, hipStream_t s) {
+    if (T.num_probs == 0) return hipSuccess;
+    const size_t lds4g = sizeof(float) * 4 * (size_t)T.max_g;
+    k_colstats<false><<<(unsigned)T.num_blocks, 256, lds4g, s>>>(T);
+    k_colstats<true><<<(unsigned)T.num_blocks, 256, lds4g, s>>>(T);
+    k_reduce_groups<<<dim3((unsigned)ceil_div(2 * T.max_g, 256), (unsigned)T.num_groups), 256, 0, s>>>(T, 1);
+    k_setup_finish<<<T.num_probs, 256, 0, s>>>(T);
+    return hipGetLastError();
+}
+
+hipError_t launch_macro_step(const Tables &T, int *done_counter,
+                             hipStream_t s, const LaunchHook *hook) {
+    if (T.num_probs == 0) return hipSuccess;
+    const unsigned nblk = (unsigned)T.num_blocks;
+    const int g = T.max_g, m = T.max_m;
+    auto mark = [&](int kind, int end) {
+        if (hook) hook->mark(hook->user, kind, end);
+    };
+    // Kernel classes follow enum bpca_kernel_class in bpca.h.
+    mark(0, 0);
+    k_spmv<<<nblk, 256, sizeof(float) * g, s>>>(T);
+    mark(0, 1);
+    mark(2, 0);
+    k_reorth_partials<<<nblk, 256, 0, s>>>(T);
+    k_reduce_columns<<<T.num_probs * (m + 1), 256, 0, s>>>(T);
+    k_reorth_apply<<<nblk, 256, sizeof(double) * (m + 1), s>>>(T);
+    k_alpha<<<T.num_probs, 256, 0, s>>>(T);
+    mark(2, 1);
+    mark(1, 0);
+    k_spmv_t<<<nblk, 256, sizeof(float) * ((size_t)4 * g + 4 * kSubRows), s>>>(T);
+    mark(1, 1);
+    mark(3, 0);
+    k_reduce_groups<<<dim3((unsigned)ceil_div(g, 256), (unsigned)T.num_groups), 256, 0, s>>>(T, 0);
+    k_vside<<<T.num_probs, 1024, sizeof(double) * ((size_t)g + m), s>>>(T);
+    mark(3, 1);
+    mark(4, 0);
+    k_svd<<<T.num_probs, 1024, sizeof(double) * m + sizeof(int) * m, s>>>(T);
+    k_tall_gemm<false><<<nblk, 256, sizeof(double) * m * kGemmCols, s>>>(T);
+    k_restart_v<<<T.num_probs, 1024, 0, s>>>(T);
+    mark(4, 1);
+    mark(5, 0);
+    k_tall_gemm<true><<<nblk, 256, sizeof(double) * m * kGemmCols, s>>>(T);
+    k_finish<<<T.num_probs, 64, 0, s>>>(T, done_counter);
+    mark(5, 1);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------- validation etc.
+
+template <typename IP>
+__global__ __launch_bounds__(256) void k_validate(
+    const int32_t *__restrict__ indices, const IP *__restrict__ indptr,
+    int64_t n, int g, int *flag) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int64_t row = (int64_t)blockIdx.x * 4 + wave_id();
+    if (row >= n) return;
+    const int64_t b = indptr[row], e = indptr[row + 1];
+    int bad = 0;
+    if (row == 0 && b != 0) bad = 1;
+    if (e < b) bad = 1;
+    for (int64_t p = b + lane; p < e && !bad; p += kWave) {
+        const int32_t c = indices[p];
+        const int32_t prev = p > b ? indices[p - 1] : -1;
+        if (c < 0 || c >= g || c <= prev) bad = 1;
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
+hipError_t launch_validate(const float *data, const int32_t *indices,
+                           const void *indptr, int indptr_width, int64_t n,
+                           int g, int *flag, hipStream_t s) {
+    (void)data;
+    dim3 grid((unsigned)ceil_div(n, 4));
+    if (indptr_width == 8)
+        k_validate<int64_t><<<grid, 256, 0, s>>>(indices, (const int64_t *)indptr, n, g, flag);
+    else
+        k_validate<int32_t><<<grid, 256, 0, s>>>(indices, (const int32_t *)indptr, n, g, flag);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_widen(const int32_t *__restrict__ in,
+                                               int64_t count,
+                                               int64_t *__restrict__ out) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < count; i += stride)
+        out[i] = (int64_t)in[i];
+}
+
+hipError_t launch_widen_indptr(const int32_t *in, int64_t count, int64_t *out,
+                               hipStream_t s) {
+    if (count <= 0) return hipSuccess;
+    const int grid = (int)std::min<int64_t>(ceil_div(count, 256), 65536);
+    k_widen<<<grid, 256, 0, s>>>(in, count, out);
+    return hipGetLastError();
+}
+
+__global__ __launch_bounds__(256) void k_narrow(const int64_t *__restrict__ in,
+                                                int64_t nnz, int g,
+                                                int32_t *__restrict__ out,
+                                                int *flag) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    int bad = 0;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < nnz; i += stride) {
+        const int64_t v = in[i];
+        if (v < 0 || v >= g) bad = 1;
+        out[i] = (int32_t)v;
+    }
+    if (bad) atomicOr(flag, 1);
+}
+
+hipError_t launch_narrow_indices(const int64_t *in, int64_t nnz, int g,
+                                 int32_t *out, int *flag, hipStream_t s) {
+    if (nnz <= 0) return hipSuccess;
+    const int grid = (int)std::min<int64_t>(ceil_div(nnz, 256), 65536);
+    k_narrow<<<grid, 256, 0, s>>>(in, nnz, g, out, flag);
+    return hipGetLastError();
+}
+
+// Single-matrix SVD through the same device code (test entry point).
+__global__ __launch_bounds__(1024) void k_svd_single(const double *B, int m,
+                                                     double *W, double *Jv,
+                                                     double *Ub, double *Sb,
+                                                     double *Vb, int *sweeps) {
+    __shared__ int sh_int[2];
+    extern __shared__ double sh[];
+    const int mm = m + (m & 1);
+    for (int idx = threadIdx.x; idx < m * mm; idx += blockDim.x)
+        W[idx] = idx / m < m ? B[idx] : 0.0;
+    for (int idx = threadIdx.x; idx < mm * mm; idx += blockDim.x)
+        Jv[idx] = (idx / mm == idx % mm) ? 1.0 : 0.0;
+    __syncthreads();
+    jacobi_svd(W, Jv, m, mm, &sh_int[0], &sh_int[1]);
+    __syncthreads();
+    svd_sort_outputs(W, Jv, m, mm, sh, (int *)(sh + m), Sb, Ub, Vb);
+    if (threadIdx.x == 0) *sweeps = sh_int[0];
+}
+
+hipError_t launch_svd_single(const double *B, int m, double *W, double *Jv,
+                             double *Ub, double *Sb, int *sweeps,
+                             hipStream_t s) {
+    const int mm = m + (m & 1);
+    double *Vb = Jv + (int64_t)mm * mm;
+    k_svd_single<<<1, 1024, sizeof(double) * m + sizeof(int) * m, s>>>(
+        B, m, W, Jv, Ub, Sb, Vb, sweeps);
     return hipGetLastError();
 }
 

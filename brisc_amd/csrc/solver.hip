@@ -1,14 +1,12 @@
-// Host driver of the IRLBA solve and the C ABI declared in include/bpca.h.
+// Host side of the IRLBA engine and the C ABI declared in include/bpca.h.
 //
-// The restart loop follows brisc/pca.pyx:973-1254 (parallel path):
-//   per Lanczos step j = k_current .. m-1
-//     u = Z v_j            spmv + reorth partials/apply + alpha   (device)
-//     w = Z^T u_j - a v_j  spmv_t + group sums + vside (CGS, beta) (device)
-//   then B (m x (m+1)) comes to the host once per restart for the FP64 SVD,
-//   the residual test |r Ub[m-1, i]| < tol * Smax, k_current update, and the
-//   restart U <- U Ub[:, :kc], V <- V Vbt[:kc]^T, V[:, kc] = remainder.
-// The sweep itself has no host synchronisation: kernels are queued back to
-// back on one stream and the host waits only at the end of each sweep.
+// The IRLBA of brisc/pca.pyx:973-1254 runs entirely on the device.  The host
+// admits problems (workspace, start vector, set-up kernels) and then replays
+// one captured "macro step" graph: every active problem advances one Lanczos
+// step per replay, and a problem whose sweep ends gets its small SVD,
+// convergence test and restart (or its final PCs) inside the same replay.
+// The host reads a done-counter every few replays.  A single solve is a batch
+// of one problem, so single, batched and served solves share every kernel.
 #include "bpca.h"
 
 #include <hip/hip_runtime.h>
@@ -18,14 +16,16 @@
 #include <chrono>
 #include <climits>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
 #include <vector>
 
 #include "kernels.h"
-#include "svd.h"
 
 namespace {
 
@@ -41,66 +41,65 @@ int hip_fail(hipError_t e, const char *what) {
                 std::string(what) + ": " + hipGetErrorString(e));
 }
 
-#define HIP_TRY(expr)                                    \
-    do {                                                 \
-        hipError_t _e = (expr);                          \
-        if (_e != hipSuccess) return hip_fail(_e, #expr); \
+#define HIP_TRY(expr)                                      \
+    do {                                                   \
+        hipError_t _e = (expr);                            \
+        if (_e != hipSuccess) return hip_fail(_e, #expr);  \
     } while (0)
 
-#define BPCA_TRY(expr)            \
-    do {                          \
-        int _rc = (expr);         \
+#define BPCA_TRY(expr)                  \
+    do {                                \
+        int _rc = (expr);               \
         if (_rc != BPCA_OK) return _rc; \
     } while (0)
 
 constexpr size_t kAlign = 256;
-
 inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
-// One device allocation carved into the solve's buffers.
-struct Layout {
-    size_t U, U2, u32, P, red, sq, tpart, grp, V, V2, B, sd, vs, scal, rem,
-        rscal, coef, vbt, flag, total;
+// Per-problem workspace: one allocation carved into these buffers.
+struct WsLayout {
+    size_t U0, U1, V0, V1, u32, P, red, sq, tpart, grp, B, sd, rem, vs, W, Jv,
+        Ub, Sb, indptr64, total;
 };
 
-Layout make_layout(int64_t n, int g, int m, int k) {
-    using bpca::ceil_div;
-    const int64_t nb = ceil_div(n, bpca::kRowBlock);
-    const int64_t ng = ceil_div(nb, bpca::kGroup);
-    Layout L{};
+WsLayout ws_layout(int64_t n, int g, int m, bool widen) {
+    const int64_t nb = bpca::ceil_div(n, bpca::kRowBlock);
+    const int64_t ng = bpca::ceil_div(nb, bpca::kGroup);
+    const int64_t mm = m + (m & 1);
+    WsLayout L{};
     size_t off = 0;
     auto take = [&](size_t bytes) {
         size_t at = off;
-        off += align_up(std::max<size_t>(bytes, 1));
+        off += align_up(std::max<size_t>(bytes, 8));
         return at;
     };
-    L.U = take(sizeof(double) * n * m);
-    L.U2 = take(sizeof(double) * n * m);
+    L.U0 = take(sizeof(double) * n * m);
+    L.U1 = take(sizeof(double) * n * m);
+    L.V0 = take(sizeof(double) * g * (m + 1));
+    L.V1 = take(sizeof(double) * g * (m + 1));
     L.u32 = take(sizeof(float) * n);
     L.P = take(sizeof(double) * (m + 1) * nb);
     L.red = take(sizeof(double) * (m + 1));
     L.sq = take(sizeof(double) * nb);
     L.tpart = take(sizeof(float) * nb * 2 * g);
     L.grp = take(sizeof(double) * ng * 2 * g);
-    L.V = take(sizeof(double) * g * (m + 1));
-    L.V2 = take(sizeof(double) * g * (m + 1));
     L.B = take(sizeof(double) * m * (m + 1));
     L.sd = take(sizeof(double) * g);
-    L.vs = take(sizeof(float) * g);
-    L.scal = take(sizeof(double) * 4);
     L.rem = take(sizeof(double) * g);
-    L.rscal = take(sizeof(double) * 4);
-    L.coef = take(sizeof(double) * m * std::max(m, k));
-    L.vbt = take(sizeof(double) * m * (m + 1));
-    L.flag = take(sizeof(int) * 4);
+    L.vs = take(sizeof(float) * g);
+    L.W = take(sizeof(double) * m * mm);
+    L.Jv = take(sizeof(double) * (mm * mm + (int64_t)m * m));
+    L.Ub = take(sizeof(double) * m * m);
+    L.Sb = take(sizeof(double) * m);
+    L.indptr64 = widen ? take(sizeof(int64_t) * (n + 1)) : 0;
     L.total = off;
-    (void)k;
     return L;
 }
 
 // PCG-XSH-RR 64/32 (O'Neill 2014) seeded as brisc's srand does
 // (cyutils.pyx:1153-1158, 1228-1232), with the FP32 Box-Muller draw of
-// cyutils.pyx:1173-1191; gives the start vector of pca.pyx:801-806.
+// cyutils.pyx:1173-1191: the start vector of pca.pyx:801-806, normalised
+// here in FP64.
 struct Pcg32 {
     uint64_t state;
     explicit Pcg32(uint64_t seed) : state(seed + 1442695040888963407ULL) { next(); }
@@ -122,94 +121,31 @@ struct Pcg32 {
     }
 };
 
-}  // namespace
-
-struct bpca_context {
-    int device = 0;
-    hipStream_t own_stream = nullptr;
-    hipStream_t stream = nullptr;
-    uint64_t budget = 0;
-    size_t max_lds = 65536;  // dynamic LDS per workgroup on this device
-    char *ws = nullptr;
-    size_t ws_bytes = 0;
-    double *host = nullptr;  // pinned staging: B, coef, Vbt, v0, rscal
-    size_t host_doubles = 0;
-    std::atomic<int> cancel{0};
-    std::mutex lock;
-    // Kernel timing: events recorded around launch classes, resolved after
-    // each host synchronisation.
-    bool profiling = false;
-    std::vector<hipEvent_t> events;
-    size_t next_event = 0;
-    struct Pending { int kind, a, b; };
-    std::vector<Pending> pending;
-    double k_ms[BPCA_NUM_KERNEL_CLASSES] = {};
-    int64_t k_count[BPCA_NUM_KERNEL_CLASSES] = {};
-};
-
-namespace {
-
-int ensure_workspace(bpca_context *ctx, size_t bytes) {
-    if (ctx->ws_bytes >= bytes) return BPCA_OK;
-    if (ctx->ws) {
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        HIP_TRY(hipFree(ctx->ws));
-        ctx->ws = nullptr;
-        ctx->ws_bytes = 0;
+std::vector<double> start_vector(uint64_t seed, int g) {
+    Pcg32 rng(seed);
+    std::vector<double> v(g);
+    double nrm = 0.0;
+    for (int i = 0; i < g; i++) {
+        v[i] = (double)rng.normal();
+        nrm += v[i] * v[i];
     }
-    hipError_t e = hipMalloc((void **)&ctx->ws, bytes);
-    if (e != hipSuccess) {
-        ctx->ws = nullptr;
-        return hip_fail(e, "hipMalloc(workspace)");
-    }
-    ctx->ws_bytes = bytes;
+    nrm = std::sqrt(nrm);
+    for (int i = 0; i < g; i++) v[i] /= nrm;
+    return v;
+}
+
+int check_shape(int64_t n, int64_t g, int64_t k, int64_t m, float tol,
+                int64_t maxit) {
+    if (n < 2) return fail(BPCA_ERR_INVALID_ARGUMENT, "there is only one cell, so principal components cannot be calculated");
+    if (g < 2) return fail(BPCA_ERR_INVALID_ARGUMENT, "there is only one gene, so principal components cannot be calculated");
+    if (k < 1) return fail(BPCA_ERR_INVALID_ARGUMENT, "k must be a positive integer");
+    if (m < k) return fail(BPCA_ERR_INVALID_ARGUMENT, "subspace_size must be >= k");
+    if (m > n || m > g) return fail(BPCA_ERR_INVALID_ARGUMENT, "subspace_size must be <= min(num_cells, num_genes)");
+    if (!(tol > 0.0f)) return fail(BPCA_ERR_INVALID_ARGUMENT, "tolerance must be positive");
+    if (maxit < 1) return fail(BPCA_ERR_INVALID_ARGUMENT, "max_iterations must be a positive integer");
+    if (g > bpca::kMaxGenes) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_genes above 8192 is not supported by the device path");
+    if (bpca::ceil_div(n, bpca::kRowBlock) > INT32_MAX) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells too large");
     return BPCA_OK;
-}
-
-int ensure_host(bpca_context *ctx, size_t doubles) {
-    if (ctx->host_doubles >= doubles) return BPCA_OK;
-    if (ctx->host) {
-        HIP_TRY(hipStreamSynchronize(ctx->stream));
-        HIP_TRY(hipHostFree(ctx->host));
-        ctx->host = nullptr;
-    }
-    HIP_TRY(hipHostMalloc((void **)&ctx->host, doubles * sizeof(double), hipHostMallocDefault));
-    ctx->host_doubles = doubles;
-    return BPCA_OK;
-}
-
-int prof_record(bpca_context *ctx) {
-    if (ctx->next_event == ctx->events.size()) {
-        hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return -1;
-        ctx->events.push_back(e);
-    }
-    int i = (int)ctx->next_event++;
-    if (hipEventRecord(ctx->events[i], ctx->stream) != hipSuccess) return -1;
-    return i;
-}
-
-inline int prof_begin(bpca_context *ctx) {
-    return ctx->profiling ? prof_record(ctx) : -1;
-}
-
-inline void prof_end(bpca_context *ctx, int kind, int a) {
-    if (a < 0) return;
-    int b = prof_record(ctx);
-    if (b >= 0) ctx->pending.push_back({kind, a, b});
-}
-
-// Call after the stream has been synchronised.
-void prof_flush(bpca_context *ctx) {
-    for (const auto &p : ctx->pending) {
-        float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, ctx->events[p.a], ctx->events[p.b]) == hipSuccess) {
-            ctx->k_ms[p.kind] += ms;
-            ctx->k_count[p.kind] += 1;
-        }
-    }
-    ctx->pending.clear();
-    ctx->next_event = 0;
 }
 
 struct DeviceGuard {
@@ -223,191 +159,665 @@ struct DeviceGuard {
     }
 };
 
-int check_shape(int64_t n, int64_t g, int64_t k, int64_t m, float tol,
-                int64_t maxit) {
-    if (n < 2) return fail(BPCA_ERR_INVALID_ARGUMENT, "there is only one cell, so principal components cannot be calculated");
-    if (g < 2) return fail(BPCA_ERR_INVALID_ARGUMENT, "there is only one gene, so principal components cannot be calculated");
-    if (k < 1) return fail(BPCA_ERR_INVALID_ARGUMENT, "k must be a positive integer");
-    if (m < k) return fail(BPCA_ERR_INVALID_ARGUMENT, "subspace_size must be >= k");
-    if (m > n || m > g) return fail(BPCA_ERR_INVALID_ARGUMENT, "subspace_size must be <= min(num_cells, num_genes)");
-    if (!(tol > 0.0f)) return fail(BPCA_ERR_INVALID_ARGUMENT, "tolerance must be positive");
-    if (maxit < 1) return fail(BPCA_ERR_INVALID_ARGUMENT, "max_iterations must be a positive integer");
-    if (g > bpca::kMaxGenesTranspose) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_genes above 8192 is not supported by the device path");
-    if (n > (int64_t)INT32_MAX * (int64_t)bpca::kRowBlock) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells too large");
+}  // namespace
+
+// ------------------------------------------------------------ context
+
+struct bpca_context {
+    int device = 0;
+    hipStream_t stream = nullptr;       // the engine's own stream
+    hipStream_t user_stream = nullptr;  // caller's stream to order against
+    hipEvent_t order_event = nullptr;
+    uint64_t budget = 0;                // 0 = unlimited
+    size_t max_lds = 65536;
+    std::atomic<int> cancel{0};
+    std::recursive_mutex lock;
+    // Device memory pool of whole workspaces, reused across problems.
+    struct Block { void *p; size_t bytes; };
+    std::vector<Block> free_blocks;
+    std::map<void *, size_t> live_blocks;
+    uint64_t live_bytes = 0, cached_bytes = 0;
+    // Kernel timing: with profiling on, macro steps are launched directly
+    // with events around each kernel class on the engine's stream.
+    bool profiling = false;
+    std::vector<hipEvent_t> events;
+    size_t next_event = 0;
+    struct Pending { int kind, a, b; };
+    std::vector<Pending> pending;
+    int open_event[BPCA_NUM_KERNEL_CLASSES] = {};
+    double k_ms[BPCA_NUM_KERNEL_CLASSES] = {};
+    int64_t k_count[BPCA_NUM_KERNEL_CLASSES] = {};
+    bool use_graph = true;
+};
+
+namespace {
+
+int pool_alloc(bpca_context *ctx, size_t bytes, void **out) {
+    bytes = align_up(bytes);
+    int best = -1;  // best fit among cached blocks at most twice the request
+    for (size_t i = 0; i < ctx->free_blocks.size(); i++) {
+        const auto &b = ctx->free_blocks[i];
+        if (b.bytes >= bytes && b.bytes <= 2 * bytes &&
+            (best < 0 || b.bytes < ctx->free_blocks[best].bytes))
+            best = (int)i;
+    }
+    if (best >= 0) {
+        auto b = ctx->free_blocks[best];
+        ctx->free_blocks.erase(ctx->free_blocks.begin() + best);
+        ctx->cached_bytes -= b.bytes;
+        ctx->live_blocks[b.p] = b.bytes;
+        ctx->live_bytes += b.bytes;
+        *out = b.p;
+        return BPCA_OK;
+    }
+    if (ctx->budget && ctx->live_bytes + bytes > ctx->budget)
+        return fail(BPCA_ERR_BUDGET, "request does not fit the context's HBM budget now");
+    while (ctx->budget && !ctx->free_blocks.empty() &&
+           ctx->live_bytes + ctx->cached_bytes + bytes > ctx->budget) {
+        auto b = ctx->free_blocks.back();
+        ctx->free_blocks.pop_back();
+        ctx->cached_bytes -= b.bytes;
+        HIP_TRY(hipFree(b.p));
+    }
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e == hipErrorOutOfMemory && !ctx->free_blocks.empty()) {
+        (void)hipGetLastError();
+        for (auto &b : ctx->free_blocks) (void)hipFree(b.p);
+        ctx->free_blocks.clear();
+        ctx->cached_bytes = 0;
+        e = hipMalloc(&p, bytes);
+    }
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return hip_fail(e, "hipMalloc(workspace)");
+    }
+    ctx->live_blocks[p] = bytes;
+    ctx->live_bytes += bytes;
+    *out = p;
     return BPCA_OK;
 }
 
-// The solve proper, on device-resident CSR.
-int solve(bpca_context *ctx, const bpca::Csr &X, int k, int m, float tol,
-          int maxit, uint64_t seed, bool validate, float *d_PCs,
-          double *sv_out, bpca_solve_stats *st) {
-    using namespace bpca;
-    const auto t_start = std::chrono::steady_clock::now();
-    const int64_t n = X.n;
-    const int g = X.g;
-    BPCA_TRY(check_shape(n, g, k, m, tol, maxit));
-    if (lds_bytes_needed(g, m) > ctx->max_lds)
-        return fail(BPCA_ERR_INVALID_ARGUMENT,
-                    "num_genes / subspace_size need more LDS per workgroup than this device has (" +
-                        std::to_string(lds_bytes_needed(g, m)) + " > " + std::to_string(ctx->max_lds) + " bytes)");
-    const int64_t nb = ceil_div(n, kRowBlock);
-    const int64_t ngrp = ceil_div(nb, kGroup);
-    const Layout L = make_layout(n, g, m, k);
-    if (ctx->budget && L.total > ctx->budget)
-        return fail(BPCA_ERR_BUDGET, "solve workspace exceeds the context's HBM budget");
-    BPCA_TRY(ensure_workspace(ctx, L.total));
-    const size_t hB = 0, hCoef = hB + (size_t)m * (m + 1),
-                 hVbt = hCoef + (size_t)m * std::max(m, k),
-                 hV0 = hVbt + (size_t)m * (m + 1), hR = hV0 + g,
-                 hEnd = hR + 4;
-    BPCA_TRY(ensure_host(ctx, hEnd));
-    hipStream_t s = ctx->stream;
-    char *base = ctx->ws;
-    double *U = (double *)(base + L.U), *U2 = (double *)(base + L.U2);
-    float *u32 = (float *)(base + L.u32);
-    double *P = (double *)(base + L.P), *red = (double *)(base + L.red);
-    double *sq = (double *)(base + L.sq);
-    float *tpart = (float *)(base + L.tpart);
-    double *grp = (double *)(base + L.grp);
-    double *V = (double *)(base + L.V), *V2 = (double *)(base + L.V2);
-    double *B = (double *)(base + L.B), *sd = (double *)(base + L.sd);
-    float *vs = (float *)(base + L.vs);
-    double *scal = (double *)(base + L.scal), *rem = (double *)(base + L.rem);
-    double *rscal = (double *)(base + L.rscal);
-    double *coef = (double *)(base + L.coef), *vbt = (double *)(base + L.vbt);
-    int *flag = (int *)(base + L.flag);
-    double *H = ctx->host;
+void pool_free(bpca_context *ctx, void *p) {
+    auto it = ctx->live_blocks.find(p);
+    if (it == ctx->live_blocks.end()) return;
+    ctx->free_blocks.push_back({p, it->second});
+    ctx->cached_bytes += it->second;
+    ctx->live_bytes -= it->second;
+    ctx->live_blocks.erase(it);
+}
 
+int prof_record(bpca_context *ctx) {
+    if (ctx->next_event == ctx->events.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return -1;
+        ctx->events.push_back(e);
+    }
+    int i = (int)ctx->next_event++;
+    if (hipEventRecord(ctx->events[i], ctx->stream) != hipSuccess) return -1;
+    return i;
+}
+
+void prof_mark(void *user, int kind, int end) {
+    bpca_context *ctx = (bpca_context *)user;
+    if (!end) {
+        ctx->open_event[kind] = prof_record(ctx);
+    } else {
+        int a = ctx->open_event[kind];
+        int b = prof_record(ctx);
+        if (a >= 0 && b >= 0) ctx->pending.push_back({kind, a, b});
+    }
+}
+
+void prof_flush(bpca_context *ctx) {
+    for (const auto &p : ctx->pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, ctx->events[p.a], ctx->events[p.b]) == hipSuccess) {
+            ctx->k_ms[p.kind] += ms;
+            ctx->k_count[p.kind] += 1;
+        }
+    }
     ctx->pending.clear();
     ctx->next_event = 0;
-    int tp = prof_begin(ctx);
-    if (validate) {
-        HIP_TRY(hipMemsetAsync(flag, 0, sizeof(int), s));
-        HIP_TRY(launch_validate(X, flag, s));
+}
+
+void begin_call(bpca_context *ctx) {
+    // Order the engine's stream after work already queued on the caller's.
+    if (ctx->order_event) {
+        (void)hipEventRecord(ctx->order_event, ctx->user_stream);
+        (void)hipStreamWaitEvent(ctx->stream, ctx->order_event, 0);
+    }
+}
+
+struct Slot {
+    bool used = false;
+    bpca::Prob desc{};   // host copy of the descriptor as admitted
+    void *ws = nullptr;
+    std::chrono::steady_clock::time_point t_admit;
+};
+
+}  // namespace
+
+struct bpca_engine {
+    bpca_context *ctx = nullptr;
+    std::vector<Slot> slots;
+    bpca::Prob *d_probs = nullptr;
+    int capacity = 0;
+    int2 *d_blocks = nullptr, *d_groups = nullptr;
+    int64_t blocks_cap = 0, groups_cap = 0;
+    int *d_done = nullptr;
+    int *h_done = nullptr;   // pinned
+    bpca::Tables T{};
+    bool dirty = true, setup_pending = false;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    int done_seen = 0;       // finished problems already collected
+    int64_t replays = 0;
+};
+
+namespace {
+
+void engine_drop_graph(bpca_engine *E) {
+    if (E->exec) (void)hipGraphExecDestroy(E->exec);
+    if (E->graph) (void)hipGraphDestroy(E->graph);
+    E->exec = nullptr;
+    E->graph = nullptr;
+}
+
+int engine_rebuild_tables(bpca_engine *E) {
+    bpca_context *ctx = E->ctx;
+    std::vector<int2> blocks, groups;
+    int max_g = 2, max_m = 1;
+    for (int s = 0; s < (int)E->slots.size(); s++) {
+        const Slot &sl = E->slots[s];
+        if (!sl.used) continue;
+        for (int64_t b = 0; b < sl.desc.nb; b++) blocks.push_back(make_int2(s, (int)b));
+        for (int64_t G = 0; G < sl.desc.ngrp; G++) groups.push_back(make_int2(s, (int)G));
+        max_g = std::max(max_g, sl.desc.g);
+        max_m = std::max(max_m, sl.desc.m);
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if ((int64_t)blocks.size() > E->blocks_cap) {
+        if (E->d_blocks) HIP_TRY(hipFree(E->d_blocks));
+        E->d_blocks = nullptr;
+        E->blocks_cap = std::max<int64_t>(blocks.size(), 2 * E->blocks_cap);
+        HIP_TRY(hipMalloc((void **)&E->d_blocks, sizeof(int2) * E->blocks_cap));
+    }
+    if ((int64_t)groups.size() > E->groups_cap) {
+        if (E->d_groups) HIP_TRY(hipFree(E->d_groups));
+        E->d_groups = nullptr;
+        E->groups_cap = std::max<int64_t>(groups.size(), 2 * E->groups_cap);
+        HIP_TRY(hipMalloc((void **)&E->d_groups, sizeof(int2) * E->groups_cap));
+    }
+    if (!blocks.empty())
+        HIP_TRY(hipMemcpyAsync(E->d_blocks, blocks.data(), sizeof(int2) * blocks.size(), hipMemcpyHostToDevice, ctx->stream));
+    if (!groups.empty())
+        HIP_TRY(hipMemcpyAsync(E->d_groups, groups.data(), sizeof(int2) * groups.size(), hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    E->T.probs = E->d_probs;
+    E->T.num_probs = (int)E->slots.size();
+    E->T.blocks = E->d_blocks;
+    E->T.num_blocks = (int64_t)blocks.size();
+    E->T.groups = E->d_groups;
+    E->T.num_groups = (int64_t)groups.size();
+    E->T.max_g = max_g;
+    E->T.max_m = max_m;
+    engine_drop_graph(E);
+    E->dirty = false;
+    return BPCA_OK;
+}
+
+int engine_launch_steps(bpca_engine *E, int count) {
+    bpca_context *ctx = E->ctx;
+    hipStream_t s = ctx->stream;
+    if (E->T.num_blocks == 0) return BPCA_OK;
+    if (ctx->profiling || !ctx->use_graph) {
+        bpca::LaunchHook hook{prof_mark, ctx};
+        for (int i = 0; i < count; i++)
+            HIP_TRY(bpca::launch_macro_step(E->T, E->d_done, s, ctx->profiling ? &hook : nullptr));
+        return BPCA_OK;
+    }
+    if (!E->exec) {
+        HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        hipError_t e = bpca::launch_macro_step(E->T, E->d_done, s, nullptr);
+        hipGraph_t g = nullptr;
+        hipError_t e2 = hipStreamEndCapture(s, &g);
+        if (e != hipSuccess || e2 != hipSuccess) {
+            if (g) (void)hipGraphDestroy(g);
+            return hip_fail(e != hipSuccess ? e : e2, "macro-step graph capture");
+        }
+        E->graph = g;
+        HIP_TRY(hipGraphInstantiate(&E->exec, E->graph, nullptr, nullptr, 0));
+    }
+    for (int i = 0; i < count; i++) HIP_TRY(hipGraphLaunch(E->exec, s));
+    E->replays += count;
+    return BPCA_OK;
+}
+
+int engine_grow(bpca_engine *E, int need) {
+    if (need <= E->capacity) return BPCA_OK;
+    const int cap = std::max(need, std::max(4, 2 * E->capacity));
+    bpca::Prob *np = nullptr;
+    HIP_TRY(hipMalloc((void **)&np, sizeof(bpca::Prob) * cap));
+    if (E->d_probs) {
+        HIP_TRY(hipMemcpyAsync(np, E->d_probs, sizeof(bpca::Prob) * E->capacity, hipMemcpyDeviceToDevice, E->ctx->stream));
+        HIP_TRY(hipStreamSynchronize(E->ctx->stream));
+        HIP_TRY(hipFree(E->d_probs));
+    }
+    E->d_probs = np;
+    E->capacity = cap;
+    E->dirty = true;
+    return BPCA_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ------------------------------------------------------------ context ABI
+
+const char *bpca_last_error(void) { return g_last_error.c_str(); }
+
+int bpca_version(void) { return 1 * 10000 + 1 * 100 + 0; }
+
+int bpca_context_create(int device, uint64_t hbm_budget_bytes,
+                        bpca_context **out) {
+    if (!out) return fail(BPCA_ERR_INVALID_ARGUMENT, "out is NULL");
+    *out = nullptr;
+    int count = 0;
+    HIP_TRY(hipGetDeviceCount(&count));
+    if (device < 0 || device >= count)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "no such device");
+    bpca_context *ctx = new (std::nothrow) bpca_context();
+    if (!ctx) return fail(BPCA_ERR_OUT_OF_MEMORY, "context allocation failed");
+    ctx->device = device;
+    ctx->budget = hbm_budget_bytes;
+    DeviceGuard dg(device);
+    hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->order_event, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return hip_fail(e, "context stream");
+    }
+    int lds = 0;
+    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
+        ctx->max_lds = (size_t)lds;
+    const char *env = std::getenv("BPCA_NO_GRAPH");
+    ctx->use_graph = !(env && env[0] == '1');
+    *out = ctx;
+    return BPCA_OK;
+}
+
+int bpca_context_destroy(bpca_context *ctx) {
+    if (!ctx) return BPCA_OK;
+    {
+        DeviceGuard dg(ctx->device);
+        (void)hipStreamSynchronize(ctx->stream);
+        for (auto &b : ctx->free_blocks) (void)hipFree(b.p);
+        for (auto &kv : ctx->live_blocks) (void)hipFree(kv.first);
+        for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+        if (ctx->order_event) (void)hipEventDestroy(ctx->order_event);
+        (void)hipStreamDestroy(ctx->stream);
+    }
+    delete ctx;
+    return BPCA_OK;
+}
+
+int bpca_context_set_stream(bpca_context *ctx, void *hip_stream) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    ctx->user_stream = (hipStream_t)hip_stream;
+    return BPCA_OK;
+}
+
+int bpca_context_cancel(bpca_context *ctx) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    ctx->cancel.store(1);
+    return BPCA_OK;
+}
+
+int bpca_context_set_profiling(bpca_context *ctx, int enable) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    ctx->profiling = enable != 0;
+    return BPCA_OK;
+}
+
+int bpca_context_kernel_times(bpca_context *ctx, double *total_ms,
+                              int64_t *launches) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    for (int i = 0; i < BPCA_NUM_KERNEL_CLASSES; i++) {
+        if (total_ms) total_ms[i] = ctx->k_ms[i];
+        if (launches) launches[i] = ctx->k_count[i];
+    }
+    return BPCA_OK;
+}
+
+int bpca_context_reset_kernel_times(bpca_context *ctx) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    for (int i = 0; i < BPCA_NUM_KERNEL_CLASSES; i++) {
+        ctx->k_ms[i] = 0.0;
+        ctx->k_count[i] = 0;
+    }
+    return BPCA_OK;
+}
+
+uint64_t bpca_workspace_bytes(uint64_t num_cells, uint32_t num_genes,
+                              uint32_t k, uint32_t subspace_size) {
+    (void)k;
+    return ws_layout((int64_t)num_cells, (int)num_genes, (int)subspace_size,
+                     true).total;
+}
+
+// ------------------------------------------------------------ engine ABI
+
+int bpca_engine_create(bpca_context *ctx, bpca_engine **out) {
+    if (!ctx || !out) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *out = nullptr;
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    auto *E = new (std::nothrow) bpca_engine();
+    if (!E) return fail(BPCA_ERR_OUT_OF_MEMORY, "engine allocation failed");
+    E->ctx = ctx;
+    hipError_t e = hipMalloc((void **)&E->d_done, sizeof(int));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&E->h_done, sizeof(int), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipMemsetAsync(E->d_done, 0, sizeof(int), ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+    if (e != hipSuccess) {
+        if (E->d_done) (void)hipFree(E->d_done);
+        if (E->h_done) (void)hipHostFree(E->h_done);
+        delete E;
+        return hip_fail(e, "engine counters");
+    }
+    *E->h_done = 0;
+    *out = E;
+    return BPCA_OK;
+}
+
+int bpca_engine_destroy(bpca_engine *E) {
+    if (!E) return BPCA_OK;
+    bpca_context *ctx = E->ctx;
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    engine_drop_graph(E);
+    for (auto &sl : E->slots)
+        if (sl.used && sl.ws) pool_free(ctx, sl.ws);
+    if (E->d_probs) (void)hipFree(E->d_probs);
+    if (E->d_blocks) (void)hipFree(E->d_blocks);
+    if (E->d_groups) (void)hipFree(E->d_groups);
+    if (E->d_done) (void)hipFree(E->d_done);
+    if (E->h_done) (void)hipHostFree(E->h_done);
+    delete E;
+    return BPCA_OK;
+}
+
+int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
+    if (!E || !pb) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL argument");
+    bpca_context *ctx = E->ctx;
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    if (!pb->data || !pb->indices || !pb->indptr || !pb->PCs)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL device pointer in problem");
+    if (pb->indptr_width != 4 && pb->indptr_width != 8)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "indptr_width must be 4 or 8");
+    const int64_t n = (int64_t)pb->num_cells;
+    const int g = (int)pb->num_genes, k = (int)pb->k, m = (int)pb->subspace_size;
+    BPCA_TRY(check_shape(n, g, k, m, pb->tolerance, pb->max_iterations));
+    if (bpca::lds_bytes_needed(g, m) > ctx->max_lds)
+        return fail(BPCA_ERR_INVALID_ARGUMENT,
+                    "num_genes / subspace_size need " + std::to_string(bpca::lds_bytes_needed(g, m)) +
+                        " bytes of LDS per workgroup; the device has " + std::to_string(ctx->max_lds));
+    hipStream_t s = ctx->stream;
+    begin_call(ctx);
+    if (pb->validate) {
+        int *flag = nullptr;
+        HIP_TRY(hipMalloc((void **)&flag, sizeof(int)));
         int h_flag = 0;
-        HIP_TRY(hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), s);
+        if (e == hipSuccess) e = bpca::launch_validate(pb->data, pb->indices, pb->indptr, pb->indptr_width, n, g, flag, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        (void)hipFree(flag);
+        if (e != hipSuccess) return hip_fail(e, "CSR validation");
         if (h_flag)
             return fail(BPCA_ERR_NONCANONICAL,
                         "CSR is not canonical: indptr must start at 0 and be non-decreasing, and each row's column indices strictly increasing and below num_genes");
     }
-
-    // Per-gene clipped standard deviation.
-    HIP_TRY(launch_colstats(X, tpart, s));
-    HIP_TRY(launch_reduce_groups(tpart, nb, 2 * g, grp, s));
-    HIP_TRY(launch_finish_stddev(grp, ngrp, g, n, sd, s));
-    prof_end(ctx, BPCA_K_SETUP, tp);
-
-    // Start vector: FP32 Box-Muller draws, normalised in FP64.
-    {
-        Pcg32 rng(seed);
-        double nrm = 0.0;
-        for (int i = 0; i < g; i++) {
-            double x = (double)rng.normal();
-            H[hV0 + i] = x;
-            nrm += x * x;
-        }
-        nrm = std::sqrt(nrm);
-        for (int i = 0; i < g; i++) H[hV0 + i] /= nrm;
-        HIP_TRY(hipMemcpyAsync(V, H + hV0, sizeof(double) * g, hipMemcpyHostToDevice, s));
+    const bool widen = pb->indptr_width == 4;
+    const WsLayout L = ws_layout(n, g, m, widen);
+    void *ws = nullptr;
+    BPCA_TRY(pool_alloc(ctx, L.total, &ws));
+    int slot = -1;
+    for (int i = 0; i < (int)E->slots.size(); i++)
+        if (!E->slots[i].used) { slot = i; break; }
+    if (slot < 0) {
+        slot = (int)E->slots.size();
+        E->slots.emplace_back();
     }
-    HIP_TRY(launch_init_vs(V, sd, g, vs, s));
-    HIP_TRY(hipMemsetAsync(B, 0, sizeof(double) * m * (m + 1), s));
+    int rc = engine_grow(E, (int)E->slots.size());
+    if (rc != BPCA_OK) {
+        pool_free(ctx, ws);
+        return rc;
+    }
+    char *base = (char *)ws;
+    bpca::Prob d{};
+    d.data = pb->data;
+    d.indices = pb->indices;
+    d.indptr = widen ? (const int64_t *)(base + L.indptr64) : (const int64_t *)pb->indptr;
+    d.PCs = pb->PCs;
+    d.n = n;
+    d.nb = bpca::ceil_div(n, bpca::kRowBlock);
+    d.ngrp = bpca::ceil_div(d.nb, bpca::kGroup);
+    d.g = g;
+    d.k = k;
+    d.m = m;
+    d.maxit = (int)pb->max_iterations;
+    d.tol = pb->tolerance;
+    d.id = slot;
+    d.U[0] = (double *)(base + L.U0);
+    d.U[1] = (double *)(base + L.U1);
+    d.V[0] = (double *)(base + L.V0);
+    d.V[1] = (double *)(base + L.V1);
+    d.u32 = (float *)(base + L.u32);
+    d.P = (double *)(base + L.P);
+    d.red = (double *)(base + L.red);
+    d.sq = (double *)(base + L.sq);
+    d.tpart = (float *)(base + L.tpart);
+    d.grp = (double *)(base + L.grp);
+    d.B = (double *)(base + L.B);
+    d.sd = (double *)(base + L.sd);
+    d.rem = (double *)(base + L.rem);
+    d.vs = (float *)(base + L.vs);
+    d.W = (double *)(base + L.W);
+    d.Jv = (double *)(base + L.Jv);
+    d.Ub = (double *)(base + L.Ub);
+    d.Sb = (double *)(base + L.Sb);
+    d.phase = bpca::PH_SETUP;
+    const std::vector<double> v0 = start_vector(pb->seed, g);
+    hipError_t e = hipSuccess;
+    if (widen) e = bpca::launch_widen_indptr((const int32_t *)pb->indptr, n + 1, (int64_t *)(base + L.indptr64), s);
+    if (e == hipSuccess) e = hipMemcpyAsync(d.V[0], v0.data(), sizeof(double) * g, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(E->d_probs + slot, &d, sizeof(d), hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) {
+        pool_free(ctx, ws);
+        return hip_fail(e, "problem upload");
+    }
+    Slot &sl = E->slots[slot];
+    sl.used = true;
+    sl.desc = d;
+    sl.ws = ws;
+    sl.t_admit = std::chrono::steady_clock::now();
+    E->dirty = true;
+    E->setup_pending = true;
+    if (slot_out) *slot_out = slot;
+    return BPCA_OK;
+}
 
-    std::vector<double> Ub((size_t)m * m), Sb(m), Vbt((size_t)m * (m + 1));
-    double Smax = 2.4221817809573368e-05;  // FP32 eps^(2/3), pca.pyx:688
-    int kc = 0, iteration = 0;
-    int64_t steps = 0;
-    bool converged = false;
-    double svd_seconds = 0.0;
-    for (iteration = 0; iteration < maxit; iteration++) {
-        for (int j = kc; j < m; j++) {
-            if (ctx->cancel.load(std::memory_order_relaxed)) {
-                (void)hipStreamSynchronize(s);
-                ctx->cancel.store(0);
-                return fail(BPCA_ERR_CANCELLED, "solve cancelled");
-            }
-            tp = prof_begin(ctx);
-            HIP_TRY(launch_spmv(X, vs, u32, 0, s));
-            prof_end(ctx, BPCA_K_SPMV, tp);
-            tp = prof_begin(ctx);
-            HIP_TRY(launch_reorth_partials(u32, U, n, j, P, s));
-            HIP_TRY(launch_reduce_columns(P, nb, j + 1, red, s));
-            HIP_TRY(launch_reorth_apply(u32, U, n, j, red, sq, s));
-            HIP_TRY(launch_alpha(sq, nb, scal, B, m, j, s));
-            prof_end(ctx, BPCA_K_REORTH, tp);
-            tp = prof_begin(ctx);
-            HIP_TRY(launch_spmv_t(X, U + (int64_t)j * n, scal, tpart, s));
-            prof_end(ctx, BPCA_K_SPMV_T, tp);
-            tp = prof_begin(ctx);
-            HIP_TRY(launch_reduce_groups(tpart, nb, g, grp, s));
-            HIP_TRY(launch_vside(grp, ngrp, g, sd, V, j, m, B, scal, rem, rscal, vs, s));
-            prof_end(ctx, BPCA_K_VSIDE, tp);
-            steps++;
+int bpca_engine_step(bpca_engine *E, int32_t max_steps, int32_t *num_done) {
+    if (!E) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL engine");
+    bpca_context *ctx = E->ctx;
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = ctx->stream;
+    begin_call(ctx);
+    if (E->dirty) BPCA_TRY(engine_rebuild_tables(E));
+    if (E->setup_pending) {
+        if (ctx->profiling) prof_mark(ctx, BPCA_K_SETUP, 0);
+        HIP_TRY(bpca::launch_setup(E->T, s));
+        if (ctx->profiling) prof_mark(ctx, BPCA_K_SETUP, 1);
+        E->setup_pending = false;
+    }
+    int remaining = std::max(0, (int)max_steps);
+    while (remaining > 0) {
+        if (ctx->cancel.load()) {
+            ctx->cancel.store(0);
+            (void)hipStreamSynchronize(s);
+            return fail(BPCA_ERR_CANCELLED, "solve cancelled");
         }
-        HIP_TRY(hipMemcpyAsync(H + hB, B, sizeof(double) * m * (m + 1), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipMemcpyAsync(H + hR, rscal, sizeof(double), hipMemcpyDeviceToHost, s));
+        const int batch = std::min(remaining, 8);
+        BPCA_TRY(engine_launch_steps(E, batch));
+        remaining -= batch;
+        HIP_TRY(hipMemcpyAsync(E->h_done, E->d_done, sizeof(int), hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
-        prof_flush(ctx);
-        const auto t0 = std::chrono::steady_clock::now();
-        if (bidiag_svd(m, H + hB, Ub.data(), Sb.data(), Vbt.data()) != 0)
-            return fail(BPCA_ERR_NUMERICAL, "SVD of the projected matrix did not converge");
-        svd_seconds += std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
-        const double rnorm = H[hR];
-        Smax = std::max(Sb[0], Smax);
-        int nconv = 0;
-        for (int i = 0; i < k; i++) {
-            double res = rnorm * Ub[(m - 1) + (size_t)i * m];
-            nconv += std::fabs(res) < (double)tol * Smax;
-        }
-        if (nconv == k) {
-            converged = true;
-            break;
-        }
-        if (iteration == maxit - 1) break;  // keep U, Ub consistent: no restart
-        kc = std::min(std::max(kc, k + nconv), m - 1);
-        std::copy(Ub.begin(), Ub.begin() + (size_t)m * kc, H + hCoef);
-        std::copy(Vbt.begin(), Vbt.end(), H + hVbt);
-        HIP_TRY(hipMemcpyAsync(coef, H + hCoef, sizeof(double) * m * kc, hipMemcpyHostToDevice, s));
-        HIP_TRY(hipMemcpyAsync(vbt, H + hVbt, sizeof(double) * m * (m + 1), hipMemcpyHostToDevice, s));
-        tp = prof_begin(ctx);
-        HIP_TRY(launch_tall_gemm_f64(U, n, n, m, coef, kc, U2, n, s));
-        std::swap(U, U2);
-        HIP_TRY(launch_v_restart(V, g, m, vbt, kc, rem, V2, s));
-        std::swap(V, V2);
-        prof_end(ctx, BPCA_K_RESTART, tp);
-        std::fill(H + hB, H + hB + (size_t)m * (m + 1), 0.0);
-        for (int i = 0; i < kc; i++) {
-            H[hB + i + (size_t)i * m] = Sb[i];
-            H[hB + i + (size_t)kc * m] = rnorm * Ub[(m - 1) + (size_t)i * m];
-        }
-        HIP_TRY(hipMemcpyAsync(B, H + hB, sizeof(double) * m * (m + 1), hipMemcpyHostToDevice, s));
-        HIP_TRY(launch_init_vs(V + (int64_t)kc * g, sd, g, vs, s));
+        if (ctx->profiling) prof_flush(ctx);
+        if (*E->h_done > E->done_seen) break;
     }
+    if (num_done) *num_done = *E->h_done - E->done_seen;
+    return BPCA_OK;
+}
 
-    // PCs = U (Ub[:, :k] diag(Sb[:k])), float32 row-major.
-    for (int c = 0; c < k; c++)
-        for (int l = 0; l < m; l++)
-            H[hCoef + l + (size_t)c * m] = Ub[l + (size_t)c * m] * Sb[c];
-    HIP_TRY(hipMemcpyAsync(coef, H + hCoef, sizeof(double) * m * k, hipMemcpyHostToDevice, s));
-    tp = prof_begin(ctx);
-    HIP_TRY(launch_tall_gemm_rowmajor_f32(U, n, n, m, coef, k, d_PCs, s));
-    prof_end(ctx, BPCA_K_FINAL, tp);
+int bpca_engine_collect(bpca_engine *E, int32_t max_out, int32_t *slots,
+                        bpca_solve_stats *stats, double *singular_values,
+                        int32_t sv_stride, int32_t *num_out) {
+    if (!E) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL engine");
+    bpca_context *ctx = E->ctx;
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = ctx->stream;
+    int nout = 0;
+    const int nslots = (int)E->slots.size();
+    if (num_out) *num_out = 0;
+    if (nslots == 0) return BPCA_OK;
+    std::vector<bpca::Prob> cur(nslots);
+    HIP_TRY(hipMemcpyAsync(cur.data(), E->d_probs, sizeof(bpca::Prob) * nslots, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
-    prof_flush(ctx);
-    if (sv_out)
-        for (int c = 0; c < k; c++) sv_out[c] = Sb[c];
-    if (st) {
-        st->converged = converged ? 1 : 0;
-        st->restarts = std::min(iteration + 1, maxit);
-        st->lanczos_steps = steps;
-        st->matvecs = 2 * steps;
-        st->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t_start).count();
-        st->svd_seconds = svd_seconds;
-        st->smax = Smax;
+    const auto now = std::chrono::steady_clock::now();
+    for (int i = 0; i < nslots && nout < max_out; i++) {
+        Slot &sl = E->slots[i];
+        if (!sl.used || cur[i].phase != bpca::PH_DONE) continue;
+        if (singular_values)
+            HIP_TRY(hipMemcpyAsync(singular_values + (int64_t)nout * sv_stride, sl.desc.Sb,
+                                   sizeof(double) * sl.desc.k, hipMemcpyDeviceToHost, s));
+        if (stats) {
+            bpca_solve_stats &st = stats[nout];
+            st.converged = cur[i].converged;
+            st.restarts = cur[i].iter + 1;
+            st.lanczos_steps = cur[i].steps;
+            st.matvecs = 2 * cur[i].steps;
+            st.seconds = std::chrono::duration<double>(now - sl.t_admit).count();
+            st.svd_sweeps = cur[i].svd_sweeps;
+            st.slot = i;
+            st.smax = cur[i].smax;
+        }
+        if (slots) slots[nout] = i;
+        nout++;
+        bpca::Prob idle = cur[i];
+        idle.phase = bpca::PH_IDLE;
+        HIP_TRY(hipMemcpyAsync(E->d_probs + i, &idle, sizeof(idle), hipMemcpyHostToDevice, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        pool_free(ctx, sl.ws);
+        sl = Slot();
+        E->done_seen++;
+        E->dirty = true;
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    if (num_out) *num_out = nout;
+    return BPCA_OK;
+}
+
+int bpca_engine_active(bpca_engine *E, int32_t *active) {
+    if (!E || !active) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL argument");
+    int a = 0;
+    for (const auto &sl : E->slots) a += sl.used;
+    *active = a;
+    return BPCA_OK;
+}
+
+// ------------------------------------------------------------ solves
+
+int bpca_irlba_batched(bpca_context *ctx, int32_t num_problems,
+                       const bpca_problem *problems, bpca_solve_stats *stats,
+                       double *singular_values, int32_t sv_stride) {
+    if (!ctx || num_problems < 0 || (num_problems > 0 && !problems))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL argument");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    if (singular_values)
+        for (int i = 0; i < num_problems; i++)
+            if ((int)problems[i].k > sv_stride)
+                return fail(BPCA_ERR_INVALID_ARGUMENT, "sv_stride is smaller than a problem's k");
+    bpca_engine *E = nullptr;
+    BPCA_TRY(bpca_engine_create(ctx, &E));
+    std::unique_ptr<bpca_engine, int (*)(bpca_engine *)> hold(E, bpca_engine_destroy);
+    std::vector<int32_t> slot_of(num_problems);
+    for (int i = 0; i < num_problems; i++) BPCA_TRY(bpca_engine_add(E, &problems[i], &slot_of[i]));
+    std::vector<int> index_of_slot(E->slots.size(), -1);
+    for (int i = 0; i < num_problems; i++) index_of_slot[slot_of[i]] = i;
+    int collected = 0;
+    std::vector<int32_t> slots(num_problems);
+    std::vector<bpca_solve_stats> st(num_problems);
+    const int stride = std::max(1, (int)sv_stride);
+    std::vector<double> sv((size_t)num_problems * stride);
+    while (collected < num_problems) {
+        int32_t nd = 0;
+        BPCA_TRY(bpca_engine_step(E, 1 << 30, &nd));
+        int32_t got = 0;
+        BPCA_TRY(bpca_engine_collect(E, num_problems - collected, slots.data() + collected,
+                                     st.data() + collected,
+                                     singular_values ? sv.data() + (size_t)collected * stride : nullptr,
+                                     stride, &got));
+        collected += got;
+    }
+    for (int c = 0; c < num_problems; c++) {
+        const int i = index_of_slot[slots[c]];
+        if (stats) stats[i] = st[c];
+        if (singular_values)
+            std::copy(sv.begin() + (size_t)c * stride, sv.begin() + (size_t)c * stride + problems[i].k,
+                      singular_values + (size_t)i * sv_stride);
     }
     return BPCA_OK;
 }
+
+int bpca_irlba_device(bpca_context *ctx, const float *d_data,
+                      const int32_t *d_indices, const void *d_indptr,
+                      int indptr_width, uint64_t num_cells, uint32_t num_genes,
+                      uint32_t k, uint32_t subspace_size, float tolerance,
+                      uint32_t max_iterations, uint64_t seed, int validate,
+                      float *d_PCs, double *singular_values,
+                      bpca_solve_stats *stats) {
+    bpca_problem pb{};
+    pb.data = d_data;
+    pb.indices = d_indices;
+    pb.indptr = d_indptr;
+    pb.indptr_width = indptr_width;
+    pb.validate = validate;
+    pb.num_cells = num_cells;
+    pb.num_genes = num_genes;
+    pb.k = k;
+    pb.subspace_size = subspace_size;
+    pb.max_iterations = max_iterations;
+    pb.tolerance = tolerance;
+    pb.seed = seed;
+    pb.PCs = d_PCs;
+    return bpca_irlba_batched(ctx, 1, &pb, stats, singular_values, (int32_t)k);
+}
+
+}  // extern "C"
+
+namespace {
 
 std::mutex g_default_lock;
 bpca_context *g_default = nullptr;
@@ -438,126 +848,6 @@ struct DevMem {
 
 extern "C" {
 
-const char *bpca_last_error(void) { return g_last_error.c_str(); }
-
-int bpca_version(void) { return 1 * 10000 + 0 * 100 + 0; }
-
-int bpca_context_create(int device, uint64_t hbm_budget_bytes,
-                        bpca_context **out) {
-    if (!out) return fail(BPCA_ERR_INVALID_ARGUMENT, "out is NULL");
-    *out = nullptr;
-    int count = 0;
-    HIP_TRY(hipGetDeviceCount(&count));
-    if (device < 0 || device >= count)
-        return fail(BPCA_ERR_INVALID_ARGUMENT, "no such device");
-    bpca_context *ctx = new (std::nothrow) bpca_context();
-    if (!ctx) return fail(BPCA_ERR_OUT_OF_MEMORY, "context allocation failed");
-    ctx->device = device;
-    ctx->budget = hbm_budget_bytes;
-    DeviceGuard dg(device);
-    hipError_t e = hipStreamCreateWithFlags(&ctx->own_stream, hipStreamNonBlocking);
-    if (e != hipSuccess) {
-        delete ctx;
-        return hip_fail(e, "hipStreamCreateWithFlags");
-    }
-    ctx->stream = ctx->own_stream;
-    int lds = 0;
-    if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
-        ctx->max_lds = (size_t)lds;
-    *out = ctx;
-    return BPCA_OK;
-}
-
-int bpca_context_destroy(bpca_context *ctx) {
-    if (!ctx) return BPCA_OK;
-    {
-        DeviceGuard dg(ctx->device);
-        if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
-        if (ctx->ws) (void)hipFree(ctx->ws);
-        if (ctx->host) (void)hipHostFree(ctx->host);
-        for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
-        if (ctx->own_stream) (void)hipStreamDestroy(ctx->own_stream);
-    }
-    {
-        std::lock_guard<std::mutex> guard(g_default_lock);
-        if (ctx == g_default) g_default = nullptr;
-    }
-    delete ctx;
-    return BPCA_OK;
-}
-
-int bpca_context_set_stream(bpca_context *ctx, void *hip_stream) {
-    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
-    ctx->stream = hip_stream ? (hipStream_t)hip_stream : ctx->own_stream;
-    return BPCA_OK;
-}
-
-int bpca_context_cancel(bpca_context *ctx) {
-    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
-    ctx->cancel.store(1);
-    return BPCA_OK;
-}
-
-int bpca_context_set_profiling(bpca_context *ctx, int enable) {
-    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
-    std::lock_guard<std::mutex> guard(ctx->lock);
-    ctx->profiling = enable != 0;
-    return BPCA_OK;
-}
-
-int bpca_context_kernel_times(bpca_context *ctx, double *total_ms,
-                              int64_t *launches) {
-    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
-    std::lock_guard<std::mutex> guard(ctx->lock);
-    for (int i = 0; i < BPCA_NUM_KERNEL_CLASSES; i++) {
-        if (total_ms) total_ms[i] = ctx->k_ms[i];
-        if (launches) launches[i] = ctx->k_count[i];
-    }
-    return BPCA_OK;
-}
-
-int bpca_context_reset_kernel_times(bpca_context *ctx) {
-    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
-    std::lock_guard<std::mutex> guard(ctx->lock);
-    for (int i = 0; i < BPCA_NUM_KERNEL_CLASSES; i++) {
-        ctx->k_ms[i] = 0.0;
-        ctx->k_count[i] = 0;
-    }
-    return BPCA_OK;
-}
-
-uint64_t bpca_workspace_bytes(uint64_t num_cells, uint32_t num_genes,
-                              uint32_t k, uint32_t subspace_size) {
-    return make_layout((int64_t)num_cells, (int)num_genes, (int)subspace_size,
-                       (int)k)
-        .total;
-}
-
-int bpca_irlba_device(bpca_context *ctx, const float *d_data,
-                      const int32_t *d_indices, const void *d_indptr,
-                      int indptr_width, uint64_t num_cells, uint32_t num_genes,
-                      uint32_t k, uint32_t subspace_size, float tolerance,
-                      uint32_t max_iterations, uint64_t seed, int validate,
-                      float *d_PCs, double *singular_values,
-                      bpca_solve_stats *stats) {
-    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
-    if (!d_data || !d_indices || !d_indptr || !d_PCs)
-        return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL device pointer");
-    if (indptr_width != 4 && indptr_width != 8)
-        return fail(BPCA_ERR_INVALID_ARGUMENT, "indptr_width must be 4 or 8");
-    std::lock_guard<std::mutex> guard(ctx->lock);
-    DeviceGuard dg(ctx->device);
-    bpca::Csr X{d_data, d_indices, d_indptr, indptr_width, (int64_t)num_cells,
-                (int)num_genes};
-    try {
-        return solve(ctx, X, (int)k, (int)subspace_size, tolerance,
-                     (int)max_iterations, seed, validate != 0, d_PCs,
-                     singular_values, stats);
-    } catch (const std::bad_alloc &) {
-        return fail(BPCA_ERR_OUT_OF_MEMORY, "host allocation failed");
-    }
-}
-
 int bpca_irlba_f32(const float *data_matvec, const void *indices_matvec,
                    const void *indptr_matvec, const float *data_rmatvec,
                    const void *indices_rmatvec, const void *indptr_rmatvec,
@@ -583,7 +873,7 @@ int bpca_irlba_f32(const float *data_matvec, const void *indices_matvec,
     BPCA_TRY(check_shape((int64_t)num_cells, num_genes, k, subspace_size, tolerance, max_iterations));
     bpca_context *ctx = nullptr;
     BPCA_TRY(default_context(&ctx));
-    std::lock_guard<std::mutex> guard(ctx->lock);
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
     DeviceGuard dg(ctx->device);
     const int64_t n = (int64_t)num_cells;
     const int64_t nnz = index_width == 8 ? ((const int64_t *)indptr_matvec)[n]
@@ -614,19 +904,41 @@ int bpca_irlba_f32(const float *data_matvec, const void *indices_matvec,
     HIP_TRY(hipMemcpyAsync(&h_flag, d_flag.p, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (h_flag) return fail(BPCA_ERR_NONCANONICAL, "column index outside [0, num_genes)");
-    bpca::Csr X{d_data.p, d_idx.p, d_ptr.p, index_width, n, (int)num_genes};
     bpca_solve_stats st{};
-    int rc;
-    try {
-        rc = solve(ctx, X, (int)k, (int)subspace_size, tolerance,
-                   (int)max_iterations, seed, true, d_pcs.p, nullptr, &st);
-    } catch (const std::bad_alloc &) {
-        return fail(BPCA_ERR_OUT_OF_MEMORY, "host allocation failed");
-    }
-    if (rc != BPCA_OK) return rc;
+    BPCA_TRY(bpca_irlba_device(ctx, d_data.p, d_idx.p, d_ptr.p, index_width, num_cells,
+                               num_genes, k, subspace_size, tolerance, max_iterations,
+                               seed, 1, d_pcs.p, nullptr, &st));
     HIP_TRY(hipMemcpyAsync(PCs, d_pcs.p, sizeof(float) * n * k, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     if (converged) *converged = st.converged;
+    return BPCA_OK;
+}
+
+int bpca_test_small_svd(bpca_context *ctx, int32_t m, const double *B,
+                        double *Ub, double *Sb, double *Vb, int32_t *sweeps) {
+    if (!ctx || !B || !Ub || !Sb || !Vb || m < 1)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "bad argument");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = ctx->stream;
+    const int64_t mm = m + (m & 1);
+    DevMem<double> dB, dW, dJ, dU, dS;
+    DevMem<int> dsw;
+    HIP_TRY(dB.alloc((size_t)m * (m + 1)));
+    HIP_TRY(dW.alloc((size_t)m * mm));
+    HIP_TRY(dJ.alloc((size_t)(mm * mm + (int64_t)m * m)));
+    HIP_TRY(dU.alloc((size_t)m * m));
+    HIP_TRY(dS.alloc(m));
+    HIP_TRY(dsw.alloc(1));
+    HIP_TRY(hipMemcpyAsync(dB.p, B, sizeof(double) * m * (m + 1), hipMemcpyHostToDevice, s));
+    HIP_TRY(bpca::launch_svd_single(dB.p, m, dW.p, dJ.p, dU.p, dS.p, dsw.p, s));
+    HIP_TRY(hipMemcpyAsync(Ub, dU.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(Sb, dS.p, sizeof(double) * m, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(Vb, dJ.p + mm * mm, sizeof(double) * m * m, hipMemcpyDeviceToHost, s));
+    int h_sw = 0;
+    HIP_TRY(hipMemcpyAsync(&h_sw, dsw.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (sweeps) *sweeps = h_sw;
     return BPCA_OK;
 }
 
@@ -636,7 +948,9 @@ int bpca_synth_row_nnz(bpca_context *ctx, uint64_t seed, uint64_t row0,
                        const int32_t *d_cell_type, const double *d_cell_factor,
                        int64_t *d_row_nnz) {
     if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
     DeviceGuard dg(ctx->device);
+    begin_call(ctx);
     HIP_TRY(bpca::launch_synth_row_nnz(seed, row0, (int64_t)num_rows, (int)num_genes,
                                        d_type_gene_mean, d_cell_type, d_cell_factor,
                                        d_row_nnz, ctx->stream));
@@ -650,7 +964,9 @@ int bpca_synth_fill(bpca_context *ctx, uint64_t seed, uint64_t row0,
                     const double *d_cell_factor, const int64_t *d_indptr,
                     int32_t *d_indices, int32_t *d_counts) {
     if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
     DeviceGuard dg(ctx->device);
+    begin_call(ctx);
     HIP_TRY(bpca::launch_synth_fill(seed, row0, (int64_t)num_rows, (int)num_genes,
                                     d_type_gene_mean, d_cell_type, d_cell_factor,
                                     d_indptr, d_indices, d_counts, ctx->stream));
@@ -663,7 +979,9 @@ int bpca_normalize_log1ppf(bpca_context *ctx, const int32_t *d_counts,
                            float *d_out) {
     if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (num_cells < 1) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells must be positive");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
     DeviceGuard dg(ctx->device);
+    begin_call(ctx);
     hipStream_t s = ctx->stream;
     DevMem<uint64_t> sums;
     HIP_TRY(sums.alloc(num_cells + 1));

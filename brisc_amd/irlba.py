@@ -203,3 +203,157 @@ def irlba_device(data, indices, indptr, num_cells: int, num_genes: int,
         sv.ctypes.data, ctypes.byref(st))
     _lib.check(rc)
     return PCs, sv, st.as_dict()
+
+
+def _problem(data, indices, indptr, num_cells, num_genes, k, subspace_size,
+             tolerance, max_iterations, seed, validate, PCs):
+    import torch
+    _check_device_csr(data, indices, indptr, num_cells)
+    subspace_size = min(int(subspace_size), int(num_cells), int(num_genes))
+    if PCs is None:
+        PCs = torch.empty((num_cells, k), dtype=torch.float32,
+                          device=data.device)
+    elif PCs.shape != (num_cells, k) or PCs.dtype != torch.float32 or \
+            not PCs.is_contiguous():
+        raise ValueError('PCs must be a contiguous float32 (num_cells, k) '
+                         'tensor')
+    pb = _lib.Problem(
+        data=data.data_ptr(), indices=indices.data_ptr(),
+        indptr=indptr.data_ptr(), indptr_width=indptr.element_size(),
+        validate=int(bool(validate)), num_cells=int(num_cells),
+        num_genes=int(num_genes), k=int(k), subspace_size=subspace_size,
+        max_iterations=int(max_iterations), tolerance=float(tolerance),
+        pad=0, seed=int(seed) & 0xFFFFFFFFFFFFFFFF, PCs=PCs.data_ptr())
+    return pb, PCs
+
+
+def irlba_batched(problems, context: Context | None = None,
+                  tolerance: float = 1e-6, max_iterations: int = 100,
+                  seed: int = 0, validate: bool = True):
+    """Solve many independent PCAs in shared launches.
+
+    ``problems`` is a sequence of dicts with keys ``data``, ``indices``,
+    ``indptr`` (device tensors), ``num_cells``, ``num_genes``, ``k`` and
+    optionally ``subspace_size`` (default 100), ``tolerance``,
+    ``max_iterations``, ``seed``.  Every problem advances one Lanczos step per
+    launch sequence (a replayed HIP graph); each result is bit-identical to
+    solving that problem alone.  Returns a list of ``(PCs, singular_values,
+    stats)`` in input order.
+    """
+    if not problems:
+        return []
+    ctx = context or default_context(problems[0]['data'].device.index or 0)
+    arr = (_lib.Problem * len(problems))()
+    outs = []
+    kmax = 1
+    for i, p in enumerate(problems):
+        pb, PCs = _problem(p['data'], p['indices'], p['indptr'],
+                           p['num_cells'], p['num_genes'], p['k'],
+                           p.get('subspace_size', 100),
+                           p.get('tolerance', tolerance),
+                           p.get('max_iterations', max_iterations),
+                           p.get('seed', seed), p.get('validate', validate),
+                           p.get('PCs'))
+        arr[i] = pb
+        outs.append(PCs)
+        kmax = max(kmax, int(p['k']))
+    stats = (_lib.SolveStats * len(problems))()
+    sv = np.zeros((len(problems), kmax), dtype=np.float64)
+    _lib.check(_lib.lib().bpca_irlba_batched(ctx.handle, len(problems), arr,
+                                            stats, sv.ctypes.data, kmax))
+    return [(outs[i], sv[i, :int(problems[i]['k'])].copy(),
+             stats[i].as_dict()) for i in range(len(problems))]
+
+
+class Engine:
+    """Continuous-batching IRLBA engine on one device.
+
+    ``add`` admits a problem whenever its workspace fits the context's HBM
+    budget (``BudgetError`` otherwise); ``step`` advances every admitted
+    problem; ``collect`` returns finished ones.  Problems admitted while
+    others run share the same launches, and each result is bit-identical to
+    solving the problem alone.
+    """
+
+    def __init__(self, context: Context | None = None, device: int = 0):
+        self.context = context or default_context(device)
+        handle = ctypes.c_void_p()
+        _lib.check(_lib.lib().bpca_engine_create(self.context.handle,
+                                                 ctypes.byref(handle)))
+        self._h = handle
+        self._live = {}   # slot -> (tag, PCs, tensors kept alive, k)
+
+    def add(self, data, indices, indptr, num_cells, num_genes, k,
+            subspace_size=100, tolerance=1e-6, max_iterations=100, seed=0,
+            validate=True, PCs=None, tag=None) -> int:
+        pb, PCs = _problem(data, indices, indptr, num_cells, num_genes, k,
+                           subspace_size, tolerance, max_iterations, seed,
+                           validate, PCs)
+        slot = ctypes.c_int32(-1)
+        self.context.handle  # order after the caller's torch stream
+        _lib.check(_lib.lib().bpca_engine_add(self._h, ctypes.byref(pb),
+                                              ctypes.byref(slot)))
+        self._live[slot.value] = (tag, PCs, (data, indices, indptr), int(k))
+        return slot.value
+
+    def step(self, max_steps: int = 1 << 30) -> int:
+        """Advance up to max_steps Lanczos steps; stops early when a problem
+        finishes.  Returns the number of finished, uncollected problems."""
+        done = ctypes.c_int32(0)
+        _lib.check(_lib.lib().bpca_engine_step(self._h, int(max_steps),
+                                               ctypes.byref(done)))
+        return done.value
+
+    def collect(self):
+        """[(tag, PCs, singular_values, stats)] of finished problems."""
+        n = len(self._live)
+        if n == 0:
+            return []
+        kmax = max(v[3] for v in self._live.values())
+        slots = np.zeros(n, dtype=np.int32)
+        stats = (_lib.SolveStats * n)()
+        sv = np.zeros((n, kmax), dtype=np.float64)
+        got = ctypes.c_int32(0)
+        _lib.check(_lib.lib().bpca_engine_collect(
+            self._h, n, slots.ctypes.data, stats, sv.ctypes.data, kmax,
+            ctypes.byref(got)))
+        out = []
+        for i in range(got.value):
+            tag, PCs, _, k = self._live.pop(int(slots[i]))
+            out.append((tag, PCs, sv[i, :k].copy(), stats[i].as_dict()))
+        return out
+
+    @property
+    def active(self) -> int:
+        return len(self._live)
+
+    def close(self) -> None:
+        if self._h is not None:
+            _lib.lib().bpca_engine_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def small_svd(B, context: Context | None = None):
+    """The device's restart SVD on one m x (m+1) matrix (float64).
+
+    Returns ``(Ub, Sb, Vb, sweeps)`` with ``B[:, :m] = Ub diag(Sb) Vb^T``.
+    """
+    B = np.asfortranarray(B, dtype=np.float64)
+    m = B.shape[0]
+    if B.shape != (m, m + 1):
+        raise ValueError('B must have shape (m, m+1)')
+    ctx = context or default_context(0)
+    Ub = np.zeros((m, m), order='F')
+    Sb = np.zeros(m)
+    Vb = np.zeros((m, m), order='F')
+    sweeps = ctypes.c_int32(0)
+    _lib.check(_lib.lib().bpca_test_small_svd(
+        ctx.handle, m, B.ctypes.data, Ub.ctypes.data, Sb.ctypes.data,
+        Vb.ctypes.data, ctypes.byref(sweeps)))
+    return Ub, Sb, Vb, sweeps.value

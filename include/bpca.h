@@ -92,8 +92,9 @@ typedef struct bpca_solve_stats {
     int32_t restarts;       /* IRLBA iterations run (1 = no restart) */
     int64_t lanczos_steps;  /* Lanczos steps over all iterations */
     int64_t matvecs;        /* X.v plus X^T.u products (2 per step) */
-    double seconds;         /* wall time of the device-resident solve */
-    double svd_seconds;     /* host time in the small FP64 SVDs */
+    double seconds;         /* wall time from admission to collection */
+    int32_t svd_sweeps;     /* Jacobi sweeps of the last small SVD */
+    int32_t slot;           /* engine slot the problem ran in */
     double smax;            /* largest Ritz value seen (convergence scale) */
 } bpca_solve_stats;
 
@@ -133,6 +134,65 @@ int bpca_irlba_device(bpca_context *ctx, const float *d_data,
                       uint32_t max_iterations, uint64_t seed, int validate,
                       float *d_PCs, double *singular_values,
                       bpca_solve_stats *stats);
+
+/* ------------------------------------------------- batched / engine */
+
+/* One IRLBA problem on device-resident CSR (the arguments of
+ * bpca_irlba_device gathered in a struct).  No counterpart in the reference,
+ * which solves one matrix per call; SingleCell.pca() per cluster is the use
+ * it batches. */
+typedef struct bpca_problem {
+    const float *data;        /* device, nnz float32 */
+    const int32_t *indices;   /* device, nnz int32 column indices */
+    const void *indptr;       /* device, num_cells+1 int32 or int64 */
+    int32_t indptr_width;     /* 4 or 8 */
+    int32_t validate;         /* check canonical CSR on admission */
+    uint64_t num_cells;
+    uint32_t num_genes;
+    uint32_t k;
+    uint32_t subspace_size;   /* <= min(num_cells, num_genes) */
+    uint32_t max_iterations;
+    float tolerance;
+    uint32_t pad;
+    uint64_t seed;
+    float *PCs;               /* device, num_cells x k float32, C order */
+} bpca_problem;
+
+/* Solve many independent problems together: every launch advances all of
+ * them (one Lanczos step each; restarts and final products inside the same
+ * launch sequence), replayed as a captured HIP graph.  Each problem's result
+ * is bit-identical to solving it alone.  stats (num_problems entries) and
+ * singular_values (num_problems rows of sv_stride doubles) may be NULL. */
+int bpca_irlba_batched(bpca_context *ctx, int32_t num_problems,
+                       const bpca_problem *problems, bpca_solve_stats *stats,
+                       double *singular_values, int32_t sv_stride);
+
+/* Continuous-batching engine (the serving loop drives it): problems are
+ * admitted while others run, subject to the context's HBM budget
+ * (BPCA_ERR_BUDGET when a problem's workspace does not fit now). */
+typedef struct bpca_engine bpca_engine;
+int bpca_engine_create(bpca_context *ctx, bpca_engine **out);
+int bpca_engine_destroy(bpca_engine *engine);
+/* Admit a problem (set-up kernels run at the next step); *slot names it. */
+int bpca_engine_add(bpca_engine *engine, const bpca_problem *problem,
+                    int32_t *slot);
+/* Advance every admitted problem by up to max_steps Lanczos steps; returns
+ * early once a problem has finished.  *num_done = finished, uncollected. */
+int bpca_engine_step(bpca_engine *engine, int32_t max_steps,
+                     int32_t *num_done);
+/* Retire up to max_out finished problems: their slots, statistics and
+ * singular values (rows of sv_stride doubles); frees their workspace. */
+int bpca_engine_collect(bpca_engine *engine, int32_t max_out, int32_t *slots,
+                        bpca_solve_stats *stats, double *singular_values,
+                        int32_t sv_stride, int32_t *num_out);
+/* Number of admitted, uncollected problems. */
+int bpca_engine_active(bpca_engine *engine, int32_t *active);
+
+/* The device's small SVD on one m x (m+1) matrix B (host, column-major):
+ * B = Ub diag(Sb) Vb^T with Ub, Vb m x m column-major.  Test entry point of
+ * the kernel the restart uses (role of sgesdd, brisc/pca.pyx:514-545). */
+int bpca_test_small_svd(bpca_context *ctx, int32_t m, const double *B,
+                        double *Ub, double *Sb, double *Vb, int32_t *sweeps);
 
 /* ------------------------------------------------ synthetic workloads */
 

@@ -176,17 +176,17 @@ def test_nonconverged_exit_returns_ritz_vectors(cells50k):
     and the Rayleigh quotient of each PC reproduces it."""
     w, data, indices, indptr = cells50k
     print('case max_iterations=1', flush=True)
-    PCs, conv = run_host(data, indices, indptr, w.num_cells, w.num_genes, 10,
+    PCs, conv = run_host(data, indices, indptr, w.num_cells, w.num_genes, 50,
                          max_iterations=1)
     assert not conv
-    out = oracle.fp64_pca(data, indices, indptr, w.num_cells, w.num_genes, 10,
+    out = oracle.fp64_pca(data, indices, indptr, w.num_cells, w.num_genes, 50,
                           want_pcs=False)
     sv = helpers.singular_values(PCs)
     assert np.all(sv <= out['singular_values'] * (1 + 1e-6))
     assert np.all(sv >= 0.5 * out['singular_values'])
     # U is orthonormal: columns of PCs are mutually orthogonal
     G = (PCs.astype(np.float64).T @ PCs.astype(np.float64)) / np.outer(sv, sv)
-    assert np.abs(G - np.eye(10)).max() < 1e-4
+    assert np.abs(G - np.eye(50)).max() < 1e-4
 
 
 def test_noncanonical_rows_are_rejected(golden):
@@ -248,3 +248,91 @@ def test_one_million_cells_against_fp64_oracle():
     assert np.max(np.abs(sv1 - out['singular_values']) / out['singular_values']) < SV_RTOL
     PCs = P1.cpu().numpy()
     assert helpers.relative_sv_error(PCs, out['singular_values']).max() < SV_RTOL
+
+
+def test_device_small_svd_matches_numpy():
+    """The restart's one-sided Jacobi SVD on IRLBA-shaped matrices (diagonal
+    plus spike column after a restart, bidiagonal tail), against LAPACK."""
+    from brisc_amd import small_svd
+    rng = np.random.default_rng(3)
+    for m, kc in [(10, 0), (40, 17), (100, 62), (99, 50)]:
+        B = np.zeros((m, m + 1))
+        for i in range(kc):
+            B[i, i] = rng.uniform(5, 50)
+            B[i, kc] = rng.normal() * 1e-2
+        for j in range(kc, m):
+            B[j, j] = rng.uniform(0.5, 3)
+            if j + 1 < m:
+                B[j, j + 1] = rng.uniform(0.5, 3)
+        print(f'case small_svd m={m} kc={kc}', flush=True)
+        Ub, Sb, Vb, sweeps = small_svd(B)
+        S = np.linalg.svd(B, compute_uv=False)
+        assert np.allclose(Sb, S, rtol=1e-12, atol=1e-13 * S[0])
+        assert np.allclose(Ub @ np.diag(Sb) @ Vb.T, B[:, :m], atol=1e-12 * S[0])
+        assert np.allclose(Ub.T @ Ub, np.eye(m), atol=1e-12)
+        assert np.allclose(Vb.T @ Vb, np.eye(m), atol=1e-12)
+        assert np.all(np.diff(Sb) <= 0)
+        assert sweeps < 30
+
+
+def _subclusters(num, low, high, seed):
+    rng = np.random.default_rng(seed)
+    sizes = np.exp(rng.uniform(np.log(low), np.log(high), num)).astype(int)
+    out = []
+    for i, n in enumerate(sizes):
+        w = synth.make_workload(1000 + i, int(n), num_types=5)
+        out.append((w, oracle.generate_csr(w)))
+    return out
+
+
+def test_batched_is_bitwise_equal_to_sequential():
+    """Uneven problems solved together give, bit for bit, the results of
+    solving each alone (fixed-order reductions keyed to each problem's own
+    data blocks)."""
+    from brisc_amd import irlba_batched
+    probs = _subclusters(6, 300, 6000, 5)
+    print('case batched vs sequential', [w.num_cells for w, _ in probs], flush=True)
+    dev = [to_device(*csr) for _, csr in probs]
+    specs = [dict(data=d, indices=i, indptr=p, num_cells=w.num_cells,
+                  num_genes=w.num_genes, k=15, subspace_size=40)
+             for (w, _), (d, i, p) in zip(probs, dev)]
+    batched = irlba_batched(specs)
+    for spec, (Pb, svb, stb), ((w, csr), _) in zip(specs, batched, zip(probs, dev)):
+        Ps, svs, sts = irlba_device(spec['data'], spec['indices'], spec['indptr'],
+                                    w.num_cells, w.num_genes, 15, subspace_size=40)
+        assert stb['converged'] == sts['converged'] == 1
+        assert stb['restarts'] == sts['restarts']
+        assert np.array_equal(Pb.cpu().numpy().view(np.uint32), Ps.cpu().numpy().view(np.uint32))
+        assert np.array_equal(svb, svs)
+        out = oracle.fp64_pca(*csr, w.num_cells, w.num_genes, 15, want_pcs=False)
+        assert np.max(np.abs(svb - out['singular_values']) / out['singular_values']) < SV_RTOL
+
+
+def test_engine_admits_while_running_and_matches_solo():
+    """Continuous batching: problems admitted while others are mid-solve
+    finish with results bit-identical to solo solves."""
+    from brisc_amd import Engine
+    probs = _subclusters(5, 500, 5000, 9)
+    print('case engine', [w.num_cells for w, _ in probs], flush=True)
+    dev = [to_device(*csr) for _, csr in probs]
+    eng = Engine()
+    results = {}
+    pending = list(range(len(probs)))
+    eng.add(*dev[pending[0]], probs[0][0].num_cells, 2000, 12, subspace_size=30, tag=pending.pop(0))
+    steps = 0
+    while eng.active or pending:
+        eng.step(7)              # a few Lanczos steps, then admit more
+        steps += 1
+        if pending and steps % 2 == 1:
+            i = pending.pop(0)
+            eng.add(*dev[i], probs[i][0].num_cells, 2000, 12, subspace_size=30, tag=i)
+        for tag, P, sv, st in eng.collect():
+            results[tag] = (P.cpu().numpy(), sv, st)
+        assert steps < 10000
+    eng.close()
+    for i, ((w, csr), d) in enumerate(zip(probs, dev)):
+        Ps, svs, sts = irlba_device(*d, w.num_cells, w.num_genes, 12, subspace_size=30)
+        P, sv, st = results[i]
+        assert st['converged'] == 1
+        assert np.array_equal(P.view(np.uint32), Ps.cpu().numpy().view(np.uint32))
+        assert np.array_equal(sv, svs)
