@@ -115,6 +115,10 @@ def lib() -> ctypes.CDLL:
         'bpca_synth_fill': (i32, [p, u64, u64, u64, u32, p, p, p, p, p, p]),
         'bpca_normalize_log1ppf': (i32, [p, p, p, u64, p]),
         'bpca_test_small_svd': (i32, [p, i32, p, p, p, p, p]),
+        'bpca_test_gemm': (i32, [p, i64, i32, i32, p, p, p]),
+        'bpca_context_info': (i32, [p, ctypes.POINTER(ctypes.c_int64),
+                                    ctypes.POINTER(ctypes.c_int32),
+                                    ctypes.POINTER(ctypes.c_int32)]),
         'bpca_irlba_batched': (i32, [p, i32, ctypes.POINTER(Problem),
                                      ctypes.POINTER(SolveStats), p, i32]),
         'bpca_engine_create': (i32, [p, ctypes.POINTER(p)]),

@@ -80,6 +80,7 @@ struct Tables {
     const int2 *groups;   // (problem, group) pairs
     int64_t num_groups;
     int max_g, max_m;
+    int64_t max_lds;      // dynamic LDS bytes a workgroup may use
 };
 
 // Optional timing hook around each class of launches (null when captured
@@ -107,7 +108,13 @@ hipError_t launch_narrow_indices(const int64_t *in, int64_t nnz, int g,
 // One-sided Jacobi SVD of a single m x (m+1) matrix (test entry point).
 hipError_t launch_svd_single(const double *B, int m, double *W, double *Jv,
                              double *Ub, double *Sb, int *sweeps,
-                             hipStream_t s);
+                             int64_t max_lds, hipStream_t s);
+
+// out (n x kout, column-major) = A (n x m) C (m x kout) through the MFMA
+// restart kernel's tile code (test entry point; m <= 128, kout <= 128).
+hipError_t launch_gemm_mfma_test(const double *A, int64_t n, int m,
+                                 const double *C, int kout, double *out,
+                                 hipStream_t s);
 
 // Synthetic workload generator and log1pPF (synth.hip).
 hipError_t launch_synth_row_nnz(uint64_t seed, uint64_t row0, int64_t rows,

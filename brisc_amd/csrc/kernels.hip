@@ -10,7 +10,7 @@
 //   k_svd                       sgesdd + convergence + k_current  514-545,
 //                                                                 1178-1202
 //   k_tall_gemm<false>,         matmul_nn_f_out + V sgemm + B     578-616,
-//   k_restart_v                                                   1204-1231
+//   k_restart_v, _commit                                          1204-1231
 //   k_tall_gemm<true>           matmul_nn_c_out with Ub*Sb        619-658,
 //                                                                 1233-1254
 //
@@ -579,6 +579,72 @@ __device__ void jacobi_svd(double *W, double *Jv, int m, int mm, int *sweeps,
     if (threadIdx.x == 0) *sweeps = sweep + 1;
 }
 
+// Same rotations with W and Jv in LDS and one 16-lane group per pair (up to
+// blockDim/16 pairs at once, all m/2 pairs of a round for m <= 128 with 1024
+// threads).  Partial sums over a pair's rows run lane-strided, then the xor
+// butterfly 8,4,2,1 inside the group.
+__device__ void jacobi_svd_lds(double *W, double *Jv, int m, int mm,
+                               int *sweeps, int *rot_count) {
+    const int sub = threadIdx.x & 15;
+    const int grp = threadIdx.x >> 4;
+    const int ngrp = blockDim.x >> 4;
+    const double tol = 2.220446049250313e-16 * sqrt((double)m);
+    int sweep = 0;
+    for (; sweep < 60; sweep++) {
+        if (threadIdx.x == 0) *rot_count = 0;
+        __syncthreads();
+        for (int round = 0; round < mm - 1; round++) {
+            for (int slot = grp; slot < mm / 2; slot += ngrp) {
+                int p = rr_player(slot, round, mm);
+                int q = rr_player(mm - 1 - slot, round, mm);
+                if (p > q) {
+                    const int t = p;
+                    p = q;
+                    q = t;
+                }
+                double *wp = W + p * m, *wq = W + q * m;
+                double a = 0.0, b = 0.0, c = 0.0;
+                for (int i = sub; i < m; i += 16) {
+                    const double x = wp[i], y = wq[i];
+                    a = fma(x, x, a);
+                    b = fma(y, y, b);
+                    c = fma(x, y, c);
+                }
+#pragma unroll
+                for (int off = 8; off > 0; off >>= 1) {
+                    a += __shfl_xor(a, off);
+                    b += __shfl_xor(b, off);
+                    c += __shfl_xor(c, off);
+                }
+                if (c != 0.0 && fabs(c) > tol * sqrt(a * b)) {
+                    const double zeta = (b - a) / (2.0 * c);
+                    const double t = (zeta >= 0.0 ? 1.0 : -1.0) /
+                                     (fabs(zeta) + sqrt(1.0 + zeta * zeta));
+                    const double cs = 1.0 / sqrt(1.0 + t * t);
+                    const double sn = cs * t;
+                    for (int i = sub; i < m; i += 16) {
+                        const double x = wp[i], y = wq[i];
+                        wp[i] = cs * x - sn * y;
+                        wq[i] = sn * x + cs * y;
+                    }
+                    double *jp = Jv + p * mm, *jq = Jv + q * mm;
+                    for (int i = sub; i < mm; i += 16) {
+                        const double x = jp[i], y = jq[i];
+                        jp[i] = cs * x - sn * y;
+                        jq[i] = sn * x + cs * y;
+                    }
+                    if (sub == 0) atomicAdd(rot_count, 1);
+                }
+            }
+            __syncthreads();
+        }
+        const int rc = *rot_count;
+        __syncthreads();
+        if (rc == 0) break;
+    }
+    if (threadIdx.x == 0) *sweeps = sweep + 1;
+}
+
 // Sort columns by norm (descending, ties by index) and write Sb, Ub = W/s
 // and the right vectors Vb (m x m).
 __device__ void svd_sort_outputs(const double *W, const double *Jv, int m,
@@ -621,13 +687,21 @@ __device__ void svd_sort_outputs(const double *W, const double *Jv, int m,
 
 }  // namespace
 
+// The sweep's projected matrix B -> Ub, Sb, right vectors (in Jv after the
+// Jacobi work area), then the convergence test of pca.pyx:1181-1202 and,
+// for a restart, the new B of pca.pyx:1225-1231.  kLds keeps the Jacobi
+// work arrays in LDS (when the device's workgroup LDS holds them).
+template <bool kLds>
 __global__ __launch_bounds__(1024) void k_svd(Tables T) {
     __shared__ int sh_int[4];
-    extern __shared__ double sh[];  // s[m] then perm[m] (as int)
+    extern __shared__ double sh[];
     Prob &pr = T.probs[blockIdx.x];
     if (pr.phase != PH_RESTART) return;
     const int m = pr.m, mm = m + (m & 1), k = pr.k;
-    double *W = pr.W, *Jv = pr.Jv;
+    double *s = sh;                       // m
+    int *perm = (int *)(sh + m);          // m
+    double *W = kLds ? sh + m + (m + 1) / 2 : pr.W;
+    double *Jv = kLds ? W + m * mm : pr.Jv;
     for (int idx = threadIdx.x; idx < m * mm; idx += blockDim.x) {
         const int c = idx / m;
         W[idx] = c < m ? pr.B[idx] : 0.0;  // B column-major with ld m
@@ -635,12 +709,12 @@ __global__ __launch_bounds__(1024) void k_svd(Tables T) {
     for (int idx = threadIdx.x; idx < mm * mm; idx += blockDim.x)
         Jv[idx] = (idx / mm == idx % mm) ? 1.0 : 0.0;
     __syncthreads();
-    jacobi_svd(W, Jv, m, mm, &sh_int[0], &sh_int[1]);
+    if (kLds)
+        jacobi_svd_lds(W, Jv, m, mm, &sh_int[0], &sh_int[1]);
+    else
+        jacobi_svd(W, Jv, m, mm, &sh_int[0], &sh_int[1]);
     __syncthreads();
-    double *s = sh;
-    int *perm = (int *)(sh + m);
-    // Vb (right vectors, m x m) is stored in B's memory after the SVD reads
-    // B; B is rebuilt below for the restart.
+    // Right vectors go after the Jacobi work area of Jv in global memory.
     double *Vb = pr.Jv + (int64_t)mm * mm;
     svd_sort_outputs(W, Jv, m, mm, s, perm, pr.Sb, pr.Ub, Vb);
     __syncthreads();
@@ -684,6 +758,13 @@ __global__ __launch_bounds__(1024) void k_svd(Tables T) {
             pr.B[idx] = v;
         }
     }
+}
+
+size_t svd_lds_bytes(int m, bool resident) {
+    const size_t mm = m + (m & 1);
+    size_t bytes = sizeof(double) * (size_t)m + sizeof(int) * (size_t)m + sizeof(double);
+    if (resident) bytes += sizeof(double) * ((size_t)m * mm + mm * mm);
+    return bytes;
 }
 
 // ================================================================ restart
@@ -743,32 +824,163 @@ __global__ __launch_bounds__(256) void k_tall_gemm(Tables T) {
     }
 }
 
-// V <- V Vb[:, :kc], V[:, kc] = remainder, vs = remainder / sd; flip the U and
-// V buffers; continue the Lanczos sweep at j = kc.
-__global__ __launch_bounds__(1024) void k_restart_v(Tables T) {
-    Prob &pr = T.probs[blockIdx.x];
+// MFMA variant (m <= 128, kout <= 128): each wave takes 16-row tiles and
+// accumulates out_tile(16 x 16t) += A_tile(16 x 4) coef(4 x 16t) with
+// v_mfma_f64_16x16x4f64 over k-steps of 4, k ascending.  All A fragments of
+// a tile are loaded before the first MFMA (up to 32 loads in flight per
+// wave).  Operand layout (16x16x4 f64, checked against numpy): A[i][k] in
+// lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i%4),
+// element i/4.
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+constexpr int kMfmaMaxSteps = 32;  // m <= 128
+constexpr int kMfmaMaxTiles = 8;   // kout <= 128
+
+namespace {
+
+// coef slab in LDS: cs[l * npad + c], zero-padded to mpad x npad.
+template <typename Store>
+__device__ inline void mfma_row_tiles(const double *__restrict__ A, int64_t n,
+                                      int m, const double *cs, int npad,
+                                      int64_t row0, int64_t row_end, int wave,
+                                      int nwaves, Store store) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int i = lane & 15, kq = lane >> 4;
+    const int ksteps = (m + 3) >> 2, ntiles = npad >> 4;
+    for (int64_t r0 = row0 + (int64_t)wave * 16; r0 < row_end; r0 += (int64_t)nwaves * 16) {
+        const int64_t r = r0 + i;
+        double a[kMfmaMaxSteps];
+#pragma unroll
+        for (int ks = 0; ks < kMfmaMaxSteps; ks++) {
+            const int l = ks * 4 + kq;
+            a[ks] = (ks < ksteps && l < m && r < row_end) ? A[(int64_t)l * n + r] : 0.0;
+        }
+        v4d acc[kMfmaMaxTiles];
+#pragma unroll
+        for (int t = 0; t < kMfmaMaxTiles; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+        for (int ks = 0; ks < kMfmaMaxSteps; ks++) {
+            if (ks < ksteps) {
+                const double *brow = cs + (ks * 4 + kq) * npad + i;
+#pragma unroll
+                for (int t = 0; t < kMfmaMaxTiles; t++)
+                    if (t < ntiles)
+                        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], brow[t * 16], acc[t], 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < kMfmaMaxTiles; t++) {
+            if (t < ntiles) {
+#pragma unroll
+                for (int v = 0; v < 4; v++) store(r0 + kq + 4 * v, t * 16 + i, acc[t][v]);
+            }
+        }
+    }
+}
+
+}  // namespace
+
+template <bool kFinal>
+__global__ __launch_bounds__(256) void k_tall_gemm_mfma(Tables T) {
+    extern __shared__ double cs[];
+    const int2 bp = T.blocks[blockIdx.x];
+    Prob &pr = T.probs[bp.x];
+    if (pr.phase != (kFinal ? PH_FINAL : PH_RESTARTING)) return;
+    const int m = pr.m;
+    const int kout = kFinal ? pr.k : pr.kc;
+    const int mpad = (m + 3) & ~3, npad = (kout + 15) & ~15;
+    for (int idx = threadIdx.x; idx < mpad * npad; idx += 256) {
+        const int l = idx / npad, c = idx % npad;
+        double v = 0.0;
+        if (l < m && c < kout) {
+            v = pr.Ub[l + (int64_t)c * m];
+            if (kFinal) v *= pr.Sb[c];
+        }
+        cs[idx] = v;
+    }
+    __syncthreads();
+    const int64_t n = pr.n;
+    const int64_t row0 = (int64_t)bp.y * kRowBlock;
+    const int64_t row_end = row0 + kRowBlock < n ? row0 + kRowBlock : n;
+    const double *A = pr.U[pr.ubuf];
+    if (kFinal) {
+        float *out = pr.PCs;
+        const int k = pr.k;
+        mfma_row_tiles(A, n, m, cs, npad, row0, row_end, wave_id(), 4,
+                       [&](int64_t row, int col, double v) {
+                           if (row < row_end && col < kout) out[row * k + col] = (float)v;
+                       });
+    } else {
+        double *out = pr.U[pr.ubuf ^ 1];
+        mfma_row_tiles(A, n, m, cs, npad, row0, row_end, wave_id(), 4,
+                       [&](int64_t row, int col, double v) {
+                           if (row < row_end && col < kout) out[(int64_t)col * n + row] = v;
+                       });
+    }
+}
+
+// Test entry: out (n x kout, column-major) = A (n x m) C (m x kout).
+__global__ __launch_bounds__(256) void k_gemm_mfma_test(const double *A,
+                                                        int64_t n, int m,
+                                                        const double *C,
+                                                        int kout,
+                                                        double *out) {
+    extern __shared__ double cs[];
+    const int mpad = (m + 3) & ~3, npad = (kout + 15) & ~15;
+    for (int idx = threadIdx.x; idx < mpad * npad; idx += 256) {
+        const int l = idx / npad, c = idx % npad;
+        cs[idx] = (l < m && c < kout) ? C[l + (int64_t)c * m] : 0.0;
+    }
+    __syncthreads();
+    const int64_t row0 = (int64_t)blockIdx.x * kRowBlock;
+    const int64_t row_end = row0 + kRowBlock < n ? row0 + kRowBlock : n;
+    mfma_row_tiles(A, n, m, cs, npad, row0, row_end, wave_id(), 4,
+                   [&](int64_t row, int col, double v) {
+                       if (row < row_end && col < kout) out[(int64_t)col * n + row] = v;
+                   });
+}
+
+hipError_t launch_gemm_mfma_test(const double *A, int64_t n, int m,
+                                 const double *C, int kout, double *out,
+                                 hipStream_t s) {
+    if (m > 4 * kMfmaMaxSteps || kout > 16 * kMfmaMaxTiles) return hipErrorInvalidValue;
+    const size_t lds = sizeof(double) * (size_t)((m + 3) & ~3) * ((kout + 15) & ~15);
+    k_gemm_mfma_test<<<(unsigned)ceil_div(n, kRowBlock), 256, lds, s>>>(A, n, m, C, kout, out);
+    return hipGetLastError();
+}
+
+// V <- V Vb[:, :kc], V[:, kc] = remainder, vs = remainder / sd.  Grid:
+// (problem, chunk of 64 genes); thread t of a chunk computes gene t % 64 for
+// columns t / 64, t / 64 + 4, ...; each entry sums over l ascending.
+__global__ __launch_bounds__(256) void k_restart_v(Tables T) {
+    const int chunks = (T.max_g + 63) / 64;
+    Prob &pr = T.probs[blockIdx.x / chunks];
     if (pr.phase != PH_RESTARTING) return;
     const int g = pr.g, m = pr.m, kc = pr.kc, mm = m + (m & 1);
+    const int i = (blockIdx.x % chunks) * 64 + (threadIdx.x & 63);
+    if (i >= g) return;
     const double *V = pr.V[pr.vbuf];
     double *V2 = pr.V[pr.vbuf ^ 1];
     const double *Vb = pr.Jv + (int64_t)mm * mm;
-    for (int idx = threadIdx.x; idx < g * kc; idx += 1024) {
-        const int c = idx / g, i = idx % g;
+    for (int c = threadIdx.x >> 6; c < kc; c += 4) {
         double acc = 0.0;
         for (int l = 0; l < m; l++) acc = fma(V[(int64_t)l * g + i], Vb[l + (int64_t)c * m], acc);
         V2[(int64_t)c * g + i] = acc;
     }
-    for (int i = threadIdx.x; i < g; i += 1024) {
+    if (threadIdx.x < 64) {
         V2[(int64_t)kc * g + i] = pr.rem[i];
         pr.vs[i] = (float)(pr.rem[i] / pr.sd[i]);
     }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        pr.ubuf ^= 1;
-        pr.vbuf ^= 1;
-        pr.j = kc;
-        pr.phase = PH_LANCZOS;
-    }
+}
+
+// Flip the U and V buffers and continue the Lanczos sweep at j = kc.
+__global__ void k_restart_commit(Tables T) {
+    Prob &pr = T.probs[blockIdx.x];
+    if (threadIdx.x != 0 || pr.phase != PH_RESTARTING) return;
+    pr.ubuf ^= 1;
+    pr.vbuf ^= 1;
+    pr.j = pr.kc;
+    pr.phase = PH_LANCZOS;
 }
 
 __global__ void k_finish(Tables T, int *done_counter) {
@@ -824,12 +1036,24 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     k_vside<<<T.num_probs, 1024, sizeof(double) * ((size_t)g + m), s>>>(T);
     mark(3, 1);
     mark(4, 0);
-    k_svd<<<T.num_probs, 1024, sizeof(double) * m + sizeof(int) * m, s>>>(T);
-    k_tall_gemm<false><<<nblk, 256, sizeof(double) * m * kGemmCols, s>>>(T);
-    k_restart_v<<<T.num_probs, 1024, 0, s>>>(T);
+    if ((int64_t)svd_lds_bytes(m, true) <= T.max_lds)
+        k_svd<true><<<T.num_probs, 1024, svd_lds_bytes(m, true), s>>>(T);
+    else
+        k_svd<false><<<T.num_probs, 1024, svd_lds_bytes(m, false), s>>>(T);
+    const size_t mfma_lds = sizeof(double) * (size_t)((m + 3) & ~3) * ((m + 15) & ~15);
+    const bool mfma = m <= 4 * kMfmaMaxSteps && (int64_t)mfma_lds <= T.max_lds;
+    if (mfma)
+        k_tall_gemm_mfma<false><<<nblk, 256, mfma_lds, s>>>(T);
+    else
+        k_tall_gemm<false><<<nblk, 256, sizeof(double) * m * kGemmCols, s>>>(T);
+    k_restart_v<<<T.num_probs * (unsigned)ceil_div(g, 64), 256, 0, s>>>(T);
+    k_restart_commit<<<T.num_probs, 64, 0, s>>>(T);
     mark(4, 1);
     mark(5, 0);
-    k_tall_gemm<true><<<nblk, 256, sizeof(double) * m * kGemmCols, s>>>(T);
+    if (mfma)
+        k_tall_gemm_mfma<true><<<nblk, 256, mfma_lds, s>>>(T);
+    else
+        k_tall_gemm<true><<<nblk, 256, sizeof(double) * m * kGemmCols, s>>>(T);
     k_finish<<<T.num_probs, 64, 0, s>>>(T, done_counter);
     mark(5, 1);
     return hipGetLastError();
@@ -907,19 +1131,25 @@ hipError_t launch_narrow_indices(const int64_t *in, int64_t nnz, int g,
 }
 
 // Single-matrix SVD through the same device code (test entry point).
+template <bool kLds>
 __global__ __launch_bounds__(1024) void k_svd_single(const double *B, int m,
-                                                     double *W, double *Jv,
+                                                     double *Wg, double *Jg,
                                                      double *Ub, double *Sb,
                                                      double *Vb, int *sweeps) {
     __shared__ int sh_int[2];
     extern __shared__ double sh[];
     const int mm = m + (m & 1);
+    double *W = kLds ? sh + m + (m + 1) / 2 : Wg;
+    double *Jv = kLds ? W + m * mm : Jg;
     for (int idx = threadIdx.x; idx < m * mm; idx += blockDim.x)
         W[idx] = idx / m < m ? B[idx] : 0.0;
     for (int idx = threadIdx.x; idx < mm * mm; idx += blockDim.x)
         Jv[idx] = (idx / mm == idx % mm) ? 1.0 : 0.0;
     __syncthreads();
-    jacobi_svd(W, Jv, m, mm, &sh_int[0], &sh_int[1]);
+    if (kLds)
+        jacobi_svd_lds(W, Jv, m, mm, &sh_int[0], &sh_int[1]);
+    else
+        jacobi_svd(W, Jv, m, mm, &sh_int[0], &sh_int[1]);
     __syncthreads();
     svd_sort_outputs(W, Jv, m, mm, sh, (int *)(sh + m), Sb, Ub, Vb);
     if (threadIdx.x == 0) *sweeps = sh_int[0];
@@ -927,11 +1157,13 @@ __global__ __launch_bounds__(1024) void k_svd_single(const double *B, int m,
 
 hipError_t launch_svd_single(const double *B, int m, double *W, double *Jv,
                              double *Ub, double *Sb, int *sweeps,
-                             hipStream_t s) {
+                             int64_t max_lds, hipStream_t s) {
     const int mm = m + (m & 1);
     double *Vb = Jv + (int64_t)mm * mm;
-    k_svd_single<<<1, 1024, sizeof(double) * m + sizeof(int) * m, s>>>(
-        B, m, W, Jv, Ub, Sb, Vb, sweeps);
+    if ((int64_t)svd_lds_bytes(m, true) <= max_lds)
+        k_svd_single<true><<<1, 1024, svd_lds_bytes(m, true), s>>>(B, m, W, Jv, Ub, Sb, Vb, sweeps);
+    else
+        k_svd_single<false><<<1, 1024, svd_lds_bytes(m, false), s>>>(B, m, W, Jv, Ub, Sb, Vb, sweeps);
     return hipGetLastError();
 }
 

@@ -362,6 +362,7 @@ int engine_rebuild_tables(bpca_engine *E) {
     E->T.num_groups = (int64_t)groups.size();
     E->T.max_g = max_g;
     E->T.max_m = max_m;
+    E->T.max_lds = (int64_t)ctx->max_lds;
     engine_drop_graph(E);
     E->dirty = false;
     return BPCA_OK;
@@ -419,6 +420,42 @@ extern "C" {
 const char *bpca_last_error(void) { return g_last_error.c_str(); }
 
 int bpca_version(void) { return 1 * 10000 + 1 * 100 + 0; }
+
+int bpca_context_info(bpca_context *ctx, int64_t *max_lds_per_workgroup,
+                      int32_t *num_cus, int32_t *uses_graph) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (max_lds_per_workgroup) *max_lds_per_workgroup = (int64_t)ctx->max_lds;
+    if (num_cus) {
+        int cus = 0;
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device);
+        *num_cus = cus;
+    }
+    if (uses_graph) *uses_graph = ctx->use_graph ? 1 : 0;
+    return BPCA_OK;
+}
+
+int bpca_test_gemm(bpca_context *ctx, int64_t n, int32_t m, int32_t kout,
+                   const double *A, const double *C, double *out) {
+    if (!ctx || !A || !C || !out || n < 1 || m < 1 || kout < 1)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "bad argument");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    hipStream_t s = ctx->stream;
+    double *dA = nullptr, *dC = nullptr, *dO = nullptr;
+    hipError_t e = hipMalloc((void **)&dA, sizeof(double) * n * m);
+    if (e == hipSuccess) e = hipMalloc((void **)&dC, sizeof(double) * m * kout);
+    if (e == hipSuccess) e = hipMalloc((void **)&dO, sizeof(double) * n * kout);
+    if (e == hipSuccess) e = hipMemcpyAsync(dA, A, sizeof(double) * n * m, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(dC, C, sizeof(double) * m * kout, hipMemcpyHostToDevice, s);
+    if (e == hipSuccess) e = bpca::launch_gemm_mfma_test(dA, n, m, dC, kout, dO, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dO, sizeof(double) * n * kout, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (dA) (void)hipFree(dA);
+    if (dC) (void)hipFree(dC);
+    if (dO) (void)hipFree(dO);
+    if (e != hipSuccess) return hip_fail(e, "test gemm");
+    return BPCA_OK;
+}
 
 int bpca_context_create(int device, uint64_t hbm_budget_bytes,
                         bpca_context **out) {
@@ -931,7 +968,7 @@ int bpca_test_small_svd(bpca_context *ctx, int32_t m, const double *B,
     HIP_TRY(dS.alloc(m));
     HIP_TRY(dsw.alloc(1));
     HIP_TRY(hipMemcpyAsync(dB.p, B, sizeof(double) * m * (m + 1), hipMemcpyHostToDevice, s));
-    HIP_TRY(bpca::launch_svd_single(dB.p, m, dW.p, dJ.p, dU.p, dS.p, dsw.p, s));
+    HIP_TRY(bpca::launch_svd_single(dB.p, m, dW.p, dJ.p, dU.p, dS.p, dsw.p, (int64_t)ctx->max_lds, s));
     HIP_TRY(hipMemcpyAsync(Ub, dU.p, sizeof(double) * m * m, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(Sb, dS.p, sizeof(double) * m, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipMemcpyAsync(Vb, dJ.p + mm * mm, sizeof(double) * m * m, hipMemcpyDeviceToHost, s));

@@ -120,6 +120,17 @@ class Context:
         return {name: (float(ms[i]), int(cnt[i]))
                 for i, name in enumerate(_lib.KERNEL_CLASSES)}
 
+    def info(self) -> dict:
+        """Device facts: max LDS per workgroup, CUs, graph replay on/off."""
+        lds = ctypes.c_int64(0)
+        cus = ctypes.c_int32(0)
+        graph = ctypes.c_int32(0)
+        _lib.check(_lib.lib().bpca_context_info(self._h, ctypes.byref(lds),
+                                                ctypes.byref(cus),
+                                                ctypes.byref(graph)))
+        return {'max_lds_per_workgroup': lds.value, 'num_cus': cus.value,
+                'graph_replay': bool(graph.value)}
+
     def cancel(self) -> None:
         """Ask a running solve on this context to stop at its next step."""
         _lib.check(_lib.lib().bpca_context_cancel(self._h))
@@ -337,6 +348,21 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+
+def tall_gemm(A, C, context: Context | None = None):
+    """A (n x m) @ C (m x kout) through the device's MFMA restart product
+    (float64; m, kout <= 128)."""
+    A = np.asfortranarray(A, dtype=np.float64)
+    C = np.asfortranarray(C, dtype=np.float64)
+    n, m = A.shape
+    kout = C.shape[1]
+    ctx = context or default_context(0)
+    out = np.zeros((n, kout), order='F')
+    _lib.check(_lib.lib().bpca_test_gemm(ctx.handle, n, m, kout,
+                                         A.ctypes.data, C.ctypes.data,
+                                         out.ctypes.data))
+    return out
 
 
 def small_svd(B, context: Context | None = None):

@@ -82,6 +82,11 @@ int bpca_context_kernel_times(bpca_context *ctx, double *total_ms,
                               int64_t *launches);
 int bpca_context_reset_kernel_times(bpca_context *ctx);
 
+/* Device facts the kernels adapt to: dynamic LDS per workgroup, compute
+ * units, and whether macro steps are replayed as a captured graph. */
+int bpca_context_info(bpca_context *ctx, int64_t *max_lds_per_workgroup,
+                      int32_t *num_cus, int32_t *uses_graph);
+
 /* Bytes of device workspace a solve of this shape allocates. */
 uint64_t bpca_workspace_bytes(uint64_t num_cells, uint32_t num_genes,
                               uint32_t k, uint32_t subspace_size);
@@ -193,6 +198,12 @@ int bpca_engine_active(bpca_engine *engine, int32_t *active);
  * the kernel the restart uses (role of sgesdd, brisc/pca.pyx:514-545). */
 int bpca_test_small_svd(bpca_context *ctx, int32_t m, const double *B,
                         double *Ub, double *Sb, double *Vb, int32_t *sweeps);
+
+/* out (n x kout) = A (n x m) C (m x kout), all column-major float64 host
+ * arrays, through the MFMA tile code of the restart product (test entry
+ * point; m <= 128, kout <= 128). */
+int bpca_test_gemm(bpca_context *ctx, int64_t n, int32_t m, int32_t kout,
+                   const double *A, const double *C, double *out);
 
 /* ------------------------------------------------ synthetic workloads */
 

@@ -336,3 +336,17 @@ def test_engine_admits_while_running_and_matches_solo():
         assert st['converged'] == 1
         assert np.array_equal(P.view(np.uint32), Ps.cpu().numpy().view(np.uint32))
         assert np.array_equal(sv, svs)
+
+
+def test_mfma_tall_gemm_matches_numpy():
+    """The MFMA FP64 restart product (16x16x4 tiles) against numpy: checks
+    the operand lane layout, k padding and ragged row / column edges."""
+    from brisc_amd.irlba import default_context, tall_gemm
+    print('case mfma gemm', default_context(0).info(), flush=True)
+    rng = np.random.default_rng(1)
+    for n, m, kout in [(16, 4, 16), (1000, 100, 99), (2049, 37, 50), (3, 5, 1)]:
+        A = rng.normal(size=(n, m))
+        C = rng.normal(size=(m, kout))
+        out = tall_gemm(A, C)
+        ref = A @ C
+        assert np.allclose(out, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max()), (n, m, kout)
