@@ -164,8 +164,8 @@ __global__ __launch_bounds__(256) void k_setup_finish(Tables T) {
 
 // ================================================================ u = X vs
 //
-// One workgroup per data block; wave w walks its 256-row sub-block two rows
-// at a time (loads of both rows in flight together).  Canonical order of a
+// One workgroup per data block; wave w walks its 256-row sub-block.  Canonical
+// order of a
 // row's sum: lane l accumulates the row's elements at positions l, l+64,
 // l+128, ... (relative to the row start) with FP32 fma in that order, then
 // the 64 lane sums meet in the xor butterfly 32,16,8,4,2,1.
@@ -201,6 +201,31 @@ __device__ inline float dot_chunk(const RowChunk &c, const float *vs,
 
 }  // namespace
 
+// Rows are taken in groups of 4 with the first chunk (256 elements) of every
+// row of the next group loaded while the current group is reduced, so up to
+// 8 rows of loads are in flight per wave.  Longer rows continue in further
+// chunks in the same per-lane order.
+namespace {
+
+struct RowGroup {
+    RowChunk c[4];
+    int64_t b[5];  // row offsets of the group's 4 rows (+ end)
+};
+
+__device__ inline void load_group(const int64_t *__restrict__ indptr,
+                                  const float *__restrict__ data,
+                                  const int32_t *__restrict__ indices,
+                                  int64_t r, int64_t rows_left, int lane,
+                                  RowGroup &G) {
+    const int nr = rows_left < 4 ? (int)rows_left : 4;
+#pragma unroll
+    for (int q = 0; q < 5; q++) G.b[q] = indptr[r + (q < nr ? q : nr)];
+#pragma unroll
+    for (int q = 0; q < 4; q++) load_chunk(data, indices, G.b[q], G.b[q + 1], lane, G.c[q]);
+}
+
+}  // namespace
+
 __global__ __launch_bounds__(256) void k_spmv(Tables T) {
     extern __shared__ float lds_v[];
     const int2 bp = T.blocks[blockIdx.x];
@@ -212,46 +237,36 @@ __global__ __launch_bounds__(256) void k_spmv(Tables T) {
     const int lane = threadIdx.x & (kWave - 1);
     int64_t r0, nrows;
     sub_block(pr.n, bp.y, wave_id(), r0, nrows);
+    if (nrows == 0) return;
     const int64_t *__restrict__ indptr = pr.indptr;
     const int32_t *__restrict__ indices = pr.indices;
     const float *__restrict__ data = pr.data;
     float *__restrict__ u = pr.u32;
-    int64_t rr = 0;
-    for (; rr + 1 < nrows; rr += 2) {
-        const int64_t ra = r0 + rr;
-        const int64_t ba = indptr[ra], ea = indptr[ra + 1], eb = indptr[ra + 2];
-        const int64_t bb = ea;
-        RowChunk ca, cb;
-        load_chunk(data, indices, ba, ea, lane, ca);
-        load_chunk(data, indices, bb, eb, lane, cb);
-        float acca = dot_chunk(ca, lds_v, ba, ea, lane, 0.0f);
-        float accb = dot_chunk(cb, lds_v, bb, eb, lane, 0.0f);
-        for (int64_t base = ba + 4 * kWave; base < ea; base += 4 * kWave) {
-            load_chunk(data, indices, base, ea, lane, ca);
-            acca = dot_chunk(ca, lds_v, base, ea, lane, acca);
+    RowGroup cur, nxt;
+    load_group(indptr, data, indices, r0, nrows, lane, cur);
+    for (int64_t rr = 0; rr < nrows; rr += 4) {
+        if (rr + 4 < nrows) load_group(indptr, data, indices, r0 + rr + 4, nrows - rr - 4, lane, nxt);
+        float acc[4];
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int64_t b = cur.b[q], e = cur.b[q + 1];
+            acc[q] = dot_chunk(cur.c[q], lds_v, b, e, lane, 0.0f);
+            for (int64_t base = b + 4 * kWave; base < e; base += 4 * kWave) {
+                RowChunk more;
+                load_chunk(data, indices, base, e, lane, more);
+                acc[q] = dot_chunk(more, lds_v, base, e, lane, acc[q]);
+            }
         }
-        for (int64_t base = bb + 4 * kWave; base < eb; base += 4 * kWave) {
-            load_chunk(data, indices, base, eb, lane, cb);
-            accb = dot_chunk(cb, lds_v, base, eb, lane, accb);
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+#pragma unroll
+            for (int q = 0; q < 4; q++) acc[q] += __shfl_xor(acc[q], off);
         }
-        acca = wave_sum(acca);
-        accb = wave_sum(accb);
-        if (lane == 0) {
-            u[ra] = acca;
-            u[ra + 1] = accb;
+        if (lane < 4 && rr + lane < nrows) {
+            const float v = lane == 0 ? acc[0] : lane == 1 ? acc[1] : lane == 2 ? acc[2] : acc[3];
+            u[r0 + rr + lane] = v;
         }
-    }
-    if (rr < nrows) {
-        const int64_t ra = r0 + rr;
-        const int64_t ba = indptr[ra], ea = indptr[ra + 1];
-        RowChunk ca;
-        float acca = 0.0f;
-        for (int64_t base = ba; base < ea; base += 4 * kWave) {
-            load_chunk(data, indices, base, ea, lane, ca);
-            acca = dot_chunk(ca, lds_v, base, ea, lane, acca);
-        }
-        acca = wave_sum(acca);
-        if (lane == 0) u[ra] = acca;
+        cur = nxt;
     }
 }
 
@@ -370,6 +385,8 @@ __global__ __launch_bounds__(256) void k_alpha(Tables T) {
 // private LDS accumulator of g floats.  u = U[:, j] / alpha is written back
 // normalised.  Per gene the sub-block sum is the FP32 fma chain over its rows
 // in order (row indices are unique); sub-blocks combine as (w0+w1)+(w2+w3).
+// Loads of the next group of 4 rows are issued before this group's LDS
+// updates.
 
 __global__ __launch_bounds__(256) void k_spmv_t(Tables T) {
     extern __shared__ float lds[];
@@ -396,35 +413,28 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T) {
     const int32_t *__restrict__ indices = pr.indices;
     const float *__restrict__ data = pr.data;
     if (nrows > 0) {
-        int64_t b = indptr[r0], e = indptr[r0 + 1];
-        RowChunk cur;
-        load_chunk(data, indices, b, e, lane, cur);
-        for (int64_t rr = 0; rr < nrows; rr++) {
-            const float ur = uval[rr];
-            // Prefetch the first chunk of the next row before the LDS
-            // updates of this one.
-            int64_t nb_ = 0, ne_ = 0;
-            RowChunk nxt;
-            if (rr + 1 < nrows) {
-                nb_ = e;
-                ne_ = indptr[r0 + rr + 2];
-                load_chunk(data, indices, nb_, ne_, lane, nxt);
-            }
+        RowGroup cur, nxt;
+        load_group(indptr, data, indices, r0, nrows, lane, cur);
+        for (int64_t rr = 0; rr < nrows; rr += 4) {
+            if (rr + 4 < nrows) load_group(indptr, data, indices, r0 + rr + 4, nrows - rr - 4, lane, nxt);
+            // Rows update the accumulator strictly in row order.
 #pragma unroll
-            for (int t = 0; t < 4; t++)
-                if (b + lane + t * kWave < e) acc[cur.ix[t]] = fmaf(cur.d[t], ur, acc[cur.ix[t]]);
-            for (int64_t base = b + 4 * kWave; base < e; base += 4 * kWave) {
-                RowChunk more;
-                load_chunk(data, indices, base, e, lane, more);
+            for (int q = 0; q < 4; q++) {
+                const int64_t b = cur.b[q], e = cur.b[q + 1];
+                if (b == e) continue;
+                const float ur = uval[rr + q];
 #pragma unroll
                 for (int t = 0; t < 4; t++)
-                    if (base + lane + t * kWave < e) acc[more.ix[t]] = fmaf(more.d[t], ur, acc[more.ix[t]]);
+                    if (b + lane + t * kWave < e) acc[cur.c[q].ix[t]] = fmaf(cur.c[q].d[t], ur, acc[cur.c[q].ix[t]]);
+                for (int64_t base = b + 4 * kWave; base < e; base += 4 * kWave) {
+                    RowChunk more;
+                    load_chunk(data, indices, base, e, lane, more);
+#pragma unroll
+                    for (int t = 0; t < 4; t++)
+                        if (base + lane + t * kWave < e) acc[more.ix[t]] = fmaf(more.d[t], ur, acc[more.ix[t]]);
+                }
             }
-            if (rr + 1 < nrows) {
-                b = nb_;
-                e = ne_;
-                cur = nxt;
-            }
+            cur = nxt;
         }
     }
     __syncthreads();
