@@ -1,10 +1,18 @@
 #!/usr/bin/env python3
 """Seconds to 50 PCs of a synthetic cells x 2000-gene matrix on one MI355X.
 
-One step = one full IRLBA solve (k=50, subspace 100, tol 1e-6, seed 0) of
-the device-resident CSR of the named workload (default: 10M cells x 2000
-genes, 100 cell types, ~1.4B nonzeros), inputs generated on the device and
-resident in HBM before the timed region.
+Default mode (the headline): one step = one full IRLBA solve (k=50,
+subspace 100, tol 1e-6, seed 0) of the device-resident CSR of the named
+workload (default: 10M cells x 2000 genes, 100 cell types, ~1.4B nonzeros),
+inputs generated on the device and resident in HBM before the timed region.
+
+--mode batched: the subclustering batch (256 problems, cells log-uniform on
+[500, 200k], 5 subtypes, 30 PCs) solved in shared launches, against the same
+problems solved one after another; reports solves/s for both.
+--mode serve: a Poisson stream of PCA requests (cells log-uniform on
+[1k, 2M], 20-50 PCs) through the continuous-batching request loop, against
+the one-at-a-time GPU path on the same stream; reports p50/p99 latency and
+solves/s for both.
 
 The JSON line carries the solve time, the roofline of the dominant kernel
 (algorithmic bytes per launch / its average launch time, from HIP events
@@ -64,21 +72,138 @@ def cpu_baseline(data, indices, indptr, n_full, nnz_full, g, k, rows):
                       f'restarts={st["restarts"]}, x{scale:.1f} by nnz'}
 
 
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def bench_batched(args):
+    import torch
+    from brisc_amd import Context, irlba_batched, irlba_device, synth
+
+    ctx = Context(0)
+    t0 = time.perf_counter()
+    works = synth.subcluster_batch(args.problems)
+    mats = [synth.generate_device(w, context=ctx) for w in works]
+    log(f'# batch of {len(works)} problems, cells {sum(w.num_cells for w in works)} '
+        f'(min {min(w.num_cells for w in works)}, max {max(w.num_cells for w in works)}), '
+        f'nnz {sum(int(m[0].numel()) for m in mats)}, prepared in {time.perf_counter() - t0:.1f} s')
+    specs = [dict(data=d, indices=i, indptr=p, num_cells=w.num_cells,
+                  num_genes=w.num_genes, k=args.batch_k, subspace_size=100,
+                  validate=False)
+             for w, (d, i, p) in zip(works, mats)]
+    irlba_batched(specs[:4], context=ctx)          # warm-up (code objects, pool)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        out = irlba_batched(specs, context=ctx)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+        log(f'# batched: {times[-1]:.3f} s')
+    batched_s = min(times)
+    t0 = time.perf_counter()
+    seq = [irlba_device(sp['data'], sp['indices'], sp['indptr'], sp['num_cells'],
+                        sp['num_genes'], sp['k'], subspace_size=100,
+                        validate=False, context=ctx) for sp in specs]
+    torch.cuda.synchronize()
+    seq_s = time.perf_counter() - t0
+    log(f'# sequential: {seq_s:.3f} s')
+    same = sum(int(np.array_equal(a[0].cpu().numpy().view(np.uint32),
+                                  b[0].cpu().numpy().view(np.uint32)))
+               for a, b in zip(out, seq))
+    conv = sum(int(o[2]['converged']) for o in out)
+    steps = [o[2]['lanczos_steps'] for o in out]
+    line = {
+        'metric': 'batched PCA solves/s (256 subclustering problems)',
+        'value': round(len(specs) / batched_s, 3), 'unit': 'solves/s',
+        'n_gpus': 1, 'steps': args.steps, 'warmup': 1,
+        'ms_per_step': round(batched_s * 1e3, 2), 'higher_is_better': True,
+        'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32 spmv / f64 lanczos', 'data': 'synthetic',
+        'config': {'workload': 'subcluster_batch', 'problems': len(specs),
+                   'cells_low': 500, 'cells_high': 200_000, 'num_types': 5,
+                   'k': args.batch_k, 'subspace_size': 100},
+        'batched_seconds': round(batched_s, 4),
+        'sequential_seconds': round(seq_s, 4),
+        'sequential_solves_per_s': round(len(specs) / seq_s, 3),
+        'speedup_vs_sequential': round(seq_s / batched_s, 3),
+        'bitwise_equal_to_sequential': f'{same}/{len(specs)}',
+        'converged': f'{conv}/{len(specs)}',
+        'lanczos_steps_max': int(max(steps)), 'lanczos_steps_sum': int(sum(steps)),
+    }
+    print(json.dumps(line), flush=True)
+
+
+def bench_serve(args):
+    import torch
+    from brisc_amd import Context, serve, synth
+
+    ctx = Context(0, hbm_budget_bytes=int(args.hbm_budget_gb * (1 << 30)))
+    t0 = time.perf_counter()
+    stream = synth.request_stream(args.requests, args.rate, high=args.max_cells)
+    reqs = []
+    for rid, (arrival, w, k) in enumerate(stream):
+        d, i, p = synth.generate_device(w, context=ctx)
+        reqs.append(serve.Request(rid, arrival, d, i, p, w.num_cells,
+                                  w.num_genes, k))
+    torch.cuda.synchronize()
+    cells = [r.num_cells for r in reqs]
+    log(f'# {len(reqs)} requests, cells {sum(cells)} (max {max(cells)}), '
+        f'rate {args.rate}/s, prepared in {time.perf_counter() - t0:.1f} s')
+    serve.serve(reqs[:2], context=ctx)                 # warm-up
+    res_b, span_b = serve.serve(reqs, context=ctx)
+    sb = serve.summarize(res_b, span_b)
+    log(f'# continuous batching: {sb}')
+    res_s, span_s = serve.serve_sequential(reqs, context=ctx)
+    ss = serve.summarize(res_s, span_s)
+    log(f'# one at a time: {ss}')
+    by_id = {r.rid: r for r in res_s}
+    same = sum(int(np.array_equal(r.PCs.cpu().numpy().view(np.uint32),
+                                  by_id[r.rid].PCs.cpu().numpy().view(np.uint32)))
+               for r in res_b)
+    line = {
+        'metric': 'serving PCA requests: p50/p99 ms and solves/s',
+        'value': sb['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1,
+        'steps': 1, 'warmup': 1, 'ms_per_step': round(span_b * 1e3, 2),
+        'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
+        'dtype': 'f32 spmv / f64 lanczos', 'data': 'synthetic',
+        'config': {'workload': 'request_stream', 'requests': len(reqs),
+                   'rate_per_s': args.rate, 'cells_low': 1000,
+                   'cells_high': args.max_cells, 'k_range': [20, 50],
+                   'hbm_budget_gb': args.hbm_budget_gb},
+        'continuous_batching': sb, 'one_at_a_time': ss,
+        'bitwise_equal_to_one_at_a_time': f'{same}/{len(res_b)}',
+    }
+    print(json.dumps(line), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
+    ap.add_argument('--mode', choices=('single', 'batched', 'serve'),
+                    default='single')
     ap.add_argument('--workload', default='cells10m')
     ap.add_argument('--k', type=int, default=50)
-    ap.add_argument('--cpu-sample-rows', type=int, default=100_000)
+    ap.add_argument('--cpu-sample-rows', type=int, default=1_000_000)
     ap.add_argument('--no-cpu-baseline', action='store_true')
+    ap.add_argument('--problems', type=int, default=256)
+    ap.add_argument('--batch-k', type=int, default=30)
+    ap.add_argument('--requests', type=int, default=200)
+    ap.add_argument('--rate', type=float, default=8.0)
+    ap.add_argument('--max-cells', type=int, default=2_000_000)
+    ap.add_argument('--hbm-budget-gb', type=float, default=200.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (asked for {args.gpus}); the '
               f'multi-GPU sharding of this path is described in DESIGN.md and '
               f'not built', file=sys.stderr)
         sys.exit(2)
+    if args.mode == 'batched':
+        return bench_batched(args)
+    if args.mode == 'serve':
+        return bench_serve(args)
 
     import torch
     from brisc_amd import Context, irlba_device, synth

@@ -14,6 +14,7 @@ ctypes.
 """
 from .irlba import (Context, Engine, default_context, irlba, irlba_batched,
                     irlba_device, small_svd)
+from . import serve, synth
 from .single_cell import SingleCell
 
 __all__ = ['Context', 'Engine', 'SingleCell', 'default_context', 'irlba',

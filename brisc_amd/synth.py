@@ -57,21 +57,22 @@ def _quadrature(points: int = 48):
     return np.exp(0.4 * x), w / w.sum()
 
 
-def expected_density(base_means: np.ndarray, c: float) -> float:
+def expected_density(base_means: np.ndarray, c: float,
+                     points: int = 48) -> float:
     """Expected nonzero fraction of NB(c*base*s, size 5) over s~LogNormal(0,.4)."""
-    s, w = _quadrature()
+    s, w = _quadrature(points)
     m = c * base_means[..., None] * s
     q = NB_SIZE / (NB_SIZE + m)
     return float(((1.0 - q ** 5) * w).sum(-1).mean())
 
 
-def calibrate_scale(base_means: np.ndarray,
-                    target: float = TARGET_DENSITY) -> float:
+def calibrate_scale(base_means: np.ndarray, target: float = TARGET_DENSITY,
+                    iterations: int = 80, points: int = 48) -> float:
     """Bisection on log c so the expected density equals ``target``."""
     lo, hi = -20.0, 5.0
-    for _ in range(80):
+    for _ in range(iterations):
         mid = 0.5 * (lo + hi)
-        if expected_density(base_means, float(np.exp(mid))) < target:
+        if expected_density(base_means, float(np.exp(mid)), points) < target:
             lo = mid
         else:
             hi = mid
@@ -80,8 +81,14 @@ def calibrate_scale(base_means: np.ndarray,
 
 def make_workload(seed: int, num_cells: int, num_types: int = 20,
                   num_genes: int = DEFAULT_NUM_GENES,
-                  target_density: float = TARGET_DENSITY) -> Workload:
-    """Draw the parameters of one workload from ``seed``."""
+                  target_density: float = TARGET_DENSITY,
+                  quick: bool = False) -> Workload:
+    """Draw the parameters of one workload from ``seed``.
+
+    ``quick`` calibrates the density scale with 30 bisection steps on an
+    8-point quadrature (for streams of many small problems); the default is
+    80 steps on 48 points.
+    """
     rng = np.random.Generator(np.random.PCG64(seed))
     mu = np.exp(rng.normal(-1.0, 1.5, num_genes))
     fold = np.ones((num_types, num_genes))
@@ -90,7 +97,8 @@ def make_workload(seed: int, num_cells: int, num_types: int = 20,
         genes = rng.choice(num_genes, num_marked, replace=False)
         fold[t, genes] = np.exp(rng.normal(0.0, 1.0, num_marked))
     base = mu[None, :] * fold
-    c = calibrate_scale(base, target_density)
+    c = (calibrate_scale(base, target_density, 30, 8) if quick
+         else calibrate_scale(base, target_density))
     type_gene_mean = np.ascontiguousarray(c * base, dtype=np.float64)
     cell_type = rng.integers(0, num_types, num_cells).astype(np.int32)
     cell_factor = np.exp(rng.normal(0.0, 0.4, num_cells)).astype(np.float64)
@@ -180,3 +188,27 @@ def batch_sizes(num_problems: int = 256, low: int = 500, high: int = 200_000,
     rng = np.random.Generator(np.random.PCG64(seed))
     return np.exp(rng.uniform(np.log(low), np.log(high),
                               num_problems)).astype(np.int64)
+
+
+def subcluster_batch(num_problems: int = 256, low: int = 500,
+                     high: int = 200_000, seed: int = 256):
+    """Workloads of the subclustering batch: log-uniform cell counts, 2000
+    genes, 5 subtypes per problem, one seed per problem."""
+    sizes = batch_sizes(num_problems, low, high, seed)
+    return [make_workload(seed * 100_003 + i, int(n), num_types=5, quick=True)
+            for i, n in enumerate(sizes)]
+
+
+def request_stream(num_requests: int = 1000, rate: float = 8.0,
+                   low: int = 1_000, high: int = 2_000_000,
+                   k_range=(20, 50), seed: int = 1000):
+    """Serving stream: Poisson arrivals at ``rate`` per second, cells
+    log-uniform on [low, high], num_PCs uniform on k_range (inclusive).
+    Returns a list of (arrival seconds, Workload, k)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, num_requests))
+    sizes = np.exp(rng.uniform(np.log(low), np.log(high), num_requests))
+    ks = rng.integers(k_range[0], k_range[1] + 1, num_requests)
+    return [(float(a), make_workload(seed * 100_003 + i, int(n), num_types=5,
+                                     quick=True), int(kk))
+            for i, (a, n, kk) in enumerate(zip(arrivals, sizes, ks))]
