@@ -177,12 +177,27 @@ def bench_serve(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_copy(args):
+    """A plain device copy of known size (calibrates HBM byte counters)."""
+    import torch
+    nbytes = int(args.copy_gib * (1 << 30))
+    src = torch.ones(nbytes // 4, dtype=torch.float32, device='cuda')
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(args.steps):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    print(json.dumps({'metric': 'device copy', 'bytes_read_per_launch': nbytes,
+                      'bytes_written_per_launch': nbytes,
+                      'launches': args.steps}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--mode', choices=('single', 'batched', 'serve'),
+    ap.add_argument('--mode', choices=('single', 'batched', 'serve', 'copy'),
                     default='single')
     ap.add_argument('--workload', default='cells10m')
     ap.add_argument('--k', type=int, default=50)
@@ -194,6 +209,7 @@ def main():
     ap.add_argument('--rate', type=float, default=8.0)
     ap.add_argument('--max-cells', type=int, default=2_000_000)
     ap.add_argument('--hbm-budget-gb', type=float, default=200.0)
+    ap.add_argument('--copy-gib', type=float, default=4.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (asked for {args.gpus}); the '
@@ -204,6 +220,8 @@ def main():
         return bench_batched(args)
     if args.mode == 'serve':
         return bench_serve(args)
+    if args.mode == 'copy':
+        return bench_copy(args)
 
     import torch
     from brisc_amd import Context, irlba_device, synth
