@@ -236,6 +236,8 @@ def main():
     print(f'# workload {args.workload}: n={n} g={g} nnz={nnz} '
           f'(generated on device in {gen_s:.2f} s)', file=sys.stderr, flush=True)
     PCs = torch.empty((n, args.k), dtype=torch.float32, device='cuda')
+    tuned = ctx.autotune(data, indices, indptr, n, g, reps=3)
+    log(f'# autotune: {tuned}')
 
     for i in range(args.warmup):
         _, sv, st = irlba_device(data, indices, indptr, n, g, args.k,
@@ -306,6 +308,7 @@ def main():
         'spmv_pair_gbs': round(bytes_per * spmv_launches / (spmv_ms * 1e-3) / 1e9, 1) if spmv_ms else None,
         'kernels': kern, 'breakdown': breakdown,
         'generate_seconds': round(gen_s, 3),
+        'autotune': tuned,
     }
     print(json.dumps(line), flush=True)
 

@@ -116,6 +116,10 @@ def lib() -> ctypes.CDLL:
         'bpca_normalize_log1ppf': (i32, [p, p, p, u64, p]),
         'bpca_test_small_svd': (i32, [p, i32, p, p, p, p, p]),
         'bpca_test_gemm': (i32, [p, i64, i32, i32, p, p, p]),
+        'bpca_context_set_kernel_config': (i32, [p, i32, i32]),
+        'bpca_context_get_kernel_config': (i32, [p, ctypes.POINTER(ctypes.c_int32),
+                                                 ctypes.POINTER(ctypes.c_int32)]),
+        'bpca_autotune': (i32, [p, ctypes.POINTER(Problem), i32, p]),
         'bpca_context_info': (i32, [p, ctypes.POINTER(ctypes.c_int64),
                                     ctypes.POINTER(ctypes.c_int32),
                                     ctypes.POINTER(ctypes.c_int32)]),

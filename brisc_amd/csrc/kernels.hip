@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_colstats(Tables T) {
     int64_t r0, nrows;
     sub_block(pr.n, bp.y, w, r0, nrows);
     const int64_t *indptr = pr.indptr;
-    const int32_t *indices = pr.indices;
+    const uint16_t *indices = pr.idx16;
     const float *data = pr.data;
     for (int64_t rr = 0; rr < nrows; rr++) {
         const int64_t b = indptr[r0 + rr], e = indptr[r0 + rr + 1];
@@ -178,7 +178,7 @@ struct RowChunk {
 };
 
 __device__ inline void load_chunk(const float *__restrict__ data,
-                                  const int32_t *__restrict__ indices,
+                                  const uint16_t *__restrict__ indices,
                                   int64_t base, int64_t e, int lane,
                                   RowChunk &c) {
 #pragma unroll
@@ -186,7 +186,7 @@ __device__ inline void load_chunk(const float *__restrict__ data,
         const int64_t p = base + lane + t * kWave;
         const bool ok = p < e;
         c.d[t] = ok ? data[p] : 0.0f;
-        c.ix[t] = ok ? indices[p] : 0;
+        c.ix[t] = ok ? (int)indices[p] : 0;
     }
 }
 
@@ -201,31 +201,35 @@ __device__ inline float dot_chunk(const RowChunk &c, const float *vs,
 
 }  // namespace
 
-// Rows are taken in groups of 4 with the first chunk (256 elements) of every
+// Rows are taken in groups of R with the first chunk (256 elements) of every
 // row of the next group loaded while the current group is reduced, so up to
-// 8 rows of loads are in flight per wave.  Longer rows continue in further
-// chunks in the same per-lane order.
+// 2R rows of loads are in flight per wave.  Longer rows continue in further
+// chunks in the same per-lane order.  R (2, 4 or 8) only changes how many
+// loads are in flight, never an order of summation; the autotuner picks it.
 namespace {
 
+template <int R>
 struct RowGroup {
-    RowChunk c[4];
-    int64_t b[5];  // row offsets of the group's 4 rows (+ end)
+    RowChunk c[R];
+    int64_t b[R + 1];  // row offsets of the group's R rows (+ end)
 };
 
+template <int R>
 __device__ inline void load_group(const int64_t *__restrict__ indptr,
                                   const float *__restrict__ data,
-                                  const int32_t *__restrict__ indices,
+                                  const uint16_t *__restrict__ indices,
                                   int64_t r, int64_t rows_left, int lane,
-                                  RowGroup &G) {
-    const int nr = rows_left < 4 ? (int)rows_left : 4;
+                                  RowGroup<R> &G) {
+    const int nr = rows_left < R ? (int)rows_left : R;
 #pragma unroll
-    for (int q = 0; q < 5; q++) G.b[q] = indptr[r + (q < nr ? q : nr)];
+    for (int q = 0; q <= R; q++) G.b[q] = indptr[r + (q < nr ? q : nr)];
 #pragma unroll
-    for (int q = 0; q < 4; q++) load_chunk(data, indices, G.b[q], G.b[q + 1], lane, G.c[q]);
+    for (int q = 0; q < R; q++) load_chunk(data, indices, G.b[q], G.b[q + 1], lane, G.c[q]);
 }
 
 }  // namespace
 
+template <int R>
 __global__ __launch_bounds__(256) void k_spmv(Tables T) {
     extern __shared__ float lds_v[];
     const int2 bp = T.blocks[blockIdx.x];
@@ -239,16 +243,16 @@ __global__ __launch_bounds__(256) void k_spmv(Tables T) {
     sub_block(pr.n, bp.y, wave_id(), r0, nrows);
     if (nrows == 0) return;
     const int64_t *__restrict__ indptr = pr.indptr;
-    const int32_t *__restrict__ indices = pr.indices;
+    const uint16_t *__restrict__ indices = pr.idx16;
     const float *__restrict__ data = pr.data;
     float *__restrict__ u = pr.u32;
-    RowGroup cur, nxt;
-    load_group(indptr, data, indices, r0, nrows, lane, cur);
-    for (int64_t rr = 0; rr < nrows; rr += 4) {
-        if (rr + 4 < nrows) load_group(indptr, data, indices, r0 + rr + 4, nrows - rr - 4, lane, nxt);
-        float acc[4];
+    RowGroup<R> cur, nxt;
+    load_group<R>(indptr, data, indices, r0, nrows, lane, cur);
+    for (int64_t rr = 0; rr < nrows; rr += R) {
+        if (rr + R < nrows) load_group<R>(indptr, data, indices, r0 + rr + R, nrows - rr - R, lane, nxt);
+        float acc[R];
 #pragma unroll
-        for (int q = 0; q < 4; q++) {
+        for (int q = 0; q < R; q++) {
             const int64_t b = cur.b[q], e = cur.b[q + 1];
             acc[q] = dot_chunk(cur.c[q], lds_v, b, e, lane, 0.0f);
             for (int64_t base = b + 4 * kWave; base < e; base += 4 * kWave) {
@@ -260,12 +264,13 @@ __global__ __launch_bounds__(256) void k_spmv(Tables T) {
 #pragma unroll
         for (int off = 32; off > 0; off >>= 1) {
 #pragma unroll
-            for (int q = 0; q < 4; q++) acc[q] += __shfl_xor(acc[q], off);
+            for (int q = 0; q < R; q++) acc[q] += __shfl_xor(acc[q], off);
         }
-        if (lane < 4 && rr + lane < nrows) {
-            const float v = lane == 0 ? acc[0] : lane == 1 ? acc[1] : lane == 2 ? acc[2] : acc[3];
-            u[r0 + rr + lane] = v;
-        }
+        float mine = acc[0];
+#pragma unroll
+        for (int q = 1; q < R; q++)
+            if (lane == q) mine = acc[q];
+        if (lane < R && rr + lane < nrows) u[r0 + rr + lane] = mine;
         cur = nxt;
     }
 }
@@ -385,9 +390,10 @@ __global__ __launch_bounds__(256) void k_alpha(Tables T) {
 // private LDS accumulator of g floats.  u = U[:, j] / alpha is written back
 // normalised.  Per gene the sub-block sum is the FP32 fma chain over its rows
 // in order (row indices are unique); sub-blocks combine as (w0+w1)+(w2+w3).
-// Loads of the next group of 4 rows are issued before this group's LDS
+// Loads of the next group of R rows are issued before this group's LDS
 // updates.
 
+template <int R>
 __global__ __launch_bounds__(256) void k_spmv_t(Tables T) {
     extern __shared__ float lds[];
     const int2 bp = T.blocks[blockIdx.x];
@@ -410,16 +416,16 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T) {
     }
     __syncthreads();
     const int64_t *__restrict__ indptr = pr.indptr;
-    const int32_t *__restrict__ indices = pr.indices;
+    const uint16_t *__restrict__ indices = pr.idx16;
     const float *__restrict__ data = pr.data;
     if (nrows > 0) {
-        RowGroup cur, nxt;
-        load_group(indptr, data, indices, r0, nrows, lane, cur);
-        for (int64_t rr = 0; rr < nrows; rr += 4) {
-            if (rr + 4 < nrows) load_group(indptr, data, indices, r0 + rr + 4, nrows - rr - 4, lane, nxt);
+        RowGroup<R> cur, nxt;
+        load_group<R>(indptr, data, indices, r0, nrows, lane, cur);
+        for (int64_t rr = 0; rr < nrows; rr += R) {
+            if (rr + R < nrows) load_group<R>(indptr, data, indices, r0 + rr + R, nrows - rr - R, lane, nxt);
             // Rows update the accumulator strictly in row order.
 #pragma unroll
-            for (int q = 0; q < 4; q++) {
+            for (int q = 0; q < R; q++) {
                 const int64_t b = cur.b[q], e = cur.b[q + 1];
                 if (b == e) continue;
                 const float ur = uval[rr + q];
@@ -1010,6 +1016,30 @@ size_t lds_bytes_needed(int g, int m) {
     return need;
 }
 
+hipError_t launch_spmv_variant(const Tables &T, int which, int group,
+                               hipStream_t s) {
+    Tables V = T;
+    V.spmv_group = group;
+    V.spmvt_group = group;
+    const unsigned nblk = (unsigned)T.num_blocks;
+    const int g = T.max_g;
+    if (which == 0) {
+        switch (group) {
+            case 2: k_spmv<2><<<nblk, 256, sizeof(float) * g, s>>>(V); break;
+            case 8: k_spmv<8><<<nblk, 256, sizeof(float) * g, s>>>(V); break;
+            default: k_spmv<4><<<nblk, 256, sizeof(float) * g, s>>>(V); break;
+        }
+    } else {
+        const size_t tlds = sizeof(float) * ((size_t)4 * g + 4 * kSubRows);
+        switch (group) {
+            case 2: k_spmv_t<2><<<nblk, 256, tlds, s>>>(V); break;
+            case 8: k_spmv_t<8><<<nblk, 256, tlds, s>>>(V); break;
+            default: k_spmv_t<4><<<nblk, 256, tlds, s>>>(V); break;
+        }
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_setup(const Tables &T, hipStream_t s) {
     if (T.num_probs == 0) return hipSuccess;
     const size_t lds4g = sizeof(float) * 4 * (size_t)T.max_g;
@@ -1030,7 +1060,11 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     };
     // Kernel classes follow enum bpca_kernel_class in bpca.h.
     mark(0, 0);
-    k_spmv<<<nblk, 256, sizeof(float) * g, s>>>(T);
+    switch (T.spmv_group) {
+        case 2: k_spmv<2><<<nblk, 256, sizeof(float) * g, s>>>(T); break;
+        case 8: k_spmv<8><<<nblk, 256, sizeof(float) * g, s>>>(T); break;
+        default: k_spmv<4><<<nblk, 256, sizeof(float) * g, s>>>(T); break;
+    }
     mark(0, 1);
     mark(2, 0);
     k_reorth_partials<<<nblk, 256, 0, s>>>(T);
@@ -1039,7 +1073,12 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     k_alpha<<<T.num_probs, 256, 0, s>>>(T);
     mark(2, 1);
     mark(1, 0);
-    k_spmv_t<<<nblk, 256, sizeof(float) * ((size_t)4 * g + 4 * kSubRows), s>>>(T);
+    const size_t tlds = sizeof(float) * ((size_t)4 * g + 4 * kSubRows);
+    switch (T.spmvt_group) {
+        case 2: k_spmv_t<2><<<nblk, 256, tlds, s>>>(T); break;
+        case 8: k_spmv_t<8><<<nblk, 256, tlds, s>>>(T); break;
+        default: k_spmv_t<4><<<nblk, 256, tlds, s>>>(T); break;
+    }
     mark(1, 1);
     mark(3, 0);
     k_reduce_groups<<<dim3((unsigned)ceil_div(g, 256), (unsigned)T.num_groups), 256, 0, s>>>(T, 0);
@@ -1072,33 +1111,35 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
 // ------------------------------------------------------- validation etc.
 
 template <typename IP>
-__global__ __launch_bounds__(256) void k_validate(
+__global__ __launch_bounds__(256) void k_validate_pack(
     const int32_t *__restrict__ indices, const IP *__restrict__ indptr,
-    int64_t n, int g, int *flag) {
+    int64_t n, int64_t nnz, int g, uint16_t *__restrict__ idx16, int *flag) {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t row = (int64_t)blockIdx.x * 4 + wave_id();
     if (row >= n) return;
     const int64_t b = indptr[row], e = indptr[row + 1];
     int bad = 0;
     if (row == 0 && b != 0) bad = 1;
-    if (e < b) bad = 1;
-    for (int64_t p = b + lane; p < e && !bad; p += kWave) {
-        const int32_t c = indices[p];
-        const int32_t prev = p > b ? indices[p - 1] : -1;
-        if (c < 0 || c >= g || c <= prev) bad = 1;
+    if (e < b || b < 0 || e > nnz) bad = 1;
+    if (!bad) {
+        for (int64_t p = b + lane; p < e; p += kWave) {
+            const int32_t c = indices[p];
+            const int32_t prev = p > b ? indices[p - 1] : -1;
+            if (c < 0 || c >= g || c <= prev) bad = 1;
+            idx16[p] = (uint16_t)c;
+        }
     }
     if (bad) atomicOr(flag, 1);
 }
 
-hipError_t launch_validate(const float *data, const int32_t *indices,
-                           const void *indptr, int indptr_width, int64_t n,
-                           int g, int *flag, hipStream_t s) {
-    (void)data;
+hipError_t launch_validate_pack(const int32_t *indices, const void *indptr,
+                                int indptr_width, int64_t n, int64_t nnz, int g,
+                                uint16_t *idx16, int *flag, hipStream_t s) {
     dim3 grid((unsigned)ceil_div(n, 4));
     if (indptr_width == 8)
-        k_validate<int64_t><<<grid, 256, 0, s>>>(indices, (const int64_t *)indptr, n, g, flag);
+        k_validate_pack<int64_t><<<grid, 256, 0, s>>>(indices, (const int64_t *)indptr, n, nnz, g, idx16, flag);
     else
-        k_validate<int32_t><<<grid, 256, 0, s>>>(indices, (const int32_t *)indptr, n, g, flag);
+        k_validate_pack<int32_t><<<grid, 256, 0, s>>>(indices, (const int32_t *)indptr, n, nnz, g, idx16, flag);
     return hipGetLastError();
 }
 

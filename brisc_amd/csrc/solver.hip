@@ -59,10 +59,10 @@ inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 // Per-problem workspace: one allocation carved into these buffers.
 struct WsLayout {
     size_t U0, U1, V0, V1, u32, P, red, sq, tpart, grp, B, sd, rem, vs, W, Jv,
-        Ub, Sb, indptr64, total;
+        Ub, Sb, indptr64, idx16, total;
 };
 
-WsLayout ws_layout(int64_t n, int g, int m, bool widen) {
+WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz) {
     const int64_t nb = bpca::ceil_div(n, bpca::kRowBlock);
     const int64_t ng = bpca::ceil_div(nb, bpca::kGroup);
     const int64_t mm = m + (m & 1);
@@ -92,6 +92,7 @@ WsLayout ws_layout(int64_t n, int g, int m, bool widen) {
     L.Ub = take(sizeof(double) * m * m);
     L.Sb = take(sizeof(double) * m);
     L.indptr64 = widen ? take(sizeof(int64_t) * (n + 1)) : 0;
+    L.idx16 = take(sizeof(uint16_t) * nnz);
     L.total = off;
     return L;
 }
@@ -188,6 +189,7 @@ struct bpca_context {
     double k_ms[BPCA_NUM_KERNEL_CLASSES] = {};
     int64_t k_count[BPCA_NUM_KERNEL_CLASSES] = {};
     bool use_graph = true;
+    int spmv_group = 4, spmvt_group = 4;   // autotuned load-group sizes
 };
 
 namespace {
@@ -363,6 +365,8 @@ int engine_rebuild_tables(bpca_engine *E) {
     E->T.max_g = max_g;
     E->T.max_m = max_m;
     E->T.max_lds = (int64_t)ctx->max_lds;
+    E->T.spmv_group = ctx->spmv_group;
+    E->T.spmvt_group = ctx->spmvt_group;
     engine_drop_graph(E);
     E->dirty = false;
     return BPCA_OK;
@@ -507,6 +511,26 @@ int bpca_context_set_stream(bpca_context *ctx, void *hip_stream) {
     return BPCA_OK;
 }
 
+int bpca_context_set_kernel_config(bpca_context *ctx, int32_t spmv_group,
+                                   int32_t spmvt_group) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    auto ok = [](int v) { return v == 2 || v == 4 || v == 8; };
+    if (!ok(spmv_group) || !ok(spmvt_group))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "load-group sizes must be 2, 4 or 8");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    ctx->spmv_group = spmv_group;
+    ctx->spmvt_group = spmvt_group;
+    return BPCA_OK;
+}
+
+int bpca_context_get_kernel_config(bpca_context *ctx, int32_t *spmv_group,
+                                   int32_t *spmvt_group) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (spmv_group) *spmv_group = ctx->spmv_group;
+    if (spmvt_group) *spmvt_group = ctx->spmvt_group;
+    return BPCA_OK;
+}
+
 int bpca_context_cancel(bpca_context *ctx) {
     if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     ctx->cancel.store(1);
@@ -544,8 +568,10 @@ int bpca_context_reset_kernel_times(bpca_context *ctx) {
 uint64_t bpca_workspace_bytes(uint64_t num_cells, uint32_t num_genes,
                               uint32_t k, uint32_t subspace_size) {
     (void)k;
+    // Without the CSR's nnz this counts the index copy at 7% density.
+    const int64_t nnz = (int64_t)((double)num_cells * num_genes * 0.07);
     return ws_layout((int64_t)num_cells, (int)num_genes, (int)subspace_size,
-                     true).total;
+                     true, nnz).total;
 }
 
 // ------------------------------------------------------------ engine ABI
@@ -609,24 +635,43 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
                         " bytes of LDS per workgroup; the device has " + std::to_string(ctx->max_lds));
     hipStream_t s = ctx->stream;
     begin_call(ctx);
-    if (pb->validate) {
-        int *flag = nullptr;
-        HIP_TRY(hipMalloc((void **)&flag, sizeof(int)));
-        int h_flag = 0;
-        hipError_t e = hipMemsetAsync(flag, 0, sizeof(int), s);
-        if (e == hipSuccess) e = bpca::launch_validate(pb->data, pb->indices, pb->indptr, pb->indptr_width, n, g, flag, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        (void)hipFree(flag);
-        if (e != hipSuccess) return hip_fail(e, "CSR validation");
-        if (h_flag)
-            return fail(BPCA_ERR_NONCANONICAL,
-                        "CSR is not canonical: indptr must start at 0 and be non-decreasing, and each row's column indices strictly increasing and below num_genes");
+    int64_t nnz = 0;
+    if (pb->indptr_width == 8) {
+        HIP_TRY(hipMemcpyAsync(&nnz, (const int64_t *)pb->indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    } else {
+        int32_t nnz32 = 0;
+        HIP_TRY(hipMemcpyAsync(&nnz32, (const int32_t *)pb->indptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        nnz = nnz32;
     }
+    HIP_TRY(hipStreamSynchronize(s));
+    if (nnz < 0) return fail(BPCA_ERR_NONCANONICAL, "indptr[num_cells] is negative");
     const bool widen = pb->indptr_width == 4;
-    const WsLayout L = ws_layout(n, g, m, widen);
+    const WsLayout L = ws_layout(n, g, m, widen, nnz);
     void *ws = nullptr;
     BPCA_TRY(pool_alloc(ctx, L.total, &ws));
+    {
+        // Canonical-CSR check fused with packing the column indices.
+        int *flag = nullptr;
+        hipError_t e = hipMalloc((void **)&flag, sizeof(int));
+        int h_flag = 0;
+        if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(int), s);
+        if (e == hipSuccess)
+            e = bpca::launch_validate_pack(pb->indices, pb->indptr, pb->indptr_width, n, nnz, g,
+                                           (uint16_t *)((char *)ws + L.idx16), flag, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (flag) (void)hipFree(flag);
+        if (e != hipSuccess) {
+            pool_free(ctx, ws);
+            return hip_fail(e, "CSR validation");
+        }
+        if (h_flag) {
+            pool_free(ctx, ws);
+            return fail(BPCA_ERR_NONCANONICAL,
+                        "CSR is not canonical: indptr must start at 0 and be non-decreasing, and each row's column indices strictly increasing and below num_genes");
+        }
+    }
     int slot = -1;
     for (int i = 0; i < (int)E->slots.size(); i++)
         if (!E->slots[i].used) { slot = i; break; }
@@ -642,7 +687,7 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
     char *base = (char *)ws;
     bpca::Prob d{};
     d.data = pb->data;
-    d.indices = pb->indices;
+    d.idx16 = (const uint16_t *)(base + L.idx16);
     d.indptr = widen ? (const int64_t *)(base + L.indptr64) : (const int64_t *)pb->indptr;
     d.PCs = pb->PCs;
     d.n = n;
@@ -701,6 +746,8 @@ int bpca_engine_step(bpca_engine *E, int32_t max_steps, int32_t *num_done) {
     DeviceGuard dg(ctx->device);
     hipStream_t s = ctx->stream;
     begin_call(ctx);
+    if (E->T.spmv_group != ctx->spmv_group || E->T.spmvt_group != ctx->spmvt_group)
+        E->dirty = true;
     if (E->dirty) BPCA_TRY(engine_rebuild_tables(E));
     if (E->setup_pending) {
         if (ctx->profiling) prof_mark(ctx, BPCA_K_SETUP, 0);
@@ -785,6 +832,56 @@ int bpca_engine_active(bpca_engine *E, int32_t *active) {
 }
 
 // ------------------------------------------------------------ solves
+
+int bpca_autotune(bpca_context *ctx, const bpca_problem *problem, int32_t reps,
+                  double *timings_ms) {
+    if (!ctx || !problem || reps < 1)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "bad argument");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    bpca_engine *E = nullptr;
+    BPCA_TRY(bpca_engine_create(ctx, &E));
+    std::unique_ptr<bpca_engine, int (*)(bpca_engine *)> hold(E, bpca_engine_destroy);
+    int32_t slot = -1, done = 0;
+    BPCA_TRY(bpca_engine_add(E, problem, &slot));
+    const bool prof = ctx->profiling;
+    ctx->profiling = false;
+    int rc = bpca_engine_step(E, 1, &done);  // set-up and one Lanczos step
+    ctx->profiling = prof;
+    BPCA_TRY(rc);
+    hipStream_t s = ctx->stream;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    HIP_TRY(hipEventCreate(&e0));
+    HIP_TRY(hipEventCreate(&e1));
+    const int groups[3] = {2, 4, 8};
+    double ms[2][3] = {};
+    hipError_t e = hipSuccess;
+    for (int which = 0; which < 2 && e == hipSuccess; which++)
+        for (int gi = 0; gi < 3 && e == hipSuccess; gi++) {
+            e = bpca::launch_spmv_variant(E->T, which, groups[gi], s);
+            if (e == hipSuccess) e = hipEventRecord(e0, s);
+            for (int r = 0; r < reps && e == hipSuccess; r++)
+                e = bpca::launch_spmv_variant(E->T, which, groups[gi], s);
+            if (e == hipSuccess) e = hipEventRecord(e1, s);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float t = 0.0f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+            ms[which][gi] = t / reps;
+        }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (e != hipSuccess) return hip_fail(e, "autotune timing");
+    int best[2] = {1, 1};
+    for (int which = 0; which < 2; which++)
+        for (int gi = 0; gi < 3; gi++)
+            if (ms[which][gi] < ms[which][best[which]]) best[which] = gi;
+    ctx->spmv_group = groups[best[0]];
+    ctx->spmvt_group = groups[best[1]];
+    if (timings_ms)
+        for (int which = 0; which < 2; which++)
+            for (int gi = 0; gi < 3; gi++) timings_ms[which * 3 + gi] = ms[which][gi];
+    return BPCA_OK;
+}
 
 int bpca_irlba_batched(bpca_context *ctx, int32_t num_problems,
                        const bpca_problem *problems, bpca_solve_stats *stats,

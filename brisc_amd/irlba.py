@@ -131,6 +131,35 @@ class Context:
         return {'max_lds_per_workgroup': lds.value, 'num_cus': cus.value,
                 'graph_replay': bool(graph.value)}
 
+    def set_kernel_config(self, spmv_group: int, spmvt_group: int) -> None:
+        """Rows per load group of X v / X^T u (2, 4 or 8; speed only)."""
+        _lib.check(_lib.lib().bpca_context_set_kernel_config(
+            self._h, int(spmv_group), int(spmvt_group)))
+
+    def kernel_config(self) -> tuple:
+        a = ctypes.c_int32(0)
+        b = ctypes.c_int32(0)
+        _lib.check(_lib.lib().bpca_context_get_kernel_config(
+            self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def autotune(self, data, indices, indptr, num_cells, num_genes,
+                 subspace_size=100, reps=5) -> dict:
+        """Time every load-group size of X v and X^T u on this matrix and
+        keep the fastest (results are bit-identical for all sizes)."""
+        import torch
+        scratch = torch.empty((num_cells, 1), dtype=torch.float32,
+                              device=data.device)
+        pb, _ = _problem(data, indices, indptr, num_cells, num_genes, 1,
+                         subspace_size, 1e-6, 100, 0, True, scratch)
+        t = np.zeros(6, dtype=np.float64)
+        _lib.check(_lib.lib().bpca_autotune(self.handle, ctypes.byref(pb),
+                                            int(reps), t.ctypes.data))
+        groups = (2, 4, 8)
+        return {'spmv_ms': dict(zip(groups, t[:3].round(4).tolist())),
+                'spmv_t_ms': dict(zip(groups, t[3:].round(4).tolist())),
+                'chosen': self.kernel_config()}
+
     def cancel(self) -> None:
         """Ask a running solve on this context to stop at its next step."""
         _lib.check(_lib.lib().bpca_context_cancel(self._h))

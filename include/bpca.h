@@ -87,6 +87,13 @@ int bpca_context_reset_kernel_times(bpca_context *ctx);
 int bpca_context_info(bpca_context *ctx, int64_t *max_lds_per_workgroup,
                       int32_t *num_cus, int32_t *uses_graph);
 
+/* Load-group sizes of the X v and X^T u kernels (rows whose loads a wave
+ * keeps in flight: 2, 4 or 8).  They change speed only: every order of
+ * summation, hence every result bit, is the same for all sizes. */
+int bpca_context_set_kernel_config(bpca_context *ctx, int32_t spmv_group,
+                                   int32_t spmvt_group);
+int bpca_context_get_kernel_config(bpca_context *ctx, int32_t *spmv_group,
+                                   int32_t *spmvt_group);
 /* Bytes of device workspace a solve of this shape allocates. */
 uint64_t bpca_workspace_bytes(uint64_t num_cells, uint32_t num_genes,
                               uint32_t k, uint32_t subspace_size);
@@ -130,8 +137,11 @@ int bpca_irlba_f32(const float *data_matvec, const void *indices_matvec,
  * d_indices are int32 column indices, d_indptr int32 or int64 (indptr_width
  * 4 or 8) row offsets with d_indptr[0] == 0.  d_PCs: num_cells x k float32,
  * C order.  singular_values (host, k doubles) and stats may be NULL.
- * validate != 0 checks on the device that each row's column indices are
- * strictly increasing and below num_genes (BPCA_ERR_NONCANONICAL if not). */
+ * Admission always checks on the device that indptr starts at 0 and never
+ * decreases and that each row's column indices are strictly increasing and
+ * below num_genes (BPCA_ERR_NONCANONICAL if not), in the same pass that packs
+ * the indices to 16 bits for the solve; `validate` is accepted for
+ * compatibility and has no further effect. */
 int bpca_irlba_device(bpca_context *ctx, const float *d_data,
                       const int32_t *d_indices, const void *d_indptr,
                       int indptr_width, uint64_t num_cells, uint32_t num_genes,
@@ -151,7 +161,7 @@ typedef struct bpca_problem {
     const int32_t *indices;   /* device, nnz int32 column indices */
     const void *indptr;       /* device, num_cells+1 int32 or int64 */
     int32_t indptr_width;     /* 4 or 8 */
-    int32_t validate;         /* check canonical CSR on admission */
+    int32_t validate;         /* accepted; the CSR is always checked */
     uint64_t num_cells;
     uint32_t num_genes;
     uint32_t k;
@@ -171,6 +181,14 @@ typedef struct bpca_problem {
 int bpca_irlba_batched(bpca_context *ctx, int32_t num_problems,
                        const bpca_problem *problems, bpca_solve_stats *stats,
                        double *singular_values, int32_t sv_stride);
+
+/* Autotuner: time each load-group size of both kernels on `problem` (a
+ * throwaway solve state; the problem's arrays are only read and its PCs
+ * buffer is not written), and make the fastest the context's configuration.
+ * timings_ms (may be NULL) receives 6 averages: X v at 2, 4, 8 then X^T u
+ * at 2, 4, 8. */
+int bpca_autotune(bpca_context *ctx, const bpca_problem *problem, int32_t reps,
+                  double *timings_ms);
 
 /* Continuous-batching engine (the serving loop drives it): problems are
  * admitted while others run, subject to the context's HBM budget

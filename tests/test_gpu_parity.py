@@ -350,3 +350,27 @@ def test_mfma_tall_gemm_matches_numpy():
         out = tall_gemm(A, C)
         ref = A @ C
         assert np.allclose(out, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max()), (n, m, kout)
+
+
+def test_autotuned_load_groups_do_not_change_bits():
+    """The autotuner's knob (rows per load group of X v / X^T u) changes
+    speed only: every configuration gives the same bits."""
+    from brisc_amd import Context
+    w = synth.make_workload(88, 20_000, num_types=10)
+    d, i, p = to_device(*oracle.generate_csr(w))
+    ctx = Context(0)
+    print('case load groups', flush=True)
+    tuned = ctx.autotune(d, i, p, w.num_cells, w.num_genes, reps=2)
+    print('autotune', tuned, flush=True)
+    assert tuned['chosen'][0] in (2, 4, 8) and tuned['chosen'][1] in (2, 4, 8)
+    assert all(v > 0 for v in tuned['spmv_ms'].values())
+    ref = None
+    for cfg in [(4, 4), (2, 8), (8, 2), (2, 2), (8, 8)]:
+        ctx.set_kernel_config(*cfg)
+        P, sv, st = irlba_device(d, i, p, w.num_cells, w.num_genes, 20,
+                                 subspace_size=50, context=ctx)
+        bits = P.cpu().numpy().view(np.uint32)
+        if ref is None:
+            ref = bits
+        assert np.array_equal(bits, ref), cfg
+    ctx.close()
