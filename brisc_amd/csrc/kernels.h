@@ -5,8 +5,10 @@
 // a batch of uneven problems and a serving mix all run the same arithmetic.
 //
 // Data layout in HBM for one problem (n cells, g genes, m subspace, k PCs):
-//   X        CSR: data float32[nnz], idx16 uint16[nnz] (column indices packed
-//            from the caller's int32 ones at admission), indptr int64[n+1]
+//   X        CSR: data float32[nnz4], idx16 uint16[nnz4] -- copies made at
+//            admission, 16-byte aligned and zero-padded to nnz4 = nnz rounded
+//            up to a multiple of 4, so every aligned quad of 4 elements can be
+//            loaded whole -- and indptr int64[n+1]
 //   U[0/1]   float64 column-major n x m (ld n): left Lanczos vectors
 //   V[0/1]   float64 column-major g x (m+1) (ld g): right Lanczos vectors
 //   u32      float32[n]: raw X v' of the current step
@@ -47,7 +49,7 @@ enum Phase : int32_t {
 
 struct Prob {
     // Input CSR and output.
-    const float *data;
+    const float *data;       // packed copy of the values (see above)
     const uint16_t *idx16;   // column indices packed to 16 bits (g <= 65535)
     const int64_t *indptr;
     float *PCs;  // n x k float32 row-major
@@ -82,8 +84,8 @@ struct Tables {
     int64_t num_groups;
     int max_g, max_m;
     int64_t max_lds;      // dynamic LDS bytes a workgroup may use
-    int spmv_group;       // rows per load group in X v   (2, 4 or 8)
-    int spmvt_group;      // rows per load group in X^T u (2, 4 or 8)
+    int spmv_group;       // segments per load group in X v   (2, 4 or 8)
+    int spmvt_group;      // segments per load group in X^T u (2, 4 or 8)
 };
 
 // Optional timing hook around each class of launches (null when captured
@@ -103,11 +105,14 @@ hipError_t launch_spmv_variant(const Tables &T, int which, int group,
                                hipStream_t s);
 // Largest dynamic LDS request of any kernel for g genes and subspace m.
 size_t lds_bytes_needed(int g, int m);
-// Canonical-CSR check of one problem's operand fused with packing its column
-// indices to 16 bits: sets *flag when indptr does not start at 0, decreases
-// or exceeds nnz, or a row's indices are not strictly increasing in [0, g).
-hipError_t launch_validate_pack(const int32_t *indices, const void *indptr,
-                                int indptr_width, int64_t n, int64_t nnz, int g,
+// Canonical-CSR check of one problem's operand fused with packing it: column
+// indices to 16 bits (idx16) and the values into the aligned copy (data4);
+// both copies must hold nnz rounded up to 4 elements, the tail zeroed by the
+// caller.  Sets *flag when indptr does not start at 0, decreases or exceeds
+// nnz, or a row's indices are not strictly increasing in [0, g).
+hipError_t launch_validate_pack(const float *data, const int32_t *indices,
+                                const void *indptr, int indptr_width, int64_t n,
+                                int64_t nnz, int g, float *data4,
                                 uint16_t *idx16, int *flag, hipStream_t s);
 // indptr (int32) -> int64 and int64 indices -> int32 (flagging range errors).
 hipError_t launch_widen_indptr(const int32_t *in, int64_t count, int64_t *out,

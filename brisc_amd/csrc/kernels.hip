@@ -164,71 +164,185 @@ __global__ __launch_bounds__(256) void k_setup_finish(Tables T) {
 
 // ================================================================ u = X vs
 //
-// One workgroup per data block; wave w walks its 256-row sub-block.  Canonical
-// order of a
-// row's sum: lane l accumulates the row's elements at positions l, l+64,
-// l+128, ... (relative to the row start) with FP32 fma in that order, then
-// the 64 lane sums meet in the xor butterfly 32,16,8,4,2,1.
+// One workgroup per data block; wave w walks its 256-row sub-block.  Each
+// lane loads whole aligned quads of 4 elements (16-byte values, 8-byte
+// packed indices), so a 256-element window of a row costs two load
+// instructions.  Canonical order of a row's sum: the row is cut into windows
+// of 64 quads starting at the quad holding its first element; lane l takes
+// quad l of each window and accumulates its quad's elements of the row in
+// position order with FP32 fma, window after window; then the 64 lane sums
+// meet in the xor butterfly 32,16,8,4,2,1.  The order is fixed by the data
+// (a row's position modulo 4 included), never by the launch.
 
 namespace {
 
-struct RowChunk {
-    float d[4];
-    int ix[4];
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef const f32x4 __attribute__((address_space(1))) gquad;    // 4 values
+typedef const u32x2 __attribute__((address_space(1))) gquad16;  // 4 indices
+typedef float __attribute__((address_space(1))) gfloat_w;
+
+struct Quad {
+    f32x4 d;
+    u32x2 ix;  // four 16-bit column indices, element 0 in the low half of x
 };
 
-__device__ inline void load_chunk(const float *__restrict__ data,
-                                  const uint16_t *__restrict__ indices,
-                                  int64_t base, int64_t e, int lane,
-                                  RowChunk &c) {
+__device__ __forceinline__ int quad_index(const Quad &c, int k) {
+    const unsigned w = k < 2 ? c.ix.x : c.ix.y;
+    return (int)((k & 1) ? (w >> 16) : (w & 0xffffu));
+}
+
+__device__ __forceinline__ float quad_value(const Quad &c, int k) {
+    return k == 0 ? c.d.x : k == 1 ? c.d.y : k == 2 ? c.d.z : c.d.w;
+}
+
+// Row offsets of rows [r0, r0 + nrows], staged by the wave in its own LDS
+// slice `off` as positions relative to the quad holding indptr[r0] (so
+// off[0] = indptr[r0] mod 4), so that a row's bounds never wait behind the
+// wave's outstanding data loads.  Returns that quad's index (nrows > 0).
+__device__ inline int64_t stage_row_offsets(const int64_t *__restrict__ indptr,
+                                            int64_t r0, int nrows, int lane,
+                                            int *off) {
+    const int64_t q0 = indptr[r0] >> 2;
+    for (int q = lane; q <= nrows; q += kWave) off[q] = (int)(indptr[r0 + q] - 4 * q0);
+    __builtin_amdgcn_wave_barrier();
+    return q0;
+}
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// A wave walks its sub-block as a stream of segments: a row is cut into
+// windows of 64 quads (an empty row is one empty segment).  Segments are
+// taken R at a time; the next group's loads are issued before the current
+// group is reduced, so R segments of loads are always in flight per wave,
+// and no load sits behind a branch (lanes past a segment's end re-load its
+// last quad, a cache line the segment already touches; positions are tested
+// before use).  A row's per-lane partial sum runs across its windows in
+// order, so the order of summation is the one above for every R; R (2, 4
+// or 8) is chosen by the autotuner.
+
+constexpr int kWindow = 4 * kWave;  // elements per full window
+
+// Uniform walk state: row r, its next position p and its end e.
+struct Cursor {
+    int r, p, e;
+};
+
+template <int R>
+struct SegGroup {
+    Quad c[R];
+    int b[R], e[R];  // segment bounds (positions relative to the first quad)
+    int q0[R];       // first quad of the segment's window
+    int row[R];      // row of the segment (nrows past the last row)
+    bool fin[R];     // the segment ends row `row`
+};
+
+// Forms the next R segments from `cs` and issues their loads.
+template <int R>
+__device__ __forceinline__ void form_group(const int *off, int nrows,
+                                           gquad *data, gquad16 *indices,
+                                           int lane, Cursor &cs,
+                                           SegGroup<R> &G) {
+    int W[R + 1];  // ends of rows cs.r + 1 ... cs.r + R + 1
 #pragma unroll
-    for (int t = 0; t < 4; t++) {
-        const int64_t p = base + lane + t * kWave;
-        const bool ok = p < e;
-        c.d[t] = ok ? data[p] : 0.0f;
-        c.ix[t] = ok ? (int)indices[p] : 0;
+    for (int i = 0; i <= R; i++) {
+        const int r = cs.r + 2 + i;
+        W[i] = uniform(off[r < nrows ? r : nrows]);
+    }
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        const int q0 = cs.p >> 2;
+        const int wend = 4 * (q0 + kWave);
+        const int se = wend < cs.e ? wend : cs.e;
+        const bool fin = wend >= cs.e;
+        const int lastq = (se > 0 ? se - 1 : 0) >> 2;
+        G.b[q] = cs.p;
+        G.e[q] = se;
+        G.q0[q] = q0;
+        G.row[q] = cs.r;
+        G.fin[q] = fin && cs.r < nrows;
+        const int qi = q0 + lane < lastq ? q0 + lane : lastq;
+        G.c[q].d = data[qi];
+        G.c[q].ix = indices[qi];
+        if (fin) {
+            cs.r = cs.r < nrows ? cs.r + 1 : nrows;
+            cs.p = cs.e;
+            cs.e = W[0];
+#pragma unroll
+            for (int i = 0; i < R; i++) W[i] = W[i + 1];
+        } else {
+            cs.p = wend;
+        }
     }
 }
 
-__device__ inline float dot_chunk(const RowChunk &c, const float *vs,
-                                  int64_t base, int64_t e, int lane,
-                                  float acc) {
+// Partial sums continue across a row's segments (`carry` holds the open
+// row's); the rows that end in the group go through one batched xor
+// butterfly.  The butterfly leaves every row's sum in all lanes; lane q
+// takes slot q's.  The rows of a group's slots are consecutive, so lane q's
+// row is the first slot's row plus the finished slots before q.  Every lane
+// stores -- lanes without a finished row into the sink slot -- so the store
+// is not behind a branch and the waits for loads stay fixed counts.
+template <int R>
+__device__ __forceinline__ void spmv_reduce(const SegGroup<R> &G,
+                                            const float *lds_v, int lane,
+                                            float &carry, gfloat_w *u,
+                                            gfloat_w *sink) {
+    float done[R];
+    float acc = carry;
 #pragma unroll
-    for (int t = 0; t < 4; t++)
-        if (base + lane + t * kWave < e) acc = fmaf(c.d[t], vs[c.ix[t]], acc);
-    return acc;
+    for (int q = 0; q < R; q++) {
+        // Branch-free: every lane gathers all four v entries (a quad's
+        // indices are all below g, padding included) and elements outside
+        // the segment enter the chain as 0 * v.
+        const int a0 = 4 * (G.q0[q] + lane);
+        float vv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) vv[k] = lds_v[quad_index(G.c[q], k)];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool in = a0 + k >= G.b[q] && a0 + k < G.e[q];
+            acc = fmaf(in ? quad_value(G.c[q], k) : 0.0f, vv[k], acc);
+        }
+        done[q] = acc;
+        acc = G.fin[q] ? 0.0f : acc;
+    }
+    carry = acc;
+#pragma unroll
+    for (int off2 = 32; off2 > 0; off2 >>= 1) {
+#pragma unroll
+        for (int q = 0; q < R; q++) done[q] += __shfl_xor(done[q], off2);
+    }
+    float mine = done[0];
+    unsigned finmask = 0;
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        mine = lane == q ? done[q] : mine;
+        finmask |= (unsigned)G.fin[q] << q;
+    }
+    const bool wr = lane < R && ((finmask >> lane) & 1u);
+    const int row = G.row[0] + __popc(finmask & ((1u << lane) - 1u));
+    *(wr ? u + row : sink) = mine;
 }
 
 }  // namespace
 
-// Rows are taken in groups of R with the first chunk (256 elements) of every
-// row of the next group loaded while the current group is reduced, so up to
-// 2R rows of loads are in flight per wave.  Longer rows continue in further
-// chunks in the same per-lane order.  R (2, 4 or 8) only changes how many
-// loads are in flight, never an order of summation; the autotuner picks it.
-namespace {
-
-template <int R>
-struct RowGroup {
-    RowChunk c[R];
-    int64_t b[R + 1];  // row offsets of the group's R rows (+ end)
-};
-
-template <int R>
-__device__ inline void load_group(const int64_t *__restrict__ indptr,
-                                  const float *__restrict__ data,
-                                  const uint16_t *__restrict__ indices,
-                                  int64_t r, int64_t rows_left, int lane,
-                                  RowGroup<R> &G) {
-    const int nr = rows_left < R ? (int)rows_left : R;
-#pragma unroll
-    for (int q = 0; q <= R; q++) G.b[q] = indptr[r + (q < nr ? q : nr)];
-#pragma unroll
-    for (int q = 0; q < R; q++) load_chunk(data, indices, G.b[q], G.b[q + 1], lane, G.c[q]);
+// LDS of k_spmv: vs (g floats), then 4 wave slices of kSubRows + 1 offsets.
+size_t spmv_lds_bytes(int g) {
+    return sizeof(float) * (size_t)g + sizeof(int) * 4 * (size_t)(kSubRows + 1);
 }
 
-}  // namespace
+// LDS of k_spmv_t: 4 wave accumulators of g + 1 floats (acc[g] is a spare
+// slot), 4 slices of kSubRows u values, 4 slices of kSubRows + 1 offsets.
+size_t spmv_t_lds_bytes(int g) {
+    return sizeof(float) * ((size_t)4 * (g + 1) + 4 * kSubRows) +
+           sizeof(int) * 4 * (size_t)(kSubRows + 1);
+}
 
+// Two segment groups alternate without register copies.  The loop has one
+// exit, after reducing the group that finished the last row (groups formed
+// past it are empty: their loads re-read the sub-block's last quad and their
+// reductions write nothing).
 template <int R>
 __global__ __launch_bounds__(256) void k_spmv(Tables T) {
     extern __shared__ float lds_v[];
@@ -237,41 +351,35 @@ __global__ __launch_bounds__(256) void k_spmv(Tables T) {
     if (pr.phase != PH_LANCZOS) return;
     const int g = pr.g;
     for (int i = threadIdx.x; i < g; i += 256) lds_v[i] = pr.vs[i];
-    __syncthreads();
     const int lane = threadIdx.x & (kWave - 1);
-    int64_t r0, nrows;
-    sub_block(pr.n, bp.y, wave_id(), r0, nrows);
-    if (nrows == 0) return;
-    const int64_t *__restrict__ indptr = pr.indptr;
-    const uint16_t *__restrict__ indices = pr.idx16;
-    const float *__restrict__ data = pr.data;
-    float *__restrict__ u = pr.u32;
-    RowGroup<R> cur, nxt;
-    load_group<R>(indptr, data, indices, r0, nrows, lane, cur);
-    for (int64_t rr = 0; rr < nrows; rr += R) {
-        if (rr + R < nrows) load_group<R>(indptr, data, indices, r0 + rr + R, nrows - rr - R, lane, nxt);
-        float acc[R];
-#pragma unroll
-        for (int q = 0; q < R; q++) {
-            const int64_t b = cur.b[q], e = cur.b[q + 1];
-            acc[q] = dot_chunk(cur.c[q], lds_v, b, e, lane, 0.0f);
-            for (int64_t base = b + 4 * kWave; base < e; base += 4 * kWave) {
-                RowChunk more;
-                load_chunk(data, indices, base, e, lane, more);
-                acc[q] = dot_chunk(more, lds_v, base, e, lane, acc[q]);
-            }
-        }
-#pragma unroll
-        for (int off = 32; off > 0; off >>= 1) {
-#pragma unroll
-            for (int q = 0; q < R; q++) acc[q] += __shfl_xor(acc[q], off);
-        }
-        float mine = acc[0];
-#pragma unroll
-        for (int q = 1; q < R; q++)
-            if (lane == q) mine = acc[q];
-        if (lane < R && rr + lane < nrows) u[r0 + rr + lane] = mine;
-        cur = nxt;
+    const int w = wave_id();
+    int *off = (int *)(lds_v + g) + w * (kSubRows + 1);
+    int64_t r0, nrows64;
+    sub_block(pr.n, bp.y, w, r0, nrows64);
+    const int nrows = (int)nrows64;
+    const int64_t q0 = nrows > 0 ? stage_row_offsets(pr.indptr, r0, nrows, lane, off) : 0;
+    __syncthreads();
+    if (nrows == 0 || off[nrows] == off[0]) {
+        for (int q = lane; q < nrows; q += kWave) pr.u32[r0 + q] = 0.0f;
+        return;
+    }
+    gquad *data = (gquad *)pr.data + q0;
+    gquad16 *indices = (gquad16 *)pr.idx16 + q0;
+    gfloat_w *u = (gfloat_w *)(pr.u32 + r0);
+    gfloat_w *sink = (gfloat_w *)(pr.u32 + pr.n);
+    Cursor cs{0, uniform(off[0]), uniform(off[1])};
+    SegGroup<R> ga, gb;
+    float carry = 0.0f;
+    form_group<R>(off, nrows, data, indices, lane, cs, ga);
+    for (;;) {
+        form_group<R>(off, nrows, data, indices, lane, cs, gb);
+        __builtin_amdgcn_sched_barrier(0);
+        spmv_reduce<R>(ga, lds_v, lane, carry, u, sink);
+        const bool b_last = cs.r >= nrows;  // gb finished the last row
+        form_group<R>(off, nrows, data, indices, lane, cs, ga);
+        __builtin_amdgcn_sched_barrier(0);
+        spmv_reduce<R>(gb, lds_v, lane, carry, u, sink);
+        if (b_last) break;
     }
 }
 
@@ -385,14 +493,42 @@ __global__ __launch_bounds__(256) void k_alpha(Tables T) {
 }
 
 // ================================================================ X^T u
-//
+
+namespace {
+
+// acc[col] += x * u[row] for the segments of a loaded group, in row order.
+template <int R>
+__device__ __forceinline__ void spmv_t_reduce(const SegGroup<R> &G,
+                                              const float *uval, int lane,
+                                              int g, float *acc) {
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        // Branch-free: elements outside the segment update the spare slot
+        // acc[g] instead, so the lanes of one LDS update still touch
+        // distinct live genes (a row's indices are unique).
+        const float ur = uval[G.row[q] < kSubRows ? G.row[q] : 0];
+        const int a0 = 4 * (G.q0[q] + lane);
+        int c[4];
+        float x[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool in = a0 + k >= G.b[q] && a0 + k < G.e[q];
+            c[k] = in ? quad_index(G.c[q], k) : g;
+            x[k] = acc[c[k]];
+        }
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc[c[k]] = fmaf(quad_value(G.c[q], k), ur, x[k]);
+    }
+}
+
+}  // namespace
+
 // One workgroup per data block; wave w owns a 256-row sub-block and a
 // private LDS accumulator of g floats.  u = U[:, j] / alpha is written back
 // normalised.  Per gene the sub-block sum is the FP32 fma chain over its rows
-// in order (row indices are unique); sub-blocks combine as (w0+w1)+(w2+w3).
-// Loads of the next group of R rows are issued before this group's LDS
-// updates.
-
+// in order (a row's column indices are unique, so the lanes of one LDS update
+// touch distinct genes); sub-blocks combine as (w0+w1)+(w2+w3).  Loads are
+// the segment groups of k_spmv.
 template <int R>
 __global__ __launch_bounds__(256) void k_spmv_t(Tables T) {
     extern __shared__ float lds[];
@@ -402,11 +538,14 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T) {
     const int g = pr.g;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = wave_id();
-    float *acc = lds + (size_t)w * g;
-    float *uval = lds + (size_t)4 * g + (size_t)w * kSubRows;
-    for (int i = lane; i < g; i += kWave) acc[i] = 0.0f;
-    int64_t r0, nrows;
-    sub_block(pr.n, bp.y, w, r0, nrows);
+    const int gs = g + 1;  // accumulator stride (spare slot at [g])
+    float *acc = lds + (size_t)w * gs;
+    float *uval = lds + (size_t)4 * gs + (size_t)w * kSubRows;
+    int *off = (int *)(lds + (size_t)4 * gs + 4 * kSubRows) + w * (kSubRows + 1);
+    for (int i = lane; i < gs; i += kWave) acc[i] = 0.0f;
+    int64_t r0, nrows64;
+    sub_block(pr.n, bp.y, w, r0, nrows64);
+    const int nrows = (int)nrows64;
     double *Uj = pr.U[pr.ubuf] + (int64_t)pr.j * pr.n;
     const double inv_alpha = pr.inv_alpha;
     for (int t = lane; t < nrows; t += kWave) {
@@ -414,39 +553,29 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T) {
         Uj[r0 + t] = x;
         uval[t] = (float)x;
     }
+    const int64_t q0 = nrows > 0 ? stage_row_offsets(pr.indptr, r0, nrows, lane, off) : 0;
     __syncthreads();
-    const int64_t *__restrict__ indptr = pr.indptr;
-    const uint16_t *__restrict__ indices = pr.idx16;
-    const float *__restrict__ data = pr.data;
-    if (nrows > 0) {
-        RowGroup<R> cur, nxt;
-        load_group<R>(indptr, data, indices, r0, nrows, lane, cur);
-        for (int64_t rr = 0; rr < nrows; rr += R) {
-            if (rr + R < nrows) load_group<R>(indptr, data, indices, r0 + rr + R, nrows - rr - R, lane, nxt);
-            // Rows update the accumulator strictly in row order.
-#pragma unroll
-            for (int q = 0; q < R; q++) {
-                const int64_t b = cur.b[q], e = cur.b[q + 1];
-                if (b == e) continue;
-                const float ur = uval[rr + q];
-#pragma unroll
-                for (int t = 0; t < 4; t++)
-                    if (b + lane + t * kWave < e) acc[cur.c[q].ix[t]] = fmaf(cur.c[q].d[t], ur, acc[cur.c[q].ix[t]]);
-                for (int64_t base = b + 4 * kWave; base < e; base += 4 * kWave) {
-                    RowChunk more;
-                    load_chunk(data, indices, base, e, lane, more);
-#pragma unroll
-                    for (int t = 0; t < 4; t++)
-                        if (base + lane + t * kWave < e) acc[more.ix[t]] = fmaf(more.d[t], ur, acc[more.ix[t]]);
-                }
-            }
-            cur = nxt;
+    if (nrows > 0 && off[nrows] > off[0]) {
+        gquad *data = (gquad *)pr.data + q0;
+        gquad16 *indices = (gquad16 *)pr.idx16 + q0;
+        Cursor cs{0, uniform(off[0]), uniform(off[1])};
+        SegGroup<R> ga, gb;
+        form_group<R>(off, nrows, data, indices, lane, cs, ga);
+        for (;;) {
+            form_group<R>(off, nrows, data, indices, lane, cs, gb);
+            __builtin_amdgcn_sched_barrier(0);
+            spmv_t_reduce<R>(ga, uval, lane, g, acc);
+            const bool b_last = cs.r >= nrows;  // gb finished the last row
+            form_group<R>(off, nrows, data, indices, lane, cs, ga);
+            __builtin_amdgcn_sched_barrier(0);
+            spmv_t_reduce<R>(gb, uval, lane, g, acc);
+            if (b_last) break;
         }
     }
     __syncthreads();
     float *out = pr.tpart + (int64_t)bp.y * 2 * g;
     for (int i = threadIdx.x; i < g; i += 256)
-        out[i] = (lds[i] + lds[g + i]) + (lds[2 * g + i] + lds[3 * g + i]);
+        out[i] = (lds[i] + lds[gs + i]) + (lds[2 * gs + i] + lds[3 * gs + i]);
 }
 
 // ================================================================ V side
@@ -1009,7 +1138,8 @@ __global__ void k_finish(Tables T, int *done_counter) {
 // ================================================================ launchers
 
 size_t lds_bytes_needed(int g, int m) {
-    size_t need = sizeof(float) * ((size_t)4 * g + 4 * kSubRows);   // spmv_t
+    size_t need = spmv_t_lds_bytes(g);
+    need = std::max(need, spmv_lds_bytes(g));
     need = std::max(need, sizeof(double) * ((size_t)g + m));         // vside
     need = std::max(need, sizeof(double) * (size_t)m * kGemmCols);   // gemm
     need = std::max(need, sizeof(double) * (size_t)m + sizeof(int) * (size_t)m);
@@ -1025,12 +1155,12 @@ hipError_t launch_spmv_variant(const Tables &T, int which, int group,
     const int g = T.max_g;
     if (which == 0) {
         switch (group) {
-            case 2: k_spmv<2><<<nblk, 256, sizeof(float) * g, s>>>(V); break;
-            case 8: k_spmv<8><<<nblk, 256, sizeof(float) * g, s>>>(V); break;
-            default: k_spmv<4><<<nblk, 256, sizeof(float) * g, s>>>(V); break;
+            case 2: k_spmv<2><<<nblk, 256, spmv_lds_bytes(g), s>>>(V); break;
+            case 8: k_spmv<8><<<nblk, 256, spmv_lds_bytes(g), s>>>(V); break;
+            default: k_spmv<4><<<nblk, 256, spmv_lds_bytes(g), s>>>(V); break;
         }
     } else {
-        const size_t tlds = sizeof(float) * ((size_t)4 * g + 4 * kSubRows);
+        const size_t tlds = spmv_t_lds_bytes(g);
         switch (group) {
             case 2: k_spmv_t<2><<<nblk, 256, tlds, s>>>(V); break;
             case 8: k_spmv_t<8><<<nblk, 256, tlds, s>>>(V); break;
@@ -1061,9 +1191,9 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     // Kernel classes follow enum bpca_kernel_class in bpca.h.
     mark(0, 0);
     switch (T.spmv_group) {
-        case 2: k_spmv<2><<<nblk, 256, sizeof(float) * g, s>>>(T); break;
-        case 8: k_spmv<8><<<nblk, 256, sizeof(float) * g, s>>>(T); break;
-        default: k_spmv<4><<<nblk, 256, sizeof(float) * g, s>>>(T); break;
+        case 2: k_spmv<2><<<nblk, 256, spmv_lds_bytes(g), s>>>(T); break;
+        case 8: k_spmv<8><<<nblk, 256, spmv_lds_bytes(g), s>>>(T); break;
+        default: k_spmv<4><<<nblk, 256, spmv_lds_bytes(g), s>>>(T); break;
     }
     mark(0, 1);
     mark(2, 0);
@@ -1073,7 +1203,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     k_alpha<<<T.num_probs, 256, 0, s>>>(T);
     mark(2, 1);
     mark(1, 0);
-    const size_t tlds = sizeof(float) * ((size_t)4 * g + 4 * kSubRows);
+    const size_t tlds = spmv_t_lds_bytes(g);
     switch (T.spmvt_group) {
         case 2: k_spmv_t<2><<<nblk, 256, tlds, s>>>(T); break;
         case 8: k_spmv_t<8><<<nblk, 256, tlds, s>>>(T); break;
@@ -1112,8 +1242,9 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
 
 template <typename IP>
 __global__ __launch_bounds__(256) void k_validate_pack(
-    const int32_t *__restrict__ indices, const IP *__restrict__ indptr,
-    int64_t n, int64_t nnz, int g, uint16_t *__restrict__ idx16, int *flag) {
+    const float *__restrict__ data, const int32_t *__restrict__ indices,
+    const IP *__restrict__ indptr, int64_t n, int64_t nnz, int g,
+    float *__restrict__ data4, uint16_t *__restrict__ idx16, int *flag) {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t row = (int64_t)blockIdx.x * 4 + wave_id();
     if (row >= n) return;
@@ -1127,19 +1258,21 @@ __global__ __launch_bounds__(256) void k_validate_pack(
             const int32_t prev = p > b ? indices[p - 1] : -1;
             if (c < 0 || c >= g || c <= prev) bad = 1;
             idx16[p] = (uint16_t)c;
+            data4[p] = data[p];
         }
     }
     if (bad) atomicOr(flag, 1);
 }
 
-hipError_t launch_validate_pack(const int32_t *indices, const void *indptr,
-                                int indptr_width, int64_t n, int64_t nnz, int g,
+hipError_t launch_validate_pack(const float *data, const int32_t *indices,
+                                const void *indptr, int indptr_width, int64_t n,
+                                int64_t nnz, int g, float *data4,
                                 uint16_t *idx16, int *flag, hipStream_t s) {
     dim3 grid((unsigned)ceil_div(n, 4));
     if (indptr_width == 8)
-        k_validate_pack<int64_t><<<grid, 256, 0, s>>>(indices, (const int64_t *)indptr, n, nnz, g, idx16, flag);
+        k_validate_pack<int64_t><<<grid, 256, 0, s>>>(data, indices, (const int64_t *)indptr, n, nnz, g, data4, idx16, flag);
     else
-        k_validate_pack<int32_t><<<grid, 256, 0, s>>>(indices, (const int32_t *)indptr, n, nnz, g, idx16, flag);
+        k_validate_pack<int32_t><<<grid, 256, 0, s>>>(data, indices, (const int32_t *)indptr, n, nnz, g, data4, idx16, flag);
     return hipGetLastError();
 }
 

@@ -59,7 +59,8 @@ inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 // Per-problem workspace: one allocation carved into these buffers.
 struct WsLayout {
     size_t U0, U1, V0, V1, u32, P, red, sq, tpart, grp, B, sd, rem, vs, W, Jv,
-        Ub, Sb, indptr64, idx16, total;
+        Ub, Sb, indptr64, idx16, data4, total;
+    int64_t nnz4;  // nnz rounded up to whole quads of 4 elements
 };
 
 WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz) {
@@ -77,7 +78,7 @@ WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz) {
     L.U1 = take(sizeof(double) * n * m);
     L.V0 = take(sizeof(double) * g * (m + 1));
     L.V1 = take(sizeof(double) * g * (m + 1));
-    L.u32 = take(sizeof(float) * n);
+    L.u32 = take(sizeof(float) * (n + bpca::kWave));  // + a sink slot for masked lanes at [n]
     L.P = take(sizeof(double) * (m + 1) * nb);
     L.red = take(sizeof(double) * (m + 1));
     L.sq = take(sizeof(double) * nb);
@@ -92,7 +93,9 @@ WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz) {
     L.Ub = take(sizeof(double) * m * m);
     L.Sb = take(sizeof(double) * m);
     L.indptr64 = widen ? take(sizeof(int64_t) * (n + 1)) : 0;
-    L.idx16 = take(sizeof(uint16_t) * nnz);
+    L.nnz4 = (nnz + 3) / 4 * 4;
+    L.idx16 = take(sizeof(uint16_t) * L.nnz4);
+    L.data4 = take(sizeof(float) * L.nnz4);
     L.total = off;
     return L;
 }
@@ -568,7 +571,7 @@ int bpca_context_reset_kernel_times(bpca_context *ctx) {
 uint64_t bpca_workspace_bytes(uint64_t num_cells, uint32_t num_genes,
                               uint32_t k, uint32_t subspace_size) {
     (void)k;
-    // Without the CSR's nnz this counts the index copy at 7% density.
+    // Without the CSR's nnz this counts the packed copies at 7% density.
     const int64_t nnz = (int64_t)((double)num_cells * num_genes * 0.07);
     return ws_layout((int64_t)num_cells, (int)num_genes, (int)subspace_size,
                      true, nnz).total;
@@ -656,8 +659,16 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
         hipError_t e = hipMalloc((void **)&flag, sizeof(int));
         int h_flag = 0;
         if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(int), s);
+        // Zero the quad padding past nnz in both packed copies.
+        if (e == hipSuccess && L.nnz4 > nnz)
+            e = hipMemsetAsync((char *)ws + L.idx16 + sizeof(uint16_t) * nnz, 0,
+                               sizeof(uint16_t) * (L.nnz4 - nnz), s);
+        if (e == hipSuccess && L.nnz4 > nnz)
+            e = hipMemsetAsync((char *)ws + L.data4 + sizeof(float) * nnz, 0,
+                               sizeof(float) * (L.nnz4 - nnz), s);
         if (e == hipSuccess)
-            e = bpca::launch_validate_pack(pb->indices, pb->indptr, pb->indptr_width, n, nnz, g,
+            e = bpca::launch_validate_pack(pb->data, pb->indices, pb->indptr, pb->indptr_width, n, nnz, g,
+                                           (float *)((char *)ws + L.data4),
                                            (uint16_t *)((char *)ws + L.idx16), flag, s);
         if (e == hipSuccess) e = hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -686,7 +697,7 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
     }
     char *base = (char *)ws;
     bpca::Prob d{};
-    d.data = pb->data;
+    d.data = (const float *)(base + L.data4);
     d.idx16 = (const uint16_t *)(base + L.idx16);
     d.indptr = widen ? (const int64_t *)(base + L.indptr64) : (const int64_t *)pb->indptr;
     d.PCs = pb->PCs;
