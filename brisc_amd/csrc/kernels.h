@@ -19,6 +19,7 @@
 //   P        float64 (m+1) x nb: reorthogonalisation partials per data block
 //   tpart    float32 nb x 2g: X^T u (or column-statistics) block partials
 //   grp      float64 ngrp x 2g: first-level group sums of tpart
+//   vw, vdot, vsq  float64 g / m / ceil(g/256): V-side work vectors
 // Reductions over cells are keyed to fixed data blocks of kRowBlock rows
 // (and fixed sub-blocks / groups inside them), never to the launch shape.
 #pragma once
@@ -34,6 +35,7 @@ constexpr int kSubRows = 256;     // rows per wave sub-block of X^T u
 constexpr int kGroup = 64;        // data blocks per first-level group
 constexpr int kMaxGenes = 8192;   // LDS bound of the X^T u kernel
 constexpr int kGemmCols = 32;     // output columns per restart-GEMM pass
+constexpr int kMaxSubspace = 1024;  // largest subspace_size (m) accepted
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
@@ -67,6 +69,7 @@ struct Prob {
     double *B, *sd, *rem;
     float *vs;
     double *W, *Jv, *Ub, *Sb;
+    double *vw, *vdot, *vsq;  // V-side work: w (g), dots (m), chunk sums
     // State, advanced by the kernels.
     int32_t phase, j, kc, iter;
     int32_t ubuf, vbuf, converged, nconv;

@@ -580,59 +580,96 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T) {
 
 // ================================================================ V side
 //
-// One workgroup of 16 waves per problem.  w[gene] sums the group partials in
-// group order (FP64); w = w/sd - alpha V[:, j]; dots with the V columns are
-// lane-strided over genes then xor butterfly (wave c%16 owns column c); the
-// correction accumulates over columns ascending; beta = sqrt(block_sum<16>).
+// Four short launches over genes (the V side is g long, so one workgroup per
+// problem would leave the step latency-bound):
+//   k_vside_w    w = (sum of the group partials in group order) / sd
+//                    - alpha V[:, j]                     (FP64, per gene)
+//   k_vside_dot  dot_c = V[:, c] . w, thread-strided over genes then
+//                block_sum<4>, one workgroup per column
+//   k_vside_upd  x = w - sum_c V[:, c] dot_c (c ascending); per-chunk sum
+//                of squares block_sum<4> over its 256 genes
+//   k_vside_fin  beta = sqrt(chunk sums in chunk order); V[:, j+1], vs and
+//                B, or the remainder and rnorm at the end of a sweep.
 
-__global__ __launch_bounds__(1024) void k_vside(Tables T) {
-    extern __shared__ double sh[];
-    __shared__ double tmp[16];
+constexpr int kVChunk = 256;  // genes per V-side workgroup
+
+__global__ __launch_bounds__(kVChunk) void k_vside_w(Tables T) {
+    const int chunks = (T.max_g + kVChunk - 1) / kVChunk;
+    Prob &pr = T.probs[blockIdx.x / chunks];
+    if (pr.phase != PH_LANCZOS) return;
+    const int g = pr.g;
+    const int i = (blockIdx.x % chunks) * kVChunk + threadIdx.x;
+    if (i >= g) return;
+    double acc = 0.0;
+    for (int64_t G = 0; G < pr.ngrp; G++) acc += pr.grp[G * 2 * g + i];
+    const double *Vj = pr.V[pr.vbuf] + (int64_t)pr.j * g;
+    pr.vw[i] = acc / pr.sd[i] - pr.alpha * Vj[i];
+}
+
+__global__ __launch_bounds__(256) void k_vside_dot(Tables T) {
+    __shared__ double tmp[4];
+    const int cols = T.max_m;
+    Prob &pr = T.probs[blockIdx.x / cols];
+    const int c = blockIdx.x % cols;
+    if (pr.phase != PH_LANCZOS) return;
+    const int ncols = pr.j == pr.m - 1 ? pr.m : pr.j + 1;
+    if (c >= ncols) return;
+    const int g = pr.g;
+    const double *Vc = pr.V[pr.vbuf] + (int64_t)c * g;
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < g; i += 256) acc = fma(Vc[i], pr.vw[i], acc);
+    acc = block_sum<4>(acc, tmp);
+    if (threadIdx.x == 0) pr.vdot[c] = acc;
+}
+
+__global__ __launch_bounds__(kVChunk) void k_vside_upd(Tables T) {
+    __shared__ double tmp[kVChunk / kWave];
+    __shared__ double dots[kMaxSubspace];
+    const int chunks = (T.max_g + kVChunk - 1) / kVChunk;
+    Prob &pr = T.probs[blockIdx.x / chunks];
+    if (pr.phase != PH_LANCZOS) return;
+    const int g = pr.g;
+    const int chunk = blockIdx.x % chunks;
+    if (chunk * kVChunk >= g) return;
+    const int ncols = pr.j == pr.m - 1 ? pr.m : pr.j + 1;
+    for (int c = threadIdx.x; c < ncols; c += kVChunk) dots[c] = pr.vdot[c];
+    __syncthreads();
+    const int i = chunk * kVChunk + threadIdx.x;
+    double x = 0.0;
+    if (i < g) {
+        const double *V = pr.V[pr.vbuf];
+        double d = 0.0;
+        for (int c = 0; c < ncols; c++) d = fma(V[(int64_t)c * g + i], dots[c], d);
+        x = pr.vw[i] - d;
+        pr.vw[i] = x;
+    }
+    const double sq = block_sum<kVChunk / kWave>(x * x, tmp);
+    if (threadIdx.x == 0) pr.vsq[chunk] = sq;
+}
+
+__global__ __launch_bounds__(256) void k_vside_fin(Tables T) {
+    __shared__ double sbeta;
     Prob &pr = T.probs[blockIdx.x];
     if (pr.phase != PH_LANCZOS) return;
     const int g = pr.g, m = pr.m, j = pr.j;
-    double *w = sh;           // g
-    double *work = sh + g;    // m
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wv = wave_id();
-    double *V = pr.V[pr.vbuf];
-    const double alpha = pr.alpha;
-    const double *Vj = V + (int64_t)j * g;
-    for (int i = threadIdx.x; i < g; i += 1024) {
-        double acc = 0.0;
-        for (int64_t G = 0; G < pr.ngrp; G++) acc += pr.grp[G * 2 * g + i];
-        w[i] = acc / pr.sd[i] - alpha * Vj[i];
+    if (threadIdx.x == 0) {
+        const int nchunk = (g + kVChunk - 1) / kVChunk;
+        double nrm = 0.0;
+        for (int c = 0; c < nchunk; c++) nrm += pr.vsq[c];
+        sbeta = sqrt(nrm);
     }
     __syncthreads();
-    const bool last = (j == m - 1);
-    const int ncols = last ? m : j + 1;
-    for (int c = wv; c < ncols; c += 16) {
-        const double *Vc = V + (int64_t)c * g;
-        double acc = 0.0;
-        for (int i = lane; i < g; i += kWave) acc = fma(Vc[i], w[i], acc);
-        acc = wave_sum(acc);
-        if (lane == 0) work[c] = acc;
-    }
-    __syncthreads();
-    double nrm = 0.0;
-    for (int i = threadIdx.x; i < g; i += 1024) {
-        double d = 0.0;
-        for (int c = 0; c < ncols; c++) d = fma(V[(int64_t)c * g + i], work[c], d);
-        const double x = w[i] - d;
-        w[i] = x;
-        nrm = fma(x, x, nrm);
-    }
-    nrm = block_sum<16>(nrm, tmp);
-    const double beta = sqrt(nrm);
-    if (!last) {
+    const double beta = sbeta;
+    if (j != m - 1) {
         const double bclip = beta < 1e-8 ? 1e-8 : beta;
         const double inv = 1.0 / bclip;
-        double *Vn = V + (int64_t)(j + 1) * g;
-        for (int i = threadIdx.x; i < g; i += 1024) {
-            const double x = w[i] * inv;
+        double *Vn = pr.V[pr.vbuf] + (int64_t)(j + 1) * g;
+        for (int i = threadIdx.x; i < g; i += 256) {
+            const double x = pr.vw[i] * inv;
             Vn[i] = x;
             pr.vs[i] = (float)(x / pr.sd[i]);
         }
+        __syncthreads();
         if (threadIdx.x == 0) {
             pr.B[(int64_t)j + (int64_t)(j + 1) * m] = bclip;
             pr.j = j + 1;
@@ -640,7 +677,8 @@ __global__ __launch_bounds__(1024) void k_vside(Tables T) {
         }
     } else {
         const double inv = beta > 0.0 ? 1.0 / beta : 0.0;
-        for (int i = threadIdx.x; i < g; i += 1024) pr.rem[i] = w[i] * inv;
+        for (int i = threadIdx.x; i < g; i += 256) pr.rem[i] = pr.vw[i] * inv;
+        __syncthreads();
         if (threadIdx.x == 0) {
             pr.rnorm = beta;
             pr.steps += 1;
@@ -971,53 +1009,110 @@ __global__ __launch_bounds__(256) void k_tall_gemm(Tables T) {
 
 // MFMA variant (m <= 128, kout <= 128): each wave takes 16-row tiles and
 // accumulates out_tile(16 x 16t) += A_tile(16 x 4) coef(4 x 16t) with
-// v_mfma_f64_16x16x4f64 over k-steps of 4, k ascending.  All A fragments of
-// a tile are loaded before the first MFMA (up to 32 loads in flight per
-// wave).  Operand layout (16x16x4 f64, checked against numpy): A[i][k] in
+// v_mfma_f64_16x16x4f64 over k-steps of 4, k ascending.  A fragments arrive
+// in chunks of kChunkSteps k-steps, the next chunk's loads issued before the
+// current chunk's MFMAs (addresses clamped to the last live column, whose
+// coefficient rows past m are zero, so no load needs a branch).  One
+// workgroup of kGemmThreads threads per data block shares the coefficient
+// slab.  Operand layout (16x16x4 f64, checked against numpy): A[i][k] in
 // lane i + 16k, B[k][j] in lane j + 16k, D[i][j] in lane j + 16(i%4),
 // element i/4.
 
 typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int kMfmaMaxSteps = 32;  // m <= 128
 constexpr int kMfmaMaxTiles = 8;   // kout <= 128
+constexpr int kChunkSteps = 4;
+constexpr int kGemmThreads = 1024;
+
+// Row stride of the coefficient slab: the four k-rows read by one B operand
+// load land on different LDS bank halves when the stride is an odd multiple
+// of 16 doubles.
+__host__ __device__ inline int slab_ld(int npad) { return (npad & 31) ? npad : npad + 16; }
+
+// Dynamic LDS of the restart / final products for subspace m (kout <= m).
+size_t gemm_lds_bytes(int m) {
+    return sizeof(double) * (size_t)((m + 3) & ~3) * (size_t)slab_ld((m + 15) & ~15);
+}
 
 namespace {
 
-// coef slab in LDS: cs[l * npad + c], zero-padded to mpad x npad.
+// coef slab in LDS: cs[l * ldc + c], zero-padded to mpad x npad.  The
+// output columns of a tile go in passes of up to kPassTiles 16-column tiles
+// (the A fragments are re-read per pass, from cache), which keeps the
+// accumulators within the register budget of 16 waves per CU.
+constexpr int kPassTiles = 6;
+
 template <typename Store>
 __device__ inline void mfma_row_tiles(const double *__restrict__ A, int64_t n,
-                                      int m, const double *cs, int npad,
-                                      int64_t row0, int64_t row_end, int wave,
-                                      int nwaves, Store store) {
+                                      int m, const double *cs, int ldc,
+                                      int ntiles, int64_t row0, int64_t row_end,
+                                      int wave, int nwaves, Store store) {
     const int lane = threadIdx.x & (kWave - 1);
     const int i = lane & 15, kq = lane >> 4;
-    const int ksteps = (m + 3) >> 2, ntiles = npad >> 4;
+    const int ksteps = (m + 3) >> 2;
+    const int nchunks = (ksteps + kChunkSteps - 1) / kChunkSteps;
     for (int64_t r0 = row0 + (int64_t)wave * 16; r0 < row_end; r0 += (int64_t)nwaves * 16) {
-        const int64_t r = r0 + i;
-        double a[kMfmaMaxSteps];
+        const int64_t r = r0 + i < row_end ? r0 + i : row_end - 1;
+        const double *__restrict__ Ar = A + r;
+        for (int t0 = 0; t0 < ntiles; t0 += kPassTiles) {
+            const int nt = ntiles - t0 < kPassTiles ? ntiles - t0 : kPassTiles;
+            v4d acc[kPassTiles];
 #pragma unroll
-        for (int ks = 0; ks < kMfmaMaxSteps; ks++) {
-            const int l = ks * 4 + kq;
-            a[ks] = (ks < ksteps && l < m && r < row_end) ? A[(int64_t)l * n + r] : 0.0;
-        }
-        v4d acc[kMfmaMaxTiles];
+            for (int t = 0; t < kPassTiles; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+            // Chunk loads (clamped columns) and MFMAs; the two fragment
+            // buffers alternate, the loop advancing two chunks at a time.
+            auto load = [&](int c, double *ab) {
 #pragma unroll
-        for (int t = 0; t < kMfmaMaxTiles; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+                for (int u = 0; u < kChunkSteps; u++) {
+                    const int l = (c * kChunkSteps + u) * 4 + kq;
+                    ab[u] = Ar[(int64_t)(l < m ? l : m - 1) * n];
+                }
+            };
+            // A fragment that no MFMA reads is still consumed (an empty asm
+            // use), so every path leaves no load pending into registers that
+            // the next chunk reuses and the waits stay fixed counts.
+            auto consume = [&](const double *ab) {
 #pragma unroll
-        for (int ks = 0; ks < kMfmaMaxSteps; ks++) {
-            if (ks < ksteps) {
-                const double *brow = cs + (ks * 4 + kq) * npad + i;
+                for (int u = 0; u < kChunkSteps; u++) asm volatile("" ::"v"(ab[u]));
+            };
+            auto mma = [&](int c, const double *ab) {
 #pragma unroll
-                for (int t = 0; t < kMfmaMaxTiles; t++)
-                    if (t < ntiles)
-                        acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a[ks], brow[t * 16], acc[t], 0, 0, 0);
+                for (int u = 0; u < kChunkSteps; u++) {
+                    const int ks = c * kChunkSteps + u;
+                    if (ks < ksteps) {
+                        const double *brow = cs + (ks * 4 + kq) * ldc + t0 * 16 + i;
+#pragma unroll
+                        for (int t = 0; t < kPassTiles; t++)
+                            if (t < nt)
+                                acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(ab[u], brow[t * 16], acc[t], 0, 0, 0);
+                    } else {
+                        asm volatile("" ::"v"(ab[u]));
+                    }
+                }
+            };
+            double ab0[kChunkSteps], ab1[kChunkSteps];
+            load(0, ab0);
+#pragma unroll 1
+            for (int c = 0; c < nchunks; c += 2) {
+                load(c + 1, ab1);
+                __builtin_amdgcn_sched_barrier(0);
+                mma(c, ab0);
+                __builtin_amdgcn_sched_barrier(0);
+                load(c + 2, ab0);
+                __builtin_amdgcn_sched_barrier(0);
+                if (c + 1 < nchunks)
+                    mma(c + 1, ab1);
+                else
+                    consume(ab1);
+                __builtin_amdgcn_sched_barrier(0);
             }
-        }
+            consume(ab0);
 #pragma unroll
-        for (int t = 0; t < kMfmaMaxTiles; t++) {
-            if (t < ntiles) {
+            for (int t = 0; t < kPassTiles; t++) {
+                if (t < nt) {
 #pragma unroll
-                for (int v = 0; v < 4; v++) store(r0 + kq + 4 * v, t * 16 + i, acc[t][v]);
+                    for (int v = 0; v < 4; v++) store(r0 + kq + 4 * v, (t0 + t) * 16 + i, acc[t][v]);
+                }
             }
         }
     }
@@ -1026,38 +1121,39 @@ __device__ inline void mfma_row_tiles(const double *__restrict__ A, int64_t n,
 }  // namespace
 
 template <bool kFinal>
-__global__ __launch_bounds__(256) void k_tall_gemm_mfma(Tables T) {
+__global__ __launch_bounds__(kGemmThreads) void k_tall_gemm_mfma(Tables T) {
     extern __shared__ double cs[];
     const int2 bp = T.blocks[blockIdx.x];
     Prob &pr = T.probs[bp.x];
     if (pr.phase != (kFinal ? PH_FINAL : PH_RESTARTING)) return;
     const int m = pr.m;
     const int kout = kFinal ? pr.k : pr.kc;
-    const int mpad = (m + 3) & ~3, npad = (kout + 15) & ~15;
-    for (int idx = threadIdx.x; idx < mpad * npad; idx += 256) {
+    const int mpad = (m + 3) & ~3, npad = (kout + 15) & ~15, ldc = slab_ld(npad);
+    for (int idx = threadIdx.x; idx < mpad * npad; idx += kGemmThreads) {
         const int l = idx / npad, c = idx % npad;
         double v = 0.0;
         if (l < m && c < kout) {
             v = pr.Ub[l + (int64_t)c * m];
             if (kFinal) v *= pr.Sb[c];
         }
-        cs[idx] = v;
+        cs[l * ldc + c] = v;
     }
     __syncthreads();
     const int64_t n = pr.n;
     const int64_t row0 = (int64_t)bp.y * kRowBlock;
     const int64_t row_end = row0 + kRowBlock < n ? row0 + kRowBlock : n;
     const double *A = pr.U[pr.ubuf];
+    const int nw = kGemmThreads / kWave;
     if (kFinal) {
         float *out = pr.PCs;
         const int k = pr.k;
-        mfma_row_tiles(A, n, m, cs, npad, row0, row_end, wave_id(), 4,
+        mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wave_id(), nw,
                        [&](int64_t row, int col, double v) {
                            if (row < row_end && col < kout) out[row * k + col] = (float)v;
                        });
     } else {
         double *out = pr.U[pr.ubuf ^ 1];
-        mfma_row_tiles(A, n, m, cs, npad, row0, row_end, wave_id(), 4,
+        mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wave_id(), nw,
                        [&](int64_t row, int col, double v) {
                            if (row < row_end && col < kout) out[(int64_t)col * n + row] = v;
                        });
@@ -1065,21 +1161,21 @@ __global__ __launch_bounds__(256) void k_tall_gemm_mfma(Tables T) {
 }
 
 // Test entry: out (n x kout, column-major) = A (n x m) C (m x kout).
-__global__ __launch_bounds__(256) void k_gemm_mfma_test(const double *A,
-                                                        int64_t n, int m,
-                                                        const double *C,
-                                                        int kout,
-                                                        double *out) {
+__global__ __launch_bounds__(kGemmThreads) void k_gemm_mfma_test(const double *A,
+                                                                 int64_t n, int m,
+                                                                 const double *C,
+                                                                 int kout,
+                                                                 double *out) {
     extern __shared__ double cs[];
-    const int mpad = (m + 3) & ~3, npad = (kout + 15) & ~15;
-    for (int idx = threadIdx.x; idx < mpad * npad; idx += 256) {
+    const int mpad = (m + 3) & ~3, npad = (kout + 15) & ~15, ldc = slab_ld(npad);
+    for (int idx = threadIdx.x; idx < mpad * npad; idx += kGemmThreads) {
         const int l = idx / npad, c = idx % npad;
-        cs[idx] = (l < m && c < kout) ? C[l + (int64_t)c * m] : 0.0;
+        cs[l * ldc + c] = (l < m && c < kout) ? C[l + (int64_t)c * m] : 0.0;
     }
     __syncthreads();
     const int64_t row0 = (int64_t)blockIdx.x * kRowBlock;
     const int64_t row_end = row0 + kRowBlock < n ? row0 + kRowBlock : n;
-    mfma_row_tiles(A, n, m, cs, npad, row0, row_end, wave_id(), 4,
+    mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wave_id(), kGemmThreads / kWave,
                    [&](int64_t row, int col, double v) {
                        if (row < row_end && col < kout) out[(int64_t)col * n + row] = v;
                    });
@@ -1088,9 +1184,9 @@ __global__ __launch_bounds__(256) void k_gemm_mfma_test(const double *A,
 hipError_t launch_gemm_mfma_test(const double *A, int64_t n, int m,
                                  const double *C, int kout, double *out,
                                  hipStream_t s) {
-    if (m > 4 * kMfmaMaxSteps || kout > 16 * kMfmaMaxTiles) return hipErrorInvalidValue;
-    const size_t lds = sizeof(double) * (size_t)((m + 3) & ~3) * ((kout + 15) & ~15);
-    k_gemm_mfma_test<<<(unsigned)ceil_div(n, kRowBlock), 256, lds, s>>>(A, n, m, C, kout, out);
+    if (m > 4 * kMfmaMaxSteps || kout > 16 * kMfmaMaxTiles || m < 1 || kout < 1) return hipErrorInvalidValue;
+    const size_t lds = sizeof(double) * (size_t)((m + 3) & ~3) * slab_ld((kout + 15) & ~15);
+    k_gemm_mfma_test<<<(unsigned)ceil_div(n, kRowBlock), kGemmThreads, lds, s>>>(A, n, m, C, kout, out);
     return hipGetLastError();
 }
 
@@ -1140,7 +1236,7 @@ __global__ void k_finish(Tables T, int *done_counter) {
 size_t lds_bytes_needed(int g, int m) {
     size_t need = spmv_t_lds_bytes(g);
     need = std::max(need, spmv_lds_bytes(g));
-    need = std::max(need, sizeof(double) * ((size_t)g + m));         // vside
+
     need = std::max(need, sizeof(double) * (size_t)m * kGemmCols);   // gemm
     need = std::max(need, sizeof(double) * (size_t)m + sizeof(int) * (size_t)m);
     return need;
@@ -1212,17 +1308,21 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     mark(1, 1);
     mark(3, 0);
     k_reduce_groups<<<dim3((unsigned)ceil_div(g, 256), (unsigned)T.num_groups), 256, 0, s>>>(T, 0);
-    k_vside<<<T.num_probs, 1024, sizeof(double) * ((size_t)g + m), s>>>(T);
+    const unsigned vchunks = (unsigned)ceil_div(g, kVChunk);
+    k_vside_w<<<T.num_probs * vchunks, kVChunk, 0, s>>>(T);
+    k_vside_dot<<<T.num_probs * (unsigned)m, 256, 0, s>>>(T);
+    k_vside_upd<<<T.num_probs * vchunks, kVChunk, 0, s>>>(T);
+    k_vside_fin<<<T.num_probs, 256, 0, s>>>(T);
     mark(3, 1);
     mark(4, 0);
     if ((int64_t)svd_lds_bytes(m, true) <= T.max_lds)
         k_svd<true><<<T.num_probs, 1024, svd_lds_bytes(m, true), s>>>(T);
     else
         k_svd<false><<<T.num_probs, 1024, svd_lds_bytes(m, false), s>>>(T);
-    const size_t mfma_lds = sizeof(double) * (size_t)((m + 3) & ~3) * ((m + 15) & ~15);
+    const size_t mfma_lds = gemm_lds_bytes(m);
     const bool mfma = m <= 4 * kMfmaMaxSteps && (int64_t)mfma_lds <= T.max_lds;
     if (mfma)
-        k_tall_gemm_mfma<false><<<nblk, 256, mfma_lds, s>>>(T);
+        k_tall_gemm_mfma<false><<<nblk, kGemmThreads, mfma_lds, s>>>(T);
     else
         k_tall_gemm<false><<<nblk, 256, sizeof(double) * m * kGemmCols, s>>>(T);
     k_restart_v<<<T.num_probs * (unsigned)ceil_div(g, 64), 256, 0, s>>>(T);
@@ -1230,7 +1330,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     mark(4, 1);
     mark(5, 0);
     if (mfma)
-        k_tall_gemm_mfma<true><<<nblk, 256, mfma_lds, s>>>(T);
+        k_tall_gemm_mfma<true><<<nblk, kGemmThreads, mfma_lds, s>>>(T);
     else
         k_tall_gemm<true><<<nblk, 256, sizeof(double) * m * kGemmCols, s>>>(T);
     k_finish<<<T.num_probs, 64, 0, s>>>(T, done_counter);

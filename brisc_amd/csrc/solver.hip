@@ -59,7 +59,7 @@ inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 // Per-problem workspace: one allocation carved into these buffers.
 struct WsLayout {
     size_t U0, U1, V0, V1, u32, P, red, sq, tpart, grp, B, sd, rem, vs, W, Jv,
-        Ub, Sb, indptr64, idx16, data4, total;
+        Ub, Sb, vw, vdot, vsq, indptr64, idx16, data4, total;
     int64_t nnz4;  // nnz rounded up to whole quads of 4 elements
 };
 
@@ -92,6 +92,9 @@ WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz) {
     L.Jv = take(sizeof(double) * (mm * mm + (int64_t)m * m));
     L.Ub = take(sizeof(double) * m * m);
     L.Sb = take(sizeof(double) * m);
+    L.vw = take(sizeof(double) * g);
+    L.vdot = take(sizeof(double) * (m + 1));
+    L.vsq = take(sizeof(double) * bpca::ceil_div(g, 256));
     L.indptr64 = widen ? take(sizeof(int64_t) * (n + 1)) : 0;
     L.nnz4 = (nnz + 3) / 4 * 4;
     L.idx16 = take(sizeof(uint16_t) * L.nnz4);
@@ -148,6 +151,7 @@ int check_shape(int64_t n, int64_t g, int64_t k, int64_t m, float tol,
     if (!(tol > 0.0f)) return fail(BPCA_ERR_INVALID_ARGUMENT, "tolerance must be positive");
     if (maxit < 1) return fail(BPCA_ERR_INVALID_ARGUMENT, "max_iterations must be a positive integer");
     if (g > bpca::kMaxGenes) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_genes above 8192 is not supported by the device path");
+    if (m > bpca::kMaxSubspace) return fail(BPCA_ERR_INVALID_ARGUMENT, "subspace_size above 1024 is not supported by the device path");
     if (bpca::ceil_div(n, bpca::kRowBlock) > INT32_MAX) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells too large");
     return BPCA_OK;
 }
@@ -728,6 +732,9 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
     d.Jv = (double *)(base + L.Jv);
     d.Ub = (double *)(base + L.Ub);
     d.Sb = (double *)(base + L.Sb);
+    d.vw = (double *)(base + L.vw);
+    d.vdot = (double *)(base + L.vdot);
+    d.vsq = (double *)(base + L.vsq);
     d.phase = bpca::PH_SETUP;
     const std::vector<double> v0 = start_vector(pb->seed, g);
     hipError_t e = hipSuccess;
