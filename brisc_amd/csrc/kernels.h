@@ -43,10 +43,11 @@ enum Phase : int32_t {
     PH_IDLE = 0,        // slot unused or finished and collected
     PH_SETUP = 1,       // column statistics pending
     PH_LANCZOS = 2,     // Lanczos step j pending
-    PH_RESTART = 3,     // sweep done: small SVD + convergence test pending
+    PH_RESTART = 3,     // sweep done, waiting for the next macro step's SVD
     PH_RESTARTING = 4,  // U, V restart products pending
     PH_FINAL = 5,       // PCs product pending
-    PH_DONE = 6         // results written
+    PH_DONE = 6,        // results written
+    PH_SVD = 7          // small SVD + convergence test run in this macro step
 };
 
 struct Prob {
@@ -100,8 +101,17 @@ struct LaunchHook {
 
 // Fixed launch sequences (queued on one stream; capturable in a graph).
 hipError_t launch_setup(const Tables &T, hipStream_t s);
+// With a side stream (and its fork/join events), the small SVDs of the
+// problems whose sweep ended in an earlier step run on it concurrently with
+// the other problems' Lanczos launches; without one (side = nullptr) every
+// launch goes to s and a sweep that ends in a step restarts in that step.
+// Each problem sees the same sequence of operations either way, so its
+// results are the same.
 hipError_t launch_macro_step(const Tables &T, int *done_counter,
-                             hipStream_t s, const LaunchHook *hook);
+                             hipStream_t s, const LaunchHook *hook,
+                             hipStream_t side = nullptr,
+                             hipEvent_t fork = nullptr,
+                             hipEvent_t join = nullptr);
 // One launch of X v or X^T u alone (autotuner timing; problems must be in the
 // Lanczos phase).  which = 0: X v, 1: X^T u; group = 2, 4 or 8.
 hipError_t launch_spmv_variant(const Tables &T, int which, int group,

@@ -879,7 +879,7 @@ __global__ __launch_bounds__(1024) void k_svd(Tables T) {
     __shared__ int sh_int[4];
     extern __shared__ double sh[];
     Prob &pr = T.probs[blockIdx.x];
-    if (pr.phase != PH_RESTART) return;
+    if (pr.phase != PH_SVD) return;
     const int m = pr.m, mm = m + (m & 1), k = pr.k;
     double *s = sh;                       // m
     int *perm = (int *)(sh + m);          // m
@@ -941,6 +941,14 @@ __global__ __launch_bounds__(1024) void k_svd(Tables T) {
             pr.B[idx] = v;
         }
     }
+}
+
+// Problems whose sweep ended in an earlier macro step take their SVD in
+// this one (the SVD launch may run concurrently with the Lanczos launches,
+// which never see PH_SVD).
+__global__ void k_stage_svd(Tables T) {
+    Prob &pr = T.probs[blockIdx.x];
+    if (threadIdx.x == 0 && pr.phase == PH_RESTART) pr.phase = PH_SVD;
 }
 
 size_t svd_lds_bytes(int m, bool resident) {
@@ -1277,7 +1285,9 @@ hipError_t launch_setup(const Tables &T, hipStream_t s) {
 }
 
 hipError_t launch_macro_step(const Tables &T, int *done_counter,
-                             hipStream_t s, const LaunchHook *hook) {
+                             hipStream_t s, const LaunchHook *hook,
+                             hipStream_t side, hipEvent_t fork,
+                             hipEvent_t join) {
     if (T.num_probs == 0) return hipSuccess;
     const unsigned nblk = (unsigned)T.num_blocks;
     const int g = T.max_g, m = T.max_m;
@@ -1285,6 +1295,26 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
         if (hook) hook->mark(hook->user, kind, end);
     };
     // Kernel classes follow enum bpca_kernel_class in bpca.h.
+    auto svd = [&](hipStream_t st) {
+        if ((int64_t)svd_lds_bytes(m, true) <= T.max_lds)
+            k_svd<true><<<T.num_probs, 1024, svd_lds_bytes(m, true), st>>>(T);
+        else
+            k_svd<false><<<T.num_probs, 1024, svd_lds_bytes(m, false), st>>>(T);
+    };
+    k_stage_svd<<<T.num_probs, 64, 0, s>>>(T);
+    const bool forked = side && fork && join;
+    if (forked) {
+        hipError_t e = hipEventRecord(fork, s);
+        if (e == hipSuccess) e = hipStreamWaitEvent(side, fork, 0);
+        if (e != hipSuccess) return e;
+        svd(side);
+        e = hipEventRecord(join, side);
+        if (e != hipSuccess) return e;
+    } else {
+        mark(4, 0);
+        svd(s);
+        mark(4, 1);
+    }
     mark(0, 0);
     switch (T.spmv_group) {
         case 2: k_spmv<2><<<nblk, 256, spmv_lds_bytes(g), s>>>(T); break;
@@ -1314,11 +1344,18 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     k_vside_upd<<<T.num_probs * vchunks, kVChunk, 0, s>>>(T);
     k_vside_fin<<<T.num_probs, 256, 0, s>>>(T);
     mark(3, 1);
+    if (forked) {
+        const hipError_t e = hipStreamWaitEvent(s, join, 0);
+        if (e != hipSuccess) return e;
+    } else {
+        // Nothing to overlap with: sweeps that ended in this step restart in
+        // this step (same arithmetic, one macro step sooner).
+        mark(4, 0);
+        k_stage_svd<<<T.num_probs, 64, 0, s>>>(T);
+        svd(s);
+        mark(4, 1);
+    }
     mark(4, 0);
-    if ((int64_t)svd_lds_bytes(m, true) <= T.max_lds)
-        k_svd<true><<<T.num_probs, 1024, svd_lds_bytes(m, true), s>>>(T);
-    else
-        k_svd<false><<<T.num_probs, 1024, svd_lds_bytes(m, false), s>>>(T);
     const size_t mfma_lds = gemm_lds_bytes(m);
     const bool mfma = m <= 4 * kMfmaMaxSteps && (int64_t)mfma_lds <= T.max_lds;
     if (mfma)

@@ -174,6 +174,8 @@ struct DeviceGuard {
 struct bpca_context {
     int device = 0;
     hipStream_t stream = nullptr;       // the engine's own stream
+    hipStream_t side = nullptr;         // small SVDs, forked from `stream`
+    hipEvent_t fork_event = nullptr, join_event = nullptr;
     hipStream_t user_stream = nullptr;  // caller's stream to order against
     hipEvent_t order_event = nullptr;
     uint64_t budget = 0;                // 0 = unlimited
@@ -384,6 +386,7 @@ int engine_launch_steps(bpca_engine *E, int count) {
     hipStream_t s = ctx->stream;
     if (E->T.num_blocks == 0) return BPCA_OK;
     if (ctx->profiling || !ctx->use_graph) {
+        // Timed per kernel class: everything on the engine's stream.
         bpca::LaunchHook hook{prof_mark, ctx};
         for (int i = 0; i < count; i++)
             HIP_TRY(bpca::launch_macro_step(E->T, E->d_done, s, ctx->profiling ? &hook : nullptr));
@@ -391,7 +394,11 @@ int engine_launch_steps(bpca_engine *E, int count) {
     }
     if (!E->exec) {
         HIP_TRY(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-        hipError_t e = bpca::launch_macro_step(E->T, E->d_done, s, nullptr);
+        // Fork the small SVDs only when other problems can overlap them.
+        const bool fork = E->T.num_probs > 1;
+        hipError_t e = bpca::launch_macro_step(E->T, E->d_done, s, nullptr,
+                                               fork ? ctx->side : nullptr,
+                                               ctx->fork_event, ctx->join_event);
         hipGraph_t g = nullptr;
         hipError_t e2 = hipStreamEndCapture(s, &g);
         if (e != hipSuccess || e2 != hipSuccess) {
@@ -482,8 +489,14 @@ int bpca_context_create(int device, uint64_t hbm_budget_bytes,
     ctx->budget = hbm_budget_bytes;
     DeviceGuard dg(device);
     hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->order_event, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->fork_event, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&ctx->join_event, hipEventDisableTiming);
     if (e != hipSuccess) {
+        if (ctx->order_event) (void)hipEventDestroy(ctx->order_event);
+        if (ctx->fork_event) (void)hipEventDestroy(ctx->fork_event);
+        if (ctx->side) (void)hipStreamDestroy(ctx->side);
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         delete ctx;
         return hip_fail(e, "context stream");
@@ -505,7 +518,11 @@ int bpca_context_destroy(bpca_context *ctx) {
         for (auto &b : ctx->free_blocks) (void)hipFree(b.p);
         for (auto &kv : ctx->live_blocks) (void)hipFree(kv.first);
         for (hipEvent_t e : ctx->events) (void)hipEventDestroy(e);
+        (void)hipStreamSynchronize(ctx->side);
         if (ctx->order_event) (void)hipEventDestroy(ctx->order_event);
+        if (ctx->fork_event) (void)hipEventDestroy(ctx->fork_event);
+        if (ctx->join_event) (void)hipEventDestroy(ctx->join_event);
+        (void)hipStreamDestroy(ctx->side);
         (void)hipStreamDestroy(ctx->stream);
     }
     delete ctx;
