@@ -114,6 +114,8 @@ def lib() -> ctypes.CDLL:
         'bpca_synth_row_nnz': (i32, [p, u64, u64, u64, u32, p, p, p, p]),
         'bpca_synth_fill': (i32, [p, u64, u64, u64, u32, p, p, p, p, p, p]),
         'bpca_normalize_log1ppf': (i32, [p, p, p, u64, p]),
+        'bpca_csr_subset_count': (i32, [p, p, i32, p, i32, i64, i64, i64, p, i64, p, p]),
+        'bpca_csr_subset_fill': (i32, [p, p, p, i32, p, i32, p, i64, p, i32, p, i64, p, p]),
         'bpca_test_small_svd': (i32, [p, i32, p, p, p, p, p]),
         'bpca_test_gemm': (i32, [p, i64, i32, i32, p, p, p]),
         'bpca_context_set_kernel_config': (i32, [p, i32, i32]),

@@ -143,6 +143,29 @@ hipError_t launch_gemm_mfma_test(const double *A, int64_t n, int m,
                                  const double *C, int kout, double *out,
                                  hipStream_t s);
 
+// QC x HVG subset of a CSR matrix (subset.hip).  Output row r is input row
+// rows[r] (rows = nullptr: all rows); input column c becomes col_map[c] or is
+// dropped when col_map[c] < 0 (col_map = nullptr: identity).  The count pass
+// validates the rows, row offsets and column indices it reads (*bad) and
+// writes the kept entries per output row; the fill writes them in ascending
+// new column order, by compaction when col_map is increasing over the kept
+// columns (ordered != 0), else by a per-row sort in LDS (rows of at most
+// subset_sort_max() kept entries).
+hipError_t launch_subset_count(const void *indptr, int indptr_width,
+                               const void *indices, int index_width,
+                               int64_t num_rows, int64_t nnz, int64_t num_cols,
+                               const int64_t *rows, int64_t num_out_rows,
+                               const int32_t *col_map, int64_t *row_nnz,
+                               int *bad, hipStream_t s);
+hipError_t launch_subset_fill(const float *data, const void *indptr,
+                              int indptr_width, const void *indices,
+                              int index_width, const int64_t *rows,
+                              int64_t num_out_rows, const int32_t *col_map,
+                              int ordered, const int64_t *out_indptr,
+                              float *out_data, int32_t *out_indices,
+                              hipStream_t s);
+int subset_sort_max();
+
 // Synthetic workload generator and log1pPF (synth.hip).
 hipError_t launch_synth_row_nnz(uint64_t seed, uint64_t row0, int64_t rows,
                                 int g, const double *tgm, const int32_t *type,

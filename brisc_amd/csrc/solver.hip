@@ -1162,4 +1162,66 @@ int bpca_normalize_log1ppf(bpca_context *ctx, const int32_t *d_counts,
     return BPCA_OK;
 }
 
+int bpca_csr_subset_count(bpca_context *ctx, const void *d_indptr,
+                          int indptr_width, const void *d_indices,
+                          int index_width, int64_t num_rows, int64_t nnz,
+                          int64_t num_cols, const int64_t *d_rows,
+                          int64_t num_out_rows, const int32_t *d_col_map,
+                          int64_t *d_row_nnz) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!d_indptr || !d_indices || !d_row_nnz) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL device pointer");
+    if ((indptr_width != 4 && indptr_width != 8) || (index_width != 4 && index_width != 8))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "indptr_width and index_width must be 4 or 8");
+    if (num_rows < 0 || nnz < 0 || num_cols < 0 || num_out_rows < 0)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "negative size");
+    if (!d_rows && num_out_rows != num_rows)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "without rows, num_out_rows must equal num_rows");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    begin_call(ctx);
+    hipStream_t s = ctx->stream;
+    DevMem<int> bad;
+    HIP_TRY(bad.alloc(1));
+    HIP_TRY(hipMemsetAsync(bad.p, 0, sizeof(int), s));
+    HIP_TRY(bpca::launch_subset_count(d_indptr, indptr_width, d_indices, index_width, num_rows, nnz,
+                                      num_cols, d_rows, num_out_rows, d_col_map, d_row_nnz, bad.p, s));
+    int h_bad = 0;
+    HIP_TRY(hipMemcpyAsync(&h_bad, bad.p, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (h_bad)
+        return fail(BPCA_ERR_NONCANONICAL,
+                    "CSR subset: a selected row is out of range, its row offsets are invalid, or a column index is outside [0, num_cols)");
+    return BPCA_OK;
+}
+
+int bpca_csr_subset_fill(bpca_context *ctx, const float *d_data,
+                         const void *d_indptr, int indptr_width,
+                         const void *d_indices, int index_width,
+                         const int64_t *d_rows, int64_t num_out_rows,
+                         const int32_t *d_col_map, int col_map_increasing,
+                         const int64_t *d_out_indptr, int64_t max_row_nnz,
+                         float *d_out_data, int32_t *d_out_indices) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!d_data || !d_indptr || !d_indices || !d_out_indptr ||
+        (num_out_rows > 0 && max_row_nnz > 0 && (!d_out_data || !d_out_indices)))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL device pointer");
+    if ((indptr_width != 4 && indptr_width != 8) || (index_width != 4 && index_width != 8))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "indptr_width and index_width must be 4 or 8");
+    if (num_out_rows < 0 || max_row_nnz < 0) return fail(BPCA_ERR_INVALID_ARGUMENT, "negative size");
+    const bool ordered = col_map_increasing != 0 || d_col_map == nullptr;
+    if (!ordered && max_row_nnz > bpca::subset_sort_max())
+        return fail(BPCA_ERR_INVALID_ARGUMENT,
+                    "CSR subset with a reordering column map: rows may keep at most " +
+                        std::to_string(bpca::subset_sort_max()) + " entries");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    begin_call(ctx);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(bpca::launch_subset_fill(d_data, d_indptr, indptr_width, d_indices, index_width, d_rows,
+                                     num_out_rows, d_col_map, ordered ? 1 : 0, d_out_indptr,
+                                     d_out_data, d_out_indices, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BPCA_OK;
+}
+
 }  // extern "C"

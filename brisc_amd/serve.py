@@ -33,7 +33,7 @@ from .irlba import Context, Engine, irlba_device
 class Request:
     rid: int
     arrival: float            # seconds after the stream starts
-    data: object              # device tensors
+    data: object              # device tensors (None until materialised)
     indices: object
     indptr: object
     num_cells: int
@@ -43,6 +43,18 @@ class Request:
     tolerance: float = 1e-6
     max_iterations: int = 100
     seed: int = 0
+    # Optional producer of (data, indices, indptr) on the device, called when
+    # the request arrives; the loop drops the request's matrix once it is
+    # solved, so a long stream never holds every matrix at once.
+    make: object = None
+
+    def materialise(self):
+        if self.data is None and self.make is not None:
+            self.data, self.indices, self.indptr = self.make()
+
+    def release(self):
+        if self.make is not None:
+            self.data = self.indices = self.indptr = None
 
 
 @dataclasses.dataclass
@@ -72,13 +84,15 @@ def summarize(results, makespan: float) -> dict:
 
 
 def serve(requests, context: Context | None = None, step_quantum: int = 16,
-          max_active: int = 64, clock=time.perf_counter):
+          max_active: int = 64, clock=time.perf_counter, progress=None):
     """Run the continuous-batching loop over ``requests`` (sorted by arrival).
 
-    Returns ``(results, makespan_seconds)``.
+    ``progress(done, total)`` is called as results complete.  Returns
+    ``(results, makespan_seconds)``.
     """
     eng = Engine(context)
     queue = deque(sorted(requests, key=lambda r: r.arrival))
+    total = len(queue)
     waiting = deque()
     results = []
     admitted_at = {}
@@ -87,7 +101,9 @@ def serve(requests, context: Context | None = None, step_quantum: int = 16,
         while queue or waiting or eng.active:
             now = clock() - t0
             while queue and queue[0].arrival <= now:
-                waiting.append(queue.popleft())
+                r = queue.popleft()
+                r.materialise()
+                waiting.append(r)
             while waiting and eng.active < max_active:
                 r = waiting[0]
                 try:
@@ -110,6 +126,9 @@ def serve(requests, context: Context | None = None, step_quantum: int = 16,
                     results.append(Result(tag.rid, tag.arrival,
                                           admitted_at[tag.rid], t, PCs, sv,
                                           st))
+                    tag.release()
+                if done and progress:
+                    progress(len(results), total)
             elif queue:
                 delay = queue[0].arrival - (clock() - t0)
                 if delay > 0:
@@ -121,9 +140,14 @@ def serve(requests, context: Context | None = None, step_quantum: int = 16,
 
 
 def serve_sequential(requests, context: Context | None = None,
-                     clock=time.perf_counter):
-    """One request at a time, FIFO, on the plain single-solve GPU path."""
+                     clock=time.perf_counter, progress=None):
+    """One request at a time, FIFO, on the plain single-solve GPU path.
+
+    A request's matrix is materialised when it is taken up (after its
+    arrival), so requests that queue up hold no device memory meanwhile.
+    """
     queue = deque(sorted(requests, key=lambda r: r.arrival))
+    total = len(queue)
     results = []
     t0 = clock()
     while queue:
@@ -132,6 +156,7 @@ def serve_sequential(requests, context: Context | None = None,
         if r.arrival > now:
             time.sleep(r.arrival - now)
         start = clock() - t0
+        r.materialise()
         P, sv, st = irlba_device(r.data, r.indices, r.indptr, r.num_cells,
                                  r.num_genes, r.k,
                                  subspace_size=r.subspace_size,
@@ -140,4 +165,7 @@ def serve_sequential(requests, context: Context | None = None,
                                  validate=False, context=context)
         results.append(Result(r.rid, r.arrival, start, clock() - t0, P, sv,
                               st))
+        r.release()
+        if progress:
+            progress(len(results), total)
     return results, clock() - t0

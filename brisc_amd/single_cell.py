@@ -21,8 +21,6 @@ from typing import Any
 import numpy as np
 import scipy.sparse as sp
 
-from .irlba import irlba
-
 
 def check_type(variable: Any, variable_name: str, expected_types,
                expected_type_name: str) -> None:
@@ -245,18 +243,50 @@ class SingleCell:
         else:
             gene_indices = [np.flatnonzero(h) if h is not None else None
                             for h in hvg_columns]
-        Xs = []
+        # Operand preparation on the device: each dataset's CSR is uploaded
+        # once and the QC x HVG subset is formed there (device subset of
+        # brisc_amd.subset; the reference takes X[np.ix_(QC, HVG)] on the
+        # host, single_cell.py:11853-11908), then the datasets' rows are
+        # stacked in order.  A gene order that is not increasing (joint PCA)
+        # is sorted per row on the device when rows keep at most 4096 genes,
+        # else that dataset is subset on the host.
+        import torch
+
+        from .irlba import irlba_device
+        from .subset import csr_subset
+        dev = torch.device('cuda', 0)
+        parts = []
         for d, q, genes in zip(datasets, QC_columns, gene_indices):
-            X = sp.csr_array(d._X)
-            if q is not None:
-                X = X[np.flatnonzero(q)]
-            if genes is not None:
-                X = X[:, genes]
-            Xs.append(X)
-        num_cells_per_dataset = np.array([X.shape[0] for X in Xs])
-        X = Xs[0] if len(Xs) == 1 else sp.vstack(Xs, format='csr')
-        del Xs
-        num_cells, num_genes = X.shape
+            X = d._X if isinstance(d._X, (sp.csr_array, sp.csr_matrix)) \
+                else sp.csr_array(d._X)
+            if not X.has_canonical_format:
+                X = sp.csr_array(X, copy=True)
+                X.sum_duplicates()
+                X.sort_indices()
+            if genes is not None and len(genes) > 4096 and \
+                    not np.all(np.diff(genes) > 0):
+                X = sp.csr_array(X[np.flatnonzero(q)] if q is not None else X)[:, genes]
+                X = sp.csr_array(X)
+                X.sort_indices()
+                q = genes = None
+            upload = [torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+                      for a in (X.data.astype(np.float32, copy=False),
+                                X.indices, X.indptr)]
+            parts.append(csr_subset(*upload, X.shape[1], rows=q, genes=genes))
+            del upload
+        num_cells_per_dataset = np.array([p[2].numel() - 1 for p in parts])
+        num_cells = int(num_cells_per_dataset.sum())
+        num_genes = (len(gene_indices[0]) if gene_indices[0] is not None
+                     else datasets[0]._X.shape[1])
+        if len(parts) == 1:
+            data, indices, indptr = parts[0]
+        else:
+            offsets = np.cumsum([0] + [int(p[2][-1].item()) for p in parts])
+            data = torch.cat([p[0] for p in parts])
+            indices = torch.cat([p[1] for p in parts])
+            indptr = torch.cat([parts[0][2][:1]] +
+                               [p[2][1:] + int(o) for p, o in zip(parts, offsets[:-1])])
+        del parts
         if num_cells < num_PCs:
             raise ValueError(f'num_PCs is {num_PCs:,}, but must be ≤ the '
                              f'number of cells ({num_cells:,})')
@@ -270,23 +300,13 @@ class SingleCell:
             raise ValueError('there is only one gene, so principal components '
                              'cannot be calculated')
         subspace_size = min(subspace_size, num_cells, num_genes)
-
-        # The GPU path takes the CSR of X for both products; canonical rows.
-        X = sp.csr_array(X)
-        X.sum_duplicates()
-        X.sort_indices()
-        index_dtype = np.int64 if X.nnz > np.iinfo(np.int32).max else np.int32
-        PCs = np.empty((num_cells, num_PCs), dtype=np.float32)
-        converged = irlba(
-            data_matvec=np.ascontiguousarray(X.data, dtype=np.float32),
-            indices_matvec=X.indices.astype(index_dtype, copy=False),
-            indptr_matvec=X.indptr.astype(index_dtype, copy=False),
-            data_rmatvec=None, indices_rmatvec=None, indptr_rmatvec=None,
-            num_cells=num_cells, num_genes=num_genes, k=num_PCs,
-            subspace_size=subspace_size, tolerance=tolerance,
-            max_iterations=max_iterations, seed=seed,
-            match_parallel=match_parallel, is_csr=True,
-            num_threads=num_threads, chunk_size=chunk_size, PCs=PCs)
+        P, _, stats = irlba_device(data, indices, indptr, num_cells, num_genes,
+                                   num_PCs, subspace_size=subspace_size,
+                                   tolerance=tolerance,
+                                   max_iterations=max_iterations, seed=seed)
+        del data, indices, indptr
+        PCs = P.cpu().numpy()
+        converged = bool(stats['converged'])
         if not converged and verbose:
             print(f'PCA did not converge to a tolerance of {tolerance:.2g} '
                   f'after {max_iterations:,} iterations; consider increasing '

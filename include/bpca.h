@@ -250,6 +250,39 @@ int bpca_normalize_log1ppf(bpca_context *ctx, const int32_t *d_counts,
                            const int64_t *d_indptr, uint64_t num_cells,
                            float *d_out);
 
+/* QC x HVG subset of a device-resident CSR matrix, the operand preparation
+ * of SingleCell.pca() (brisc/single_cell.py:11853-11908, which takes
+ * X[np.ix_(QC rows, HVG columns)] through brisc/cysparse.pyx:926-1029), done
+ * on the device in two calls.  All pointers are device pointers.
+ *   rows      int64[num_out_rows]: output row r is input row rows[r]
+ *             (NULL: all num_rows rows, num_out_rows = num_rows)
+ *   col_map   int32[num_cols]: input column c becomes column col_map[c] of
+ *             the output, or is dropped when col_map[c] < 0 (NULL: identity)
+ * bpca_csr_subset_count writes the kept entries of every output row to
+ * row_nnz (int64[num_out_rows]) after checking the rows, row offsets and
+ * column indices it reads (BPCA_ERR_NONCANONICAL if any is out of range).
+ * The caller forms out_indptr (int64[num_out_rows+1], exclusive prefix sum
+ * of row_nnz) and allocates the outputs; bpca_csr_subset_fill writes the
+ * values and int32 column indices of each output row in ascending new
+ * column order (the input rows' column indices must be strictly
+ * increasing).  col_map_increasing = 1 promises that col_map increases over
+ * the kept columns (one dataset's HVG positions); otherwise each row is
+ * sorted on the device and max_row_nnz (the largest row_nnz) must not
+ * exceed 4096.  Both calls are synchronous on the context's stream. */
+int bpca_csr_subset_count(bpca_context *ctx, const void *d_indptr,
+                          int indptr_width, const void *d_indices,
+                          int index_width, int64_t num_rows, int64_t nnz,
+                          int64_t num_cols, const int64_t *d_rows,
+                          int64_t num_out_rows, const int32_t *d_col_map,
+                          int64_t *d_row_nnz);
+int bpca_csr_subset_fill(bpca_context *ctx, const float *d_data,
+                         const void *d_indptr, int indptr_width,
+                         const void *d_indices, int index_width,
+                         const int64_t *d_rows, int64_t num_out_rows,
+                         const int32_t *d_col_map, int col_map_increasing,
+                         const int64_t *d_out_indptr, int64_t max_row_nnz,
+                         float *d_out_data, int32_t *d_out_indices);
+
 #ifdef __cplusplus
 }
 #endif

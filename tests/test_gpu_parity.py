@@ -338,6 +338,43 @@ def test_engine_admits_while_running_and_matches_solo():
         assert np.array_equal(sv, svs)
 
 
+def test_request_loop_matches_one_at_a_time():
+    """The serving loop (continuous batching over a Poisson stream, each
+    request's CSR materialised on the device at arrival and dropped once
+    solved) returns, per request, bit for bit the one-at-a-time result, with
+    consistent latency accounting."""
+    from brisc_amd import serve
+    stream = synth.request_stream(10, rate=200.0, low=500, high=8000,
+                                  k_range=(5, 15), seed=77)
+    print('case serve', [(round(a, 3), w.num_cells, k) for a, w, k in stream], flush=True)
+
+    def requests():
+        return [serve.Request(rid, a, None, None, None, w.num_cells, w.num_genes, k,
+                              subspace_size=30,
+                              make=(lambda w=w: synth.generate_device(w)))
+                for rid, (a, w, k) in enumerate(stream)]
+
+    reqs = requests()
+    seen = []
+    res_b, span_b = serve.serve(reqs, step_quantum=4, max_active=4,
+                                progress=lambda d, t: seen.append((d, t)))
+    res_s, span_s = serve.serve_sequential(requests())
+    assert len(res_b) == len(res_s) == len(stream)
+    assert seen and seen[-1] == (len(stream), len(stream))
+    assert all(r.data is None for r in reqs)            # released once solved
+    by_id = {r.rid: r for r in res_s}
+    for r in res_b:
+        assert r.finished >= r.admitted >= r.arrival - 1e-9
+        assert r.stats['converged'] == 1
+        o = by_id[r.rid]
+        assert np.array_equal(r.PCs.cpu().numpy().view(np.uint32),
+                              o.PCs.cpu().numpy().view(np.uint32))
+        assert np.array_equal(r.singular_values, o.singular_values)
+    summ = serve.summarize(res_b, span_b)
+    assert summ['requests'] == len(stream) and summ['converged'] == len(stream)
+    assert summ['p50_ms'] <= summ['p99_ms']
+
+
 def test_mfma_tall_gemm_matches_numpy():
     """The MFMA FP64 restart product (16x16x4 tiles) against numpy: checks
     the operand lane layout, k padding and ragged row / column edges."""
@@ -374,3 +411,67 @@ def test_autotuned_load_groups_do_not_change_bits():
             ref = bits
         assert np.array_equal(bits, ref), cfg
     ctx.close()
+
+
+def _random_csr(n, g, density, seed, empty_every=7):
+    import scipy.sparse as sp
+    rng = np.random.default_rng(seed)
+    X = sp.random(n, g, density=density, format='csr', random_state=rng,
+                  dtype=np.float32)
+    X.data[:] = rng.random(X.nnz, dtype=np.float32) + 0.5
+    dense_rows = np.arange(0, n, empty_every)
+    X = X.tolil()
+    for r in dense_rows:
+        X.rows[r] = []
+        X.data[r] = []
+    X = sp.csr_array(X)
+    X.sort_indices()
+    return X
+
+
+@pytest.mark.parametrize('wide', [False, True])
+def test_device_qc_hvg_subset_matches_scipy(wide):
+    """Device QC x HVG subset against SciPy's X[rows][:, genes] (sorted
+    indices), bit for bit: increasing and reordering gene maps, empty rows,
+    rows that keep nothing, int32 and int64 index arrays."""
+    torch = _torch()
+    from brisc_amd.subset import csr_subset
+    X = _random_csr(3000, 700, 0.07, 11)
+    rng = np.random.default_rng(3)
+    qc = rng.random(X.shape[0]) < 0.6
+    hvg = rng.random(X.shape[1]) < 0.3
+    genes_perm = rng.permutation(np.flatnonzero(hvg))     # joint-PCA style order
+    dt = np.int64 if wide else np.int32
+    d = (torch.from_numpy(X.data).cuda(), torch.from_numpy(X.indices.astype(dt)).cuda(),
+         torch.from_numpy(X.indptr.astype(dt)).cuda())
+    for rows, genes in [(qc, hvg), (np.flatnonzero(qc), genes_perm), (None, None),
+                        (qc, np.flatnonzero(hvg)[:3])]:
+        print('case subset', wide, None if rows is None else len(np.flatnonzero(rows) if np.asarray(rows).dtype == bool else rows),
+              None if genes is None else len(np.flatnonzero(genes) if np.asarray(genes).dtype == bool else genes), flush=True)
+        ref = X
+        if rows is not None:
+            ref = ref[np.flatnonzero(rows) if np.asarray(rows).dtype == bool else rows]
+        if genes is not None:
+            ref = ref[:, np.flatnonzero(genes) if np.asarray(genes).dtype == bool else genes]
+        ref = ref.tocsr()
+        ref.sort_indices()
+        od, oi, op = csr_subset(*d, X.shape[1], rows=rows, genes=genes)
+        assert np.array_equal(op.cpu().numpy(), ref.indptr.astype(np.int64))
+        assert np.array_equal(oi.cpu().numpy(), ref.indices.astype(np.int32))
+        assert np.array_equal(od.cpu().numpy().view(np.uint32), ref.data.view(np.uint32))
+
+
+def test_device_subset_rejects_bad_input():
+    torch = _torch()
+    from brisc_amd.subset import csr_subset
+    X = _random_csr(200, 50, 0.2, 5)
+    idx = X.indices.copy()
+    idx[idx.size // 2] = 50                                   # column out of range
+    d = (torch.from_numpy(X.data).cuda(), torch.from_numpy(idx).cuda(),
+         torch.from_numpy(X.indptr).cuda())
+    print('case subset bad input', flush=True)
+    with pytest.raises(ValueError, match='column index'):
+        csr_subset(*d, 50, rows=np.arange(200), genes=np.arange(50))
+    good = (d[0], torch.from_numpy(X.indices).cuda(), d[2])
+    with pytest.raises(ValueError, match='out of range'):
+        csr_subset(*good, 50, rows=np.array([0, 200]))
