@@ -40,7 +40,8 @@ def column_map(num_cols: int, genes, device='cuda'):
 
 def csr_subset(data, indices, indptr, num_cols: int, rows=None, genes=None,
                context=None):
-    """Rows ``rows`` (int64 positions, or a boolean mask; None = all) and
+    """Rows ``rows`` (int64 positions or a boolean mask, as NumPy arrays or
+    device tensors; None = all) and
     columns ``genes`` (see :func:`column_map`; None = all) of a CSR matrix
     held in device tensors (float32 data; int32/int64 indices and indptr).
 
@@ -64,6 +65,15 @@ def csr_subset(data, indices, indptr, num_cols: int, rows=None, genes=None,
             raise ValueError('data, indices and indptr must be contiguous tensors on one device')
     if rows is None:
         d_rows, n_out = None, num_rows
+    elif isinstance(rows, torch.Tensor):
+        # Device-resident row selection: a boolean mask or int64 positions.
+        r = rows.to(dev)
+        if r.dtype == torch.bool:
+            if r.shape != (num_rows,):
+                raise ValueError('a boolean row mask must have one entry per row')
+            r = torch.nonzero(r).flatten()
+        d_rows = r.to(torch.int64).contiguous()
+        n_out = int(d_rows.numel())
     else:
         r = np.asarray(rows)
         if r.dtype == bool:

@@ -444,13 +444,17 @@ def test_device_qc_hvg_subset_matches_scipy(wide):
     dt = np.int64 if wide else np.int32
     d = (torch.from_numpy(X.data).cuda(), torch.from_numpy(X.indices.astype(dt)).cuda(),
          torch.from_numpy(X.indptr.astype(dt)).cuda())
-    for rows, genes in [(qc, hvg), (np.flatnonzero(qc), genes_perm), (None, None),
-                        (qc, np.flatnonzero(hvg)[:3])]:
-        print('case subset', wide, None if rows is None else len(np.flatnonzero(rows) if np.asarray(rows).dtype == bool else rows),
-              None if genes is None else len(np.flatnonzero(genes) if np.asarray(genes).dtype == bool else genes), flush=True)
+    qc_dev = torch.from_numpy(qc).cuda()
+    cases = [(qc, qc, hvg), (np.flatnonzero(qc), qc, genes_perm), (None, None, None),
+             (qc, qc, np.flatnonzero(hvg)[:3]), (qc_dev, qc, hvg),
+             (torch.from_numpy(np.flatnonzero(qc)).cuda(), qc, hvg)]
+    for rows, rows_np, genes in cases:
+        print('case subset', wide, None if rows_np is None else int(rows_np.sum()),
+              None if genes is None else len(np.flatnonzero(genes) if np.asarray(genes).dtype == bool else genes),
+              type(rows).__name__, flush=True)
         ref = X
-        if rows is not None:
-            ref = ref[np.flatnonzero(rows) if np.asarray(rows).dtype == bool else rows]
+        if rows_np is not None:
+            ref = ref[np.flatnonzero(rows_np)]
         if genes is not None:
             ref = ref[:, np.flatnonzero(genes) if np.asarray(genes).dtype == bool else genes]
         ref = ref.tocsr()
