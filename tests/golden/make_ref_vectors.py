@@ -3,6 +3,7 @@
 Run in the development container only (it reads the reference sources):
 
     python tests/golden/make_ref_vectors.py [--reference /root/reference]
+    python tests/golden/make_ref_vectors.py --cells1m   (adds the 1M-cell fixture)
 
 It cythonizes ``brisc/cyutils.pyx``, ``brisc/pca.pyx`` and
 ``brisc/normalize.pyx`` from the reference tree into the git-ignored scratch
@@ -120,12 +121,45 @@ def ref_irlba(pca_mod, data, indices, indptr, n, g, k, m, chunk,
     return PCs, bool(conv), time.perf_counter() - t0
 
 
+def cells1m(pca_mod, norm_mod):
+    """The 1M-cell configuration: the reference's singular values, 64 PC
+    rows and converged flag, written beside the other fixtures."""
+    w = synth.make_workload(**synth.WORKLOADS['cells1m'])
+    indptr, indices, counts = oracle.generate_counts(w)
+    data = ref_normalize(norm_mod, counts, indices, indptr)
+    PCs, conv, t8 = ref_irlba(pca_mod, data, indices, indptr, w.num_cells,
+                              w.num_genes, 50, 100, 1024, num_threads=8)
+    rows = np.random.Generator(np.random.PCG64(1)).choice(
+        w.num_cells, ROW_SAMPLE, replace=False)
+    rows.sort()
+    np.savez_compressed(
+        os.path.join(HERE, 'ref_irlba_cells1m.npz'),
+        singular_values=np.linalg.norm(PCs.astype(np.float64), axis=0),
+        rows=rows, pcs_rows=PCs[rows])
+    path = os.path.join(HERE, 'ref_vectors.json')
+    with open(path) as f:
+        meta = json.load(f)
+    meta['cells1m'] = dict(
+        workload='cells1m', k=50, subspace_size=100, chunk_size=1024,
+        tolerance=1e-6, max_iterations=100, seed=0, converged=conv,
+        nnz=int(len(data)), scale=w.scale,
+        csr_sha256=synth.csr_sha256(data, indices, indptr),
+        seconds_8_threads=round(t8, 3), host_cpus=os.cpu_count())
+    with open(path, 'w') as f:
+        json.dump(meta, f, indent=1)
+    print(json.dumps(meta['cells1m'], indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--reference', default='/root/reference')
+    ap.add_argument('--cells1m', action='store_true',
+                    help='only add the 1M-cell fixture (ref_irlba_cells1m.npz)')
     args = ap.parse_args()
     build_reference(args.reference)
     pca_mod, norm_mod = import_reference()
+    if args.cells1m:
+        return cells1m(pca_mod, norm_mod)
 
     tiny = {}
     meta = {'tiny': [], 'large': None,

@@ -57,3 +57,12 @@ def cluster_projection_deficit(A, B, sv_ref, rel_window):
         proj = np.linalg.norm(Qm.T @ A[:, i])
         out.append(1.0 - proj)
     return np.array(out)
+
+
+def principal_angles(A, B):
+    """Principal angles (radians, ascending) between the column spans of A
+    and B (both n x k): arccos of the singular values of Qa^T Qb."""
+    Qa, _ = np.linalg.qr(np.asarray(A, np.float64))
+    Qb, _ = np.linalg.qr(np.asarray(B, np.float64))
+    c = np.linalg.svd(Qa.T @ Qb, compute_uv=False)
+    return np.sort(np.arccos(np.clip(c, -1.0, 1.0)))

@@ -174,3 +174,34 @@ def test_reference_port_oracle_agree_at_50k(golden, cells50k):
     ws = helpers.well_separated(ref_sv, 1e-2)
     errs = helpers.column_errors_up_to_sign(P[rows][:, ws], large['pcs_rows'][:, ws])
     assert errs.max() < COL_RTOL
+
+
+def test_principal_angles_closed_form():
+    """The subspace-angle helper on bases rotated by known angles."""
+    rng = np.random.default_rng(4)
+    Q, _ = np.linalg.qr(rng.standard_normal((60, 6)))
+    t = np.array([0.0, 1e-6, 1e-3, 0.3, 0.7, 1.2])
+    A = Q[:, :3]
+    B = np.cos(t[3:]) * Q[:, :3] + np.sin(t[3:]) * Q[:, 3:]
+    ang = helpers.principal_angles(A @ rng.standard_normal((3, 3)), B)
+    assert np.allclose(ang, np.sort(t[3:]), atol=1e-12)
+    assert helpers.principal_angles(A, A @ np.diag([2.0, -1.0, 3.0])).max() < 1e-7
+
+
+@pytest.mark.slow
+def test_fp64_oracle_matches_reference_at_one_million_cells():
+    """The FP64 oracle against the reference's own irlba singular values on
+    the 1M-cell workload (same bytes: SHA-256 of the generated CSR)."""
+    import json
+    import os
+    here = os.path.join(os.path.dirname(__file__), 'golden')
+    meta = json.load(open(os.path.join(here, 'ref_vectors.json')))['cells1m']
+    ref = np.load(os.path.join(here, 'ref_irlba_cells1m.npz'))
+    w = synth.make_workload(**synth.WORKLOADS['cells1m'])
+    indptr, indices, counts = oracle.generate_counts(w)
+    data = oracle.log1ppf(counts, indptr)
+    assert synth.csr_sha256(data, indices, indptr) == meta['csr_sha256']
+    out = oracle.fp64_pca(data, indices, indptr, w.num_cells, w.num_genes, 50,
+                          want_pcs=False)
+    err = np.abs(out['singular_values'] - ref['singular_values']) / ref['singular_values']
+    assert err.max() < SV_RTOL
