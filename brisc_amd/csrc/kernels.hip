@@ -277,12 +277,11 @@ __device__ __forceinline__ void form_group(const int *off, int nrows,
 }
 
 // Partial sums continue across a row's segments (`carry` holds the open
-// row's); the rows that end in the group go through one batched xor
-// butterfly.  The butterfly leaves every row's sum in all lanes; lane q
-// takes slot q's.  The rows of a group's slots are consecutive, so lane q's
-// row is the first slot's row plus the finished slots before q.  Every lane
-// stores -- lanes without a finished row into the sink slot -- so the store
-// is not behind a branch and the waits for loads stay fixed counts.
+// row's); the rows that end in the group are reduced together (below).  The
+// rows of a group's slots are consecutive, so slot q's row is the first
+// slot's row plus the finished slots before q.  Every lane stores -- lanes
+// without a finished row into the sink slot -- so the store is not behind a
+// branch and the waits for loads stay fixed counts.
 template <int R>
 __device__ __forceinline__ void spmv_reduce(const SegGroup<R> &G,
                                             const float *lds_v, int lane,
@@ -308,21 +307,34 @@ __device__ __forceinline__ void spmv_reduce(const SegGroup<R> &G,
         acc = G.fin[q] ? 0.0f : acc;
     }
     carry = acc;
+    // Halving butterfly: at offsets 32, 16, ... (log2 R steps) each lane
+    // keeps half of its rows and adds its partner's share of them, so after
+    // those steps lane l holds a partial of row l / (64 / R); the remaining
+    // offsets finish each row within its 64 / R lanes.  Every row is summed
+    // in the xor-butterfly tree 32, 16, 8, 4, 2, 1 (floating addition is
+    // commutative, so the bits are those of a plain per-row butterfly) with
+    // R - 1 + 6 - log2 R exchanges instead of 6 R.
+    constexpr int kLanesPerRow = kWave / R;
+    int off2 = 32;
 #pragma unroll
-    for (int off2 = 32; off2 > 0; off2 >>= 1) {
+    for (int half = R / 2; half >= 1; half >>= 1, off2 >>= 1) {
+        const bool upper = (lane & off2) != 0;
 #pragma unroll
-        for (int q = 0; q < R; q++) done[q] += __shfl_xor(done[q], off2);
+        for (int i = 0; i < half; i++) {
+            const float keep = upper ? done[i + half] : done[i];
+            const float send = upper ? done[i] : done[i + half];
+            done[i] = keep + __shfl_xor(send, off2);
+        }
     }
-    float mine = done[0];
+#pragma unroll
+    for (; off2 > 0; off2 >>= 1) done[0] += __shfl_xor(done[0], off2);
     unsigned finmask = 0;
 #pragma unroll
-    for (int q = 0; q < R; q++) {
-        mine = lane == q ? done[q] : mine;
-        finmask |= (unsigned)G.fin[q] << q;
-    }
-    const bool wr = lane < R && ((finmask >> lane) & 1u);
-    const int row = G.row[0] + __popc(finmask & ((1u << lane) - 1u));
-    *(wr ? u + row : sink) = mine;
+    for (int q = 0; q < R; q++) finmask |= (unsigned)G.fin[q] << q;
+    const int q = lane / kLanesPerRow;
+    const bool wr = (lane % kLanesPerRow) == 0 && ((finmask >> q) & 1u);
+    const int row = G.row[0] + __popc(finmask & ((1u << q) - 1u));
+    *(wr ? u + row : sink) = done[0];
 }
 
 }  // namespace
