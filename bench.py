@@ -273,6 +273,63 @@ def bench_subset(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_knn(args):
+    """Exact k nearest neighbours (the exhaustive form of
+    SingleCell.neighbors) of num_cells cells in 50-dimensional PC space:
+    a seeded Gaussian mixture (100 centres) generated on the device.  The
+    dominant work is the FP32 dot products, 2 * n^2 * d flops per call,
+    priced against the FP32 vector peak; the CPU baseline is the same
+    exhaustive search in NumPy (FP32 sgemm + partial sort) on a sample of
+    queries against all cells, scaled to all queries."""
+    import torch
+    from brisc_amd import Context
+    from brisc_amd.knn import knn_exact
+
+    n, d, k = args.knn_cells, 50, args.knn_k
+    gen = torch.Generator(device='cuda').manual_seed(0)
+    centres = 3.0 * torch.randn((100, d), generator=gen, device='cuda')
+    lab = torch.randint(0, 100, (n,), generator=gen, device='cuda')
+    X = (centres[lab] + 0.5 * torch.randn((n, d), generator=gen, device='cuda')).contiguous()
+    ctx = Context(0)
+    nb, dist = knn_exact(X, k, context=ctx)
+    torch.cuda.synchronize()
+    times = []
+    for _ in range(args.steps):
+        t0 = time.perf_counter()
+        nb, dist = knn_exact(X, k, context=ctx)
+        torch.cuda.synchronize()
+        times.append(time.perf_counter() - t0)
+    sec = sum(times) / len(times)
+    flops = 2.0 * n * n * d
+    line = {
+        'metric': 'exact kNN of every cell (exhaustive neighbors search)',
+        'value': round(sec, 4), 'unit': 's', 'n_gpus': 1, 'steps': args.steps,
+        'higher_is_better': False, 'dtype': 'f32', 'data': 'synthetic',
+        'config': {'workload': f'pcs{n}', 'num_cells': n, 'num_dims': d,
+                   'num_neighbors': k},
+        'roofline': {'bound': 'valu', 'achieved': round(flops / sec / 1e12, 2),
+                     'peak': 157.3, 'unit': 'TFLOP/s',
+                     'frac': round(flops / sec / 1e12 / 157.3, 4),
+                     'traffic': None},
+    }
+    if not args.no_cpu_baseline:
+        Xh = X.cpu().numpy()
+        norms = np.einsum('ij,ij->i', Xh, Xh)
+        sample = min(n, 2048)
+        t0 = time.perf_counter()
+        for q0 in range(0, sample, 256):
+            Q = Xh[q0:q0 + 256]
+            D = norms[q0:q0 + 256, None] + (norms[None, :] - 2.0 * (Q @ Xh.T))
+            D[np.arange(len(Q)), np.arange(q0, q0 + len(Q))] = np.inf
+            np.argpartition(D, k, axis=1)
+        cpu_s = time.perf_counter() - t0
+        line['cpu_baseline'] = {
+            'value': round(cpu_s * n / sample, 2), 'unit': 's (scaled by queries)',
+            'cores': int(os.environ.get('OMP_NUM_THREADS', os.cpu_count())),
+            'kind': 'port', 'sample': f'{sample} queries against all {n} cells, {cpu_s:.2f} s'}
+    print(json.dumps(line), flush=True)
+
+
 def bench_copy(args):
     """A plain device copy of known size (calibrates HBM byte counters and
     gives the achievable copy rate)."""
@@ -300,7 +357,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--mode', choices=('single', 'batched', 'serve', 'copy', 'subset'),
+    ap.add_argument('--mode', choices=('single', 'batched', 'serve', 'copy', 'subset', 'knn'),
                     default='single')
     ap.add_argument('--workload', default='cells10m')
     ap.add_argument('--k', type=int, default=50)
@@ -315,6 +372,8 @@ def main():
     ap.add_argument('--max-cells', type=int, default=2_000_000)
     ap.add_argument('--hbm-budget-gb', type=float, default=150.0)
     ap.add_argument('--copy-gib', type=float, default=4.0)
+    ap.add_argument('--knn-cells', type=int, default=1_000_000)
+    ap.add_argument('--knn-k', type=int, default=20)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (asked for {args.gpus}); the '
@@ -329,6 +388,8 @@ def main():
         return bench_copy(args)
     if args.mode == 'subset':
         return bench_subset(args)
+    if args.mode == 'knn':
+        return bench_knn(args)
 
     import torch
     from brisc_amd import Context, irlba_device, synth

@@ -166,6 +166,18 @@ hipError_t launch_subset_fill(const float *data, const void *indptr,
                               hipStream_t s);
 int subset_sort_max();
 
+// Exact k nearest neighbours among the rows of X (n x d, row-major FP32;
+// knn.hip): squared Euclidean distances ||q||^2 + (||c||^2 - 2 q.c), self
+// excluded, each row ascending (ties by index).  k <= 64; norms from
+// launch_row_norms.
+hipError_t launch_row_norms(const float *X, int64_t n, int d, float *norms,
+                            hipStream_t s);
+hipError_t launch_knn_exact(const float *X, const float *norms, int64_t n,
+                            int d, int k, uint32_t *out_idx, float *out_dist,
+                            hipStream_t s);
+size_t knn_lds_bytes(int d, int kmax);
+int knn_kmax(int k);
+
 // Synthetic workload generator and log1pPF (synth.hip).
 hipError_t launch_synth_row_nnz(uint64_t seed, uint64_t row0, int64_t rows,
                                 int g, const double *tgm, const int32_t *type,

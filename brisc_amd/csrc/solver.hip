@@ -1224,4 +1224,28 @@ int bpca_csr_subset_fill(bpca_context *ctx, const float *d_data,
     return BPCA_OK;
 }
 
+int bpca_knn_exact(bpca_context *ctx, const float *d_X, int64_t num_cells,
+                   int32_t num_dims, int32_t k, uint32_t *d_neighbors,
+                   float *d_distances) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!d_X || !d_neighbors || !d_distances) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL device pointer");
+    if (num_cells < 2) return fail(BPCA_ERR_INVALID_ARGUMENT, "there is only one cell, so nearest neighbors cannot be calculated");
+    if (num_dims < 1 || num_dims > 256) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_dims must be in [1, 256]");
+    if (k < 1 || k > 64 || (int64_t)k >= num_cells)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "k must be >= 1, <= 64 and less than the number of cells");
+    if (num_cells > (int64_t)UINT32_MAX) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells too large");
+    if ((int64_t)bpca::knn_lds_bytes(num_dims, bpca::knn_kmax(k)) > (int64_t)ctx->max_lds)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "num_dims / k need more LDS than the device has");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    begin_call(ctx);
+    hipStream_t s = ctx->stream;
+    DevMem<float> norms;
+    HIP_TRY(norms.alloc(num_cells));
+    HIP_TRY(bpca::launch_row_norms(d_X, num_cells, num_dims, norms.p, s));
+    HIP_TRY(bpca::launch_knn_exact(d_X, norms.p, num_cells, num_dims, k, d_neighbors, d_distances, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return BPCA_OK;
+}
+
 }  // extern "C"

@@ -283,6 +283,20 @@ int bpca_csr_subset_fill(bpca_context *ctx, const float *d_data,
                          const int64_t *d_out_indptr, int64_t max_row_nnz,
                          float *d_out_data, int32_t *d_out_indices);
 
+/* The exact k nearest neighbours of every cell among all cells, from PCs
+ * held on the device: the exhaustive form of SingleCell.neighbors()
+ * (brisc/single_cell.py:12139-12665, whose knn_self in brisc/knn.pyx:13-516
+ * returns the exact neighbours when every cluster is searched).
+ *   d_X          float32 num_cells x num_dims, row-major (num_dims <= 256)
+ *   d_neighbors  uint32 num_cells x k: neighbour indices (rows of X)
+ *   d_distances  float32 num_cells x k: squared Euclidean distances formed
+ *                as ||q||^2 + (||c||^2 - 2 q.c) in FP32, as the reference
+ * A cell is not its own neighbour; each row is in ascending distance, ties
+ * in ascending index.  1 <= k <= min(64, num_cells - 1).  Synchronous. */
+int bpca_knn_exact(bpca_context *ctx, const float *d_X, int64_t num_cells,
+                   int32_t num_dims, int32_t k, uint32_t *d_neighbors,
+                   float *d_distances);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,10 @@ Contents:
   singular values, right singular vectors and PCs (``Z V_k``).
 * ``clipped_stddev`` -- closed form of ``clipped_stddev_csc``
   (``brisc/pca.pyx:51-103``) in FP64.
+* ``knn_fp64`` -- brute-force k nearest neighbours in FP64 (squared
+  Euclidean, self excluded, ties by index): what ``knn_self``
+  (``brisc/knn.pyx:13-516``) returns when every cluster is searched, up to the
+  FP32 rounding of its distances.
 
 Pinning (what fixes each function independently of its own formula):
 golden vectors written by the reference itself (``tests/golden``), a dense
@@ -219,3 +223,38 @@ def compare_up_to_sign(A: np.ndarray, B: np.ndarray) -> np.ndarray:
     sign[sign == 0] = 1
     diff = np.linalg.norm(A * sign - B, axis=0)
     return diff / np.maximum(np.linalg.norm(B, axis=0), 1e-300)
+
+
+def knn_fp64(X: np.ndarray, k: int, block: int = 2048):
+    """Exact k nearest neighbours of every row of ``X`` among the rows of
+    ``X``, in FP64: ``(indices int64 n x k, squared distances float64)``,
+    each row ascending by distance, ties by index, a row never its own
+    neighbour (``brisc/knn.pyx:240-252`` skips the query cell itself).
+
+    Distances are ``sum_j (x_qj - x_cj)^2`` accumulated directly (no norm
+    identity), so they carry no cancellation error; a stable sort on
+    distance keeps equal distances in index order."""
+    X = np.asarray(X, np.float64)
+    n = X.shape[0]
+    if not 1 <= k < n:
+        raise ValueError('k must be in [1, n)')
+    norms = np.einsum('ij,ij->i', X, X)
+    idx = np.empty((n, k), np.int64)
+    dist = np.empty((n, k), np.float64)
+    for q0 in range(0, n, block):
+        q1 = min(n, q0 + block)
+        Q = X[q0:q1]
+        # Candidate ranking from the norm identity, widened to 2k + 8 so
+        # that the exact re-evaluation below decides the order.
+        D = norms[q0:q1, None] + norms[None, :] - 2.0 * (Q @ X.T)
+        D[np.arange(q1 - q0), np.arange(q0, q1)] = np.inf
+        w = min(n - 1, 2 * k + 8)
+        cand = np.argpartition(D, w - 1, axis=1)[:, :w]
+        exact = ((Q[:, None, :] - X[cand]) ** 2).sum(axis=2)
+        self_hit = cand == np.arange(q0, q1)[:, None]
+        exact[self_hit] = np.inf
+        order = np.lexsort((cand, exact), axis=1)[:, :k]
+        rows = np.arange(q1 - q0)[:, None]
+        idx[q0:q1] = cand[rows, order]
+        dist[q0:q1] = exact[rows, order]
+    return idx, dist

@@ -205,3 +205,28 @@ def test_fp64_oracle_matches_reference_at_one_million_cells():
                           want_pcs=False)
     err = np.abs(out['singular_values'] - ref['singular_values']) / ref['singular_values']
     assert err.max() < SV_RTOL
+
+
+@pytest.mark.parametrize('n,d,k', [(2, 1, 1), (40, 3, 5), (300, 6, 39)])
+def test_knn_oracle_matches_brute_force(n, d, k):
+    # Brute force over every pair in pure Python (exact squared distances,
+    # self excluded, ties by index) pins the oracle's candidate widening and
+    # tie order; a small integer lattice gives many exact ties.
+    rng = np.random.default_rng(n + d)
+    X = rng.integers(-3, 4, size=(n, d)).astype(np.float32)
+    idx, dist = oracle.knn_fp64(X, k, block=17)
+    Xl = X.astype(np.float64).tolist()
+    for q in range(n):
+        pairs = sorted((sum((a - b) ** 2 for a, b in zip(Xl[q], Xl[c])), c)
+                       for c in range(n) if c != q)[:k]
+        assert [c for _, c in pairs] == idx[q].tolist()
+        assert [s for s, _ in pairs] == dist[q].tolist()
+
+
+def test_knn_oracle_planted_self_mistake_is_caught():
+    # A restatement that forgot to exclude the query itself would return it
+    # first at distance 0; the brute-force check above rejects that.
+    X = np.random.default_rng(1).standard_normal((50, 4)).astype(np.float32)
+    idx, dist = oracle.knn_fp64(X, 3)
+    assert not (idx == np.arange(50)[:, None]).any()
+    assert (dist > 0).all()
