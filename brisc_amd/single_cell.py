@@ -75,8 +75,12 @@ class SingleCell:
                                                 sp.csr_matrix, sp.csc_matrix)):
             raise TypeError('X must be a CSR or CSC sparse array')
         self._X = X
-        n = X.shape[0] if X is not None else 0
-        g = X.shape[1] if X is not None else 0
+        if X is not None:
+            n, g = X.shape
+        else:  # annotations and obsm only (e.g. PCs for neighbors())
+            n = len(next(iter(_columns(obs).values()), ())) if obs is not None \
+                else 0
+            g = 0
         self._obs = obs if obs is not None else {'cell': np.arange(n)}
         self._var = var if var is not None else {
             'gene': np.array([f'gene{i}' for i in range(g)])}
@@ -328,3 +332,149 @@ class SingleCell:
                                   varm=d._varm, uns=d._uns,
                                   num_threads=d._num_threads))
         return out[0] if not others else tuple(out)
+
+    def neighbors(self, *, QC_column='passed_QC', PC_key: str = 'pca',
+                  neighbors_key: str = 'neighbors',
+                  distances_key: str = 'distances', num_neighbors: int = 20,
+                  num_clusters: int | None = None,
+                  num_clusters_searched: int | None = None,
+                  num_kmeans_iterations: int = 2,
+                  kmeans_tolerance: float = 1e-2, kmeans_barbar: bool = False,
+                  num_init_iterations: int = 5,
+                  oversampling_factor: float = 1,
+                  chunk_size_kmeans: int | None = None,
+                  chunk_size_search: int | None = None, seed: int = 0,
+                  overwrite: bool = False, verbose: bool = False,
+                  num_threads: int | None = None) -> 'SingleCell':
+        """The ``num_neighbors`` nearest neighbours of each cell in PC space,
+        on the GPU.
+
+        Same signature, checks and outputs as
+        ``brisc/single_cell.py:12139-12665``: ``obsm[neighbors_key]`` holds
+        uint32 indices (among QC'd cells) of each cell's nearest cells, not
+        counting the cell itself, ascending by distance;
+        ``obsm[distances_key]`` holds the FP32 squared Euclidean distances;
+        cells failing QC get ``UINT32_MAX`` and NaN.
+
+        The search is exhaustive -- every cell is a candidate for every
+        cell, which the reference does when every k-means cluster is
+        searched (``num_clusters_searched=num_clusters``) -- so the k-means
+        parameters are validated as the reference validates them but do not
+        change the result, which is the exact k nearest neighbours up to
+        FP32 near-ties.  ``num_neighbors`` is at most 64 and the PCs have at
+        most 256 columns.
+        """
+        import torch
+
+        from .knn import MAX_DIMS, MAX_NEIGHBORS, knn_exact
+        check_type(neighbors_key, 'neighbors_key', str, 'a string')
+        check_type(distances_key, 'distances_key', str, 'a string')
+        check_type(overwrite, 'overwrite', bool, 'Boolean')
+        if not overwrite and neighbors_key in self._obsm:
+            raise ValueError(
+                f'neighbors_key {neighbors_key!r} is already a key of obsm; '
+                f'did you already run neighbors()? Set overwrite=True to '
+                f'overwrite.')
+        if QC_column is not None:
+            QC_column = self._get_column('obs', QC_column, 'QC_column',
+                                         QC_column == 'passed_QC', None,
+                                         len(self))
+            if QC_column is not None and not QC_column.any():
+                raise ValueError('no cells pass QC: QC_column is all-False')
+        check_type(PC_key, 'PC_key', str, 'a string')
+        if PC_key not in self._obsm:
+            message = f'PC_key {PC_key!r} is not a key of obsm'
+            if PC_key == 'pca':
+                message += '; did you forget to run PCA() before neighbors()?'
+            raise ValueError(message)
+        PCs = self._obsm[PC_key]
+        if PCs.dtype != np.float32:
+            raise TypeError(f'obsm[{PC_key!r}].dtype is {PCs.dtype!r}, but '
+                            f'must be float32')
+        if not PCs.flags['C_CONTIGUOUS']:
+            raise ValueError(
+                f'obsm[{PC_key!r}] is not C-contiguous; make it C-contiguous '
+                f'with pipe_obsm_key({PC_key!r}, np.ascontiguousarray)')
+        check_type(num_kmeans_iterations, 'num_kmeans_iterations', int,
+                   'a positive integer')
+        check_bounds(num_kmeans_iterations, 'num_kmeans_iterations', 1)
+        check_type(kmeans_tolerance, 'kmeans_tolerance', (int, float),
+                   'a positive number')
+        check_bounds(kmeans_tolerance, 'kmeans_tolerance', 0, left_open=True)
+        check_type(kmeans_barbar, 'kmeans_barbar', bool, 'Boolean')
+        check_type(num_init_iterations, 'num_init_iterations', int,
+                   'a positive integer')
+        check_bounds(num_init_iterations, 'num_init_iterations', 1)
+        check_type(oversampling_factor, 'oversampling_factor', (int, float),
+                   'a positive number')
+        check_bounds(oversampling_factor, 'oversampling_factor', 0,
+                     left_open=True)
+        if not kmeans_barbar:
+            if num_init_iterations != 5:
+                raise ValueError('num_init_iterations can only be specified '
+                                 'when kmeans_barbar=True')
+            if oversampling_factor != 1:
+                raise ValueError('oversampling_factor can only be specified '
+                                 'when kmeans_barbar=True')
+        check_type(seed, 'seed', int, 'an integer')
+        check_type(verbose, 'verbose', bool, 'Boolean')
+        self._process_num_threads(num_threads)
+        if QC_column is not None:
+            PCs = PCs[QC_column]
+        num_cells = len(PCs)
+        if num_cells == 1:
+            raise ValueError('there is only one cell, so nearest neighbors '
+                             'cannot be calculated')
+        check_type(num_neighbors, 'num_neighbors', int, 'a positive integer')
+        if not 1 <= num_neighbors < num_cells:
+            raise ValueError(
+                f'num_neighbors is {num_neighbors:,}, but must be ≥ 1 and '
+                f'less than the number of cells ({num_cells:,})')
+        if num_clusters is not None:
+            check_type(num_clusters, 'num_clusters', int, 'a positive integer')
+            if not 1 <= num_clusters <= num_cells:
+                raise ValueError(
+                    f'num_clusters is {num_clusters:,}, but must be ≥ 1 and '
+                    f'≤ the number of cells ({num_cells:,})')
+        else:
+            num_clusters = max(1, int(np.ceil(min(4 * np.sqrt(num_cells),
+                                                  num_cells / 100))))
+        if num_clusters_searched is not None:
+            check_type(num_clusters_searched, 'num_clusters_searched', int,
+                       'a positive integer')
+            if not 1 <= num_clusters_searched <= num_clusters:
+                raise ValueError(
+                    f'num_clusters_searched is {num_clusters_searched:,}, but '
+                    f'must be ≥ 1 and ≤ num_clusters ({num_clusters:,})')
+        for name, value in (('chunk_size_kmeans', chunk_size_kmeans),
+                            ('chunk_size_search', chunk_size_search)):
+            if value is not None:
+                check_type(value, name, int, 'a positive integer')
+                if not 1 <= value <= num_cells:
+                    raise ValueError(
+                        f'{name} is {value:,}, but must be ≥ 1 and ≤ the '
+                        f'number of cells ({num_cells:,})')
+        if num_neighbors > MAX_NEIGHBORS:
+            raise ValueError(f'num_neighbors is {num_neighbors:,}; at most '
+                             f'{MAX_NEIGHBORS} are supported on the GPU')
+        if PCs.shape[1] > MAX_DIMS:
+            raise ValueError(f'obsm[{PC_key!r}] has {PCs.shape[1]:,} columns; '
+                             f'at most {MAX_DIMS} are supported on the GPU')
+        X = torch.from_numpy(np.ascontiguousarray(PCs)).to('cuda')
+        nb, dist = knn_exact(X, num_neighbors)
+        neighbors = nb.cpu().numpy().view(np.uint32)
+        distances = dist.cpu().numpy()
+        del X, nb, dist
+        if QC_column is not None:
+            full_nb = np.full((len(self), num_neighbors), 4_294_967_295,
+                              dtype=np.uint32)
+            full_nb[QC_column] = neighbors
+            full_d = np.full((len(self), num_neighbors), np.nan,
+                             dtype=np.float32)
+            full_d[QC_column] = distances
+            neighbors, distances = full_nb, full_d
+        return SingleCell(X=self._X, obs=self._obs, var=self._var,
+                          obsm=self._obsm | {neighbors_key: neighbors,
+                                             distances_key: distances},
+                          varm=self._varm, uns=self._uns,
+                          num_threads=self._num_threads)

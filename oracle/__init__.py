@@ -225,9 +225,10 @@ def compare_up_to_sign(A: np.ndarray, B: np.ndarray) -> np.ndarray:
     return diff / np.maximum(np.linalg.norm(B, axis=0), 1e-300)
 
 
-def knn_fp64(X: np.ndarray, k: int, block: int = 2048):
-    """Exact k nearest neighbours of every row of ``X`` among the rows of
-    ``X``, in FP64: ``(indices int64 n x k, squared distances float64)``,
+def knn_fp64(X: np.ndarray, k: int, block: int = 2048, rows=None):
+    """Exact k nearest neighbours of every row of ``X`` (or of the rows
+    ``rows``) among the rows of ``X``, in FP64: ``(indices int64 x k,
+    squared distances float64)``,
     each row ascending by distance, ties by index, a row never its own
     neighbour (``brisc/knn.pyx:240-252`` skips the query cell itself).
 
@@ -239,22 +240,24 @@ def knn_fp64(X: np.ndarray, k: int, block: int = 2048):
     if not 1 <= k < n:
         raise ValueError('k must be in [1, n)')
     norms = np.einsum('ij,ij->i', X, X)
-    idx = np.empty((n, k), np.int64)
-    dist = np.empty((n, k), np.float64)
-    for q0 in range(0, n, block):
-        q1 = min(n, q0 + block)
-        Q = X[q0:q1]
+    qrows = np.arange(n) if rows is None else np.asarray(rows, np.int64)
+    nq = len(qrows)
+    idx = np.empty((nq, k), np.int64)
+    dist = np.empty((nq, k), np.float64)
+    for q0 in range(0, nq, block):
+        q1 = min(nq, q0 + block)
+        qr = qrows[q0:q1]
+        Q = X[qr]
         # Candidate ranking from the norm identity, widened to 2k + 8 so
         # that the exact re-evaluation below decides the order.
-        D = norms[q0:q1, None] + norms[None, :] - 2.0 * (Q @ X.T)
-        D[np.arange(q1 - q0), np.arange(q0, q1)] = np.inf
+        D = norms[qr, None] + norms[None, :] - 2.0 * (Q @ X.T)
+        D[np.arange(q1 - q0), qr] = np.inf
         w = min(n - 1, 2 * k + 8)
         cand = np.argpartition(D, w - 1, axis=1)[:, :w]
         exact = ((Q[:, None, :] - X[cand]) ** 2).sum(axis=2)
-        self_hit = cand == np.arange(q0, q1)[:, None]
-        exact[self_hit] = np.inf
+        exact[cand == qr[:, None]] = np.inf
         order = np.lexsort((cand, exact), axis=1)[:, :k]
-        rows = np.arange(q1 - q0)[:, None]
-        idx[q0:q1] = cand[rows, order]
-        dist[q0:q1] = exact[rows, order]
+        r = np.arange(q1 - q0)[:, None]
+        idx[q0:q1] = cand[r, order]
+        dist[q0:q1] = exact[r, order]
     return idx, dist

@@ -4,9 +4,11 @@ Run in the development container only (it reads the reference sources):
 
     python tests/golden/make_ref_vectors.py [--reference /root/reference]
     python tests/golden/make_ref_vectors.py --cells1m   (adds the 1M-cell fixture)
+    python tests/golden/make_ref_vectors.py --knn       (adds the neighbour fixtures)
 
-It cythonizes ``brisc/cyutils.pyx``, ``brisc/pca.pyx`` and
-``brisc/normalize.pyx`` from the reference tree into the git-ignored scratch
+It cythonizes ``brisc/cyutils.pyx``, ``brisc/pca.pyx``,
+``brisc/normalize.pyx``, ``brisc/kmeans.pyx`` and ``brisc/knn.pyx`` from the
+reference tree into the git-ignored scratch
 directory ``tests/golden/_refbuild`` (never shipped anywhere), compiles them
 with the reference's Linux flags (``setup.py:93-98``) and compiler directives
 (``setup.py:148-151``), imports them under the package name ``brisc``, and
@@ -38,12 +40,14 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, 'tests'))
 
 from brisc_amd import synth  # noqa: E402
 import oracle  # noqa: E402
+import helpers  # noqa: E402
 
 BUILD = os.path.join(HERE, '_refbuild')
-MODULES = ('cyutils', 'pca', 'normalize')
+MODULES = ('cyutils', 'pca', 'normalize', 'kmeans', 'knn')
 
 # Tiny inputs: (name, seed, cells, genes, types, density, k, subspace, chunk)
 TINY = [
@@ -150,16 +154,152 @@ def cells1m(pca_mod, norm_mod):
     print(json.dumps(meta['cells1m'], indent=1))
 
 
+# Neighbour-search inputs: (name, seed, cells, dims, centres, k, full rows)
+KNN = [
+    ('knn_a', 31, 2000, 50, 30, 20, True),
+    ('knn_b', 32, 500, 10, 8, 64, True),
+    ('knn_c', 33, 50000, 50, 60, 20, False),
+]
+KNN_ROW_SAMPLE = 256
+
+
+def import_knn():
+    return (importlib.import_module('brisc.kmeans'),
+            importlib.import_module('brisc.knn'))
+
+
+def ref_neighbors(kmeans_mod, knn_mod, X, k, num_clusters_searched=None,
+                  num_threads=8, seed=0):
+    """The steps of SingleCell.neighbors() (single_cell.py:12452-12650) with
+    its defaults, on PCs ``X``; ``num_clusters_searched='all'`` searches every
+    cluster (the exhaustive search).  Returns neighbours (uint32), squared
+    distances (float32) and the number of clusters."""
+    from threadpoolctl import threadpool_limits
+    n, d = X.shape
+    C = max(1, int(np.ceil(min(4 * np.sqrt(n), n / 100))))
+    labels = np.zeros(n, np.uint32)
+    min_dist = np.zeros(n, np.float32)
+    norms = np.zeros(n, np.float32)
+    cent = np.empty((C, d), np.float32)
+    cent_new = np.empty((C, d), np.float32)
+    counts = np.empty(C, np.uint32)
+    with threadpool_limits(1, user_api='blas'):
+        it = kmeans_mod.kmeans(
+            X=X, cluster_labels=labels, centroids=cent, centroids_new=cent_new,
+            num_cells_per_cluster=counts, min_distances=min_dist,
+            cell_norms=norms, kmeans_barbar=False, num_init_iterations=5,
+            num_kmeans_iterations=2, tolerance=1e-2, oversampling_factor=1,
+            seed=seed, chunk_size=min(4096, n), is_mac=False,
+            num_threads=num_threads)
+    if it & 1:
+        cent = cent_new
+    keep = counts > 0
+    if not keep.all():
+        cent, counts = cent[keep], counts[keep]
+        remap = np.zeros(len(keep), np.uint32)
+        remap[keep] = np.arange(keep.sum(), dtype=np.uint32)
+        labels = remap[labels]
+    C = len(cent)
+    searched = C if num_clusters_searched == 'all' else min(64, C)
+    smallest = np.sort(counts).cumsum(dtype=np.uint64)
+    searched = min(max(searched, int(np.searchsorted(smallest, k + 1)) + 1), C)
+    corder = np.argsort(cent[:, 0]).astype(np.uint32)
+    cent = np.ascontiguousarray(cent[corder])
+    counts = counts[corder]
+    inv = np.empty_like(corder)
+    inv[corder] = np.arange(C, dtype=np.uint32)
+    labels = inv[labels]
+    order = np.argsort(labels).astype(np.uint32)
+    Xs = np.ascontiguousarray(X[order])
+    norms_s = np.ascontiguousarray(norms[order])
+    nb = np.zeros((n, k), np.uint32)
+    dist = np.zeros((n, k), np.float32)
+    with threadpool_limits(1, user_api='blas'):
+        knn_mod.knn_self(
+            X=Xs, sorted_order=order, centroids=cent,
+            num_cells_per_cluster=counts, neighbors=nb, distances=dist,
+            cell_norms=norms_s,
+            centroid_distances=np.zeros((n, searched), np.float32),
+            nearest_clusters=np.zeros((n, searched), np.uint32),
+            num_neighbors=k, num_clusters_searched=searched,
+            chunk_size_kmeans=min(4096, n), chunk_size_search=min(256, n),
+            is_mac=False, num_threads=num_threads)
+    out_nb = np.empty_like(nb)
+    out_nb[order] = nb
+    out_dist = np.empty_like(dist)
+    out_dist[order] = dist
+    return out_nb, out_dist, C
+
+
+def knn_fixtures():
+    """Exhaustive and default (approximate) neighbour searches of the
+    reference on seeded PC-space mixtures: full outputs for the small sets,
+    a row sample, SHA-256 digests and recall figures for the large one."""
+    import hashlib
+    kmeans_mod, knn_mod = import_knn()
+    arrays, meta = {}, []
+    for (name, seed, n, d, centers, k, full) in KNN:
+        X = helpers.knn_points(seed, n, d, centers)
+        t0 = time.perf_counter()
+        nb_all, dist_all, C = ref_neighbors(kmeans_mod, knn_mod, X, k, 'all')
+        t_all = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        nb_def, dist_def, _ = ref_neighbors(kmeans_mod, knn_mod, X, k)
+        t_def = time.perf_counter() - t0
+        nb1, dist1, _ = ref_neighbors(kmeans_mod, knn_mod, X, k, 'all',
+                                      num_threads=1)
+        exact_i, _ = oracle.knn_fp64(X, k)
+        recall_def = float(np.mean([len(set(a) & set(b)) / k for a, b in
+                                    zip(nb_def.tolist(), exact_i.tolist())]))
+        recall_all = float(np.mean([len(set(a) & set(b)) / k for a, b in
+                                    zip(nb_all.tolist(), exact_i.tolist())]))
+        entry = dict(name=name, seed=seed, num_cells=n, num_dims=d,
+                     centers=centers, k=k, num_clusters=C,
+                     x_sha256=hashlib.sha256(X.tobytes()).hexdigest(),
+                     exhaustive_sha256=hashlib.sha256(nb_all.tobytes()).hexdigest(),
+                     thread_invariant=bool(np.array_equal(nb_all, nb1) and
+                                           np.array_equal(dist_all, dist1)),
+                     recall_default_vs_fp64=recall_def,
+                     recall_exhaustive_vs_fp64=recall_all,
+                     seconds_exhaustive_8_threads=round(t_all, 3),
+                     seconds_default_8_threads=round(t_def, 3))
+        if full:
+            arrays[f'{name}_neighbors'] = nb_all.astype(np.uint16)
+            arrays[f'{name}_distances'] = dist_all
+            arrays[f'{name}_default_neighbors'] = nb_def.astype(np.uint16)
+        else:
+            rows = np.random.Generator(np.random.PCG64(2)).choice(
+                n, KNN_ROW_SAMPLE, replace=False)
+            rows.sort()
+            arrays[f'{name}_rows'] = rows
+            arrays[f'{name}_neighbors'] = nb_all[rows]
+            arrays[f'{name}_distances'] = dist_all[rows]
+            arrays[f'{name}_default_neighbors'] = nb_def[rows]
+        meta.append(entry)
+        print(json.dumps(entry), flush=True)
+    np.savez_compressed(os.path.join(HERE, 'ref_knn.npz'), **arrays)
+    path = os.path.join(HERE, 'ref_vectors.json')
+    with open(path) as f:
+        allmeta = json.load(f)
+    allmeta['knn'] = meta
+    with open(path, 'w') as f:
+        json.dump(allmeta, f, indent=1)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--reference', default='/root/reference')
     ap.add_argument('--cells1m', action='store_true',
                     help='only add the 1M-cell fixture (ref_irlba_cells1m.npz)')
+    ap.add_argument('--knn', action='store_true',
+                    help='only add the neighbour-search fixtures (ref_knn.npz)')
     args = ap.parse_args()
     build_reference(args.reference)
     pca_mod, norm_mod = import_reference()
     if args.cells1m:
         return cells1m(pca_mod, norm_mod)
+    if args.knn:
+        return knn_fixtures()
 
     tiny = {}
     meta = {'tiny': [], 'large': None,

@@ -66,3 +66,41 @@ def principal_angles(A, B):
     Qb, _ = np.linalg.qr(np.asarray(B, np.float64))
     c = np.linalg.svd(Qa.T @ Qb, compute_uv=False)
     return np.sort(np.arccos(np.clip(c, -1.0, 1.0)))
+
+
+def knn_points(seed, n, d, centers, spread=0.5):
+    """Seeded FP32 Gaussian mixture in PC space (``centers`` centres, scale
+    3, within-centre spread ``spread``): the inputs of the neighbour-search
+    fixtures, regenerated bit for bit from the seed."""
+    rng = np.random.default_rng(seed)
+    C = rng.standard_normal((centers, d)) * 3.0
+    lab = rng.integers(0, centers, n)
+    return (C[lab] + spread * rng.standard_normal((n, d))).astype(np.float32)
+
+
+def knn_sets_agree(X, nb_a, nb_b, rtol, rows=None):
+    """Whether two neighbour lists of the same points (row i answering query
+    ``rows[i]``, default i) differ only by near-ties: every neighbour in one
+    list and not the other must lie within the slack
+    ``2 rtol (||q||^2 + max ||c||^2)`` of the k-th exact (FP64) distance of
+    both lists.  Returns (ok, fraction of identical rows, fraction of
+    identical sets)."""
+    X64 = np.asarray(X, np.float64)
+    norms = (X64 * X64).sum(1)
+    a = np.asarray(nb_a, np.int64)
+    b = np.asarray(nb_b, np.int64)
+    qrows = np.arange(len(a)) if rows is None else np.asarray(rows, np.int64)
+    slack = 2.0 * rtol * (norms + norms.max())
+    ok = True
+    for i in np.flatnonzero((np.sort(a, 1) != np.sort(b, 1)).any(1)):
+        q = qrows[i]
+        da = ((X64[a[i]] - X64[q]) ** 2).sum(1)
+        db = ((X64[b[i]] - X64[q]) ** 2).sum(1)
+        kth = min(da.max(), db.max())
+        diff = set(a[i].tolist()) ^ set(b[i].tolist())
+        dd = ((X64[sorted(diff)] - X64[q]) ** 2).sum(1)
+        if (np.abs(dd - kth) > slack[q]).any():
+            ok = False
+    same_rows = float((a == b).all(1).mean())
+    same_sets = float((np.sort(a, 1) == np.sort(b, 1)).all(1).mean())
+    return ok, same_rows, same_sets

@@ -134,3 +134,79 @@ def test_exact_knn_rejects_bad_arguments():
         knn_exact(torch.zeros((10, 257), device='cuda'), 3)
     with pytest.raises(ValueError):
         knn_exact(X[:1], 1)
+
+
+def _ref_fixture(name):
+    import hashlib
+    import json
+    import os
+    import helpers
+    here = os.path.join(os.path.dirname(__file__), 'golden')
+    with open(os.path.join(here, 'ref_vectors.json')) as f:
+        meta = {m['name']: m for m in json.load(f)['knn']}[name]
+    X = helpers.knn_points(meta['seed'], meta['num_cells'], meta['num_dims'],
+                           meta['centers'])
+    assert hashlib.sha256(X.tobytes()).hexdigest() == meta['x_sha256']
+    ref = np.load(os.path.join(here, 'ref_knn.npz'))
+    rows = ref[f'{name}_rows'] if f'{name}_rows' in ref.files else \
+        np.arange(len(X))
+    return (meta, X, rows, ref[f'{name}_neighbors'].astype(np.int64),
+            ref[f'{name}_distances'],
+            ref[f'{name}_default_neighbors'].astype(np.int64))
+
+
+@pytest.mark.parametrize('name', ['knn_a', 'knn_b', 'knn_c'])
+def test_exact_knn_matches_reference_exhaustive_search(name):
+    # The reference's knn_self with every cluster searched, run on the same
+    # points (fixtures written by the reference itself).
+    import helpers
+    meta, X, rows, ref_nb, ref_dist, ref_default = _ref_fixture(name)
+    k = meta['k']
+    nb, dist = _knn(X, k)
+    nb, dist = nb[rows], dist[rows]
+    ok, same_rows, same_sets = helpers.knn_sets_agree(X, nb, ref_nb,
+                                                      DIST_RTOL, rows)
+    assert ok
+    X64 = X.astype(np.float64)
+    norms = (X64 * X64).sum(1)
+    same_pair = nb == ref_nb
+    bound = 2 * DIST_RTOL * (norms[rows][:, None] + norms[nb])
+    assert (np.abs(dist - ref_dist)[same_pair] <= bound[same_pair]).all()
+    # The reference's default search (64 nearest of its k-means clusters)
+    # against the exact search: its recall.
+    recall = np.mean([len(set(a) & set(b)) / k for a, b in
+                      zip(ref_default.tolist(), nb.tolist())])
+    print(f'{name}: rows identical to the reference {same_rows:.4f}, same '
+          f'sets {same_sets:.4f}; reference default-search recall against '
+          f'the device exact search {recall:.5f} (reference, against FP64: '
+          f'{meta["recall_default_vs_fp64"]:.5f})')
+    assert same_sets > 0.99
+
+
+def test_single_cell_neighbors_with_qc():
+    from brisc_amd import SingleCell
+    meta, X, rows, ref_nb, _, _ = _ref_fixture('knn_a')
+    n, k = len(X), meta['k']
+    qc = np.random.default_rng(4).random(n) < 0.9
+    sc = SingleCell(obs={'cell': np.arange(n), 'passed_QC': qc},
+                    obsm={'pca': X})
+    out = sc.neighbors(num_neighbors=k)
+    nb, dist = out.obsm['neighbors'], out.obsm['distances']
+    assert nb.dtype == np.uint32 and dist.dtype == np.float32
+    assert nb.shape == (n, k) and dist.shape == (n, k)
+    assert (nb[~qc] == np.uint32(4_294_967_295)).all()
+    assert np.isnan(dist[~qc]).all()
+    sub_nb, sub_dist = _knn(X[qc], k)
+    assert np.array_equal(nb[qc].astype(np.int64), sub_nb)
+    assert np.array_equal(dist[qc], sub_dist)
+    # Without QC every cell is a query and a candidate: the reference's
+    # exhaustive output up to near-ties.
+    out = sc.neighbors(QC_column=None, num_neighbors=k)
+    import helpers
+    ok, _, _ = helpers.knn_sets_agree(X, out.obsm['neighbors'].astype(np.int64),
+                                      ref_nb, DIST_RTOL)
+    assert ok
+    with pytest.raises(ValueError, match='already a key'):
+        out.neighbors()
+    with pytest.raises(ValueError, match='PC_key'):
+        sc.neighbors(PC_key='missing')

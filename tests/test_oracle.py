@@ -5,6 +5,8 @@ formula: the reference's golden vectors (written by the compiled reference
 ``irlba`` / ``normalize_csr``, ``tests/golden``), a dense FP64 SVD on tiny
 inputs, closed forms and exact integer identities.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -230,3 +232,36 @@ def test_knn_oracle_planted_self_mistake_is_caught():
     idx, dist = oracle.knn_fp64(X, 3)
     assert not (idx == np.arange(50)[:, None]).any()
     assert (dist > 0).all()
+
+
+def _knn_fixture(golden, name):
+    import hashlib
+    meta = {m['name']: m for m in golden[0]['knn']}[name]
+    X = helpers.knn_points(meta['seed'], meta['num_cells'], meta['num_dims'],
+                           meta['centers'])
+    assert hashlib.sha256(X.tobytes()).hexdigest() == meta['x_sha256']
+    ref = np.load(os.path.join(os.path.dirname(__file__), 'golden',
+                               'ref_knn.npz'))
+    rows = ref[f'{name}_rows'] if f'{name}_rows' in ref.files else \
+        np.arange(len(X))
+    return (meta, X, rows, ref[f'{name}_neighbors'].astype(np.int64),
+            ref[f'{name}_distances'])
+
+
+@pytest.mark.parametrize('name', ['knn_a', 'knn_b', 'knn_c'])
+def test_knn_oracle_matches_reference_exhaustive_search(golden, name):
+    # The reference's knn_self with every cluster searched (fixtures written
+    # by the reference itself) against the FP64 brute-force oracle: the same
+    # neighbour sets up to near-ties, and the reference's FP32 distances
+    # within the FP32 error of the norm identity of the exact ones.
+    meta, X, rows, nb, dist = _knn_fixture(golden, name)
+    k = meta['k']
+    oi, od = oracle.knn_fp64(X, k, rows=rows)
+    ok, same_rows, same_sets = helpers.knn_sets_agree(X, nb, oi, 2e-5, rows)
+    assert ok
+    assert same_sets > 0.99
+    X64 = X.astype(np.float64)
+    norms = (X64 * X64).sum(1)
+    exact = ((X64[rows][:, None, :] - X64[nb]) ** 2).sum(2)
+    assert (np.abs(dist - exact) <=
+            2e-5 * (norms[rows][:, None] + norms[nb])).all()
