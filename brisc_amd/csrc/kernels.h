@@ -177,6 +177,16 @@ hipError_t launch_knn_exact(const float *X, const float *norms, int64_t n,
                             hipStream_t s);
 size_t knn_lds_bytes(int d, int kmax);
 int knn_kmax(int k);
+// Matrix-core path for num_dims <= kKnnMfmaMaxDims: Xp is scratch of
+// (n + kKnnPadRows) * knn_mfma_stride(d) floats (the padded copy of X) and
+// norms holds n + kKnnPadRows floats (the pad is zeroed here).
+constexpr int kKnnMfmaMaxDims = 128;
+constexpr int kKnnPadRows = 64;
+int knn_mfma_stride(int d);
+size_t knn_mfma_lds_bytes(int d);
+hipError_t launch_knn_mfma(const float *X, float *norms, int64_t n,
+                           int d, int k, float *Xp, uint32_t *out_idx,
+                           float *out_dist, hipStream_t s);
 
 // Synthetic workload generator and log1pPF (synth.hip).
 hipError_t launch_synth_row_nnz(uint64_t seed, uint64_t row0, int64_t rows,

@@ -160,6 +160,350 @@ __global__ __launch_bounds__(256) void k_knn_exact(
     }
 }
 
+// ---------------------------------------------------------------------------
+// Matrix-core path (num_dims <= 128): distances from FP32 MFMA tiles,
+// candidates buffered per query in LDS, lists merged with a bitonic network.
+//
+// The points are first copied to a padded row-major layout (row stride st =
+// round_up(d, 4) + 4 floats, zero padding) so that a 64-cell tile is a run
+// of 16-byte loads.  A workgroup of four waves takes 64 queries (wave w owns
+// rows 16w..16w+15) and streams the cells in tiles of 64 through LDS (double
+// buffered).  Wave w forms its 16 x 64 block of dot products with
+// v_mfma_f32_16x16x4f32 (four 16 x 16 blocks; lane l holds rows
+// 4(l/16)..4(l/16)+3 of column l%16 of each block); within a group of 16
+// dimensions lane group g supplies dimensions 4g..4g+3, one per MFMA, to both
+// operands, so every dimension enters each dot product once.
+//
+// Selection: each query row keeps its 64 best (distance, index) pairs, sorted,
+// one per lane, in registers; a candidate that beats the row's k-th entry is
+// appended to the row's LDS buffer (64 slots).  A row whose buffer would
+// overflow, and every row at the end, is merged: the buffer is bitonically
+// sorted descending across the lanes, the elementwise minimum with the
+// ascending list holds the 64 smallest of both (a bitonic sequence), and a
+// bitonic merge sorts it.  Keys compare by (distance, index), so the result
+// does not depend on arrival order.
+// ---------------------------------------------------------------------------
+namespace {
+
+constexpr int kTile = 64;  // cells per tile; queries per workgroup
+constexpr int kBuf = 64;   // buffered candidates per query row
+typedef float knn_f4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ bool key_lt(float da, uint32_t ia, float db, uint32_t ib) {
+    return da < db || (da == db && ia < ib);
+}
+
+// Compare-exchange with lane ^ stride; keep the smaller key when keep_min.
+__device__ __forceinline__ void cmp_xchg(float &d, uint32_t &i, int stride, bool keep_min) {
+    const float od = __shfl_xor(d, stride);
+    const uint32_t oi = (uint32_t)__shfl_xor((int)i, stride);
+    const bool take = keep_min ? key_lt(od, oi, d, i) : key_lt(d, i, od, oi);
+    d = take ? od : d;
+    i = take ? oi : i;
+}
+
+// a[r] for a wave-uniform r, as a chain of selects.  The empty asm keeps
+// each element an opaque register value: otherwise the selects of loads are
+// folded into one load at a variable index, which puts the array in scratch.
+template <int N, typename T>
+__device__ __forceinline__ T pick(const T (&a)[N], int r) {
+    T v = a[0];
+    asm("" : "+v"(v));
+#pragma unroll
+    for (int j = 1; j < N; j++) {
+        T x = a[j];
+        asm("" : "+v"(x));
+        v = r == j ? x : v;
+    }
+    return v;
+}
+
+template <int N, typename T>
+__device__ __forceinline__ void put(T (&a)[N], int r, T v) {
+#pragma unroll
+    for (int j = 0; j < N; j++) a[j] = r == j ? v : a[j];
+}
+
+// Merge row r (wave-uniform) of this wave: list (Ld, Li lane = entry) with
+// the row's cnt buffered candidates; refresh the row's threshold (entry k-1).
+__device__ __forceinline__ void merge_row(float (&Ld)[16], uint32_t (&Li)[16],
+                                          float (&tau_d)[4], uint32_t (&tau_i)[4],
+                                          int (&cnt)[4], const float *bd,
+                                          const uint32_t *bi, int r, int k,
+                                          int lane) {
+    const int g = r >> 2, i = r & 3;
+    const int c = __shfl(pick(cnt, i), 16 * g);
+    float B = INFINITY;
+    uint32_t BI = 0xffffffffu;
+    if (lane < c) {
+        B = bd[lane];
+        BI = bi[lane];
+    }
+    // Bitonic sort, descending overall.
+#pragma unroll
+    for (int size = 2; size <= kWave; size <<= 1) {
+        const bool asc = (lane & size) == 0;
+#pragma unroll
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            const bool lower = (lane & stride) == 0;
+            cmp_xchg(B, BI, stride, lower != asc);
+        }
+    }
+    float L = pick(Ld, r);
+    uint32_t LI = pick(Li, r);
+    if (key_lt(B, BI, L, LI)) {
+        L = B;
+        LI = BI;
+    }
+#pragma unroll
+    for (int stride = kWave >> 1; stride > 0; stride >>= 1)
+        cmp_xchg(L, LI, stride, (lane & stride) == 0);
+    put(Ld, r, L);
+    put(Li, r, LI);
+    const float td = __shfl(L, k - 1);
+    const uint32_t ti = (uint32_t)__shfl((int)LI, k - 1);
+    if ((lane >> 4) == g) {
+        put(tau_d, i, td);
+        put(tau_i, i, ti);
+        put(cnt, i, 0);
+    }
+}
+
+}  // namespace
+
+// Padded copy: Xp[c * st + j] = X[c * d + j] (j < d), 0 for d <= j < st and
+// for the kTile pad rows c >= n; norms[n .. n + kTile) = 0.
+__global__ __launch_bounds__(256) void k_knn_pack(const float *__restrict__ X,
+                                                  int64_t n, int d, int st,
+                                                  float *__restrict__ Xp,
+                                                  float *__restrict__ norms) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e < kTile) norms[n + e] = 0.0f;
+    if (e >= (n + kTile) * st) return;
+    const int64_t c = e / st;
+    const int j = (int)(e - c * st);
+    Xp[e] = (c < n && j < d) ? X[c * d + j] : 0.0f;
+}
+
+template <int PF>
+__global__ __launch_bounds__(256) void k_knn_mfma(
+    const float *__restrict__ Xp, const float *__restrict__ norms, int64_t n,
+    int st, int ksteps, int k, uint32_t *__restrict__ out_idx,
+    float *__restrict__ out_dist) {
+    extern __shared__ __align__(16) float lds_f[];
+    float *qs = lds_f;                            // 64 x st queries
+    float *cs0 = qs + kTile * st;                 // 2 x (64 x st) cells
+    float *cn0 = cs0 + 2 * kTile * st;            // 2 x 64 cell norms
+    float *bufd = cn0 + 2 * kTile;                // 64 rows x kBuf
+    uint32_t *bufi = (uint32_t *)(bufd + kTile * kBuf);
+    const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid >> 6;
+    const int g = lane >> 4, m = lane & 15;
+    const int64_t q0 = (int64_t)blockIdx.x * kTile;
+    const int tile_v4 = kTile * (st >> 2);    // 16-byte vectors per tile (<= 256 * PF)
+    const int64_t tile_floats = (int64_t)kTile * st;
+
+    // This thread's 16-byte vectors of a tile: the same offset in the padded
+    // rows (global) and in the LDS tile, both of row stride st.
+    int off[PF];
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+        const int e = tid + 256 * u;
+        const int r = e / (st >> 2);
+        off[u] = e < tile_v4 ? r * st + 4 * (e - r * (st >> 2)) : -1;
+    }
+    // Queries and the first tile (pad rows make every load in bounds).
+#pragma unroll
+    for (int u = 0; u < PF; u++) {
+        if (off[u] >= 0) {
+            *(knn_f4 *)(qs + off[u]) = *(const knn_f4 *)(Xp + q0 * st + off[u]);
+            *(knn_f4 *)(cs0 + off[u]) = *(const knn_f4 *)(Xp + off[u]);
+        }
+    }
+    if (tid < kTile) cn0[tid] = norms[tid];
+
+    float qn[4];
+    int64_t qid[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        qid[i] = q0 + 16 * w + 4 * g + i;
+        qn[i] = norms[qid[i] < n ? qid[i] : n];
+    }
+    float Ld[16];
+    uint32_t Li[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        Ld[r] = INFINITY;
+        Li[r] = 0xffffffffu;
+    }
+    float tau_d[4];
+    uint32_t tau_i[4];
+    int cnt[4];
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        tau_d[i] = INFINITY;
+        tau_i[i] = 0xffffffffu;
+        cnt[i] = 0;
+    }
+    const int groups = ksteps >> 2, tail = ksteps & 3;
+    float *const myb_d = bufd + (16 * w) * kBuf;
+    uint32_t *const myb_i = bufi + (16 * w) * kBuf;
+    const uint32_t below = (1u << m) - 1u;
+    __syncthreads();
+
+    int t = 0;
+    for (int64_t c0 = 0; c0 < n; c0 += kTile, t ^= 1) {
+        const float *cs = cs0 + t * (kTile * st);
+        const float *cn = cn0 + t * kTile;
+        const int64_t c1 = c0 + kTile;
+        // Prefetch the next tile (pad rows past n read as zeros).
+        knn_f4 pf[PF];
+        const float *src = Xp + (c1 < n ? c1 : c0) * st;
+#pragma unroll
+        for (int u = 0; u < PF; u++)
+            if (off[u] >= 0) pf[u] = *(const knn_f4 *)(src + off[u]);
+        const float pfn = tid < kTile ? norms[(c1 < n ? c1 : c0) + tid] : 0.0f;
+
+        // 16 x 64 dot products of this wave's queries with the tile.
+        knn_f4 acc[4];
+#pragma unroll
+        for (int b = 0; b < 4; b++) acc[b] = {0.0f, 0.0f, 0.0f, 0.0f};
+        const float *qrow = qs + (16 * w + m) * st;
+        const float *crow = cs + m * st;
+        for (int gg = 0; gg < groups; gg++) {
+            const knn_f4 a = *(const knn_f4 *)(qrow + 16 * gg + 4 * g);
+            knn_f4 bv[4];
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                bv[b] = *(const knn_f4 *)(crow + 16 * b * st + 16 * gg + 4 * g);
+#pragma unroll
+            for (int s = 0; s < 4; s++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], bv[b][s], acc[b], 0, 0, 0);
+        }
+        for (int s = 0; s < tail; s++) {
+            const int dim = 16 * groups + 4 * s + g;
+            const float a = qrow[dim];
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+                acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, crow[16 * b * st + dim], acc[b], 0, 0, 0);
+        }
+
+        // Distances ||q||^2 + (||c||^2 - 2 q.c) (the product by 2 is exact,
+        // so the fma rounds as the subtraction does); a candidate must beat
+        // its row's k-th entry.  Cells arrive in increasing index, so a tie
+        // with that entry loses.  Only the tile holding the queries
+        // themselves and the last tile need the self / range masks.
+        const bool special = c1 > n || c0 == q0;
+        float dd[4][4];
+        uint64_t bal[4][4];
+        uint64_t any = 0;
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
+            const float cnv = cn[16 * b + m];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                float v = qn[i] + __builtin_fmaf(-2.0f, acc[b][i], cnv);
+                if (special) {
+                    const int64_t c = c0 + 16 * b + m;
+                    if (c >= n || c == qid[i]) v = INFINITY;
+                }
+                dd[b][i] = v;
+                bal[b][i] = __ballot(v < tau_d[i]);
+                any |= bal[b][i];
+            }
+        }
+        if (any) {
+            int arr[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int b = 0; b < 4; b++)
+#pragma unroll
+                for (int i = 0; i < 4; i++)
+                    arr[i] += __popcll((bal[b][i] >> (16 * g)) & 0xffffull);
+            // Rows whose buffer would overflow are merged first.
+            uint32_t rmask = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint64_t need = __ballot(cnt[i] + arr[i] > kBuf);
+#pragma unroll
+                for (int gq = 0; gq < 4; gq++)
+                    rmask |= (uint32_t)((need >> (16 * gq)) & 1ull) << (4 * gq + i);
+            }
+            while (rmask) {
+                const int r = __builtin_ctz(rmask);
+                rmask &= rmask - 1;
+                merge_row(Ld, Li, tau_d, tau_i, cnt, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
+            }
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint32_t gb = (uint32_t)((bal[b][i] >> (16 * g)) & 0xffffull);
+                    if ((gb >> m) & 1u) {
+                        const int slot = (4 * g + i) * kBuf + cnt[i] + __popc(gb & below);
+                        myb_d[slot] = dd[b][i];
+                        myb_i[slot] = (uint32_t)(c0 + 16 * b + m);
+                    }
+                    cnt[i] += __popc(gb);
+                }
+            }
+        }
+        // Stage the next tile.
+        float *csn = cs0 + (t ^ 1) * tile_floats;
+#pragma unroll
+        for (int u = 0; u < PF; u++)
+            if (off[u] >= 0) *(knn_f4 *)(csn + off[u]) = pf[u];
+        if (tid < kTile) cn0[(t ^ 1) * kTile + tid] = pfn;
+        __syncthreads();
+    }
+    // Final merges.
+    uint32_t rmask = 0;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+        const uint64_t need = __ballot(cnt[i] > 0);
+#pragma unroll
+        for (int gq = 0; gq < 4; gq++)
+            rmask |= (uint32_t)((need >> (16 * gq)) & 1ull) << (4 * gq + i);
+    }
+    while (rmask) {
+        const int r = __builtin_ctz(rmask);
+        rmask &= rmask - 1;
+        merge_row(Ld, Li, tau_d, tau_i, cnt, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int64_t q = q0 + 16 * w + r;
+        if (q < n && lane < k) {
+            out_idx[q * k + lane] = Li[r];
+            out_dist[q * k + lane] = Ld[r];
+        }
+    }
+}
+
+int knn_mfma_stride(int d) { return ((d + 3) & ~3) + 4; }
+
+size_t knn_mfma_lds_bytes(int d) {
+    const size_t st = (size_t)knn_mfma_stride(d);
+    return sizeof(float) * (3 * kTile * st + 2 * kTile + 2 * (size_t)kTile * kBuf);
+}
+
+hipError_t launch_knn_mfma(const float *X, float *norms, int64_t n,
+                           int d, int k, float *Xp, uint32_t *out_idx,
+                           float *out_dist, hipStream_t s) {
+    if (n <= 0 || d < 1 || d > kKnnMfmaMaxDims || k < 1 || k > kWave)
+        return hipErrorInvalidValue;
+    const int st = knn_mfma_stride(d);
+    k_knn_pack<<<(unsigned)ceil_div((n + kTile) * st, 256), 256, 0, s>>>(X, n, d, st, Xp, norms);
+    const unsigned grid = (unsigned)ceil_div(n, kTile);
+    const size_t lds = knn_mfma_lds_bytes(d);
+    const int ksteps = (d + 3) / 4;
+    if (kTile * (st / 4) <= 256 * 4)
+        k_knn_mfma<4><<<grid, 256, lds, s>>>(Xp, norms, n, st, ksteps, k, out_idx, out_dist);
+    else
+        k_knn_mfma<9><<<grid, 256, lds, s>>>(Xp, norms, n, st, ksteps, k, out_idx, out_dist);
+    return hipGetLastError();
+}
+
 size_t knn_lds_bytes(int d, int kmax) {
     const size_t dp = (size_t)((d + 3) & ~3);
     const size_t tiles = sizeof(float) * (dp * kQueries + dp * kCells + kQueries * (kCells + 1) + kQueries + kCells);

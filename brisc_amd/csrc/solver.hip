@@ -1234,15 +1234,25 @@ int bpca_knn_exact(bpca_context *ctx, const float *d_X, int64_t num_cells,
     if (k < 1 || k > 64 || (int64_t)k >= num_cells)
         return fail(BPCA_ERR_INVALID_ARGUMENT, "k must be >= 1, <= 64 and less than the number of cells");
     if (num_cells > (int64_t)UINT32_MAX) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells too large");
-    if ((int64_t)bpca::knn_lds_bytes(num_dims, bpca::knn_kmax(k)) > (int64_t)ctx->max_lds)
+    const bool mfma = num_dims <= bpca::kKnnMfmaMaxDims;
+    const size_t lds = mfma ? bpca::knn_mfma_lds_bytes(num_dims)
+                            : bpca::knn_lds_bytes(num_dims, bpca::knn_kmax(k));
+    if ((int64_t)lds > (int64_t)ctx->max_lds)
         return fail(BPCA_ERR_INVALID_ARGUMENT, "num_dims / k need more LDS than the device has");
     std::lock_guard<std::recursive_mutex> guard(ctx->lock);
     DeviceGuard dg(ctx->device);
     begin_call(ctx);
     hipStream_t s = ctx->stream;
     DevMem<float> norms;
-    HIP_TRY(norms.alloc(num_cells));
+    HIP_TRY(norms.alloc(num_cells + bpca::kKnnPadRows));
     HIP_TRY(bpca::launch_row_norms(d_X, num_cells, num_dims, norms.p, s));
+    if (mfma) {
+        DevMem<float> padded;
+        HIP_TRY(padded.alloc((num_cells + bpca::kKnnPadRows) * bpca::knn_mfma_stride(num_dims)));
+        HIP_TRY(bpca::launch_knn_mfma(d_X, norms.p, num_cells, num_dims, k, padded.p, d_neighbors, d_distances, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        return BPCA_OK;
+    }
     HIP_TRY(bpca::launch_knn_exact(d_X, norms.p, num_cells, num_dims, k, d_neighbors, d_distances, s));
     HIP_TRY(hipStreamSynchronize(s));
     return BPCA_OK;
