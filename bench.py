@@ -283,24 +283,43 @@ def bench_knn(args):
     queries against all cells, scaled to all queries."""
     import torch
     from brisc_amd import Context
-    from brisc_amd.knn import knn_exact
+    from brisc_amd.knn import knn_exact, knn_ivf
 
     n, d, k = args.knn_cells, 50, args.knn_k
+    ivf = args.knn_search == 'ivf'
+    search = (lambda X, k, context: knn_ivf(X, k, context=context)[:2]) if ivf \
+        else knn_exact
     gen = torch.Generator(device='cuda').manual_seed(0)
     centres = 3.0 * torch.randn((100, d), generator=gen, device='cuda')
     lab = torch.randint(0, 100, (n,), generator=gen, device='cuda')
     X = (centres[lab] + 0.5 * torch.randn((n, d), generator=gen, device='cuda')).contiguous()
     ctx = Context(0)
-    nb, dist = knn_exact(X, k, context=ctx)
+    nb, dist = search(X, k, context=ctx)
     torch.cuda.synchronize()
     times = []
     for _ in range(args.steps):
         t0 = time.perf_counter()
-        nb, dist = knn_exact(X, k, context=ctx)
+        nb, dist = search(X, k, context=ctx)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
     sec = sum(times) / len(times)
     flops = 2.0 * n * n * d
+    if ivf:
+        _, _, info = knn_ivf(X, k, context=ctx)
+        ex, _ = knn_exact(X, k, context=ctx)
+        sample = torch.arange(0, n, max(1, n // 20000), device='cuda')
+        a, b = nb[sample].cpu().numpy(), ex[sample].cpu().numpy()
+        recall = float(np.mean([len(set(x) & set(y)) / k for x, y in zip(a.tolist(), b.tolist())]))
+        line = {
+            'metric': 'k nearest neighbours of every cell (neighbors() default inverted-file search)',
+            'value': round(sec, 4), 'unit': 's', 'n_gpus': 1, 'steps': args.steps,
+            'higher_is_better': False, 'dtype': 'f32', 'data': 'synthetic',
+            'config': {'workload': f'pcs{n}', 'num_cells': n, 'num_dims': d,
+                       'num_neighbors': k, **info},
+            'recall_vs_exact': round(recall, 5),
+        }
+        print(json.dumps(line), flush=True)
+        return
     line = {
         'metric': 'exact kNN of every cell (exhaustive neighbors search)',
         'value': round(sec, 4), 'unit': 's', 'n_gpus': 1, 'steps': args.steps,
@@ -374,6 +393,7 @@ def main():
     ap.add_argument('--copy-gib', type=float, default=4.0)
     ap.add_argument('--knn-cells', type=int, default=1_000_000)
     ap.add_argument('--knn-k', type=int, default=20)
+    ap.add_argument('--knn-search', choices=('exact', 'ivf'), default='exact')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (asked for {args.gpus}); the '

@@ -117,6 +117,11 @@ def lib() -> ctypes.CDLL:
         'bpca_csr_subset_count': (i32, [p, p, i32, p, i32, i64, i64, i64, p, i64, p, p]),
         'bpca_csr_subset_fill': (i32, [p, p, p, i32, p, i32, p, i64, p, i32, p, i64, p, p]),
         'bpca_knn_exact': (i32, [p, p, i64, i32, i32, p, p]),
+        'bpca_knn_ivf': (i32, [p, p, i64, i32, i32, i32, i32, i32,
+                               ctypes.c_float, i64, ctypes.c_uint64, p, p, p]),
+        'bpca_kmeans': (i32, [p, p, i64, i32, i32, i32, ctypes.c_float, i64,
+                              ctypes.c_uint64, p, p, p, p]),
+        'bpca_kmeans_init_cells': (i32, [i64, i32, ctypes.c_uint64, p]),
         'bpca_test_small_svd': (i32, [p, i32, p, p, p, p, p]),
         'bpca_test_gemm': (i32, [p, i64, i32, i32, p, p, p]),
         'bpca_context_set_kernel_config': (i32, [p, i32, i32]),

@@ -1,0 +1,607 @@
+// Inverted-file k nearest neighbours: SingleCell.neighbors() with its
+// default search, on device-resident PCs.
+//
+// Reference: brisc/single_cell.py:12452-12650 drives brisc/kmeans.pyx
+// (kmeans: random initialisation 13-39, Lloyd iterations 889-1100, empty
+// cluster relocation 747-820) and brisc/knn.pyx:13-516 (knn_self).  The
+// steps, each on the device unless noted:
+//   1. k-means: centroids = num_clusters distinct random cells, drawn on the
+//      host with the reference's PCG32 / Lemire generator (cyutils.pyx
+//      1153-1232) so the same cells are drawn; then per iteration an E-step
+//      (each cell's nearest centroid by the partial distance ||c||^2 - 2 x.c,
+//      ties to the lower index: the tile engine with k = 1) and an M-step
+//      whose sums follow the reference's order exactly (per chunk of
+//      chunk_size cells, cells in order, then the chunk partials in order),
+//      the relocation of empty clusters to the farthest cells (host, rare)
+//      and the division by the count, 1 / count rounded from double; after
+//      the last iteration a final E-step gives the labels and counts.
+//   2. Empty clusters dropped; clusters renumbered by the first coordinate of
+//      their centroid (host); cells sorted by cluster (stable radix sort);
+//      cluster row ranges from the counts.
+//   3. Each cell's num_clusters_searched nearest centroids (partial
+//      distances; the tile engine with k = num_clusters_searched).
+//   4. Per 64 consecutive sorted cells, the union of their clusters with a
+//      64-bit mask of the cells that search each (LDS bitonic sort).
+//   5. The search: the tile engine in inverted-file mode over the sorted
+//      cells, a cell never its own neighbour, ids = original indices.
+// The E-step's dot products use the FP32 matrix cores, not the reference's
+// sgemm blocking, so a cell equidistant (to FP32 rounding) from two
+// centroids may be assigned differently; everything else is exact integer
+// work or follows the reference's FP32 operation order.
+#include <rocprim/device/device_radix_sort.hpp>
+
+#include <algorithm>
+#include <cmath>
+#include <numeric>
+#include <vector>
+
+#include "kernels.h"
+
+namespace bpca {
+
+namespace {
+
+constexpr int kIvfQueries = 64;           // queries per search workgroup
+constexpr int kIvfMaxKeys = 64 * 64;      // (cluster, query) pairs per workgroup
+
+__global__ __launch_bounds__(256) void k_iota(uint32_t *__restrict__ a, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) a[i] = (uint32_t)i;
+}
+
+__global__ __launch_bounds__(256) void k_gather_rows(const float *__restrict__ X,
+                                                     int d,
+                                                     const uint32_t *__restrict__ rows,
+                                                     int64_t num, float *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= num * d) return;
+    const int64_t r = e / d;
+    out[e] = X[(int64_t)rows[r] * d + (e - r * d)];
+}
+
+__global__ __launch_bounds__(256) void k_count_labels(const uint32_t *__restrict__ labels,
+                                                      int64_t n, uint32_t *__restrict__ counts) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) atomicAdd(&counts[labels[i]], 1u);
+}
+
+__global__ __launch_bounds__(256) void k_remap_labels(uint32_t *__restrict__ labels,
+                                                      int64_t n,
+                                                      const uint32_t *__restrict__ map) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) labels[i] = map[labels[i]];
+}
+
+// Cluster sums in the reference's order (kmeans.pyx:1040-1062): one wave per
+// cluster, a lane per dimension; the cluster's cells in ascending index
+// (order[off[j] .. off[j + 1]) from the stable sort), a partial sum per chunk
+// of `chunk` cells started from zero, the partials added in chunk order.
+__global__ __launch_bounds__(256) void k_cluster_sums(const float *__restrict__ X,
+                                                      int d,
+                                                      const uint32_t *__restrict__ order,
+                                                      const int64_t *__restrict__ off,
+                                                      int num_clusters, int64_t chunk,
+                                                      float *__restrict__ sums) {
+    const int j = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (j >= num_clusters) return;
+    const int lane = threadIdx.x & 63;
+    const int64_t b = off[j], e = off[j + 1];
+    for (int d0 = 0; d0 < d; d0 += 64) {
+        const int dim = d0 + lane;
+        const bool on = dim < d;
+        float total = 0.0f, part = 0.0f;
+        int64_t cur = -1;
+        for (int64_t p = b; p < e; p++) {
+            const int64_t cell = order[p];
+            const int64_t ch = cell / chunk;
+            if (ch != cur) {
+                if (cur >= 0) total += part;
+                part = 0.0f;
+                cur = ch;
+            }
+            if (on) part += X[cell * d + dim];
+        }
+        if (cur >= 0) total += part;
+        if (on) sums[(int64_t)j * d + dim] = total;
+    }
+}
+
+// centroid = sum * (float)(1.0 / count) for non-empty clusters
+// (kmeans.pyx:1068-1073: a double quotient stored in a float).
+__global__ __launch_bounds__(256) void k_cluster_normalize(float *__restrict__ sums,
+                                                           int d, int num_clusters,
+                                                           const uint32_t *__restrict__ counts) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= (int64_t)num_clusters * d) return;
+    const uint32_t c = counts[e / d];
+    if (c > 0) sums[e] *= (float)(1.0 / (double)c);
+}
+
+// Squared distance of each cell to its assigned centroid, dimensions in
+// order (kmeans.pyx:779-787), for the relocation of empty clusters.
+__global__ __launch_bounds__(256) void k_dist_to_centroid(const float *__restrict__ X,
+                                                          int64_t n, int d,
+                                                          const uint32_t *__restrict__ labels,
+                                                          const float *__restrict__ cent,
+                                                          float *__restrict__ out) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float *x = X + i * d;
+    const float *c = cent + (int64_t)labels[i] * d;
+    float diff = x[0] - c[0];
+    float dist = diff * diff;
+    for (int j = 1; j < d; j++) {
+        diff = x[j] - c[j];
+        dist = __builtin_fmaf(diff, diff, dist);
+    }
+    out[i] = dist;
+}
+
+// Per workgroup of 64 consecutive sorted cells: the distinct clusters they
+// search, with the mask of the cells searching each.  Keys (cluster << 6 |
+// cell - 64 b) are bitonically sorted in LDS; a run of equal clusters is one
+// entry.  WRITE = false counts the entries (counts[b]); WRITE = true writes
+// them from wg_off[b] on, in ascending cluster order.
+template <bool WRITE>
+__global__ __launch_bounds__(256) void k_ivf_entries(const uint32_t *__restrict__ near,
+                                                     int64_t n, int searched,
+                                                     int64_t *__restrict__ counts,
+                                                     const int64_t *__restrict__ wg_off,
+                                                     uint32_t *__restrict__ ent_cl,
+                                                     uint64_t *__restrict__ ent_mask) {
+    __shared__ uint32_t key[kIvfMaxKeys];
+    __shared__ int part[256];
+    const int tid = threadIdx.x;
+    const int64_t q0 = (int64_t)blockIdx.x * kIvfQueries;
+    const int nq = (int)(n - q0 < kIvfQueries ? n - q0 : kIvfQueries);
+    const int total = kIvfQueries * searched;
+    int pow2 = 1;
+    while (pow2 < total) pow2 <<= 1;
+    for (int i = tid; i < pow2; i += 256) {
+        const int q = i / searched;
+        key[i] = (i < total && q < nq)
+                     ? (near[(q0 + q) * searched + (i - q * searched)] << 6) | (uint32_t)q
+                     : 0xffffffffu;
+    }
+    __syncthreads();
+    for (int size = 2; size <= pow2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = tid; i < pow2; i += 256) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const uint32_t a = key[i], b = key[j];
+                    if ((a > b) == ((i & size) == 0)) {
+                        key[i] = b;
+                        key[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    // Run starts; thread t owns positions [t * per, (t + 1) * per).
+    const int per = (pow2 + 255) / 256;
+    int mine = 0;
+    for (int i = tid * per; i < (tid + 1) * per && i < pow2; i++) {
+        const uint32_t k = key[i];
+        if (k != 0xffffffffu && (i == 0 || (key[i - 1] >> 6) != (k >> 6))) mine++;
+    }
+    part[tid] = mine;
+    __syncthreads();
+    if (!WRITE) {
+        if (tid == 0) {
+            int64_t s = 0;
+            for (int t = 0; t < 256; t++) s += part[t];
+            counts[blockIdx.x] = s;
+        }
+        return;
+    }
+    int before = 0;
+    for (int t = 0; t < tid; t++) before += part[t];
+    int64_t out = wg_off[blockIdx.x] + before;
+    for (int i = tid * per; i < (tid + 1) * per && i < pow2; i++) {
+        const uint32_t k = key[i];
+        if (k == 0xffffffffu || (i > 0 && (key[i - 1] >> 6) == (k >> 6))) continue;
+        uint64_t mask = 0;
+        for (int r = i; r < pow2 && key[r] != 0xffffffffu && (key[r] >> 6) == (k >> 6); r++)
+            mask |= 1ull << (key[r] & 63u);
+        ent_cl[out] = k >> 6;
+        ent_mask[out] = mask;
+        out++;
+    }
+}
+
+// The reference's generator (cyutils.pyx:1153-1232): PCG32 (XSH-RR) with
+// Lemire's bounded draw.
+struct Pcg32 {
+    uint64_t state;
+    explicit Pcg32(uint64_t seed) : state(seed + 1442695040888963407ULL) { next(); }
+    uint32_t next() {
+        const uint64_t x = state;
+        state = x * 6364136223846793005ULL + 1442695040888963407ULL;
+        const uint32_t s = (uint32_t)((x ^ (x >> 18)) >> 27);
+        const uint32_t rot = (uint32_t)(x >> 59);
+        return (s >> rot) | (s << ((-rot) & 31u));
+    }
+    uint32_t bounded(uint32_t bound) {
+        uint64_t m = (uint64_t)next() * bound;
+        uint32_t l = (uint32_t)m;
+        if (l < bound) {
+            const uint32_t threshold = (0u - bound) % bound;
+            while (l < threshold) {
+                m = (uint64_t)next() * bound;
+                l = (uint32_t)m;
+            }
+        }
+        return (uint32_t)(m >> 32);
+    }
+};
+
+template <typename T>
+struct Buf {
+    T *p = nullptr;
+    size_t count = 0;
+    ~Buf() {
+        if (p) (void)hipFree(p);
+    }
+    hipError_t alloc(size_t c) {
+        count = c;
+        return hipMalloc((void **)&p, sizeof(T) * std::max<size_t>(c, 1));
+    }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+    }
+};
+
+#define IVF_TRY(expr)                        \
+    do {                                     \
+        hipError_t _e = (expr);              \
+        if (_e != hipSuccess) {              \
+            *herr = _e;                      \
+            return 2;                        \
+        }                                    \
+    } while (0)
+
+unsigned grid_of(int64_t n) { return (unsigned)ceil_div(n, 256); }
+
+}  // namespace
+
+std::vector<uint32_t> kmeans_random_init_cells(int64_t n, int num_clusters, uint64_t seed) {
+    Pcg32 rng(seed);
+    std::vector<uint64_t> taken((size_t)(n + 63) / 64, 0);
+    std::vector<uint32_t> out;
+    out.reserve(num_clusters);
+    for (int i = 0; i < num_clusters; i++) {
+        for (;;) {
+            const uint32_t j = rng.bounded((uint32_t)n);
+            if (!(taken[j >> 6] & (1ull << (j & 63)))) {
+                taken[j >> 6] |= 1ull << (j & 63);
+                out.push_back(j);
+                break;
+            }
+        }
+    }
+    return out;
+}
+
+int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
+                float *out_dist, IvfInfo *info, std::string *msg,
+                hipError_t *herr, hipStream_t s) {
+    const int64_t n = P.n;
+    const int d = P.d, k = P.k;
+    int C = P.num_clusters;
+    const int st = knn_mfma_stride(d);
+    const int ksteps = (d + 3) / 4;
+    const int64_t pad = kKnnPadRows;
+
+    // Cell norms (kept for the search), the padded cells.
+    Buf<float> norms, Xp;
+    IVF_TRY(norms.alloc(n + pad));
+    IVF_TRY(Xp.alloc((n + pad) * st));
+    IVF_TRY(launch_row_norms(X, n, d, norms.p, s));
+    IVF_TRY(launch_knn_pack(X, nullptr, n, d, Xp.p, nullptr, norms.p, s));
+
+    // 1. k-means.
+    std::vector<uint32_t> init = kmeans_random_init_cells(n, C, P.seed);
+    Buf<uint32_t> rows, labels, counts, order, order_tmp, labels_sorted;
+    Buf<float> cent, centp, cnorm, mind, sums;
+    Buf<int64_t> off;
+    IVF_TRY(rows.alloc(C));
+    IVF_TRY(labels.alloc(n));
+    IVF_TRY(labels_sorted.alloc(n));
+    IVF_TRY(order.alloc(n));
+    IVF_TRY(order_tmp.alloc(n));
+    IVF_TRY(counts.alloc(C));
+    IVF_TRY(cent.alloc((size_t)C * d));
+    IVF_TRY(sums.alloc((size_t)C * d));
+    IVF_TRY(centp.alloc((size_t)(C + pad) * st));
+    IVF_TRY(cnorm.alloc(C + pad));
+    IVF_TRY(mind.alloc(n));
+    IVF_TRY(off.alloc(C + 1));
+    IVF_TRY(hipMemcpyAsync(rows.p, init.data(), sizeof(uint32_t) * C, hipMemcpyHostToDevice, s));
+    IVF_TRY(hipStreamSynchronize(s));
+    k_gather_rows<<<grid_of((int64_t)C * d), 256, 0, s>>>(X, d, rows.p, C, cent.p);
+    IVF_TRY(hipGetLastError());
+
+    // Radix sort scratch (labels < 2^bits).
+    int bits = 1;
+    while ((1ll << bits) < (int64_t)C) bits++;
+    size_t sort_bytes = 0;
+    IVF_TRY(rocprim::radix_sort_pairs(nullptr, sort_bytes, labels.p, labels_sorted.p,
+                                      order_tmp.p, order.p, (size_t)n, 0, bits, s));
+    Buf<char> sort_tmp;
+    IVF_TRY(sort_tmp.alloc(sort_bytes));
+
+    auto sort_by_label = [&]() -> hipError_t {
+        k_iota<<<grid_of(n), 256, 0, s>>>(order_tmp.p, n);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        size_t bytes = sort_bytes;
+        return rocprim::radix_sort_pairs(sort_tmp.p, bytes, labels.p, labels_sorted.p,
+                                         order_tmp.p, order.p, (size_t)n, 0, bits, s);
+    };
+
+    KnnTiles E{};
+    E.Q = Xp.p;
+    E.C = centp.p;
+    E.qnorm = nullptr;
+    E.cnorm = cnorm.p;
+    E.nq = n;
+    E.nc = C;
+    E.st = st;
+    E.ksteps = ksteps;
+    E.k = 1;
+    E.self = 0;
+    E.out_idx = labels.p;
+    E.out_dist = mind.p;
+
+    std::vector<uint32_t> h_counts(C);
+    std::vector<int64_t> h_off(C + 1);
+    float old_total = 0.0f;
+    float sum_sq_norms = 0.0f;
+    if (P.iterations > 2) {
+        std::vector<float> h_norms(n);
+        IVF_TRY(hipMemcpyAsync(h_norms.data(), norms.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        IVF_TRY(hipStreamSynchronize(s));
+        for (int64_t i = 0; i < n; i++) sum_sq_norms += h_norms[i];
+    }
+    int iteration = 0;
+    for (;;) {
+        // E-step against the current centroids.
+        IVF_TRY(launch_row_norms(cent.p, C, d, cnorm.p, s));
+        IVF_TRY(launch_knn_pack(cent.p, nullptr, C, d, centp.p, nullptr, cnorm.p, s));
+        IVF_TRY(launch_knn_tiles(E, d, false, s));
+        // M-step: counts, the stable order by label, sums in the reference's order.
+        IVF_TRY(hipMemsetAsync(counts.p, 0, sizeof(uint32_t) * C, s));
+        k_count_labels<<<grid_of(n), 256, 0, s>>>(labels.p, n, counts.p);
+        IVF_TRY(hipGetLastError());
+        IVF_TRY(sort_by_label());
+        IVF_TRY(hipMemcpyAsync(h_counts.data(), counts.p, sizeof(uint32_t) * C, hipMemcpyDeviceToHost, s));
+        IVF_TRY(hipStreamSynchronize(s));
+        h_off[0] = 0;
+        for (int j = 0; j < C; j++) h_off[j + 1] = h_off[j] + h_counts[j];
+        IVF_TRY(hipMemcpyAsync(off.p, h_off.data(), sizeof(int64_t) * (C + 1), hipMemcpyHostToDevice, s));
+        k_cluster_sums<<<(unsigned)ceil_div(C, 4), 256, 0, s>>>(X, d, order.p, off.p, C, P.chunk, sums.p);
+        IVF_TRY(hipGetLastError());
+        // Empty clusters take the farthest cells (kmeans.pyx:747-820).
+        std::vector<int> empty;
+        for (int j = 0; j < C; j++)
+            if (h_counts[j] == 0) empty.push_back(j);
+        if (!empty.empty()) {
+            Buf<float> far;
+            IVF_TRY(far.alloc(n));
+            k_dist_to_centroid<<<grid_of(n), 256, 0, s>>>(X, n, d, labels.p, cent.p, far.p);
+            IVF_TRY(hipGetLastError());
+            std::vector<float> h_far(n), h_sums((size_t)C * d);
+            std::vector<uint32_t> h_labels(n);
+            IVF_TRY(hipMemcpyAsync(h_far.data(), far.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+            IVF_TRY(hipMemcpyAsync(h_labels.data(), labels.p, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, s));
+            IVF_TRY(hipMemcpyAsync(h_sums.data(), sums.p, sizeof(float) * C * d, hipMemcpyDeviceToHost, s));
+            IVF_TRY(hipStreamSynchronize(s));
+            std::vector<int64_t> idx(n);
+            std::iota(idx.begin(), idx.end(), 0);
+            const size_t ne = empty.size();
+            std::partial_sort(idx.begin(), idx.begin() + ne, idx.end(), [&](int64_t a, int64_t b) {
+                return h_far[a] > h_far[b] || (h_far[a] == h_far[b] && a < b);
+            });
+            if (h_far[idx[0]] == 0.0f) {
+                *msg = "num_clusters (" + std::to_string(C) +
+                       ") is greater than the number of cells with distinct principal component loadings (" +
+                       std::to_string(n - (int64_t)ne) + "); decrease num_clusters";
+                return 1;
+            }
+            std::vector<float> h_x((size_t)d);
+            for (size_t i = 0; i < ne; i++) {
+                const int nl = empty[i];
+                const int64_t cell = idx[i];
+                const uint32_t ol = h_labels[cell];
+                IVF_TRY(hipMemcpyAsync(h_x.data(), X + cell * d, sizeof(float) * d, hipMemcpyDeviceToHost, s));
+                IVF_TRY(hipStreamSynchronize(s));
+                for (int j = 0; j < d; j++) {
+                    h_sums[(size_t)ol * d + j] -= h_x[j];
+                    h_sums[(size_t)nl * d + j] = h_x[j];
+                }
+                h_counts[ol] -= 1;
+                h_counts[nl] += 1;
+            }
+            IVF_TRY(hipMemcpyAsync(sums.p, h_sums.data(), sizeof(float) * C * d, hipMemcpyHostToDevice, s));
+            IVF_TRY(hipMemcpyAsync(counts.p, h_counts.data(), sizeof(uint32_t) * C, hipMemcpyHostToDevice, s));
+            IVF_TRY(hipStreamSynchronize(s));
+        }
+        k_cluster_normalize<<<grid_of((int64_t)C * d), 256, 0, s>>>(sums.p, d, C, counts.p);
+        IVF_TRY(hipGetLastError());
+        std::swap(cent.p, sums.p);
+        iteration++;
+        if (iteration == P.iterations) break;
+        // Convergence on the relative change of the inertia (kmeans.pyx
+        // 1088-1100), summed in FP32 in cell order as the reference does.
+        std::vector<float> h_mind(n);
+        IVF_TRY(hipMemcpyAsync(h_mind.data(), mind.p, sizeof(float) * n, hipMemcpyDeviceToHost, s));
+        IVF_TRY(hipStreamSynchronize(s));
+        float total = sum_sq_norms;
+        for (int64_t i = 0; i < n; i++) total += h_mind[i];
+        if (iteration > 1 && old_total - total < P.tolerance * old_total) break;
+        old_total = total;
+    }
+    info->iterations = iteration;
+    // A last E-step against the final centroids gives the labels and the
+    // counts (kmeans.pyx:1357-1432).
+    IVF_TRY(launch_row_norms(cent.p, C, d, cnorm.p, s));
+    IVF_TRY(launch_knn_pack(cent.p, nullptr, C, d, centp.p, nullptr, cnorm.p, s));
+    IVF_TRY(launch_knn_tiles(E, d, false, s));
+    IVF_TRY(hipMemsetAsync(counts.p, 0, sizeof(uint32_t) * C, s));
+    k_count_labels<<<grid_of(n), 256, 0, s>>>(labels.p, n, counts.p);
+    IVF_TRY(hipGetLastError());
+    IVF_TRY(hipMemcpyAsync(h_counts.data(), counts.p, sizeof(uint32_t) * C, hipMemcpyDeviceToHost, s));
+    IVF_TRY(hipStreamSynchronize(s));
+    if (P.kmeans_labels) {
+        IVF_TRY(hipMemcpyAsync(P.kmeans_labels, labels.p, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, s));
+        IVF_TRY(hipMemcpyAsync(P.kmeans_centroids, cent.p, sizeof(float) * C * d, hipMemcpyDeviceToDevice, s));
+        IVF_TRY(hipMemcpyAsync(P.kmeans_counts, h_counts.data(), sizeof(uint32_t) * C, hipMemcpyHostToDevice, s));
+        IVF_TRY(hipStreamSynchronize(s));
+        info->clusters = C;
+        return 0;
+    }
+
+    // 2. Drop empty clusters, renumber by the first centroid coordinate.
+    std::vector<float> h_cent((size_t)C * d);
+    IVF_TRY(hipMemcpyAsync(h_cent.data(), cent.p, sizeof(float) * C * d, hipMemcpyDeviceToHost, s));
+    IVF_TRY(hipStreamSynchronize(s));
+    std::vector<uint32_t> kept;
+    for (int j = 0; j < C; j++)
+        if (h_counts[j] > 0) kept.push_back((uint32_t)j);
+    std::stable_sort(kept.begin(), kept.end(), [&](uint32_t a, uint32_t b) {
+        return h_cent[(size_t)a * d] < h_cent[(size_t)b * d];
+    });
+    const int Ck = (int)kept.size();
+    std::vector<uint32_t> map(C, 0);
+    std::vector<int64_t> h_off2(Ck + 1);
+    h_off2[0] = 0;
+    for (int j = 0; j < Ck; j++) {
+        map[kept[j]] = (uint32_t)j;
+        h_off2[j + 1] = h_off2[j] + h_counts[kept[j]];
+    }
+    info->clusters = Ck;
+    // Enough clusters for k + 1 candidates even if the nearest are the
+    // smallest (single_cell.py:12555-12565).
+    std::vector<uint32_t> sizes(Ck);
+    for (int j = 0; j < Ck; j++) sizes[j] = h_counts[kept[j]];
+    std::sort(sizes.begin(), sizes.end());
+    int min_searched = Ck;
+    uint64_t run = 0;
+    for (int j = 0; j < Ck; j++) {
+        run += sizes[j];
+        if (run >= (uint64_t)k + 1) {
+            min_searched = j + 1;
+            break;
+        }
+    }
+    int searched = std::min(std::max(P.searched, min_searched), Ck);
+    info->searched = searched;
+    if (searched == Ck || searched > kWave) {
+        // Every cluster searched (or more than the lists hold): the
+        // exhaustive search, which is what searching every cluster returns.
+        info->exhaustive = 1;
+        KnnTiles T{};
+        T.Q = T.C = Xp.p;
+        T.qnorm = T.cnorm = norms.p;
+        T.nq = T.nc = n;
+        T.st = st;
+        T.ksteps = ksteps;
+        T.k = k;
+        T.self = 1;
+        T.out_idx = out_idx;
+        T.out_dist = out_dist;
+        IVF_TRY(launch_knn_tiles(T, d, false, s));
+        IVF_TRY(hipStreamSynchronize(s));
+        return 0;
+    }
+    info->exhaustive = 0;
+    Buf<uint32_t> dmap;
+    IVF_TRY(dmap.alloc(C));
+    IVF_TRY(hipMemcpyAsync(dmap.p, map.data(), sizeof(uint32_t) * C, hipMemcpyHostToDevice, s));
+    IVF_TRY(hipMemcpyAsync(off.p, h_off2.data(), sizeof(int64_t) * (Ck + 1), hipMemcpyHostToDevice, s));
+    IVF_TRY(hipMemcpyAsync(rows.p, kept.data(), sizeof(uint32_t) * Ck, hipMemcpyHostToDevice, s));
+    k_remap_labels<<<grid_of(n), 256, 0, s>>>(labels.p, n, dmap.p);
+    IVF_TRY(hipGetLastError());
+    k_gather_rows<<<grid_of((int64_t)Ck * d), 256, 0, s>>>(cent.p, d, rows.p, Ck, sums.p);
+    IVF_TRY(hipGetLastError());
+    IVF_TRY(launch_row_norms(sums.p, Ck, d, cnorm.p, s));
+    IVF_TRY(launch_knn_pack(sums.p, nullptr, Ck, d, centp.p, nullptr, cnorm.p, s));
+    IVF_TRY(sort_by_label());
+
+    // Cells in cluster order: padded rows and norms.
+    Buf<float> Xs, norms_s;
+    IVF_TRY(Xs.alloc((n + pad) * st));
+    IVF_TRY(norms_s.alloc(n + pad));
+    IVF_TRY(launch_knn_pack(X, order.p, n, d, Xs.p, norms.p, norms_s.p, s));
+    Xp.reset();  // the unsorted copy is no longer needed
+
+    // 3. Each cell's nearest clusters (by partial distance).
+    Buf<uint32_t> near;
+    Buf<float> near_d;
+    IVF_TRY(near.alloc((size_t)n * searched));
+    IVF_TRY(near_d.alloc((size_t)n * searched));
+    {
+        KnnTiles R{};
+        R.Q = Xs.p;
+        R.C = centp.p;
+        R.qnorm = nullptr;
+        R.cnorm = cnorm.p;
+        R.nq = n;
+        R.nc = Ck;
+        R.st = st;
+        R.ksteps = ksteps;
+        R.k = searched;
+        R.self = 0;
+        R.out_idx = near.p;
+        R.out_dist = near_d.p;
+        IVF_TRY(launch_knn_tiles(R, d, false, s));
+    }
+    near_d.reset();
+
+    // 4. Per-workgroup cluster lists.
+    const int64_t nwg = ceil_div(n, kIvfQueries);
+    Buf<int64_t> wg_cnt, wg_off;
+    IVF_TRY(wg_cnt.alloc(nwg));
+    IVF_TRY(wg_off.alloc(nwg + 1));
+    k_ivf_entries<false><<<(unsigned)nwg, 256, 0, s>>>(near.p, n, searched, wg_cnt.p, nullptr, nullptr, nullptr);
+    IVF_TRY(hipGetLastError());
+    std::vector<int64_t> h_wg((size_t)nwg + 1);
+    IVF_TRY(hipMemcpyAsync(h_wg.data() + 1, wg_cnt.p, sizeof(int64_t) * nwg, hipMemcpyDeviceToHost, s));
+    IVF_TRY(hipStreamSynchronize(s));
+    h_wg[0] = 0;
+    for (int64_t b = 0; b < nwg; b++) h_wg[b + 1] += h_wg[b];
+    const int64_t num_entries = h_wg[nwg];
+    IVF_TRY(hipMemcpyAsync(wg_off.p, h_wg.data(), sizeof(int64_t) * (nwg + 1), hipMemcpyHostToDevice, s));
+    Buf<uint32_t> ent_cl;
+    Buf<uint64_t> ent_mask;
+    IVF_TRY(ent_cl.alloc(num_entries));
+    IVF_TRY(ent_mask.alloc(num_entries));
+    k_ivf_entries<true><<<(unsigned)nwg, 256, 0, s>>>(near.p, n, searched, nullptr, wg_off.p, ent_cl.p, ent_mask.p);
+    IVF_TRY(hipGetLastError());
+    info->entries = num_entries;
+
+    // 5. The search.
+    KnnTiles T{};
+    T.Q = T.C = Xs.p;
+    T.qnorm = T.cnorm = norms_s.p;
+    T.nq = T.nc = n;
+    T.st = st;
+    T.ksteps = ksteps;
+    T.k = k;
+    T.self = 1;
+    T.wg_off = wg_off.p;
+    T.ent_cl = ent_cl.p;
+    T.ent_mask = ent_mask.p;
+    T.cl_off = off.p;
+    T.ids = order.p;
+    T.out_idx = out_idx;
+    T.out_dist = out_dist;
+    IVF_TRY(launch_knn_tiles(T, d, true, s));
+    IVF_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
+}  // namespace bpca

@@ -25,6 +25,9 @@
 #pragma once
 
 #include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
 #include <cstdint>
 
 namespace bpca {
@@ -187,6 +190,67 @@ size_t knn_mfma_lds_bytes(int d);
 hipError_t launch_knn_mfma(const float *X, float *norms, int64_t n,
                            int d, int k, float *Xp, uint32_t *out_idx,
                            float *out_dist, hipStream_t s);
+
+// The tile engine behind launch_knn_mfma, for the inverted-file search and
+// the centroid rankings of k-means (knn.hip).  Q and C are padded copies
+// (launch_knn_pack: row stride knn_mfma_stride(d), kKnnPadRows zero rows
+// after the last; norms likewise padded).  Query q's k best candidates go
+// to row (ids ? ids[q] : q) of out_idx / out_dist, ascending by (distance,
+// id); a candidate's id is ids[c] (or c).
+struct KnnTiles {
+    const float *Q;        // padded queries
+    const float *C;        // padded candidates
+    const float *qnorm;    // query norms, or null: partial distances ||c||^2 - 2 q.c
+    const float *cnorm;    // candidate norms
+    int64_t nq, nc;
+    int st, ksteps, k;     // row stride, ceil(d / 4), neighbours (<= 64)
+    int self;              // queries are the candidates: skip q == c
+    // Inverted file (ivf launches): workgroup b (queries 64b..64b+63) scans
+    // entries [wg_off[b], wg_off[b + 1]); entry e covers candidate rows
+    // [cl_off[ent_cl[e]], cl_off[ent_cl[e] + 1]) for the queries whose bit
+    // (q - 64b) is set in ent_mask[e].
+    const int64_t *wg_off;
+    const uint32_t *ent_cl;
+    const uint64_t *ent_mask;
+    const int64_t *cl_off;
+    const uint32_t *ids;
+    uint32_t *out_idx;
+    float *out_dist;
+};
+hipError_t launch_knn_pack(const float *X, const uint32_t *order, int64_t n,
+                           int d, float *Xp, const float *norms_in,
+                           float *norms_out, hipStream_t s);
+hipError_t launch_knn_tiles(const KnnTiles &T, int d, bool ivf, hipStream_t s);
+
+// SingleCell.neighbors() with its inverted-file search (ivf.hip).
+struct IvfParams {
+    int64_t n;          // cells
+    int d, k;           // dimensions (<= kKnnMfmaMaxDims), neighbours (<= 64)
+    int num_clusters, searched, iterations;
+    float tolerance;    // k-means relative inertia change
+    int64_t chunk;      // k-means chunk size (the M-step's summation order)
+    uint64_t seed;
+    // When kmeans_labels is set: stop after k-means and copy out the labels
+    // (n; from the final E-step against the final centroids), centroids
+    // (num_clusters x d) and counts (num_clusters, of those labels), as
+    // kmeans() leaves them.
+    uint32_t *kmeans_labels = nullptr;
+    float *kmeans_centroids = nullptr;
+    uint32_t *kmeans_counts = nullptr;
+};
+struct IvfInfo {
+    int clusters;       // non-empty clusters
+    int searched;       // clusters searched per cell
+    int iterations;     // k-means iterations run
+    int exhaustive;     // every cluster searched: the exhaustive search ran
+    int64_t entries;    // (workgroup, cluster) pairs scanned
+};
+// 0: done; 1: invalid input (*msg); 2: HIP error (*herr).  Synchronous.
+int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
+                float *out_dist, IvfInfo *info, std::string *msg,
+                hipError_t *herr, hipStream_t s);
+std::vector<uint32_t> kmeans_random_init_cells(int64_t n, int num_clusters,
+                                               uint64_t seed);
 
 // Synthetic workload generator and log1pPF (synth.hip).
 hipError_t launch_synth_row_nnz(uint64_t seed, uint64_t row0, int64_t rows,

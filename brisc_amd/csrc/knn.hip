@@ -271,36 +271,41 @@ __device__ __forceinline__ void merge_row(float (&Ld)[16], uint32_t (&Li)[16],
 
 }  // namespace
 
-// Padded copy: Xp[c * st + j] = X[c * d + j] (j < d), 0 for d <= j < st and
-// for the kTile pad rows c >= n; norms[n .. n + kTile) = 0.
+// Padded gather: row c of Xp is row (order ? order[c] : c) of X (n x d),
+// zero padded to st floats, and the kTile rows past n are zero; likewise
+// norms_out[c] = norms_in[order[c]] (when both are given) with a zero pad.
 __global__ __launch_bounds__(256) void k_knn_pack(const float *__restrict__ X,
+                                                  const uint32_t *__restrict__ order,
                                                   int64_t n, int d, int st,
                                                   float *__restrict__ Xp,
-                                                  float *__restrict__ norms) {
+                                                  const float *__restrict__ norms_in,
+                                                  float *__restrict__ norms_out) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (e < kTile) norms[n + e] = 0.0f;
+    if (norms_out && e < n + kTile)
+        norms_out[e] = e < n ? (norms_in ? norms_in[order ? order[e] : e] : norms_out[e]) : 0.0f;
     if (e >= (n + kTile) * st) return;
     const int64_t c = e / st;
     const int j = (int)(e - c * st);
-    Xp[e] = (c < n && j < d) ? X[c * d + j] : 0.0f;
+    float v = 0.0f;
+    if (c < n && j < d) v = X[(order ? (int64_t)order[c] : c) * d + j];
+    Xp[e] = v;
 }
 
-template <int PF>
-__global__ __launch_bounds__(256) void k_knn_mfma(
-    const float *__restrict__ Xp, const float *__restrict__ norms, int64_t n,
-    int st, int ksteps, int k, uint32_t *__restrict__ out_idx,
-    float *__restrict__ out_dist) {
+template <int PF, bool IVF>
+__global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     extern __shared__ __align__(16) float lds_f[];
+    const int st = T.st;
     float *qs = lds_f;                            // 64 x st queries
-    float *cs0 = qs + kTile * st;                 // 2 x (64 x st) cells
-    float *cn0 = cs0 + 2 * kTile * st;            // 2 x 64 cell norms
-    float *bufd = cn0 + 2 * kTile;                // 64 rows x kBuf
+    float *cs0 = qs + kTile * st;                 // 2 x (64 x st) candidates
+    float *cn0 = cs0 + 2 * kTile * st;            // 2 x 64 candidate norms
+    uint32_t *cid0 = (uint32_t *)(cn0 + 2 * kTile);  // 2 x 64 candidate ids
+    float *bufd = (float *)(cid0 + 2 * kTile);    // 64 rows x kBuf
     uint32_t *bufi = (uint32_t *)(bufd + kTile * kBuf);
     const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid >> 6;
     const int g = lane >> 4, m = lane & 15;
     const int64_t q0 = (int64_t)blockIdx.x * kTile;
     const int tile_v4 = kTile * (st >> 2);    // 16-byte vectors per tile (<= 256 * PF)
-    const int64_t tile_floats = (int64_t)kTile * st;
+    const int k = T.k;
 
     // This thread's 16-byte vectors of a tile: the same offset in the padded
     // rows (global) and in the LDS tile, both of row stride st.
@@ -311,22 +316,44 @@ __global__ __launch_bounds__(256) void k_knn_mfma(
         const int r = e / (st >> 2);
         off[u] = e < tile_v4 ? r * st + 4 * (e - r * (st >> 2)) : -1;
     }
+    // The candidate tiles, in scan order (wave-uniform state).  Exhaustive:
+    // rows 0, 64, ... of the candidates.  Inverted file: the rows of each of
+    // the workgroup's (cluster, query mask) entries in turn.
+    int64_t pos = 0, lim = T.nc, e = 0, e_end = 0;
+    uint64_t mask = ~0ull;
+    bool valid = T.nc > 0;
+    if (IVF) {
+        e = T.wg_off[blockIdx.x];
+        e_end = T.wg_off[blockIdx.x + 1];
+        valid = e < e_end;
+        if (valid) {
+            const uint32_t cl = T.ent_cl[e];
+            pos = T.cl_off[cl];
+            lim = T.cl_off[cl + 1];
+            mask = T.ent_mask[e];
+        }
+    }
     // Queries and the first tile (pad rows make every load in bounds).
 #pragma unroll
     for (int u = 0; u < PF; u++) {
         if (off[u] >= 0) {
-            *(knn_f4 *)(qs + off[u]) = *(const knn_f4 *)(Xp + q0 * st + off[u]);
-            *(knn_f4 *)(cs0 + off[u]) = *(const knn_f4 *)(Xp + off[u]);
+            *(knn_f4 *)(qs + off[u]) = *(const knn_f4 *)(T.Q + q0 * st + off[u]);
+            if (valid)
+                *(knn_f4 *)(cs0 + off[u]) = *(const knn_f4 *)(T.C + pos * st + off[u]);
         }
     }
-    if (tid < kTile) cn0[tid] = norms[tid];
+    if (tid < kTile && valid) {
+        const int64_t c = pos + tid;
+        cn0[tid] = T.cnorm[c];
+        cid0[tid] = c < T.nc ? (T.ids ? T.ids[c] : (uint32_t)c) : 0xffffffffu;
+    }
 
     float qn[4];
     int64_t qid[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         qid[i] = q0 + 16 * w + 4 * g + i;
-        qn[i] = norms[qid[i] < n ? qid[i] : n];
+        qn[i] = T.qnorm ? T.qnorm[qid[i] < T.nq ? qid[i] : T.nq] : 0.0f;
     }
     float Ld[16];
     uint32_t Li[16];
@@ -344,24 +371,45 @@ __global__ __launch_bounds__(256) void k_knn_mfma(
         tau_i[i] = 0xffffffffu;
         cnt[i] = 0;
     }
-    const int groups = ksteps >> 2, tail = ksteps & 3;
+    const int groups = T.ksteps >> 2, tail = T.ksteps & 3;
     float *const myb_d = bufd + (16 * w) * kBuf;
     uint32_t *const myb_i = bufi + (16 * w) * kBuf;
     const uint32_t below = (1u << m) - 1u;
     __syncthreads();
 
     int t = 0;
-    for (int64_t c0 = 0; c0 < n; c0 += kTile, t ^= 1) {
+    while (valid) {
         const float *cs = cs0 + t * (kTile * st);
         const float *cn = cn0 + t * kTile;
-        const int64_t c1 = c0 + kTile;
-        // Prefetch the next tile (pad rows past n read as zeros).
+        const uint32_t *cid = cid0 + t * kTile;
+        int64_t npos = pos + kTile, nlim = lim, ne = e;
+        uint64_t nmask = mask;
+        bool nvalid = npos < lim;
+        if (IVF && !nvalid) {
+            ne = e + 1;
+            nvalid = ne < e_end;
+            if (nvalid) {
+                const uint32_t cl = T.ent_cl[ne];
+                npos = T.cl_off[cl];
+                nlim = T.cl_off[cl + 1];
+                nmask = T.ent_mask[ne];
+            }
+        }
+        // Prefetch the next tile (pad rows past the end read as zeros).
         knn_f4 pf[PF];
-        const float *src = Xp + (c1 < n ? c1 : c0) * st;
+        float pfn = 0.0f;
+        uint32_t pfi = 0xffffffffu;
+        if (nvalid) {
+            const float *src = T.C + npos * st;
 #pragma unroll
-        for (int u = 0; u < PF; u++)
-            if (off[u] >= 0) pf[u] = *(const knn_f4 *)(src + off[u]);
-        const float pfn = tid < kTile ? norms[(c1 < n ? c1 : c0) + tid] : 0.0f;
+            for (int u = 0; u < PF; u++)
+                if (off[u] >= 0) pf[u] = *(const knn_f4 *)(src + off[u]);
+            if (tid < kTile) {
+                const int64_t c = npos + tid;
+                pfn = T.cnorm[c];
+                if (c < T.nc) pfi = T.ids ? T.ids[c] : (uint32_t)c;
+            }
+        }
 
         // 16 x 64 dot products of this wave's queries with the tile.
         knn_f4 acc[4];
@@ -389,27 +437,35 @@ __global__ __launch_bounds__(256) void k_knn_mfma(
                 acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, crow[16 * b * st + dim], acc[b], 0, 0, 0);
         }
 
-        // Distances ||q||^2 + (||c||^2 - 2 q.c) (the product by 2 is exact,
-        // so the fma rounds as the subtraction does); a candidate must beat
-        // its row's k-th entry.  Cells arrive in increasing index, so a tie
-        // with that entry loses.  Only the tile holding the queries
-        // themselves and the last tile need the self / range masks.
-        const bool special = c1 > n || c0 == q0;
+        // Distances ||q||^2 + (||c||^2 - 2 q.c), or the partial distance
+        // ||c||^2 - 2 q.c without a query norm (the product by 2 is exact, so
+        // the fma rounds as the subtraction does).  A candidate must beat its
+        // row's k-th entry by (distance, id); in the exhaustive scan ids
+        // increase, so a tie with that entry loses.  Masks: candidates past
+        // the tile's end, the query itself, and (inverted file) queries that
+        // do not search this cluster.
+        const bool special = IVF || pos + kTile > lim ||
+                             (T.self && pos < q0 + kTile && pos + kTile > q0);
         float dd[4][4];
         uint64_t bal[4][4];
         uint64_t any = 0;
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             const float cnv = cn[16 * b + m];
+            const uint32_t idv = cid[16 * b + m];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
                 float v = qn[i] + __builtin_fmaf(-2.0f, acc[b][i], cnv);
                 if (special) {
-                    const int64_t c = c0 + 16 * b + m;
-                    if (c >= n || c == qid[i]) v = INFINITY;
+                    const int64_t c = pos + 16 * b + m;
+                    const bool out = c >= lim || (T.self && c == qid[i]) ||
+                                     !((mask >> (16 * w + 4 * g + i)) & 1ull);
+                    if (out) v = INFINITY;
                 }
                 dd[b][i] = v;
-                bal[b][i] = __ballot(v < tau_d[i]);
+                bool p = v < tau_d[i];
+                if (IVF) p = v < INFINITY && key_lt(v, idv, tau_d[i], tau_i[i]);
+                bal[b][i] = __ballot(p);
                 any |= bal[b][i];
             }
         }
@@ -436,25 +492,37 @@ __global__ __launch_bounds__(256) void k_knn_mfma(
             }
 #pragma unroll
             for (int b = 0; b < 4; b++) {
+                const uint32_t idv = cid[16 * b + m];
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
                     const uint32_t gb = (uint32_t)((bal[b][i] >> (16 * g)) & 0xffffull);
                     if ((gb >> m) & 1u) {
                         const int slot = (4 * g + i) * kBuf + cnt[i] + __popc(gb & below);
                         myb_d[slot] = dd[b][i];
-                        myb_i[slot] = (uint32_t)(c0 + 16 * b + m);
+                        myb_i[slot] = idv;
                     }
                     cnt[i] += __popc(gb);
                 }
             }
         }
         // Stage the next tile.
-        float *csn = cs0 + (t ^ 1) * tile_floats;
+        if (nvalid) {
+            float *csn = cs0 + (t ^ 1) * (kTile * st);
 #pragma unroll
-        for (int u = 0; u < PF; u++)
-            if (off[u] >= 0) *(knn_f4 *)(csn + off[u]) = pf[u];
-        if (tid < kTile) cn0[(t ^ 1) * kTile + tid] = pfn;
+            for (int u = 0; u < PF; u++)
+                if (off[u] >= 0) *(knn_f4 *)(csn + off[u]) = pf[u];
+            if (tid < kTile) {
+                cn0[(t ^ 1) * kTile + tid] = pfn;
+                cid0[(t ^ 1) * kTile + tid] = pfi;
+            }
+        }
         __syncthreads();
+        pos = npos;
+        lim = nlim;
+        e = ne;
+        mask = nmask;
+        valid = nvalid;
+        t ^= 1;
     }
     // Final merges.
     uint32_t rmask = 0;
@@ -473,9 +541,10 @@ __global__ __launch_bounds__(256) void k_knn_mfma(
 #pragma unroll
     for (int r = 0; r < 16; r++) {
         const int64_t q = q0 + 16 * w + r;
-        if (q < n && lane < k) {
-            out_idx[q * k + lane] = Li[r];
-            out_dist[q * k + lane] = Ld[r];
+        if (q < T.nq && lane < k) {
+            const int64_t row = T.ids ? (int64_t)T.ids[q] : q;
+            T.out_idx[row * k + lane] = Li[r];
+            T.out_dist[row * k + lane] = Ld[r];
         }
     }
 }
@@ -484,7 +553,33 @@ int knn_mfma_stride(int d) { return ((d + 3) & ~3) + 4; }
 
 size_t knn_mfma_lds_bytes(int d) {
     const size_t st = (size_t)knn_mfma_stride(d);
-    return sizeof(float) * (3 * kTile * st + 2 * kTile + 2 * (size_t)kTile * kBuf);
+    return sizeof(float) * (3 * kTile * st + 4 * kTile + 2 * (size_t)kTile * kBuf);
+}
+
+hipError_t launch_knn_pack(const float *X, const uint32_t *order, int64_t n,
+                           int d, float *Xp, const float *norms_in,
+                           float *norms_out, hipStream_t s) {
+    const int st = knn_mfma_stride(d);
+    const int64_t total = (n + kTile) * st;
+    k_knn_pack<<<(unsigned)ceil_div(total > n + kTile ? total : n + kTile, 256), 256, 0, s>>>(
+        X, order, n, d, st, Xp, norms_in, norms_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_knn_tiles(const KnnTiles &T, int d, bool ivf, hipStream_t s) {
+    if (T.nq <= 0 || d < 1 || d > kKnnMfmaMaxDims || T.k < 1 || T.k > kWave)
+        return hipErrorInvalidValue;
+    const unsigned grid = (unsigned)ceil_div(T.nq, kTile);
+    const size_t lds = knn_mfma_lds_bytes(d);
+    const bool small = kTile * (T.st / 4) <= 256 * 4;
+    if (ivf) {
+        if (small) k_knn_mfma<4, true><<<grid, 256, lds, s>>>(T);
+        else k_knn_mfma<9, true><<<grid, 256, lds, s>>>(T);
+    } else {
+        if (small) k_knn_mfma<4, false><<<grid, 256, lds, s>>>(T);
+        else k_knn_mfma<9, false><<<grid, 256, lds, s>>>(T);
+    }
+    return hipGetLastError();
 }
 
 hipError_t launch_knn_mfma(const float *X, float *norms, int64_t n,
@@ -492,16 +587,20 @@ hipError_t launch_knn_mfma(const float *X, float *norms, int64_t n,
                            float *out_dist, hipStream_t s) {
     if (n <= 0 || d < 1 || d > kKnnMfmaMaxDims || k < 1 || k > kWave)
         return hipErrorInvalidValue;
-    const int st = knn_mfma_stride(d);
-    k_knn_pack<<<(unsigned)ceil_div((n + kTile) * st, 256), 256, 0, s>>>(X, n, d, st, Xp, norms);
-    const unsigned grid = (unsigned)ceil_div(n, kTile);
-    const size_t lds = knn_mfma_lds_bytes(d);
-    const int ksteps = (d + 3) / 4;
-    if (kTile * (st / 4) <= 256 * 4)
-        k_knn_mfma<4><<<grid, 256, lds, s>>>(Xp, norms, n, st, ksteps, k, out_idx, out_dist);
-    else
-        k_knn_mfma<9><<<grid, 256, lds, s>>>(Xp, norms, n, st, ksteps, k, out_idx, out_dist);
-    return hipGetLastError();
+    // Pad the norms in place (norms_in == norms_out keeps the first n).
+    hipError_t err = launch_knn_pack(X, nullptr, n, d, Xp, nullptr, norms, s);
+    if (err != hipSuccess) return err;
+    KnnTiles T{};
+    T.Q = T.C = Xp;
+    T.qnorm = T.cnorm = norms;
+    T.nq = T.nc = n;
+    T.st = knn_mfma_stride(d);
+    T.ksteps = (d + 3) / 4;
+    T.k = k;
+    T.self = 1;
+    T.out_idx = out_idx;
+    T.out_dist = out_dist;
+    return launch_knn_tiles(T, d, false, s);
 }
 
 size_t knn_lds_bytes(int d, int kmax) {

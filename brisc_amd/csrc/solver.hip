@@ -1258,4 +1258,94 @@ int bpca_knn_exact(bpca_context *ctx, const float *d_X, int64_t num_cells,
     return BPCA_OK;
 }
 
+int bpca_knn_ivf(bpca_context *ctx, const float *d_X, int64_t num_cells,
+                 int32_t num_dims, int32_t k, int32_t num_clusters,
+                 int32_t num_clusters_searched, int32_t num_kmeans_iterations,
+                 float kmeans_tolerance, int64_t chunk_size_kmeans,
+                 uint64_t seed, uint32_t *d_neighbors, float *d_distances,
+                 int64_t *info) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!d_X || !d_neighbors || !d_distances || !info) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL pointer");
+    if (num_cells < 2) return fail(BPCA_ERR_INVALID_ARGUMENT, "there is only one cell, so nearest neighbors cannot be calculated");
+    if (num_cells > (int64_t)UINT32_MAX) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells too large");
+    if (num_dims < 1 || num_dims > bpca::kKnnMfmaMaxDims)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "num_dims must be in [1, " + std::to_string(bpca::kKnnMfmaMaxDims) + "]");
+    if (k < 1 || k > 64 || (int64_t)k >= num_cells)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "k must be >= 1, <= 64 and less than the number of cells");
+    if (num_clusters < 1 || (int64_t)num_clusters > num_cells || num_clusters >= (1 << 26))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "num_clusters must be >= 1 and <= the number of cells (and < 2^26)");
+    if (num_clusters_searched < 1 || num_clusters_searched > num_clusters)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "num_clusters_searched must be >= 1 and <= num_clusters");
+    if (num_kmeans_iterations < 1) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_kmeans_iterations must be >= 1");
+    if (!(kmeans_tolerance > 0.0f)) return fail(BPCA_ERR_INVALID_ARGUMENT, "kmeans_tolerance must be > 0");
+    if (chunk_size_kmeans < 1 || chunk_size_kmeans > num_cells)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "chunk_size_kmeans must be >= 1 and <= the number of cells");
+    if ((int64_t)bpca::knn_mfma_lds_bytes(num_dims) > (int64_t)ctx->max_lds)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "num_dims needs more LDS than the device has");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    begin_call(ctx);
+    bpca::IvfParams P{num_cells, num_dims, k, num_clusters, num_clusters_searched,
+                      num_kmeans_iterations, kmeans_tolerance, chunk_size_kmeans, seed};
+    bpca::IvfInfo I{};
+    std::string msg;
+    hipError_t herr = hipSuccess;
+    const int rc = bpca::run_knn_ivf(d_X, P, d_neighbors, d_distances, &I, &msg, &herr, ctx->stream);
+    if (rc == 1) return fail(BPCA_ERR_INVALID_ARGUMENT, msg);
+    if (rc == 2) return hip_fail(herr, "run_knn_ivf");
+    info[0] = I.clusters;
+    info[1] = I.searched;
+    info[2] = I.iterations;
+    info[3] = I.exhaustive;
+    info[4] = I.entries;
+    return BPCA_OK;
+}
+
+int bpca_kmeans(bpca_context *ctx, const float *d_X, int64_t num_cells,
+                int32_t num_dims, int32_t num_clusters,
+                int32_t num_kmeans_iterations, float kmeans_tolerance,
+                int64_t chunk_size_kmeans, uint64_t seed, uint32_t *d_labels,
+                float *d_centroids, uint32_t *d_counts,
+                int32_t *iterations_run) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!d_X || !d_labels || !d_centroids || !d_counts || !iterations_run)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL pointer");
+    if (num_cells < 1 || num_cells > (int64_t)UINT32_MAX) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells out of range");
+    if (num_dims < 1 || num_dims > bpca::kKnnMfmaMaxDims)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "num_dims must be in [1, " + std::to_string(bpca::kKnnMfmaMaxDims) + "]");
+    if (num_clusters < 1 || (int64_t)num_clusters > num_cells || num_clusters >= (1 << 26))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "num_clusters must be >= 1 and <= the number of cells (and < 2^26)");
+    if (num_kmeans_iterations < 1) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_kmeans_iterations must be >= 1");
+    if (!(kmeans_tolerance > 0.0f)) return fail(BPCA_ERR_INVALID_ARGUMENT, "kmeans_tolerance must be > 0");
+    if (chunk_size_kmeans < 1 || chunk_size_kmeans > num_cells)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "chunk_size_kmeans must be >= 1 and <= the number of cells");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    begin_call(ctx);
+    bpca::IvfParams P{num_cells, num_dims, 1, num_clusters, 1,
+                      num_kmeans_iterations, kmeans_tolerance, chunk_size_kmeans, seed};
+    P.kmeans_labels = d_labels;
+    P.kmeans_centroids = d_centroids;
+    P.kmeans_counts = d_counts;
+    bpca::IvfInfo I{};
+    std::string msg;
+    hipError_t herr = hipSuccess;
+    const int rc = bpca::run_knn_ivf(d_X, P, nullptr, nullptr, &I, &msg, &herr, ctx->stream);
+    if (rc == 1) return fail(BPCA_ERR_INVALID_ARGUMENT, msg);
+    if (rc == 2) return hip_fail(herr, "run_knn_ivf");
+    *iterations_run = I.iterations;
+    return BPCA_OK;
+}
+
+int bpca_kmeans_init_cells(int64_t num_cells, int32_t num_clusters,
+                           uint64_t seed, uint32_t *cells) {
+    if (!cells) return fail(BPCA_ERR_INVALID_ARGUMENT, "cells is NULL");
+    if (num_cells < 1 || num_cells > (int64_t)UINT32_MAX) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells out of range");
+    if (num_clusters < 1 || (int64_t)num_clusters > num_cells)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "num_clusters must be >= 1 and <= the number of cells");
+    const std::vector<uint32_t> v = bpca::kmeans_random_init_cells(num_cells, num_clusters, seed);
+    std::copy(v.begin(), v.end(), cells);
+    return BPCA_OK;
+}
+
 }  // extern "C"

@@ -297,6 +297,48 @@ int bpca_knn_exact(bpca_context *ctx, const float *d_X, int64_t num_cells,
                    int32_t num_dims, int32_t k, uint32_t *d_neighbors,
                    float *d_distances);
 
+/* SingleCell.neighbors() with its default inverted-file search
+ * (brisc/single_cell.py:12139-12665: kmeans() of brisc/kmeans.pyx:889-1100
+ * with random initialisation, then knn_self of brisc/knn.pyx:13-516).
+ *   d_X            float32 num_cells x num_dims PCs (num_dims <= 128)
+ *   num_clusters   k-means clusters, 1 <= num_clusters <= min(num_cells, 2^26)
+ *   num_clusters_searched  nearest clusters searched per cell (raised, as
+ *                  the reference does, until k + 1 candidates are certain)
+ *   num_kmeans_iterations, kmeans_tolerance, chunk_size_kmeans, seed: as
+ *                  the reference's arguments of the same names
+ *   d_neighbors / d_distances: as bpca_knn_exact (indices are rows of X)
+ *   info           int64[5] out: non-empty clusters, clusters searched,
+ *                  k-means iterations run, 1 if every cluster was searched
+ *                  (the exhaustive search ran), (workgroup, cluster) pairs
+ * Synchronous. */
+int bpca_knn_ivf(bpca_context *ctx, const float *d_X, int64_t num_cells,
+                 int32_t num_dims, int32_t k, int32_t num_clusters,
+                 int32_t num_clusters_searched, int32_t num_kmeans_iterations,
+                 float kmeans_tolerance, int64_t chunk_size_kmeans,
+                 uint64_t seed, uint32_t *d_neighbors, float *d_distances,
+                 int64_t *info);
+
+/* kmeans() with random initialisation (brisc/kmeans.pyx:889-1432), the
+ * first stage of bpca_knn_ivf, on its own: d_labels (uint32 num_cells,
+ * from the final E-step against the final centroids), d_centroids (float32
+ * num_clusters x num_dims) and d_counts (uint32 num_clusters, of those
+ * labels) as the reference's kmeans leaves them; *iterations_run =
+ * iterations done.
+ * Arguments as bpca_knn_ivf.  Synchronous. */
+int bpca_kmeans(bpca_context *ctx, const float *d_X, int64_t num_cells,
+                int32_t num_dims, int32_t num_clusters,
+                int32_t num_kmeans_iterations, float kmeans_tolerance,
+                int64_t chunk_size_kmeans, uint64_t seed, uint32_t *d_labels,
+                float *d_centroids, uint32_t *d_counts,
+                int32_t *iterations_run);
+
+/* The cells kmeans() takes as initial centroids with random initialisation
+ * (kmeans_random_init, brisc/kmeans.pyx:13-39): num_clusters distinct cells
+ * drawn with the reference's PCG32 generator and Lemire's bounded draw
+ * (brisc/cyutils.pyx:1153-1232) from srand(seed).  Host only, no device. */
+int bpca_kmeans_init_cells(int64_t num_cells, int32_t num_clusters,
+                           uint64_t seed, uint32_t *cells);
+
 #ifdef __cplusplus
 }
 #endif

@@ -17,6 +17,10 @@ Contents:
   singular values, right singular vectors and PCs (``Z V_k``).
 * ``clipped_stddev`` -- closed form of ``clipped_stddev_csc``
   (``brisc/pca.pyx:51-103``) in FP64.
+* ``kmeans_fp64`` -- Lloyd's k-means as ``kmeans()`` runs it
+  (``brisc/kmeans.pyx:889-1432``: given initial cells, E-step by partial
+  distance with ties to the lower index, M-step means, empty clusters moved
+  to the farthest cells, a final E-step), in FP64.
 * ``knn_fp64`` -- brute-force k nearest neighbours in FP64 (squared
   Euclidean, self excluded, ties by index): what ``knn_self``
   (``brisc/knn.pyx:13-516``) returns when every cluster is searched, up to the
@@ -261,3 +265,40 @@ def knn_fp64(X: np.ndarray, k: int, block: int = 2048, rows=None):
         idx[q0:q1] = cand[r, order]
         dist[q0:q1] = exact[r, order]
     return idx, dist
+
+
+def kmeans_fp64(X: np.ndarray, init_cells, iterations: int):
+    """Labels, centroids and counts of ``kmeans()`` with the given initial
+    cells (``brisc/kmeans.pyx:889-1432``), in FP64: ``iterations`` E/M
+    steps, then a final E-step against the final centroids."""
+    X = np.asarray(X, np.float64)
+    cent = X[np.asarray(init_cells, np.int64)].copy()
+    C = len(cent)
+
+    def e_step(cent):
+        cn = (cent * cent).sum(1)
+        labels = np.empty(len(X), np.int64)
+        for b in range(0, len(X), 8192):
+            pd = cn[None, :] - 2.0 * (X[b:b + 8192] @ cent.T)
+            labels[b:b + 8192] = pd.argmin(1)   # first minimum: lower index
+        return labels
+
+    for _ in range(iterations):
+        labels = e_step(cent)
+        counts = np.bincount(labels, minlength=C)
+        sums = np.zeros_like(cent)
+        np.add.at(sums, labels, X)
+        empty = np.flatnonzero(counts == 0)
+        if empty.size:
+            far = ((X - cent[labels]) ** 2).sum(1)
+            order = np.lexsort((np.arange(len(X)), -far))[:empty.size]
+            for j, cell in zip(empty, order):
+                sums[labels[cell]] -= X[cell]
+                sums[j] = X[cell]
+                counts[labels[cell]] -= 1
+                counts[j] += 1
+        nz = counts > 0
+        sums[nz] /= counts[nz, None]
+        cent = sums
+    labels = e_step(cent)
+    return labels, cent, np.bincount(labels, minlength=C)

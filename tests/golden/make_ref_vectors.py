@@ -231,6 +231,27 @@ def ref_neighbors(kmeans_mod, knn_mod, X, k, num_clusters_searched=None,
     return out_nb, out_dist, C
 
 
+def ref_kmeans(kmeans_mod, X, num_clusters, iterations, seed=0,
+               num_threads=8):
+    """kmeans() with random initialisation, as neighbors() calls it:
+    labels (last E-step), centroids, counts and iterations run."""
+    from threadpoolctl import threadpool_limits
+    n, d = X.shape
+    labels = np.zeros(n, np.uint32)
+    cent = np.empty((num_clusters, d), np.float32)
+    cent_new = np.empty((num_clusters, d), np.float32)
+    counts = np.empty(num_clusters, np.uint32)
+    with threadpool_limits(1, user_api='blas'):
+        it = kmeans_mod.kmeans(
+            X=X, cluster_labels=labels, centroids=cent, centroids_new=cent_new,
+            num_cells_per_cluster=counts, min_distances=np.zeros(n, np.float32),
+            cell_norms=np.zeros(n, np.float32), kmeans_barbar=False,
+            num_init_iterations=5, num_kmeans_iterations=iterations,
+            tolerance=1e-2, oversampling_factor=1, seed=seed,
+            chunk_size=min(4096, n), is_mac=False, num_threads=num_threads)
+    return labels, (cent_new if it & 1 else cent), counts, it
+
+
 def knn_fixtures():
     """Exhaustive and default (approximate) neighbour searches of the
     reference on seeded PC-space mixtures: full outputs for the small sets,
@@ -268,6 +289,15 @@ def knn_fixtures():
             arrays[f'{name}_distances'] = dist_all
             arrays[f'{name}_default_neighbors'] = nb_def.astype(np.uint16)
         else:
+            # The k-means stage on its own: labels after one and two
+            # iterations, and the centroids / counts after two.
+            l1, _, _, _ = ref_kmeans(kmeans_mod, X, C, 1)
+            l2, c2, n2, it2 = ref_kmeans(kmeans_mod, X, C, 2)
+            arrays[f'{name}_kmeans1_labels'] = l1.astype(np.uint16)
+            arrays[f'{name}_kmeans2_labels'] = l2.astype(np.uint16)
+            arrays[f'{name}_kmeans2_centroids'] = c2
+            arrays[f'{name}_kmeans2_counts'] = n2
+            entry['kmeans_iterations_run'] = int(it2)
             rows = np.random.Generator(np.random.PCG64(2)).choice(
                 n, KNN_ROW_SAMPLE, replace=False)
             rows.sort()

@@ -1,0 +1,172 @@
+"""The inverted-file neighbour search (SingleCell.neighbors()' default) on
+the device against the reference's kmeans / knn_self outputs.
+
+Tolerances (written here):
+  * k-means initial draw: exact (pinned on the CPU in test_oracle.py);
+  * k-means labels (final E-step): at least 99.9% equal to the
+    reference's after one and two iterations (the E-step's FP32 dot products are blocked differently
+    from the reference's sgemm, so a cell equidistant to rounding from two
+    centroids may go either way);
+  * centroids: at least 95% bit-identical to the reference's (the M-step
+    sums in the reference's order, so only clusters touched by a differing
+    label differ); at least 95% within 1e-5 of the largest coordinate;
+  * neighbour sets against the reference's default search: at least 99.5%
+    of rows identical, recall against the exact search within 0.002 of the
+    reference's;
+  * with every cluster searched: bit-identical to the exhaustive search;
+  * run-to-run: bit-identical.
+"""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import helpers
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.join(os.path.dirname(__file__), 'golden')
+
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), 'GPU tests need a HIP device'
+    return torch
+
+
+def _fixture(name):
+    with open(os.path.join(HERE, 'ref_vectors.json')) as f:
+        meta = {m['name']: m for m in json.load(f)['knn']}[name]
+    X = helpers.knn_points(meta['seed'], meta['num_cells'], meta['num_dims'],
+                           meta['centers'])
+    assert hashlib.sha256(X.tobytes()).hexdigest() == meta['x_sha256']
+    return meta, X, np.load(os.path.join(HERE, 'ref_knn.npz'))
+
+
+def _dev(X):
+    torch = _torch()
+    return torch.from_numpy(np.ascontiguousarray(X, np.float32)).cuda()
+
+
+def _recall(a, b, k):
+    return float(np.mean([len(set(x) & set(y)) / k
+                          for x, y in zip(a.tolist(), b.tolist())]))
+
+
+def test_kmeans_matches_reference():
+    from brisc_amd.knn import kmeans
+    meta, X, ref = _fixture('knn_c')
+    C = meta['num_clusters']
+    Xd = _dev(X)
+    l1, _, _, it1 = kmeans(Xd, C, 1)
+    l1 = l1.cpu().numpy()
+    same1 = float((l1 == ref['knn_c_kmeans1_labels']).mean())
+    l2, c2, n2, it2 = kmeans(Xd, C, 2)
+    l2, c2, n2 = l2.cpu().numpy(), c2.cpu().numpy(), n2.cpu().numpy()
+    rl2 = ref['knn_c_kmeans2_labels'].astype(np.int64)
+    rc2 = ref['knn_c_kmeans2_centroids']
+    same2 = float((l2 == rl2).mean())
+    print(f'k-means labels equal to the reference: {same1:.6f} after one '
+          f'iteration, {same2:.6f} after two')
+    assert it1 == 1 and it2 == meta['kmeans_iterations_run'] == 2
+    assert same1 >= 0.999 and same2 >= 0.999
+    if same2 == 1.0:
+        assert np.array_equal(n2, ref['knn_c_kmeans2_counts'])
+    # The M-step sums in the reference's order: where the iterations'
+    # memberships agree, the centroids are bit-identical.
+    same_c = (c2.view(np.uint32) == rc2.view(np.uint32)).all(1)
+    rel = np.abs(c2 - rc2).max() / np.abs(rc2).max()
+    print(f'{int(same_c.sum())}/{C} centroids bit-identical; max relative '
+          f'difference {rel:.2e}')
+    assert same_c.mean() >= 0.95
+    close = (np.abs(c2 - rc2) <= 1e-5 * np.abs(rc2).max()).all(1)
+    assert close.mean() >= 0.95
+
+
+def test_ivf_matches_reference_default_search():
+    from brisc_amd.knn import knn_exact, knn_ivf
+    meta, X, ref = _fixture('knn_c')
+    k = meta['k']
+    rows = ref['knn_c_rows']
+    Xd = _dev(X)
+    nb, dist, info = knn_ivf(Xd, k)
+    nb, dist = nb.cpu().numpy().astype(np.int64), dist.cpu().numpy()
+    ex, exd = knn_exact(Xd, k)
+    ex = ex.cpu().numpy().astype(np.int64)
+    assert info['num_clusters'] <= meta['num_clusters']
+    assert info['num_clusters_searched'] == 64 and not info['exhaustive']
+    ref_def = ref['knn_c_default_neighbors'].astype(np.int64)
+    same = float((np.sort(nb[rows], 1) == np.sort(ref_def, 1)).all(1).mean())
+    rec_gpu = _recall(nb, ex, k)
+    rec_ref = _recall(ref_def, ex[rows], k)
+    print(f'IVF: {info}; rows with the reference default search\'s set '
+          f'{same:.4f}; recall against the exact search: device '
+          f'{rec_gpu:.5f}, reference {rec_ref:.5f} (on its row sample)')
+    assert same >= 0.995
+    assert abs(rec_gpu - rec_ref) <= 0.002
+    # Structure: no self, ascending, distances equal the exact search's for
+    # the shared pairs (same dot-product order per pair).
+    n = len(X)
+    assert not (nb == np.arange(n)[:, None]).any()
+    dd, ii = dist[:, 1:] - dist[:, :-1], nb[:, 1:] - nb[:, :-1]
+    assert ((dd > 0) | ((dd == 0) & (ii > 0))).all()
+    hit = nb == ex
+    assert np.array_equal(dist[hit].view(np.uint32),
+                          exd.cpu().numpy()[hit].view(np.uint32))
+
+
+@pytest.mark.parametrize('name', ['knn_a', 'knn_b'])
+def test_ivf_with_every_cluster_searched_is_the_exhaustive_search(name):
+    from brisc_amd.knn import knn_exact, knn_ivf
+    meta, X, ref = _fixture(name)
+    k = meta['k']
+    Xd = _dev(X)
+    nb, dist, info = knn_ivf(Xd, k)
+    ex, exd = knn_exact(Xd, k)
+    assert info['exhaustive']
+    assert np.array_equal(nb.cpu().numpy(), ex.cpu().numpy())
+    assert np.array_equal(dist.cpu().numpy(), exd.cpu().numpy())
+    ok, _, same_sets = helpers.knn_sets_agree(
+        X, nb.cpu().numpy().astype(np.int64),
+        ref[f'{name}_default_neighbors'].astype(np.int64), 2e-5)
+    assert ok and same_sets > 0.99
+
+
+def test_ivf_is_deterministic_and_recalls_at_scale():
+    # 200k cells in 50 dimensions, 100 centres: the default search against
+    # the exact search (size-independent: recall, self-exclusion, equal
+    # distances for shared pairs), twice for bitwise reproducibility.
+    from brisc_amd.knn import knn_exact, knn_ivf
+    X = helpers.knn_points(41, 200_000, 50, 100, spread=1.0)
+    Xd = _dev(X)
+    a = knn_ivf(Xd, 20)
+    b = knn_ivf(Xd, 20)
+    assert np.array_equal(a[0].cpu().numpy(), b[0].cpu().numpy())
+    assert np.array_equal(a[1].cpu().numpy().view(np.uint32),
+                          b[1].cpu().numpy().view(np.uint32))
+    ex, exd = knn_exact(Xd, 20)
+    nb, ex = a[0].cpu().numpy(), ex.cpu().numpy()
+    rec = _recall(nb, ex, 20)
+    print(f'200k cells: {a[2]}; recall against the exact search {rec:.5f}')
+    assert rec > 0.98
+    assert not (nb == np.arange(len(X))[:, None]).any()
+    hit = nb == ex
+    assert np.array_equal(a[1].cpu().numpy()[hit].view(np.uint32),
+                          exd.cpu().numpy()[hit].view(np.uint32))
+
+
+def test_single_cell_neighbors_default_search():
+    from brisc_amd import SingleCell
+    meta, X, ref = _fixture('knn_c')
+    n, k = len(X), meta['k']
+    sc = SingleCell(obs={'cell': np.arange(n)}, obsm={'pca': X})
+    out = sc.neighbors(QC_column=None, num_neighbors=k)
+    nb = out.obsm['neighbors']
+    assert nb.dtype == np.uint32 and nb.shape == (n, k)
+    rows = ref['knn_c_rows']
+    same = float((np.sort(nb[rows].astype(np.int64), 1) ==
+                  np.sort(ref['knn_c_default_neighbors'].astype(np.int64),
+                          1)).all(1).mean())
+    assert same >= 0.995

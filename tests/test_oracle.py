@@ -265,3 +265,31 @@ def test_knn_oracle_matches_reference_exhaustive_search(golden, name):
     exact = ((X64[rows][:, None, :] - X64[nb]) ** 2).sum(2)
     assert (np.abs(dist - exact) <=
             2e-5 * (norms[rows][:, None] + norms[nb])).all()
+
+
+def test_kmeans_init_draw_matches_reference(golden):
+    # The library's draw of the initial cells (the reference's PCG32 and
+    # Lemire generator, host code of the library) fed to the FP64 k-means
+    # restatement reproduces the reference's own k-means labels after one
+    # and two iterations (fixtures written by the reference), and its
+    # centroids; another seed's draw does not.
+    from brisc_amd.knn import kmeans_init_cells
+    meta, X, rows, _, _ = _knn_fixture(golden, 'knn_c')
+    ref = np.load(os.path.join(os.path.dirname(__file__), 'golden',
+                               'ref_knn.npz'))
+    C = meta['num_clusters']
+    cells = kmeans_init_cells(len(X), C, 0).astype(np.int64)
+    assert len(np.unique(cells)) == C
+    l1, _, _ = oracle.kmeans_fp64(X, cells, 1)
+    l2, c2, n2 = oracle.kmeans_fp64(X, cells, 2)
+    same1 = (l1 == ref['knn_c_kmeans1_labels']).mean()
+    same2 = (l2 == ref['knn_c_kmeans2_labels']).mean()
+    assert same1 >= 0.999 and same2 >= 0.999, (same1, same2)
+    # Centroids: equal to FP32 rounding except for clusters that a cell
+    # near a tie joined or left.
+    rc2 = ref['knn_c_kmeans2_centroids'].astype(np.float64)
+    close = (np.abs(c2 - rc2) <= 1e-5 * np.abs(rc2).max()).all(1)
+    assert close.mean() >= 0.95, close.mean()
+    other = kmeans_init_cells(len(X), C, 1).astype(np.int64)
+    lo, _, _ = oracle.kmeans_fp64(X, other, 1)
+    assert (lo == ref['knn_c_kmeans1_labels']).mean() < 0.5
