@@ -212,3 +212,16 @@ def request_stream(num_requests: int = 1000, rate: float = 8.0,
     return [(float(a), make_workload(seed * 100_003 + i, int(n), num_types=5,
                                      quick=True), int(kk))
             for i, (a, n, kk) in enumerate(zip(arrivals, sizes, ks))]
+
+
+def pc_mixture(seed: int, num_cells: int, num_dims: int, centers: int,
+               spread: float = 0.5) -> np.ndarray:
+    """Synthetic PC-space points for the neighbour search: a seeded FP32
+    Gaussian mixture (``centers`` centres drawn N(0, 9) per dimension, each
+    cell a centre plus N(0, spread^2) noise), regenerated bit for bit from
+    the seed on any host."""
+    rng = np.random.default_rng(seed)
+    C = rng.standard_normal((centers, num_dims)) * 3.0
+    lab = rng.integers(0, centers, num_cells)
+    return (C[lab] + spread * rng.standard_normal((num_cells, num_dims))
+            ).astype(np.float32)

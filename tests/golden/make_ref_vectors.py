@@ -5,6 +5,7 @@ Run in the development container only (it reads the reference sources):
     python tests/golden/make_ref_vectors.py [--reference /root/reference]
     python tests/golden/make_ref_vectors.py --cells1m   (adds the 1M-cell fixture)
     python tests/golden/make_ref_vectors.py --knn       (adds the neighbour fixtures)
+    python tests/golden/make_ref_vectors.py --knn-large (adds the 1M-cell neighbour fixture)
 
 It cythonizes ``brisc/cyutils.pyx``, ``brisc/pca.pyx``,
 ``brisc/normalize.pyx``, ``brisc/kmeans.pyx`` and ``brisc/knn.pyx`` from the
@@ -316,6 +317,39 @@ def knn_fixtures():
         json.dump(allmeta, f, indent=1)
 
 
+KNN_LARGE = ('pcs1m', 34, 1_000_000, 50, 100, 20, 1.0)
+
+
+def knn_large():
+    """The reference's default neighbour search on the 1M-cell PC mixture
+    that bench.py --mode knn times: seconds on this host's 8 threads, the
+    SHA-256 of its neighbour indices, a row sample of its output."""
+    import hashlib
+    kmeans_mod, knn_mod = import_knn()
+    name, seed, n, d, centers, k, spread = KNN_LARGE
+    X = synth.pc_mixture(seed, n, d, centers, spread)
+    t0 = time.perf_counter()
+    nb, dist, C = ref_neighbors(kmeans_mod, knn_mod, X, k)
+    secs = time.perf_counter() - t0
+    rows = np.random.Generator(np.random.PCG64(3)).choice(
+        n, KNN_ROW_SAMPLE, replace=False)
+    rows.sort()
+    np.savez_compressed(os.path.join(HERE, f'ref_knn_{name}.npz'), rows=rows,
+                        neighbors=nb[rows], distances=dist[rows])
+    path = os.path.join(HERE, 'ref_vectors.json')
+    with open(path) as f:
+        meta = json.load(f)
+    meta['knn_large'] = dict(
+        name=name, seed=seed, num_cells=n, num_dims=d, centers=centers,
+        spread=spread, k=k, num_clusters=C,
+        x_sha256=hashlib.sha256(X.tobytes()).hexdigest(),
+        default_sha256=hashlib.sha256(nb.tobytes()).hexdigest(),
+        seconds_default_8_threads=round(secs, 3), host_cpus=os.cpu_count())
+    with open(path, 'w') as f:
+        json.dump(meta, f, indent=1)
+    print(json.dumps(meta['knn_large'], indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--reference', default='/root/reference')
@@ -323,6 +357,8 @@ def main():
                     help='only add the 1M-cell fixture (ref_irlba_cells1m.npz)')
     ap.add_argument('--knn', action='store_true',
                     help='only add the neighbour-search fixtures (ref_knn.npz)')
+    ap.add_argument('--knn-large', action='store_true',
+                    help='only add the 1M-cell neighbour fixture and timing')
     args = ap.parse_args()
     build_reference(args.reference)
     pca_mod, norm_mod = import_reference()
@@ -330,6 +366,8 @@ def main():
         return cells1m(pca_mod, norm_mod)
     if args.knn:
         return knn_fixtures()
+    if args.knn_large:
+        return knn_large()
 
     tiny = {}
     meta = {'tiny': [], 'large': None,

@@ -69,13 +69,10 @@ def principal_angles(A, B):
 
 
 def knn_points(seed, n, d, centers, spread=0.5):
-    """Seeded FP32 Gaussian mixture in PC space (``centers`` centres, scale
-    3, within-centre spread ``spread``): the inputs of the neighbour-search
-    fixtures, regenerated bit for bit from the seed."""
-    rng = np.random.default_rng(seed)
-    C = rng.standard_normal((centers, d)) * 3.0
-    lab = rng.integers(0, centers, n)
-    return (C[lab] + spread * rng.standard_normal((n, d))).astype(np.float32)
+    """The inputs of the neighbour-search fixtures: brisc_amd.synth's
+    seeded PC-space mixture, regenerated bit for bit from the seed."""
+    from brisc_amd import synth
+    return synth.pc_mixture(seed, n, d, centers, spread)
 
 
 def knn_sets_agree(X, nb_a, nb_b, rtol, rows=None):

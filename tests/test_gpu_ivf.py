@@ -170,3 +170,31 @@ def test_single_cell_neighbors_default_search():
                   np.sort(ref['knn_c_default_neighbors'].astype(np.int64),
                           1)).all(1).mean())
     assert same >= 0.995
+
+
+def test_ivf_one_million_cells_against_reference():
+    # The reference's default search on 1M cells (row sample and SHA-256 of
+    # its full output, fixture written by the reference) against the device.
+    from brisc_amd import synth
+    from brisc_amd.knn import knn_ivf
+    with open(os.path.join(HERE, 'ref_vectors.json')) as f:
+        meta = json.load(f)['knn_large']
+    X = synth.pc_mixture(meta['seed'], meta['num_cells'], meta['num_dims'],
+                         meta['centers'], meta['spread'])
+    assert hashlib.sha256(X.tobytes()).hexdigest() == meta['x_sha256']
+    ref = np.load(os.path.join(HERE, f'ref_knn_{meta["name"]}.npz'))
+    nb, dist, info = knn_ivf(_dev(X), meta['k'])
+    assert info['num_clusters'] <= meta['num_clusters']
+    rows = ref['rows']
+    mine = nb.cpu().numpy()[rows].astype(np.int64)
+    same = float((np.sort(mine, 1) ==
+                  np.sort(ref['neighbors'].astype(np.int64), 1)).all(1).mean())
+    full = hashlib.sha256(nb.cpu().numpy().astype(np.uint32).tobytes()).hexdigest()
+    print(f'1M cells: {info}; sampled rows with the reference\'s neighbour '
+          f'set {same:.4f}; identical full output: {full == meta["default_sha256"]}')
+    assert same >= 0.995
+    X64 = X[rows].astype(np.float64)
+    norms = (X.astype(np.float64) ** 2).sum(1)
+    exact = ((X64[:, None, :] - X[mine].astype(np.float64)) ** 2).sum(2)
+    d = dist.cpu().numpy()[rows]
+    assert (np.abs(d - exact) <= 2e-5 * (norms[rows][:, None] + norms[mine])).all()
