@@ -141,7 +141,7 @@ __global__ __launch_bounds__(256) void k_dist_to_centroid(const float *__restric
 // search, with the mask of the cells searching each.  Keys (cluster << 6 |
 // cell - 64 b) are bitonically sorted in LDS; a run of equal clusters is one
 // entry.  WRITE = false counts the entries (counts[b]); WRITE = true writes
-// them from wg_off[b] on, in ascending cluster order.
+// them from wg_off[b] on, the clusters searched by the most cells first.
 template <bool WRITE>
 __global__ __launch_bounds__(256) void k_ivf_entries(const uint32_t *__restrict__ near,
                                                      int64_t n, int searched,
@@ -196,18 +196,53 @@ __global__ __launch_bounds__(256) void k_ivf_entries(const uint32_t *__restrict_
         }
         return;
     }
+    // Entries in cluster order into LDS ...
+    __shared__ uint32_t ecl[kIvfMaxKeys];
+    __shared__ uint64_t emask[kIvfMaxKeys];
     int before = 0;
     for (int t = 0; t < tid; t++) before += part[t];
-    int64_t out = wg_off[blockIdx.x] + before;
+    int e = before;
     for (int i = tid * per; i < (tid + 1) * per && i < pow2; i++) {
         const uint32_t k = key[i];
         if (k == 0xffffffffu || (i > 0 && (key[i - 1] >> 6) == (k >> 6))) continue;
         uint64_t mask = 0;
         for (int r = i; r < pow2 && key[r] != 0xffffffffu && (key[r] >> 6) == (k >> 6); r++)
             mask |= 1ull << (key[r] & 63u);
-        ent_cl[out] = k >> 6;
-        ent_mask[out] = mask;
-        out++;
+        ecl[e] = k >> 6;
+        emask[e] = mask;
+        e++;
+    }
+    __syncthreads();
+    int U = 0;
+    for (int t = 0; t < 256; t++) U += part[t];
+    // ... then ordered by the number of cells searching each, most first
+    // (clusters every cell of the block searches are the nearest: scanning
+    // them first tightens the thresholds early), ties in cluster order.
+    int p2 = 1;
+    while (p2 < U) p2 <<= 1;
+    for (int i = tid; i < p2; i += 256)
+        key[i] = i < U ? ((uint32_t)(64 - __popcll(emask[i])) << 12) | (uint32_t)i : 0xffffffffu;
+    __syncthreads();
+    for (int size = 2; size <= p2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            for (int i = tid; i < p2; i += 256) {
+                const int j = i ^ stride;
+                if (j > i) {
+                    const uint32_t a = key[i], b = key[j];
+                    if ((a > b) == ((i & size) == 0)) {
+                        key[i] = b;
+                        key[j] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const int64_t out = wg_off[blockIdx.x];
+    for (int i = tid; i < U; i += 256) {
+        const uint32_t src = key[i] & 0xfffu;
+        ent_cl[out + i] = ecl[src];
+        ent_mask[out + i] = emask[src];
     }
 }
 

@@ -185,6 +185,7 @@ int knn_kmax(int k);
 // norms holds n + kKnnPadRows floats (the pad is zeroed here).
 constexpr int kKnnMfmaMaxDims = 128;
 constexpr int kKnnPadRows = 64;
+constexpr int64_t kKnnMaxRows = (1ll << 31) - 1 - kKnnPadRows;  // 32-bit row positions
 int knn_mfma_stride(int d);
 size_t knn_mfma_lds_bytes(int d);
 hipError_t launch_knn_mfma(const float *X, float *norms, int64_t n,
@@ -195,8 +196,9 @@ hipError_t launch_knn_mfma(const float *X, float *norms, int64_t n,
 // the centroid rankings of k-means (knn.hip).  Q and C are padded copies
 // (launch_knn_pack: row stride knn_mfma_stride(d), kKnnPadRows zero rows
 // after the last; norms likewise padded).  Query q's k best candidates go
-// to row (ids ? ids[q] : q) of out_idx / out_dist, ascending by (distance,
-// id); a candidate's id is ids[c] (or c).
+// to row q (inverted file: ids[q]) of out_idx / out_dist, ascending by
+// (distance, id); a candidate's id is c (inverted file: ids[c]).  ids is
+// read by inverted-file launches only.
 struct KnnTiles {
     const float *Q;        // padded queries
     const float *C;        // padded candidates

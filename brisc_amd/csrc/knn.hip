@@ -291,10 +291,27 @@ __global__ __launch_bounds__(256) void k_knn_pack(const float *__restrict__ X,
     Xp[e] = v;
 }
 
-template <int PF, bool IVF>
+typedef __attribute__((address_space(3))) void lds_void;
+
+// The 16-byte vectors [0, 64 chunks) of a run starting at src, copied into
+// LDS at dst by the LDS DMA path (global_load_lds_dwordx4: lane l of a
+// 64-vector chunk lands 16 l bytes past the chunk's base), the chunks dealt
+// to the four waves in turn.  Completion is waited for with vmcnt.
+__device__ __forceinline__ void dma_run(const float *src, float *dst, int chunks,
+                                        int w, int lane) {
+    for (int c = w; c < chunks; c += 4)
+        __builtin_amdgcn_global_load_lds(src + 4 * (kWave * c + lane),
+                                         (lds_void *)(dst + 4 * kWave * c), 16, 0, 0);
+}
+
+// ARGMIN (k = 1, exhaustive scans: the k-means E-step): each lane keeps the
+// best (distance, id) of its own columns in registers; the 16 lanes of a
+// row group are reduced at the end.  No buffers, no merges.
+template <bool IVF, bool ARGMIN>
 __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     extern __shared__ __align__(16) float lds_f[];
     const int st = T.st;
+    const int chunks = st >> 2;                   // 64 rows x st floats
     float *qs = lds_f;                            // 64 x st queries
     float *cs0 = qs + kTile * st;                 // 2 x (64 x st) candidates
     float *cn0 = cs0 + 2 * kTile * st;            // 2 x 64 candidate norms
@@ -303,57 +320,48 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     uint32_t *bufi = (uint32_t *)(bufd + kTile * kBuf);
     const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid >> 6;
     const int g = lane >> 4, m = lane & 15;
-    const int64_t q0 = (int64_t)blockIdx.x * kTile;
-    const int tile_v4 = kTile * (st >> 2);    // 16-byte vectors per tile (<= 256 * PF)
+    const int q0 = (int)blockIdx.x * kTile;
+    const int nq = (int)T.nq, nc = (int)T.nc;
     const int k = T.k;
 
-    // This thread's 16-byte vectors of a tile: the same offset in the padded
-    // rows (global) and in the LDS tile, both of row stride st.
-    int off[PF];
-#pragma unroll
-    for (int u = 0; u < PF; u++) {
-        const int e = tid + 256 * u;
-        const int r = e / (st >> 2);
-        off[u] = e < tile_v4 ? r * st + 4 * (e - r * (st >> 2)) : -1;
-    }
     // The candidate tiles, in scan order (wave-uniform state).  Exhaustive:
     // rows 0, 64, ... of the candidates.  Inverted file: the rows of each of
     // the workgroup's (cluster, query mask) entries in turn.
-    int64_t pos = 0, lim = T.nc, e = 0, e_end = 0;
+    int pos = 0, lim = nc;
+    int64_t e = 0, e_end = 0;
     uint64_t mask = ~0ull;
-    bool valid = T.nc > 0;
+    bool valid = nc > 0;
     if (IVF) {
         e = T.wg_off[blockIdx.x];
         e_end = T.wg_off[blockIdx.x + 1];
         valid = e < e_end;
         if (valid) {
             const uint32_t cl = T.ent_cl[e];
-            pos = T.cl_off[cl];
-            lim = T.cl_off[cl + 1];
+            pos = (int)T.cl_off[cl];
+            lim = (int)T.cl_off[cl + 1];
             mask = T.ent_mask[e];
         }
     }
-    // Queries and the first tile (pad rows make every load in bounds).
-#pragma unroll
-    for (int u = 0; u < PF; u++) {
-        if (off[u] >= 0) {
-            *(knn_f4 *)(qs + off[u]) = *(const knn_f4 *)(T.Q + q0 * st + off[u]);
-            if (valid)
-                *(knn_f4 *)(cs0 + off[u]) = *(const knn_f4 *)(T.C + pos * st + off[u]);
-        }
-    }
-    if (tid < kTile && valid) {
-        const int64_t c = pos + tid;
-        cn0[tid] = T.cnorm[c];
-        cid0[tid] = c < T.nc ? (T.ids ? T.ids[c] : (uint32_t)c) : 0xffffffffu;
-    }
+    // Tile rows p .. p + 63 (pad rows and pad norms keep every read in
+    // bounds; ids are clamped, and their rows are masked) into buffer b.
+    auto stage = [&](int p, int b) {
+        dma_run(T.C + (size_t)p * st, cs0 + b * (kTile * st), chunks, w, lane);
+        if (w == 0)
+            __builtin_amdgcn_global_load_lds(T.cnorm + p + lane,
+                                             (lds_void *)(cn0 + b * kTile), 4, 0, 0);
+        if (IVF && w == 1)
+            __builtin_amdgcn_global_load_lds(T.ids + min(p + lane, nc - 1),
+                                             (lds_void *)(cid0 + b * kTile), 4, 0, 0);
+    };
+    dma_run(T.Q + (size_t)q0 * st, qs, chunks, w, lane);
+    if (valid) stage(pos, 0);
 
     float qn[4];
-    int64_t qid[4];
+    int qid[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         qid[i] = q0 + 16 * w + 4 * g + i;
-        qn[i] = T.qnorm ? T.qnorm[qid[i] < T.nq ? qid[i] : T.nq] : 0.0f;
+        qn[i] = T.qnorm ? T.qnorm[qid[i] < nq ? qid[i] : nq] : 0.0f;
     }
     float Ld[16];
     uint32_t Li[16];
@@ -365,16 +373,21 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     float tau_d[4];
     uint32_t tau_i[4];
     int cnt[4];
+    float bestd[4];
+    uint32_t besti[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         tau_d[i] = INFINITY;
         tau_i[i] = 0xffffffffu;
         cnt[i] = 0;
+        bestd[i] = INFINITY;
+        besti[i] = 0xffffffffu;
     }
     const int groups = T.ksteps >> 2, tail = T.ksteps & 3;
     float *const myb_d = bufd + (16 * w) * kBuf;
     uint32_t *const myb_i = bufi + (16 * w) * kBuf;
     const uint32_t below = (1u << m) - 1u;
+    __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
 
     int t = 0;
@@ -382,7 +395,8 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         const float *cs = cs0 + t * (kTile * st);
         const float *cn = cn0 + t * kTile;
         const uint32_t *cid = cid0 + t * kTile;
-        int64_t npos = pos + kTile, nlim = lim, ne = e;
+        int npos = pos + kTile, nlim = lim;
+        int64_t ne = e;
         uint64_t nmask = mask;
         bool nvalid = npos < lim;
         if (IVF && !nvalid) {
@@ -390,27 +404,18 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
             nvalid = ne < e_end;
             if (nvalid) {
                 const uint32_t cl = T.ent_cl[ne];
-                npos = T.cl_off[cl];
-                nlim = T.cl_off[cl + 1];
+                npos = (int)T.cl_off[cl];
+                nlim = (int)T.cl_off[cl + 1];
                 nmask = T.ent_mask[ne];
             }
         }
-        // Prefetch the next tile (pad rows past the end read as zeros).
-        knn_f4 pf[PF];
-        float pfn = 0.0f;
-        uint32_t pfi = 0xffffffffu;
-        if (nvalid) {
-            const float *src = T.C + npos * st;
-#pragma unroll
-            for (int u = 0; u < PF; u++)
-                if (off[u] >= 0) pf[u] = *(const knn_f4 *)(src + off[u]);
-            if (tid < kTile) {
-                const int64_t c = npos + tid;
-                pfn = T.cnorm[c];
-                if (c < T.nc) pfi = T.ids ? T.ids[c] : (uint32_t)c;
-            }
-        }
+        // The next tile streams into the other buffer while this one is used.
+        if (nvalid) stage(npos, t ^ 1);
 
+        // Inverted file: the wave's queries that search this cluster.  A wave
+        // with none skips the tile (it still stages and meets the barrier).
+        const uint32_t wmask = IVF ? (uint32_t)((mask >> (16 * w)) & 0xffffull) : 0xffffu;
+        if (wmask != 0) {
         // 16 x 64 dot products of this wave's queries with the tile.
         knn_f4 acc[4];
 #pragma unroll
@@ -444,39 +449,67 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         // increase, so a tie with that entry loses.  Masks: candidates past
         // the tile's end, the query itself, and (inverted file) queries that
         // do not search this cluster.
-        const bool special = IVF || pos + kTile > lim ||
+        const bool special = (IVF && wmask != 0xffffu) || pos + kTile > lim ||
                              (T.self && pos < q0 + kTile && pos + kTile > q0);
+        if constexpr (ARGMIN) {
+            // Columns arrive in increasing id per lane: a tie keeps the first.
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const float cnv = cn[16 * b + m];
+                const int c = pos + 16 * b + m;
+                const uint32_t idv = (uint32_t)c;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    float v = qn[i] + __builtin_fmaf(-2.0f, acc[b][i], cnv);
+                    if (special && (c >= lim || (T.self && c == qid[i]))) v = INFINITY;
+                    const bool better = v < bestd[i];
+                    bestd[i] = better ? v : bestd[i];
+                    besti[i] = better ? idv : besti[i];
+                }
+            }
+        } else {
         float dd[4][4];
-        uint64_t bal[4][4];
+        uint32_t idc[4];
+        int arr[4] = {0, 0, 0, 0};
         uint64_t any = 0;
+        auto passes = [&](float v, uint32_t idv, int i) {
+            return IVF ? (v < INFINITY && key_lt(v, idv, tau_d[i], tau_i[i]))
+                       : v < tau_d[i];
+        };
 #pragma unroll
         for (int b = 0; b < 4; b++) {
             const float cnv = cn[16 * b + m];
-            const uint32_t idv = cid[16 * b + m];
+            idc[b] = IVF ? cid[16 * b + m] : (uint32_t)(pos + 16 * b + m);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                dd[b][i] = qn[i] + __builtin_fmaf(-2.0f, acc[b][i], cnv);
+        }
+        if (special) {
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const int c = pos + 16 * b + m;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const bool out = c >= lim || (T.self && c == qid[i]) ||
+                                     !((wmask >> (4 * g + i)) & 1u);
+                    dd[b][i] = out ? INFINITY : dd[b][i];
+                }
+            }
+        }
+#pragma unroll
+        for (int b = 0; b < 4; b++) {
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                float v = qn[i] + __builtin_fmaf(-2.0f, acc[b][i], cnv);
-                if (special) {
-                    const int64_t c = pos + 16 * b + m;
-                    const bool out = c >= lim || (T.self && c == qid[i]) ||
-                                     !((mask >> (16 * w + 4 * g + i)) & 1ull);
-                    if (out) v = INFINITY;
-                }
-                dd[b][i] = v;
-                bool p = v < tau_d[i];
-                if (IVF) p = v < INFINITY && key_lt(v, idv, tau_d[i], tau_i[i]);
-                bal[b][i] = __ballot(p);
-                any |= bal[b][i];
+                const uint64_t bal = __ballot(passes(dd[b][i], idc[b], i));
+                arr[i] += __popcll((bal >> (16 * g)) & 0xffffull);
+                any |= bal;
             }
         }
         if (any) {
-            int arr[4] = {0, 0, 0, 0};
-#pragma unroll
-            for (int b = 0; b < 4; b++)
-#pragma unroll
-                for (int i = 0; i < 4; i++)
-                    arr[i] += __popcll((bal[b][i] >> (16 * g)) & 0xffffull);
-            // Rows whose buffer would overflow are merged first.
+            // Rows whose buffer would overflow are merged first.  (The
+            // appends below test against the thresholds as they are then: a
+            // candidate a merge has put out of reach is not kept, so arr
+            // bounds the appends.)
             uint32_t rmask = 0;
 #pragma unroll
             for (int i = 0; i < 4; i++) {
@@ -492,30 +525,23 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
             }
 #pragma unroll
             for (int b = 0; b < 4; b++) {
-                const uint32_t idv = cid[16 * b + m];
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    const uint32_t gb = (uint32_t)((bal[b][i] >> (16 * g)) & 0xffffull);
-                    if ((gb >> m) & 1u) {
+                    const bool p = passes(dd[b][i], idc[b], i);
+                    const uint32_t gb = (uint32_t)((__ballot(p) >> (16 * g)) & 0xffffull);
+                    if (p) {
                         const int slot = (4 * g + i) * kBuf + cnt[i] + __popc(gb & below);
                         myb_d[slot] = dd[b][i];
-                        myb_i[slot] = idv;
+                        myb_i[slot] = idc[b];
                     }
                     cnt[i] += __popc(gb);
                 }
             }
         }
-        // Stage the next tile.
-        if (nvalid) {
-            float *csn = cs0 + (t ^ 1) * (kTile * st);
-#pragma unroll
-            for (int u = 0; u < PF; u++)
-                if (off[u] >= 0) *(knn_f4 *)(csn + off[u]) = pf[u];
-            if (tid < kTile) {
-                cn0[(t ^ 1) * kTile + tid] = pfn;
-                cid0[(t ^ 1) * kTile + tid] = pfi;
-            }
-        }
+        }  // selection
+        }  // wmask
+        // The next tile has landed (this wave's share), then every wave's.
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         pos = npos;
         lim = nlim;
@@ -523,6 +549,29 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         mask = nmask;
         valid = nvalid;
         t ^= 1;
+    }
+    if constexpr (ARGMIN) {
+        // Reduce each row over the 16 lanes of its group.
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            float d = bestd[i];
+            uint32_t id = besti[i];
+#pragma unroll
+            for (int x = 1; x < 16; x <<= 1) {
+                const float od = __shfl_xor(d, x);
+                const uint32_t oi = (uint32_t)__shfl_xor((int)id, x);
+                const bool take = key_lt(od, oi, d, id);
+                d = take ? od : d;
+                id = take ? oi : id;
+            }
+            const int q = qid[i];
+            if (m == 0 && q < nq) {
+                const size_t row = (size_t)q;
+                T.out_idx[row] = id;
+                T.out_dist[row] = d;
+            }
+        }
+        return;
     }
     // Final merges.
     uint32_t rmask = 0;
@@ -540,9 +589,9 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     }
 #pragma unroll
     for (int r = 0; r < 16; r++) {
-        const int64_t q = q0 + 16 * w + r;
-        if (q < T.nq && lane < k) {
-            const int64_t row = T.ids ? (int64_t)T.ids[q] : q;
+        const int q = q0 + 16 * w + r;
+        if (q < nq && lane < k) {
+            const size_t row = IVF ? (size_t)T.ids[q] : (size_t)q;
             T.out_idx[row * k + lane] = Li[r];
             T.out_dist[row * k + lane] = Ld[r];
         }
@@ -567,17 +616,19 @@ hipError_t launch_knn_pack(const float *X, const uint32_t *order, int64_t n,
 }
 
 hipError_t launch_knn_tiles(const KnnTiles &T, int d, bool ivf, hipStream_t s) {
-    if (T.nq <= 0 || d < 1 || d > kKnnMfmaMaxDims || T.k < 1 || T.k > kWave)
+    if (T.nq <= 0 || d < 1 || d > kKnnMfmaMaxDims || T.k < 1 || T.k > kWave ||
+        T.nq > kKnnMaxRows || T.nc > kKnnMaxRows)
         return hipErrorInvalidValue;
     const unsigned grid = (unsigned)ceil_div(T.nq, kTile);
     const size_t lds = knn_mfma_lds_bytes(d);
-    const bool small = kTile * (T.st / 4) <= 256 * 4;
     if (ivf) {
-        if (small) k_knn_mfma<4, true><<<grid, 256, lds, s>>>(T);
-        else k_knn_mfma<9, true><<<grid, 256, lds, s>>>(T);
+        k_knn_mfma<true, false><<<grid, 256, lds, s>>>(T);
+    } else if (T.k == 1) {
+        // No candidate buffers: the tiles, norms and ids only.
+        const size_t lds1 = sizeof(float) * (3 * (size_t)kTile * T.st + 4 * kTile);
+        k_knn_mfma<false, true><<<grid, 256, lds1, s>>>(T);
     } else {
-        if (small) k_knn_mfma<4, false><<<grid, 256, lds, s>>>(T);
-        else k_knn_mfma<9, false><<<grid, 256, lds, s>>>(T);
+        k_knn_mfma<false, false><<<grid, 256, lds, s>>>(T);
     }
     return hipGetLastError();
 }
