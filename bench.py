@@ -364,6 +364,66 @@ def bench_knn(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_pipeline(args):
+    """The device chain SingleCell.normalize (log1pPF) -> pca (50 PCs) ->
+    neighbors (k = 20, default search) on the 1M-cell workload: counts
+    generated on the device, every stage device-resident, PCs never leave
+    HBM.  Stage times from the host clock around each synchronous stage;
+    the reference's own chain on the same workload (fixture written on the
+    development host, 8 threads) is reported beside it, with the share of
+    sampled cells whose neighbour set equals the reference's."""
+    import torch
+    from brisc_amd import Context, irlba_device, synth
+    from brisc_amd.knn import knn_ivf
+
+    w = synth.make_workload(**synth.WORKLOADS[args.workload])
+    ctx = Context(0)
+    runs = []
+    for rep in range(max(1, args.steps) + 1):       # the first is a warm-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        data, indices, indptr = synth.generate_device(w, context=ctx)
+        t1 = time.perf_counter()
+        P, sv, stats = irlba_device(data, indices, indptr, w.num_cells,
+                                    w.num_genes, 50, context=ctx)
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        nb, dist, info = knn_ivf(P, 20, context=ctx)
+        torch.cuda.synchronize()
+        t3 = time.perf_counter()
+        runs.append((t1 - t0, t2 - t1, t3 - t2))
+        del data, indices, indptr
+    gen, pca, knn = (min(r[i] for r in runs[1:]) for i in range(3))
+    line = {
+        'metric': 'device chain generate+normalize -> pca -> neighbors',
+        'value': round(pca + knn, 4), 'unit': 's (pca + neighbors)',
+        'n_gpus': 1, 'steps': args.steps, 'higher_is_better': False,
+        'dtype': 'f32 products / f64 Lanczos', 'data': 'synthetic',
+        'config': {'workload': args.workload, 'num_cells': w.num_cells,
+                   'num_genes': w.num_genes, 'num_PCs': 50, 'num_neighbors': 20,
+                   **info},
+        'seconds': {'generate_normalize': round(gen, 4), 'pca': round(pca, 4),
+                    'neighbors': round(knn, 4)},
+        'pca_converged': bool(stats['converged']),
+    }
+    ref_meta = os.path.join(ROOT, 'tests', 'golden', 'ref_vectors.json')
+    ref_rows = os.path.join(ROOT, 'tests', 'golden', 'ref_pipeline_cells1m.npz')
+    if args.workload == 'cells1m' and os.path.exists(ref_rows):
+        with open(ref_meta) as f:
+            ref = json.load(f).get('pipeline_cells1m')
+        r = np.load(ref_rows)
+        mine = nb.cpu().numpy()[r['rows']].astype(np.int64)
+        same = float((np.sort(mine, 1) == np.sort(r['neighbors'].astype(np.int64), 1)).all(1).mean())
+        line['reference'] = {
+            'seconds_8_threads': {'normalize': ref['seconds_normalize_8_threads'],
+                                  'pca': ref['seconds_pca_8_threads'],
+                                  'neighbors': ref['seconds_neighbors_8_threads']},
+            'host': 'development container (8 cores); the reference does not run here',
+            'rows_with_same_neighbour_set': round(same, 4),
+            'rows_compared': int(len(r['rows']))}
+    print(json.dumps(line), flush=True)
+
+
 def bench_copy(args):
     """A plain device copy of known size (calibrates HBM byte counters and
     gives the achievable copy rate)."""
@@ -391,7 +451,7 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--mode', choices=('single', 'batched', 'serve', 'copy', 'subset', 'knn'),
+    ap.add_argument('--mode', choices=('single', 'batched', 'serve', 'copy', 'subset', 'knn', 'pipeline'),
                     default='single')
     ap.add_argument('--workload', default='cells10m')
     ap.add_argument('--k', type=int, default=50)
@@ -425,6 +485,10 @@ def main():
         return bench_subset(args)
     if args.mode == 'knn':
         return bench_knn(args)
+    if args.mode == 'pipeline':
+        if args.workload == 'cells10m' and '--workload' not in sys.argv:
+            args.workload = 'cells1m'
+        return bench_pipeline(args)
 
     import torch
     from brisc_amd import Context, irlba_device, synth

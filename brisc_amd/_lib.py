@@ -124,6 +124,7 @@ def lib() -> ctypes.CDLL:
         'bpca_kmeans_init_cells': (i32, [i64, i32, ctypes.c_uint64, p]),
         'bpca_test_small_svd': (i32, [p, i32, p, p, p, p, p]),
         'bpca_test_gemm': (i32, [p, i64, i32, i32, p, p, p]),
+        'bpca_test_gemm_device': (i32, [p, i64, i32, i32, p, p, p]),
         'bpca_context_set_kernel_config': (i32, [p, i32, i32]),
         'bpca_context_get_kernel_config': (i32, [p, ctypes.POINTER(ctypes.c_int32),
                                                  ctypes.POINTER(ctypes.c_int32)]),

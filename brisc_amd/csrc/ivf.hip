@@ -590,6 +590,7 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
         R.ksteps = ksteps;
         R.k = searched;
         R.self = 0;
+        R.start_hint = labels_sorted.p;  // the block's own cluster first
         R.out_idx = near.p;
         R.out_dist = near_d.p;
         IVF_TRY(launch_knn_tiles(R, d, false, s));

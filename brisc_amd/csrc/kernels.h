@@ -216,6 +216,9 @@ struct KnnTiles {
     const uint64_t *ent_mask;
     const int64_t *cl_off;
     const uint32_t *ids;
+    // Exhaustive launches: optional, per query; the scan of query block b
+    // starts at the 64-row tile holding row start_hint[64 b] and wraps.
+    const uint32_t *start_hint;
     uint32_t *out_idx;
     float *out_dist;
 };

@@ -307,7 +307,10 @@ __device__ __forceinline__ void dma_run(const float *src, float *dst, int chunks
 // ARGMIN (k = 1, exhaustive scans: the k-means E-step): each lane keeps the
 // best (distance, id) of its own columns in registers; the 16 lanes of a
 // row group are reduced at the end.  No buffers, no merges.
-template <bool IVF, bool ARGMIN>
+// ROT (exhaustive scans with a start hint: the centroid ranking): the scan
+// starts at the tile holding the block's own cluster and wraps, so the
+// thresholds are tight from the first tiles; ids then arrive out of order.
+template <bool IVF, bool ARGMIN, bool ROT>
 __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     extern __shared__ __align__(16) float lds_f[];
     const int st = T.st;
@@ -331,6 +334,13 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     int64_t e = 0, e_end = 0;
     uint64_t mask = ~0ull;
     bool valid = nc > 0;
+    int start = 0;
+    bool wrapped = false;
+    if (ROT) {
+        start = min((int)(T.start_hint[min(q0, nq - 1)] & ~(uint32_t)(kTile - 1)),
+                    (nc - 1) & ~(kTile - 1));
+        pos = start;
+    }
     if (IVF) {
         e = T.wg_off[blockIdx.x];
         e_end = T.wg_off[blockIdx.x + 1];
@@ -399,6 +409,13 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         int64_t ne = e;
         uint64_t nmask = mask;
         bool nvalid = npos < lim;
+        bool nwrapped = wrapped;
+        if (ROT && !nvalid && !wrapped && start > 0) {
+            npos = 0;
+            nlim = start;
+            nwrapped = true;
+            nvalid = true;
+        }
         if (IVF && !nvalid) {
             ne = e + 1;
             nvalid = ne < e_end;
@@ -473,8 +490,8 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         int arr[4] = {0, 0, 0, 0};
         uint64_t any = 0;
         auto passes = [&](float v, uint32_t idv, int i) {
-            return IVF ? (v < INFINITY && key_lt(v, idv, tau_d[i], tau_i[i]))
-                       : v < tau_d[i];
+            return (IVF || ROT) ? (v < INFINITY && key_lt(v, idv, tau_d[i], tau_i[i]))
+                                : v < tau_d[i];
         };
 #pragma unroll
         for (int b = 0; b < 4; b++) {
@@ -548,6 +565,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         e = ne;
         mask = nmask;
         valid = nvalid;
+        wrapped = nwrapped;
         t ^= 1;
     }
     if constexpr (ARGMIN) {
@@ -622,13 +640,15 @@ hipError_t launch_knn_tiles(const KnnTiles &T, int d, bool ivf, hipStream_t s) {
     const unsigned grid = (unsigned)ceil_div(T.nq, kTile);
     const size_t lds = knn_mfma_lds_bytes(d);
     if (ivf) {
-        k_knn_mfma<true, false><<<grid, 256, lds, s>>>(T);
+        k_knn_mfma<true, false, false><<<grid, 256, lds, s>>>(T);
     } else if (T.k == 1) {
         // No candidate buffers: the tiles, norms and ids only.
         const size_t lds1 = sizeof(float) * (3 * (size_t)kTile * T.st + 4 * kTile);
-        k_knn_mfma<false, true><<<grid, 256, lds1, s>>>(T);
+        k_knn_mfma<false, true, false><<<grid, 256, lds1, s>>>(T);
+    } else if (T.start_hint) {
+        k_knn_mfma<false, false, true><<<grid, 256, lds, s>>>(T);
     } else {
-        k_knn_mfma<false, false><<<grid, 256, lds, s>>>(T);
+        k_knn_mfma<false, false, false><<<grid, 256, lds, s>>>(T);
     }
     return hipGetLastError();
 }

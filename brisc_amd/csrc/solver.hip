@@ -475,6 +475,19 @@ int bpca_test_gemm(bpca_context *ctx, int64_t n, int32_t m, int32_t kout,
     return BPCA_OK;
 }
 
+int bpca_test_gemm_device(bpca_context *ctx, int64_t n, int32_t m,
+                          int32_t kout, const double *d_A, const double *d_C,
+                          double *d_out) {
+    if (!ctx || !d_A || !d_C || !d_out || n < 1 || m < 1 || kout < 1 || m > 128 || kout > m)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "bad argument");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    begin_call(ctx);
+    HIP_TRY(bpca::launch_gemm_mfma_test(d_A, n, m, d_C, kout, d_out, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return BPCA_OK;
+}
+
 int bpca_context_create(int device, uint64_t hbm_budget_bytes,
                         bpca_context **out) {
     if (!out) return fail(BPCA_ERR_INVALID_ARGUMENT, "out is NULL");

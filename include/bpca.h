@@ -222,6 +222,11 @@ int bpca_test_small_svd(bpca_context *ctx, int32_t m, const double *B,
  * point; m <= 128, kout <= 128). */
 int bpca_test_gemm(bpca_context *ctx, int64_t n, int32_t m, int32_t kout,
                    const double *A, const double *C, double *out);
+/* The same product on device arrays (column-major d_A n x m, d_C m x kout,
+ * d_out n x kout); synchronous. */
+int bpca_test_gemm_device(bpca_context *ctx, int64_t n, int32_t m,
+                          int32_t kout, const double *d_A, const double *d_C,
+                          double *d_out);
 
 /* ------------------------------------------------ synthetic workloads */
 

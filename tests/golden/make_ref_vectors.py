@@ -6,6 +6,7 @@ Run in the development container only (it reads the reference sources):
     python tests/golden/make_ref_vectors.py --cells1m   (adds the 1M-cell fixture)
     python tests/golden/make_ref_vectors.py --knn       (adds the neighbour fixtures)
     python tests/golden/make_ref_vectors.py --knn-large (adds the 1M-cell neighbour fixture)
+    python tests/golden/make_ref_vectors.py --pipeline1m (adds the 1M-cell pipeline fixture)
 
 It cythonizes ``brisc/cyutils.pyx``, ``brisc/pca.pyx``,
 ``brisc/normalize.pyx``, ``brisc/kmeans.pyx`` and ``brisc/knn.pyx`` from the
@@ -350,6 +351,44 @@ def knn_large():
     print(json.dumps(meta['knn_large'], indent=1))
 
 
+def pipeline1m(pca_mod, norm_mod):
+    """The reference's normalize -> irlba -> neighbors chain on the 1M-cell
+    workload (SingleCell.pca() + SingleCell.neighbors() defaults, all genes,
+    no QC): stage times on this host's 8 threads and a row sample of the
+    neighbours found on the reference's own PCs."""
+    import hashlib
+    kmeans_mod, knn_mod = import_knn()
+    w = synth.make_workload(**synth.WORKLOADS['cells1m'])
+    indptr, indices, counts = oracle.generate_counts(w)
+    t0 = time.perf_counter()
+    data = ref_normalize(norm_mod, counts, indices, indptr)
+    t_norm = time.perf_counter() - t0
+    PCs, conv, t_pca = ref_irlba(pca_mod, data, indices, indptr, w.num_cells,
+                                 w.num_genes, 50, 100, 1024, num_threads=8)
+    t0 = time.perf_counter()
+    nb, dist, C = ref_neighbors(kmeans_mod, knn_mod, np.ascontiguousarray(PCs), 20)
+    t_knn = time.perf_counter() - t0
+    rows = np.random.Generator(np.random.PCG64(5)).choice(
+        w.num_cells, KNN_ROW_SAMPLE, replace=False)
+    rows.sort()
+    np.savez_compressed(os.path.join(HERE, 'ref_pipeline_cells1m.npz'),
+                        rows=rows, neighbors=nb[rows], distances=dist[rows])
+    path = os.path.join(HERE, 'ref_vectors.json')
+    with open(path) as f:
+        meta = json.load(f)
+    meta['pipeline_cells1m'] = dict(
+        workload='cells1m', k=50, num_neighbors=20, num_clusters=C,
+        converged=conv, seconds_normalize_8_threads=round(t_norm, 3),
+        seconds_pca_8_threads=round(t_pca, 3),
+        seconds_neighbors_8_threads=round(t_knn, 3),
+        pcs_sha256=hashlib.sha256(PCs.tobytes()).hexdigest(),
+        neighbors_sha256=hashlib.sha256(nb.tobytes()).hexdigest(),
+        host_cpus=os.cpu_count())
+    with open(path, 'w') as f:
+        json.dump(meta, f, indent=1)
+    print(json.dumps(meta['pipeline_cells1m'], indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--reference', default='/root/reference')
@@ -359,6 +398,8 @@ def main():
                     help='only add the neighbour-search fixtures (ref_knn.npz)')
     ap.add_argument('--knn-large', action='store_true',
                     help='only add the 1M-cell neighbour fixture and timing')
+    ap.add_argument('--pipeline1m', action='store_true',
+                    help='only add the 1M-cell normalize -> pca -> neighbors fixture')
     args = ap.parse_args()
     build_reference(args.reference)
     pca_mod, norm_mod = import_reference()
@@ -368,6 +409,8 @@ def main():
         return knn_fixtures()
     if args.knn_large:
         return knn_large()
+    if args.pipeline1m:
+        return pipeline1m(pca_mod, norm_mod)
 
     tiny = {}
     meta = {'tiny': [], 'large': None,

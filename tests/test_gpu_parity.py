@@ -531,3 +531,25 @@ def test_device_subset_rejects_bad_input():
     good = (d[0], torch.from_numpy(X.indices).cuda(), d[2])
     with pytest.raises(ValueError, match='out of range'):
         csr_subset(*good, 50, rows=np.array([0, 200]))
+
+
+def test_restart_product_on_device_arrays_matches_numpy():
+    # bpca_test_gemm_device (the restart / final product on device arrays)
+    # against NumPy in FP64, ragged sizes (rows not a multiple of the 16-row
+    # tile or the 1024-row block, widths not multiples of 4 or 16).
+    import ctypes
+    torch = _torch()
+    from brisc_amd import _lib, default_context
+    rng = np.random.default_rng(3)
+    for n, m, kout in [(1, 1, 1), (3000, 37, 19), (5000, 100, 99), (2049, 128, 128)]:
+        A = rng.standard_normal((n, m))
+        C = rng.standard_normal((m, kout))
+        dA = torch.from_numpy(np.asfortranarray(A).ravel(order='F')).cuda()
+        dC = torch.from_numpy(np.asfortranarray(C).ravel(order='F')).cuda()
+        dO = torch.empty(n * kout, dtype=torch.float64, device='cuda')
+        _lib.check(_lib.lib().bpca_test_gemm_device(
+            default_context(0).handle, n, m, kout, dA.data_ptr(),
+            dC.data_ptr(), dO.data_ptr()))
+        out = dO.cpu().numpy().reshape((kout, n)).T
+        ref = A @ C
+        assert np.abs(out - ref).max() <= 1e-12 * np.abs(ref).max() * m
