@@ -7,6 +7,7 @@ Run in the development container only (it reads the reference sources):
     python tests/golden/make_ref_vectors.py --knn       (adds the neighbour fixtures)
     python tests/golden/make_ref_vectors.py --knn-large (adds the 1M-cell neighbour fixture)
     python tests/golden/make_ref_vectors.py --pipeline1m (adds the 1M-cell pipeline fixture)
+    python tests/golden/make_ref_vectors.py --knn-approx (adds few-clusters-searched fixtures)
 
 It cythonizes ``brisc/cyutils.pyx``, ``brisc/pca.pyx``,
 ``brisc/normalize.pyx``, ``brisc/kmeans.pyx`` and ``brisc/knn.pyx`` from the
@@ -171,14 +172,15 @@ def import_knn():
 
 
 def ref_neighbors(kmeans_mod, knn_mod, X, k, num_clusters_searched=None,
-                  num_threads=8, seed=0):
+                  num_threads=8, seed=0, num_clusters=None):
     """The steps of SingleCell.neighbors() (single_cell.py:12452-12650) with
     its defaults, on PCs ``X``; ``num_clusters_searched='all'`` searches every
     cluster (the exhaustive search).  Returns neighbours (uint32), squared
     distances (float32) and the number of clusters."""
     from threadpoolctl import threadpool_limits
     n, d = X.shape
-    C = max(1, int(np.ceil(min(4 * np.sqrt(n), n / 100))))
+    C = max(1, int(np.ceil(min(4 * np.sqrt(n), n / 100)))) \
+        if num_clusters is None else num_clusters
     labels = np.zeros(n, np.uint32)
     min_dist = np.zeros(n, np.float32)
     norms = np.zeros(n, np.float32)
@@ -202,7 +204,8 @@ def ref_neighbors(kmeans_mod, knn_mod, X, k, num_clusters_searched=None,
         remap[keep] = np.arange(keep.sum(), dtype=np.uint32)
         labels = remap[labels]
     C = len(cent)
-    searched = C if num_clusters_searched == 'all' else min(64, C)
+    searched = C if num_clusters_searched == 'all' else \
+        min(64, C) if num_clusters_searched is None else num_clusters_searched
     smallest = np.sort(counts).cumsum(dtype=np.uint64)
     searched = min(max(searched, int(np.searchsorted(smallest, k + 1)) + 1), C)
     corder = np.argsort(cent[:, 0]).astype(np.uint32)
@@ -320,6 +323,48 @@ def knn_fixtures():
 
 KNN_LARGE = ('pcs1m', 34, 1_000_000, 50, 100, 20, 1.0)
 
+# Approximate searches with non-default settings on the 50k-cell set:
+# (name, num_clusters, num_clusters_searched, k, seed)
+KNN_APPROX = [
+    ('approx_c100_s8', 100, 8, 20, 0),
+    ('approx_c300_s3', 300, 3, 10, 7),
+]
+
+
+def knn_approx():
+    """The reference's search with few clusters searched (recall well below
+    1): a 4096-row sample of its neighbours and the SHA-256 of all of them,
+    for parity of the approximate path."""
+    kmeans_mod, knn_mod = import_knn()
+    path = os.path.join(HERE, 'ref_vectors.json')
+    with open(path) as f:
+        meta = json.load(f)
+    base = {m['name']: m for m in meta['knn']}['knn_c']
+    X = helpers.knn_points(base['seed'], base['num_cells'], base['num_dims'],
+                           base['centers'])
+    arrays, out = {}, []
+    for (name, C, searched, k, seed) in KNN_APPROX:
+        nb, dist, Ck = ref_neighbors(kmeans_mod, knn_mod, X, k, searched,
+                                     seed=seed, num_clusters=C)
+        exact_i, _ = oracle.knn_fp64(X, k)
+        recall = float(np.mean([len(set(a) & set(b)) / k for a, b in
+                                zip(nb.tolist(), exact_i.tolist())]))
+        import hashlib
+        rows = np.random.Generator(np.random.PCG64(4)).choice(
+            len(X), 4096, replace=False)
+        rows.sort()
+        arrays[f'{name}_rows'] = rows.astype(np.uint32)
+        arrays[f'{name}_neighbors'] = nb[rows].astype(np.uint16)
+        out.append(dict(name=name, base='knn_c', num_clusters=C,
+                        num_clusters_searched=searched, k=k, seed=seed,
+                        clusters_kept=Ck, recall_vs_fp64=recall,
+                        neighbors_sha256=hashlib.sha256(nb.tobytes()).hexdigest()))
+        print(json.dumps(out[-1]), flush=True)
+    np.savez_compressed(os.path.join(HERE, 'ref_knn_approx.npz'), **arrays)
+    meta['knn_approx'] = out
+    with open(path, 'w') as f:
+        json.dump(meta, f, indent=1)
+
 
 def knn_large():
     """The reference's default neighbour search on the 1M-cell PC mixture
@@ -400,6 +445,8 @@ def main():
                     help='only add the 1M-cell neighbour fixture and timing')
     ap.add_argument('--pipeline1m', action='store_true',
                     help='only add the 1M-cell normalize -> pca -> neighbors fixture')
+    ap.add_argument('--knn-approx', action='store_true',
+                    help='only add the few-clusters-searched neighbour fixtures')
     args = ap.parse_args()
     build_reference(args.reference)
     pca_mod, norm_mod = import_reference()
@@ -411,6 +458,8 @@ def main():
         return knn_large()
     if args.pipeline1m:
         return pipeline1m(pca_mod, norm_mod)
+    if args.knn_approx:
+        return knn_approx()
 
     tiny = {}
     meta = {'tiny': [], 'large': None,

@@ -240,3 +240,37 @@ def test_device_chain_pca_then_neighbors_one_million_cells():
     assert stats['converged']
     assert overlap >= 0.97
     assert np.median(ratio) <= 1.001 and ratio.max() <= 1.02
+
+
+@pytest.mark.parametrize('name', ['approx_c100_s8', 'approx_c300_s3'])
+def test_ivf_matches_reference_with_few_clusters_searched(name):
+    # Non-default settings where the reference's search is far from exact
+    # (3 of 300 clusters searched: recall 0.84): the device must reproduce
+    # the reference's approximate answer, not the exact one.
+    from brisc_amd.knn import knn_exact, knn_ivf
+    with open(os.path.join(HERE, 'ref_vectors.json')) as f:
+        meta = {m['name']: m for m in json.load(f)['knn_approx']}[name]
+    base, X, _ = _fixture(meta['base'])
+    ref = np.load(os.path.join(HERE, 'ref_knn_approx.npz'))
+    rows = ref[f'{name}_rows'].astype(np.int64)
+    ref_nb = ref[f'{name}_neighbors'].astype(np.int64)
+    k = meta['k']
+    Xd = _dev(X)
+    nb, dist, info = knn_ivf(Xd, k, num_clusters=meta['num_clusters'],
+                             num_clusters_searched=meta['num_clusters_searched'],
+                             seed=meta['seed'])
+    nb = nb.cpu().numpy().astype(np.int64)
+    ex, _ = knn_exact(Xd, k)
+    ex = ex.cpu().numpy().astype(np.int64)
+    mine = nb[rows]
+    same = float((np.sort(mine, 1) == np.sort(ref_nb, 1)).all(1).mean())
+    rec_gpu = _recall(nb, ex, k)
+    rec_ref_sample = _recall(ref_nb, ex[rows], k)
+    rec_gpu_sample = _recall(mine, ex[rows], k)
+    print(f'{name}: {info}; sampled rows with the reference\'s set {same:.4f}; '
+          f'recall vs exact: device {rec_gpu:.5f} (sample {rec_gpu_sample:.5f}), '
+          f'reference {meta["recall_vs_fp64"]:.5f} (sample {rec_ref_sample:.5f})')
+    assert info['num_clusters'] == meta['clusters_kept']
+    assert info['num_clusters_searched'] == meta['num_clusters_searched']
+    assert same >= 0.97
+    assert abs(rec_gpu - meta['recall_vs_fp64']) <= 0.005
