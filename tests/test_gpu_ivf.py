@@ -274,3 +274,31 @@ def test_ivf_matches_reference_with_few_clusters_searched(name):
     assert info['num_clusters_searched'] == meta['num_clusters_searched']
     assert same >= 0.97
     assert abs(rec_gpu - meta['recall_vs_fp64']) <= 0.005
+
+
+@pytest.mark.parametrize('n,d,k,centers', [
+    (20000, 128, 64, 40), (20000, 7, 5, 30), (5000, 1, 3, 10),
+    (30001, 50, 20, 200)])
+def test_ivf_shapes_against_exact_search(n, d, k, centers):
+    # Widths at the edges of the tile engine (1 and 128 dimensions, the
+    # 64-entry lists) and a ragged cell count, in the inverted-file mode:
+    # structure exact, recall against the exhaustive search high, distances
+    # of shared pairs bit-identical to the exhaustive search's.
+    from brisc_amd.knn import knn_exact, knn_ivf
+    X = helpers.knn_points(n + d, n, d, centers, spread=0.7)
+    Xd = _dev(X)
+    nb, dist, info = knn_ivf(Xd, k)
+    ex, exd = knn_exact(Xd, k)
+    nb, dist = nb.cpu().numpy().astype(np.int64), dist.cpu().numpy()
+    ex, exd = ex.cpu().numpy().astype(np.int64), exd.cpu().numpy()
+    assert nb.min() >= 0 and nb.max() < n
+    assert not (nb == np.arange(n)[:, None]).any()
+    for row in nb[:2000]:
+        assert np.unique(row).size == k
+    dd, ii = dist[:, 1:] - dist[:, :-1], nb[:, 1:] - nb[:, :-1]
+    assert ((dd > 0) | ((dd == 0) & (ii > 0))).all()
+    rec = _recall(nb, ex, k)
+    hit = nb == ex
+    print(f'n={n} d={d} k={k}: {info}; recall {rec:.5f}')
+    assert rec >= 0.95
+    assert np.array_equal(dist[hit].view(np.uint32), exd[hit].view(np.uint32))
