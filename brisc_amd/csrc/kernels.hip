@@ -397,12 +397,17 @@ __global__ __launch_bounds__(256) void k_spmv(Tables T) {
 
 // ============================================ reorthogonalisation (FP64)
 //
-// Block partials: lane l of a wave handles rows l, l+64, ..., l+960 of the
-// data block in that order, then the xor butterfly.  P[c*nb + b]: c = 0 the
-// block sum of u, c >= 1 the dot of U[:, c-1] with u.
+// Block partials: lane l of a wave handles the row pairs (2l + 128t,
+// 2l + 128t + 1), t = 0..7, of the data block in that order -- one 16-byte
+// load per pair when the block is full and n is even (every column of U then
+// starts 16-byte aligned), element loads otherwise, same order -- then the
+// xor butterfly.  P[c*nb + b]: c = 0 the block sum of u, c >= 1 the dot of
+// U[:, c-1] with u.
+
+typedef double v2d __attribute__((ext_vector_type(2)));
 
 __global__ __launch_bounds__(256) void k_reorth_partials(Tables T) {
-    constexpr int R = kRowBlock / kWave;  // 16 rows per lane
+    constexpr int R = kRowBlock / (2 * kWave);  // 8 row pairs per lane
     const int2 bp = T.blocks[blockIdx.x];
     Prob &pr = T.probs[bp.x];
     if (pr.phase != PH_LANCZOS) return;
@@ -410,27 +415,43 @@ __global__ __launch_bounds__(256) void k_reorth_partials(Tables T) {
     const int w = wave_id();
     const int64_t n = pr.n, b = bp.y, r0 = b * kRowBlock;
     const int64_t nrows = n - r0 < kRowBlock ? n - r0 : kRowBlock;
+    const bool vec = nrows == kRowBlock && (n & 1) == 0;
     const int j = pr.j;
     const float *u32 = pr.u32;
     const double *U = pr.U[pr.ubuf];
-    double uv[R];
+    double ua[R], ub[R];
 #pragma unroll
     for (int t = 0; t < R; t++) {
-        const int r = lane + t * kWave;
-        uv[t] = r < nrows ? (double)u32[r0 + r] : 0.0;
+        const int r = 2 * lane + t * 2 * kWave;
+        ua[t] = r < nrows ? (double)u32[r0 + r] : 0.0;
+        ub[t] = r + 1 < nrows ? (double)u32[r0 + r + 1] : 0.0;
     }
     for (int c = w; c <= j; c += 4) {
         double acc = 0.0;
         if (c == 0) {
 #pragma unroll
-            for (int t = 0; t < R; t++) acc += uv[t];
+            for (int t = 0; t < R; t++) {
+                acc += ua[t];
+                acc += ub[t];
+            }
         } else {
             const double *Uc = U + (int64_t)(c - 1) * n + r0;
+            if (vec) {
 #pragma unroll
-            for (int t = 0; t < R; t++) {
-                const int r = lane + t * kWave;
-                const double x = r < nrows ? Uc[r] : 0.0;
-                acc = fma(x, uv[t], acc);
+                for (int t = 0; t < R; t++) {
+                    const v2d x = *(const v2d *)(Uc + 2 * lane + t * 2 * kWave);
+                    acc = fma(x.x, ua[t], acc);
+                    acc = fma(x.y, ub[t], acc);
+                }
+            } else {
+#pragma unroll
+                for (int t = 0; t < R; t++) {
+                    const int r = 2 * lane + t * 2 * kWave;
+                    const double x0 = r < nrows ? Uc[r] : 0.0;
+                    const double x1 = r + 1 < nrows ? Uc[r + 1] : 0.0;
+                    acc = fma(x0, ua[t], acc);
+                    acc = fma(x1, ub[t], acc);
+                }
             }
         }
         acc = wave_sum(acc);
@@ -454,8 +475,10 @@ __global__ __launch_bounds__(256) void k_reduce_columns(Tables T) {
 }
 
 // U[:, j] = u - (mean + sum_c U[:, c] red[c+1]) with the correction summed
-// over c ascending; thread t handles rows r0+t, r0+t+256, ... in order and
-// the block's sum of squares is block_sum<4> of the thread sums.
+// over c ascending; thread t handles the row pairs (r0 + 2t + 512i,
+// r0 + 2t + 512i + 1), i = 0, 1 (16-byte loads and stores when the block is
+// full and n is even), and the block's sum of squares is block_sum<4> of the
+// thread sums (rows in that order).
 __global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
     extern __shared__ double coef[];
     __shared__ double tmp[4];
@@ -472,15 +495,32 @@ __global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
     double *Uj = U + (int64_t)j * n;
     const float *u32 = pr.u32;
     double acc = 0.0;
+    const bool vec = r0 + kRowBlock <= n && (n & 1) == 0;
 #pragma unroll
-    for (int t = 0; t < kRowBlock / 256; t++) {
-        const int64_t r = r0 + threadIdx.x + t * 256;
-        if (r < n) {
-            double d = mean;
-            for (int c = 0; c < j; c++) d = fma(U[(int64_t)c * n + r], coef[c + 1], d);
-            const double x = (double)u32[r] - d;
-            Uj[r] = x;
-            acc = fma(x, x, acc);
+    for (int t = 0; t < kRowBlock / 512; t++) {
+        const int64_t r = r0 + 2 * threadIdx.x + t * 512;
+        if (vec) {
+            double d0 = mean, d1 = mean;
+            for (int c = 0; c < j; c++) {
+                const v2d x = *(const v2d *)(U + (int64_t)c * n + r);
+                d0 = fma(x.x, coef[c + 1], d0);
+                d1 = fma(x.y, coef[c + 1], d1);
+            }
+            const double x0 = (double)u32[r] - d0, x1 = (double)u32[r + 1] - d1;
+            *(v2d *)(Uj + r) = v2d{x0, x1};
+            acc = fma(x0, x0, acc);
+            acc = fma(x1, x1, acc);
+        } else {
+#pragma unroll
+            for (int e = 0; e < 2; e++) {
+                if (r + e < n) {
+                    double d = mean;
+                    for (int c = 0; c < j; c++) d = fma(U[(int64_t)c * n + r + e], coef[c + 1], d);
+                    const double x = (double)u32[r + e] - d;
+                    Uj[r + e] = x;
+                    acc = fma(x, x, acc);
+                }
+            }
         }
     }
     acc = block_sum<4>(acc, tmp);
