@@ -327,7 +327,7 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
     const int d = P.d, k = P.k;
     int C = P.num_clusters;
     const int st = knn_mfma_stride(d);
-    const int ksteps = (d + 3) / 4;
+    const int ksteps = knn_mfma_ksteps(d);
     const int64_t pad = kKnnPadRows;
 
     // Cell norms (kept for the search), the padded cells.
@@ -335,7 +335,7 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
     IVF_TRY(norms.alloc(n + pad));
     IVF_TRY(Xp.alloc((n + pad) * st));
     IVF_TRY(launch_row_norms(X, n, d, norms.p, s));
-    IVF_TRY(launch_knn_pack(X, nullptr, n, d, Xp.p, nullptr, norms.p, s));
+    IVF_TRY(launch_knn_pack(X, nullptr, n, d, norms.p, Xp.p, s));
 
     // 1. k-means.
     std::vector<uint32_t> init = kmeans_random_init_cells(n, C, P.seed);
@@ -380,8 +380,8 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
     KnnTiles E{};
     E.Q = Xp.p;
     E.C = centp.p;
-    E.qnorm = nullptr;
-    E.cnorm = cnorm.p;
+    E.qnorm = 0;
+    E.d = d;
     E.nq = n;
     E.nc = C;
     E.st = st;
@@ -405,8 +405,8 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
     for (;;) {
         // E-step against the current centroids.
         IVF_TRY(launch_row_norms(cent.p, C, d, cnorm.p, s));
-        IVF_TRY(launch_knn_pack(cent.p, nullptr, C, d, centp.p, nullptr, cnorm.p, s));
-        IVF_TRY(launch_knn_tiles(E, d, false, s));
+        IVF_TRY(launch_knn_pack(cent.p, nullptr, C, d, cnorm.p, centp.p, s));
+        IVF_TRY(launch_knn_tiles(E, false, s));
         // M-step: counts, the stable order by label, sums in the reference's order.
         IVF_TRY(hipMemsetAsync(counts.p, 0, sizeof(uint32_t) * C, s));
         k_count_labels<<<grid_of(n), 256, 0, s>>>(labels.p, n, counts.p);
@@ -483,8 +483,8 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
     // A last E-step against the final centroids gives the labels and the
     // counts (kmeans.pyx:1357-1432).
     IVF_TRY(launch_row_norms(cent.p, C, d, cnorm.p, s));
-    IVF_TRY(launch_knn_pack(cent.p, nullptr, C, d, centp.p, nullptr, cnorm.p, s));
-    IVF_TRY(launch_knn_tiles(E, d, false, s));
+    IVF_TRY(launch_knn_pack(cent.p, nullptr, C, d, cnorm.p, centp.p, s));
+    IVF_TRY(launch_knn_tiles(E, false, s));
     IVF_TRY(hipMemsetAsync(counts.p, 0, sizeof(uint32_t) * C, s));
     k_count_labels<<<grid_of(n), 256, 0, s>>>(labels.p, n, counts.p);
     IVF_TRY(hipGetLastError());
@@ -540,7 +540,8 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
         info->exhaustive = 1;
         KnnTiles T{};
         T.Q = T.C = Xp.p;
-        T.qnorm = T.cnorm = norms.p;
+        T.qnorm = 1;
+        T.d = d;
         T.nq = T.nc = n;
         T.st = st;
         T.ksteps = ksteps;
@@ -548,7 +549,7 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
         T.self = 1;
         T.out_idx = out_idx;
         T.out_dist = out_dist;
-        IVF_TRY(launch_knn_tiles(T, d, false, s));
+        IVF_TRY(launch_knn_tiles(T, false, s));
         IVF_TRY(hipStreamSynchronize(s));
         return 0;
     }
@@ -563,14 +564,13 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
     k_gather_rows<<<grid_of((int64_t)Ck * d), 256, 0, s>>>(cent.p, d, rows.p, Ck, sums.p);
     IVF_TRY(hipGetLastError());
     IVF_TRY(launch_row_norms(sums.p, Ck, d, cnorm.p, s));
-    IVF_TRY(launch_knn_pack(sums.p, nullptr, Ck, d, centp.p, nullptr, cnorm.p, s));
+    IVF_TRY(launch_knn_pack(sums.p, nullptr, Ck, d, cnorm.p, centp.p, s));
     IVF_TRY(sort_by_label());
 
-    // Cells in cluster order: padded rows and norms.
-    Buf<float> Xs, norms_s;
+    // Cells in cluster order, padded.
+    Buf<float> Xs;
     IVF_TRY(Xs.alloc((n + pad) * st));
-    IVF_TRY(norms_s.alloc(n + pad));
-    IVF_TRY(launch_knn_pack(X, order.p, n, d, Xs.p, norms.p, norms_s.p, s));
+    IVF_TRY(launch_knn_pack(X, order.p, n, d, norms.p, Xs.p, s));
     Xp.reset();  // the unsorted copy is no longer needed
 
     // 3. Each cell's nearest clusters (by partial distance).
@@ -582,8 +582,8 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
         KnnTiles R{};
         R.Q = Xs.p;
         R.C = centp.p;
-        R.qnorm = nullptr;
-        R.cnorm = cnorm.p;
+        R.qnorm = 0;
+        R.d = d;
         R.nq = n;
         R.nc = Ck;
         R.st = st;
@@ -593,7 +593,7 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
         R.start_hint = labels_sorted.p;  // the block's own cluster first
         R.out_idx = near.p;
         R.out_dist = near_d.p;
-        IVF_TRY(launch_knn_tiles(R, d, false, s));
+        IVF_TRY(launch_knn_tiles(R, false, s));
     }
     near_d.reset();
 
@@ -622,7 +622,8 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
     // 5. The search.
     KnnTiles T{};
     T.Q = T.C = Xs.p;
-    T.qnorm = T.cnorm = norms_s.p;
+    T.qnorm = 1;
+    T.d = d;
     T.nq = T.nc = n;
     T.st = st;
     T.ksteps = ksteps;
@@ -635,7 +636,7 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
     T.ids = order.p;
     T.out_idx = out_idx;
     T.out_dist = out_dist;
-    IVF_TRY(launch_knn_tiles(T, d, true, s));
+    IVF_TRY(launch_knn_tiles(T, true, s));
     IVF_TRY(hipStreamSynchronize(s));
     return 0;
 }

@@ -181,31 +181,32 @@ hipError_t launch_knn_exact(const float *X, const float *norms, int64_t n,
 size_t knn_lds_bytes(int d, int kmax);
 int knn_kmax(int k);
 // Matrix-core path for num_dims <= kKnnMfmaMaxDims: Xp is scratch of
-// (n + kKnnPadRows) * knn_mfma_stride(d) floats (the padded copy of X) and
-// norms holds n + kKnnPadRows floats (the pad is zeroed here).
+// (n + kKnnPadRows) * knn_mfma_stride(d) floats (the padded copy of X);
+// norms (n floats) from launch_row_norms.
 constexpr int kKnnMfmaMaxDims = 128;
 constexpr int kKnnPadRows = 64;
 constexpr int64_t kKnnMaxRows = (1ll << 31) - 1 - kKnnPadRows;  // 32-bit row positions
-int knn_mfma_stride(int d);
+int knn_mfma_ksteps(int d);   // ceil((d + 2) / 4): columns x, 1, ||x||^2
+int knn_mfma_stride(int d);   // 4 ksteps + 4 floats
 size_t knn_mfma_lds_bytes(int d);
-hipError_t launch_knn_mfma(const float *X, float *norms, int64_t n,
+hipError_t launch_knn_mfma(const float *X, const float *norms, int64_t n,
                            int d, int k, float *Xp, uint32_t *out_idx,
                            float *out_dist, hipStream_t s);
 
 // The tile engine behind launch_knn_mfma, for the inverted-file search and
 // the centroid rankings of k-means (knn.hip).  Q and C are padded copies
-// (launch_knn_pack: row stride knn_mfma_stride(d), kKnnPadRows zero rows
-// after the last; norms likewise padded).  Query q's k best candidates go
+// (launch_knn_pack: row [x, 1, ||x||^2, 0 ...], stride knn_mfma_stride(d),
+// kKnnPadRows zero rows after the last).  Query q's k best candidates go
 // to row q (inverted file: ids[q]) of out_idx / out_dist, ascending by
 // (distance, id); a candidate's id is c (inverted file: ids[c]).  ids is
 // read by inverted-file launches only.
 struct KnnTiles {
     const float *Q;        // padded queries
     const float *C;        // padded candidates
-    const float *qnorm;    // query norms, or null: partial distances ||c||^2 - 2 q.c
-    const float *cnorm;    // candidate norms
+    int qnorm;             // 1: ||q||^2 + ||c||^2 - 2 q.c; 0: partial ||c||^2 - 2 q.c
+    int d;                 // dimensions
     int64_t nq, nc;
-    int st, ksteps, k;     // row stride, ceil(d / 4), neighbours (<= 64)
+    int st, ksteps, k;     // knn_mfma_stride(d), knn_mfma_ksteps(d), neighbours (<= 64)
     int self;              // queries are the candidates: skip q == c
     // Inverted file (ivf launches): workgroup b (queries 64b..64b+63) scans
     // entries [wg_off[b], wg_off[b + 1]); entry e covers candidate rows
@@ -222,10 +223,10 @@ struct KnnTiles {
     uint32_t *out_idx;
     float *out_dist;
 };
+// Row c of Xp = [x_src, 1, norms[src], 0 ...], src = order ? order[c] : c.
 hipError_t launch_knn_pack(const float *X, const uint32_t *order, int64_t n,
-                           int d, float *Xp, const float *norms_in,
-                           float *norms_out, hipStream_t s);
-hipError_t launch_knn_tiles(const KnnTiles &T, int d, bool ivf, hipStream_t s);
+                           int d, const float *norms, float *Xp, hipStream_t s);
+hipError_t launch_knn_tiles(const KnnTiles &T, bool ivf, hipStream_t s);
 
 // SingleCell.neighbors() with its inverted-file search (ivf.hip).
 struct IvfParams {

@@ -164,15 +164,19 @@ __global__ __launch_bounds__(256) void k_knn_exact(
 // Matrix-core path (num_dims <= 128): distances from FP32 MFMA tiles,
 // candidates buffered per query in LDS, lists merged with a bitonic network.
 //
-// The points are first copied to a padded row-major layout (row stride st =
-// round_up(d, 4) + 4 floats, zero padding) so that a 64-cell tile is a run
-// of 16-byte loads.  A workgroup of four waves takes 64 queries (wave w owns
-// rows 16w..16w+15) and streams the cells in tiles of 64 through LDS (double
-// buffered).  Wave w forms its 16 x 64 block of dot products with
+// The points are first copied to a padded row-major layout, row c being
+// [x_c, 1, ||x_c||^2, 0 ...] (kdim = round_up(d + 2, 4) columns used, row
+// stride st = kdim + 4 floats) so that a 64-cell tile is a run of 16-byte
+// loads.  A workgroup of four waves takes 64 queries (wave w owns rows
+// 16w..16w+15) and streams the cells in tiles of 64 through LDS (double
+// buffered).  In LDS a query row becomes [-2 x_q, ||x_q||^2, 1, 0 ...] (or
+// [-2 x_q, 0, 1, 0 ...] for the partial distance ||c||^2 - 2 q.c), so one
+// product gives the distance: ||q||^2 + ||c||^2 - 2 q.c with all terms
+// summed in the matrix core.  Wave w forms its 16 x 64 block with
 // v_mfma_f32_16x16x4f32 (four 16 x 16 blocks; lane l holds rows
 // 4(l/16)..4(l/16)+3 of column l%16 of each block); within a group of 16
-// dimensions lane group g supplies dimensions 4g..4g+3, one per MFMA, to both
-// operands, so every dimension enters each dot product once.
+// columns lane group g supplies columns 4g..4g+3, one per MFMA, to both
+// operands, so every column enters each product once.
 //
 // Selection: each query row keeps its 64 best (distance, index) pairs, sorted,
 // one per lane, in registers; a candidate that beats the row's k-th entry is
@@ -271,23 +275,22 @@ __device__ __forceinline__ void merge_row(float (&Ld)[16], uint32_t (&Li)[16],
 
 }  // namespace
 
-// Padded gather: row c of Xp is row (order ? order[c] : c) of X (n x d),
-// zero padded to st floats, and the kTile rows past n are zero; likewise
-// norms_out[c] = norms_in[order[c]] (when both are given) with a zero pad.
+// Padded gather: row c of Xp is [x, 1, norms[src], 0 ...] for source row
+// src = order ? order[c] : c of X (n x d), with kTile zero rows past n.
 __global__ __launch_bounds__(256) void k_knn_pack(const float *__restrict__ X,
                                                   const uint32_t *__restrict__ order,
                                                   int64_t n, int d, int st,
-                                                  float *__restrict__ Xp,
-                                                  const float *__restrict__ norms_in,
-                                                  float *__restrict__ norms_out) {
+                                                  const float *__restrict__ norms,
+                                                  float *__restrict__ Xp) {
     const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
-    if (norms_out && e < n + kTile)
-        norms_out[e] = e < n ? (norms_in ? norms_in[order ? order[e] : e] : norms_out[e]) : 0.0f;
     if (e >= (n + kTile) * st) return;
     const int64_t c = e / st;
     const int j = (int)(e - c * st);
     float v = 0.0f;
-    if (c < n && j < d) v = X[(order ? (int64_t)order[c] : c) * d + j];
+    if (c < n) {
+        const int64_t src = order ? (int64_t)order[c] : c;
+        v = j < d ? X[src * d + j] : j == d ? 1.0f : j == d + 1 ? norms[src] : 0.0f;
+    }
     Xp[e] = v;
 }
 
@@ -299,8 +302,9 @@ typedef __attribute__((address_space(3))) void lds_void;
 // to the four waves in turn.  Completion is waited for with vmcnt.
 __device__ __forceinline__ void dma_run(const float *src, float *dst, int chunks,
                                         int w, int lane) {
+    const char *base = (const char *)src;  // wave-uniform; lanes add a 32-bit offset
     for (int c = w; c < chunks; c += 4)
-        __builtin_amdgcn_global_load_lds(src + 4 * (kWave * c + lane),
+        __builtin_amdgcn_global_load_lds(base + (uint32_t)(16 * (kWave * c + lane)),
                                          (lds_void *)(dst + 4 * kWave * c), 16, 0, 0);
 }
 
@@ -310,15 +314,14 @@ __device__ __forceinline__ void dma_run(const float *src, float *dst, int chunks
 // ROT (exhaustive scans with a start hint: the centroid ranking): the scan
 // starts at the tile holding the block's own cluster and wraps, so the
 // thresholds are tight from the first tiles; ids then arrive out of order.
-template <bool IVF, bool ARGMIN, bool ROT>
+template <bool IVF, bool ARGMIN, bool ROT, int KS>
 __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     extern __shared__ __align__(16) float lds_f[];
     const int st = T.st;
     const int chunks = st >> 2;                   // 64 rows x st floats
     float *qs = lds_f;                            // 64 x st queries
     float *cs0 = qs + kTile * st;                 // 2 x (64 x st) candidates
-    float *cn0 = cs0 + 2 * kTile * st;            // 2 x 64 candidate norms
-    uint32_t *cid0 = (uint32_t *)(cn0 + 2 * kTile);  // 2 x 64 candidate ids
+    uint32_t *cid0 = (uint32_t *)(cs0 + 2 * kTile * st);  // 2 x 64 candidate ids
     float *bufd = (float *)(cid0 + 2 * kTile);    // 64 rows x kBuf
     uint32_t *bufi = (uint32_t *)(bufd + kTile * kBuf);
     const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid >> 6;
@@ -352,27 +355,20 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
             mask = T.ent_mask[e];
         }
     }
-    // Tile rows p .. p + 63 (pad rows and pad norms keep every read in
-    // bounds; ids are clamped, and their rows are masked) into buffer b.
+    // Tile rows p .. p + 63 (pad rows keep every read in bounds; ids are
+    // clamped, and their rows are masked) into buffer b.
     auto stage = [&](int p, int b) {
         dma_run(T.C + (size_t)p * st, cs0 + b * (kTile * st), chunks, w, lane);
-        if (w == 0)
-            __builtin_amdgcn_global_load_lds(T.cnorm + p + lane,
-                                             (lds_void *)(cn0 + b * kTile), 4, 0, 0);
-        if (IVF && w == 1)
+        if (IVF && w == 0)
             __builtin_amdgcn_global_load_lds(T.ids + min(p + lane, nc - 1),
                                              (lds_void *)(cid0 + b * kTile), 4, 0, 0);
     };
     dma_run(T.Q + (size_t)q0 * st, qs, chunks, w, lane);
     if (valid) stage(pos, 0);
 
-    float qn[4];
     int qid[4];
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
-        qid[i] = q0 + 16 * w + 4 * g + i;
-        qn[i] = T.qnorm ? T.qnorm[qid[i] < nq ? qid[i] : nq] : 0.0f;
-    }
+    for (int i = 0; i < 4; i++) qid[i] = q0 + 16 * w + 4 * g + i;
     float Ld[16];
     uint32_t Li[16];
 #pragma unroll
@@ -399,11 +395,24 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     const uint32_t below = (1u << m) - 1u;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
+    {
+        // The wave's 16 query rows [x, 1, n, 0 ...] become [-2 x, n or 0, 1,
+        // 0 ...].  Each lane reads its row's norm before any lane writes.
+        const int d = T.d;
+        float *myq = qs + 16 * w * st;
+        const float nrm = lane < 16 ? myq[lane * st + d + 1] : 0.0f;
+        for (int x = lane; x < 16 * st; x += kWave) {
+            const int r = x / st, j = x - r * st;
+            const float v = myq[x];
+            const float nr = __shfl(nrm, r);
+            myq[x] = j < d ? -2.0f * v : j == d ? (T.qnorm ? nr : 0.0f) : j == d + 1 ? 1.0f : 0.0f;
+        }
+    }
+    __syncthreads();
 
     int t = 0;
     while (valid) {
         const float *cs = cs0 + t * (kTile * st);
-        const float *cn = cn0 + t * kTile;
         const uint32_t *cid = cid0 + t * kTile;
         int npos = pos + kTile, nlim = lim;
         int64_t ne = e;
@@ -433,96 +442,181 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         // with none skips the tile (it still stages and meets the barrier).
         const uint32_t wmask = IVF ? (uint32_t)((mask >> (16 * w)) & 0xffffull) : 0xffffu;
         if (wmask != 0) {
-        // 16 x 64 dot products of this wave's queries with the tile.
+        // The wave's 16 x 64 block of distances.
         knn_f4 acc[4];
-#pragma unroll
-        for (int b = 0; b < 4; b++) acc[b] = {0.0f, 0.0f, 0.0f, 0.0f};
-        const float *qrow = qs + (16 * w + m) * st;
-        const float *crow = cs + m * st;
-        for (int gg = 0; gg < groups; gg++) {
-            const knn_f4 a = *(const knn_f4 *)(qrow + 16 * gg + 4 * g);
-            knn_f4 bv[4];
-#pragma unroll
-            for (int b = 0; b < 4; b++)
-                bv[b] = *(const knn_f4 *)(crow + 16 * b * st + 16 * gg + 4 * g);
-#pragma unroll
-            for (int s = 0; s < 4; s++)
+        if constexpr (KS > 0) {
+            // k-steps known at compile time: straight-line code in which the
+            // next group's (or the tail's) operands are read from LDS before
+            // the current group's products are issued; the scheduling
+            // barriers keep that order, so the reads complete under the
+            // products.  Two operand sets alternate.
+            constexpr int G = KS / 4, TL = KS % 4;
+            const float *qrow = qs + (16 * w + m) * st + 4 * g;
+            const float *crow = cs + m * st + 4 * g;
+            knn_f4 sa[2], sb[2][4];
+            float ta[TL > 0 ? TL : 1], tb[TL > 0 ? TL : 1][4];
+            auto load_group = [&](int gg, int set) {
+                sa[set] = *(const knn_f4 *)(qrow + 16 * gg);
 #pragma unroll
                 for (int b = 0; b < 4; b++)
-                    acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], bv[b][s], acc[b], 0, 0, 0);
-        }
-        for (int s = 0; s < tail; s++) {
-            const int dim = 16 * groups + 4 * s + g;
-            const float a = qrow[dim];
+                    sb[set][b] = *(const knn_f4 *)(crow + 16 * b * st + 16 * gg);
+            };
+            auto load_tail = [&]() {
 #pragma unroll
-            for (int b = 0; b < 4; b++)
-                acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, crow[16 * b * st + dim], acc[b], 0, 0, 0);
+                for (int s = 0; s < TL; s++) {
+                    const int off = 16 * G + 4 * s - 3 * g;
+                    ta[s] = qrow[off];
+#pragma unroll
+                    for (int b = 0; b < 4; b++) tb[s][b] = crow[16 * b * st + off];
+                }
+            };
+            const knn_f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+            if constexpr (G > 0) load_group(0, 0);
+            else load_tail();
+#pragma unroll
+            for (int gg = 0; gg < G; gg++) {
+                if (gg + 1 < G) load_group(gg + 1, (gg + 1) & 1);
+                else load_tail();
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+#pragma unroll
+                    for (int b = 0; b < 4; b++)
+                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                            sa[gg & 1][s], sb[gg & 1][b][s], gg == 0 && s == 0 ? zero : acc[b], 0, 0, 0);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll
+            for (int s = 0; s < TL; s++)
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                        ta[s], tb[s][b], G == 0 && s == 0 ? zero : acc[b], 0, 0, 0);
+        } else {
+            // Two operand sets alternate: the next group's operands are read
+            // from LDS while the current group's products run (past the last
+            // group the last is read again, unused).  The first group starts
+            // from zero accumulators.
+            const float *qrow = qs + (16 * w + m) * st + 4 * g;
+            const float *crow = cs + m * st + 4 * g;
+            auto load = [&](int gg, knn_f4 &a, knn_f4 (&bv)[4]) {
+                a = *(const knn_f4 *)(qrow + 16 * gg);
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    bv[b] = *(const knn_f4 *)(crow + 16 * b * st + 16 * gg);
+            };
+            auto mma = [&](const knn_f4 &a, const knn_f4 (&bv)[4]) {
+#pragma unroll
+                for (int s = 0; s < 4; s++)
+#pragma unroll
+                    for (int b = 0; b < 4; b++)
+                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], bv[b][s], acc[b], 0, 0, 0);
+            };
+            if (groups > 0) {
+                knn_f4 xa, xb[4], ya, yb[4];
+                load(0, xa, xb);
+                load(groups > 1 ? 1 : 0, ya, yb);
+                const knn_f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[0], xb[b][0], zero, 0, 0, 0);
+#pragma unroll
+                for (int s = 1; s < 4; s++)
+#pragma unroll
+                    for (int b = 0; b < 4; b++)
+                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[s], xb[b][s], acc[b], 0, 0, 0);
+                for (int gg = 1; gg < groups; gg += 2) {
+                    // y holds group gg.
+                    load(gg + 1 < groups ? gg + 1 : gg, xa, xb);
+                    mma(ya, yb);
+                    if (gg + 1 >= groups) break;
+                    // x holds group gg + 1.
+                    load(gg + 2 < groups ? gg + 2 : gg + 1, ya, yb);
+                    mma(xa, xb);
+                }
+            } else {
+#pragma unroll
+                for (int b = 0; b < 4; b++) acc[b] = {0.0f, 0.0f, 0.0f, 0.0f};
+            }
+            // Tail steps: column 16 groups + 4 s + g (qrow, crow start at 4 g).
+            for (int s = 0; s < tail; s++) {
+                const int off = 16 * groups + 4 * s - 3 * g;
+                const float a = qrow[off];
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+                    acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, crow[16 * b * st + off], acc[b], 0, 0, 0);
+            }
         }
-
-        // Distances ||q||^2 + (||c||^2 - 2 q.c), or the partial distance
-        // ||c||^2 - 2 q.c without a query norm (the product by 2 is exact, so
-        // the fma rounds as the subtraction does).  A candidate must beat its
-        // row's k-th entry by (distance, id); in the exhaustive scan ids
-        // increase, so a tie with that entry loses.  Masks: candidates past
-        // the tile's end, the query itself, and (inverted file) queries that
-        // do not search this cluster.
-        const bool special = (IVF && wmask != 0xffffu) || pos + kTile > lim ||
-                             (T.self && pos < q0 + kTile && pos + kTile > q0);
+        // Masks (tiles that need them only): candidates past the tile's end,
+        // the query itself, and (inverted file) queries that do not search
+        // this cluster.  A masked distance becomes NaN, which no comparison
+        // below accepts.
+        const bool selftile = T.self && pos < q0 + 16 * w + 16 && pos + kTile > q0 + 16 * w;
+        const bool special = (IVF && wmask != 0xffffu) || pos + kTile > lim || selftile;
+        if (special) {
+            bool rok[4], cok[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) rok[i] = (wmask >> (4 * g + i)) & 1u;
+#pragma unroll
+            for (int b = 0; b < 4; b++) cok[b] = pos + 16 * b + m < lim;
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const bool ok = rok[i] && cok[b] &&
+                                    !(selftile && pos + 16 * b + m == qid[i]);
+                    acc[b][i] = ok ? acc[b][i] : __builtin_nanf("");
+                }
+            }
+        }
         if constexpr (ARGMIN) {
             // Columns arrive in increasing id per lane: a tie keeps the first.
 #pragma unroll
             for (int b = 0; b < 4; b++) {
-                const float cnv = cn[16 * b + m];
-                const int c = pos + 16 * b + m;
-                const uint32_t idv = (uint32_t)c;
+                const uint32_t idv = (uint32_t)(pos + 16 * b + m);
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    float v = qn[i] + __builtin_fmaf(-2.0f, acc[b][i], cnv);
-                    if (special && (c >= lim || (T.self && c == qid[i]))) v = INFINITY;
-                    const bool better = v < bestd[i];
-                    bestd[i] = better ? v : bestd[i];
+                    const bool better = acc[b][i] < bestd[i];
+                    bestd[i] = better ? acc[b][i] : bestd[i];
                     besti[i] = better ? idv : besti[i];
                 }
             }
         } else {
-        float dd[4][4];
-        uint32_t idc[4];
-        int arr[4] = {0, 0, 0, 0};
-        uint64_t any = 0;
-        auto passes = [&](float v, uint32_t idv, int i) {
-            return (IVF || ROT) ? (v < INFINITY && key_lt(v, idv, tau_d[i], tau_i[i]))
-                                : v < tau_d[i];
+        // A candidate is kept if it does not exceed its row's k-th entry
+        // (exhaustive scans in increasing id: if it is below it, as a tie
+        // loses).  Ties kept here are ordered by (distance, id) when merged,
+        // so the k best are the same whichever order the tiles come in.
+        auto passes = [&](float v, int i) {
+            return (IVF || ROT) ? v <= tau_d[i] : v < tau_d[i];
         };
+        uint64_t any = 0;
 #pragma unroll
-        for (int b = 0; b < 4; b++) {
-            const float cnv = cn[16 * b + m];
-            idc[b] = IVF ? cid[16 * b + m] : (uint32_t)(pos + 16 * b + m);
+        for (int b = 0; b < 4; b++)
 #pragma unroll
-            for (int i = 0; i < 4; i++)
-                dd[b][i] = qn[i] + __builtin_fmaf(-2.0f, acc[b][i], cnv);
-        }
-        if (special) {
+            for (int i = 0; i < 4; i++) any |= __ballot(passes(acc[b][i], i));
+        if (any) {
+            uint32_t idc[4];
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-                const int c = pos + 16 * b + m;
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const bool out = c >= lim || (T.self && c == qid[i]) ||
-                                     !((wmask >> (4 * g + i)) & 1u);
-                    dd[b][i] = out ? INFINITY : dd[b][i];
-                }
-            }
-        }
-#pragma unroll
-        for (int b = 0; b < 4; b++) {
+            for (int b = 0; b < 4; b++)
+                idc[b] = IVF ? cid[16 * b + m] : (uint32_t)(pos + 16 * b + m);
+            // Per-row counts.  (The thresholds are passed through an opaque
+            // copy so that the compares above are not kept live for this.)
+            float th[4];
 #pragma unroll
             for (int i = 0; i < 4; i++) {
-                const uint64_t bal = __ballot(passes(dd[b][i], idc[b], i));
-                arr[i] += __popcll((bal >> (16 * g)) & 0xffffull);
-                any |= bal;
+                th[i] = tau_d[i];
+                asm("" : "+v"(th[i]));
             }
-        }
-        if (any) {
+            int arr[4] = {0, 0, 0, 0};
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const bool p = (IVF || ROT) ? acc[b][i] <= th[i] : acc[b][i] < th[i];
+                    const uint64_t bal = __ballot(p);
+                    arr[i] += __popcll((bal >> (16 * g)) & 0xffffull);
+                }
+            }
             // Rows whose buffer would overflow are merged first.  (The
             // appends below test against the thresholds as they are then: a
             // candidate a merge has put out of reach is not kept, so arr
@@ -544,11 +638,11 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
             for (int b = 0; b < 4; b++) {
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    const bool p = passes(dd[b][i], idc[b], i);
+                    const bool p = passes(acc[b][i], i);
                     const uint32_t gb = (uint32_t)((__ballot(p) >> (16 * g)) & 0xffffull);
                     if (p) {
                         const int slot = (4 * g + i) * kBuf + cnt[i] + __popc(gb & below);
-                        myb_d[slot] = dd[b][i];
+                        myb_d[slot] = acc[b][i];
                         myb_i[slot] = idc[b];
                     }
                     cnt[i] += __popc(gb);
@@ -616,62 +710,69 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     }
 }
 
-int knn_mfma_stride(int d) { return ((d + 3) & ~3) + 4; }
+int knn_mfma_ksteps(int d) { return (d + 2 + 3) / 4; }
+
+int knn_mfma_stride(int d) { return 4 * knn_mfma_ksteps(d) + 4; }
 
 size_t knn_mfma_lds_bytes(int d) {
     const size_t st = (size_t)knn_mfma_stride(d);
-    return sizeof(float) * (3 * kTile * st + 4 * kTile + 2 * (size_t)kTile * kBuf);
+    return sizeof(float) * (3 * kTile * st + 2 * kTile + 2 * (size_t)kTile * kBuf);
 }
 
 hipError_t launch_knn_pack(const float *X, const uint32_t *order, int64_t n,
-                           int d, float *Xp, const float *norms_in,
-                           float *norms_out, hipStream_t s) {
+                           int d, const float *norms, float *Xp, hipStream_t s) {
     const int st = knn_mfma_stride(d);
     const int64_t total = (n + kTile) * st;
-    k_knn_pack<<<(unsigned)ceil_div(total > n + kTile ? total : n + kTile, 256), 256, 0, s>>>(
-        X, order, n, d, st, Xp, norms_in, norms_out);
+    k_knn_pack<<<(unsigned)ceil_div(total, 256), 256, 0, s>>>(X, order, n, d, st, norms, Xp);
     return hipGetLastError();
 }
 
-hipError_t launch_knn_tiles(const KnnTiles &T, int d, bool ivf, hipStream_t s) {
-    if (T.nq <= 0 || d < 1 || d > kKnnMfmaMaxDims || T.k < 1 || T.k > kWave ||
-        T.nq > kKnnMaxRows || T.nc > kKnnMaxRows)
+hipError_t launch_knn_tiles(const KnnTiles &T, bool ivf, hipStream_t s) {
+    if (T.nq <= 0 || T.d < 1 || T.d > kKnnMfmaMaxDims || T.k < 1 || T.k > kWave ||
+        T.nq > kKnnMaxRows || T.nc > kKnnMaxRows || T.st != knn_mfma_stride(T.d) ||
+        T.ksteps != knn_mfma_ksteps(T.d))
         return hipErrorInvalidValue;
     const unsigned grid = (unsigned)ceil_div(T.nq, kTile);
-    const size_t lds = knn_mfma_lds_bytes(d);
+    const size_t lds = knn_mfma_lds_bytes(T.d);
+    // No candidate buffers for k = 1: the tiles and ids only.
+    const size_t lds1 = sizeof(float) * (3 * (size_t)kTile * T.st + 2 * kTile);
+    // Straight-line tiles for 47..50 dimensions (13 k-steps), loops otherwise.
+    const bool ks13 = T.ksteps == 13;
     if (ivf) {
-        k_knn_mfma<true, false, false><<<grid, 256, lds, s>>>(T);
+        if (ks13) k_knn_mfma<true, false, false, 13><<<grid, 256, lds, s>>>(T);
+        else k_knn_mfma<true, false, false, 0><<<grid, 256, lds, s>>>(T);
     } else if (T.k == 1) {
-        // No candidate buffers: the tiles, norms and ids only.
-        const size_t lds1 = sizeof(float) * (3 * (size_t)kTile * T.st + 4 * kTile);
-        k_knn_mfma<false, true, false><<<grid, 256, lds1, s>>>(T);
+        if (ks13) k_knn_mfma<false, true, false, 13><<<grid, 256, lds1, s>>>(T);
+        else k_knn_mfma<false, true, false, 0><<<grid, 256, lds1, s>>>(T);
     } else if (T.start_hint) {
-        k_knn_mfma<false, false, true><<<grid, 256, lds, s>>>(T);
+        if (ks13) k_knn_mfma<false, false, true, 13><<<grid, 256, lds, s>>>(T);
+        else k_knn_mfma<false, false, true, 0><<<grid, 256, lds, s>>>(T);
     } else {
-        k_knn_mfma<false, false, false><<<grid, 256, lds, s>>>(T);
+        if (ks13) k_knn_mfma<false, false, false, 13><<<grid, 256, lds, s>>>(T);
+        else k_knn_mfma<false, false, false, 0><<<grid, 256, lds, s>>>(T);
     }
     return hipGetLastError();
 }
 
-hipError_t launch_knn_mfma(const float *X, float *norms, int64_t n,
+hipError_t launch_knn_mfma(const float *X, const float *norms, int64_t n,
                            int d, int k, float *Xp, uint32_t *out_idx,
                            float *out_dist, hipStream_t s) {
     if (n <= 0 || d < 1 || d > kKnnMfmaMaxDims || k < 1 || k > kWave)
         return hipErrorInvalidValue;
-    // Pad the norms in place (norms_in == norms_out keeps the first n).
-    hipError_t err = launch_knn_pack(X, nullptr, n, d, Xp, nullptr, norms, s);
+    hipError_t err = launch_knn_pack(X, nullptr, n, d, norms, Xp, s);
     if (err != hipSuccess) return err;
     KnnTiles T{};
     T.Q = T.C = Xp;
-    T.qnorm = T.cnorm = norms;
+    T.qnorm = 1;
     T.nq = T.nc = n;
+    T.d = d;
     T.st = knn_mfma_stride(d);
-    T.ksteps = (d + 3) / 4;
+    T.ksteps = knn_mfma_ksteps(d);
     T.k = k;
     T.self = 1;
     T.out_idx = out_idx;
     T.out_dist = out_dist;
-    return launch_knn_tiles(T, d, false, s);
+    return launch_knn_tiles(T, false, s);
 }
 
 size_t knn_lds_bytes(int d, int kmax) {
