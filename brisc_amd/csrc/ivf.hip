@@ -215,13 +215,28 @@ __global__ __launch_bounds__(256) void k_ivf_entries(const uint32_t *__restrict_
     __syncthreads();
     int U = 0;
     for (int t = 0; t < 256; t++) U += part[t];
-    // ... then ordered by the number of cells searching each, most first
-    // (clusters every cell of the block searches are the nearest: scanning
-    // them first tightens the thresholds early), ties in cluster order.
+    // ... then ordered nearest first, as the block's first cell ranks the
+    // clusters (its own cluster first; clusters it does not search after
+    // those it does), then by the number of cells searching each, most
+    // first, then in cluster order.  The neighbours of the block's cells are
+    // found in the first tiles, so the thresholds are tight early.
+    __syncthreads();  // part is read above; it now holds that ranking
+    uint32_t *rep = (uint32_t *)part;
+    if (tid < searched) rep[tid] = near[q0 * searched + tid];
+    __syncthreads();
     int p2 = 1;
     while (p2 < U) p2 <<= 1;
-    for (int i = tid; i < p2; i += 256)
-        key[i] = i < U ? ((uint32_t)(64 - __popcll(emask[i])) << 12) | (uint32_t)i : 0xffffffffu;
+    for (int i = tid; i < p2; i += 256) {
+        uint32_t rank = 64;
+        if (i < U)
+            for (int j = 0; j < searched; j++)
+                if (rep[j] == ecl[i]) {
+                    rank = (uint32_t)j;
+                    break;
+                }
+        key[i] = i < U ? (rank << 19) | ((uint32_t)(64 - __popcll(emask[i])) << 12) | (uint32_t)i
+                       : 0xffffffffu;
+    }
     __syncthreads();
     for (int size = 2; size <= p2; size <<= 1) {
         for (int stride = size >> 1; stride > 0; stride >>= 1) {

@@ -547,26 +547,24 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
                     acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, crow[16 * b * st + off], acc[b], 0, 0, 0);
             }
         }
-        // Masks (tiles that need them only): candidates past the tile's end,
-        // the query itself, and (inverted file) queries that do not search
-        // this cluster.  A masked distance becomes NaN, which no comparison
-        // below accepts.
+        // Masks: candidates past the tile's end (the last tile of a run) and
+        // the query itself (the tile holding it) become NaN, which no
+        // comparison below accepts.  Inverted file: rows whose query does not
+        // search this cluster get a NaN threshold instead (below).
         const bool selftile = T.self && pos < q0 + 16 * w + 16 && pos + kTile > q0 + 16 * w;
-        const bool special = (IVF && wmask != 0xffffu) || pos + kTile > lim || selftile;
-        if (special) {
-            bool rok[4], cok[4];
-#pragma unroll
-            for (int i = 0; i < 4; i++) rok[i] = (wmask >> (4 * g + i)) & 1u;
-#pragma unroll
-            for (int b = 0; b < 4; b++) cok[b] = pos + 16 * b + m < lim;
+        if (pos + kTile > lim || selftile) {
 #pragma unroll
             for (int b = 0; b < 4; b++) {
+                const bool ok = pos + 16 * b + m < lim;
 #pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const bool ok = rok[i] && cok[b] &&
-                                    !(selftile && pos + 16 * b + m == qid[i]);
-                    acc[b][i] = ok ? acc[b][i] : __builtin_nanf("");
-                }
+                for (int i = 0; i < 4; i++) acc[b][i] = ok ? acc[b][i] : __builtin_nanf("");
+            }
+            if (selftile) {
+#pragma unroll
+                for (int b = 0; b < 4; b++)
+#pragma unroll
+                    for (int i = 0; i < 4; i++)
+                        acc[b][i] = pos + 16 * b + m == qid[i] ? __builtin_nanf("") : acc[b][i];
             }
         }
         if constexpr (ARGMIN) {
@@ -586,8 +584,15 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         // (exhaustive scans in increasing id: if it is below it, as a tie
         // loses).  Ties kept here are ordered by (distance, id) when merged,
         // so the k best are the same whichever order the tiles come in.
+        float te[4];  // the thresholds, NaN for rows that skip this tile
+        auto set_te = [&]() {
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+                te[i] = (!IVF || ((wmask >> (4 * g + i)) & 1u)) ? tau_d[i] : __builtin_nanf("");
+        };
+        set_te();
         auto passes = [&](float v, int i) {
-            return (IVF || ROT) ? v <= tau_d[i] : v < tau_d[i];
+            return (IVF || ROT) ? v <= te[i] : v < te[i];
         };
         uint64_t any = 0;
 #pragma unroll
@@ -599,53 +604,101 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
 #pragma unroll
             for (int b = 0; b < 4; b++)
                 idc[b] = IVF ? cid[16 * b + m] : (uint32_t)(pos + 16 * b + m);
-            // Per-row counts.  (The thresholds are passed through an opaque
-            // copy so that the compares above are not kept live for this.)
-            float th[4];
+            // Two forms of the same appends (either keeps the k best, and
+            // the merges order by (distance, id)); each is the faster one,
+            // as measured, for its kind of scan.
+            if constexpr (IVF) {
+                // One 16-column block at a time (at most 16 per row): rows
+                // whose buffer cannot take the block's candidates are merged
+                // first, and the block is then tested against their new
+                // thresholds.
 #pragma unroll
-            for (int i = 0; i < 4; i++) {
-                th[i] = tau_d[i];
-                asm("" : "+v"(th[i]));
-            }
-            int arr[4] = {0, 0, 0, 0};
+                for (int b = 0; b < 4; b++) {
+                    uint32_t gb[4];
+                    uint64_t need[4], need_any = 0;
 #pragma unroll
-            for (int b = 0; b < 4; b++) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const bool p = (IVF || ROT) ? acc[b][i] <= th[i] : acc[b][i] < th[i];
-                    const uint64_t bal = __ballot(p);
-                    arr[i] += __popcll((bal >> (16 * g)) & 0xffffull);
-                }
-            }
-            // Rows whose buffer would overflow are merged first.  (The
-            // appends below test against the thresholds as they are then: a
-            // candidate a merge has put out of reach is not kept, so arr
-            // bounds the appends.)
-            uint32_t rmask = 0;
-#pragma unroll
-            for (int i = 0; i < 4; i++) {
-                const uint64_t need = __ballot(cnt[i] + arr[i] > kBuf);
-#pragma unroll
-                for (int gq = 0; gq < 4; gq++)
-                    rmask |= (uint32_t)((need >> (16 * gq)) & 1ull) << (4 * gq + i);
-            }
-            while (rmask) {
-                const int r = __builtin_ctz(rmask);
-                rmask &= rmask - 1;
-                merge_row(Ld, Li, tau_d, tau_i, cnt, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
-            }
-#pragma unroll
-            for (int b = 0; b < 4; b++) {
-#pragma unroll
-                for (int i = 0; i < 4; i++) {
-                    const bool p = passes(acc[b][i], i);
-                    const uint32_t gb = (uint32_t)((__ballot(p) >> (16 * g)) & 0xffffull);
-                    if (p) {
-                        const int slot = (4 * g + i) * kBuf + cnt[i] + __popc(gb & below);
-                        myb_d[slot] = acc[b][i];
-                        myb_i[slot] = idc[b];
+                    for (int i = 0; i < 4; i++) {
+                        gb[i] = (uint32_t)((__ballot(passes(acc[b][i], i)) >> (16 * g)) & 0xffffull);
+                        need[i] = __ballot(cnt[i] + __popc(gb[i]) > kBuf);
+                        need_any |= need[i];
                     }
-                    cnt[i] += __popc(gb);
+                    if (need_any) {
+                        uint32_t rmask = 0;
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+#pragma unroll
+                            for (int gq = 0; gq < 4; gq++)
+                                rmask |= (uint32_t)((need[i] >> (16 * gq)) & 1ull) << (4 * gq + i);
+                        while (rmask) {
+                            const int r = __builtin_ctz(rmask);
+                            rmask &= rmask - 1;
+                            merge_row(Ld, Li, tau_d, tau_i, cnt, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
+                        }
+                        set_te();
+#pragma unroll
+                        for (int i = 0; i < 4; i++)
+                            gb[i] = (uint32_t)((__ballot(passes(acc[b][i], i)) >> (16 * g)) & 0xffffull);
+                    }
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        if ((gb[i] >> m) & 1u) {
+                            const int slot = (4 * g + i) * kBuf + cnt[i] + __popc(gb[i] & below);
+                            myb_d[slot] = acc[b][i];
+                            myb_i[slot] = idc[b];
+                        }
+                        cnt[i] += __popc(gb[i]);
+                    }
+                }
+            } else {
+                // Per-row counts first.  (The thresholds are passed through an
+                // opaque copy so that the compares above are not kept live.)
+                float th[4];
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    th[i] = te[i];
+                    asm("" : "+v"(th[i]));
+                }
+                int arr[4] = {0, 0, 0, 0};
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const bool p = ROT ? acc[b][i] <= th[i] : acc[b][i] < th[i];
+                        const uint64_t bal = __ballot(p);
+                        arr[i] += __popcll((bal >> (16 * g)) & 0xffffull);
+                    }
+                }
+                // Rows whose buffer would overflow are merged first.  (The
+                // appends below test against the thresholds as they are then:
+                // a candidate a merge has put out of reach is not kept, so
+                // arr bounds the appends.)
+                uint32_t rmask = 0;
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const uint64_t need = __ballot(cnt[i] + arr[i] > kBuf);
+#pragma unroll
+                    for (int gq = 0; gq < 4; gq++)
+                        rmask |= (uint32_t)((need >> (16 * gq)) & 1ull) << (4 * gq + i);
+                }
+                while (rmask) {
+                    const int r = __builtin_ctz(rmask);
+                    rmask &= rmask - 1;
+                    merge_row(Ld, Li, tau_d, tau_i, cnt, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
+                }
+                set_te();
+#pragma unroll
+                for (int b = 0; b < 4; b++) {
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        const bool p = passes(acc[b][i], i);
+                        const uint32_t gb = (uint32_t)((__ballot(p) >> (16 * g)) & 0xffffull);
+                        if (p) {
+                            const int slot = (4 * g + i) * kBuf + cnt[i] + __popc(gb & below);
+                            myb_d[slot] = acc[b][i];
+                            myb_i[slot] = idc[b];
+                        }
+                        cnt[i] += __popc(gb);
+                    }
                 }
             }
         }
