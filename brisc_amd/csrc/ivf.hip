@@ -59,6 +59,24 @@ __global__ __launch_bounds__(256) void k_gather_rows(const float *__restrict__ X
     out[e] = X[(int64_t)rows[r] * d + (e - r * d)];
 }
 
+// Block c of out (64 padded rows of stride st): rows near[c * kp + r] of the
+// num padded centroids P (d dimensions) for r < kp.  Other rows, and any
+// id that is not a centroid, are at infinite distance ([0, 0, inf, 0 ...]).
+__global__ __launch_bounds__(256) void k_gather_blocks(const float *__restrict__ P, int d,
+                                                       int st,
+                                                       const uint32_t *__restrict__ near,
+                                                       int64_t num, int kp,
+                                                       float *__restrict__ out) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= num * 64 * st) return;
+    const int64_t row = e / st;
+    const int j = (int)(e - row * st);
+    const int64_t c = row / 64;
+    const int r = (int)(row - c * 64);
+    const uint32_t id = r < kp ? near[c * kp + r] : 0xffffffffu;
+    out[e] = id < num ? P[(int64_t)id * st + j] : (j == d + 1 ? INFINITY : 0.0f);
+}
+
 __global__ __launch_bounds__(256) void k_count_labels(const uint32_t *__restrict__ labels,
                                                       int64_t n, uint32_t *__restrict__ counts) {
     const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -588,7 +606,32 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
     IVF_TRY(launch_knn_pack(X, order.p, n, d, norms.p, Xs.p, s));
     Xp.reset();  // the unsorted copy is no longer needed
 
-    // 3. Each cell's nearest clusters (by partial distance).
+    // 3. Each cell's nearest clusters (by partial distance).  A first tile
+    // per query block, the kp clusters nearest its first cell's own cluster,
+    // caps the thresholds of the scan (the same ranking, found sooner).
+    const int kp = std::min(kWave, Ck);
+    Buf<uint32_t> cc;
+    Buf<float> cc_d, boot;
+    IVF_TRY(cc.alloc((size_t)Ck * kp));
+    IVF_TRY(cc_d.alloc((size_t)Ck * kp));
+    IVF_TRY(boot.alloc((size_t)Ck * kKnnPadRows * st));
+    {
+        KnnTiles Cc{};
+        Cc.Q = Cc.C = centp.p;
+        Cc.qnorm = 1;
+        Cc.d = d;
+        Cc.nq = Cc.nc = Ck;
+        Cc.st = st;
+        Cc.ksteps = ksteps;
+        Cc.k = kp;
+        Cc.self = 0;
+        Cc.out_idx = cc.p;
+        Cc.out_dist = cc_d.p;
+        IVF_TRY(launch_knn_tiles(Cc, false, s));
+        k_gather_blocks<<<grid_of((int64_t)Ck * kKnnPadRows * st), 256, 0, s>>>(
+            centp.p, d, st, cc.p, Ck, kp, boot.p);
+        IVF_TRY(hipGetLastError());
+    }
     Buf<uint32_t> near;
     Buf<float> near_d;
     IVF_TRY(near.alloc((size_t)n * searched));
@@ -606,11 +649,14 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
         R.k = searched;
         R.self = 0;
         R.start_hint = labels_sorted.p;  // the block's own cluster first
+        R.boot = boot.p;
+        R.boot_rows = kp;
         R.out_idx = near.p;
         R.out_dist = near_d.p;
         IVF_TRY(launch_knn_tiles(R, false, s));
     }
     near_d.reset();
+    boot.reset();
 
     // 4. Per-workgroup cluster lists.
     const int64_t nwg = ceil_div(n, kIvfQueries);

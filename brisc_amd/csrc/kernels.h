@@ -220,6 +220,11 @@ struct KnnTiles {
     // Exhaustive launches: optional, per query; the scan of query block b
     // starts at the 64-row tile holding row start_hint[64 b] and wraps.
     const uint32_t *start_hint;
+    // With start_hint, optional: 64 padded rows per start_hint value (the
+    // first boot_rows of them candidates) scanned first; their k-th distance
+    // per query caps its threshold for the scan proper.  boot_rows >= k.
+    const float *boot;
+    int boot_rows;
     uint32_t *out_idx;
     float *out_dist;
 };

@@ -229,12 +229,13 @@ __device__ __forceinline__ void put(T (&a)[N], int r, T v) {
 }
 
 // Merge row r (wave-uniform) of this wave: list (Ld, Li lane = entry) with
-// the row's cnt buffered candidates; refresh the row's threshold (entry k-1).
+// the row's cnt buffered candidates; refresh the row's threshold (entry k-1,
+// or the row's cap if that is smaller).
 __device__ __forceinline__ void merge_row(float (&Ld)[16], uint32_t (&Li)[16],
                                           float (&tau_d)[4], uint32_t (&tau_i)[4],
-                                          int (&cnt)[4], const float *bd,
-                                          const uint32_t *bi, int r, int k,
-                                          int lane) {
+                                          int (&cnt)[4], const float (&cap)[4],
+                                          const float *bd, const uint32_t *bi, int r,
+                                          int k, int lane) {
     const int g = r >> 2, i = r & 3;
     const int c = __shfl(pick(cnt, i), 16 * g);
     float B = INFINITY;
@@ -267,8 +268,12 @@ __device__ __forceinline__ void merge_row(float (&Ld)[16], uint32_t (&Li)[16],
     const float td = __shfl(L, k - 1);
     const uint32_t ti = (uint32_t)__shfl((int)LI, k - 1);
     if ((lane >> 4) == g) {
-        put(tau_d, i, td);
-        put(tau_i, i, ti);
+        // A row's cap (a distance no greater than its k-th nearest) bounds
+        // the threshold while the list holds fewer than k close entries.
+        const float c = pick(cap, i);
+        const bool capped = c < td;
+        put(tau_d, i, capped ? c : td);
+        put(tau_i, i, capped ? 0xffffffffu : ti);
         put(cnt, i, 0);
     }
 }
@@ -314,6 +319,10 @@ __device__ __forceinline__ void dma_run(const float *src, float *dst, int chunks
 // ROT (exhaustive scans with a start hint: the centroid ranking): the scan
 // starts at the tile holding the block's own cluster and wraps, so the
 // thresholds are tight from the first tiles; ids then arrive out of order.
+// With T.boot, a first tile holds the clusters nearest the block's own
+// (T.boot_rows of them): its k-th distance per row caps that row's threshold
+// (the k-th nearest over all candidates is no farther), and the lists then
+// restart empty, so every candidate is still seen once.
 template <bool IVF, bool ARGMIN, bool ROT, int KS>
 __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     extern __shared__ __align__(16) float lds_f[];
@@ -339,10 +348,18 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     bool valid = nc > 0;
     int start = 0;
     bool wrapped = false;
+    bool boot = false;
+    const float *boot_src = nullptr;
     if (ROT) {
-        start = min((int)(T.start_hint[min(q0, nq - 1)] & ~(uint32_t)(kTile - 1)),
-                    (nc - 1) & ~(kTile - 1));
+        const uint32_t own = T.start_hint[min(q0, nq - 1)];
+        start = min((int)(own & ~(uint32_t)(kTile - 1)), (nc - 1) & ~(kTile - 1));
         pos = start;
+        if (T.boot) {
+            boot = true;
+            boot_src = T.boot + (size_t)own * kTile * st;
+            pos = 0;
+            lim = T.boot_rows;
+        }
     }
     if (IVF) {
         e = T.wg_off[blockIdx.x];
@@ -357,14 +374,14 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     }
     // Tile rows p .. p + 63 (pad rows keep every read in bounds; ids are
     // clamped, and their rows are masked) into buffer b.
-    auto stage = [&](int p, int b) {
-        dma_run(T.C + (size_t)p * st, cs0 + b * (kTile * st), chunks, w, lane);
+    auto stage = [&](const float *src, int p, int b) {
+        dma_run(src + (size_t)p * st, cs0 + b * (kTile * st), chunks, w, lane);
         if (IVF && w == 0)
             __builtin_amdgcn_global_load_lds(T.ids + min(p + lane, nc - 1),
                                              (lds_void *)(cid0 + b * kTile), 4, 0, 0);
     };
     dma_run(T.Q + (size_t)q0 * st, qs, chunks, w, lane);
-    if (valid) stage(pos, 0);
+    if (valid) stage(ROT && boot ? boot_src : T.C, pos, 0);
 
     int qid[4];
 #pragma unroll
@@ -376,7 +393,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         Ld[r] = INFINITY;
         Li[r] = 0xffffffffu;
     }
-    float tau_d[4];
+    float tau_d[4], cap[4];
     uint32_t tau_i[4];
     int cnt[4];
     float bestd[4];
@@ -384,6 +401,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
 #pragma unroll
     for (int i = 0; i < 4; i++) {
         tau_d[i] = INFINITY;
+        cap[i] = INFINITY;
         tau_i[i] = 0xffffffffu;
         cnt[i] = 0;
         bestd[i] = INFINITY;
@@ -419,7 +437,12 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         uint64_t nmask = mask;
         bool nvalid = npos < lim;
         bool nwrapped = wrapped;
-        if (ROT && !nvalid && !wrapped && start > 0) {
+        if (ROT && boot) {
+            // The first tile of the scan proper follows the first tile.
+            npos = start;
+            nlim = nc;
+            nvalid = true;
+        } else if (ROT && !nvalid && !wrapped && start > 0) {
             npos = 0;
             nlim = start;
             nwrapped = true;
@@ -436,7 +459,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
             }
         }
         // The next tile streams into the other buffer while this one is used.
-        if (nvalid) stage(npos, t ^ 1);
+        if (nvalid) stage(T.C, npos, t ^ 1);
 
         // Inverted file: the wave's queries that search this cluster.  A wave
         // with none skips the tile (it still stages and meets the barrier).
@@ -632,7 +655,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
                         while (rmask) {
                             const int r = __builtin_ctz(rmask);
                             rmask &= rmask - 1;
-                            merge_row(Ld, Li, tau_d, tau_i, cnt, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
+                            merge_row(Ld, Li, tau_d, tau_i, cnt, cap, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
                         }
                         set_te();
 #pragma unroll
@@ -683,7 +706,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
                 while (rmask) {
                     const int r = __builtin_ctz(rmask);
                     rmask &= rmask - 1;
-                    merge_row(Ld, Li, tau_d, tau_i, cnt, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
+                    merge_row(Ld, Li, tau_d, tau_i, cnt, cap, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
                 }
                 set_te();
 #pragma unroll
@@ -704,6 +727,34 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         }
         }  // selection
         }  // wmask
+        if (ROT && boot) {
+            // The first tile's k-th distances become the caps; the lists
+            // restart empty (the scan proper sees these candidates again).
+            uint32_t rmask = 0;
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                const uint64_t need = __ballot(cnt[i] > 0);
+#pragma unroll
+                for (int gq = 0; gq < 4; gq++)
+                    rmask |= (uint32_t)((need >> (16 * gq)) & 1ull) << (4 * gq + i);
+            }
+            while (rmask) {
+                const int r = __builtin_ctz(rmask);
+                rmask &= rmask - 1;
+                merge_row(Ld, Li, tau_d, tau_i, cnt, cap, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
+            }
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                cap[i] = tau_d[i];
+                tau_i[i] = 0xffffffffu;
+            }
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                Ld[r] = INFINITY;
+                Li[r] = 0xffffffffu;
+            }
+            boot = false;
+        }
         // The next tile has landed (this wave's share), then every wave's.
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
@@ -750,7 +801,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     while (rmask) {
         const int r = __builtin_ctz(rmask);
         rmask &= rmask - 1;
-        merge_row(Ld, Li, tau_d, tau_i, cnt, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
+        merge_row(Ld, Li, tau_d, tau_i, cnt, cap, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
     }
 #pragma unroll
     for (int r = 0; r < 16; r++) {
