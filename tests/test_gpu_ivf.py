@@ -302,3 +302,53 @@ def test_ivf_shapes_against_exact_search(n, d, k, centers):
     print(f'n={n} d={d} k={k}: {info}; recall {rec:.5f}')
     assert rec >= 0.95
     assert np.array_equal(dist[hit].view(np.uint32), exd[hit].view(np.uint32))
+
+
+@pytest.mark.parametrize('d', [50, 20])
+def test_ivf_fewer_than_64_clusters_searches_each_cells_nearest_clusters(d):
+    # 40 clusters, 6 searched: the ranking's first tile (the clusters nearest
+    # a block's own cluster) holds fewer than 64 rows.  Against the device's
+    # own k-means (the same computation knn_ivf runs first), in FP64: every
+    # neighbour lies in one of the cell's 6 nearest non-empty clusters, and
+    # the neighbours are the exact ones among those clusters' cells (up to
+    # near-ties, as in test_gpu_knn).  Cells whose 6th and 7th clusters are
+    # within rounding of each other are not checked.
+    from brisc_amd.knn import kmeans, knn_ivf
+    n, k, C, s = 20000, 10, 40, 6
+    X = helpers.knn_points(77 + d, n, d, 25, spread=0.8)
+    Xd = _dev(X)
+    labels, cent, counts, _ = kmeans(Xd, C)
+    nb, dist, info = knn_ivf(Xd, k, num_clusters=C, num_clusters_searched=s)
+    labels = labels.cpu().numpy().astype(np.int64)
+    cent = cent.cpu().numpy().astype(np.float64)
+    counts = counts.cpu().numpy()
+    nb = nb.cpu().numpy().astype(np.int64)
+    dist = dist.cpu().numpy()
+    kept = np.flatnonzero(counts > 0)
+    assert info['num_clusters'] == kept.size < 64
+    assert info['num_clusters_searched'] == s and not info['exhaustive']
+    X64 = X.astype(np.float64)
+    norms = (X64 * X64).sum(1)
+    cn = (cent[kept] ** 2).sum(1)
+    rows = np.random.default_rng(d).choice(n, 1500, replace=False)
+    checked = 0
+    for q in rows:
+        pd = cn - 2.0 * cent[kept] @ X64[q]
+        order = np.argsort(pd, kind='stable')
+        tol = 1e-4 * (norms[q] + cn.max())
+        if pd[order[s]] - pd[order[s - 1]] < tol:
+            continue
+        checked += 1
+        allowed = np.flatnonzero(np.isin(labels, kept[order[:s]]))
+        allowed = allowed[allowed != q]
+        assert np.isin(nb[q], allowed).all(), q
+        ex = ((X64[allowed] - X64[q]) ** 2).sum(1)
+        o = np.argsort(ex, kind='stable')[:k]
+        got = ((X64[nb[q]] - X64[q]) ** 2).sum(1)
+        slack = 4e-5 * (norms[q] + norms.max())
+        assert got.max() <= ex[o[-1]] + slack, q
+        need = allowed[ex < got.max() - slack]
+        assert set(need.tolist()) <= set(nb[q].tolist()), q
+        assert np.abs(dist[q] - got).max() <= 2e-5 * (norms[q] + norms.max()), q
+    print(f'd={d}: {info}; rows checked {checked} of {rows.size}')
+    assert checked >= 0.9 * rows.size
