@@ -326,13 +326,18 @@ __device__ __forceinline__ void dma_run(const float *src, float *dst, int chunks
 template <bool IVF, bool ARGMIN, bool ROT, int KS>
 __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     extern __shared__ __align__(16) float lds_f[];
+    // With the k-steps known at compile time the queries' operands are held
+    // in registers: the queries pass through candidate buffer 1 on the way
+    // (LDS: two tiles, ids, buffers of 32 per row; three workgroups fit a
+    // CU).  Otherwise they stay in their own LDS tile (buffers of 64).
+    constexpr int kB = KS > 0 ? 32 : kBuf;
     const int st = T.st;
     const int chunks = st >> 2;                   // 64 rows x st floats
-    float *qs = lds_f;                            // 64 x st queries
-    float *cs0 = qs + kTile * st;                 // 2 x (64 x st) candidates
+    float *const cs0 = lds_f + (KS > 0 ? 0 : kTile * st);  // 2 x (64 x st) candidates
+    float *const qs = KS > 0 ? cs0 + kTile * st : lds_f;   // 64 x st queries
     uint32_t *cid0 = (uint32_t *)(cs0 + 2 * kTile * st);  // 2 x 64 candidate ids
-    float *bufd = (float *)(cid0 + 2 * kTile);    // 64 rows x kBuf
-    uint32_t *bufi = (uint32_t *)(bufd + kTile * kBuf);
+    float *bufd = (float *)(cid0 + 2 * kTile);    // 64 rows x kB
+    uint32_t *bufi = (uint32_t *)(bufd + kTile * kB);
     const int tid = threadIdx.x, lane = tid & (kWave - 1), w = tid >> 6;
     const int g = lane >> 4, m = lane & 15;
     const int q0 = (int)blockIdx.x * kTile;
@@ -408,8 +413,8 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         besti[i] = 0xffffffffu;
     }
     const int groups = T.ksteps >> 2, tail = T.ksteps & 3;
-    float *const myb_d = bufd + (16 * w) * kBuf;
-    uint32_t *const myb_i = bufi + (16 * w) * kBuf;
+    float *const myb_d = bufd + (16 * w) * kB;
+    uint32_t *const myb_i = bufi + (16 * w) * kB;
     const uint32_t below = (1u << m) - 1u;
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
@@ -427,6 +432,17 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         }
     }
     __syncthreads();
+    constexpr int QG = KS > 0 ? KS / 4 : 1, QT = KS > 0 && KS % 4 ? KS % 4 : 1;
+    knn_f4 qa[QG];  // this lane's query operands: columns 16 gg + 4 g .. + 3
+    float qt[QT];   // and the tail steps' columns 16 QG + 4 s + g
+    if constexpr (KS > 0) {
+        const float *qrow = qs + (16 * w + m) * st + 4 * g;
+#pragma unroll
+        for (int gg = 0; gg < KS / 4; gg++) qa[gg] = *(const knn_f4 *)(qrow + 16 * gg);
+#pragma unroll
+        for (int s = 0; s < KS % 4; s++) qt[s] = qrow[16 * (KS / 4) + 4 * s - 3 * g];
+        __syncthreads();  // buffer 1 is free for the tiles
+    }
 
     int t = 0;
     while (valid) {
@@ -469,17 +485,16 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         knn_f4 acc[4];
         if constexpr (KS > 0) {
             // k-steps known at compile time: straight-line code in which the
-            // next group's (or the tail's) operands are read from LDS before
-            // the current group's products are issued; the scheduling
-            // barriers keep that order, so the reads complete under the
-            // products.  Two operand sets alternate.
+            // next group's (or the tail's) candidate operands are read from
+            // LDS before the current group's products are issued; the
+            // scheduling barriers keep that order, so the reads complete
+            // under the products.  Two operand sets alternate; the query
+            // operands are in registers.
             constexpr int G = KS / 4, TL = KS % 4;
-            const float *qrow = qs + (16 * w + m) * st + 4 * g;
             const float *crow = cs + m * st + 4 * g;
-            knn_f4 sa[2], sb[2][4];
-            float ta[TL > 0 ? TL : 1], tb[TL > 0 ? TL : 1][4];
+            knn_f4 sb[2][4];
+            float tb[TL > 0 ? TL : 1][4];
             auto load_group = [&](int gg, int set) {
-                sa[set] = *(const knn_f4 *)(qrow + 16 * gg);
 #pragma unroll
                 for (int b = 0; b < 4; b++)
                     sb[set][b] = *(const knn_f4 *)(crow + 16 * b * st + 16 * gg);
@@ -488,7 +503,6 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
 #pragma unroll
                 for (int s = 0; s < TL; s++) {
                     const int off = 16 * G + 4 * s - 3 * g;
-                    ta[s] = qrow[off];
 #pragma unroll
                     for (int b = 0; b < 4; b++) tb[s][b] = crow[16 * b * st + off];
                 }
@@ -506,7 +520,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
 #pragma unroll
                     for (int b = 0; b < 4; b++)
                         acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                            sa[gg & 1][s], sb[gg & 1][b][s], gg == 0 && s == 0 ? zero : acc[b], 0, 0, 0);
+                            qa[gg][s], sb[gg & 1][b][s], gg == 0 && s == 0 ? zero : acc[b], 0, 0, 0);
                 __builtin_amdgcn_sched_barrier(0);
             }
 #pragma unroll
@@ -514,7 +528,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
 #pragma unroll
                 for (int b = 0; b < 4; b++)
                     acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                        ta[s], tb[s][b], G == 0 && s == 0 ? zero : acc[b], 0, 0, 0);
+                        qt[s], tb[s][b], G == 0 && s == 0 ? zero : acc[b], 0, 0, 0);
         } else {
             // Two operand sets alternate: the next group's operands are read
             // from LDS while the current group's products run (past the last
@@ -629,8 +643,10 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
                 idc[b] = IVF ? cid[16 * b + m] : (uint32_t)(pos + 16 * b + m);
             // Two forms of the same appends (either keeps the k best, and
             // the merges order by (distance, id)); each is the faster one,
-            // as measured, for its kind of scan.
-            if constexpr (IVF) {
+            // as measured, for its kind of scan.  The counts-first form
+            // needs room for a whole tile (64 per row) after a merge, so
+            // the 32-slot buffers take the per-block form.
+            if constexpr (IVF || kB < kWave) {
                 // One 16-column block at a time (at most 16 per row): rows
                 // whose buffer cannot take the block's candidates are merged
                 // first, and the block is then tested against their new
@@ -642,7 +658,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
                         gb[i] = (uint32_t)((__ballot(passes(acc[b][i], i)) >> (16 * g)) & 0xffffull);
-                        need[i] = __ballot(cnt[i] + __popc(gb[i]) > kBuf);
+                        need[i] = __ballot(cnt[i] + __popc(gb[i]) > kB);
                         need_any |= need[i];
                     }
                     if (need_any) {
@@ -655,7 +671,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
                         while (rmask) {
                             const int r = __builtin_ctz(rmask);
                             rmask &= rmask - 1;
-                            merge_row(Ld, Li, tau_d, tau_i, cnt, cap, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
+                            merge_row(Ld, Li, tau_d, tau_i, cnt, cap, myb_d + r * kB, myb_i + r * kB, r, k, lane);
                         }
                         set_te();
 #pragma unroll
@@ -665,7 +681,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
                         if ((gb[i] >> m) & 1u) {
-                            const int slot = (4 * g + i) * kBuf + cnt[i] + __popc(gb[i] & below);
+                            const int slot = (4 * g + i) * kB + cnt[i] + __popc(gb[i] & below);
                             myb_d[slot] = acc[b][i];
                             myb_i[slot] = idc[b];
                         }
@@ -698,7 +714,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
                 uint32_t rmask = 0;
 #pragma unroll
                 for (int i = 0; i < 4; i++) {
-                    const uint64_t need = __ballot(cnt[i] + arr[i] > kBuf);
+                    const uint64_t need = __ballot(cnt[i] + arr[i] > kB);
 #pragma unroll
                     for (int gq = 0; gq < 4; gq++)
                         rmask |= (uint32_t)((need >> (16 * gq)) & 1ull) << (4 * gq + i);
@@ -706,7 +722,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
                 while (rmask) {
                     const int r = __builtin_ctz(rmask);
                     rmask &= rmask - 1;
-                    merge_row(Ld, Li, tau_d, tau_i, cnt, cap, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
+                    merge_row(Ld, Li, tau_d, tau_i, cnt, cap, myb_d + r * kB, myb_i + r * kB, r, k, lane);
                 }
                 set_te();
 #pragma unroll
@@ -716,7 +732,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
                         const bool p = passes(acc[b][i], i);
                         const uint32_t gb = (uint32_t)((__ballot(p) >> (16 * g)) & 0xffffull);
                         if (p) {
-                            const int slot = (4 * g + i) * kBuf + cnt[i] + __popc(gb & below);
+                            const int slot = (4 * g + i) * kB + cnt[i] + __popc(gb & below);
                             myb_d[slot] = acc[b][i];
                             myb_i[slot] = idc[b];
                         }
@@ -741,7 +757,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
             while (rmask) {
                 const int r = __builtin_ctz(rmask);
                 rmask &= rmask - 1;
-                merge_row(Ld, Li, tau_d, tau_i, cnt, cap, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
+                merge_row(Ld, Li, tau_d, tau_i, cnt, cap, myb_d + r * kB, myb_i + r * kB, r, k, lane);
             }
 #pragma unroll
             for (int i = 0; i < 4; i++) {
@@ -801,7 +817,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     while (rmask) {
         const int r = __builtin_ctz(rmask);
         rmask &= rmask - 1;
-        merge_row(Ld, Li, tau_d, tau_i, cnt, cap, myb_d + r * kBuf, myb_i + r * kBuf, r, k, lane);
+        merge_row(Ld, Li, tau_d, tau_i, cnt, cap, myb_d + r * kB, myb_i + r * kB, r, k, lane);
     }
 #pragma unroll
     for (int r = 0; r < 16; r++) {
@@ -818,10 +834,15 @@ int knn_mfma_ksteps(int d) { return (d + 2 + 3) / 4; }
 
 int knn_mfma_stride(int d) { return 4 * knn_mfma_ksteps(d) + 4; }
 
-size_t knn_mfma_lds_bytes(int d) {
-    const size_t st = (size_t)knn_mfma_stride(d);
-    return sizeof(float) * (3 * kTile * st + 2 * kTile + 2 * (size_t)kTile * kBuf);
+// LDS of a launch: query tile (unless the k-steps are compile-time), two
+// candidate tiles, ids, and (unless k = 1) the row buffers.
+static size_t tile_lds_bytes(int st, bool ks_fixed, bool buffers) {
+    const size_t tiles = (ks_fixed ? 2 : 3) * (size_t)kTile * st + 2 * kTile;
+    const size_t bufs = buffers ? 2 * (size_t)kTile * (ks_fixed ? 32 : kBuf) : 0;
+    return sizeof(float) * (tiles + bufs);
 }
+
+size_t knn_mfma_lds_bytes(int d) { return tile_lds_bytes(knn_mfma_stride(d), false, true); }
 
 hipError_t launch_knn_pack(const float *X, const uint32_t *order, int64_t n,
                            int d, const float *norms, float *Xp, hipStream_t s) {
@@ -837,11 +858,10 @@ hipError_t launch_knn_tiles(const KnnTiles &T, bool ivf, hipStream_t s) {
         T.ksteps != knn_mfma_ksteps(T.d))
         return hipErrorInvalidValue;
     const unsigned grid = (unsigned)ceil_div(T.nq, kTile);
-    const size_t lds = knn_mfma_lds_bytes(T.d);
-    // No candidate buffers for k = 1: the tiles and ids only.
-    const size_t lds1 = sizeof(float) * (3 * (size_t)kTile * T.st + 2 * kTile);
     // Straight-line tiles for 47..50 dimensions (13 k-steps), loops otherwise.
     const bool ks13 = T.ksteps == 13;
+    const size_t lds = tile_lds_bytes(T.st, ks13, true);
+    const size_t lds1 = tile_lds_bytes(T.st, ks13, false);  // k = 1: no buffers
     if (ivf) {
         if (ks13) k_knn_mfma<true, false, false, 13><<<grid, 256, lds, s>>>(T);
         else k_knn_mfma<true, false, false, 0><<<grid, 256, lds, s>>>(T);
