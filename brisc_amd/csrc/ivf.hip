@@ -159,14 +159,15 @@ __global__ __launch_bounds__(256) void k_dist_to_centroid(const float *__restric
 // search, with the mask of the cells searching each.  Keys (cluster << 6 |
 // cell - 64 b) are bitonically sorted in LDS; a run of equal clusters is one
 // entry.  WRITE = false counts the entries (counts[b]); WRITE = true writes
-// them from wg_off[b] on, the clusters searched by the most cells first.
+// them from wg_off[b] on (order below), each as four words: the cluster's
+// rows [cl_off[c], cl_off[c + 1]) and the 64-bit mask of its cells.
 template <bool WRITE>
 __global__ __launch_bounds__(256) void k_ivf_entries(const uint32_t *__restrict__ near,
                                                      int64_t n, int searched,
                                                      int64_t *__restrict__ counts,
                                                      const int64_t *__restrict__ wg_off,
-                                                     uint32_t *__restrict__ ent_cl,
-                                                     uint64_t *__restrict__ ent_mask) {
+                                                     const int64_t *__restrict__ cl_off,
+                                                     uint32_t *__restrict__ ent) {
     __shared__ uint32_t key[kIvfMaxKeys];
     __shared__ int part[256];
     const int tid = threadIdx.x;
@@ -274,8 +275,13 @@ __global__ __launch_bounds__(256) void k_ivf_entries(const uint32_t *__restrict_
     const int64_t out = wg_off[blockIdx.x];
     for (int i = tid; i < U; i += 256) {
         const uint32_t src = key[i] & 0xfffu;
-        ent_cl[out + i] = ecl[src];
-        ent_mask[out + i] = emask[src];
+        const uint32_t cl = ecl[src];
+        const uint64_t mk = emask[src];
+        uint32_t *rec = ent + 4 * (out + i);
+        rec[0] = (uint32_t)cl_off[cl];
+        rec[1] = (uint32_t)cl_off[cl + 1];
+        rec[2] = (uint32_t)mk;
+        rec[3] = (uint32_t)(mk >> 32);
     }
 }
 
@@ -672,11 +678,9 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
     for (int64_t b = 0; b < nwg; b++) h_wg[b + 1] += h_wg[b];
     const int64_t num_entries = h_wg[nwg];
     IVF_TRY(hipMemcpyAsync(wg_off.p, h_wg.data(), sizeof(int64_t) * (nwg + 1), hipMemcpyHostToDevice, s));
-    Buf<uint32_t> ent_cl;
-    Buf<uint64_t> ent_mask;
-    IVF_TRY(ent_cl.alloc(num_entries));
-    IVF_TRY(ent_mask.alloc(num_entries));
-    k_ivf_entries<true><<<(unsigned)nwg, 256, 0, s>>>(near.p, n, searched, nullptr, wg_off.p, ent_cl.p, ent_mask.p);
+    Buf<uint32_t> ent;
+    IVF_TRY(ent.alloc(4 * num_entries));
+    k_ivf_entries<true><<<(unsigned)nwg, 256, 0, s>>>(near.p, n, searched, nullptr, wg_off.p, off.p, ent.p);
     IVF_TRY(hipGetLastError());
     info->entries = num_entries;
 
@@ -691,9 +695,7 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
     T.k = k;
     T.self = 1;
     T.wg_off = wg_off.p;
-    T.ent_cl = ent_cl.p;
-    T.ent_mask = ent_mask.p;
-    T.cl_off = off.p;
+    T.ent = ent.p;
     T.ids = order.p;
     T.out_idx = out_idx;
     T.out_dist = out_dist;

@@ -209,13 +209,11 @@ struct KnnTiles {
     int st, ksteps, k;     // knn_mfma_stride(d), knn_mfma_ksteps(d), neighbours (<= 64)
     int self;              // queries are the candidates: skip q == c
     // Inverted file (ivf launches): workgroup b (queries 64b..64b+63) scans
-    // entries [wg_off[b], wg_off[b + 1]); entry e covers candidate rows
-    // [cl_off[ent_cl[e]], cl_off[ent_cl[e] + 1]) for the queries whose bit
-    // (q - 64b) is set in ent_mask[e].
+    // entries [wg_off[b], wg_off[b + 1]); entry e is four words ent[4e ..]:
+    // candidate rows [first, end) and a 64-bit mask (low, high word): the
+    // queries whose bit (q - 64b) is set search those rows.
     const int64_t *wg_off;
-    const uint32_t *ent_cl;
-    const uint64_t *ent_mask;
-    const int64_t *cl_off;
+    const uint32_t *ent;
     const uint32_t *ids;
     // Exhaustive launches: optional, per query; the scan of query block b
     // starts at the 64-row tile holding row start_hint[64 b] and wraps.

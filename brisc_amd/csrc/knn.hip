@@ -366,15 +366,24 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
             lim = T.boot_rows;
         }
     }
+    // Inverted file: an entry's four words are read by lanes 0..3 (a vector
+    // load) and taken with readlane; the next entry's are loaded when an
+    // entry starts, so they have landed (the end-of-tile wait) before use.
+    uint32_t nrec = 0;
+    auto fetch = [&](int64_t x) { return T.ent[4 * x + (lane & 3)]; };
+    auto unpack = [&](uint32_t r, int &p, int &l, uint64_t &mk) {
+        p = __builtin_amdgcn_readlane((int)r, 0);
+        l = __builtin_amdgcn_readlane((int)r, 1);
+        mk = (uint32_t)__builtin_amdgcn_readlane((int)r, 2) |
+             ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)r, 3) << 32);
+    };
     if (IVF) {
         e = T.wg_off[blockIdx.x];
         e_end = T.wg_off[blockIdx.x + 1];
         valid = e < e_end;
         if (valid) {
-            const uint32_t cl = T.ent_cl[e];
-            pos = (int)T.cl_off[cl];
-            lim = (int)T.cl_off[cl + 1];
-            mask = T.ent_mask[e];
+            unpack(fetch(e), pos, lim, mask);
+            if (e + 1 < e_end) nrec = fetch(e + 1);
         }
     }
     // Tile rows p .. p + 63 (pad rows keep every read in bounds; ids are
@@ -468,10 +477,8 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
             ne = e + 1;
             nvalid = ne < e_end;
             if (nvalid) {
-                const uint32_t cl = T.ent_cl[ne];
-                npos = (int)T.cl_off[cl];
-                nlim = (int)T.cl_off[cl + 1];
-                nmask = T.ent_mask[ne];
+                unpack(nrec, npos, nlim, nmask);
+                if (ne + 1 < e_end) nrec = fetch(ne + 1);
             }
         }
         // The next tile streams into the other buffer while this one is used.
