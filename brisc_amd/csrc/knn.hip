@@ -660,11 +660,20 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
                 // thresholds.
 #pragma unroll
                 for (int b = 0; b < 4; b++) {
+                    // Late in a scan a tile has a candidate or two: blocks
+                    // and rows with none are skipped (wave-uniform tests).
+                    uint64_t bal[4], bany = 0;
+#pragma unroll
+                    for (int i = 0; i < 4; i++) {
+                        bal[i] = __ballot(passes(acc[b][i], i));
+                        bany |= bal[i];
+                    }
+                    if (!bany) continue;
                     uint32_t gb[4];
                     uint64_t need[4], need_any = 0;
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
-                        gb[i] = (uint32_t)((__ballot(passes(acc[b][i], i)) >> (16 * g)) & 0xffffull);
+                        gb[i] = (uint32_t)((bal[i] >> (16 * g)) & 0xffffull);
                         need[i] = __ballot(cnt[i] + __popc(gb[i]) > kB);
                         need_any |= need[i];
                     }
@@ -682,11 +691,14 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
                         }
                         set_te();
 #pragma unroll
-                        for (int i = 0; i < 4; i++)
-                            gb[i] = (uint32_t)((__ballot(passes(acc[b][i], i)) >> (16 * g)) & 0xffffull);
+                        for (int i = 0; i < 4; i++) {
+                            bal[i] = __ballot(passes(acc[b][i], i));
+                            gb[i] = (uint32_t)((bal[i] >> (16 * g)) & 0xffffull);
+                        }
                     }
 #pragma unroll
                     for (int i = 0; i < 4; i++) {
+                        if (!bal[i]) continue;
                         if ((gb[i] >> m) & 1u) {
                             const int slot = (4 * g + i) * kB + cnt[i] + __popc(gb[i] & below);
                             myb_d[slot] = acc[b][i];
