@@ -330,7 +330,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     // in registers: the queries pass through candidate buffer 1 on the way
     // (LDS: two tiles, ids, buffers of 32 per row; three workgroups fit a
     // CU).  Otherwise they stay in their own LDS tile (buffers of 64).
-    constexpr int kB = KS > 0 ? 32 : kBuf;
+    constexpr int kB = KS > 0 && !ROT ? 32 : kBuf;
     const int st = T.st;
     const int chunks = st >> 2;                   // 64 rows x st floats
     float *const cs0 = lds_f + (KS > 0 ? 0 : kTile * st);  // 2 x (64 x st) candidates
@@ -855,9 +855,9 @@ int knn_mfma_stride(int d) { return 4 * knn_mfma_ksteps(d) + 4; }
 
 // LDS of a launch: query tile (unless the k-steps are compile-time), two
 // candidate tiles, ids, and (unless k = 1) the row buffers.
-static size_t tile_lds_bytes(int st, bool ks_fixed, bool buffers) {
+static size_t tile_lds_bytes(int st, bool ks_fixed, bool buffers, bool rot = false) {
     const size_t tiles = (ks_fixed ? 2 : 3) * (size_t)kTile * st + 2 * kTile;
-    const size_t bufs = buffers ? 2 * (size_t)kTile * (ks_fixed ? 32 : kBuf) : 0;
+    const size_t bufs = buffers ? 2 * (size_t)kTile * (ks_fixed && !rot ? 32 : kBuf) : 0;
     return sizeof(float) * (tiles + bufs);
 }
 
@@ -888,8 +888,9 @@ hipError_t launch_knn_tiles(const KnnTiles &T, bool ivf, hipStream_t s) {
         if (ks13) k_knn_mfma<false, true, false, 13><<<grid, 256, lds1, s>>>(T);
         else k_knn_mfma<false, true, false, 0><<<grid, 256, lds1, s>>>(T);
     } else if (T.start_hint) {
-        if (ks13) k_knn_mfma<false, false, true, 13><<<grid, 256, lds, s>>>(T);
-        else k_knn_mfma<false, false, true, 0><<<grid, 256, lds, s>>>(T);
+        const size_t ldsr = tile_lds_bytes(T.st, ks13, true, true);
+        if (ks13) k_knn_mfma<false, false, true, 13><<<grid, 256, ldsr, s>>>(T);
+        else k_knn_mfma<false, false, true, 0><<<grid, 256, ldsr, s>>>(T);
     } else {
         if (ks13) k_knn_mfma<false, false, false, 13><<<grid, 256, lds, s>>>(T);
         else k_knn_mfma<false, false, false, 0><<<grid, 256, lds, s>>>(T);
