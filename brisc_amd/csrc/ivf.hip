@@ -155,6 +155,41 @@ __global__ __launch_bounds__(256) void k_dist_to_centroid(const float *__restric
     out[i] = dist;
 }
 
+// Ascending bitonic sort of key[0, pow2) in LDS by a workgroup of 256
+// (pow2 a power of two, at most 4096).  At each step pair p compares
+// positions i = p + (p & ~(stride - 1)) and i + stride; a thread loads all
+// its pairs before it compares and stores any, so the step costs one LDS
+// round trip.
+__device__ __forceinline__ void lds_bitonic_sort(uint32_t *key, int pow2, int tid) {
+    const int pairs = pow2 >> 1;
+    for (int size = 2; size <= pow2; size <<= 1) {
+        for (int stride = size >> 1; stride > 0; stride >>= 1) {
+            uint32_t a[8], b[8];
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int p = tid + 256 * r;
+                if (p < pairs) {
+                    const int i = p + (p & ~(stride - 1));
+                    a[r] = key[i];
+                    b[r] = key[i + stride];
+                }
+            }
+#pragma unroll
+            for (int r = 0; r < 8; r++) {
+                const int p = tid + 256 * r;
+                if (p < pairs) {
+                    const int i = p + (p & ~(stride - 1));
+                    if ((a[r] > b[r]) == ((i & size) == 0)) {
+                        key[i] = b[r];
+                        key[i + stride] = a[r];
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
 // Per workgroup of 64 consecutive sorted cells: the distinct clusters they
 // search, with the mask of the cells searching each.  Keys (cluster << 6 |
 // cell - 64 b) are bitonically sorted in LDS; a run of equal clusters is one
@@ -183,21 +218,7 @@ __global__ __launch_bounds__(256) void k_ivf_entries(const uint32_t *__restrict_
                      : 0xffffffffu;
     }
     __syncthreads();
-    for (int size = 2; size <= pow2; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = tid; i < pow2; i += 256) {
-                const int j = i ^ stride;
-                if (j > i) {
-                    const uint32_t a = key[i], b = key[j];
-                    if ((a > b) == ((i & size) == 0)) {
-                        key[i] = b;
-                        key[j] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
+    lds_bitonic_sort(key, pow2, tid);
     // Run starts; thread t owns positions [t * per, (t + 1) * per).
     const int per = (pow2 + 255) / 256;
     int mine = 0;
@@ -215,25 +236,29 @@ __global__ __launch_bounds__(256) void k_ivf_entries(const uint32_t *__restrict_
         }
         return;
     }
-    // Entries in cluster order into LDS ...
+    // Entries in cluster order into LDS: the run starts before each
+    // thread's range (an inclusive scan of part), then each key ORs its
+    // cell's bit into its entry's mask.
     __shared__ uint32_t ecl[kIvfMaxKeys];
     __shared__ uint64_t emask[kIvfMaxKeys];
-    int before = 0;
-    for (int t = 0; t < tid; t++) before += part[t];
-    int e = before;
+    for (int off = 1; off < 256; off <<= 1) {
+        const int v = tid >= off ? part[tid - off] : 0;
+        __syncthreads();
+        part[tid] += v;
+        __syncthreads();
+    }
+    const int before = part[tid] - mine;
+    const int U = part[255];
+    for (int x = tid; x < U; x += 256) emask[x] = 0;
+    __syncthreads();
+    int e = before - 1;
     for (int i = tid * per; i < (tid + 1) * per && i < pow2; i++) {
         const uint32_t k = key[i];
-        if (k == 0xffffffffu || (i > 0 && (key[i - 1] >> 6) == (k >> 6))) continue;
-        uint64_t mask = 0;
-        for (int r = i; r < pow2 && key[r] != 0xffffffffu && (key[r] >> 6) == (k >> 6); r++)
-            mask |= 1ull << (key[r] & 63u);
-        ecl[e] = k >> 6;
-        emask[e] = mask;
-        e++;
+        if (k == 0xffffffffu) break;  // the invalid keys sort last
+        if (i == 0 || (key[i - 1] >> 6) != (k >> 6)) ecl[++e] = k >> 6;
+        atomicOr((unsigned long long *)&emask[e], 1ull << (k & 63u));
     }
     __syncthreads();
-    int U = 0;
-    for (int t = 0; t < 256; t++) U += part[t];
     // ... then ordered nearest first, as the block's first cell ranks the
     // clusters (its own cluster first; clusters it does not search after
     // those it does), then by the number of cells searching each, most
@@ -257,21 +282,7 @@ __global__ __launch_bounds__(256) void k_ivf_entries(const uint32_t *__restrict_
                        : 0xffffffffu;
     }
     __syncthreads();
-    for (int size = 2; size <= p2; size <<= 1) {
-        for (int stride = size >> 1; stride > 0; stride >>= 1) {
-            for (int i = tid; i < p2; i += 256) {
-                const int j = i ^ stride;
-                if (j > i) {
-                    const uint32_t a = key[i], b = key[j];
-                    if ((a > b) == ((i & size) == 0)) {
-                        key[i] = b;
-                        key[j] = a;
-                    }
-                }
-            }
-            __syncthreads();
-        }
-    }
+    lds_bitonic_sort(key, p2, tid);
     const int64_t out = wg_off[blockIdx.x];
     for (int i = tid; i < U; i += 256) {
         const uint32_t src = key[i] & 0xfffu;
