@@ -108,18 +108,30 @@ __global__ __launch_bounds__(256) void k_cluster_sums(const float *__restrict__ 
         const int dim = d0 + lane;
         const bool on = dim < d;
         float total = 0.0f, part = 0.0f;
-        int64_t cur = -1;
-        for (int64_t p = b; p < e; p++) {
-            const int64_t cell = order[p];
-            const int64_t ch = cell / chunk;
-            if (ch != cur) {
-                if (cur >= 0) total += part;
-                part = 0.0f;
-                cur = ch;
+        int64_t end = -1;  // first cell past the current chunk (cells ascend)
+        bool started = false;
+        // Eight cells' values are loaded ahead; the adds stay in cell order.
+        for (int64_t p = b; p < e; p += 8) {
+            int64_t cell[8];
+            float v[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                cell[u] = p + u < e ? (int64_t)order[p + u] : 0;
+                v[u] = on && p + u < e ? X[cell[u] * d + dim] : 0.0f;
             }
-            if (on) part += X[cell * d + dim];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                if (p + u >= e) break;
+                if (cell[u] >= end) {
+                    if (started) total += part;
+                    part = 0.0f;
+                    started = true;
+                    end = (cell[u] / chunk + 1) * chunk;
+                }
+                part += v[u];
+            }
         }
-        if (cur >= 0) total += part;
+        if (started) total += part;
         if (on) sums[(int64_t)j * d + dim] = total;
     }
 }
