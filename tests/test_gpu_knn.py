@@ -83,7 +83,8 @@ def check_knn(X, nb, dist, k):
 @pytest.mark.parametrize('n,d,k', [
     (2, 1, 1), (65, 3, 1), (257, 50, 20), (1000, 50, 20), (3000, 50, 64),
     (4099, 256, 16), (5000, 10, 32), (700, 7, 33), (1000, 60, 5),
-    (1500, 61, 64), (2000, 128, 20), (999, 129, 7), (20000, 50, 20)])
+    (1500, 61, 64), (2000, 128, 20), (999, 129, 7), (20000, 50, 20),
+    (3000, 47, 20), (2500, 49, 64)])
 def test_exact_knn_matches_fp64_oracle(n, d, k):
     X = _clustered(n * 7 + d, n, d)
     nb, dist = _knn(X, k)
