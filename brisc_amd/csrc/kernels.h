@@ -186,7 +186,7 @@ int knn_kmax(int k);
 constexpr int kKnnMfmaMaxDims = 128;
 constexpr int kKnnPadRows = 64;
 constexpr int64_t kKnnMaxRows = (1ll << 31) - 1 - kKnnPadRows;  // 32-bit row positions
-int knn_mfma_ksteps(int d);   // ceil((d + 2) / 4): columns x, 1, ||x||^2
+int knn_mfma_ksteps(int d);   // compiled k-steps covering ceil((d + 2) / 4): x, 1, ||x||^2
 int knn_mfma_stride(int d);   // 4 ksteps + 4 floats
 size_t knn_mfma_lds_bytes(int d);
 hipError_t launch_knn_mfma(const float *X, const float *norms, int64_t n,

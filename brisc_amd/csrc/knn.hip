@@ -326,15 +326,15 @@ __device__ __forceinline__ void dma_run(const float *src, float *dst, int chunks
 template <bool IVF, bool ARGMIN, bool ROT, int KS>
 __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     extern __shared__ __align__(16) float lds_f[];
-    // With the k-steps known at compile time the queries' operands are held
-    // in registers: the queries pass through candidate buffer 1 on the way
-    // (LDS: two tiles, ids, buffers of 32 per row; three workgroups fit a
-    // CU).  Otherwise they stay in their own LDS tile (buffers of 64).
-    constexpr int kB = KS > 0 && !ROT ? 32 : kBuf;
+    // The queries' operands are held in registers: the queries pass through
+    // candidate buffer 1 on the way (LDS: two tiles, ids, row buffers of 32
+    // slots, 64 for the ranking, where k = 64).
+    static_assert(KS >= 1 && KS <= 33, "k-steps");
+    constexpr int kB = ROT ? kBuf : 32;
     const int st = T.st;
     const int chunks = st >> 2;                   // 64 rows x st floats
-    float *const cs0 = lds_f + (KS > 0 ? 0 : kTile * st);  // 2 x (64 x st) candidates
-    float *const qs = KS > 0 ? cs0 + kTile * st : lds_f;   // 64 x st queries
+    float *const cs0 = lds_f;                     // 2 x (64 x st) candidates
+    float *const qs = cs0 + kTile * st;           // 64 x st queries, at first
     uint32_t *cid0 = (uint32_t *)(cs0 + 2 * kTile * st);  // 2 x 64 candidate ids
     float *bufd = (float *)(cid0 + 2 * kTile);    // 64 rows x kB
     uint32_t *bufi = (uint32_t *)(bufd + kTile * kB);
@@ -421,7 +421,6 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         bestd[i] = INFINITY;
         besti[i] = 0xffffffffu;
     }
-    const int groups = T.ksteps >> 2, tail = T.ksteps & 3;
     float *const myb_d = bufd + (16 * w) * kB;
     uint32_t *const myb_i = bufi + (16 * w) * kB;
     const uint32_t below = (1u << m) - 1u;
@@ -441,10 +440,10 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         }
     }
     __syncthreads();
-    constexpr int QG = KS > 0 ? KS / 4 : 1, QT = KS > 0 && KS % 4 ? KS % 4 : 1;
+    constexpr int QG = KS / 4 > 0 ? KS / 4 : 1, QT = KS % 4 ? KS % 4 : 1;
     knn_f4 qa[QG];  // this lane's query operands: columns 16 gg + 4 g .. + 3
-    float qt[QT];   // and the tail steps' columns 16 QG + 4 s + g
-    if constexpr (KS > 0) {
+    float qt[QT];   // and the tail steps' columns 16 (KS / 4) + 4 s + g
+    {
         const float *qrow = qs + (16 * w + m) * st + 4 * g;
 #pragma unroll
         for (int gg = 0; gg < KS / 4; gg++) qa[gg] = *(const knn_f4 *)(qrow + 16 * gg);
@@ -490,13 +489,12 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
         if (wmask != 0) {
         // The wave's 16 x 64 block of distances.
         knn_f4 acc[4];
-        if constexpr (KS > 0) {
-            // k-steps known at compile time: straight-line code in which the
-            // next group's (or the tail's) candidate operands are read from
-            // LDS before the current group's products are issued; the
-            // scheduling barriers keep that order, so the reads complete
-            // under the products.  Two operand sets alternate; the query
-            // operands are in registers.
+        {
+            // Straight-line code in which the next group's (or the tail's)
+            // candidate operands are read from LDS before the current
+            // group's products are issued; the scheduling barriers keep
+            // that order, so the reads complete under the products.  Two
+            // operand sets alternate; the query operands are in registers.
             constexpr int G = KS / 4, TL = KS % 4;
             const float *crow = cs + m * st + 4 * g;
             knn_f4 sb[2][4];
@@ -536,60 +534,6 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
                 for (int b = 0; b < 4; b++)
                     acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(
                         qt[s], tb[s][b], G == 0 && s == 0 ? zero : acc[b], 0, 0, 0);
-        } else {
-            // Two operand sets alternate: the next group's operands are read
-            // from LDS while the current group's products run (past the last
-            // group the last is read again, unused).  The first group starts
-            // from zero accumulators.
-            const float *qrow = qs + (16 * w + m) * st + 4 * g;
-            const float *crow = cs + m * st + 4 * g;
-            auto load = [&](int gg, knn_f4 &a, knn_f4 (&bv)[4]) {
-                a = *(const knn_f4 *)(qrow + 16 * gg);
-#pragma unroll
-                for (int b = 0; b < 4; b++)
-                    bv[b] = *(const knn_f4 *)(crow + 16 * b * st + 16 * gg);
-            };
-            auto mma = [&](const knn_f4 &a, const knn_f4 (&bv)[4]) {
-#pragma unroll
-                for (int s = 0; s < 4; s++)
-#pragma unroll
-                    for (int b = 0; b < 4; b++)
-                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[s], bv[b][s], acc[b], 0, 0, 0);
-            };
-            if (groups > 0) {
-                knn_f4 xa, xb[4], ya, yb[4];
-                load(0, xa, xb);
-                load(groups > 1 ? 1 : 0, ya, yb);
-                const knn_f4 zero = {0.0f, 0.0f, 0.0f, 0.0f};
-#pragma unroll
-                for (int b = 0; b < 4; b++)
-                    acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[0], xb[b][0], zero, 0, 0, 0);
-#pragma unroll
-                for (int s = 1; s < 4; s++)
-#pragma unroll
-                    for (int b = 0; b < 4; b++)
-                        acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xa[s], xb[b][s], acc[b], 0, 0, 0);
-                for (int gg = 1; gg < groups; gg += 2) {
-                    // y holds group gg.
-                    load(gg + 1 < groups ? gg + 1 : gg, xa, xb);
-                    mma(ya, yb);
-                    if (gg + 1 >= groups) break;
-                    // x holds group gg + 1.
-                    load(gg + 2 < groups ? gg + 2 : gg + 1, ya, yb);
-                    mma(xa, xb);
-                }
-            } else {
-#pragma unroll
-                for (int b = 0; b < 4; b++) acc[b] = {0.0f, 0.0f, 0.0f, 0.0f};
-            }
-            // Tail steps: column 16 groups + 4 s + g (qrow, crow start at 4 g).
-            for (int s = 0; s < tail; s++) {
-                const int off = 16 * groups + 4 * s - 3 * g;
-                const float a = qrow[off];
-#pragma unroll
-                for (int b = 0; b < 4; b++)
-                    acc[b] = __builtin_amdgcn_mfma_f32_16x16x4f32(a, crow[16 * b * st + off], acc[b], 0, 0, 0);
-            }
         }
         // Masks: candidates past the tile's end (the last tile of a run) and
         // the query itself (the tile holding it) become NaN, which no
@@ -653,7 +597,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
             // as measured, for its kind of scan.  The counts-first form
             // needs room for a whole tile (64 per row) after a merge, so
             // the 32-slot buffers take the per-block form.
-            if constexpr (IVF || kB < kWave) {
+            if constexpr (IVF || kB < kBuf) {
                 // One 16-column block at a time (at most 16 per row): rows
                 // whose buffer cannot take the block's candidates are merged
                 // first, and the block is then tested against their new
@@ -849,19 +793,42 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     }
 }
 
-int knn_mfma_ksteps(int d) { return (d + 2 + 3) / 4; }
+// The tile kernels are compiled for these k-step counts; a width runs with
+// the smallest that covers its ceil((d + 2) / 4) columns of four (the extra
+// columns are zero in every padded row, so they add nothing).
+static constexpr int kKsBuckets[] = {2, 3, 4, 5, 6, 8, 10, 13, 16, 20, 26, 33};
+
+int knn_mfma_ksteps(int d) {
+    const int need = (d + 2 + 3) / 4;
+    for (int b : kKsBuckets)
+        if (b >= need) return b;
+    return need;  // past the widest: rejected by the launch
+}
 
 int knn_mfma_stride(int d) { return 4 * knn_mfma_ksteps(d) + 4; }
 
-// LDS of a launch: query tile (unless the k-steps are compile-time), two
-// candidate tiles, ids, and (unless k = 1) the row buffers.
-static size_t tile_lds_bytes(int st, bool ks_fixed, bool buffers, bool rot = false) {
-    const size_t tiles = (ks_fixed ? 2 : 3) * (size_t)kTile * st + 2 * kTile;
-    const size_t bufs = buffers ? 2 * (size_t)kTile * (ks_fixed && !rot ? 32 : kBuf) : 0;
+// LDS of a launch: two candidate tiles (the first holds the queries at the
+// start), ids, and (unless k = 1) the row buffers: 32 slots per row, 64 for
+// the ranking.
+static size_t tile_lds_bytes(int st, bool buffers, bool rot) {
+    const size_t tiles = 2 * (size_t)kTile * st + 2 * kTile;
+    const size_t bufs = buffers ? 2 * (size_t)kTile * (rot ? kBuf : 32) : 0;
     return sizeof(float) * (tiles + bufs);
 }
 
-size_t knn_mfma_lds_bytes(int d) { return tile_lds_bytes(knn_mfma_stride(d), false, true); }
+size_t knn_mfma_lds_bytes(int d) { return tile_lds_bytes(knn_mfma_stride(d), true, true); }
+
+template <int KS>
+static void launch_ks(const KnnTiles &T, bool ivf, unsigned grid, hipStream_t s) {
+    if (ivf)
+        k_knn_mfma<true, false, false, KS><<<grid, 256, tile_lds_bytes(T.st, true, false), s>>>(T);
+    else if (T.k == 1)
+        k_knn_mfma<false, true, false, KS><<<grid, 256, tile_lds_bytes(T.st, false, false), s>>>(T);
+    else if (T.start_hint)
+        k_knn_mfma<false, false, true, KS><<<grid, 256, tile_lds_bytes(T.st, true, true), s>>>(T);
+    else
+        k_knn_mfma<false, false, false, KS><<<grid, 256, tile_lds_bytes(T.st, true, false), s>>>(T);
+}
 
 hipError_t launch_knn_pack(const float *X, const uint32_t *order, int64_t n,
                            int d, const float *norms, float *Xp, hipStream_t s) {
@@ -877,23 +844,20 @@ hipError_t launch_knn_tiles(const KnnTiles &T, bool ivf, hipStream_t s) {
         T.ksteps != knn_mfma_ksteps(T.d))
         return hipErrorInvalidValue;
     const unsigned grid = (unsigned)ceil_div(T.nq, kTile);
-    // Straight-line tiles for 47..50 dimensions (13 k-steps), loops otherwise.
-    const bool ks13 = T.ksteps == 13;
-    const size_t lds = tile_lds_bytes(T.st, ks13, true);
-    const size_t lds1 = tile_lds_bytes(T.st, ks13, false);  // k = 1: no buffers
-    if (ivf) {
-        if (ks13) k_knn_mfma<true, false, false, 13><<<grid, 256, lds, s>>>(T);
-        else k_knn_mfma<true, false, false, 0><<<grid, 256, lds, s>>>(T);
-    } else if (T.k == 1) {
-        if (ks13) k_knn_mfma<false, true, false, 13><<<grid, 256, lds1, s>>>(T);
-        else k_knn_mfma<false, true, false, 0><<<grid, 256, lds1, s>>>(T);
-    } else if (T.start_hint) {
-        const size_t ldsr = tile_lds_bytes(T.st, ks13, true, true);
-        if (ks13) k_knn_mfma<false, false, true, 13><<<grid, 256, ldsr, s>>>(T);
-        else k_knn_mfma<false, false, true, 0><<<grid, 256, ldsr, s>>>(T);
-    } else {
-        if (ks13) k_knn_mfma<false, false, false, 13><<<grid, 256, lds, s>>>(T);
-        else k_knn_mfma<false, false, false, 0><<<grid, 256, lds, s>>>(T);
+    switch (T.ksteps) {
+        case 2: launch_ks<2>(T, ivf, grid, s); break;
+        case 3: launch_ks<3>(T, ivf, grid, s); break;
+        case 4: launch_ks<4>(T, ivf, grid, s); break;
+        case 5: launch_ks<5>(T, ivf, grid, s); break;
+        case 6: launch_ks<6>(T, ivf, grid, s); break;
+        case 8: launch_ks<8>(T, ivf, grid, s); break;
+        case 10: launch_ks<10>(T, ivf, grid, s); break;
+        case 13: launch_ks<13>(T, ivf, grid, s); break;
+        case 16: launch_ks<16>(T, ivf, grid, s); break;
+        case 20: launch_ks<20>(T, ivf, grid, s); break;
+        case 26: launch_ks<26>(T, ivf, grid, s); break;
+        case 33: launch_ks<33>(T, ivf, grid, s); break;
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

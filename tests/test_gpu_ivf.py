@@ -281,8 +281,8 @@ def test_ivf_matches_reference_with_few_clusters_searched(name):
     (30001, 50, 20, 200), (20000, 50, 64, 40), (20000, 48, 33, 40)])
 def test_ivf_shapes_against_exact_search(n, d, k, centers):
     # Widths at the edges of the tile engine (1 and 128 dimensions, the
-    # 64-entry lists), the compile-time widths (47..50 dimensions) with
-    # more neighbours than their 32-slot row buffers hold, and a ragged
+    # 64-entry lists), 13-step tiles (48 and 50 dimensions) with more
+    # neighbours than the 32-slot row buffers hold, and a ragged
     # cell count, in the inverted-file mode:
     # structure exact, recall against the exhaustive search high, distances
     # of shared pairs bit-identical to the exhaustive search's.
