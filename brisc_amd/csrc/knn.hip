@@ -256,7 +256,7 @@ __device__ __forceinline__ void merge_row(float (&Ld)[16], uint32_t (&Li)[16],
                                           const float *bd, const uint32_t *bi, int r,
                                           int k, int lane) {
     const int g = r >> 2, i = r & 3;
-    const int c = __shfl(pick(cnt, i), 16 * g);
+    const int c = __builtin_amdgcn_readlane(pick(cnt, i), 16 * g);  // r is wave-uniform
     float B = INFINITY;
     uint32_t BI = 0xffffffffu;
     if (lane < c) {
@@ -284,8 +284,8 @@ __device__ __forceinline__ void merge_row(float (&Ld)[16], uint32_t (&Li)[16],
         cmp_xchg(L, LI, stride, (lane & stride) == 0);
     put(Ld, r, L);
     put(Li, r, LI);
-    const float td = __shfl(L, k - 1);
-    const uint32_t ti = (uint32_t)__shfl((int)LI, k - 1);
+    const float td = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(L), k - 1));
+    const uint32_t ti = (uint32_t)__builtin_amdgcn_readlane((int)LI, k - 1);
     if ((lane >> 4) == g) {
         // A row's cap (a distance no greater than its k-th nearest) bounds
         // the threshold while the list holds fewer than k close entries.
