@@ -237,8 +237,9 @@ struct SegGroup {
     bool fin[R];     // the segment ends row `row`
 };
 
-// Forms the next R segments from `cs` and issues their loads.
-template <int R>
+// Forms the next R segments from `cs` and issues their loads (non-temporal
+// when NT: measured faster for X v, slightly slower for X^T u).
+template <int R, bool NT>
 __device__ __forceinline__ void form_group(const int *off, int nrows,
                                            gquad *data, gquad16 *indices,
                                            int lane, Cursor &cs,
@@ -262,8 +263,13 @@ __device__ __forceinline__ void form_group(const int *off, int nrows,
         G.row[q] = cs.r;
         G.fin[q] = fin && cs.r < nrows;
         const int qi = q0 + lane < lastq ? q0 + lane : lastq;
-        G.c[q].d = data[qi];
-        G.c[q].ix = indices[qi];
+        if (NT) {
+            G.c[q].d = __builtin_nontemporal_load(data + qi);
+            G.c[q].ix = __builtin_nontemporal_load(indices + qi);
+        } else {
+            G.c[q].d = data[qi];
+            G.c[q].ix = indices[qi];
+        }
         if (fin) {
             cs.r = cs.r < nrows ? cs.r + 1 : nrows;
             cs.p = cs.e;
@@ -382,13 +388,13 @@ __global__ __launch_bounds__(256) void k_spmv(Tables T) {
     Cursor cs{0, uniform(off[0]), uniform(off[1])};
     SegGroup<R> ga, gb;
     float carry = 0.0f;
-    form_group<R>(off, nrows, data, indices, lane, cs, ga);
+    form_group<R, true>(off, nrows, data, indices, lane, cs, ga);
     for (;;) {
-        form_group<R>(off, nrows, data, indices, lane, cs, gb);
+        form_group<R, true>(off, nrows, data, indices, lane, cs, gb);
         __builtin_amdgcn_sched_barrier(0);
         spmv_reduce<R>(ga, lds_v, lane, carry, u, sink);
         const bool b_last = cs.r >= nrows;  // gb finished the last row
-        form_group<R>(off, nrows, data, indices, lane, cs, ga);
+        form_group<R, true>(off, nrows, data, indices, lane, cs, ga);
         __builtin_amdgcn_sched_barrier(0);
         spmv_reduce<R>(gb, lds_v, lane, carry, u, sink);
         if (b_last) break;
@@ -439,7 +445,7 @@ __global__ __launch_bounds__(256) void k_reorth_partials(Tables T) {
             if (vec) {
 #pragma unroll
                 for (int t = 0; t < R; t++) {
-                    const v2d x = *(const v2d *)(Uc + 2 * lane + t * 2 * kWave);
+                    const v2d x = __builtin_nontemporal_load((const v2d *)(Uc + 2 * lane + t * 2 * kWave));
                     acc = fma(x.x, ua[t], acc);
                     acc = fma(x.y, ub[t], acc);
                 }
@@ -502,7 +508,7 @@ __global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
         if (vec) {
             double d0 = mean, d1 = mean;
             for (int c = 0; c < j; c++) {
-                const v2d x = *(const v2d *)(U + (int64_t)c * n + r);
+                const v2d x = __builtin_nontemporal_load((const v2d *)(U + (int64_t)c * n + r));
                 d0 = fma(x.x, coef[c + 1], d0);
                 d1 = fma(x.y, coef[c + 1], d1);
             }
@@ -612,13 +618,13 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T) {
         gquad16 *indices = (gquad16 *)pr.idx16 + q0;
         Cursor cs{0, uniform(off[0]), uniform(off[1])};
         SegGroup<R> ga, gb;
-        form_group<R>(off, nrows, data, indices, lane, cs, ga);
+        form_group<R, false>(off, nrows, data, indices, lane, cs, ga);
         for (;;) {
-            form_group<R>(off, nrows, data, indices, lane, cs, gb);
+            form_group<R, false>(off, nrows, data, indices, lane, cs, gb);
             __builtin_amdgcn_sched_barrier(0);
             spmv_t_reduce<R>(ga, uval, lane, g, acc);
             const bool b_last = cs.r >= nrows;  // gb finished the last row
-            form_group<R>(off, nrows, data, indices, lane, cs, ga);
+            form_group<R, false>(off, nrows, data, indices, lane, cs, ga);
             __builtin_amdgcn_sched_barrier(0);
             spmv_t_reduce<R>(gb, uval, lane, g, acc);
             if (b_last) break;
