@@ -93,6 +93,9 @@ struct Tables {
     int64_t max_lds;      // dynamic LDS bytes a workgroup may use
     int spmv_group;       // segments per load group in X v   (2, 4 or 8)
     int spmvt_group;      // segments per load group in X^T u (2, 4 or 8)
+    int spmv_wg;          // workgroups per CU at most for X v   (0 = no cap)
+    int spmvt_wg;         // workgroups per CU at most for X^T u (0 = no cap)
+    int64_t lds_per_cu;   // LDS bytes of one compute unit
 };
 
 // Optional timing hook around each class of launches (null when captured
@@ -116,8 +119,9 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
                              hipEvent_t fork = nullptr,
                              hipEvent_t join = nullptr);
 // One launch of X v or X^T u alone (autotuner timing; problems must be in the
-// Lanczos phase).  which = 0: X v, 1: X^T u; group = 2, 4 or 8.
-hipError_t launch_spmv_variant(const Tables &T, int which, int group,
+// Lanczos phase).  which = 0: X v, 1: X^T u; group = 2, 4 or 8; wg = at most
+// that many workgroups per CU (0 = no cap).
+hipError_t launch_spmv_variant(const Tables &T, int which, int group, int wg,
                                hipStream_t s);
 // Largest dynamic LDS request of any kernel for g genes and subspace m.
 size_t lds_bytes_needed(int g, int m);

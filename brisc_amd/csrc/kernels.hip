@@ -432,33 +432,61 @@ __global__ __launch_bounds__(256) void k_reorth_partials(Tables T) {
         ua[t] = r < nrows ? (double)u32[r0 + r] : 0.0;
         ub[t] = r + 1 < nrows ? (double)u32[r0 + r + 1] : 0.0;
     }
-    for (int c = w; c <= j; c += 4) {
+    int c0 = w;
+    if (w == 0) {
         double acc = 0.0;
-        if (c == 0) {
+#pragma unroll
+        for (int t = 0; t < R; t++) {
+            acc += ua[t];
+            acc += ub[t];
+        }
+        acc = wave_sum(acc);
+        if (lane == 0) pr.P[b] = acc;
+        c0 = 4;
+    }
+    if (vec) {
+        // Two columns (c, c + 4) per pass: 16 loads in flight per wave.
+        for (int c = c0; c <= j; c += 8) {
+            const bool two = c + 4 <= j;
+            const double *Uc = U + (int64_t)(c - 1) * n + r0 + 2 * lane;
+            const double *Ud = two ? Uc + 4 * n : Uc;
+            v2d x[R], y[R];
+#pragma unroll
+            for (int t = 0; t < R; t++) x[t] = __builtin_nontemporal_load((const v2d *)(Uc + t * 2 * kWave));
+            if (two) {
+#pragma unroll
+                for (int t = 0; t < R; t++) y[t] = __builtin_nontemporal_load((const v2d *)(Ud + t * 2 * kWave));
+            }
+            double acc = 0.0, acd = 0.0;
 #pragma unroll
             for (int t = 0; t < R; t++) {
-                acc += ua[t];
-                acc += ub[t];
+                acc = fma(x[t].x, ua[t], acc);
+                acc = fma(x[t].y, ub[t], acc);
             }
-        } else {
-            const double *Uc = U + (int64_t)(c - 1) * n + r0;
-            if (vec) {
+            acc = wave_sum(acc);
+            if (lane == 0) pr.P[(int64_t)c * pr.nb + b] = acc;
+            if (two) {
 #pragma unroll
                 for (int t = 0; t < R; t++) {
-                    const v2d x = __builtin_nontemporal_load((const v2d *)(Uc + 2 * lane + t * 2 * kWave));
-                    acc = fma(x.x, ua[t], acc);
-                    acc = fma(x.y, ub[t], acc);
+                    acd = fma(y[t].x, ua[t], acd);
+                    acd = fma(y[t].y, ub[t], acd);
                 }
-            } else {
-#pragma unroll
-                for (int t = 0; t < R; t++) {
-                    const int r = 2 * lane + t * 2 * kWave;
-                    const double x0 = r < nrows ? Uc[r] : 0.0;
-                    const double x1 = r + 1 < nrows ? Uc[r + 1] : 0.0;
-                    acc = fma(x0, ua[t], acc);
-                    acc = fma(x1, ub[t], acc);
-                }
+                acd = wave_sum(acd);
+                if (lane == 0) pr.P[(int64_t)(c + 4) * pr.nb + b] = acd;
             }
+        }
+        return;
+    }
+    for (int c = c0; c <= j; c += 4) {
+        const double *Uc = U + (int64_t)(c - 1) * n + r0;
+        double acc = 0.0;
+#pragma unroll
+        for (int t = 0; t < R; t++) {
+            const int r = 2 * lane + t * 2 * kWave;
+            const double x0 = r < nrows ? Uc[r] : 0.0;
+            const double x1 = r + 1 < nrows ? Uc[r + 1] : 0.0;
+            acc = fma(x0, ua[t], acc);
+            acc = fma(x1, ub[t], acc);
         }
         acc = wave_sum(acc);
         if (lane == 0) pr.P[(int64_t)c * pr.nb + b] = acc;
@@ -502,21 +530,33 @@ __global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
     const float *u32 = pr.u32;
     double acc = 0.0;
     const bool vec = r0 + kRowBlock <= n && (n & 1) == 0;
+    static_assert(kRowBlock == 1024, "two 512-row halves per block");
+    if (vec) {
+        // Both halves advance together: two 16-byte loads per column.
+        const int64_t ra = r0 + 2 * threadIdx.x, rb = ra + 512;
+        double d0 = mean, d1 = mean, d2 = mean, d3 = mean;
+#pragma unroll 4
+        for (int c = 0; c < j; c++) {
+            const v2d xa = __builtin_nontemporal_load((const v2d *)(U + (int64_t)c * n + ra));
+            const v2d xb = __builtin_nontemporal_load((const v2d *)(U + (int64_t)c * n + rb));
+            const double k = coef[c + 1];
+            d0 = fma(xa.x, k, d0);
+            d1 = fma(xa.y, k, d1);
+            d2 = fma(xb.x, k, d2);
+            d3 = fma(xb.y, k, d3);
+        }
+        const double x0 = (double)u32[ra] - d0, x1 = (double)u32[ra + 1] - d1;
+        const double x2 = (double)u32[rb] - d2, x3 = (double)u32[rb + 1] - d3;
+        *(v2d *)(Uj + ra) = v2d{x0, x1};
+        *(v2d *)(Uj + rb) = v2d{x2, x3};
+        acc = fma(x0, x0, acc);
+        acc = fma(x1, x1, acc);
+        acc = fma(x2, x2, acc);
+        acc = fma(x3, x3, acc);
+    } else {
 #pragma unroll
-    for (int t = 0; t < kRowBlock / 512; t++) {
-        const int64_t r = r0 + 2 * threadIdx.x + t * 512;
-        if (vec) {
-            double d0 = mean, d1 = mean;
-            for (int c = 0; c < j; c++) {
-                const v2d x = __builtin_nontemporal_load((const v2d *)(U + (int64_t)c * n + r));
-                d0 = fma(x.x, coef[c + 1], d0);
-                d1 = fma(x.y, coef[c + 1], d1);
-            }
-            const double x0 = (double)u32[r] - d0, x1 = (double)u32[r + 1] - d1;
-            *(v2d *)(Uj + r) = v2d{x0, x1};
-            acc = fma(x0, x0, acc);
-            acc = fma(x1, x1, acc);
-        } else {
+        for (int t = 0; t < kRowBlock / 512; t++) {
+            const int64_t r = r0 + 2 * threadIdx.x + t * 512;
 #pragma unroll
             for (int e = 0; e < 2; e++) {
                 if (r + e < n) {
@@ -1299,6 +1339,16 @@ __global__ void k_finish(Tables T, int *done_counter) {
 
 // ================================================================ launchers
 
+// Dynamic LDS of a sparse-product launch: what the kernel needs, raised when
+// `wg` > 0 so that wg + 1 workgroups no longer fit on a CU of `lds_per_cu`
+// bytes (and wg still do).  Fewer resident waves contend less for LDS banks
+// in X^T u's read-modify-write accumulation; the autotuner picks the cap.
+static size_t capped_lds(size_t need, int wg, int64_t lds_per_cu) {
+    if (wg <= 0 || lds_per_cu <= 0) return need;
+    const size_t cap = (size_t)lds_per_cu / (size_t)(wg + 1) + 256;
+    return need > cap ? need : cap;
+}
+
 size_t lds_bytes_needed(int g, int m) {
     size_t need = spmv_t_lds_bytes(g);
     need = std::max(need, spmv_lds_bytes(g));
@@ -1308,7 +1358,7 @@ size_t lds_bytes_needed(int g, int m) {
     return need;
 }
 
-hipError_t launch_spmv_variant(const Tables &T, int which, int group,
+hipError_t launch_spmv_variant(const Tables &T, int which, int group, int wg,
                                hipStream_t s) {
     Tables V = T;
     V.spmv_group = group;
@@ -1316,13 +1366,14 @@ hipError_t launch_spmv_variant(const Tables &T, int which, int group,
     const unsigned nblk = (unsigned)T.num_blocks;
     const int g = T.max_g;
     if (which == 0) {
+        const size_t lds = capped_lds(spmv_lds_bytes(g), wg, T.lds_per_cu);
         switch (group) {
-            case 2: k_spmv<2><<<nblk, 256, spmv_lds_bytes(g), s>>>(V); break;
-            case 8: k_spmv<8><<<nblk, 256, spmv_lds_bytes(g), s>>>(V); break;
-            default: k_spmv<4><<<nblk, 256, spmv_lds_bytes(g), s>>>(V); break;
+            case 2: k_spmv<2><<<nblk, 256, lds, s>>>(V); break;
+            case 8: k_spmv<8><<<nblk, 256, lds, s>>>(V); break;
+            default: k_spmv<4><<<nblk, 256, lds, s>>>(V); break;
         }
     } else {
-        const size_t tlds = spmv_t_lds_bytes(g);
+        const size_t tlds = capped_lds(spmv_t_lds_bytes(g), wg, T.lds_per_cu);
         switch (group) {
             case 2: k_spmv_t<2><<<nblk, 256, tlds, s>>>(V); break;
             case 8: k_spmv_t<8><<<nblk, 256, tlds, s>>>(V); break;
@@ -1374,10 +1425,11 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
         mark(4, 1);
     }
     mark(0, 0);
+    const size_t slds = capped_lds(spmv_lds_bytes(g), T.spmv_wg, T.lds_per_cu);
     switch (T.spmv_group) {
-        case 2: k_spmv<2><<<nblk, 256, spmv_lds_bytes(g), s>>>(T); break;
-        case 8: k_spmv<8><<<nblk, 256, spmv_lds_bytes(g), s>>>(T); break;
-        default: k_spmv<4><<<nblk, 256, spmv_lds_bytes(g), s>>>(T); break;
+        case 2: k_spmv<2><<<nblk, 256, slds, s>>>(T); break;
+        case 8: k_spmv<8><<<nblk, 256, slds, s>>>(T); break;
+        default: k_spmv<4><<<nblk, 256, slds, s>>>(T); break;
     }
     mark(0, 1);
     mark(2, 0);
@@ -1387,7 +1439,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     k_alpha<<<T.num_probs, 256, 0, s>>>(T);
     mark(2, 1);
     mark(1, 0);
-    const size_t tlds = spmv_t_lds_bytes(g);
+    const size_t tlds = capped_lds(spmv_t_lds_bytes(g), T.spmvt_wg, T.lds_per_cu);
     switch (T.spmvt_group) {
         case 2: k_spmv_t<2><<<nblk, 256, tlds, s>>>(T); break;
         case 8: k_spmv_t<8><<<nblk, 256, tlds, s>>>(T); break;

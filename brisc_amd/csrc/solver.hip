@@ -199,6 +199,8 @@ struct bpca_context {
     int64_t k_count[BPCA_NUM_KERNEL_CLASSES] = {};
     bool use_graph = true;
     int spmv_group = 4, spmvt_group = 4;   // autotuned load-group sizes
+    int spmv_wg = 0, spmvt_wg = 0;         // autotuned workgroups-per-CU caps
+    int64_t lds_per_cu = 0;                // LDS bytes of one compute unit
 };
 
 namespace {
@@ -376,6 +378,9 @@ int engine_rebuild_tables(bpca_engine *E) {
     E->T.max_lds = (int64_t)ctx->max_lds;
     E->T.spmv_group = ctx->spmv_group;
     E->T.spmvt_group = ctx->spmvt_group;
+    E->T.spmv_wg = ctx->spmv_wg;
+    E->T.spmvt_wg = ctx->spmvt_wg;
+    E->T.lds_per_cu = ctx->lds_per_cu;
     engine_drop_graph(E);
     E->dirty = false;
     return BPCA_OK;
@@ -517,6 +522,9 @@ int bpca_context_create(int device, uint64_t hbm_budget_bytes,
     int lds = 0;
     if (hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && lds > 0)
         ctx->max_lds = (size_t)lds;
+    int lds_cu = 0;
+    if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess && lds_cu > 0)
+        ctx->lds_per_cu = lds_cu;
     const char *env = std::getenv("BPCA_NO_GRAPH");
     ctx->use_graph = !(env && env[0] == '1');
     *out = ctx;
@@ -565,6 +573,26 @@ int bpca_context_get_kernel_config(bpca_context *ctx, int32_t *spmv_group,
     if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (spmv_group) *spmv_group = ctx->spmv_group;
     if (spmvt_group) *spmvt_group = ctx->spmvt_group;
+    return BPCA_OK;
+}
+
+int bpca_context_set_occupancy(bpca_context *ctx, int32_t spmv_wg,
+                               int32_t spmvt_wg) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    auto ok = [](int v) { return v >= 0 && v <= 16; };
+    if (!ok(spmv_wg) || !ok(spmvt_wg))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "workgroups per CU must be 0 (no cap) to 16");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    ctx->spmv_wg = spmv_wg;
+    ctx->spmvt_wg = spmvt_wg;
+    return BPCA_OK;
+}
+
+int bpca_context_get_occupancy(bpca_context *ctx, int32_t *spmv_wg,
+                               int32_t *spmvt_wg) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (spmv_wg) *spmv_wg = ctx->spmv_wg;
+    if (spmvt_wg) *spmvt_wg = ctx->spmvt_wg;
     return BPCA_OK;
 }
 
@@ -794,7 +822,8 @@ int bpca_engine_step(bpca_engine *E, int32_t max_steps, int32_t *num_done) {
     DeviceGuard dg(ctx->device);
     hipStream_t s = ctx->stream;
     begin_call(ctx);
-    if (E->T.spmv_group != ctx->spmv_group || E->T.spmvt_group != ctx->spmvt_group)
+    if (E->T.spmv_group != ctx->spmv_group || E->T.spmvt_group != ctx->spmvt_group ||
+        E->T.spmv_wg != ctx->spmv_wg || E->T.spmvt_wg != ctx->spmvt_wg)
         E->dirty = true;
     if (E->dirty) BPCA_TRY(engine_rebuild_tables(E));
     if (E->setup_pending) {
@@ -901,33 +930,45 @@ int bpca_autotune(bpca_context *ctx, const bpca_problem *problem, int32_t reps,
     hipEvent_t e0 = nullptr, e1 = nullptr;
     HIP_TRY(hipEventCreate(&e0));
     HIP_TRY(hipEventCreate(&e1));
-    const int groups[3] = {2, 4, 8};
-    double ms[2][3] = {};
     hipError_t e = hipSuccess;
+    // Average of `reps` launches after one untimed launch.
+    auto time_variant = [&](int which, int group, int wg) -> double {
+        e = bpca::launch_spmv_variant(E->T, which, group, wg, s);
+        if (e == hipSuccess) e = hipEventRecord(e0, s);
+        for (int r = 0; r < reps && e == hipSuccess; r++)
+            e = bpca::launch_spmv_variant(E->T, which, group, wg, s);
+        if (e == hipSuccess) e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float t = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
+        return (double)t / reps;
+    };
+    // Every pair of group size and cap on the workgroups per CU (0 = none):
+    // fewer resident waves can contend less for LDS banks and memory
+    // channels, and then deeper load groups may pay.
+    const int groups[3] = {2, 4, 8};
+    const int caps[4] = {0, 4, 3, 2};
+    double ms[2][4][3] = {};
+    int best[2][2] = {{0, 1}, {0, 1}};  // (cap index, group index)
     for (int which = 0; which < 2 && e == hipSuccess; which++)
-        for (int gi = 0; gi < 3 && e == hipSuccess; gi++) {
-            e = bpca::launch_spmv_variant(E->T, which, groups[gi], s);
-            if (e == hipSuccess) e = hipEventRecord(e0, s);
-            for (int r = 0; r < reps && e == hipSuccess; r++)
-                e = bpca::launch_spmv_variant(E->T, which, groups[gi], s);
-            if (e == hipSuccess) e = hipEventRecord(e1, s);
-            if (e == hipSuccess) e = hipEventSynchronize(e1);
-            float t = 0.0f;
-            if (e == hipSuccess) e = hipEventElapsedTime(&t, e0, e1);
-            ms[which][gi] = t / reps;
-        }
+        for (int ci = 0; ci < 4 && e == hipSuccess; ci++)
+            for (int gi = 0; gi < 3 && e == hipSuccess; gi++) {
+                ms[which][ci][gi] = time_variant(which, groups[gi], caps[ci]);
+                if (ms[which][ci][gi] < ms[which][best[which][0]][best[which][1]]) {
+                    best[which][0] = ci;
+                    best[which][1] = gi;
+                }
+            }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     if (e != hipSuccess) return hip_fail(e, "autotune timing");
-    int best[2] = {1, 1};
-    for (int which = 0; which < 2; which++)
-        for (int gi = 0; gi < 3; gi++)
-            if (ms[which][gi] < ms[which][best[which]]) best[which] = gi;
-    ctx->spmv_group = groups[best[0]];
-    ctx->spmvt_group = groups[best[1]];
+    ctx->spmv_group = groups[best[0][1]];
+    ctx->spmvt_group = groups[best[1][1]];
+    ctx->spmv_wg = caps[best[0][0]];
+    ctx->spmvt_wg = caps[best[1][0]];
     if (timings_ms)
         for (int which = 0; which < 2; which++)
-            for (int gi = 0; gi < 3; gi++) timings_ms[which * 3 + gi] = ms[which][gi];
+            for (int gi = 0; gi < 3; gi++) timings_ms[which * 3 + gi] = ms[which][0][gi];
     return BPCA_OK;
 }
 

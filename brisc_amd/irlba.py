@@ -143,10 +143,24 @@ class Context:
             self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def set_occupancy(self, spmv_wg: int, spmvt_wg: int) -> None:
+        """At most this many workgroups per compute unit for X v / X^T u
+        (0 = no cap; speed only)."""
+        _lib.check(_lib.lib().bpca_context_set_occupancy(
+            self._h, int(spmv_wg), int(spmvt_wg)))
+
+    def occupancy(self) -> tuple:
+        a = ctypes.c_int32(0)
+        b = ctypes.c_int32(0)
+        _lib.check(_lib.lib().bpca_context_get_occupancy(
+            self._h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
     def autotune(self, data, indices, indptr, num_cells, num_genes,
                  subspace_size=100, reps=5) -> dict:
-        """Time every load-group size of X v and X^T u on this matrix and
-        keep the fastest (results are bit-identical for all sizes)."""
+        """Time every load-group size of X v and X^T u on this matrix, then
+        workgroup-per-CU caps at the fastest size, and keep the fastest
+        (results are bit-identical for every configuration)."""
         import torch
         scratch = torch.empty((num_cells, 1), dtype=torch.float32,
                               device=data.device)
@@ -158,7 +172,8 @@ class Context:
         groups = (2, 4, 8)
         return {'spmv_ms': dict(zip(groups, t[:3].round(4).tolist())),
                 'spmv_t_ms': dict(zip(groups, t[3:].round(4).tolist())),
-                'chosen': self.kernel_config()}
+                'chosen': self.kernel_config(),
+                'workgroups_per_cu': self.occupancy()}
 
     def cancel(self) -> None:
         """Ask a running solve on this context to stop at its next step."""

@@ -94,6 +94,15 @@ int bpca_context_set_kernel_config(bpca_context *ctx, int32_t spmv_group,
                                    int32_t spmvt_group);
 int bpca_context_get_kernel_config(bpca_context *ctx, int32_t *spmv_group,
                                    int32_t *spmvt_group);
+/* Caps on the workgroups per compute unit of the X v and X^T u launches
+ * (0 = no cap, else 1 to 16), applied by raising the launch's LDS request
+ * so that one more workgroup no longer fits.  Speed only, like the load
+ * groups: no order of summation depends on them.  No counterpart in the
+ * reference (a device-occupancy knob); bpca_autotune sets them. */
+int bpca_context_set_occupancy(bpca_context *ctx, int32_t spmv_wg,
+                               int32_t spmvt_wg);
+int bpca_context_get_occupancy(bpca_context *ctx, int32_t *spmv_wg,
+                               int32_t *spmvt_wg);
 /* Bytes of device workspace a solve of this shape allocates. */
 uint64_t bpca_workspace_bytes(uint64_t num_cells, uint32_t num_genes,
                               uint32_t k, uint32_t subspace_size);
@@ -184,9 +193,11 @@ int bpca_irlba_batched(bpca_context *ctx, int32_t num_problems,
 
 /* Autotuner: time each load-group size of both kernels on `problem` (a
  * throwaway solve state; the problem's arrays are only read and its PCs
- * buffer is not written), and make the fastest the context's configuration.
+ * buffer is not written), each with no cap and with caps of 4, 3 and 2
+ * workgroups per compute unit, and make the fastest pair the context's
+ * configuration (bpca_context_get_kernel_config / _get_occupancy).
  * timings_ms (may be NULL) receives 6 averages: X v at 2, 4, 8 then X^T u
- * at 2, 4, 8. */
+ * at 2, 4, 8, all without a cap. */
 int bpca_autotune(bpca_context *ctx, const bpca_problem *problem, int32_t reps,
                   double *timings_ms);
 

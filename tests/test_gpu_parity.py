@@ -442,8 +442,9 @@ def test_mfma_tall_gemm_matches_numpy():
 
 
 def test_autotuned_load_groups_do_not_change_bits():
-    """The autotuner's knob (rows per load group of X v / X^T u) changes
-    speed only: every configuration gives the same bits."""
+    """The autotuner's knobs (rows per load group of X v / X^T u, and caps
+    on their workgroups per compute unit) change speed only: every
+    configuration gives the same bits."""
     from brisc_amd import Context
     w = synth.make_workload(88, 20_000, num_types=10)
     d, i, p = to_device(*oracle.generate_csr(w))
@@ -453,15 +454,21 @@ def test_autotuned_load_groups_do_not_change_bits():
     print('autotune', tuned, flush=True)
     assert tuned['chosen'][0] in (2, 4, 8) and tuned['chosen'][1] in (2, 4, 8)
     assert all(v > 0 for v in tuned['spmv_ms'].values())
+    assert all(c in (0, 2, 3, 4) for c in tuned['workgroups_per_cu'])
     ref = None
-    for cfg in [(4, 4), (2, 8), (8, 2), (2, 2), (8, 8)]:
+    for cfg, occ in [((4, 4), (0, 0)), ((2, 8), (0, 0)), ((8, 2), (0, 0)),
+                     ((2, 2), (4, 2)), ((8, 8), (1, 3)), ((4, 2), (2, 1))]:
         ctx.set_kernel_config(*cfg)
+        ctx.set_occupancy(*occ)
+        assert ctx.occupancy() == occ
         P, sv, st = irlba_device(d, i, p, w.num_cells, w.num_genes, 20,
                                  subspace_size=50, context=ctx)
         bits = P.cpu().numpy().view(np.uint32)
         if ref is None:
             ref = bits
-        assert np.array_equal(bits, ref), cfg
+        assert np.array_equal(bits, ref), (cfg, occ)
+    with pytest.raises(ValueError, match='workgroups per CU'):
+        ctx.set_occupancy(17, 0)
     ctx.close()
 
 
