@@ -1341,11 +1341,13 @@ __global__ void k_finish(Tables T, int *done_counter) {
 
 // Dynamic LDS of a sparse-product launch: what the kernel needs, raised when
 // `wg` > 0 so that wg + 1 workgroups no longer fit on a CU of `lds_per_cu`
-// bytes (and wg still do).  Fewer resident waves contend less for LDS banks
-// in X^T u's read-modify-write accumulation; the autotuner picks the cap.
-static size_t capped_lds(size_t need, int wg, int64_t lds_per_cu) {
-    if (wg <= 0 || lds_per_cu <= 0) return need;
-    const size_t cap = (size_t)lds_per_cu / (size_t)(wg + 1) + 256;
+// bytes (and wg still do), unless that exceeds what a workgroup may request.
+// Fewer resident waves contend less for LDS banks in X^T u's
+// read-modify-write accumulation; the autotuner picks the cap.
+static size_t capped_lds(size_t need, int wg, const Tables &T) {
+    if (wg <= 0 || T.lds_per_cu <= 0) return need;
+    const size_t cap = (size_t)T.lds_per_cu / (size_t)(wg + 1) + 256;
+    if (cap > (size_t)T.max_lds) return need;
     return need > cap ? need : cap;
 }
 
@@ -1366,14 +1368,14 @@ hipError_t launch_spmv_variant(const Tables &T, int which, int group, int wg,
     const unsigned nblk = (unsigned)T.num_blocks;
     const int g = T.max_g;
     if (which == 0) {
-        const size_t lds = capped_lds(spmv_lds_bytes(g), wg, T.lds_per_cu);
+        const size_t lds = capped_lds(spmv_lds_bytes(g), wg, T);
         switch (group) {
             case 2: k_spmv<2><<<nblk, 256, lds, s>>>(V); break;
             case 8: k_spmv<8><<<nblk, 256, lds, s>>>(V); break;
             default: k_spmv<4><<<nblk, 256, lds, s>>>(V); break;
         }
     } else {
-        const size_t tlds = capped_lds(spmv_t_lds_bytes(g), wg, T.lds_per_cu);
+        const size_t tlds = capped_lds(spmv_t_lds_bytes(g), wg, T);
         switch (group) {
             case 2: k_spmv_t<2><<<nblk, 256, tlds, s>>>(V); break;
             case 8: k_spmv_t<8><<<nblk, 256, tlds, s>>>(V); break;
@@ -1425,7 +1427,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
         mark(4, 1);
     }
     mark(0, 0);
-    const size_t slds = capped_lds(spmv_lds_bytes(g), T.spmv_wg, T.lds_per_cu);
+    const size_t slds = capped_lds(spmv_lds_bytes(g), T.spmv_wg, T);
     switch (T.spmv_group) {
         case 2: k_spmv<2><<<nblk, 256, slds, s>>>(T); break;
         case 8: k_spmv<8><<<nblk, 256, slds, s>>>(T); break;
@@ -1439,7 +1441,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     k_alpha<<<T.num_probs, 256, 0, s>>>(T);
     mark(2, 1);
     mark(1, 0);
-    const size_t tlds = capped_lds(spmv_t_lds_bytes(g), T.spmvt_wg, T.lds_per_cu);
+    const size_t tlds = capped_lds(spmv_t_lds_bytes(g), T.spmvt_wg, T);
     switch (T.spmvt_group) {
         case 2: k_spmv_t<2><<<nblk, 256, tlds, s>>>(T); break;
         case 8: k_spmv_t<8><<<nblk, 256, tlds, s>>>(T); break;
