@@ -1140,6 +1140,18 @@ size_t gemm_lds_bytes(int m) {
     return sizeof(double) * (size_t)((m + 3) & ~3) * (size_t)slab_ld((m + 15) & ~15);
 }
 
+// Column-major FP64 outputs go through a per-wave LDS tile (16 x 16, column
+// stride 17 doubles so that the lanes writing one row land in different
+// banks): written in the matrix-core layout, read back so that each global
+// store covers 16 consecutive rows of 4 columns, whole 128-byte lines
+// instead of 32 bytes per column (whose line fills cost ~3.5 GB of reads per
+// 10M-cell restart product).
+constexpr int kStageLd = 17;
+constexpr int kStageDoubles = 16 * kStageLd;
+size_t gemm_stage_lds_bytes(int m) {
+    return gemm_lds_bytes(m) + sizeof(double) * (size_t)kStageDoubles * (kGemmThreads / kWave);
+}
+
 namespace {
 
 // coef slab in LDS: cs[l * ldc + c], zero-padded to mpad x npad.  The
@@ -1214,19 +1226,55 @@ __device__ inline void mfma_row_tiles(const double *__restrict__ A, int64_t n,
             }
             consume(ab0);
 #pragma unroll
-            for (int t = 0; t < kPassTiles; t++) {
-                if (t < nt) {
-#pragma unroll
-                    for (int v = 0; v < 4; v++) store(r0 + kq + 4 * v, (t0 + t) * 16 + i, acc[t][v]);
-                }
-            }
+            for (int t = 0; t < kPassTiles; t++)
+                if (t < nt) store(r0, t0 + t, acc[t]);
         }
     }
 }
 
+// Tile stores for mfma_row_tiles: D[i][j] of a 16 x 16 tile is in lane
+// j + 16 (i % 4), element i / 4.  `Elem` stores one element (row, col, v).
+template <typename Elem>
+struct TileByElement {
+    Elem elem;
+    __device__ void operator()(int64_t r0, int tile, const v4d &d) const {
+        const int lane = threadIdx.x & (kWave - 1);
+        const int i = lane & 15, kq = lane >> 4;
+#pragma unroll
+        for (int v = 0; v < 4; v++) elem(r0 + kq + 4 * v, tile * 16 + i, d[v]);
+    }
+};
+template <typename Elem>
+__device__ inline TileByElement<Elem> by_element(Elem e) { return TileByElement<Elem>{e}; }
+
+// Column-major FP64 output (out[col * n + row], rows < row_end, cols < kout)
+// through the wave's LDS tile `st` (kStageDoubles).
+struct TileStaged {
+    double *out;
+    int64_t n, row_end;
+    int kout;
+    double *st;
+    __device__ void operator()(int64_t r0, int tile, const v4d &d) const {
+        const int lane = threadIdx.x & (kWave - 1);
+        const int i = lane & 15, kq = lane >> 4;
+#pragma unroll
+        for (int v = 0; v < 4; v++) st[i * kStageLd + kq + 4 * v] = d[v];
+        __builtin_amdgcn_wave_barrier();
+        const int64_t row = r0 + i;
+#pragma unroll
+        for (int w = 0; w < 4; w++) {
+            const int c = kq + 4 * w;
+            const double y = st[c * kStageLd + i];
+            const int col = tile * 16 + c;
+            if (row < row_end && col < kout) out[(int64_t)col * n + row] = y;
+        }
+        __builtin_amdgcn_wave_barrier();
+    }
+};
+
 }  // namespace
 
-template <bool kFinal>
+template <bool kFinal, bool kStage = false>
 __global__ __launch_bounds__(kGemmThreads) void k_tall_gemm_mfma(Tables T) {
     extern __shared__ double cs[];
     const int2 bp = T.blocks[blockIdx.x];
@@ -1254,19 +1302,24 @@ __global__ __launch_bounds__(kGemmThreads) void k_tall_gemm_mfma(Tables T) {
         float *out = pr.PCs;
         const int k = pr.k;
         mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wave_id(), nw,
-                       [&](int64_t row, int col, double v) {
+                       by_element([&](int64_t row, int col, double v) {
                            if (row < row_end && col < kout) out[row * k + col] = (float)v;
-                       });
+                       }));
+    } else if (kStage) {
+        double *st = cs + mpad * ldc + wave_id() * kStageDoubles;
+        mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wave_id(), nw,
+                       TileStaged{pr.U[pr.ubuf ^ 1], n, row_end, kout, st});
     } else {
         double *out = pr.U[pr.ubuf ^ 1];
         mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wave_id(), nw,
-                       [&](int64_t row, int col, double v) {
+                       by_element([&](int64_t row, int col, double v) {
                            if (row < row_end && col < kout) out[(int64_t)col * n + row] = v;
-                       });
+                       }));
     }
 }
 
 // Test entry: out (n x kout, column-major) = A (n x m) C (m x kout).
+template <bool kStage>
 __global__ __launch_bounds__(kGemmThreads) void k_gemm_mfma_test(const double *A,
                                                                  int64_t n, int m,
                                                                  const double *C,
@@ -1281,10 +1334,16 @@ __global__ __launch_bounds__(kGemmThreads) void k_gemm_mfma_test(const double *A
     __syncthreads();
     const int64_t row0 = (int64_t)blockIdx.x * kRowBlock;
     const int64_t row_end = row0 + kRowBlock < n ? row0 + kRowBlock : n;
-    mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wave_id(), kGemmThreads / kWave,
-                   [&](int64_t row, int col, double v) {
-                       if (row < row_end && col < kout) out[(int64_t)col * n + row] = v;
-                   });
+    if (kStage) {
+        double *st = cs + mpad * ldc + wave_id() * kStageDoubles;
+        mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wave_id(), kGemmThreads / kWave,
+                       TileStaged{out, n, row_end, kout, st});
+    } else {
+        mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wave_id(), kGemmThreads / kWave,
+                       by_element([&](int64_t row, int col, double v) {
+                           if (row < row_end && col < kout) out[(int64_t)col * n + row] = v;
+                       }));
+    }
 }
 
 hipError_t launch_gemm_mfma_test(const double *A, int64_t n, int m,
@@ -1292,7 +1351,16 @@ hipError_t launch_gemm_mfma_test(const double *A, int64_t n, int m,
                                  hipStream_t s) {
     if (m > 4 * kMfmaMaxSteps || kout > 16 * kMfmaMaxTiles || m < 1 || kout < 1) return hipErrorInvalidValue;
     const size_t lds = sizeof(double) * (size_t)((m + 3) & ~3) * slab_ld((kout + 15) & ~15);
-    k_gemm_mfma_test<<<(unsigned)ceil_div(n, kRowBlock), kGemmThreads, lds, s>>>(A, n, m, C, kout, out);
+    const size_t slds = lds + sizeof(double) * (size_t)kStageDoubles * (kGemmThreads / kWave);
+    int dev = 0, max_lds = 0;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&max_lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+    if (e != hipSuccess) return e;
+    const unsigned grid = (unsigned)ceil_div(n, kRowBlock);
+    if ((int64_t)slds <= (int64_t)max_lds)
+        k_gemm_mfma_test<true><<<grid, kGemmThreads, slds, s>>>(A, n, m, C, kout, out);
+    else
+        k_gemm_mfma_test<false><<<grid, kGemmThreads, lds, s>>>(A, n, m, C, kout, out);
     return hipGetLastError();
 }
 
@@ -1468,9 +1536,11 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
         mark(4, 1);
     }
     mark(4, 0);
-    const size_t mfma_lds = gemm_lds_bytes(m);
+    const size_t mfma_lds = gemm_lds_bytes(m), stage_lds = gemm_stage_lds_bytes(m);
     const bool mfma = m <= 4 * kMfmaMaxSteps && (int64_t)mfma_lds <= T.max_lds;
-    if (mfma)
+    if (mfma && (int64_t)stage_lds <= T.max_lds)
+        k_tall_gemm_mfma<false, true><<<nblk, kGemmThreads, stage_lds, s>>>(T);
+    else if (mfma)
         k_tall_gemm_mfma<false><<<nblk, kGemmThreads, mfma_lds, s>>>(T);
     else
         k_tall_gemm<false><<<nblk, 256, sizeof(double) * m * kGemmCols, s>>>(T);

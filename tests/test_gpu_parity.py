@@ -543,12 +543,14 @@ def test_device_subset_rejects_bad_input():
 def test_restart_product_on_device_arrays_matches_numpy():
     # bpca_test_gemm_device (the restart / final product on device arrays)
     # against NumPy in FP64, ragged sizes (rows not a multiple of the 16-row
-    # tile or the 1024-row block, widths not multiples of 4 or 16).
+    # tile or the 1024-row block, widths not multiples of 4 or 16); m = 128
+    # leaves no LDS for the staged column-major stores, the others use them.
     import ctypes
     torch = _torch()
     from brisc_amd import _lib, default_context
     rng = np.random.default_rng(3)
-    for n, m, kout in [(1, 1, 1), (3000, 37, 19), (5000, 100, 99), (2049, 128, 128)]:
+    for n, m, kout in [(1, 1, 1), (3000, 37, 19), (5000, 100, 99), (20000, 100, 92),
+                       (2049, 128, 128)]:
         A = rng.standard_normal((n, m))
         C = rng.standard_normal((m, kout))
         dA = torch.from_numpy(np.asfortranarray(A).ravel(order='F')).cuda()
