@@ -1144,8 +1144,8 @@ size_t gemm_lds_bytes(int m) {
 // stride 17 doubles so that the lanes writing one row land in different
 // banks): written in the matrix-core layout, read back so that each global
 // store covers 16 consecutive rows of 4 columns, whole 128-byte lines
-// instead of 32 bytes per column (whose line fills cost ~3.5 GB of reads per
-// 10M-cell restart product).
+// instead of 32 bytes per column (measured 15% less time in the restart
+// product on 10M cells; the bytes read and written are unchanged).
 constexpr int kStageLd = 17;
 constexpr int kStageDoubles = 16 * kStageLd;
 size_t gemm_stage_lds_bytes(int m) {
