@@ -1128,7 +1128,7 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 constexpr int kMfmaMaxSteps = 32;  // m <= 128
 constexpr int kMfmaMaxTiles = 8;   // kout <= 128
 constexpr int kChunkSteps = 4;
-constexpr int kGemmThreads = 1024;
+constexpr int kGemmThreads = 768;  // 12 waves: 3 per SIMD, ~168 VGPRs each
 
 // Row stride of the coefficient slab: the four k-rows read by one B operand
 // load land on different LDS bank halves when the stride is an odd multiple
@@ -1156,9 +1156,11 @@ namespace {
 
 // coef slab in LDS: cs[l * ldc + c], zero-padded to mpad x npad.  The
 // output columns of a tile go in passes of up to kPassTiles 16-column tiles
-// (the A fragments are re-read per pass, from cache), which keeps the
-// accumulators within the register budget of 16 waves per CU.
-constexpr int kPassTiles = 6;
+// (the A fragments are read again for a second pass).  Seven tiles cover
+// kout <= 112, so a restart at m = 100 reads U once; their accumulators fit
+// the register budget of 12 waves per CU (16 waves with six tiles per pass
+// read U twice whenever kc > 96).
+constexpr int kPassTiles = 7;
 
 template <typename Store>
 __device__ inline void mfma_row_tiles(const double *__restrict__ A, int64_t n,
