@@ -796,8 +796,11 @@ __global__ __launch_bounds__(256) void k_vside_fin(Tables T) {
 
 namespace {
 
+// pos - 1 + round < 2 (mm - 1) for pos < mm and round < mm - 1, so one
+// conditional subtraction is the remainder.
 __device__ inline int rr_player(int pos, int round, int mm) {
-    return pos == 0 ? 0 : ((pos - 1 + round) % (mm - 1)) + 1;
+    const int v = pos - 1 + round;
+    return pos == 0 ? 0 : (v >= mm - 1 ? v - (mm - 1) : v) + 1;
 }
 
 // W (m x mm) and Jv (mm x mm) column-major; m rows of W are live.
@@ -863,7 +866,7 @@ __device__ void jacobi_svd(double *W, double *Jv, int m, int mm, int *sweeps,
 // Same rotations with W and Jv in LDS and one 16-lane group per pair (up to
 // blockDim/16 pairs at once, all m/2 pairs of a round for m <= 128 with 1024
 // threads).  Partial sums over a pair's rows run lane-strided, then the xor
-// butterfly 8,4,2,1 inside the group.
+// butterfly 8,4,2,1 inside the group (DPP moves within the 16-lane row).
 __device__ void jacobi_svd_lds(double *W, double *Jv, int m, int mm,
                                int *sweeps, int *rot_count) {
     const int sub = threadIdx.x & 15;
@@ -893,9 +896,9 @@ __device__ void jacobi_svd_lds(double *W, double *Jv, int m, int mm,
                 }
 #pragma unroll
                 for (int off = 8; off > 0; off >>= 1) {
-                    a += __shfl_xor(a, off);
-                    b += __shfl_xor(b, off);
-                    c += __shfl_xor(c, off);
+                    a += xor_lane_f64(a, off);
+                    b += xor_lane_f64(b, off);
+                    c += xor_lane_f64(c, off);
                 }
                 if (c != 0.0 && fabs(c) > tol * sqrt(a * b)) {
                     const double zeta = (b - a) / (2.0 * c);

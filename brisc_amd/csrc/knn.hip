@@ -197,25 +197,6 @@ __device__ __forceinline__ bool key_lt(float da, uint32_t ia, float db, uint32_t
     return da < db || (da == db && ia < ib);
 }
 
-// The value of lane ^ stride (stride a power of two below 64, known once
-// the loops around a call are unrolled): strides 1, 2, 4 and 8 by DPP moves
-// within a row of 16 lanes (quad permutations, the half-row mirror then a
-// quad reversal, a rotation by 8), 16 by a swizzle (xor mask within 32
-// lanes), 32 by a permute.
-__device__ __forceinline__ int xor_lane(int v, int stride) {
-    switch (stride) {
-        case 1: return __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);   // quad_perm 1,0,3,2
-        case 2: return __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);   // quad_perm 2,3,0,1
-        case 4: {
-            const int t = __builtin_amdgcn_update_dpp(0, v, 0x141, 0xF, 0xF, false);  // row_half_mirror
-            return __builtin_amdgcn_update_dpp(0, t, 0x1B, 0xF, 0xF, false);          // quad_perm 3,2,1,0
-        }
-        case 8: return __builtin_amdgcn_update_dpp(0, v, 0x128, 0xF, 0xF, false);  // row_ror 8
-        case 16: return __builtin_amdgcn_ds_swizzle(v, 0x401F);                  // xor 16
-        default: return __shfl_xor(v, stride);
-    }
-}
-
 // Compare-exchange with lane ^ stride; keep the smaller key when keep_min.
 __device__ __forceinline__ void cmp_xchg(float &d, uint32_t &i, int stride, bool keep_min) {
     const float od = __int_as_float(xor_lane(__float_as_int(d), stride));
