@@ -1370,24 +1370,27 @@ hipError_t launch_gemm_mfma_test(const double *A, int64_t n, int m,
 }
 
 // V <- V Vb[:, :kc], V[:, kc] = remainder, vs = remainder / sd.  Grid:
-// (problem, chunk of 64 genes); thread t of a chunk computes gene t % 64 for
-// columns t / 64, t / 64 + 4, ...; each entry sums over l ascending.
+// (problem, chunk of 64 genes, group of 4 columns); thread t computes gene
+// t % 64 of column 4 * group + t / 64, summing over l ascending.
 __global__ __launch_bounds__(256) void k_restart_v(Tables T) {
-    const int chunks = (T.max_g + 63) / 64;
-    Prob &pr = T.probs[blockIdx.x / chunks];
+    const int chunks = (T.max_g + 63) / 64, cgroups = (T.max_m + 3) / 4;
+    const int per_prob = chunks * cgroups;
+    Prob &pr = T.probs[blockIdx.x / per_prob];
     if (pr.phase != PH_RESTARTING) return;
     const int g = pr.g, m = pr.m, kc = pr.kc, mm = m + (m & 1);
-    const int i = (blockIdx.x % chunks) * 64 + (threadIdx.x & 63);
+    const int chunk = (blockIdx.x % per_prob) % chunks, cgrp = (blockIdx.x % per_prob) / chunks;
+    const int i = chunk * 64 + (threadIdx.x & 63);
     if (i >= g) return;
     const double *V = pr.V[pr.vbuf];
     double *V2 = pr.V[pr.vbuf ^ 1];
     const double *Vb = pr.Jv + (int64_t)mm * mm;
-    for (int c = threadIdx.x >> 6; c < kc; c += 4) {
+    const int c = cgrp * 4 + (threadIdx.x >> 6);
+    if (c < kc) {
         double acc = 0.0;
         for (int l = 0; l < m; l++) acc = fma(V[(int64_t)l * g + i], Vb[l + (int64_t)c * m], acc);
         V2[(int64_t)c * g + i] = acc;
     }
-    if (threadIdx.x < 64) {
+    if (cgrp == 0 && threadIdx.x < 64) {
         V2[(int64_t)kc * g + i] = pr.rem[i];
         pr.vs[i] = (float)(pr.rem[i] / pr.sd[i]);
     }
@@ -1549,7 +1552,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
         k_tall_gemm_mfma<false><<<nblk, kGemmThreads, mfma_lds, s>>>(T);
     else
         k_tall_gemm<false><<<nblk, 256, sizeof(double) * m * kGemmCols, s>>>(T);
-    k_restart_v<<<T.num_probs * (unsigned)ceil_div(g, 64), 256, 0, s>>>(T);
+    k_restart_v<<<T.num_probs * (unsigned)(ceil_div(g, 64) * ceil_div(m, 4)), 256, 0, s>>>(T);
     k_restart_commit<<<T.num_probs, 64, 0, s>>>(T);
     mark(4, 1);
     mark(5, 0);
