@@ -156,7 +156,8 @@ def bench_serve(args):
         # Each request's CSR is generated on the device when it arrives and
         # dropped once it is solved (the whole stream would not fit in HBM).
         return [serve.Request(rid, arrival, None, None, None, w.num_cells,
-                              w.num_genes, k, make=maker(w))
+                              w.num_genes, k, make=maker(w),
+                              make_bytes=synth.generate_device_peak_bytes(w))
                 for rid, (arrival, w, k) in enumerate(stream)]
 
     cells = [w.num_cells for _, w, _ in stream]

@@ -102,6 +102,12 @@ def lib() -> ctypes.CDLL:
         'bpca_context_set_stream': (i32, [p, p]),
         'bpca_workspace_bytes': (u64, [u64, u32, u32, u32]),
         'bpca_context_cancel': (i32, [p]),
+        'bpca_context_reserve': (i32, [p, u64, i32]),
+        'bpca_context_release': (i32, [p, u64]),
+        'bpca_context_budget': (i32, [p, ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_uint64),
+                                      ctypes.POINTER(ctypes.c_uint64)]),
+        'bpca_start_vector': (i32, [u64, u32, p, p]),
         'bpca_context_set_profiling': (i32, [p, i32]),
         'bpca_context_kernel_times': (i32, [p, p, p]),
         'bpca_context_reset_kernel_times': (i32, [p]),

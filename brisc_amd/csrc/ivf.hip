@@ -36,6 +36,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "rng.h"
 
 namespace bpca {
 
@@ -307,32 +308,6 @@ __global__ __launch_bounds__(256) void k_ivf_entries(const uint32_t *__restrict_
         rec[3] = (uint32_t)(mk >> 32);
     }
 }
-
-// The reference's generator (cyutils.pyx:1153-1232): PCG32 (XSH-RR) with
-// Lemire's bounded draw.
-struct Pcg32 {
-    uint64_t state;
-    explicit Pcg32(uint64_t seed) : state(seed + 1442695040888963407ULL) { next(); }
-    uint32_t next() {
-        const uint64_t x = state;
-        state = x * 6364136223846793005ULL + 1442695040888963407ULL;
-        const uint32_t s = (uint32_t)((x ^ (x >> 18)) >> 27);
-        const uint32_t rot = (uint32_t)(x >> 59);
-        return (s >> rot) | (s << ((-rot) & 31u));
-    }
-    uint32_t bounded(uint32_t bound) {
-        uint64_t m = (uint64_t)next() * bound;
-        uint32_t l = (uint32_t)m;
-        if (l < bound) {
-            const uint32_t threshold = (0u - bound) % bound;
-            while (l < threshold) {
-                m = (uint64_t)next() * bound;
-                l = (uint32_t)m;
-            }
-        }
-        return (uint32_t)(m >> 32);
-    }
-};
 
 template <typename T>
 struct Buf {

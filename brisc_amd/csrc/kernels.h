@@ -94,7 +94,7 @@ struct Prob {
     double *U[2];
     double *V[2];
     float *u32;
-    double *P, *red, *sq;
+    double *P, *red, *sq;   // sq: block sums of squares [nb], then block sums [nb]
     float *tpart;
     double *grp;
     double *B, *sd, *rem;
@@ -106,6 +106,7 @@ struct Prob {
     int32_t ubuf, vbuf, converged, nconv;
     int32_t svd_sweeps, pad0;
     double alpha, inv_alpha, rnorm, smax;
+    double umean;         // mean left in U[:, j] by the reorthogonalisation
     int64_t steps;
 };
 
@@ -166,6 +167,8 @@ hipError_t launch_widen_indptr(const int32_t *in, int64_t count, int64_t *out,
                                hipStream_t s);
 hipError_t launch_narrow_indices(const int64_t *in, int64_t nnz, int g,
                                  int32_t *out, int *flag, hipStream_t s);
+// Zero `bytes` bytes at p with a kernel (no memset node, no allocation).
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t s);
 // One-sided Jacobi SVD of a single m x (m+1) matrix (test entry point).
 hipError_t launch_svd_single(const double *B, int m, double *W, double *Jv,
                              double *Ub, double *Sb, int *sweeps,

@@ -511,8 +511,12 @@ __global__ __launch_bounds__(256) void k_reduce_columns(Tables T) {
 // U[:, j] = u - (mean + sum_c U[:, c] red[c+1]) with the correction summed
 // over c ascending; thread t handles the row pairs (r0 + 2t + 512i,
 // r0 + 2t + 512i + 1), i = 0, 1 (16-byte loads and stores when the block is
-// full and n is even), and the block's sum of squares is block_sum<4> of the
-// thread sums (rows in that order).
+// full and n is even), and the block's sum and sum of squares are
+// block_sum<4> of the thread sums (rows in that order): sq[b] and sq[nb + b].
+// The sum lets k_alpha remove the mean that the correction itself leaves
+// (the U columns are zero-mean only to rounding, and X^T u below assumes a
+// zero-mean u; uncorrected, that mean grows by about beta / alpha per step,
+// which exceeds 1 when there are fewer cells than genes).
 __global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
     extern __shared__ double coef[];
     __shared__ double tmp[4];
@@ -528,7 +532,7 @@ __global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
     double *U = pr.U[pr.ubuf];
     double *Uj = U + (int64_t)j * n;
     const float *u32 = pr.u32;
-    double acc = 0.0;
+    double acc = 0.0, sum = 0.0;
     const bool vec = r0 + kRowBlock <= n && (n & 1) == 0;
     static_assert(kRowBlock == 1024, "two 512-row halves per block");
     if (vec) {
@@ -553,6 +557,7 @@ __global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
         acc = fma(x1, x1, acc);
         acc = fma(x2, x2, acc);
         acc = fma(x3, x3, acc);
+        sum = ((x0 + x1) + x2) + x3;
     } else {
 #pragma unroll
         for (int t = 0; t < kRowBlock / 512; t++) {
@@ -565,27 +570,41 @@ __global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
                     const double x = (double)u32[r + e] - d;
                     Uj[r + e] = x;
                     acc = fma(x, x, acc);
+                    sum += x;
                 }
             }
         }
     }
     acc = block_sum<4>(acc, tmp);
-    if (threadIdx.x == 0) pr.sq[bp.y] = acc;
+    sum = block_sum<4>(sum, tmp);
+    if (threadIdx.x == 0) {
+        pr.sq[bp.y] = acc;
+        pr.sq[pr.nb + bp.y] = sum;
+    }
 }
 
-// alpha = max(sqrt(sum of block squares), 1e-8); B[j, j] = alpha.
+// mean = (sum of block sums) / n, removed from U[:, j] when k_spmv_t
+// normalises it; alpha = max(sqrt(sum of squares - n mean^2), 1e-8), the
+// norm of the centred column; B[j, j] = alpha.
 __global__ __launch_bounds__(256) void k_alpha(Tables T) {
     __shared__ double tmp[4];
     Prob &pr = T.probs[blockIdx.x];
     if (pr.phase != PH_LANCZOS) return;
-    double acc = 0.0;
-    for (int64_t b = threadIdx.x; b < pr.nb; b += 256) acc += pr.sq[b];
+    double acc = 0.0, sum = 0.0;
+    for (int64_t b = threadIdx.x; b < pr.nb; b += 256) {
+        acc += pr.sq[b];
+        sum += pr.sq[pr.nb + b];
+    }
     acc = block_sum<4>(acc, tmp);
+    sum = block_sum<4>(sum, tmp);
     if (threadIdx.x == 0) {
-        double alpha = sqrt(acc);
+        const double mean = sum / (double)pr.n;
+        const double ss = acc - sum * mean;
+        double alpha = sqrt(ss > 0.0 ? ss : 0.0);
         if (alpha < 1e-8) alpha = 1e-8;
         pr.alpha = alpha;
         pr.inv_alpha = 1.0 / alpha;
+        pr.umean = mean;
         pr.B[(int64_t)pr.j + (int64_t)pr.j * pr.m] = alpha;
     }
 }
@@ -622,8 +641,8 @@ __device__ __forceinline__ void spmv_t_reduce(const SegGroup<R> &G,
 }  // namespace
 
 // One workgroup per data block; wave w owns a 256-row sub-block and a
-// private LDS accumulator of g floats.  u = U[:, j] / alpha is written back
-// normalised.  Per gene the sub-block sum is the FP32 fma chain over its rows
+// private LDS accumulator of g floats.  u = (U[:, j] - mean) / alpha is
+// written back centred and normalised.  Per gene the sub-block sum is the FP32 fma chain over its rows
 // in order (a row's column indices are unique, so the lanes of one LDS update
 // touch distinct genes); sub-blocks combine as (w0+w1)+(w2+w3).  Loads are
 // the segment groups of k_spmv.
@@ -645,9 +664,9 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T) {
     sub_block(pr.n, bp.y, w, r0, nrows64);
     const int nrows = (int)nrows64;
     double *Uj = pr.U[pr.ubuf] + (int64_t)pr.j * pr.n;
-    const double inv_alpha = pr.inv_alpha;
+    const double inv_alpha = pr.inv_alpha, umean = pr.umean;
     for (int t = lane; t < nrows; t += kWave) {
-        const double x = Uj[r0 + t] * inv_alpha;
+        const double x = (Uj[r0 + t] - umean) * inv_alpha;
         Uj[r0 + t] = x;
         uval[t] = (float)x;
     }
@@ -1638,6 +1657,21 @@ hipError_t launch_narrow_indices(const int64_t *in, int64_t nnz, int g,
     if (nnz <= 0) return hipSuccess;
     const int grid = (int)std::min<int64_t>(ceil_div(nnz, 256), 65536);
     k_narrow<<<grid, 256, 0, s>>>(in, nnz, g, out, flag);
+    return hipGetLastError();
+}
+
+// Byte fill with zeros by a kernel (callers that must not record memset
+// nodes or allocate use it for counters, flags and padding).
+__global__ __launch_bounds__(256) void k_zero(unsigned char *__restrict__ p,
+                                              size_t bytes) {
+    const size_t stride = (size_t)gridDim.x * 256;
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < bytes; i += stride) p[i] = 0;
+}
+
+hipError_t launch_zero(void *p, size_t bytes, hipStream_t s) {
+    if (bytes == 0) return hipSuccess;
+    const unsigned grid = (unsigned)std::min<size_t>((bytes + 255) / 256, 4096);
+    k_zero<<<grid, 256, 0, s>>>((unsigned char *)p, bytes);
     return hipGetLastError();
 }
 

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "kernels.h"
+#include "rng.h"
 
 namespace {
 
@@ -81,7 +82,7 @@ WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz) {
     L.u32 = take(sizeof(float) * (n + bpca::kWave));  // + a sink slot for masked lanes at [n]
     L.P = take(sizeof(double) * (m + 1) * nb);
     L.red = take(sizeof(double) * (m + 1));
-    L.sq = take(sizeof(double) * nb);
+    L.sq = take(sizeof(double) * 2 * nb);
     L.tpart = take(sizeof(float) * nb * 2 * g);
     L.grp = take(sizeof(double) * ng * 2 * g);
     L.B = take(sizeof(double) * m * (m + 1));
@@ -103,37 +104,17 @@ WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz) {
     return L;
 }
 
-// PCG-XSH-RR 64/32 (O'Neill 2014) seeded as brisc's srand does
-// (cyutils.pyx:1153-1158, 1228-1232), with the FP32 Box-Muller draw of
-// cyutils.pyx:1173-1191: the start vector of pca.pyx:801-806, normalised
-// here in FP64.
-struct Pcg32 {
-    uint64_t state;
-    explicit Pcg32(uint64_t seed) : state(seed + 1442695040888963407ULL) { next(); }
-    uint32_t next() {
-        uint64_t old = state;
-        state = old * 6364136223846793005ULL + 1442695040888963407ULL;
-        uint32_t xs = (uint32_t)((old ^ (old >> 18)) >> 27);
-        uint32_t rot = (uint32_t)(old >> 59);
-        return (xs >> rot) | (xs << ((-rot) & 31));
-    }
-    float uniform() { return (float)next() / (float)UINT_MAX; }
-    float normal() {
-        float u1;
-        do { u1 = uniform(); } while (u1 == 0.0f);
-        float u2 = uniform();
-        float r = std::sqrt(-2.0f * std::log(u1));
-        float theta = (float)(2.0 * M_PI * (double)u2);
-        return r * std::cos(theta);
-    }
-};
-
-std::vector<double> start_vector(uint64_t seed, int g) {
-    Pcg32 rng(seed);
+// The start vector of brisc/pca.pyx:801-806: g FP32 normal draws of the
+// reference's generator (rng.h), normalised here in FP64.  `draws` (may be
+// null) receives the raw FP32 draws.
+std::vector<double> start_vector(uint64_t seed, int g, float *draws = nullptr) {
+    bpca::Pcg32 rng(seed);
     std::vector<double> v(g);
     double nrm = 0.0;
     for (int i = 0; i < g; i++) {
-        v[i] = (double)rng.normal();
+        const float x = rng.normal();
+        if (draws) draws[i] = x;
+        v[i] = (double)x;
         nrm += v[i] * v[i];
     }
     nrm = std::sqrt(nrm);
@@ -179,8 +160,11 @@ struct bpca_context {
     hipStream_t user_stream = nullptr;  // caller's stream to order against
     hipEvent_t order_event = nullptr;
     uint64_t budget = 0;                // 0 = unlimited
+    uint64_t reserved = 0;              // bytes charged by bpca_context_reserve
     size_t max_lds = 65536;
-    std::atomic<int> cancel{0};
+    // Incremented by bpca_context_cancel; a call that advances solves
+    // stops when it differs from the value it saw on entry.
+    std::atomic<uint64_t> cancel_gen{0};
     std::recursive_mutex lock;
     // Device memory pool of whole workspaces, reused across problems.
     struct Block { void *p; size_t bytes; };
@@ -223,10 +207,10 @@ int pool_alloc(bpca_context *ctx, size_t bytes, void **out) {
         *out = b.p;
         return BPCA_OK;
     }
-    if (ctx->budget && ctx->live_bytes + bytes > ctx->budget)
+    if (ctx->budget && ctx->live_bytes + ctx->reserved + bytes > ctx->budget)
         return fail(BPCA_ERR_BUDGET, "request does not fit the context's HBM budget now");
     while (ctx->budget && !ctx->free_blocks.empty() &&
-           ctx->live_bytes + ctx->cached_bytes + bytes > ctx->budget) {
+           ctx->live_bytes + ctx->reserved + ctx->cached_bytes + bytes > ctx->budget) {
         auto b = ctx->free_blocks.back();
         ctx->free_blocks.pop_back();
         ctx->cached_bytes -= b.bytes;
@@ -300,6 +284,49 @@ void begin_call(bpca_context *ctx) {
         (void)hipEventRecord(ctx->order_event, ctx->user_stream);
         (void)hipStreamWaitEvent(ctx->stream, ctx->order_event, 0);
     }
+}
+
+// The descriptor of problem `pb` whose workspace (layout L) starts at base.
+bpca::Prob make_prob(const bpca_problem *pb, const WsLayout &L, char *base,
+                     bool widen, int id) {
+    const int64_t n = (int64_t)pb->num_cells;
+    bpca::Prob d{};
+    d.data = (const float *)(base + L.data4);
+    d.idx16 = (const uint16_t *)(base + L.idx16);
+    d.indptr = widen ? (const int64_t *)(base + L.indptr64) : (const int64_t *)pb->indptr;
+    d.PCs = pb->PCs;
+    d.n = n;
+    d.nb = bpca::ceil_div(n, bpca::kRowBlock);
+    d.ngrp = bpca::ceil_div(d.nb, bpca::kGroup);
+    d.g = (int)pb->num_genes;
+    d.k = (int)pb->k;
+    d.m = (int)pb->subspace_size;
+    d.maxit = (int)pb->max_iterations;
+    d.tol = pb->tolerance;
+    d.id = id;
+    d.U[0] = (double *)(base + L.U0);
+    d.U[1] = (double *)(base + L.U1);
+    d.V[0] = (double *)(base + L.V0);
+    d.V[1] = (double *)(base + L.V1);
+    d.u32 = (float *)(base + L.u32);
+    d.P = (double *)(base + L.P);
+    d.red = (double *)(base + L.red);
+    d.sq = (double *)(base + L.sq);
+    d.tpart = (float *)(base + L.tpart);
+    d.grp = (double *)(base + L.grp);
+    d.B = (double *)(base + L.B);
+    d.sd = (double *)(base + L.sd);
+    d.rem = (double *)(base + L.rem);
+    d.vs = (float *)(base + L.vs);
+    d.W = (double *)(base + L.W);
+    d.Jv = (double *)(base + L.Jv);
+    d.Ub = (double *)(base + L.Ub);
+    d.Sb = (double *)(base + L.Sb);
+    d.vw = (double *)(base + L.vw);
+    d.vdot = (double *)(base + L.vdot);
+    d.vsq = (double *)(base + L.vsq);
+    d.phase = bpca::PH_SETUP;
+    return d;
 }
 
 struct Slot {
@@ -598,7 +625,43 @@ int bpca_context_get_occupancy(bpca_context *ctx, int32_t *spmv_wg,
 
 int bpca_context_cancel(bpca_context *ctx) {
     if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
-    ctx->cancel.store(1);
+    ctx->cancel_gen.fetch_add(1);
+    return BPCA_OK;
+}
+
+int bpca_context_reserve(bpca_context *ctx, uint64_t bytes, int force) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    if (!force && ctx->budget && ctx->live_bytes + ctx->reserved + bytes > ctx->budget)
+        return fail(BPCA_ERR_BUDGET, "reservation does not fit the context's HBM budget now");
+    ctx->reserved += bytes;
+    // Cached workspaces give way to what is charged.
+    DeviceGuard dg(ctx->device);
+    while (ctx->budget && !ctx->free_blocks.empty() &&
+           ctx->live_bytes + ctx->reserved + ctx->cached_bytes > ctx->budget) {
+        auto b = ctx->free_blocks.back();
+        ctx->free_blocks.pop_back();
+        ctx->cached_bytes -= b.bytes;
+        HIP_TRY(hipFree(b.p));
+    }
+    return BPCA_OK;
+}
+
+int bpca_context_release(bpca_context *ctx, uint64_t bytes) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    if (bytes > ctx->reserved) return fail(BPCA_ERR_INVALID_ARGUMENT, "release exceeds the reserved bytes");
+    ctx->reserved -= bytes;
+    return BPCA_OK;
+}
+
+int bpca_context_budget(bpca_context *ctx, uint64_t *budget, uint64_t *charged,
+                        uint64_t *cached) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    if (budget) *budget = ctx->budget;
+    if (charged) *charged = ctx->live_bytes + ctx->reserved;
+    if (cached) *cached = ctx->cached_bytes;
     return BPCA_OK;
 }
 
@@ -757,43 +820,8 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
         pool_free(ctx, ws);
         return rc;
     }
+    const bpca::Prob d = make_prob(pb, L, (char *)ws, widen, slot);
     char *base = (char *)ws;
-    bpca::Prob d{};
-    d.data = (const float *)(base + L.data4);
-    d.idx16 = (const uint16_t *)(base + L.idx16);
-    d.indptr = widen ? (const int64_t *)(base + L.indptr64) : (const int64_t *)pb->indptr;
-    d.PCs = pb->PCs;
-    d.n = n;
-    d.nb = bpca::ceil_div(n, bpca::kRowBlock);
-    d.ngrp = bpca::ceil_div(d.nb, bpca::kGroup);
-    d.g = g;
-    d.k = k;
-    d.m = m;
-    d.maxit = (int)pb->max_iterations;
-    d.tol = pb->tolerance;
-    d.id = slot;
-    d.U[0] = (double *)(base + L.U0);
-    d.U[1] = (double *)(base + L.U1);
-    d.V[0] = (double *)(base + L.V0);
-    d.V[1] = (double *)(base + L.V1);
-    d.u32 = (float *)(base + L.u32);
-    d.P = (double *)(base + L.P);
-    d.red = (double *)(base + L.red);
-    d.sq = (double *)(base + L.sq);
-    d.tpart = (float *)(base + L.tpart);
-    d.grp = (double *)(base + L.grp);
-    d.B = (double *)(base + L.B);
-    d.sd = (double *)(base + L.sd);
-    d.rem = (double *)(base + L.rem);
-    d.vs = (float *)(base + L.vs);
-    d.W = (double *)(base + L.W);
-    d.Jv = (double *)(base + L.Jv);
-    d.Ub = (double *)(base + L.Ub);
-    d.Sb = (double *)(base + L.Sb);
-    d.vw = (double *)(base + L.vw);
-    d.vdot = (double *)(base + L.vdot);
-    d.vsq = (double *)(base + L.vsq);
-    d.phase = bpca::PH_SETUP;
     const std::vector<double> v0 = start_vector(pb->seed, g);
     hipError_t e = hipSuccess;
     if (widen) e = bpca::launch_widen_indptr((const int32_t *)pb->indptr, n + 1, (int64_t *)(base + L.indptr64), s);
@@ -815,8 +843,11 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
     return BPCA_OK;
 }
 
-int bpca_engine_step(bpca_engine *E, int32_t max_steps, int32_t *num_done) {
-    if (!E) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL engine");
+namespace {
+
+// bpca_engine_step for a caller that entered at cancel generation gen0.
+int engine_step_impl(bpca_engine *E, int32_t max_steps, int32_t *num_done,
+                     uint64_t gen0) {
     bpca_context *ctx = E->ctx;
     std::lock_guard<std::recursive_mutex> guard(ctx->lock);
     DeviceGuard dg(ctx->device);
@@ -834,8 +865,7 @@ int bpca_engine_step(bpca_engine *E, int32_t max_steps, int32_t *num_done) {
     }
     int remaining = std::max(0, (int)max_steps);
     while (remaining > 0) {
-        if (ctx->cancel.load()) {
-            ctx->cancel.store(0);
+        if (ctx->cancel_gen.load() != gen0) {
             (void)hipStreamSynchronize(s);
             return fail(BPCA_ERR_CANCELLED, "solve cancelled");
         }
@@ -849,6 +879,13 @@ int bpca_engine_step(bpca_engine *E, int32_t max_steps, int32_t *num_done) {
     }
     if (num_done) *num_done = *E->h_done - E->done_seen;
     return BPCA_OK;
+}
+
+}  // namespace
+
+extern "C" int bpca_engine_step(bpca_engine *E, int32_t max_steps, int32_t *num_done) {
+    if (!E) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL engine");
+    return engine_step_impl(E, max_steps, num_done, E->ctx->cancel_gen.load());
 }
 
 int bpca_engine_collect(bpca_engine *E, int32_t max_out, int32_t *slots,
@@ -977,6 +1014,7 @@ int bpca_irlba_batched(bpca_context *ctx, int32_t num_problems,
                        double *singular_values, int32_t sv_stride) {
     if (!ctx || num_problems < 0 || (num_problems > 0 && !problems))
         return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL argument");
+    const uint64_t gen0 = ctx->cancel_gen.load();
     std::lock_guard<std::recursive_mutex> guard(ctx->lock);
     if (singular_values)
         for (int i = 0; i < num_problems; i++)
@@ -996,7 +1034,7 @@ int bpca_irlba_batched(bpca_context *ctx, int32_t num_problems,
     std::vector<double> sv((size_t)num_problems * stride);
     while (collected < num_problems) {
         int32_t nd = 0;
-        BPCA_TRY(bpca_engine_step(E, 1 << 30, &nd));
+        BPCA_TRY(engine_step_impl(E, 1 << 30, &nd, gen0));
         int32_t got = 0;
         BPCA_TRY(bpca_engine_collect(E, num_problems - collected, slots.data() + collected,
                                      st.data() + collected,
@@ -1388,6 +1426,15 @@ int bpca_kmeans(bpca_context *ctx, const float *d_X, int64_t num_cells,
     if (rc == 1) return fail(BPCA_ERR_INVALID_ARGUMENT, msg);
     if (rc == 2) return hip_fail(herr, "run_knn_ivf");
     *iterations_run = I.iterations;
+    return BPCA_OK;
+}
+
+int bpca_start_vector(uint64_t seed, uint32_t num_genes, float *draws,
+                      double *v0) {
+    if (!draws && !v0) return fail(BPCA_ERR_INVALID_ARGUMENT, "draws and v0 are both NULL");
+    if (num_genes < 1) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_genes must be positive");
+    const std::vector<double> v = start_vector(seed, (int)num_genes, draws);
+    if (v0) std::copy(v.begin(), v.end(), v0);
     return BPCA_OK;
 }
 

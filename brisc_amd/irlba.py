@@ -176,8 +176,28 @@ class Context:
                 'workgroups_per_cu': self.occupancy()}
 
     def cancel(self) -> None:
-        """Ask a running solve on this context to stop at its next step."""
+        """Ask the solves running on this context to stop within their next
+        few steps (``KeyboardInterrupt`` in the calls advancing them); a
+        solve started later is not affected."""
         _lib.check(_lib.lib().bpca_context_cancel(self._h))
+
+    def reserve(self, nbytes: int, force: bool = False) -> None:
+        """Charge ``nbytes`` held outside the library (an input CSR) to the
+        HBM budget; ``BudgetError`` if it does not fit now (unless
+        ``force``)."""
+        _lib.check(_lib.lib().bpca_context_reserve(self._h, int(nbytes),
+                                                   int(bool(force))))
+
+    def release(self, nbytes: int) -> None:
+        """Return bytes charged by :meth:`reserve`."""
+        _lib.check(_lib.lib().bpca_context_release(self._h, int(nbytes)))
+
+    def budget(self) -> dict:
+        """{'budget', 'charged', 'cached'} in bytes (budget 0 = none)."""
+        b, c, k = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        _lib.check(_lib.lib().bpca_context_budget(
+            self._h, ctypes.byref(b), ctypes.byref(c), ctypes.byref(k)))
+        return {'budget': b.value, 'charged': c.value, 'cached': k.value}
 
     def close(self) -> None:
         if self._h is not None:
@@ -392,6 +412,18 @@ class Engine:
             self.close()
         except Exception:
             pass
+
+
+def start_vector(seed: int, num_genes: int):
+    """The IRLBA start vector for ``seed``: ``(draws, v0)`` -- the FP32
+    normal draws of ``brisc/pca.pyx:801-806`` and the vector normalised in
+    FP64 (host computation, no device)."""
+    draws = np.empty(num_genes, dtype=np.float32)
+    v0 = np.empty(num_genes, dtype=np.float64)
+    _lib.check(_lib.lib().bpca_start_vector(int(seed) & 0xFFFFFFFFFFFFFFFF,
+                                            int(num_genes), draws.ctypes.data,
+                                            v0.ctypes.data))
+    return draws, v0
 
 
 def tall_gemm(A, C, context: Context | None = None):

@@ -3,9 +3,12 @@
 Requests arrive over time (a Poisson stream in the benchmark); each carries a
 device-resident CSR and its ``num_PCs``.  The loop
 
-* admits every arrived request whose workspace fits the context's HBM budget
-  (``BudgetError`` from the engine means: wait until a running request
-  finishes and frees memory);
+* admits arrived requests in arrival order while they fit the context's HBM
+  budget.  A request whose matrix the loop produces (``make``) is charged to
+  the budget before it is materialised (its peak bytes during generation,
+  then its resident CSR), and its solver workspace is charged at admission,
+  so the budget bounds the device memory of every matrix the loop holds
+  (``BudgetError`` means: wait until a running request finishes);
 * advances all admitted requests together -- one Lanczos step per request
   per replay of the engine's macro-step graph -- so requests that arrive
   while others run join the same launches;
@@ -47,10 +50,18 @@ class Request:
     # the request arrives; the loop drops the request's matrix once it is
     # solved, so a long stream never holds every matrix at once.
     make: object = None
+    # Device bytes ``make`` needs at its peak (an estimate is fine; the loop
+    # charges the resident CSR's exact size once it exists).
+    make_bytes: int = 0
+    charged: int = 0          # bytes the loop has charged to the budget
 
     def materialise(self):
         if self.data is None and self.make is not None:
             self.data, self.indices, self.indptr = self.make()
+
+    def resident_bytes(self) -> int:
+        return sum(int(t.numel()) * t.element_size()
+                   for t in (self.data, self.indices, self.indptr))
 
     def release(self):
         if self.make is not None:
@@ -91,6 +102,7 @@ def serve(requests, context: Context | None = None, step_quantum: int = 16,
     ``(results, makespan_seconds)``.
     """
     eng = Engine(context)
+    ctx = eng.context
     queue = deque(sorted(requests, key=lambda r: r.arrival))
     total = len(queue)
     waiting = deque()
@@ -101,20 +113,24 @@ def serve(requests, context: Context | None = None, step_quantum: int = 16,
         while queue or waiting or eng.active:
             now = clock() - t0
             while queue and queue[0].arrival <= now:
-                r = queue.popleft()
-                r.materialise()
-                waiting.append(r)
+                waiting.append(queue.popleft())
             while waiting and eng.active < max_active:
                 r = waiting[0]
-                try:
-                    eng.add(r.data, r.indices, r.indptr, r.num_cells,
-                            r.num_genes, r.k, subspace_size=r.subspace_size,
-                            tolerance=r.tolerance,
-                            max_iterations=r.max_iterations, seed=r.seed,
-                            validate=False, tag=r)
-                except _lib.BudgetError:
-                    if not eng.active:
-                        raise
+                if r.data is None and r.make is not None:
+                    # Charge the matrix before it exists; only the head of
+                    # the queue is ever materialised ahead of admission.
+                    try:
+                        ctx.reserve(r.make_bytes)
+                    except _lib.BudgetError:
+                        if not eng.active:
+                            raise
+                        break
+                    r.charged = r.make_bytes
+                    r.materialise()
+                    ctx.release(r.charged)
+                    r.charged = r.resident_bytes()
+                    ctx.reserve(r.charged, force=True)
+                if not _admit(eng, r):
                     break
                 admitted_at[r.rid] = clock() - t0
                 waiting.popleft()
@@ -127,6 +143,9 @@ def serve(requests, context: Context | None = None, step_quantum: int = 16,
                                           admitted_at[tag.rid], t, PCs, sv,
                                           st))
                     tag.release()
+                    if tag.charged:
+                        ctx.release(tag.charged)
+                        tag.charged = 0
                 if done and progress:
                     progress(len(results), total)
             elif queue:
@@ -136,7 +155,36 @@ def serve(requests, context: Context | None = None, step_quantum: int = 16,
         makespan = clock() - t0
     finally:
         eng.close()
+        for r in list(waiting) + list(queue):
+            if r.charged:
+                ctx.release(r.charged)
+                r.charged = 0
     return results, makespan
+
+
+def _admit(eng, r) -> bool:
+    """Add ``r`` to the engine; False when its workspace must wait for a
+    running request to finish.  Device memory that torch keeps cached after
+    finished requests is returned once before an allocation is given up."""
+    for attempt in range(2):
+        try:
+            eng.add(r.data, r.indices, r.indptr, r.num_cells, r.num_genes,
+                    r.k, subspace_size=r.subspace_size,
+                    tolerance=r.tolerance, max_iterations=r.max_iterations,
+                    seed=r.seed, validate=False, tag=r)
+            return True
+        except _lib.BudgetError:
+            if not eng.active:
+                raise
+            return False
+        except MemoryError:
+            if attempt:
+                if not eng.active:
+                    raise
+                return False
+            import torch
+            torch.cuda.empty_cache()
+    return False
 
 
 def serve_sequential(requests, context: Context | None = None,

@@ -182,6 +182,15 @@ def generate_device(w: Workload, device: int = 0, context=None,
     return data, indices, indptr
 
 
+def generate_device_peak_bytes(w: Workload, slack: float = 1.1) -> int:
+    """Device bytes :func:`generate_device` holds at its peak for ``w``
+    (row counts and offsets, then indices, counts and values of the expected
+    number of nonzeros, with ``slack`` for the draw's spread)."""
+    n = w.num_cells
+    nnz = n * w.num_genes * w.expected_density * slack
+    return int(12 * nnz + 16 * (n + 1) + 8 * w.type_gene_mean.size)
+
+
 def batch_sizes(num_problems: int = 256, low: int = 500, high: int = 200_000,
                 seed: int = 256) -> np.ndarray:
     """Cells per problem, log-uniform on [low, high] (subclustering batch)."""

@@ -58,10 +58,24 @@ int bpca_context_destroy(bpca_context *ctx);
 /* Queue all work of this context on `hip_stream` (a hipStream_t; NULL = the
  * context's own stream). */
 int bpca_context_set_stream(bpca_context *ctx, void *hip_stream);
-/* Request that the solve running on this context stop; it returns
- * BPCA_ERR_CANCELLED at its next Lanczos step (the role of the reference's
- * PyErr_CheckSignals per step, pca.pyx:1173-1174). */
+/* Request that the solves running on this context stop: every call that is
+ * advancing solves when the request arrives (bpca_irlba_f32, _device,
+ * _batched, bpca_engine_step) returns BPCA_ERR_CANCELLED within its next
+ * few Lanczos steps (the role of the reference's PyErr_CheckSignals per
+ * step, pca.pyx:1173-1174).  A call that starts later is not affected. */
 int bpca_context_cancel(bpca_context *ctx);
+/* Charge `bytes` of device memory held outside the library (for example a
+ * request's input CSR waiting to be admitted) to the context's HBM budget,
+ * so that admission counts it: BPCA_ERR_BUDGET if it does not fit beside
+ * what is charged now (force = 1 charges it regardless).  No counterpart
+ * in the reference (a serving-loop knob). */
+int bpca_context_reserve(bpca_context *ctx, uint64_t bytes, int force);
+/* Return bytes charged by bpca_context_reserve. */
+int bpca_context_release(bpca_context *ctx, uint64_t bytes);
+/* Budget (0 = none), bytes charged now (workspaces in use plus reserved
+ * bytes) and workspace bytes cached for reuse. */
+int bpca_context_budget(bpca_context *ctx, uint64_t *budget,
+                        uint64_t *charged, uint64_t *cached);
 
 /* Kernel timing with HIP events recorded on the context's stream around each
  * class of launches.  Classes: */
@@ -347,6 +361,14 @@ int bpca_kmeans(bpca_context *ctx, const float *d_X, int64_t num_cells,
                 int64_t chunk_size_kmeans, uint64_t seed, uint32_t *d_labels,
                 float *d_centroids, uint32_t *d_counts,
                 int32_t *iterations_run);
+
+/* The IRLBA start vector for `seed`: the num_genes FP32 normal draws that
+ * brisc/pca.pyx:801-806 puts in V[:, 0] (srand and random_normal of
+ * brisc/cyutils.pyx:1153-1191, 1228-1232: PCG-XSH-RR, FP32 Box-Muller) go to
+ * `draws` and the same vector normalised in FP64 (the reference normalises in
+ * FP32) to `v0`; either may be NULL.  Host only, no device. */
+int bpca_start_vector(uint64_t seed, uint32_t num_genes, float *draws,
+                      double *v0);
 
 /* The cells kmeans() takes as initial centroids with random initialisation
  * (kmeans_random_init, brisc/kmeans.pyx:13-39): num_clusters distinct cells

@@ -102,8 +102,10 @@ def log1ppf(counts: np.ndarray, indptr: np.ndarray) -> np.ndarray:
     n = len(indptr) - 1
     out = np.empty(len(counts), dtype=np.float32)
     row_sums = np.empty(n, dtype=np.uint64)
-    lib().normalize_log1ppf(_ptr(np.ascontiguousarray(counts, np.int32)),
-                            _ptr(np.ascontiguousarray(indptr, np.int64)), n,
+    # Converted copies are bound to names so they outlive the C call.
+    counts = np.ascontiguousarray(counts, np.int32)
+    indptr = np.ascontiguousarray(indptr, np.int64)
+    lib().normalize_log1ppf(_ptr(counts), _ptr(indptr), n,
                             _ptr(out), _ptr(row_sums))
     return out
 
@@ -174,9 +176,11 @@ def gram(data, indices, indptr, num_genes):
     n = len(indptr) - 1
     G = np.empty((num_genes, num_genes), dtype=np.float64)
     cs = np.empty(num_genes, dtype=np.float64)
-    rc = lib().gram_f64(_ptr(np.ascontiguousarray(data, np.float32)),
-                        _ptr(np.ascontiguousarray(indices, np.int32)),
-                        _ptr(np.ascontiguousarray(indptr, np.int64)), n,
+    # Converted copies are bound to names so they outlive the C call.
+    data = np.ascontiguousarray(data, np.float32)
+    indices = np.ascontiguousarray(indices, np.int32)
+    indptr = np.ascontiguousarray(indptr, np.int64)
+    rc = lib().gram_f64(_ptr(data), _ptr(indices), _ptr(indptr), n,
                         num_genes, _ptr(G), _ptr(cs))
     if rc != 0:
         raise MemoryError('gram_f64 allocation failed')

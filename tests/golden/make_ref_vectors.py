@@ -8,6 +8,7 @@ Run in the development container only (it reads the reference sources):
     python tests/golden/make_ref_vectors.py --knn-large (adds the 1M-cell neighbour fixture)
     python tests/golden/make_ref_vectors.py --pipeline1m (adds the 1M-cell pipeline fixture)
     python tests/golden/make_ref_vectors.py --knn-approx (adds few-clusters-searched fixtures)
+    python tests/golden/make_ref_vectors.py --rng       (adds the start-vector draws)
 
 It cythonizes ``brisc/cyutils.pyx``, ``brisc/pca.pyx``,
 ``brisc/normalize.pyx``, ``brisc/kmeans.pyx`` and ``brisc/knn.pyx`` from the
@@ -434,6 +435,51 @@ def pipeline1m(pca_mod, norm_mod):
     print(json.dumps(meta['pipeline_cells1m'], indent=1))
 
 
+# A few lines of this script's own Cython that call the reference's
+# srand / random_normal (declared in brisc/cyutils.pxd, compiled into the
+# reference's cyutils module above) to record the IRLBA start-vector draws.
+RNG_SHIM = """
+# distutils: language = c++
+from brisc.cyutils cimport srand, random_normal
+import numpy as np
+
+def normal_draws(unsigned long long seed, unsigned count):
+    cdef unsigned long long state = srand(seed)
+    out = np.empty(count, np.float32)
+    cdef float[::1] o = out
+    cdef unsigned i
+    for i in range(count):
+        o[i] = random_normal(&state)
+    return out
+"""
+
+RNG_SEEDS = [(0, 2000), (1, 300), (42, 300), (2 ** 63 + 12345, 300),
+             (2 ** 64 - 1, 300), (1_000_003, 20_000)]
+
+
+def rng_fixture(reference: str):
+    """The reference's start-vector draws (pca.pyx:801-806: srand(seed),
+    then random_normal per gene) for a few seeds, as float32 arrays."""
+    suffix = sysconfig.get_config_var('EXT_SUFFIX')
+    src = os.path.join(BUILD, 'rngshim.pyx')
+    cpp = os.path.join(BUILD, 'rngshim.cpp')
+    so = os.path.join(BUILD, f'rngshim{suffix}')
+    with open(src, 'w') as f:
+        f.write(RNG_SHIM)
+    subprocess.run([sys.executable, '-m', 'cython', '-3', '--cplus',
+                    '-I', reference, src, '-o', cpp], check=True)
+    subprocess.run(['g++', '-Ofast', '-std=c++17', '-fPIC', '-shared',
+                    '-march=native', f"-I{sysconfig.get_paths()['include']}",
+                    f'-I{np.get_include()}', cpp, '-o', so], check=True)
+    sys.path.insert(0, BUILD)
+    shim = importlib.import_module('rngshim')
+    out = {}
+    for seed, count in RNG_SEEDS:
+        out[f'seed_{seed}'] = shim.normal_draws(seed, count)
+    np.savez_compressed(os.path.join(HERE, 'ref_start_vector.npz'), **out)
+    print({k: v[:3] for k, v in out.items()})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--reference', default='/root/reference')
@@ -447,9 +493,13 @@ def main():
                     help='only add the 1M-cell normalize -> pca -> neighbors fixture')
     ap.add_argument('--knn-approx', action='store_true',
                     help='only add the few-clusters-searched neighbour fixtures')
+    ap.add_argument('--rng', action='store_true',
+                    help='only add the start-vector draws (ref_start_vector.npz)')
     args = ap.parse_args()
     build_reference(args.reference)
     pca_mod, norm_mod = import_reference()
+    if args.rng:
+        return rng_fixture(args.reference)
     if args.cells1m:
         return cells1m(pca_mod, norm_mod)
     if args.knn:

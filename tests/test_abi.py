@@ -88,3 +88,26 @@ def test_single_hip_runtime_in_process():
     maps = open('/proc/self/maps').read()
     runtimes = set(re.findall(r'\S*libamdhip64\S*', maps))
     assert len(runtimes) == 1, runtimes
+
+
+def test_start_vector_matches_reference_draws():
+    """The IRLBA start vector (brisc/pca.pyx:801-806) for several seeds
+    against the reference's own srand + random_normal draws
+    (tests/golden/ref_start_vector.npz, written by make_ref_vectors.py
+    --rng): the FP32 draws bit for bit, and the FP64 normalisation."""
+    from brisc_amd.irlba import start_vector
+    ref = np.load(os.path.join(ROOT, 'tests', 'golden', 'ref_start_vector.npz'))
+    assert len(ref.files) >= 5
+    for name in ref.files:
+        seed = int(name.split('_', 1)[1])
+        want = ref[name]
+        draws, v0 = start_vector(seed, len(want))
+        assert np.array_equal(draws.view(np.uint32), want.view(np.uint32)), name
+        x = want.astype(np.float64)
+        assert np.allclose(v0, x / np.sqrt(np.sum(x * x)), rtol=0, atol=1e-15)
+        # the draws are standard normal (mean, variance) and a planted
+        # change of the seed gives a different stream
+        if len(want) >= 2000:
+            assert abs(want.mean()) < 0.1 and abs(want.var() - 1) < 0.1
+    d1, _ = start_vector(1, 300)
+    assert not np.array_equal(d1, ref['seed_0'][:300])

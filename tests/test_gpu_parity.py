@@ -1,12 +1,19 @@
 """Parity of the HIP path with the reference's outputs and the FP64 oracle.
 
-Tolerances (written here, applied throughout):
+Tolerances (written here, applied throughout; each test prints what it
+achieved, and the bounds sit a few times above it):
   * singular values (column norms of the PCs): relative error < 1e-4 against
-    the reference's own irlba output and against the FP64 oracle;
+    the reference's own FP32 irlba output (whose own error reaches 1.7e-5 on
+    the tiny inputs) and < 1e-6 against the FP64 oracle (achieved: at most
+    2.1e-7);
   * individual PCs whose singular value is separated by > 1% from its
-    neighbours: relative error < 1e-2 up to sign;
+    neighbours: relative error up to sign < 1e-4 against the FP64 oracle's
+    PCs (achieved: at most 2e-5), and < 5e-3 against the reference's FP32
+    PCs (whose own error on the tiny inputs reaches 2.3e-3);
   * every PC inside a near-degenerate cluster (window 1e-3): projection
-    deficit onto the oracle's cluster < 1e-3;
+    deficit onto the oracle's cluster < 1e-8 (achieved: at most 2e-10);
+  * largest principal angle between the 50-PC subspace and the oracle's:
+    < 1e-3 rad at 1M cells, < 1e-4 rad at 10M cells;
   * integer work (generator counts, CSR structure): bit-exact;
   * run-to-run: bit-identical.
 """
@@ -20,9 +27,11 @@ import helpers
 
 pytestmark = pytest.mark.gpu
 
-SV_RTOL = 1e-4
-COL_RTOL = 1e-2
-CLUSTER_TOL = 1e-3
+SV_RTOL = 1e-4        # singular values vs the reference's FP32 output
+ORACLE_SV_RTOL = 1e-6  # singular values vs the FP64 oracle
+COL_RTOL = 1e-4       # separated PCs vs the FP64 oracle
+REF_COL_RTOL = 5e-3   # separated PCs vs the reference's FP32 PCs
+CLUSTER_TOL = 1e-8
 TINY = ['tiny_a', 'tiny_b', 'tiny_c', 'tiny_d']
 
 
@@ -56,15 +65,28 @@ def run_host(data, indices, indptr, n, g, k, m=100, width=4, **kw):
 
 def check_against(PCs, ref_sv, ref_pcs=None, oracle_out=None):
     assert np.isfinite(PCs).all()
-    assert helpers.relative_sv_error(PCs, ref_sv).max() < SV_RTOL
+    err = helpers.relative_sv_error(PCs, ref_sv).max()
+    print(f'  sv rel err vs reference {err:.2e}', flush=True)
+    assert err < SV_RTOL
     if ref_pcs is not None:
         ws = helpers.well_separated(ref_sv, 1e-2)
         if len(ws):
-            assert helpers.column_errors_up_to_sign(PCs[:, ws], ref_pcs[:, ws]).max() < COL_RTOL
+            col = helpers.column_errors_up_to_sign(PCs[:, ws], ref_pcs[:, ws]).max()
+            print(f'  separated PCs ({len(ws)}) vs reference {col:.2e}', flush=True)
+            assert col < REF_COL_RTOL
     if oracle_out is not None:
         sv = oracle_out['singular_values']
-        assert helpers.relative_sv_error(PCs, sv).max() < SV_RTOL
-        assert helpers.cluster_projection_deficit(PCs, oracle_out['PCs'], sv, 1e-3).max() < CLUSTER_TOL
+        err = helpers.relative_sv_error(PCs, sv).max()
+        deficit = helpers.cluster_projection_deficit(PCs, oracle_out['PCs'], sv, 1e-3)
+        ws = helpers.well_separated(sv, 1e-2)
+        col = helpers.column_errors_up_to_sign(PCs[:, ws], oracle_out['PCs'][:, ws]).max() \
+            if len(ws) else 0.0
+        print(f'  sv rel err vs FP64 oracle {err:.2e}, separated PCs ({len(ws)}) vs '
+              f'oracle {col:.2e}, cluster deficit '
+              f'{deficit.max() if len(deficit) else 0.0:.2e}', flush=True)
+        assert err < ORACLE_SV_RTOL
+        assert col < COL_RTOL
+        assert deficit.max() < CLUSTER_TOL
 
 
 @pytest.mark.parametrize('name', TINY)
@@ -96,7 +118,9 @@ def test_cells50k_matches_reference_and_oracle(golden, cells50k):
     check_against(PCs, ref_sv)
     rows = large['rows']
     ws = helpers.well_separated(ref_sv, 1e-2)
-    assert helpers.column_errors_up_to_sign(PCs[rows][:, ws], large['pcs_rows'][:, ws]).max() < COL_RTOL
+    col = helpers.column_errors_up_to_sign(PCs[rows][:, ws], large['pcs_rows'][:, ws]).max()
+    print(f'  sampled rows of separated PCs vs reference {col:.2e}', flush=True)
+    assert col < REF_COL_RTOL
     out = oracle.fp64_pca(data, indices, indptr, w.num_cells, w.num_genes, k)
     check_against(PCs, out['singular_values'], oracle_out=out)
 
@@ -228,6 +252,16 @@ def test_single_cell_qc_backfill_matches_subset(golden):
     # joint PCA over two datasets splits the rows back per dataset
     a, b = sc.pca(sub, num_PCs=5, subspace_size=20, QC_column=[qc, None])
     assert a.obsm['pca'].shape == (2500, 5) and b.obsm['pca'].shape == (int(qc.sum()), 5)
+    assert np.isnan(a.obsm['pca'][~qc]).all()
+    # pca()'s values against the FP64 oracle on the same rows and genes:
+    # the QC x HVG subset alone, and the two datasets stacked
+    Xs = sp.csr_array(X[np.flatnonzero(qc)][:, np.flatnonzero(hvg)])
+    Xs.sort_indices()
+    for P_, Y, k in [(P[qc], Xs, 12),
+                     (np.vstack([a.obsm['pca'][qc], b.obsm['pca']]), sp.vstack([Xs, Xs]).tocsr(), 5)]:
+        Y.sort_indices()
+        out = oracle.fp64_pca(Y.data, Y.indices, Y.indptr.astype(np.int64), Y.shape[0], Y.shape[1], k)
+        check_against(P_, out['singular_values'], oracle_out=out)
 
 
 def _golden_1m():
@@ -272,9 +306,9 @@ def test_one_million_cells_against_reference_and_fp64_oracle():
     print(f'1M: sv rel err vs reference max {err_ref.max():.2e}, vs FP64 oracle '
           f'max {err_orc.max():.2e}; separated PC rows vs reference max {row_err.max():.2e} '
           f'({len(ws)} PCs); largest principal angle vs oracle {ang.max():.2e} rad', flush=True)
-    assert err_ref.max() < SV_RTOL and err_orc.max() < SV_RTOL
-    assert len(ws) == 0 or row_err.max() < COL_RTOL
-    assert ang.max() < 1e-2
+    assert err_ref.max() < SV_RTOL and err_orc.max() < ORACLE_SV_RTOL
+    assert len(ws) == 0 or row_err.max() < 1e-3
+    assert ang.max() < 1e-3
 
 
 @pytest.mark.slow
@@ -298,8 +332,8 @@ def test_ten_million_cells_against_fp64_oracle():
     gap = out['singular_values'][-1]
     print(f'10M: sv rel err vs FP64 oracle max {err.max():.2e} (median {np.median(err):.2e}); '
           f'principal angles vs oracle: largest {ang.max():.2e} rad, median {np.median(ang):.2e}', flush=True)
-    assert err.max() < SV_RTOL
-    assert ang.max() < 1e-2 and gap > 0
+    assert err.max() < 1e-6
+    assert ang.max() < 1e-4 and gap > 0
 
 
 def test_device_small_svd_matches_numpy():
@@ -357,7 +391,7 @@ def test_batched_is_bitwise_equal_to_sequential():
         assert np.array_equal(Pb.cpu().numpy().view(np.uint32), Ps.cpu().numpy().view(np.uint32))
         assert np.array_equal(svb, svs)
         out = oracle.fp64_pca(*csr, w.num_cells, w.num_genes, 15, want_pcs=False)
-        assert np.max(np.abs(svb - out['singular_values']) / out['singular_values']) < SV_RTOL
+        assert np.max(np.abs(svb - out['singular_values']) / out['singular_values']) < ORACLE_SV_RTOL
 
 
 def test_engine_admits_while_running_and_matches_solo():
@@ -403,7 +437,8 @@ def test_request_loop_matches_one_at_a_time():
     def requests():
         return [serve.Request(rid, a, None, None, None, w.num_cells, w.num_genes, k,
                               subspace_size=30,
-                              make=(lambda w=w: synth.generate_device(w)))
+                              make=(lambda w=w: synth.generate_device(w)),
+                              make_bytes=synth.generate_device_peak_bytes(w))
                 for rid, (a, w, k) in enumerate(stream)]
 
     reqs = requests()
@@ -422,6 +457,14 @@ def test_request_loop_matches_one_at_a_time():
         assert np.array_equal(r.PCs.cpu().numpy().view(np.uint32),
                               o.PCs.cpu().numpy().view(np.uint32))
         assert np.array_equal(r.singular_values, o.singular_values)
+    # and each served result against the FP64 oracle on the request's matrix
+    for r in res_b:
+        _, w, k = stream[r.rid]
+        d, i, p = synth.generate_device(w)
+        out = oracle.fp64_pca(d.cpu().numpy(), i.cpu().numpy(), p.cpu().numpy(),
+                              w.num_cells, w.num_genes, k, want_pcs=False)
+        err = np.max(np.abs(r.singular_values - out['singular_values']) / out['singular_values'])
+        assert err < ORACLE_SV_RTOL, (r.rid, err)
     summ = serve.summarize(res_b, span_b)
     assert summ['requests'] == len(stream) and summ['converged'] == len(stream)
     assert summ['p50_ms'] <= summ['p99_ms']
