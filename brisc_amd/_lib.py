@@ -117,6 +117,9 @@ def lib() -> ctypes.CDLL:
         'bpca_irlba_device': (i32, [p, p, p, p, i32, u64, u32, u32, u32, f32,
                                     u32, u64, i32, p, p,
                                     ctypes.POINTER(SolveStats)]),
+        'bpca_irlba_device_noalloc': (i32, [p, p, p, p, i32, u64, u32, u32,
+                                            u32, f32, u32, u64, p, p,
+                                            ctypes.POINTER(SolveStats)]),
         'bpca_synth_row_nnz': (i32, [p, u64, u64, u64, u32, p, p, p, p]),
         'bpca_synth_fill': (i32, [p, u64, u64, u64, u32, p, p, p, p, p, p]),
         'bpca_normalize_log1ppf': (i32, [p, p, p, u64, p]),

@@ -148,6 +148,45 @@ struct DeviceGuard {
     }
 };
 
+// Scratch of the allocation-free solve (bpca_irlba_device_noalloc): a
+// file-scope device array, one copy per device, held until the process
+// ends.  Host code never uses the array's name as an address: the device
+// address of each device's copy is looked up once, when the first context on
+// that device is created, and kept in g_scratch_ptr.
+constexpr size_t kScratchBytes = (size_t)256 << 20;
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_lock;
+void *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_use[kMaxDevices];  // one allocation-free solve per device at a time
+
+}  // namespace
+
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+
+// Scratch bytes of an allocation-free solve: descriptor, block and group
+// lists, counters, workspace.
+uint64_t noalloc_bytes(int64_t n, int g, int m, int64_t nnz, bool widen) {
+    const int64_t nb = bpca::ceil_div(n, bpca::kRowBlock), ng = bpca::ceil_div(nb, bpca::kGroup);
+    return align_up(sizeof(bpca::Prob)) + align_up(sizeof(int2) * nb) + align_up(sizeof(int2) * ng) +
+           kAlign + ws_layout(n, g, m, widen, nnz).total;
+}
+
+// With `device` current: record the device address of its scratch copy.
+int scratch_init(int device) {
+    if (device < 0 || device >= kMaxDevices)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "device index above the library's limit of 64");
+    std::lock_guard<std::mutex> guard(g_scratch_lock);
+    if (g_scratch_ptr[device]) return BPCA_OK;
+    void *p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    if (!p) return fail(BPCA_ERR_DEVICE, "hipGetSymbolAddress(scratch) returned NULL");
+    g_scratch_ptr[device] = p;
+    return BPCA_OK;
+}
+
 }  // namespace
 
 // ------------------------------------------------------------ context
@@ -554,6 +593,16 @@ int bpca_context_create(int device, uint64_t hbm_budget_bytes,
         ctx->lds_per_cu = lds_cu;
     const char *env = std::getenv("BPCA_NO_GRAPH");
     ctx->use_graph = !(env && env[0] == '1');
+    const int rc = scratch_init(device);
+    if (rc != BPCA_OK) {
+        (void)hipEventDestroy(ctx->order_event);
+        (void)hipEventDestroy(ctx->fork_event);
+        (void)hipEventDestroy(ctx->join_event);
+        (void)hipStreamDestroy(ctx->side);
+        (void)hipStreamDestroy(ctx->stream);
+        delete ctx;
+        return rc;
+    }
     *out = ctx;
     return BPCA_OK;
 }
@@ -1074,6 +1123,154 @@ int bpca_irlba_device(bpca_context *ctx, const float *d_data,
     pb.seed = seed;
     pb.PCs = d_PCs;
     return bpca_irlba_batched(ctx, 1, &pb, stats, singular_values, (int32_t)k);
+}
+
+int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
+                              const int32_t *d_indices, const void *d_indptr,
+                              int indptr_width, uint64_t num_cells,
+                              uint32_t num_genes, uint32_t k,
+                              uint32_t subspace_size, float tolerance,
+                              uint32_t max_iterations, uint64_t seed,
+                              float *d_PCs, double *singular_values,
+                              bpca_solve_stats *stats) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!d_data || !d_indices || !d_indptr || !d_PCs)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL device pointer");
+    if (indptr_width != 4 && indptr_width != 8)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "indptr_width must be 4 or 8");
+    const int64_t n = (int64_t)num_cells;
+    const int g = (int)num_genes, m = (int)subspace_size;
+    BPCA_TRY(check_shape(n, g, k, m, tolerance, max_iterations));
+    if (bpca::lds_bytes_needed(g, m) > ctx->max_lds)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "num_genes / subspace_size need more LDS per workgroup than the device has");
+    const uint64_t gen0 = ctx->cancel_gen.load();
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    std::lock_guard<std::mutex> use(g_scratch_use[ctx->device]);
+    DeviceGuard dg(ctx->device);
+    char *const base = (char *)g_scratch_ptr[ctx->device];
+    if (!base) return fail(BPCA_ERR_DEVICE, "scratch address of this device was not recorded");
+    hipStream_t s = ctx->stream;
+    begin_call(ctx);
+    int64_t nnz = 0;
+    if (indptr_width == 8) {
+        HIP_TRY(hipMemcpyAsync(&nnz, (const int64_t *)d_indptr + n, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    } else {
+        int32_t nnz32 = 0;
+        HIP_TRY(hipMemcpyAsync(&nnz32, (const int32_t *)d_indptr + n, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        nnz = nnz32;
+    }
+    if (nnz < 0) return fail(BPCA_ERR_NONCANONICAL, "indptr[num_cells] is negative");
+    const uint64_t need = noalloc_bytes(n, g, m, nnz, indptr_width == 4);
+    if (need > kScratchBytes)
+        return fail(BPCA_ERR_INVALID_ARGUMENT,
+                    "the allocation-free solve needs " + std::to_string(need) +
+                        " bytes of device scratch for this problem; the library holds " +
+                        std::to_string(kScratchBytes) + " per device");
+    // Scratch layout: problem descriptor, data-block and group lists, two
+    // counters (done, flag), then the workspace.
+    const int64_t nb = bpca::ceil_div(n, bpca::kRowBlock), ngrp = bpca::ceil_div(nb, bpca::kGroup);
+    size_t off = 0;
+    bpca::Prob *d_prob = (bpca::Prob *)(base + off);
+    off += align_up(sizeof(bpca::Prob));
+    int2 *d_blocks = (int2 *)(base + off);
+    off += align_up(sizeof(int2) * nb);
+    int2 *d_groups = (int2 *)(base + off);
+    off += align_up(sizeof(int2) * ngrp);
+    int *d_done = (int *)(base + off);
+    int *d_flag = d_done + 1;
+    off += kAlign;
+    char *ws = base + off;
+    const bool widen = indptr_width == 4;
+    const WsLayout L = ws_layout(n, g, m, widen, nnz);
+    // The scratch keeps the last call's contents: clear the counters and the
+    // quad padding past nnz of both packed copies.
+    HIP_TRY(bpca::launch_zero(d_done, kAlign, s));
+    HIP_TRY(bpca::launch_zero(ws + L.idx16 + sizeof(uint16_t) * nnz, sizeof(uint16_t) * (L.nnz4 - nnz), s));
+    HIP_TRY(bpca::launch_zero(ws + L.data4 + sizeof(float) * nnz, sizeof(float) * (L.nnz4 - nnz), s));
+    HIP_TRY(bpca::launch_validate_pack(d_data, d_indices, d_indptr, indptr_width, n, nnz, g,
+                                       (float *)(ws + L.data4), (uint16_t *)(ws + L.idx16), d_flag, s));
+    int h_flag = 0;
+    HIP_TRY(hipMemcpyAsync(&h_flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (h_flag)
+        return fail(BPCA_ERR_NONCANONICAL,
+                    "CSR is not canonical: indptr must start at 0 and be non-decreasing, and each row's column indices strictly increasing and below num_genes");
+    bpca_problem pb{};
+    pb.data = d_data;
+    pb.indices = d_indices;
+    pb.indptr = d_indptr;
+    pb.indptr_width = indptr_width;
+    pb.num_cells = num_cells;
+    pb.num_genes = num_genes;
+    pb.k = k;
+    pb.subspace_size = subspace_size;
+    pb.max_iterations = max_iterations;
+    pb.tolerance = tolerance;
+    pb.seed = seed;
+    pb.PCs = d_PCs;
+    const bpca::Prob d = make_prob(&pb, L, ws, widen, 0);
+    const std::vector<double> v0 = start_vector(seed, g);
+    std::vector<int2> blocks((size_t)nb), groups((size_t)ngrp);
+    for (int64_t b = 0; b < nb; b++) blocks[b] = make_int2(0, (int)b);
+    for (int64_t G = 0; G < ngrp; G++) groups[G] = make_int2(0, (int)G);
+    if (widen) HIP_TRY(bpca::launch_widen_indptr((const int32_t *)d_indptr, n + 1, (int64_t *)(ws + L.indptr64), s));
+    HIP_TRY(hipMemcpyAsync(d.V[0], v0.data(), sizeof(double) * g, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_prob, &d, sizeof(d), hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_blocks, blocks.data(), sizeof(int2) * nb, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(d_groups, groups.data(), sizeof(int2) * ngrp, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    bpca::Tables T{};
+    T.probs = d_prob;
+    T.num_probs = 1;
+    T.blocks = d_blocks;
+    T.num_blocks = nb;
+    T.groups = d_groups;
+    T.num_groups = ngrp;
+    T.max_g = std::max(g, 2);
+    T.max_m = std::max(m, 1);
+    T.max_lds = (int64_t)ctx->max_lds;
+    T.spmv_group = ctx->spmv_group;
+    T.spmvt_group = ctx->spmvt_group;
+    T.spmv_wg = ctx->spmv_wg;
+    T.spmvt_wg = ctx->spmvt_wg;
+    T.lds_per_cu = ctx->lds_per_cu;
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(bpca::launch_setup(T, s));
+    // Every sweep ends within m + 1 macro steps (its SVD, restart or final
+    // product run in the step that ends it), so this bound is never reached
+    // by a healthy solve.
+    const int64_t max_macro = ((int64_t)max_iterations + 1) * (m + 2) + 16;
+    int64_t launched = 0;
+    int h_done = 0;
+    while (!h_done) {
+        if (ctx->cancel_gen.load() != gen0) {
+            (void)hipStreamSynchronize(s);
+            return fail(BPCA_ERR_CANCELLED, "solve cancelled");
+        }
+        if (launched >= max_macro) return fail(BPCA_ERR_NUMERICAL, "solve did not finish within its step bound");
+        for (int i = 0; i < 8; i++) HIP_TRY(bpca::launch_macro_step(T, d_done, s, nullptr));
+        launched += 8;
+        HIP_TRY(hipMemcpyAsync(&h_done, d_done, sizeof(int), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    bpca::Prob fin{};
+    HIP_TRY(hipMemcpyAsync(&fin, d_prob, sizeof(fin), hipMemcpyDeviceToHost, s));
+    if (singular_values)
+        HIP_TRY(hipMemcpyAsync(singular_values, d.Sb, sizeof(double) * k, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    if (stats) {
+        stats->converged = fin.converged;
+        stats->restarts = fin.iter + 1;
+        stats->lanczos_steps = fin.steps;
+        stats->matvecs = 2 * fin.steps;
+        stats->seconds = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count();
+        stats->svd_sweeps = fin.svd_sweeps;
+        stats->slot = 0;
+        stats->smax = fin.smax;
+    }
+    return BPCA_OK;
 }
 
 }  // extern "C"

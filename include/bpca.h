@@ -173,6 +173,29 @@ int bpca_irlba_device(bpca_context *ctx, const float *d_data,
                       float *d_PCs, double *singular_values,
                       bpca_solve_stats *stats);
 
+/* Allocation-free variant of bpca_irlba_device.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * Same arguments and results as bpca_irlba_device (validate is always on,
+ * PCs bit-identical to bpca_irlba_device's for the same inputs); the solver
+ * state lives in a library-held device array of 256 MiB per device instead
+ * of allocated workspace, so it calls no hipMalloc / hipFree and no
+ * allocating library routine.  A problem whose descriptor, work lists and
+ * workspace (two FP64 num_cells x subspace_size Lanczos bases, a packed copy
+ * of the CSR and small vectors) exceed 256 MiB is refused with
+ * BPCA_ERR_INVALID_ARGUMENT (about 100k cells at subspace 100, 2000 genes
+ * and 7% nonzeros).  All calls on one device share that array, which keeps
+ * the last call's contents; calls are serialised per device inside the
+ * process, so concurrent calls wait for each other instead of racing. */
+int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
+                              const int32_t *d_indices, const void *d_indptr,
+                              int indptr_width, uint64_t num_cells,
+                              uint32_t num_genes, uint32_t k,
+                              uint32_t subspace_size, float tolerance,
+                              uint32_t max_iterations, uint64_t seed,
+                              float *d_PCs, double *singular_values,
+                              bpca_solve_stats *stats);
+
 /* ------------------------------------------------- batched / engine */
 
 /* One IRLBA problem on device-resident CSR (the arguments of
