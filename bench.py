@@ -16,10 +16,12 @@ solves/s for both.
 
 The JSON line carries the solve time, the roofline of the dominant kernel
 (algorithmic bytes per launch / its average launch time, from HIP events
-recorded on the kernels' own stream during the timed steps) and a CPU
-baseline: the C++/OpenMP restatement of the reference's parallel IRLBA run
-on the host cores on a bounded row sample of the same matrix, scaled to the
-full matrix by nonzero count.
+recorded on the kernels' own stream during the timed steps; beside it the
+measured HBM bytes per launch over the same time) and a CPU baseline: the
+C++/OpenMP restatement of the reference's parallel IRLBA run on the host
+cores for two iterations on a row sample of the same matrix, its time per
+Lanczos step scaled to the full matrix by nonzeros and by the full solve's
+step count.
 """
 import argparse
 import json
@@ -52,7 +54,26 @@ def load_traffic(workload: str, kernel: str):
     return None if entry is None else entry.get('bytes_per_launch')
 
 
-def cpu_baseline(data, indices, indptr, n_full, nnz_full, g, k, rows):
+def port_calibration():
+    """Port / reference time ratio measured on the development host (both on
+    the same 50k-cell input and thread count; make_ref_vectors.py
+    --calibrate), or None."""
+    path = os.path.join(ROOT, 'tests', 'golden', 'ref_vectors.json')
+    if not os.path.exists(path):
+        return None
+    with open(path) as f:
+        return json.load(f).get('port_calibration')
+
+
+def cpu_baseline(data, indices, indptr, nnz_full, g, k, rows, steps_full):
+    """The oracle's C++/OpenMP restatement of the reference's parallel IRLBA
+    (oracle.irlba_port) on the host cores: two IRLBA iterations on the first
+    ``rows`` rows of the same matrix, timed without the CSC copy that the
+    reference receives ready-made from pca(); seconds per Lanczos step (each
+    step is one X v, one X^T u and the reorthogonalisations, plus its share
+    of the restart) scaled to the full matrix by nonzeros and multiplied by
+    the Lanczos steps of the full solve (``steps_full``, counted by the
+    device solve of the same algorithm)."""
     import oracle
     rows = min(rows, len(indptr) - 1)
     e = int(indptr[rows])
@@ -62,14 +83,27 @@ def cpu_baseline(data, indices, indptr, n_full, nnz_full, g, k, rows):
     threads = int(os.environ.get('OMP_NUM_THREADS', os.cpu_count() or 1))
     threads = max(1, min(threads, os.cpu_count() or 1))
     t0 = time.perf_counter()
-    _, conv, st = oracle.irlba_port(d, ix, ip, rows, g, k, num_threads=threads)
-    dt = time.perf_counter() - t0
+    _, conv, st = oracle.irlba_port(d, ix, ip, rows, g, k, max_iterations=2,
+                                    num_threads=threads)
+    dt = time.perf_counter() - t0 - st['csc_seconds']
+    steps = max(1, st['matvecs'] // 2)
     scale = nnz_full / max(e, 1)
-    return {'value': round(dt * scale, 3), 'unit': 's (scaled to the full matrix by nnz)',
-            'cores': threads, 'kind': 'port',
-            'sample': f'first {rows} rows ({e} nnz) of the same matrix, full '
-                      f'solve k={k}: {dt:.2f} s, converged={conv}, '
-                      f'restarts={st["restarts"]}, x{scale:.1f} by nnz'}
+    value = dt / steps * scale * steps_full
+    calib = port_calibration()
+    out = {'value': round(value, 3),
+           'unit': 's (seconds per Lanczos step on a row sample, scaled by '
+                   'nonzeros, times the full solve\'s Lanczos steps)',
+           'cores': threads, 'kind': 'port',
+           'sample': f'first {rows} rows ({e} nnz, {100 * e / nnz_full:.0f}% of the '
+                     f'nonzeros), 2 IRLBA iterations k={k}: {steps} Lanczos steps '
+                     f'in {dt:.2f} s (+{st["csc_seconds"]:.2f} s CSC copy, not '
+                     f'counted); x{scale:.2f} by nnz, x{steps_full} steps'}
+    if calib:
+        out['port_over_reference'] = calib['port_over_reference']
+        out['calibration'] = (f'port / reference = {calib["port_over_reference"]} on '
+                              f'{calib["workload"]} with {calib["threads"]} threads '
+                              f'(development host, {calib["host_cpus"]} CPUs)')
+    return out
 
 
 def log(msg):
@@ -140,30 +174,38 @@ def bench_batched(args):
 
 
 def bench_serve(args):
+    """The serving stream: continuous batching against the one-at-a-time GPU
+    path on the same Poisson stream, and again on a prefix of the stream at
+    an arrival rate the one-at-a-time path can sustain."""
     import hashlib
 
     import torch
     from brisc_amd import Context, serve, synth
 
-    ctx = Context(0, hbm_budget_bytes=int(args.hbm_budget_gb * (1 << 30)))
+    free = torch.cuda.mem_get_info()[0]
+    nominal = int(args.hbm_budget_gb * 1e9)
+    reserve = 4 << 30   # torch's cached blocks and the engine's small tables
+    budget = min(nominal, free - reserve)
+    ctx = Context(0, hbm_budget_bytes=budget)
     t0 = time.perf_counter()
     stream = synth.request_stream(args.requests, args.rate, high=args.max_cells)
 
     def maker(w):
         return lambda: synth.generate_device(w, context=ctx)
 
-    def requests():
+    def requests(items, rate_scale=1.0):
         # Each request's CSR is generated on the device when it arrives and
         # dropped once it is solved (the whole stream would not fit in HBM).
-        return [serve.Request(rid, arrival, None, None, None, w.num_cells,
-                              w.num_genes, k, make=maker(w),
+        return [serve.Request(rid, arrival * rate_scale, None, None, None,
+                              w.num_cells, w.num_genes, k, make=maker(w),
                               make_bytes=synth.generate_device_peak_bytes(w))
-                for rid, (arrival, w, k) in enumerate(stream)]
+                for rid, (arrival, w, k) in enumerate(items)]
 
     cells = [w.num_cells for _, w, _ in stream]
     log(f'# {len(stream)} requests, cells {sum(cells)} (max {max(cells)}), '
-        f'rate {args.rate}/s, parameters drawn in {time.perf_counter() - t0:.1f} s')
-    serve.serve(requests()[:2], context=ctx)                 # warm-up
+        f'rate {args.rate}/s, budget {budget / 1e9:.1f} GB, parameters drawn in '
+        f'{time.perf_counter() - t0:.1f} s')
+    serve.serve(requests(stream[:2]), context=ctx)                 # warm-up
 
     def progress(tag):
         t_start = time.perf_counter()
@@ -180,31 +222,53 @@ def bench_serve(args):
             r.PCs = None
         return out
 
-    res_b, span_b = serve.serve(requests(), context=ctx,
-                                progress=progress('continuous batching'))
-    sb = serve.summarize(res_b, span_b)
-    log(f'# continuous batching: {sb}')
-    dig_b = digests(res_b)
-    torch.cuda.empty_cache()
-    res_s, span_s = serve.serve_sequential(requests(), context=ctx,
-                                           progress=progress('one at a time'))
-    ss = serve.summarize(res_s, span_s)
-    log(f'# one at a time: {ss}')
-    dig_s = digests(res_s)
-    same = sum(int(dig_b[rid] == dig_s.get(rid)) for rid in dig_b)
+    def both(items, rate_scale, tag):
+        res_b, span_b = serve.serve(requests(items, rate_scale), context=ctx,
+                                    progress=progress(f'{tag} continuous batching'))
+        sb = serve.summarize(res_b, span_b)
+        log(f'# {tag} continuous batching: {sb}')
+        nonconv = [dict(rid=r.rid, num_cells=items[r.rid][1].num_cells,
+                        k=items[r.rid][2], restarts=r.stats['restarts'])
+                   for r in res_b if not r.stats['converged']]
+        dig_b = digests(res_b)
+        torch.cuda.empty_cache()
+        if args.no_sequential:
+            return sb, None, None, nonconv
+        res_s, span_s = serve.serve_sequential(requests(items, rate_scale), context=ctx,
+                                               progress=progress(f'{tag} one at a time'))
+        ss = serve.summarize(res_s, span_s)
+        log(f'# {tag} one at a time: {ss}')
+        dig_s = digests(res_s)
+        torch.cuda.empty_cache()
+        same = sum(int(dig_b[rid] == dig_s.get(rid)) for rid in dig_b)
+        return sb, ss, f'{same}/{len(dig_b)} (SHA-256 of each PCs array)', nonconv
+
+    sb, ss, same, nonconv = both(stream, 1.0, 'full stream')
     line = {
         'metric': 'serving PCA requests: p50/p99 ms and solves/s',
         'value': sb['solves_per_s'], 'unit': 'solves/s', 'n_gpus': 1,
-        'steps': 1, 'warmup': 1, 'ms_per_step': round(span_b * 1e3, 2),
+        'steps': 1, 'warmup': 1, 'ms_per_step': sb['makespan_s'] * 1e3,
         'higher_is_better': True, 'scaling': 'weak', 'vs_baseline': None,
         'dtype': 'f32 spmv / f64 lanczos', 'data': 'synthetic',
         'config': {'workload': 'request_stream', 'requests': len(stream),
                    'rate_per_s': args.rate, 'cells_low': 1000,
                    'cells_high': args.max_cells, 'k_range': [20, 50],
-                   'hbm_budget_gb': args.hbm_budget_gb},
+                   'hbm_budget_gb_nominal': args.hbm_budget_gb,
+                   'hbm_budget_bytes': budget,
+                   'budget_note': f'nominal budget, less what is not free on the device and a '
+                                  f'{reserve >> 30} GiB reserve'},
         'continuous_batching': sb, 'one_at_a_time': ss,
-        'bitwise_equal_to_one_at_a_time': f'{same}/{len(dig_b)} (SHA-256 of each PCs array)',
+        'bitwise_equal_to_one_at_a_time': same,
+        'not_converged': nonconv,
     }
+    if args.compare_requests > 0 and not args.no_sequential:
+        items = stream[:args.compare_requests]
+        scale = args.rate / args.compare_rate
+        cb, cs, csame, _ = both(items, scale, f'{args.compare_rate}/s prefix')
+        line['sustainable_rate_comparison'] = {
+            'requests': len(items), 'rate_per_s': args.compare_rate,
+            'continuous_batching': cb, 'one_at_a_time': cs,
+            'bitwise_equal_to_one_at_a_time': csame}
     print(json.dumps(line), flush=True)
 
 
@@ -456,16 +520,19 @@ def main():
                     default='single')
     ap.add_argument('--workload', default='cells10m')
     ap.add_argument('--k', type=int, default=50)
-    ap.add_argument('--cpu-sample-rows', type=int, default=1_000_000)
+    ap.add_argument('--cpu-sample-rows', type=int, default=2_500_000)
     ap.add_argument('--no-cpu-baseline', action='store_true')
     ap.add_argument('--problems', type=int, default=256)
     ap.add_argument('--batch-k', type=int, default=30)
     ap.add_argument('--no-sequential', action='store_true',
-                    help='batched mode: skip the one-at-a-time comparison')
+                    help='batched / serve modes: skip the one-at-a-time comparison')
     ap.add_argument('--requests', type=int, default=1000)
     ap.add_argument('--rate', type=float, default=8.0)
     ap.add_argument('--max-cells', type=int, default=2_000_000)
-    ap.add_argument('--hbm-budget-gb', type=float, default=150.0)
+    ap.add_argument('--hbm-budget-gb', type=float, default=288.0)
+    ap.add_argument('--compare-requests', type=int, default=200,
+                    help='serve mode: prefix of the stream run again at --compare-rate')
+    ap.add_argument('--compare-rate', type=float, default=4.0)
     ap.add_argument('--copy-gib', type=float, default=4.0)
     ap.add_argument('--knn-cells', type=int, default=1_000_000)
     ap.add_argument('--knn-k', type=int, default=20)
@@ -543,10 +610,16 @@ def main():
                       'gbs': round(bytes_per / (avg * 1e-3) / 1e9, 1)}
     dominant = max(('spmv', 'spmv_t'), key=lambda k_: ktimes[k_][0])
     achieved = kern[dominant]['gbs']
+    traffic = load_traffic(args.workload, dominant)
     roofline = {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
-                'traffic': load_traffic(args.workload, dominant),
+                'traffic': traffic,
                 'kernel': dominant, 'bytes_per_launch': bytes_per}
+    if traffic:
+        # Measured HBM bytes per launch over the same average launch time.
+        phys = traffic / (kern[dominant]['avg_ms'] * 1e-3) / 1e9
+        roofline['achieved_physical'] = round(phys, 1)
+        roofline['frac_physical'] = round(phys / HBM_PEAK_GBS, 4)
     total_ms = sum(v[0] for v in ktimes.values())
     breakdown = {k_: {'ms_per_step': round(v[0] / args.steps, 2),
                       'share': round(v[0] / total_ms, 4) if total_ms else 0}
@@ -559,8 +632,8 @@ def main():
         rows = min(args.cpu_sample_rows, n)
         e = int(indptr[rows].item())
         baseline = cpu_baseline(data[:e].cpu().numpy(), indices[:e].cpu().numpy(),
-                                indptr[:rows + 1].cpu().numpy(), n, nnz, g,
-                                args.k, rows)
+                                indptr[:rows + 1].cpu().numpy(), nnz, g,
+                                args.k, rows, int(stats[-1]['lanczos_steps']))
 
     line = {
         'metric': 'sec to 50 PCs @10M cells (HBM GB/s % peak); batched PCA solves/s; p50/p99 ms',

@@ -139,7 +139,9 @@ def irlba_port(data, indices, indptr, num_cells, num_genes, k,
     """Run the FP32 restatement of the reference parallel IRLBA.
 
     Returns ``(PCs, converged, stats)`` with ``stats`` = dict(restarts,
-    matvecs, singular_values).
+    matvecs, singular_values, csc_seconds): the last is the time spent
+    building the CSC copy that the reference receives ready-made from
+    ``pca()``.
     """
     if num_threads is None:
         num_threads = os.cpu_count() or 1
@@ -148,7 +150,7 @@ def irlba_port(data, indices, indptr, num_cells, num_genes, k,
     indices = np.ascontiguousarray(indices, np.int32)
     indptr = np.ascontiguousarray(indptr, np.int64)
     PCs = np.empty((num_cells, k), dtype=np.float32)
-    stats = np.zeros(3, dtype=np.int64)
+    stats = np.zeros(4, dtype=np.int64)
     sv = np.zeros(k, dtype=np.float32)
     sgesdd, sgemm = scipy_blas_lapack()
     conv = lib().irlba_port(_ptr(data), _ptr(indices), _ptr(indptr),
@@ -157,7 +159,8 @@ def irlba_port(data, indices, indptr, num_cells, num_genes, k,
                             chunk_size, _ptr(PCs), sgesdd, sgemm, _ptr(stats),
                             _ptr(sv))
     return PCs, bool(conv), dict(restarts=int(stats[0]),
-                                 matvecs=int(stats[1]), singular_values=sv)
+                                 matvecs=int(stats[1]), singular_values=sv,
+                                 csc_seconds=stats[3] * 1e-9)
 
 
 def clipped_stddev(data, indices, indptr, num_cells, num_genes,

@@ -89,7 +89,10 @@ extern "C" int irlba_port(const float *data, const int32_t *indices,
     omp_set_num_threads((int)std::max<uint32_t>(1, num_threads));
     const uint64_t nnz = (uint64_t)indptr[n];
 
-    // Row-sorted CSC copy (column entries in ascending cell order).
+    // Row-sorted CSC copy (column entries in ascending cell order).  The
+    // reference receives it from pca() (single_cell.py:12050); its build
+    // time is reported apart (stats[3], nanoseconds).
+    const double t_csc0 = omp_get_wtime();
     std::vector<int64_t> cptr(g + 1, 0);
     std::vector<int32_t> crow(nnz);
     std::vector<float> cval(nnz);
@@ -104,6 +107,8 @@ extern "C" int irlba_port(const float *data, const int32_t *indices,
                 cval[pos] = data[e];
             }
     }
+
+    const double t_csc = omp_get_wtime() - t_csc0;
 
     // Per-gene clipped sample standard deviation, FP32.
     std::vector<float> sd(g);
@@ -329,6 +334,7 @@ extern "C" int irlba_port(const float *data, const int32_t *indices,
         stats[0] = (int64_t)std::min(iteration + 1, max_iterations);
         stats[1] = matvecs;
         stats[2] = converged;
+        stats[3] = (int64_t)(t_csc * 1e9);
     }
     return converged ? 1 : 0;
 }

@@ -9,6 +9,7 @@ Run in the development container only (it reads the reference sources):
     python tests/golden/make_ref_vectors.py --pipeline1m (adds the 1M-cell pipeline fixture)
     python tests/golden/make_ref_vectors.py --knn-approx (adds few-clusters-searched fixtures)
     python tests/golden/make_ref_vectors.py --rng       (adds the start-vector draws)
+    python tests/golden/make_ref_vectors.py --calibrate (times the reference against the port)
 
 It cythonizes ``brisc/cyutils.pyx``, ``brisc/pca.pyx``,
 ``brisc/normalize.pyx``, ``brisc/kmeans.pyx`` and ``brisc/knn.pyx`` from the
@@ -480,6 +481,37 @@ def rng_fixture(reference: str):
     print({k: v[:3] for k, v in out.items()})
 
 
+def calibrate_port(pca_mod, norm_mod, repeats=3, threads=8):
+    """Time the reference's irlba and the oracle's C++ restatement of it
+    (oracle.irlba_port) on the same 50k-cell input with the same thread
+    count, best of `repeats` each; the CSC copy is built outside the timed
+    call for the reference (as pca() does) and subtracted for the port.  The
+    ratio port / reference calibrates the CPU baseline that bench.py times
+    on the GPU host, where the reference does not run."""
+    w = synth.make_workload(**synth.WORKLOADS['cells50k'])
+    indptr, indices, counts = oracle.generate_counts(w)
+    data = ref_normalize(norm_mod, counts, indices, indptr)
+    ref_s, port_s = [], []
+    for _ in range(repeats):
+        _, _, t = ref_irlba(pca_mod, data, indices, indptr, w.num_cells,
+                            w.num_genes, 50, 100, 1024, num_threads=threads)
+        ref_s.append(t)
+        t0 = time.perf_counter()
+        _, _, st = oracle.irlba_port(data, indices, indptr, w.num_cells,
+                                     w.num_genes, 50, num_threads=threads)
+        port_s.append(time.perf_counter() - t0 - st['csc_seconds'])
+    path = os.path.join(HERE, 'ref_vectors.json')
+    with open(path) as f:
+        meta = json.load(f)
+    meta['port_calibration'] = dict(
+        workload='cells50k', k=50, threads=threads, host_cpus=os.cpu_count(),
+        reference_seconds=round(min(ref_s), 3), port_seconds=round(min(port_s), 3),
+        port_over_reference=round(min(port_s) / min(ref_s), 3))
+    with open(path, 'w') as f:
+        json.dump(meta, f, indent=1)
+    print(json.dumps(meta['port_calibration'], indent=1))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--reference', default='/root/reference')
@@ -495,11 +527,15 @@ def main():
                     help='only add the few-clusters-searched neighbour fixtures')
     ap.add_argument('--rng', action='store_true',
                     help='only add the start-vector draws (ref_start_vector.npz)')
+    ap.add_argument('--calibrate', action='store_true',
+                    help='only time the reference against the oracle port (50k cells)')
     args = ap.parse_args()
     build_reference(args.reference)
     pca_mod, norm_mod = import_reference()
     if args.rng:
         return rng_fixture(args.reference)
+    if args.calibrate:
+        return calibrate_port(pca_mod, norm_mod)
     if args.cells1m:
         return cells1m(pca_mod, norm_mod)
     if args.knn:
