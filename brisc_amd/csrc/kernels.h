@@ -5,7 +5,7 @@
 // a batch of uneven problems and a serving mix all run the same arithmetic.
 //
 // Data layout in HBM for one problem (n cells, g genes, m subspace, k PCs):
-//   X        CSR: data float32[nnz4], idx16 uint16[nnz4] -- copies made at
+//   X        CSR: data float32[nnz4], idx uint16[nnz4] (uint32 when g > 65535) -- copies made at
 //            admission, 16-byte aligned and zero-padded to nnz4 = nnz rounded
 //            up to a multiple of 4, so every aligned quad of 4 elements can be
 //            loaded whole -- and indptr int64[n+1]
@@ -63,7 +63,16 @@ __device__ __forceinline__ double xor_lane_f64(double v, int stride) {
 constexpr int kRowBlock = 1024;   // rows per data block
 constexpr int kSubRows = 256;     // rows per wave sub-block of X^T u
 constexpr int kGroup = 64;        // data blocks per first-level group
-constexpr int kMaxGenes = 8192;   // LDS bound of the X^T u kernel
+constexpr int kGeneTile = 8192;   // genes per X^T u / column-statistics launch (LDS accumulators)
+constexpr int kMaxLdsV = 32768;   // X v stages v in LDS up to this many genes
+constexpr int kMaxPacked16 = 65535;  // column indices packed to 16 bits up to this g
+
+// Index modes of a problem (Prob::mode): how its sparse products load.
+enum IndexMode : int32_t {
+    kModeLds = 0,      // 16-bit column indices, v staged in LDS (g <= kMaxLdsV)
+    kModeGlobalV = 1,  // 16-bit column indices, v read from global memory
+    kModeWide = 2      // 32-bit column indices (g > kMaxPacked16), v from global
+};
 constexpr int kGemmCols = 32;     // output columns per restart-GEMM pass
 constexpr int kMaxSubspace = 1024;  // largest subspace_size (m) accepted
 
@@ -83,7 +92,7 @@ enum Phase : int32_t {
 struct Prob {
     // Input CSR and output.
     const float *data;       // packed copy of the values (see above)
-    const uint16_t *idx16;   // column indices packed to 16 bits (g <= 65535)
+    const void *idx;         // packed column indices: uint16 (mode < kModeWide) or uint32
     const int64_t *indptr;
     float *PCs;  // n x k float32 row-major
     int64_t n, nb, ngrp;
@@ -102,6 +111,7 @@ struct Prob {
     double *W, *Jv, *Ub, *Sb;
     double *vw, *vdot, *vsq;  // V-side work: w (g), dots (m), chunk sums
     // State, advanced by the kernels.
+    int32_t mode;            // IndexMode
     int32_t phase, j, kc, iter;
     int32_t ubuf, vbuf, converged, nconv;
     int32_t svd_sweeps, pad0;
@@ -118,6 +128,7 @@ struct Tables {
     const int2 *groups;   // (problem, group) pairs
     int64_t num_groups;
     int max_g, max_m;
+    int modes;            // bit set of the IndexModes of the problems
     int64_t max_lds;      // dynamic LDS bytes a workgroup may use
     int spmv_group;       // segments per load group in X v   (2, 4 or 8)
     int spmvt_group;      // segments per load group in X^T u (2, 4 or 8)
@@ -154,14 +165,14 @@ hipError_t launch_spmv_variant(const Tables &T, int which, int group, int wg,
 // Largest dynamic LDS request of any kernel for g genes and subspace m.
 size_t lds_bytes_needed(int g, int m);
 // Canonical-CSR check of one problem's operand fused with packing it: column
-// indices to 16 bits (idx16) and the values into the aligned copy (data4);
+// indices to 16 bits (32 bits when g > kMaxPacked16; idx) and the values into the aligned copy (data4);
 // both copies must hold nnz rounded up to 4 elements, the tail zeroed by the
 // caller.  Sets *flag when indptr does not start at 0, decreases or exceeds
 // nnz, or a row's indices are not strictly increasing in [0, g).
 hipError_t launch_validate_pack(const float *data, const int32_t *indices,
                                 const void *indptr, int indptr_width, int64_t n,
-                                int64_t nnz, int g, float *data4,
-                                uint16_t *idx16, int *flag, hipStream_t s);
+                                int64_t nnz, int g, float *data4, void *idx,
+                                int *flag, hipStream_t s);
 // indptr (int32) -> int64 and int64 indices -> int32 (flagging range errors).
 hipError_t launch_widen_indptr(const int32_t *in, int64_t count, int64_t *out,
                                hipStream_t s);
@@ -294,6 +305,17 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
                 hipError_t *herr, hipStream_t s);
 std::vector<uint32_t> kmeans_random_init_cells(int64_t n, int num_clusters,
                                                uint64_t seed);
+
+// Transpose of a sparse matrix stored by rows (transpose.hip): input
+// num_rows x num_cols with int32 or int64 offsets / indices (widths 4 or 8),
+// output num_cols x num_rows with int64 offsets (num_cols + 1) and int32
+// indices, each output row in ascending input-row order.  0: done; 1: an
+// offset or index out of range; 2: HIP error (*herr).  Synchronous.
+int run_csr_transpose(const float *data, const void *indices, int index_width,
+                      const void *indptr, int indptr_width, int64_t num_rows,
+                      int64_t num_cols, int64_t nnz, float *out_data,
+                      int32_t *out_indices, int64_t *out_indptr,
+                      hipError_t *herr, hipStream_t s);
 
 // Synthetic workload generator and log1pPF (synth.hip).
 hipError_t launch_synth_row_nnz(uint64_t seed, uint64_t row0, int64_t rows,

@@ -21,6 +21,8 @@
 // results are therefore bit-identical run to run, alone or in a batch.
 #include "kernels.h"
 
+#include <type_traits>
+
 namespace bpca {
 
 namespace {
@@ -73,39 +75,46 @@ __device__ inline void sub_block(int64_t n, int64_t lb, int w, int64_t &first,
 
 // ================================================================ set-up
 
-// Column sums of x (kSquare = false) or x^2 (true) for one data block:
+// Column sums of x (kSquare = false) or x^2 (true) for one data block and
+// the gene tile [c0, c0 + tw) (tw <= kGeneTile; one tile when g <= kGeneTile):
 // wave w accumulates its sub-block in a private LDS vector (each row's
 // column indices are unique, so the 64 lanes touch distinct entries); the
 // per-gene sum is the FP32 chain over the sub-block's rows in order, and
-// the four sub-blocks combine as (w0 + w1) + (w2 + w3).
-template <bool kSquare>
-__global__ __launch_bounds__(256) void k_colstats(Tables T) {
+// the four sub-blocks combine as (w0 + w1) + (w2 + w3), whatever the tile.
+// IX is the width of the packed column indices (problems of the other
+// width return at once).
+template <bool kSquare, typename IX, bool kTiled>
+__global__ __launch_bounds__(256) void k_colstats(Tables T, int c0) {
     extern __shared__ float lds[];
     const int2 bp = T.blocks[blockIdx.x];
     Prob &pr = T.probs[bp.x];
-    if (pr.phase != PH_SETUP) return;
+    if (pr.phase != PH_SETUP || (pr.mode == kModeWide) != (sizeof(IX) == 4)) return;
     const int g = pr.g;
+    if (c0 >= g) return;
+    const int tw = g - c0 < kGeneTile ? g - c0 : kGeneTile;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = wave_id();
-    float *acc = lds + (size_t)w * g;
-    for (int i = lane; i < g; i += kWave) acc[i] = 0.0f;
+    float *acc = lds + (size_t)w * tw;
+    for (int i = lane; i < tw; i += kWave) acc[i] = 0.0f;
     int64_t r0, nrows;
     sub_block(pr.n, bp.y, w, r0, nrows);
     const int64_t *indptr = pr.indptr;
-    const uint16_t *indices = pr.idx16;
+    const IX *indices = (const IX *)pr.idx;
     const float *data = pr.data;
     for (int64_t rr = 0; rr < nrows; rr++) {
         const int64_t b = indptr[r0 + rr], e = indptr[r0 + rr + 1];
         for (int64_t p = b + lane; p < e; p += kWave) {
-            const int c = indices[p];
-            const float x = data[p];
-            acc[c] = kSquare ? fmaf(x, x, acc[c]) : acc[c] + x;
+            const int c = (int)indices[p] - c0;
+            if (!kTiled || (c >= 0 && c < tw)) {
+                const float x = data[p];
+                acc[c] = kSquare ? fmaf(x, x, acc[c]) : acc[c] + x;
+            }
         }
     }
     __syncthreads();
-    float *out = pr.tpart + (int64_t)bp.y * 2 * g + (kSquare ? g : 0);
-    for (int i = threadIdx.x; i < g; i += 256)
-        out[i] = (lds[i] + lds[g + i]) + (lds[2 * g + i] + lds[3 * g + i]);
+    float *out = pr.tpart + (int64_t)bp.y * 2 * g + (kSquare ? g : 0) + c0;
+    for (int i = threadIdx.x; i < tw; i += 256)
+        out[i] = (lds[i] + lds[tw + i]) + (lds[2 * tw + i] + lds[3 * tw + i]);
 }
 
 // grp[G*2g + i] = FP64 sum over the data blocks of group G, in block order.
@@ -178,21 +187,43 @@ namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 typedef unsigned u32x2 __attribute__((ext_vector_type(2)));
+typedef unsigned u32x4 __attribute__((ext_vector_type(4)));
 typedef const f32x4 __attribute__((address_space(1))) gquad;    // 4 values
-typedef const u32x2 __attribute__((address_space(1))) gquad16;  // 4 indices
 typedef float __attribute__((address_space(1))) gfloat_w;
+typedef const float __attribute__((address_space(1))) gfloat_r;
 
-struct Quad {
-    f32x4 d;
-    u32x2 ix;  // four 16-bit column indices, element 0 in the low half of x
+// The four column indices of an aligned quad: packed to 16 bits (g <= 65535;
+// element 0 in the low half of x) or 32 bits.
+template <typename IX> struct IxQuad;
+template <> struct IxQuad<uint16_t> {
+    typedef u32x2 vec;
+    typedef const u32x2 __attribute__((address_space(1))) gvec;
+    __device__ static __forceinline__ int get(const vec &w, int k) {
+        const unsigned h = k < 2 ? w.x : w.y;
+        return (int)((k & 1) ? (h >> 16) : (h & 0xffffu));
+    }
+};
+template <> struct IxQuad<uint32_t> {
+    typedef u32x4 vec;
+    typedef const u32x4 __attribute__((address_space(1))) gvec;
+    __device__ static __forceinline__ int get(const vec &w, int k) {
+        return (int)(k == 0 ? w.x : k == 1 ? w.y : k == 2 ? w.z : w.w);
+    }
 };
 
-__device__ __forceinline__ int quad_index(const Quad &c, int k) {
-    const unsigned w = k < 2 ? c.ix.x : c.ix.y;
-    return (int)((k & 1) ? (w >> 16) : (w & 0xffffu));
+template <typename IX>
+struct Quad {
+    f32x4 d;
+    typename IxQuad<IX>::vec ix;
+};
+
+template <typename IX>
+__device__ __forceinline__ int quad_index(const Quad<IX> &c, int k) {
+    return IxQuad<IX>::get(c.ix, k);
 }
 
-__device__ __forceinline__ float quad_value(const Quad &c, int k) {
+template <typename IX>
+__device__ __forceinline__ float quad_value(const Quad<IX> &c, int k) {
     return k == 0 ? c.d.x : k == 1 ? c.d.y : k == 2 ? c.d.z : c.d.w;
 }
 
@@ -228,9 +259,9 @@ struct Cursor {
     int r, p, e;
 };
 
-template <int R>
+template <int R, typename IX>
 struct SegGroup {
-    Quad c[R];
+    Quad<IX> c[R];
     int b[R], e[R];  // segment bounds (positions relative to the first quad)
     int q0[R];       // first quad of the segment's window
     int row[R];      // row of the segment (nrows past the last row)
@@ -239,11 +270,12 @@ struct SegGroup {
 
 // Forms the next R segments from `cs` and issues their loads (non-temporal
 // when NT: measured faster for X v, slightly slower for X^T u).
-template <int R, bool NT>
+template <int R, bool NT, typename IX>
 __device__ __forceinline__ void form_group(const int *off, int nrows,
-                                           gquad *data, gquad16 *indices,
+                                           gquad *data,
+                                           typename IxQuad<IX>::gvec *indices,
                                            int lane, Cursor &cs,
-                                           SegGroup<R> &G) {
+                                           SegGroup<R, IX> &G) {
     int W[R + 1];  // ends of rows cs.r + 1 ... cs.r + R + 1
 #pragma unroll
     for (int i = 0; i <= R; i++) {
@@ -288,11 +320,11 @@ __device__ __forceinline__ void form_group(const int *off, int nrows,
 // slot's row plus the finished slots before q.  Every lane stores -- lanes
 // without a finished row into the sink slot -- so the store is not behind a
 // branch and the waits for loads stay fixed counts.
-template <int R>
-__device__ __forceinline__ void spmv_reduce(const SegGroup<R> &G,
-                                            const float *lds_v, int lane,
-                                            float &carry, gfloat_w *u,
-                                            gfloat_w *sink) {
+template <int R, typename IX, bool GV>
+__device__ __forceinline__ void spmv_reduce(const SegGroup<R, IX> &G,
+                                            const float *lds_v, gfloat_r *gv,
+                                            int lane, float &carry,
+                                            gfloat_w *u, gfloat_w *sink) {
     float done[R];
     float acc = carry;
 #pragma unroll
@@ -303,7 +335,7 @@ __device__ __forceinline__ void spmv_reduce(const SegGroup<R> &G,
         const int a0 = 4 * (G.q0[q] + lane);
         float vv[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) vv[k] = lds_v[quad_index(G.c[q], k)];
+        for (int k = 0; k < 4; k++) vv[k] = GV ? gv[quad_index(G.c[q], k)] : lds_v[quad_index(G.c[q], k)];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
             const bool in = a0 + k >= G.b[q] && a0 + k < G.e[q];
@@ -345,15 +377,18 @@ __device__ __forceinline__ void spmv_reduce(const SegGroup<R> &G,
 
 }  // namespace
 
-// LDS of k_spmv: vs (g floats), then 4 wave slices of kSubRows + 1 offsets.
+// LDS of k_spmv: vs (g floats; none when v is read from global memory,
+// g > kMaxLdsV), then 4 wave slices of kSubRows + 1 offsets.
 size_t spmv_lds_bytes(int g) {
-    return sizeof(float) * (size_t)g + sizeof(int) * 4 * (size_t)(kSubRows + 1);
+    return sizeof(float) * (size_t)(g <= kMaxLdsV ? g : 0) + sizeof(int) * 4 * (size_t)(kSubRows + 1);
 }
 
-// LDS of k_spmv_t: 4 wave accumulators of g + 1 floats (acc[g] is a spare
-// slot), 4 slices of kSubRows u values, 4 slices of kSubRows + 1 offsets.
+// LDS of k_spmv_t: 4 wave accumulators of tw + 1 floats for a gene tile of
+// tw = min(g, kGeneTile) genes (acc[tw] is a spare slot), 4 slices of
+// kSubRows u values, 4 slices of kSubRows + 1 offsets.
 size_t spmv_t_lds_bytes(int g) {
-    return sizeof(float) * ((size_t)4 * (g + 1) + 4 * kSubRows) +
+    const size_t tw = g < kGeneTile ? g : kGeneTile;
+    return sizeof(float) * ((size_t)4 * (tw + 1) + 4 * kSubRows) +
            sizeof(int) * 4 * (size_t)(kSubRows + 1);
 }
 
@@ -361,17 +396,24 @@ size_t spmv_t_lds_bytes(int g) {
 // exit, after reducing the group that finished the last row (groups formed
 // past it are empty: their loads re-read the sub-block's last quad and their
 // reductions write nothing).
-template <int R>
+// MODE: the problem's index width and v source (kModeLds: 16-bit indices,
+// v staged in LDS; kModeGlobalV: 16-bit, v gathered from global memory, for
+// g > kMaxLdsV; kModeWide: 32-bit indices, v from global memory).  Problems
+// of another mode return at once.  Every mode sums in the same order.
+template <int R, int MODE>
 __global__ __launch_bounds__(256) void k_spmv(Tables T) {
+    typedef typename std::conditional<MODE == kModeWide, uint32_t, uint16_t>::type IX;
+    constexpr bool GV = MODE != kModeLds;
     extern __shared__ float lds_v[];
     const int2 bp = T.blocks[blockIdx.x];
     Prob &pr = T.probs[bp.x];
-    if (pr.phase != PH_LANCZOS) return;
+    if (pr.phase != PH_LANCZOS || pr.mode != MODE) return;
     const int g = pr.g;
-    for (int i = threadIdx.x; i < g; i += 256) lds_v[i] = pr.vs[i];
+    if (!GV)
+        for (int i = threadIdx.x; i < g; i += 256) lds_v[i] = pr.vs[i];
     const int lane = threadIdx.x & (kWave - 1);
     const int w = wave_id();
-    int *off = (int *)(lds_v + g) + w * (kSubRows + 1);
+    int *off = (int *)(lds_v + (GV ? 0 : g)) + w * (kSubRows + 1);
     int64_t r0, nrows64;
     sub_block(pr.n, bp.y, w, r0, nrows64);
     const int nrows = (int)nrows64;
@@ -382,21 +424,22 @@ __global__ __launch_bounds__(256) void k_spmv(Tables T) {
         return;
     }
     gquad *data = (gquad *)pr.data + q0;
-    gquad16 *indices = (gquad16 *)pr.idx16 + q0;
+    typename IxQuad<IX>::gvec *indices = (typename IxQuad<IX>::gvec *)pr.idx + q0;
+    gfloat_r *gv = (gfloat_r *)pr.vs;
     gfloat_w *u = (gfloat_w *)(pr.u32 + r0);
     gfloat_w *sink = (gfloat_w *)(pr.u32 + pr.n);
     Cursor cs{0, uniform(off[0]), uniform(off[1])};
-    SegGroup<R> ga, gb;
+    SegGroup<R, IX> ga, gb;
     float carry = 0.0f;
-    form_group<R, true>(off, nrows, data, indices, lane, cs, ga);
+    form_group<R, true, IX>(off, nrows, data, indices, lane, cs, ga);
     for (;;) {
-        form_group<R, true>(off, nrows, data, indices, lane, cs, gb);
+        form_group<R, true, IX>(off, nrows, data, indices, lane, cs, gb);
         __builtin_amdgcn_sched_barrier(0);
-        spmv_reduce<R>(ga, lds_v, lane, carry, u, sink);
+        spmv_reduce<R, IX, GV>(ga, lds_v, gv, lane, carry, u, sink);
         const bool b_last = cs.r >= nrows;  // gb finished the last row
-        form_group<R, true>(off, nrows, data, indices, lane, cs, ga);
+        form_group<R, true, IX>(off, nrows, data, indices, lane, cs, ga);
         __builtin_amdgcn_sched_barrier(0);
-        spmv_reduce<R>(gb, lds_v, lane, carry, u, sink);
+        spmv_reduce<R, IX, GV>(gb, lds_v, gv, lane, carry, u, sink);
         if (b_last) break;
     }
 }
@@ -613,24 +656,26 @@ __global__ __launch_bounds__(256) void k_alpha(Tables T) {
 
 namespace {
 
-// acc[col] += x * u[row] for the segments of a loaded group, in row order.
-template <int R>
-__device__ __forceinline__ void spmv_t_reduce(const SegGroup<R> &G,
+// acc[col - c0] += x * u[row] for the segments of a loaded group, in row
+// order, for the columns of the gene tile [c0, c0 + tw).
+template <int R, typename IX, bool kTiled>
+__device__ __forceinline__ void spmv_t_reduce(const SegGroup<R, IX> &G,
                                               const float *uval, int lane,
-                                              int g, float *acc) {
+                                              int c0, int tw, float *acc) {
 #pragma unroll
     for (int q = 0; q < R; q++) {
-        // Branch-free: elements outside the segment update the spare slot
-        // acc[g] instead, so the lanes of one LDS update still touch
-        // distinct live genes (a row's indices are unique).
+        // Branch-free: elements outside the segment or the tile update the
+        // spare slot acc[tw] instead, so the lanes of one LDS update still
+        // touch distinct live genes (a row's indices are unique).
         const float ur = uval[G.row[q] < kSubRows ? G.row[q] : 0];
         const int a0 = 4 * (G.q0[q] + lane);
         int c[4];
         float x[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) {
-            const bool in = a0 + k >= G.b[q] && a0 + k < G.e[q];
-            c[k] = in ? quad_index(G.c[q], k) : g;
+            const int cc = quad_index(G.c[q], k) - c0;
+            const bool in = a0 + k >= G.b[q] && a0 + k < G.e[q] && (!kTiled || (unsigned)cc < (unsigned)tw);
+            c[k] = in ? cc : tw;
             x[k] = acc[c[k]];
         }
 #pragma unroll
@@ -646,16 +691,23 @@ __device__ __forceinline__ void spmv_t_reduce(const SegGroup<R> &G,
 // in order (a row's column indices are unique, so the lanes of one LDS update
 // touch distinct genes); sub-blocks combine as (w0+w1)+(w2+w3).  Loads are
 // the segment groups of k_spmv.
-template <int R>
-__global__ __launch_bounds__(256) void k_spmv_t(Tables T) {
+//
+// With g > kGeneTile the genes go in tiles of kGeneTile, one launch per tile
+// (c0 = its first gene): every launch reads the block's rows again and keeps
+// the entries of its tile, so each gene's sum is the same chain whatever the
+// tiling.  The launch with c0 = 0 normalises U[:, j]; later tiles read it.
+template <int R, typename IX, bool kTiled>
+__global__ __launch_bounds__(256) void k_spmv_t(Tables T, int c0) {
     extern __shared__ float lds[];
     const int2 bp = T.blocks[blockIdx.x];
     Prob &pr = T.probs[bp.x];
-    if (pr.phase != PH_LANCZOS) return;
+    if (pr.phase != PH_LANCZOS || (pr.mode == kModeWide) != (sizeof(IX) == 4)) return;
     const int g = pr.g;
+    if (c0 >= g) return;
+    const int tw = g - c0 < kGeneTile ? g - c0 : kGeneTile;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = wave_id();
-    const int gs = g + 1;  // accumulator stride (spare slot at [g])
+    const int gs = tw + 1;  // accumulator stride (spare slot at [tw])
     float *acc = lds + (size_t)w * gs;
     float *uval = lds + (size_t)4 * gs + (size_t)w * kSubRows;
     int *off = (int *)(lds + (size_t)4 * gs + 4 * kSubRows) + w * (kSubRows + 1);
@@ -665,33 +717,37 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T) {
     const int nrows = (int)nrows64;
     double *Uj = pr.U[pr.ubuf] + (int64_t)pr.j * pr.n;
     const double inv_alpha = pr.inv_alpha, umean = pr.umean;
-    for (int t = lane; t < nrows; t += kWave) {
-        const double x = (Uj[r0 + t] - umean) * inv_alpha;
-        Uj[r0 + t] = x;
-        uval[t] = (float)x;
+    if (c0 == 0) {
+        for (int t = lane; t < nrows; t += kWave) {
+            const double x = (Uj[r0 + t] - umean) * inv_alpha;
+            Uj[r0 + t] = x;
+            uval[t] = (float)x;
+        }
+    } else {
+        for (int t = lane; t < nrows; t += kWave) uval[t] = (float)Uj[r0 + t];
     }
     const int64_t q0 = nrows > 0 ? stage_row_offsets(pr.indptr, r0, nrows, lane, off) : 0;
     __syncthreads();
     if (nrows > 0 && off[nrows] > off[0]) {
         gquad *data = (gquad *)pr.data + q0;
-        gquad16 *indices = (gquad16 *)pr.idx16 + q0;
+        typename IxQuad<IX>::gvec *indices = (typename IxQuad<IX>::gvec *)pr.idx + q0;
         Cursor cs{0, uniform(off[0]), uniform(off[1])};
-        SegGroup<R> ga, gb;
-        form_group<R, false>(off, nrows, data, indices, lane, cs, ga);
+        SegGroup<R, IX> ga, gb;
+        form_group<R, false, IX>(off, nrows, data, indices, lane, cs, ga);
         for (;;) {
-            form_group<R, false>(off, nrows, data, indices, lane, cs, gb);
+            form_group<R, false, IX>(off, nrows, data, indices, lane, cs, gb);
             __builtin_amdgcn_sched_barrier(0);
-            spmv_t_reduce<R>(ga, uval, lane, g, acc);
+            spmv_t_reduce<R, IX, kTiled>(ga, uval, lane, c0, tw, acc);
             const bool b_last = cs.r >= nrows;  // gb finished the last row
-            form_group<R, false>(off, nrows, data, indices, lane, cs, ga);
+            form_group<R, false, IX>(off, nrows, data, indices, lane, cs, ga);
             __builtin_amdgcn_sched_barrier(0);
-            spmv_t_reduce<R>(gb, uval, lane, g, acc);
+            spmv_t_reduce<R, IX, kTiled>(gb, uval, lane, c0, tw, acc);
             if (b_last) break;
         }
     }
     __syncthreads();
-    float *out = pr.tpart + (int64_t)bp.y * 2 * g;
-    for (int i = threadIdx.x; i < g; i += 256)
+    float *out = pr.tpart + (int64_t)bp.y * 2 * g + c0;
+    for (int i = threadIdx.x; i < tw; i += 256)
         out[i] = (lds[i] + lds[gs + i]) + (lds[2 * gs + i] + lds[3 * gs + i]);
 }
 
@@ -1449,42 +1505,92 @@ static size_t capped_lds(size_t need, int wg, const Tables &T) {
 size_t lds_bytes_needed(int g, int m) {
     size_t need = spmv_t_lds_bytes(g);
     need = std::max(need, spmv_lds_bytes(g));
-
+    need = std::max(need, sizeof(float) * 4 * (size_t)std::min(g, kGeneTile));  // colstats
     need = std::max(need, sizeof(double) * (size_t)m * kGemmCols);   // gemm
     need = std::max(need, sizeof(double) * (size_t)m + sizeof(int) * (size_t)m);
     return need;
 }
 
+namespace {
+
+template <int R>
+void launch_spmv_modes(const Tables &T, size_t lds_ldsv, size_t lds_global, hipStream_t s) {
+    const unsigned nblk = (unsigned)T.num_blocks;
+    if (T.modes & (1 << kModeLds)) k_spmv<R, kModeLds><<<nblk, 256, lds_ldsv, s>>>(T);
+    if (T.modes & (1 << kModeGlobalV)) k_spmv<R, kModeGlobalV><<<nblk, 256, lds_global, s>>>(T);
+    if (T.modes & (1 << kModeWide)) k_spmv<R, kModeWide><<<nblk, 256, lds_global, s>>>(T);
+}
+
+template <int R>
+void launch_spmv_t_tiles(const Tables &T, size_t lds, hipStream_t s) {
+    const unsigned nblk = (unsigned)T.num_blocks;
+    const bool narrow = (T.modes & ((1 << kModeLds) | (1 << kModeGlobalV))) != 0;
+    const bool wide = (T.modes & (1 << kModeWide)) != 0;
+    if (T.max_g <= kGeneTile) {   // one tile: no range test per element
+        if (narrow) k_spmv_t<R, uint16_t, false><<<nblk, 256, lds, s>>>(T, 0);
+        return;
+    }
+    for (int c0 = 0; c0 < T.max_g; c0 += kGeneTile) {
+        if (narrow) k_spmv_t<R, uint16_t, true><<<nblk, 256, lds, s>>>(T, c0);
+        if (wide) k_spmv_t<R, uint32_t, true><<<nblk, 256, lds, s>>>(T, c0);
+    }
+}
+
+}  // namespace
+
+// X v for every problem of the table (one launch per index mode present),
+// `group` segments per load group, at most `wg` workgroups per CU.
+static void launch_spmv_all(const Tables &T, int group, int wg, hipStream_t s) {
+    const size_t lv = capped_lds(spmv_lds_bytes(std::min(T.max_g, kMaxLdsV)), wg, T);
+    const size_t lg = capped_lds(spmv_lds_bytes(kMaxLdsV + 1), wg, T);
+    switch (group) {
+        case 2: launch_spmv_modes<2>(T, lv, lg, s); break;
+        case 8: launch_spmv_modes<8>(T, lv, lg, s); break;
+        default: launch_spmv_modes<4>(T, lv, lg, s); break;
+    }
+}
+
+// Block partials of X^T u for every problem (one launch per gene tile and
+// index width present).
+static void launch_spmv_t_all(const Tables &T, int group, int wg, hipStream_t s) {
+    const size_t lds = capped_lds(spmv_t_lds_bytes(T.max_g), wg, T);
+    switch (group) {
+        case 2: launch_spmv_t_tiles<2>(T, lds, s); break;
+        case 8: launch_spmv_t_tiles<8>(T, lds, s); break;
+        default: launch_spmv_t_tiles<4>(T, lds, s); break;
+    }
+}
+
 hipError_t launch_spmv_variant(const Tables &T, int which, int group, int wg,
                                hipStream_t s) {
-    Tables V = T;
-    V.spmv_group = group;
-    V.spmvt_group = group;
-    const unsigned nblk = (unsigned)T.num_blocks;
-    const int g = T.max_g;
-    if (which == 0) {
-        const size_t lds = capped_lds(spmv_lds_bytes(g), wg, T);
-        switch (group) {
-            case 2: k_spmv<2><<<nblk, 256, lds, s>>>(V); break;
-            case 8: k_spmv<8><<<nblk, 256, lds, s>>>(V); break;
-            default: k_spmv<4><<<nblk, 256, lds, s>>>(V); break;
-        }
-    } else {
-        const size_t tlds = capped_lds(spmv_t_lds_bytes(g), wg, T);
-        switch (group) {
-            case 2: k_spmv_t<2><<<nblk, 256, tlds, s>>>(V); break;
-            case 8: k_spmv_t<8><<<nblk, 256, tlds, s>>>(V); break;
-            default: k_spmv_t<4><<<nblk, 256, tlds, s>>>(V); break;
-        }
-    }
+    if (which == 0)
+        launch_spmv_all(T, group, wg, s);
+    else
+        launch_spmv_t_all(T, group, wg, s);
     return hipGetLastError();
 }
 
 hipError_t launch_setup(const Tables &T, hipStream_t s) {
     if (T.num_probs == 0) return hipSuccess;
-    const size_t lds4g = sizeof(float) * 4 * (size_t)T.max_g;
-    k_colstats<false><<<(unsigned)T.num_blocks, 256, lds4g, s>>>(T);
-    k_colstats<true><<<(unsigned)T.num_blocks, 256, lds4g, s>>>(T);
+    const size_t lds4g = sizeof(float) * 4 * (size_t)std::min(T.max_g, kGeneTile);
+    const unsigned nblk = (unsigned)T.num_blocks;
+    const bool narrow = (T.modes & ((1 << kModeLds) | (1 << kModeGlobalV))) != 0;
+    const bool wide = (T.modes & (1 << kModeWide)) != 0;
+    if (T.max_g <= kGeneTile) {   // one tile: no range test per element
+        k_colstats<false, uint16_t, false><<<nblk, 256, lds4g, s>>>(T, 0);
+        k_colstats<true, uint16_t, false><<<nblk, 256, lds4g, s>>>(T, 0);
+    } else {
+        for (int c0 = 0; c0 < T.max_g; c0 += kGeneTile) {
+            if (narrow) {
+                k_colstats<false, uint16_t, true><<<nblk, 256, lds4g, s>>>(T, c0);
+                k_colstats<true, uint16_t, true><<<nblk, 256, lds4g, s>>>(T, c0);
+            }
+            if (wide) {
+                k_colstats<false, uint32_t, true><<<nblk, 256, lds4g, s>>>(T, c0);
+                k_colstats<true, uint32_t, true><<<nblk, 256, lds4g, s>>>(T, c0);
+            }
+        }
+    }
     k_reduce_groups<<<dim3((unsigned)ceil_div(2 * T.max_g, 256), (unsigned)T.num_groups), 256, 0, s>>>(T, 1);
     k_setup_finish<<<T.num_probs, 256, 0, s>>>(T);
     return hipGetLastError();
@@ -1522,12 +1628,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
         mark(4, 1);
     }
     mark(0, 0);
-    const size_t slds = capped_lds(spmv_lds_bytes(g), T.spmv_wg, T);
-    switch (T.spmv_group) {
-        case 2: k_spmv<2><<<nblk, 256, slds, s>>>(T); break;
-        case 8: k_spmv<8><<<nblk, 256, slds, s>>>(T); break;
-        default: k_spmv<4><<<nblk, 256, slds, s>>>(T); break;
-    }
+    launch_spmv_all(T, T.spmv_group, T.spmv_wg, s);
     mark(0, 1);
     mark(2, 0);
     k_reorth_partials<<<nblk, 256, 0, s>>>(T);
@@ -1536,12 +1637,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     k_alpha<<<T.num_probs, 256, 0, s>>>(T);
     mark(2, 1);
     mark(1, 0);
-    const size_t tlds = capped_lds(spmv_t_lds_bytes(g), T.spmvt_wg, T);
-    switch (T.spmvt_group) {
-        case 2: k_spmv_t<2><<<nblk, 256, tlds, s>>>(T); break;
-        case 8: k_spmv_t<8><<<nblk, 256, tlds, s>>>(T); break;
-        default: k_spmv_t<4><<<nblk, 256, tlds, s>>>(T); break;
-    }
+    launch_spmv_t_all(T, T.spmvt_group, T.spmvt_wg, s);
     mark(1, 1);
     mark(3, 0);
     k_reduce_groups<<<dim3((unsigned)ceil_div(g, 256), (unsigned)T.num_groups), 256, 0, s>>>(T, 0);
@@ -1586,11 +1682,11 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
 
 // ------------------------------------------------------- validation etc.
 
-template <typename IP>
+template <typename IP, typename IX>
 __global__ __launch_bounds__(256) void k_validate_pack(
     const float *__restrict__ data, const int32_t *__restrict__ indices,
     const IP *__restrict__ indptr, int64_t n, int64_t nnz, int g,
-    float *__restrict__ data4, uint16_t *__restrict__ idx16, int *flag) {
+    float *__restrict__ data4, IX *__restrict__ idx, int *flag) {
     const int lane = threadIdx.x & (kWave - 1);
     const int64_t row = (int64_t)blockIdx.x * 4 + wave_id();
     if (row >= n) return;
@@ -1603,7 +1699,7 @@ __global__ __launch_bounds__(256) void k_validate_pack(
             const int32_t c = indices[p];
             const int32_t prev = p > b ? indices[p - 1] : -1;
             if (c < 0 || c >= g || c <= prev) bad = 1;
-            idx16[p] = (uint16_t)c;
+            idx[p] = (IX)c;
             data4[p] = data[p];
         }
     }
@@ -1612,13 +1708,18 @@ __global__ __launch_bounds__(256) void k_validate_pack(
 
 hipError_t launch_validate_pack(const float *data, const int32_t *indices,
                                 const void *indptr, int indptr_width, int64_t n,
-                                int64_t nnz, int g, float *data4,
-                                uint16_t *idx16, int *flag, hipStream_t s) {
+                                int64_t nnz, int g, float *data4, void *idx,
+                                int *flag, hipStream_t s) {
     dim3 grid((unsigned)ceil_div(n, 4));
-    if (indptr_width == 8)
-        k_validate_pack<int64_t><<<grid, 256, 0, s>>>(data, indices, (const int64_t *)indptr, n, nnz, g, data4, idx16, flag);
+    const bool wide = g > kMaxPacked16;
+    if (indptr_width == 8 && wide)
+        k_validate_pack<int64_t, uint32_t><<<grid, 256, 0, s>>>(data, indices, (const int64_t *)indptr, n, nnz, g, data4, (uint32_t *)idx, flag);
+    else if (indptr_width == 8)
+        k_validate_pack<int64_t, uint16_t><<<grid, 256, 0, s>>>(data, indices, (const int64_t *)indptr, n, nnz, g, data4, (uint16_t *)idx, flag);
+    else if (wide)
+        k_validate_pack<int32_t, uint32_t><<<grid, 256, 0, s>>>(data, indices, (const int32_t *)indptr, n, nnz, g, data4, (uint32_t *)idx, flag);
     else
-        k_validate_pack<int32_t><<<grid, 256, 0, s>>>(data, indices, (const int32_t *)indptr, n, nnz, g, data4, idx16, flag);
+        k_validate_pack<int32_t, uint16_t><<<grid, 256, 0, s>>>(data, indices, (const int32_t *)indptr, n, nnz, g, data4, (uint16_t *)idx, flag);
     return hipGetLastError();
 }
 

@@ -60,7 +60,8 @@ inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 // Per-problem workspace: one allocation carved into these buffers.
 struct WsLayout {
     size_t U0, U1, V0, V1, u32, P, red, sq, tpart, grp, B, sd, rem, vs, W, Jv,
-        Ub, Sb, vw, vdot, vsq, indptr64, idx16, data4, total;
+        Ub, Sb, vw, vdot, vsq, indptr64, idx, data4, total;
+    size_t idx_bytes;  // bytes per packed column index (2, or 4 when g > 65535)
     int64_t nnz4;  // nnz rounded up to whole quads of 4 elements
 };
 
@@ -98,7 +99,8 @@ WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz) {
     L.vsq = take(sizeof(double) * bpca::ceil_div(g, 256));
     L.indptr64 = widen ? take(sizeof(int64_t) * (n + 1)) : 0;
     L.nnz4 = (nnz + 3) / 4 * 4;
-    L.idx16 = take(sizeof(uint16_t) * L.nnz4);
+    L.idx_bytes = g > bpca::kMaxPacked16 ? 4 : 2;
+    L.idx = take(L.idx_bytes * L.nnz4);
     L.data4 = take(sizeof(float) * L.nnz4);
     L.total = off;
     return L;
@@ -131,7 +133,7 @@ int check_shape(int64_t n, int64_t g, int64_t k, int64_t m, float tol,
     if (m > n || m > g) return fail(BPCA_ERR_INVALID_ARGUMENT, "subspace_size must be <= min(num_cells, num_genes)");
     if (!(tol > 0.0f)) return fail(BPCA_ERR_INVALID_ARGUMENT, "tolerance must be positive");
     if (maxit < 1) return fail(BPCA_ERR_INVALID_ARGUMENT, "max_iterations must be a positive integer");
-    if (g > bpca::kMaxGenes) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_genes above 8192 is not supported by the device path");
+    if (g > INT32_MAX / 2) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_genes too large");
     if (m > bpca::kMaxSubspace) return fail(BPCA_ERR_INVALID_ARGUMENT, "subspace_size above 1024 is not supported by the device path");
     if (bpca::ceil_div(n, bpca::kRowBlock) > INT32_MAX) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells too large");
     return BPCA_OK;
@@ -331,13 +333,16 @@ bpca::Prob make_prob(const bpca_problem *pb, const WsLayout &L, char *base,
     const int64_t n = (int64_t)pb->num_cells;
     bpca::Prob d{};
     d.data = (const float *)(base + L.data4);
-    d.idx16 = (const uint16_t *)(base + L.idx16);
+    d.idx = (const void *)(base + L.idx);
     d.indptr = widen ? (const int64_t *)(base + L.indptr64) : (const int64_t *)pb->indptr;
     d.PCs = pb->PCs;
     d.n = n;
     d.nb = bpca::ceil_div(n, bpca::kRowBlock);
     d.ngrp = bpca::ceil_div(d.nb, bpca::kGroup);
     d.g = (int)pb->num_genes;
+    d.mode = d.g > bpca::kMaxPacked16 ? bpca::kModeWide
+             : d.g > bpca::kMaxLdsV   ? bpca::kModeGlobalV
+                                      : bpca::kModeLds;
     d.k = (int)pb->k;
     d.m = (int)pb->subspace_size;
     d.maxit = (int)pb->max_iterations;
@@ -406,10 +411,11 @@ void engine_drop_graph(bpca_engine *E) {
 int engine_rebuild_tables(bpca_engine *E) {
     bpca_context *ctx = E->ctx;
     std::vector<int2> blocks, groups;
-    int max_g = 2, max_m = 1;
+    int max_g = 2, max_m = 1, modes = 0;
     for (int s = 0; s < (int)E->slots.size(); s++) {
         const Slot &sl = E->slots[s];
         if (!sl.used) continue;
+        modes |= 1 << sl.desc.mode;
         for (int64_t b = 0; b < sl.desc.nb; b++) blocks.push_back(make_int2(s, (int)b));
         for (int64_t G = 0; G < sl.desc.ngrp; G++) groups.push_back(make_int2(s, (int)G));
         max_g = std::max(max_g, sl.desc.g);
@@ -441,6 +447,7 @@ int engine_rebuild_tables(bpca_engine *E) {
     E->T.num_groups = (int64_t)groups.size();
     E->T.max_g = max_g;
     E->T.max_m = max_m;
+    E->T.modes = modes;
     E->T.max_lds = (int64_t)ctx->max_lds;
     E->T.spmv_group = ctx->spmv_group;
     E->T.spmvt_group = ctx->spmvt_group;
@@ -835,15 +842,15 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
         if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(int), s);
         // Zero the quad padding past nnz in both packed copies.
         if (e == hipSuccess && L.nnz4 > nnz)
-            e = hipMemsetAsync((char *)ws + L.idx16 + sizeof(uint16_t) * nnz, 0,
-                               sizeof(uint16_t) * (L.nnz4 - nnz), s);
+            e = hipMemsetAsync((char *)ws + L.idx + L.idx_bytes * nnz, 0,
+                               L.idx_bytes * (L.nnz4 - nnz), s);
         if (e == hipSuccess && L.nnz4 > nnz)
             e = hipMemsetAsync((char *)ws + L.data4 + sizeof(float) * nnz, 0,
                                sizeof(float) * (L.nnz4 - nnz), s);
         if (e == hipSuccess)
             e = bpca::launch_validate_pack(pb->data, pb->indices, pb->indptr, pb->indptr_width, n, nnz, g,
                                            (float *)((char *)ws + L.data4),
-                                           (uint16_t *)((char *)ws + L.idx16), flag, s);
+                                           (char *)ws + L.idx, flag, s);
         if (e == hipSuccess) e = hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (flag) (void)hipFree(flag);
@@ -1187,10 +1194,10 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
     // The scratch keeps the last call's contents: clear the counters and the
     // quad padding past nnz of both packed copies.
     HIP_TRY(bpca::launch_zero(d_done, kAlign, s));
-    HIP_TRY(bpca::launch_zero(ws + L.idx16 + sizeof(uint16_t) * nnz, sizeof(uint16_t) * (L.nnz4 - nnz), s));
+    HIP_TRY(bpca::launch_zero(ws + L.idx + L.idx_bytes * nnz, L.idx_bytes * (L.nnz4 - nnz), s));
     HIP_TRY(bpca::launch_zero(ws + L.data4 + sizeof(float) * nnz, sizeof(float) * (L.nnz4 - nnz), s));
     HIP_TRY(bpca::launch_validate_pack(d_data, d_indices, d_indptr, indptr_width, n, nnz, g,
-                                       (float *)(ws + L.data4), (uint16_t *)(ws + L.idx16), d_flag, s));
+                                       (float *)(ws + L.data4), ws + L.idx, d_flag, s));
     int h_flag = 0;
     HIP_TRY(hipMemcpyAsync(&h_flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -1230,6 +1237,7 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
     T.num_groups = ngrp;
     T.max_g = std::max(g, 2);
     T.max_m = std::max(m, 1);
+    T.modes = 1 << d.mode;
     T.max_lds = (int64_t)ctx->max_lds;
     T.spmv_group = ctx->spmv_group;
     T.spmvt_group = ctx->spmvt_group;
@@ -1325,19 +1333,52 @@ int bpca_irlba_f32(const float *data_matvec, const void *indices_matvec,
     if (match_parallel && num_threads != 1)
         return fail(BPCA_ERR_INVALID_ARGUMENT, "match_parallel must be False unless num_threads is 1");
     if (chunk_size < 1) return fail(BPCA_ERR_INVALID_ARGUMENT, "chunk_size must be a positive integer");
-    if (!is_csr && num_threads == 1 && !match_parallel)
-        return fail(BPCA_ERR_INVALID_ARGUMENT,
-                    "the matvec operand must be the CSR of X; on the serial CSC path pass X.tocsr() as the matvec operand");
     BPCA_TRY(check_shape((int64_t)num_cells, num_genes, k, subspace_size, tolerance, max_iterations));
     bpca_context *ctx = nullptr;
     BPCA_TRY(default_context(&ctx));
     std::lock_guard<std::recursive_mutex> guard(ctx->lock);
     DeviceGuard dg(ctx->device);
     const int64_t n = (int64_t)num_cells;
-    const int64_t nnz = index_width == 8 ? ((const int64_t *)indptr_matvec)[n]
-                                         : (int64_t)((const int32_t *)indptr_matvec)[n];
+    // On the reference's serial CSC path (single_cell.py:12054-12069) the
+    // matvec operand is the CSC of X: its offsets run over genes and its
+    // indices are cells.  It is transposed to the CSR on the device.
+    const bool csc = !is_csr && num_threads == 1 && !match_parallel;
+    const int64_t outer = csc ? (int64_t)num_genes : n;
+    const int64_t nnz = index_width == 8 ? ((const int64_t *)indptr_matvec)[outer]
+                                         : (int64_t)((const int32_t *)indptr_matvec)[outer];
     if (nnz < 0) return fail(BPCA_ERR_NONCANONICAL, "indptr[num_cells] is negative");
     hipStream_t s = ctx->stream;
+    if (csc) {
+        DevMem<float> c_data, d_data, d_pcs;
+        DevMem<char> c_idx, c_ptr;
+        DevMem<int32_t> d_idx;
+        DevMem<int64_t> d_ptr;
+        HIP_TRY(c_data.alloc(nnz));
+        HIP_TRY(c_idx.alloc((size_t)nnz * index_width));
+        HIP_TRY(c_ptr.alloc((size_t)(outer + 1) * index_width));
+        HIP_TRY(d_data.alloc(nnz));
+        HIP_TRY(d_idx.alloc(nnz));
+        HIP_TRY(d_ptr.alloc(n + 1));
+        HIP_TRY(d_pcs.alloc((size_t)n * k));
+        HIP_TRY(hipMemcpyAsync(c_data.p, data_matvec, sizeof(float) * nnz, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(c_idx.p, indices_matvec, (size_t)nnz * index_width, hipMemcpyHostToDevice, s));
+        HIP_TRY(hipMemcpyAsync(c_ptr.p, indptr_matvec, (size_t)(outer + 1) * index_width, hipMemcpyHostToDevice, s));
+        hipError_t herr = hipSuccess;
+        const int rc = bpca::run_csr_transpose(c_data.p, c_idx.p, index_width, c_ptr.p, index_width, outer, n,
+                                               nnz, d_data.p, d_idx.p, d_ptr.p, &herr, s);
+        if (rc == 1)
+            return fail(BPCA_ERR_NONCANONICAL,
+                        "CSC is not canonical: indptr must start at 0 and be non-decreasing, and row indices must lie in [0, num_cells)");
+        if (rc == 2) return hip_fail(herr, "CSC to CSR transpose");
+        bpca_solve_stats st{};
+        BPCA_TRY(bpca_irlba_device(ctx, d_data.p, d_idx.p, d_ptr.p, 8, num_cells, num_genes, k,
+                                   subspace_size, tolerance, max_iterations, seed, 1, d_pcs.p,
+                                   nullptr, &st));
+        HIP_TRY(hipMemcpyAsync(PCs, d_pcs.p, sizeof(float) * n * k, hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        if (converged) *converged = st.converged;
+        return BPCA_OK;
+    }
     DevMem<float> d_data, d_pcs;
     DevMem<int32_t> d_idx;
     DevMem<int64_t> d_idx64;
@@ -1623,6 +1664,44 @@ int bpca_kmeans(bpca_context *ctx, const float *d_X, int64_t num_cells,
     if (rc == 1) return fail(BPCA_ERR_INVALID_ARGUMENT, msg);
     if (rc == 2) return hip_fail(herr, "run_knn_ivf");
     *iterations_run = I.iterations;
+    return BPCA_OK;
+}
+
+int bpca_csr_transpose(bpca_context *ctx, const float *d_data,
+                       const void *d_indices, int index_width,
+                       const void *d_indptr, int indptr_width,
+                       int64_t num_rows, int64_t num_cols, float *d_out_data,
+                       int32_t *d_out_indices, int64_t *d_out_indptr) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!d_indptr || !d_out_indptr) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL device pointer");
+    if ((index_width != 4 && index_width != 8) || (indptr_width != 4 && indptr_width != 8))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "index_width and indptr_width must be 4 or 8");
+    if (num_rows < 0 || num_cols < 0 || num_rows > (int64_t)UINT32_MAX || num_cols > (int64_t)INT32_MAX)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "num_rows must fit 32 bits and num_cols 31 bits");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    begin_call(ctx);
+    hipStream_t s = ctx->stream;
+    int64_t nnz = 0;
+    if (indptr_width == 8) {
+        HIP_TRY(hipMemcpyAsync(&nnz, (const int64_t *)d_indptr + num_rows, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    } else {
+        int32_t v = 0;
+        HIP_TRY(hipMemcpyAsync(&v, (const int32_t *)d_indptr + num_rows, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        nnz = v;
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    if (nnz < 0) return fail(BPCA_ERR_NONCANONICAL, "indptr[num_rows] is negative");
+    if (nnz > 0 && (!d_data || !d_indices || !d_out_data || !d_out_indices))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL device pointer");
+    hipError_t herr = hipSuccess;
+    const int rc = bpca::run_csr_transpose(d_data, d_indices, index_width, d_indptr, indptr_width, num_rows,
+                                           num_cols, nnz, d_out_data, d_out_indices, d_out_indptr, &herr, s);
+    if (rc == 1)
+        return fail(BPCA_ERR_NONCANONICAL,
+                    "transpose: indptr must start at 0 and be non-decreasing, and indices must lie in [0, num_cols)");
+    if (rc == 2) return hip_fail(herr, "transpose");
     return BPCA_OK;
 }
 

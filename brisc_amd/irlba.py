@@ -37,8 +37,10 @@ def irlba(data_matvec, indices_matvec, indptr_matvec, data_rmatvec,
     """Top-k PCs of the implicitly centred and scaled matrix, on the GPU.
 
     Mirrors ``brisc.pca.irlba`` (``brisc/pca.pyx:661-678``).  The matvec
-    operand is the CSR of X; the rmatvec (CSC) operand is accepted and not
-    used, because ``X^T u`` is computed from the CSR.  ``PCs`` (float32,
+    operand is the CSR of X, or its CSC on the reference's serial CSC path
+    (``is_csr=False, num_threads=1, match_parallel=False``; transposed on the
+    device); the rmatvec operand is accepted and not used, because
+    ``X^T u`` is computed from the CSR.  ``PCs`` (float32,
     C-contiguous, ``(num_cells, k)``) is filled in place with ``U_k S_k``.
     Returns whether all ``k`` singular values converged.
     """
@@ -57,8 +59,12 @@ def irlba(data_matvec, indices_matvec, indptr_matvec, data_rmatvec,
             not PCs.flags.c_contiguous or PCs.shape != (num_cells, k):
         raise TypeError('PCs must be a C-contiguous float32 array of shape '
                         '(num_cells, k)')
-    if len(indptr) != num_cells + 1:
-        raise ValueError('indptr_matvec must have num_cells + 1 entries')
+    # On the reference's serial CSC path the matvec operand is the CSC of X.
+    csc = not is_csr and num_threads == 1 and not match_parallel
+    outer = num_genes if csc else num_cells
+    if len(indptr) != outer + 1:
+        raise ValueError(f'indptr_matvec must have '
+                         f'{"num_genes" if csc else "num_cells"} + 1 entries')
     if int(indptr[-1]) > len(data) or len(indices) < int(indptr[-1]):
         raise ValueError('indptr_matvec[-1] exceeds the number of stored '
                          'elements')

@@ -257,12 +257,25 @@ class SingleCell:
         import torch
 
         from .irlba import irlba_device
-        from .subset import csr_subset
+        from .subset import csr_subset, csr_transpose
         dev = torch.device('cuda', 0)
         parts = []
         for d, q, genes in zip(datasets, QC_columns, gene_indices):
-            X = d._X if isinstance(d._X, (sp.csr_array, sp.csr_matrix)) \
-                else sp.csr_array(d._X)
+            X = d._X
+            if isinstance(X, (sp.csc_array, sp.csc_matrix)) and X.has_canonical_format and \
+                    not (genes is not None and len(genes) > 4096 and not np.all(np.diff(genes) > 0)):
+                # CSC: uploaded as it is and turned into the CSR on the
+                # device (the reference's X.tocsr(), single_cell.py:12066).
+                upload = [torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+                          for a in (X.data.astype(np.float32, copy=False),
+                                    X.indices, X.indptr)]
+                csr = csr_transpose(*upload, X.shape[0])
+                del upload
+                parts.append(csr_subset(*csr, X.shape[1], rows=q, genes=genes))
+                del csr
+                continue
+            if not isinstance(X, (sp.csr_array, sp.csr_matrix)):
+                X = sp.csr_array(X)
             if not X.has_canonical_format:
                 X = sp.csr_array(X, copy=True)
                 X.sum_duplicates()

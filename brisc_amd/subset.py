@@ -107,3 +107,35 @@ def csr_subset(data, indices, indptr, num_cols: int, rows=None, genes=None,
         d_map.data_ptr() if d_map is not None else None, int(increasing),
         out_indptr.data_ptr(), max_row, out_data.data_ptr(), out_indices.data_ptr()))
     return out_data[:out_nnz], out_indices[:out_nnz], out_indptr
+
+
+def csr_transpose(data, indices, indptr, num_cols: int, context=None):
+    """Transpose of a sparse matrix stored by rows, on the device: the CSR of
+    X from its CSC (or the CSC from the CSR), as brisc's csr_tocsc / scipy's
+    tocsr do (``brisc/cysparse.pyx:1179-1295``).  ``data`` float32,
+    ``indices`` and ``indptr`` int32/int64 device tensors; ``num_cols`` is the
+    input's minor dimension.  Returns ``(data float32, indices int32, indptr
+    int64)`` device tensors, each output row in ascending input-row order.
+    """
+    import torch
+
+    from .irlba import default_context
+    ctx = context or default_context(data.device.index or 0)
+    dev = data.device
+    num_rows = indptr.numel() - 1
+    nnz = indices.numel()
+    if data.dtype != torch.float32 or data.numel() != nnz:
+        raise ValueError('data must be float32 with one value per index')
+    if indices.dtype not in (torch.int32, torch.int64) or indptr.dtype not in (torch.int32, torch.int64):
+        raise ValueError('indices and indptr must be int32 or int64')
+    for t in (data, indices, indptr):
+        if not t.is_contiguous() or t.device != dev:
+            raise ValueError('data, indices and indptr must be contiguous tensors on one device')
+    out_data = torch.empty(nnz, dtype=torch.float32, device=dev)
+    out_indices = torch.empty(nnz, dtype=torch.int32, device=dev)
+    out_indptr = torch.empty(int(num_cols) + 1, dtype=torch.int64, device=dev)
+    _lib.check(_lib.lib().bpca_csr_transpose(
+        ctx.handle, data.data_ptr(), indices.data_ptr(), indices.element_size(),
+        indptr.data_ptr(), indptr.element_size(), num_rows, int(num_cols),
+        out_data.data_ptr(), out_indices.data_ptr(), out_indptr.data_ptr()))
+    return out_data, out_indices, out_indptr

@@ -140,9 +140,11 @@ typedef struct bpca_solve_stats {
  * Same arguments in the same order and with the same meaning; the index
  * arrays are all int32 (index_width 4) or all int64 (index_width 8), as the
  * reference's fused type `signed_integer` requires (cyutils.pxd:25-27).
- * The matvec operand must be the CSR of X (as SingleCell.pca passes it for
- * the parallel path, single_cell.py:12037-12069); the rmatvec (CSC) operand
- * may be NULL because X^T.u is computed from the CSR.  match_parallel,
+ * The matvec operand is the CSR of X, except on the reference's serial CSC
+ * path (is_csr = 0, num_threads = 1, match_parallel = 0;
+ * single_cell.py:12054-12069), where it is the CSC of X (num_genes + 1
+ * offsets, cell indices), which is transposed to the CSR on the device.  The
+ * rmatvec operand may be NULL because X^T.u is computed from the CSR.  match_parallel,
  * is_csr, num_threads and chunk_size are validated and otherwise do not
  * change the result.  PCs (num_cells x k, float32, C order, caller-owned)
  * receives U_k S_k; *converged receives the return value of the reference
@@ -335,6 +337,21 @@ int bpca_csr_subset_fill(bpca_context *ctx, const float *d_data,
                          const int32_t *d_col_map, int col_map_increasing,
                          const int64_t *d_out_indptr, int64_t max_row_nnz,
                          float *d_out_data, int32_t *d_out_indices);
+
+/* Transpose of a sparse matrix stored by rows: CSR -> CSC or CSC -> CSR, the
+ * role of brisc's csr_tocsc / csc_tocsr (brisc/cysparse.pyx:1179-1295,
+ * sparse.py:928-947), on the device.  Input: num_rows + 1 offsets and
+ * indices in [0, num_cols), int32 or int64 (widths 4 or 8).  Output:
+ * num_cols + 1 int64 offsets, int32 indices and the values, each output row
+ * listing its entries in ascending input-row order (as the reference's
+ * stable scatter and scipy's tocsc/tocsr).  Deterministic (stable radix sort,
+ * no atomics).  BPCA_ERR_NONCANONICAL for offsets or indices out of range.
+ * Synchronous on the context's stream; allocates its work space. */
+int bpca_csr_transpose(bpca_context *ctx, const float *d_data,
+                       const void *d_indices, int index_width,
+                       const void *d_indptr, int indptr_width,
+                       int64_t num_rows, int64_t num_cols, float *d_out_data,
+                       int32_t *d_out_indices, int64_t *d_out_indptr);
 
 /* The exact k nearest neighbours of every cell among all cells, from PCs
  * held on the device: the exhaustive form of SingleCell.neighbors()

@@ -15,6 +15,8 @@ Contents:
 * ``fp64_pca`` -- the FP64 oracle: exact Gram ``Z^T Z`` of the implicitly
   centred and scaled matrix, diagonalised with ``numpy.linalg.eigh``; gives
   singular values, right singular vectors and PCs (``Z V_k``).
+* ``fp64_pca_cells`` -- the same through the cell Gram ``Z Z^T`` (for many
+  genes and few cells).
 * ``clipped_stddev`` -- closed form of ``clipped_stddev_csc``
   (``brisc/pca.pyx:51-103``) in FP64.
 * ``kmeans_fp64`` -- Lloyd's k-means as ``kmeans()`` runs it
@@ -214,6 +216,28 @@ def fp64_pca(data, indices, indptr, num_cells, num_genes, k,
         W = Vk / sd[:, None]
         out['PCs'] = X @ W - (mu @ W)[None, :]
     return out
+
+
+def fp64_pca_cells(data, indices, indptr, num_cells, num_genes, k):
+    """FP64 oracle through the cell-by-cell Gram, for many genes and few cells
+    (the gene Gram of fp64_pca would be num_genes^2): Z Z^T (n x n) of
+    Z = (X - 1 mu^T) D^-1 from the sparse product Y Y^T, Y = X D^-1, with the
+    centring applied in closed form, diagonalised with numpy.linalg.eigh.
+    Returns dict(singular_values (k,), PCs (n, k) = U_k S_k)."""
+    import scipy.sparse as sp
+    sd = clipped_stddev(data, indices, indptr, num_cells, num_genes)
+    X = sp.csr_array((np.asarray(data, np.float64), np.asarray(indices, np.int64),
+                      np.asarray(indptr, np.int64)), shape=(num_cells, num_genes))
+    Y = X @ sp.diags_array(1.0 / sd)
+    mu = np.asarray(X.sum(axis=0)).ravel() / num_cells
+    w = mu / sd                                   # D^-1 mu
+    G = (Y @ Y.T).toarray()
+    a = Y @ w                                     # Y D^-1 mu
+    G = G - a[:, None] - a[None, :] + float(w @ w)
+    evals, evecs = np.linalg.eigh(G)
+    order = np.argsort(evals)[::-1][:k]
+    lam = np.maximum(evals[order], 0.0)
+    return dict(singular_values=np.sqrt(lam), PCs=evecs[:, order] * np.sqrt(lam))
 
 
 def dense_z(data, indices, indptr, num_cells, num_genes):

@@ -61,7 +61,7 @@ def _call_irlba(n=10, g=5, k=2, m=4, tol=1e-6, maxit=10, width=4,
     (dict(width=2), 'index_width must be 4 or 8'),
     (dict(match_parallel=1, threads=2), 'match_parallel must be False unless num_threads is 1'),
     (dict(chunk=0), 'chunk_size'),
-    (dict(g=9000, m=4), 'num_genes above 8192'),
+    (dict(g=2**31 - 1, m=4), 'num_genes too large'),
 ])
 def test_argument_errors(kwargs, fragment):
     rc, msg = _call_irlba(**kwargs)

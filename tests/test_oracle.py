@@ -113,6 +113,38 @@ def test_fp64_oracle_matches_dense_svd(golden, name):
     assert errs.max() < 1e-8
 
 
+@pytest.mark.parametrize('name', TINY)
+def test_cell_gram_oracle_matches_dense_svd(golden, name):
+    """fp64_pca_cells (Z Z^T with the centring in closed form) against the
+    dense SVD of Z on the tiny inputs."""
+    case, (data, indices, indptr, _) = _tiny_case(golden, name)
+    n, g, k = case['num_cells'], case['num_genes'], case['k']
+    out = oracle.fp64_pca_cells(data, indices, indptr, n, g, k)
+    Z = oracle.dense_z(data, indices, indptr, n, g)
+    U, S, Vt = np.linalg.svd(Z, full_matrices=False)
+    assert np.allclose(out['singular_values'], S[:k], rtol=1e-10, atol=0)
+    ws = helpers.well_separated(S[:k], 1e-3)
+    errs = helpers.column_errors_up_to_sign(out['PCs'][:, ws], (U * S)[:, ws])
+    assert errs.max() < 1e-8
+
+
+def test_cell_gram_oracle_planted_centring_mistake_is_caught():
+    """Both oracles agree on a matrix with more genes than cells, and the
+    cell Gram without its centring terms (a plausible mistake) does not."""
+    from brisc_amd import synth
+    import scipy.sparse as sp
+    w = synth.make_workload(71, 600, num_types=4, num_genes=1500)
+    data, indices, indptr = oracle.generate_csr(w)
+    a = oracle.fp64_pca_cells(data, indices, indptr, 600, 1500, 20)['singular_values']
+    b = oracle.fp64_pca(data, indices, indptr, 600, 1500, 20, want_pcs=False)['singular_values']
+    assert np.allclose(a, b, rtol=1e-9, atol=0)
+    sd = oracle.clipped_stddev(data, indices, indptr, 600, 1500)
+    X = sp.csr_array((data.astype(np.float64), indices, indptr), shape=(600, 1500))
+    Y = X @ sp.diags_array(1.0 / sd)
+    wrong = np.sqrt(np.sort(np.linalg.eigvalsh((Y @ Y.T).toarray()))[::-1][:20])
+    assert np.max(np.abs(wrong - a) / a) > 100 * SV_RTOL
+
+
 def test_stddev_closed_form():
     rng = np.random.default_rng(0)
     X = (rng.random((60, 9)) < 0.3) * rng.random((60, 9)).astype(np.float32)
