@@ -109,6 +109,9 @@ def lib() -> ctypes.CDLL:
                                       ctypes.POINTER(ctypes.c_uint64)]),
         'bpca_start_vector': (i32, [u64, u32, p, p]),
         'bpca_csr_transpose': (i32, [p, p, p, i32, p, i32, i64, i64, p, p, p]),
+        'bpca_normalize': (i32, [p, p, i32, p, i32, p, i32, i32, i64, i64, p,
+                                 i32, p, p]),
+        'bpca_test_log1pf': (i32, [p, p, i64, p]),
         'bpca_context_set_profiling': (i32, [p, i32]),
         'bpca_context_kernel_times': (i32, [p, p, p]),
         'bpca_context_reset_kernel_times': (i32, [p]),

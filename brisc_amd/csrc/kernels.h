@@ -317,6 +317,27 @@ int run_csr_transpose(const float *data, const void *indices, int index_width,
                       int32_t *out_indices, int64_t *out_indptr,
                       hipError_t *herr, hipStream_t s);
 
+// SingleCell.normalize() on the device (normalize.hip).
+struct NormalizeArgs {
+    const void *data;
+    int data_type;        // 0 int32, 1 uint32, 2 int64, 3 uint64, 4 float32, 5 float64
+    const void *indices;  // CSR: gene of each entry; CSC: cell of each entry
+    int index_width;      // 4 or 8
+    const void *indptr;   // CSR: num_cells + 1 offsets; CSC: num_genes + 1
+    int indptr_width;     // 4 or 8
+    int is_csr;
+    int64_t num_cells, num_genes, nnz;
+    const uint8_t *qc;    // cells passing QC (nonzero), or null for all
+    int method;           // 0 logCP10k, 1 log1pPF, 2 PFlog1pPF
+    float *out;           // nnz normalised values (may alias float32 data)
+    uint64_t *row_sums;   // num_cells row sums of the counts
+};
+// 0: done; 1: a CSC row index out of range; 2: HIP error (*herr).
+// Synchronous.
+int run_normalize(const NormalizeArgs &A, hipError_t *herr, hipStream_t s);
+// y = the host library's log1pf of x, elementwise (test of log1pf.h).
+hipError_t launch_log1pf_eval(const float *x, int64_t n, float *y, hipStream_t s);
+
 // Synthetic workload generator and log1pPF (synth.hip).
 hipError_t launch_synth_row_nnz(uint64_t seed, uint64_t row0, int64_t rows,
                                 int g, const double *tgm, const int32_t *type,

@@ -1705,6 +1705,58 @@ int bpca_csr_transpose(bpca_context *ctx, const float *d_data,
     return BPCA_OK;
 }
 
+int bpca_normalize(bpca_context *ctx, const void *d_data, int data_type,
+                   const void *d_indices, int index_width, const void *d_indptr,
+                   int indptr_width, int is_csr, int64_t num_cells,
+                   int64_t num_genes, const uint8_t *d_qc, int method,
+                   float *d_normalized, uint64_t *d_row_sums) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!d_indptr || !d_row_sums) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL device pointer");
+    if (data_type < 0 || data_type > 5)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "data_type must be 0 (int32), 1 (uint32), 2 (int64), 3 (uint64), 4 (float32) or 5 (float64)");
+    if ((index_width != 4 && index_width != 8) || (indptr_width != 4 && indptr_width != 8))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "index_width and indptr_width must be 4 or 8");
+    if (method < 0 || method > 2)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "method must be 0 (logCP10k), 1 (log1pPF) or 2 (PFlog1pPF)");
+    if (num_cells < 1 || num_genes < 1) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells and num_genes must be positive");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    begin_call(ctx);
+    hipStream_t s = ctx->stream;
+    const int64_t outer = is_csr ? num_cells : num_genes;
+    int64_t nnz = 0;
+    if (indptr_width == 8) {
+        HIP_TRY(hipMemcpyAsync(&nnz, (const int64_t *)d_indptr + outer, sizeof(int64_t), hipMemcpyDeviceToHost, s));
+    } else {
+        int32_t v = 0;
+        HIP_TRY(hipMemcpyAsync(&v, (const int32_t *)d_indptr + outer, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+        nnz = v;
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    if (nnz < 0) return fail(BPCA_ERR_NONCANONICAL, "indptr's last offset is negative");
+    if (nnz > 0 && (!d_data || !d_indices || !d_normalized))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL device pointer");
+    bpca::NormalizeArgs A{d_data, data_type, d_indices, index_width, d_indptr, indptr_width,
+                          is_csr != 0, num_cells, num_genes, nnz, d_qc, method, d_normalized,
+                          d_row_sums};
+    hipError_t herr = hipSuccess;
+    const int rc = bpca::run_normalize(A, &herr, s);
+    if (rc == 1) return fail(BPCA_ERR_NONCANONICAL, "normalize: an index is outside [0, num_cells)");
+    if (rc == 2) return hip_fail(herr, "normalize");
+    return BPCA_OK;
+}
+
+int bpca_test_log1pf(bpca_context *ctx, const float *d_x, int64_t n, float *d_y) {
+    if (!ctx || (n > 0 && (!d_x || !d_y))) return fail(BPCA_ERR_INVALID_ARGUMENT, "bad argument");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    begin_call(ctx);
+    HIP_TRY(bpca::launch_log1pf_eval(d_x, n, d_y, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return BPCA_OK;
+}
+
 int bpca_start_vector(uint64_t seed, uint32_t num_genes, float *draws,
                       double *v0) {
     if (!draws && !v0) return fail(BPCA_ERR_INVALID_ARGUMENT, "draws and v0 are both NULL");

@@ -11,6 +11,7 @@
 // rounded separately, which makes the counts independent of where they are
 // computed.
 #include "kernels.h"
+#include "log1pf.h"
 
 namespace bpca {
 
@@ -180,7 +181,7 @@ __global__ __launch_bounds__(256) void log1ppf_kernel(
     const float factor = (float)(*total) / (float)n;
     const float isf = factor / (float)row_sums[r];
     for (int64_t p = indptr[r] + lane; p < indptr[r + 1]; p += kWave)
-        out[p] = log1pf((float)counts[p] * isf);
+        out[p] = log1pf_ref((float)counts[p] * isf);
 }
 
 hipError_t launch_log1ppf(const int32_t *counts, const int64_t *indptr,

@@ -125,6 +125,73 @@ class SingleCell:
                              f'positive integer, -1, or None')
         return num_threads
 
+    def skip_qc(self) -> 'SingleCell':
+        """The dataset with ``uns['QCed']`` set to True
+        (``single_cell.py:9888-9896``)."""
+        uns = dict(self._uns)
+        uns['QCed'] = True
+        return SingleCell(X=self._X, obs=self._obs, var=self._var,
+                          obsm=self._obsm, varm=self._varm, uns=uns,
+                          num_threads=self._num_threads)
+
+    def normalize(self, *, QC_column='passed_QC', method='log1pPF',
+                  inplace: bool = False, num_threads=None) -> 'SingleCell':
+        """Normalise the counts on the GPU (``single_cell.py:11450-11624``):
+        the same signature, checks and messages; ``method`` one of
+        ``'PFlog1pPF'``, ``'log1pPF'`` (default) or ``'logCP10k'``; cells
+        failing QC are normalised but do not count towards the mean library
+        size.  X keeps its layout (CSR or CSC) and its indices; the values
+        become float32.  Returns a new dataset with ``uns['normalized']``
+        True, or this one normalised in place (float32 counts only)."""
+        import torch
+
+        from .normalize import normalize_device
+        if self._X is None:
+            raise ValueError('X is None, so normalizing is not possible')
+        if not self._uns.get('QCed', False):
+            raise ValueError(
+                "uns['QCed'] is False; did you forget to run qc() (and "
+                "possibly hvg()) before normalize()? Set uns['QCed'] = True "
+                "or run skip_qc() to bypass this check.")
+        if self._uns['normalized']:
+            raise ValueError(
+                "uns['normalized'] is True; did you already run normalize()?")
+        qc = self._get_column('obs', QC_column, 'QC_column',
+                              QC_column == 'passed_QC', None, len(self)) \
+            if QC_column is not None else None
+        if qc is not None and not qc.any():
+            raise ValueError('no cells pass QC: QC_column is all-False')
+        if method not in ('PFlog1pPF', 'log1pPF', 'logCP10k'):
+            raise ValueError("method must be one of 'PFlog1pPF', 'log1pPF', or "
+                             "'logCP10k'")
+        check_type(inplace, 'inplace', bool, 'Boolean')
+        if inplace and self._X.dtype != np.float32:
+            raise TypeError(f'inplace normalization requires floating-point '
+                            f'count matrices, but X has data type '
+                            f'{str(self._X.dtype)!r}')
+        self._process_num_threads(num_threads)
+        X = self._X
+        is_csr = isinstance(X, (sp.csr_array, sp.csr_matrix))
+        dev = torch.device('cuda', 0)
+        d, i, p = (torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+                   for a in (X.data, X.indices, X.indptr))
+        out, _ = normalize_device(d, i, p, X.shape[0], X.shape[1],
+                                  is_csr=is_csr, QC=qc, method=method,
+                                  out=d if inplace else None)
+        values = out.cpu().numpy()
+        if inplace:
+            X.data[:] = values
+            self._uns['normalized'] = True
+            return self
+        cls = type(X) if isinstance(X, (sp.csr_array, sp.csc_array)) else \
+            (sp.csr_array if is_csr else sp.csc_array)
+        Y = cls((values, X.indices, X.indptr), shape=X.shape)
+        uns = dict(self._uns)
+        uns['normalized'] = True
+        return SingleCell(X=Y, obs=self._obs, var=self._var, obsm=self._obsm,
+                          varm=self._varm, uns=uns,
+                          num_threads=self._num_threads)
+
     def _get_column(self, where: str, column, name: str, allow_missing: bool,
                     custom_error: str | None, length: int):
         if column is None:

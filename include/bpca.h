@@ -298,6 +298,35 @@ int bpca_synth_fill(bpca_context *ctx, uint64_t seed, uint64_t row0,
                     const double *d_cell_factor, const int64_t *d_indptr,
                     int32_t *d_indices, int32_t *d_counts);
 
+/* SingleCell.normalize() on the device: brisc/normalize.pyx:9-268
+ * (normalize_csr, is_csr = 1) and 271-460 (normalize_csc, is_csr = 0).
+ *   d_data      counts, data_type 0 int32, 1 uint32, 2 int64, 3 uint64,
+ *               4 float32, 5 float64 (the reference's `numeric` types)
+ *   d_indices   CSR: gene of each entry; CSC: cell of each entry
+ *   d_indptr    CSR: num_cells + 1 offsets; CSC: num_genes + 1 offsets
+ *               (index_width / indptr_width 4 or 8)
+ *   d_qc        uint8 per cell, nonzero = passed QC (NULL: all cells); the
+ *               QC cells alone define the mean library size
+ *   method      0 logCP10k, 1 log1pPF, 2 PFlog1pPF
+ *   d_normalized  float32 output per entry (may be d_data for float32
+ *               counts: the reference's inplace=True)
+ *   d_row_sums  uint64 per cell: its library size, as the reference returns
+ * Each value is the reference's, bit for bit: the same single-precision
+ * operations in the same order (PFlog1pPF's per-cell sums and their total
+ * included) and log1p bit-identical to the host C library's log1pf.
+ * Synchronous.  When d_qc selects no cell the reference's pca() raises
+ * before calling (no cells pass QC); here the factor is then NaN. */
+int bpca_normalize(bpca_context *ctx, const void *d_data, int data_type,
+                   const void *d_indices, int index_width, const void *d_indptr,
+                   int indptr_width, int is_csr, int64_t num_cells,
+                   int64_t num_genes, const uint8_t *d_qc, int method,
+                   float *d_normalized, uint64_t *d_row_sums);
+
+/* y = log1pf(x) elementwise on the device with the restated host-library
+ * log1pf that bpca_normalize uses (test entry point). */
+int bpca_test_log1pf(bpca_context *ctx, const float *d_x, int64_t n,
+                     float *d_y);
+
 /* log1pPF normalisation of integer counts on the device
  * (brisc/normalize.pyx:9-128, method 'log1pPF', no QC column):
  * x' = log1p(x * (mean_row_sum / row_sum)) in float32. */

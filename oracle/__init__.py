@@ -6,6 +6,7 @@ CPU baseline).  The product library never links or calls it.
 
 Contents:
 
+* ``log1pf`` -- the host C library's log1pf, elementwise.
 * ``generate_csr`` -- CPU generator of the synthetic NB count workloads
   (``gen.c``), bit-identical in its integer counts to the device generator,
   followed by the log1pPF restatement of ``brisc/normalize.pyx:9-128``.
@@ -62,6 +63,7 @@ def lib() -> ctypes.CDLL:
         L.gen_row_nnz.argtypes = [u64, u64, u64, u32, p, p, p, p]
         L.gen_fill.argtypes = [u64, u64, u64, u32, p, p, p, p, p, p]
         L.normalize_log1ppf.argtypes = [p, p, u64, p, p]
+        L.log1pf_array.argtypes = [p, i64, p]
         L.gram_f64.argtypes = [p, p, p, i64, ctypes.c_int32, p, p]
         L.gram_f64.restype = ctypes.c_int
         L.irlba_port.argtypes = [p, p, p, u64, u32, u32, u32, ctypes.c_float,
@@ -110,6 +112,15 @@ def log1ppf(counts: np.ndarray, indptr: np.ndarray) -> np.ndarray:
     lib().normalize_log1ppf(_ptr(counts), _ptr(indptr), n,
                             _ptr(out), _ptr(row_sums))
     return out
+
+
+def log1pf(x: np.ndarray) -> np.ndarray:
+    """The host C library's log1pf of float32 ``x`` (what the reference's
+    normalize calls)."""
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.empty_like(x)
+    lib().log1pf_array(_ptr(x), x.size, _ptr(y))
+    return y
 
 
 def generate_csr(w):

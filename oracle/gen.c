@@ -119,3 +119,10 @@ void normalize_log1ppf(const int32_t *counts, const int64_t *indptr,
             out[e] = log1pf((float)counts[e] * f);
     }
 }
+
+/* The host C library's log1pf, elementwise (the function the reference's
+   normalize calls, normalize.pyx:70). */
+void log1pf_array(const float *x, int64_t n, float *y) {
+    #pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < n; i++) y[i] = log1pf(x[i]);
+}

@@ -10,6 +10,7 @@ Run in the development container only (it reads the reference sources):
     python tests/golden/make_ref_vectors.py --knn-approx (adds few-clusters-searched fixtures)
     python tests/golden/make_ref_vectors.py --rng       (adds the start-vector draws)
     python tests/golden/make_ref_vectors.py --calibrate (times the reference against the port)
+    python tests/golden/make_ref_vectors.py --normalize (adds the normalisation fixtures)
 
 It cythonizes ``brisc/cyutils.pyx``, ``brisc/pca.pyx``,
 ``brisc/normalize.pyx``, ``brisc/kmeans.pyx`` and ``brisc/knn.pyx`` from the
@@ -481,6 +482,78 @@ def rng_fixture(reference: str):
     print({k: v[:3] for k, v in out.items()})
 
 
+NORM_DTYPES = ('int32', 'int64', 'uint32', 'float32', 'float32_frac', 'float64_frac')
+NORM_METHODS = ('logCP10k', 'log1pPF', 'PFlog1pPF')
+
+
+def normalize_inputs():
+    """Counts of the normalisation fixtures: a 500-cell x 200-gene workload
+    (5 types, 10% nonzeros) from the seeded generator, every count type the
+    reference's normalize accepts (fractional float counts are the integer
+    counts times 0.75, so the library sizes truncate), and a QC mask."""
+    w = synth.make_workload(17, 500, num_types=5, num_genes=200,
+                            target_density=0.10)
+    indptr, indices, counts = oracle.generate_counts(w)
+    qc = np.random.Generator(np.random.PCG64(17)).random(500) < 0.8
+    data = {'int32': counts.astype(np.int32), 'int64': counts.astype(np.int64),
+            'uint32': counts.astype(np.uint32), 'float32': counts.astype(np.float32),
+            'float32_frac': (counts * 0.75).astype(np.float32),
+            'float64_frac': counts.astype(np.float64) * 0.75}
+    return w, indptr.astype(np.int64), indices.astype(np.int32), data, qc
+
+
+def normalize_fixture(norm_mod):
+    """The reference's normalize_csr and normalize_csc outputs for every
+    method, count type and QC setting (SHA-256 of the normalised values and
+    of the row sums), with 8 threads and 1, and the full PFlog1pPF outputs
+    (CSR order)."""
+    import hashlib
+    import scipy.sparse as sp
+    w, indptr, indices, data, qc = normalize_inputs()
+    n, g = 500, 200
+    cases, full = [], {}
+    for dname in NORM_DTYPES:
+        X = sp.csr_array((data[dname], indices, indptr), shape=(n, g))
+        C = X.tocsc()
+        C.sort_indices()
+        for mi, method in enumerate(NORM_METHODS):
+            for use_qc in (False, True):
+                q = qc.astype(np.int8) if use_qc else np.empty(0, np.int8)
+                res = {}
+                for layout, M, fn in (('csr', X, norm_mod.normalize_csr),
+                                      ('csc', C, norm_mod.normalize_csc)):
+                    outs = []
+                    for threads in (8, 1):
+                        out = np.empty(len(M.data), np.float32)
+                        # zeroed, as pca() passes it (the parallel CSC
+                        # path adds into it)
+                        rs = np.zeros(n, np.uint64)
+                        fn(M.data, M.indices.astype(np.int32), M.indptr.astype(np.int32),
+                           q, out, rs, n, mi, threads)
+                        outs.append((out, rs))
+                    same_threads = bool(np.array_equal(outs[0][0].view(np.uint32),
+                                                       outs[1][0].view(np.uint32)))
+                    out, rs = outs[0]
+                    res[layout] = dict(
+                        values_sha256=hashlib.sha256(out.tobytes()).hexdigest(),
+                        row_sums_sha256=hashlib.sha256(rs.tobytes()).hexdigest(),
+                        thread_invariant=same_threads)
+                    if method == 'PFlog1pPF' and layout == 'csr':
+                        full[f'pf_{dname}_qc{int(use_qc)}'] = out
+                cases.append(dict(dtype=dname, method=method, qc=use_qc, **res))
+                print(dname, method, use_qc, res['csr']['thread_invariant'],
+                      res['csc']['thread_invariant'], flush=True)
+    np.savez_compressed(os.path.join(HERE, 'ref_normalize.npz'), **full)
+    path = os.path.join(HERE, 'ref_vectors.json')
+    with open(path) as f:
+        meta = json.load(f)
+    meta['normalize'] = dict(seed=17, num_cells=n, num_genes=g, num_types=5,
+                             density=0.10, qc_seed=17, qc_fraction=0.8,
+                             threads=[8, 1], cases=cases)
+    with open(path, 'w') as f:
+        json.dump(meta, f, indent=1)
+
+
 def calibrate_port(pca_mod, norm_mod, repeats=3, threads=8):
     """Time the reference's irlba and the oracle's C++ restatement of it
     (oracle.irlba_port) on the same 50k-cell input with the same thread
@@ -529,6 +602,8 @@ def main():
                     help='only add the start-vector draws (ref_start_vector.npz)')
     ap.add_argument('--calibrate', action='store_true',
                     help='only time the reference against the oracle port (50k cells)')
+    ap.add_argument('--normalize', action='store_true',
+                    help='only add the normalisation fixtures (ref_normalize.npz)')
     args = ap.parse_args()
     build_reference(args.reference)
     pca_mod, norm_mod = import_reference()
@@ -536,6 +611,8 @@ def main():
         return rng_fixture(args.reference)
     if args.calibrate:
         return calibrate_port(pca_mod, norm_mod)
+    if args.normalize:
+        return normalize_fixture(norm_mod)
     if args.cells1m:
         return cells1m(pca_mod, norm_mod)
     if args.knn:
