@@ -71,6 +71,13 @@ class SingleCell:
 
     def __init__(self, X=None, obs=None, var=None, obsm=None, varm=None,
                  uns=None, num_threads: int = 1):
+        if isinstance(X, (str, os.PathLike)):
+            # An .h5ad file (single_cell.py:83-1158): X, obs, var, obsm.
+            from .h5ad import read_h5ad
+            loaded = read_h5ad(os.fspath(X))
+            X, obs, var = loaded._X, obs if obs is not None else loaded._obs, \
+                var if var is not None else loaded._var
+            obsm = obsm if obsm is not None else loaded._obsm
         if X is not None and not isinstance(X, (sp.csr_array, sp.csc_array,
                                                 sp.csr_matrix, sp.csc_matrix)):
             raise TypeError('X must be a CSR or CSC sparse array')
@@ -124,6 +131,12 @@ class SingleCell:
             raise ValueError(f'num_threads is {num_threads:,}, but must be a '
                              f'positive integer, -1, or None')
         return num_threads
+
+    def save(self, path, overwrite: bool = False) -> None:
+        """Write X, obs, var and obsm to an .h5ad file
+        (``single_cell.py:3954``)."""
+        from .h5ad import write_h5ad
+        write_h5ad(self, os.fspath(path), overwrite=overwrite)
 
     def skip_qc(self) -> 'SingleCell':
         """The dataset with ``uns['QCed']`` set to True
