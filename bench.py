@@ -489,6 +489,77 @@ def bench_pipeline(args):
     print(json.dumps(line), flush=True)
 
 
+def bench_normalize(args):
+    """SingleCell.normalize() on the device: the workload's int32 counts
+    (generated on the device) normalised by each method, CSR with 90% of
+    cells passing QC, and log1pPF on the same counts in CSC.  Algorithmic
+    bytes per CSR launch: counts read twice (row sums, then the values) and
+    the float32 values written (12 z), the offsets read twice, the row sums
+    written; PFlog1pPF also reads and writes the values once more (8 z).
+    CPU baseline: the oracle's log1pPF (the reference's loop in C, one core)
+    on a 1M-row sample, scaled by nonzeros."""
+    import torch
+    from brisc_amd import Context, synth
+    from brisc_amd.normalize import normalize_device
+    from brisc_amd.subset import csr_transpose
+
+    spec = synth.WORKLOADS[args.workload]
+    w = synth.make_workload(**spec)
+    ctx = Context(0)
+    _, indices, indptr, counts = synth.generate_device(w, context=ctx, keep_counts=True)
+    n, g, nnz = w.num_cells, w.num_genes, int(counts.numel())
+    qc = torch.from_numpy(np.random.default_rng(7).random(n) < 0.9).cuda()
+    out = torch.empty(nnz, dtype=torch.float32, device='cuda')
+    results = {}
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize()
+        best = None
+        for _ in range(max(args.steps, 1)):
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            dt = time.perf_counter() - t0
+            best = dt if best is None else min(best, dt)
+        return best
+
+    for method in ('logCP10k', 'log1pPF', 'PFlog1pPF'):
+        sec = timed(lambda: normalize_device(counts, indices, indptr, n, g, QC=qc,
+                                             method=method, out=out, context=ctx))
+        alg = 12 * nnz + 16 * (n + 1) + 8 * n + (8 * nnz + 8 * n if method == 'PFlog1pPF' else 0)
+        results[f'csr_{method}'] = {'ms': round(sec * 1e3, 3),
+                                    'gbs': round(alg / sec / 1e9, 1)}
+    # The CSC layout of the same counts (column offsets over genes), the
+    # counts held as float32 (a count type the reference accepts).
+    t_data, t_idx, t_ptr = csr_transpose(counts.float(), indices, indptr, g, context=ctx)
+    sec = timed(lambda: normalize_device(t_data, t_idx, t_ptr, n, g, is_csr=False, QC=qc,
+                                         method='log1pPF', out=out, context=ctx))
+    results['csc_log1pPF'] = {'ms': round(sec * 1e3, 3)}
+    del t_data, t_idx, t_ptr
+    import oracle
+    sample = min(1_000_000, n)
+    e = int(indptr[sample].item())
+    c_s = counts[:e].cpu().numpy()
+    p_s = indptr[:sample + 1].cpu().numpy()
+    t0 = time.perf_counter()
+    oracle.log1ppf(c_s, p_s)
+    cpu_s = time.perf_counter() - t0
+    line = {
+        'metric': 'device SingleCell.normalize() of the counts (log1pPF, CSR)',
+        'value': results['csr_log1pPF']['ms'], 'unit': 'ms', 'n_gpus': 1,
+        'steps': args.steps, 'higher_is_better': False, 'dtype': 'i32 counts -> f32',
+        'data': 'synthetic',
+        'config': {'workload': args.workload, 'num_cells': n, 'num_genes': g, 'nnz': nnz,
+                   'qc_fraction': 0.9},
+        'methods': results,
+        'cpu_baseline': {'value': round(cpu_s * nnz / max(e, 1) * 1e3, 1),
+                         'unit': 'ms (scaled by nonzeros)', 'cores': 1, 'kind': 'port',
+                         'sample': f'oracle log1pPF on the first {sample} rows, {cpu_s:.2f} s'},
+    }
+    print(json.dumps(line), flush=True)
+
+
 def bench_copy(args):
     """A plain device copy of known size (calibrates HBM byte counters and
     gives the achievable copy rate)."""
@@ -516,7 +587,8 @@ def main():
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
-    ap.add_argument('--mode', choices=('single', 'batched', 'serve', 'copy', 'subset', 'knn', 'pipeline'),
+    ap.add_argument('--mode', choices=('single', 'batched', 'serve', 'copy', 'subset', 'knn',
+                                       'pipeline', 'normalize'),
                     default='single')
     ap.add_argument('--workload', default='cells10m')
     ap.add_argument('--k', type=int, default=50)
@@ -553,6 +625,8 @@ def main():
         return bench_subset(args)
     if args.mode == 'knn':
         return bench_knn(args)
+    if args.mode == 'normalize':
+        return bench_normalize(args)
     if args.mode == 'pipeline':
         if args.workload == 'cells10m' and '--workload' not in sys.argv:
             args.workload = 'cells1m'
