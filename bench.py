@@ -609,6 +609,8 @@ def main():
     ap.add_argument('--knn-cells', type=int, default=1_000_000)
     ap.add_argument('--knn-k', type=int, default=20)
     ap.add_argument('--knn-search', choices=('exact', 'ivf'), default='exact')
+    ap.add_argument('--xtu', choices=('sorted', 'csr'), default='sorted',
+                    help='X^T u from the gene-sorted copy of X (default) or the CSR')
     args = ap.parse_args()
     if args.gpus != 1:
         print(f'bench.py runs on one GPU only (asked for {args.gpus}); the '
@@ -638,6 +640,7 @@ def main():
     spec = synth.WORKLOADS[args.workload]
     w = synth.make_workload(**spec)
     ctx = Context(0)
+    ctx.set_sorted_xtu(args.xtu == 'sorted')
     t0 = time.perf_counter()
     data, indices, indptr = synth.generate_device(w, context=ctx)
     gen_s = time.perf_counter() - t0
@@ -684,7 +687,9 @@ def main():
                       'gbs': round(bytes_per / (avg * 1e-3) / 1e9, 1)}
     dominant = max(('spmv', 'spmv_t'), key=lambda k_: ktimes[k_][0])
     achieved = kern[dominant]['gbs']
-    traffic = load_traffic(args.workload, dominant)
+    # X^T u reads the gene-sorted copy (5 bytes a nonzero) for g <= 4096.
+    sorted_xtu = args.xtu == 'sorted' and g <= 4096
+    traffic = load_traffic(args.workload, dominant + ('_sorted' if dominant == 'spmv_t' and sorted_xtu else ''))
     roofline = {'bound': 'hbm', 'achieved': achieved, 'peak': HBM_PEAK_GBS,
                 'unit': 'GB/s', 'frac': round(achieved / HBM_PEAK_GBS, 4),
                 'traffic': traffic,
@@ -727,6 +732,7 @@ def main():
                   'top_singular_values': [round(float(x), 4) for x in sv[:5]]},
         'spmv_pair_gbs': round(bytes_per * spmv_launches / (spmv_ms * 1e-3) / 1e9, 1) if spmv_ms else None,
         'kernels': kern, 'breakdown': breakdown,
+        'xtu_layout': 'gene-sorted copy' if sorted_xtu else 'csr',
         'generate_seconds': round(gen_s, 3),
         'autotune': tuned,
     }

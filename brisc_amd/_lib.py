@@ -142,6 +142,8 @@ def lib() -> ctypes.CDLL:
         'bpca_context_get_kernel_config': (i32, [p, ctypes.POINTER(ctypes.c_int32),
                                                  ctypes.POINTER(ctypes.c_int32)]),
         'bpca_context_set_occupancy': (i32, [p, i32, i32]),
+        'bpca_context_set_sorted_xtu': (i32, [p, i32]),
+        'bpca_context_get_sorted_xtu': (i32, [p, ctypes.POINTER(i32)]),
         'bpca_context_get_occupancy': (i32, [p, ctypes.POINTER(ctypes.c_int32),
                                              ctypes.POINTER(ctypes.c_int32)]),
         'bpca_autotune': (i32, [p, ctypes.POINTER(Problem), i32, p]),

@@ -66,6 +66,10 @@ constexpr int kGroup = 64;        // data blocks per first-level group
 constexpr int kGeneTile = 8192;   // genes per X^T u / column-statistics launch (LDS accumulators)
 constexpr int kMaxLdsV = 32768;   // X v stages v in LDS up to this many genes
 constexpr int kMaxPacked16 = 65535;  // column indices packed to 16 bits up to this g
+constexpr int kMaxSortedGenes = 4096;  // X^T u from the gene-sorted copy up to this g
+constexpr int kTChunk = 16;         // entries per lane in the gene-sorted X^T u
+constexpr int kTWindow = kWave * kTChunk;  // entries per wave step of that kernel
+constexpr int kSortedBit = 1 << 3;  // Tables::tmodes: some problem uses the gene-sorted X^T u
 
 // Index modes of a problem (Prob::mode): how its sparse products load.
 enum IndexMode : int32_t {
@@ -94,6 +98,13 @@ struct Prob {
     const float *data;       // packed copy of the values (see above)
     const void *idx;         // packed column indices: uint16 (mode < kModeWide) or uint32
     const int64_t *indptr;
+    // Gene-sorted copy of X for X^T u (tsorted: g <= kMaxSortedGenes), made
+    // at admission.  Within each 256-row sub-block the entries are ordered by
+    // (gene, row), at the sub-block's own CSR positions:
+    const float *tval;       // values
+    const uint8_t *trow;     // row within the sub-block, stored as tsort_row_slot(row)
+    const uint16_t *thb;     // run starts: bit j of word q marks position 16 q + j
+    const uint16_t *tlens;   // entries per (sub-block, gene), row stride tgl
     float *PCs;  // n x k float32 row-major
     int64_t n, nb, ngrp;
     int32_t g, k, m, maxit;
@@ -115,6 +126,7 @@ struct Prob {
     int32_t phase, j, kc, iter;
     int32_t ubuf, vbuf, converged, nconv;
     int32_t svd_sweeps, pad0;
+    int32_t tsorted, tgl;    // gene-sorted X^T u in use; tlens row stride (g rounded up to 8)
     double alpha, inv_alpha, rnorm, smax;
     double umean;         // mean left in U[:, j] by the reorthogonalisation
     int64_t steps;
@@ -129,6 +141,7 @@ struct Tables {
     int64_t num_groups;
     int max_g, max_m;
     int modes;            // bit set of the IndexModes of the problems
+    int tmodes;           // X^T u: IndexModes of the CSR-path problems, kSortedBit
     int64_t max_lds;      // dynamic LDS bytes a workgroup may use
     int spmv_group;       // segments per load group in X v   (2, 4 or 8)
     int spmvt_group;      // segments per load group in X^T u (2, 4 or 8)
@@ -173,6 +186,16 @@ hipError_t launch_validate_pack(const float *data, const int32_t *indices,
                                 const void *indptr, int indptr_width, int64_t n,
                                 int64_t nnz, int g, float *data4, void *idx,
                                 int *flag, hipStream_t s);
+// The gene-sorted copy of one problem's packed CSR (g <= kMaxSortedGenes,
+// 16-bit indices, int64 row offsets): per 256-row sub-block the entry counts
+// of each gene (tlens, row stride gl), then its entries ordered by (gene,
+// row) at the sub-block's CSR positions (tval, trow) and the run-start bits
+// (thb, cleared here first; thb_words 16-bit words).  tval and trow hold at
+// least nnz + 16 entries.  max_lds: the workgroup LDS limit of the device.
+hipError_t launch_tsort_build(const float *data4, const uint16_t *idx16, const int64_t *indptr,
+                              int64_t n, int g, int gl, float *tval, uint8_t *trow,
+                              uint16_t *thb, int64_t thb_words, uint16_t *tlens,
+                              int64_t max_lds, hipStream_t s);
 // indptr (int32) -> int64 and int64 indices -> int32 (flagging range errors).
 hipError_t launch_widen_indptr(const int32_t *in, int64_t count, int64_t *out,
                                hipStream_t s);

@@ -701,7 +701,7 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T, int c0) {
     extern __shared__ float lds[];
     const int2 bp = T.blocks[blockIdx.x];
     Prob &pr = T.probs[bp.x];
-    if (pr.phase != PH_LANCZOS || (pr.mode == kModeWide) != (sizeof(IX) == 4)) return;
+    if (pr.phase != PH_LANCZOS || pr.tsorted || (pr.mode == kModeWide) != (sizeof(IX) == 4)) return;
     const int g = pr.g;
     if (c0 >= g) return;
     const int tw = g - c0 < kGeneTile ? g - c0 : kGeneTile;
@@ -749,6 +749,491 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T, int c0) {
     float *out = pr.tpart + (int64_t)bp.y * 2 * g + c0;
     for (int i = threadIdx.x; i < tw; i += 256)
         out[i] = (lds[i] + lds[gs + i]) + (lds[2 * gs + i] + lds[3 * gs + i]);
+}
+
+// ================================================== X^T u, gene-sorted copy
+//
+// For g <= kMaxSortedGenes X^T u reads a second copy of X made at admission:
+// inside each 256-row sub-block the entries are ordered by (gene, row), so a
+// gene's entries in the sub-block form one run, and its partial is a chain
+// over that run in row order.  A wave streams its sub-block in windows of
+// kTWindow entries, kTChunk consecutive entries per lane:
+//   * per lane, a chain over its entries that restarts at each run start
+//     (a[i] = x[i] u[row[i]] + a[i-1], or without the a[i-1] at a run start);
+//   * an inclusive segmented scan over the lanes (six DPP steps, a segment
+//     starting at each lane that holds a run start) of each lane's last
+//     piece, which gives every lane the part of its first run that lies in
+//     the lanes before it (and, through `carry`, in earlier windows);
+//   * one lane per gene that ends in the window stores its total, once,
+//     into the LDS slot of its run offset (no later window reads it).
+// A gene's sub-block partial is therefore
+//   (earlier windows' carry + the lanes' scan tree) + the lane's own chain,
+// a fixed function of the entries' positions: independent of the launch.
+// Sub-blocks combine as (w0 + w1) + (w2 + w3), as in the CSR path.  Each
+// element costs one LDS read (u[row]) instead of the CSR path's read and
+// write of a scattered accumulator, every lane of a window holds entries
+// (a row-windowed CSR leaves lanes idle past each row's end), and a row
+// index is one byte, so the copy streams 5 bytes an entry against 6.
+
+namespace {
+
+// LDS slot of sub-block row r in the u table: rows are stored as this slot,
+// so that a run over consecutive rows, read kTChunk entries a lane, lands on
+// distinct banks.
+__host__ __device__ __forceinline__ int tsort_row_slot(int r) { return ((r & 15) << 4) | (r >> 4); }
+
+template <int CTRL, int RM>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(0, __builtin_bit_cast(int, v), CTRL, RM, 0xF, false));
+}
+
+// Inclusive segmented scan: lane l gets the sum of v over lanes [l - dl, l]
+// (dl = distance to the first lane of its segment).  Every DPP move runs
+// with all lanes active (a DPP read of an inactive lane returns 0), so each
+// step adds its neighbour's value times 1 or 0 instead of selecting.
+__device__ __forceinline__ float seg_scan(float v, int lane, int dl) {
+    const int r = lane & 15;
+    v = fmaf(dpp_move<0x111, 0xF>(v), dl >= 1 ? 1.f : 0.f, v);   // row_shr:1
+    v = fmaf(dpp_move<0x112, 0xF>(v), dl >= 2 ? 1.f : 0.f, v);   // row_shr:2
+    v = fmaf(dpp_move<0x114, 0xF>(v), dl >= 4 ? 1.f : 0.f, v);   // row_shr:4
+    v = fmaf(dpp_move<0x118, 0xF>(v), dl >= 8 ? 1.f : 0.f, v);   // row_shr:8
+    v = fmaf(dpp_move<0x142, 0xA>(v), ((lane & 16) && dl > r) ? 1.f : 0.f, v);          // row_bcast:15
+    v = fmaf(dpp_move<0x143, 0xC>(v), ((lane & 32) && dl > lane - 32) ? 1.f : 0.f, v);  // row_bcast:31
+    return v;
+}
+
+typedef const uint16_t __attribute__((address_space(1))) gu16_r;
+typedef const u32x4 __attribute__((address_space(1))) gu32x4_r;
+
+struct TChunk {
+    f32x4 x[kTChunk / 4];
+    u32x4 r;        // 16 row bytes
+    unsigned h;     // 16 run-start bits
+};
+
+// Loads R lane chunks (window P + kTWindow k); chunk indices are clamped to
+// the sub-block's last chunk, so no load sits behind a branch.
+template <int R>
+__device__ __forceinline__ void tload(gquad *V4, gu32x4_r *R16, gu16_r *H, int P, int clast, int lane,
+                                      TChunk (&G)[R]) {
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        int c = P / kTChunk + kWave * k + lane;
+        c = c < clast ? c : clast;
+#pragma unroll
+        for (int j = 0; j < kTChunk / 4; j++) G[k].x[j] = V4[c * (kTChunk / 4) + j];
+        G[k].r = R16[c];
+        G[k].h = H[c];
+    }
+}
+
+struct TState {
+    int gcur;     // first gene whose run has not ended
+    float carry;  // partial of the run open at the end of the last window
+};
+
+__device__ __forceinline__ unsigned row_word(const u32x4 &r, int k) {
+    return k == 0 ? r.x : k == 1 ? r.y : k == 2 ? r.z : r.w;
+}
+
+// One window: positions [Pk, Pk + kTWindow) relative to the sub-block's
+// chunk origin; valid entries are [S, E).
+__device__ __forceinline__ void tsort_window(const TChunk &C, int Pk, int S, int E, int g, int lane,
+                                             int *accoff, const float *uu, float *tot, float *sinv,
+                                             uint8_t *fhv, TState &st) {
+    const int bit0 = kTChunk * lane;
+    float ui[kTChunk], xs[kTChunk];
+#pragma unroll
+    for (int i = 0; i < kTChunk; i++) {
+        ui[i] = uu[(row_word(C.r, i >> 2) >> (8 * (i & 3))) & 255u];
+        xs[i] = C.x[i >> 2][i & 3];
+    }
+    unsigned heads = C.h;
+    if (Pk < S || Pk + kTWindow > E) {
+        __asm__ volatile("" ::: "memory");  // keep this a branch: only a sub-block's first and last windows mask
+#pragma unroll
+        for (int i = 0; i < kTChunk; i++) {
+            const int pos = bit0 + Pk + i;
+            xs[i] = (pos >= S && pos < E) ? xs[i] : 0.f;
+        }
+        // Run starts of the neighbouring sub-blocks' entries are dropped; the
+        // sub-block's first entry starts a run.
+        const int lo = S - Pk - bit0, hi = E - Pk - bit0;
+        unsigned vm = 0xffffffffu;
+        vm = lo > 0 ? (lo >= 32 ? 0u : vm << lo) : vm;
+        vm = hi < kTChunk ? (hi <= 0 ? 0u : vm & ((1u << hi) - 1u)) : vm;
+        heads &= vm;
+        if (lo >= 0 && lo < kTChunk) heads |= 1u << lo;
+    }
+    const unsigned keep = ~heads;
+    float a = 0.f, as[kTChunk];
+#pragma unroll
+    for (int i = 0; i < kTChunk; i++) {
+        const int m = ((int)(keep << (31 - i))) >> 31;  // 0 at a run start, else all ones
+        a = __builtin_bit_cast(float, __builtin_bit_cast(int, a) & m);
+        a = fmaf(xs[i], ui[i], a);
+        as[i] = a;
+    }
+    heads &= (1u << kTChunk) - 1u;
+    const unsigned long long hmask = __ballot(heads != 0);
+    const unsigned long long below = hmask & ((2ull << lane) - 1ull);
+    const int dl = below ? lane - (63 - __builtin_clzll(below)) : lane;
+    const float Sinc = seg_scan(a, lane, dl);
+    const float Sprev = dpp_move<0x138, 0xF>(Sinc);  // wave_shr:1
+    const unsigned long long before = hmask & ((1ull << lane) - 1ull);
+    const float Sin = lane == 0 ? st.carry : (before ? Sprev : st.carry + Sprev);
+#pragma unroll
+    for (int j = 0; j < kTChunk / 4; j++)
+        *(f32x4 *)(tot + kTChunk * lane + 4 * j) = f32x4{as[4 * j], as[4 * j + 1], as[4 * j + 2], as[4 * j + 3]};
+    sinv[lane] = Sin;
+    fhv[lane] = heads ? (uint8_t)__builtin_ctz(heads) : (uint8_t)kTChunk;
+    const float S63 = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, Sinc), 63));
+    st.carry = hmask ? S63 : st.carry + S63;
+    __builtin_amdgcn_wave_barrier();
+    // One lane per gene that ends in this window (empty genes store 0).
+    for (;;) {
+        const int c = st.gcur + lane;
+        const int s0 = c < g ? accoff[c] : E;
+        const int e0 = c < g ? accoff[c + 1] : E;
+        const bool done = c < g && e0 <= Pk + kTWindow;
+        float t = 0.f;
+        if (done && s0 < e0) {
+            const int es = e0 - 1 - Pk;
+            const int lo = es / kTChunk, i = es % kTChunk;
+            t = tot[es];
+            if (i < fhv[lo]) t = sinv[lo] + t;
+        }
+        const unsigned long long dm = __ballot(done);
+        __builtin_amdgcn_wave_barrier();
+        if (done) accoff[c] = __builtin_bit_cast(int, t);
+        __builtin_amdgcn_wave_barrier();
+        st.gcur += __builtin_popcountll(dm);
+        if (!((dm >> 63) & 1ull)) break;
+    }
+}
+
+}  // namespace
+
+// LDS of k_spmv_t_sorted, per wave (16-byte aligned parts): g + 1 offsets /
+// totals, 256 u values, kTWindow window totals, 64 lane carries and 64
+// first-run-start bytes.
+__host__ __device__ inline int tsort_lds_floats(int g) { return ((g + 4) & ~3) + 256 + kTWindow + kWave + kWave / 4; }
+
+size_t spmv_t_sorted_lds_bytes(int g) { return sizeof(float) * 4 * (size_t)tsort_lds_floats(g); }
+
+// One workgroup per data block, wave w on sub-block w; u = (U[:, j] - mean)
+// / alpha written back as in k_spmv_t.  Problems without the sorted copy
+// are left to k_spmv_t.
+template <int R>
+__global__ __launch_bounds__(256) void k_spmv_t_sorted(Tables T) {
+    extern __shared__ float lds[];
+    const int2 bp = T.blocks[blockIdx.x];
+    Prob &pr = T.probs[bp.x];
+    if (pr.phase != PH_LANCZOS || !pr.tsorted) return;
+    const int g = pr.g;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_id();
+    const int per = tsort_lds_floats(g);
+    int *accoff = (int *)(lds + (size_t)w * per);
+    float *uu = lds + (size_t)w * per + ((g + 4) & ~3);
+    float *tot = uu + 256;
+    float *sinv = tot + kTWindow;
+    uint8_t *fhv = (uint8_t *)(sinv + kWave);
+    int64_t r0, nrows64;
+    sub_block(pr.n, bp.y, w, r0, nrows64);
+    const int nrows = (int)nrows64;
+    double *Uj = pr.U[pr.ubuf] + (int64_t)pr.j * pr.n;
+    const double inv_alpha = pr.inv_alpha, umean = pr.umean;
+    for (int t = lane; t < kSubRows; t += kWave) {
+        float x = 0.f;
+        if (t < nrows) {
+            const double d = (Uj[r0 + t] - umean) * inv_alpha;
+            Uj[r0 + t] = d;
+            x = (float)d;
+        }
+        uu[tsort_row_slot(t)] = x;
+    }
+    int64_t s0 = 0, e0 = 0;
+    if (nrows > 0) {
+        s0 = pr.indptr[r0];
+        e0 = pr.indptr[r0 + nrows];
+    }
+    if (e0 > s0) {
+        const int64_t org = s0 / kTChunk * kTChunk;
+        const int S = (int)(s0 - org), E = (int)(e0 - org);
+        // Run offsets: accoff[c] = S + entries of genes < c, 8 genes a lane.
+        const int64_t sb = r0 / kSubRows;
+        const uint4 *L8 = (const uint4 *)(pr.tlens + sb * pr.tgl);
+        int base = S;
+        for (int cb = 0; cb < g; cb += 8 * kWave) {
+            const int c = cb + 8 * lane;
+            const uint4 q = c < g ? L8[c / 8] : uint4{0, 0, 0, 0};
+            const int v[8] = {(int)(q.x & 0xffff), (int)(q.x >> 16), (int)(q.y & 0xffff), (int)(q.y >> 16),
+                              (int)(q.z & 0xffff), (int)(q.z >> 16), (int)(q.w & 0xffff), (int)(q.w >> 16)};
+            int sum = 0;
+#pragma unroll
+            for (int k = 0; k < 8; k++) sum += v[k];
+            int incl = sum;
+#pragma unroll
+            for (int d = 1; d < kWave; d <<= 1) {
+                const int t = __shfl_up(incl, d);
+                incl = lane >= d ? incl + t : incl;
+            }
+            int run = base + incl - sum;
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                if (c + k < g) accoff[c + k] = run;
+                run += v[k];
+            }
+            base += __builtin_amdgcn_readlane(incl, kWave - 1);
+        }
+        if (lane == 0) accoff[g] = base;
+        __builtin_amdgcn_wave_barrier();
+        gquad *V4 = (gquad *)(pr.tval + org);
+        gu32x4_r *R16 = (gu32x4_r *)(pr.trow + org);
+        gu16_r *H = (gu16_r *)(pr.thb + org / kTChunk);
+        const int clast = (E - 1) / kTChunk;
+        TState st{0, 0.f};
+        TChunk ga[R], gb[R];
+        int P = 0;
+        tload<R>(V4, R16, H, P, clast, lane, ga);
+        for (;;) {
+            tload<R>(V4, R16, H, P + kTWindow * R, clast, lane, gb);
+            __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+            for (int k = 0; k < R; k++)
+                tsort_window(ga[k], P + kTWindow * k, S, E, g, lane, accoff, uu, tot, sinv, fhv, st);
+            P += kTWindow * R;
+            const bool last = P >= E;
+            tload<R>(V4, R16, H, P + kTWindow * R, clast, lane, ga);
+            __builtin_amdgcn_sched_barrier(0);
+            if (last) break;
+#pragma unroll
+            for (int k = 0; k < R; k++)
+                tsort_window(gb[k], P + kTWindow * k, S, E, g, lane, accoff, uu, tot, sinv, fhv, st);
+            P += kTWindow * R;
+            if (P >= E) break;
+        }
+    } else {
+        for (int i = lane; i < g; i += kWave) accoff[i] = 0;
+    }
+    __syncthreads();
+    float *out = pr.tpart + (int64_t)bp.y * 2 * g;
+    for (int i = threadIdx.x; i < g; i += 256)
+        out[i] = (lds[i] + lds[per + i]) + (lds[2 * per + i] + lds[3 * per + i]);
+}
+
+// ------------------------------------------------- building the sorted copy
+
+// The gene-sorted copy of one 256-row sub-block, built in LDS by one
+// workgroup of kBuildWaves waves (wave w on rows kBuildRows w and on):
+//   1. each wave counts its rows' entries per gene (LDS integer atomics, an
+//      order-free count), the sub-block's counts go to tlens;
+//   2. start[c] = the sub-block's exclusive prefix of the counts, and each
+//      wave's counter of gene c becomes start[c] + the counts of the waves
+//      before it: the position of its first entry of c;
+//   3. the genes are placed in passes [ga, gb) whose entries fit the LDS
+//      stage: every wave walks its rows in order and puts each entry of a
+//      gene in the pass at its counter (a row's genes are distinct, so the
+//      lanes of one update touch distinct counters), then the whole
+//      workgroup copies the stage out with consecutive stores and marks
+//      the pass's run starts in thb (cleared beforehand) by atomic OR.
+// Entries of one gene therefore follow in row order, as X^T u requires.
+constexpr int kBuildWaves = 8;
+constexpr int kBuildRows = kSubRows / kBuildWaves;
+constexpr int kBuildThreads = kBuildWaves * kWave;
+
+__global__ __launch_bounds__(kBuildThreads) void k_tsort_build(const float *__restrict__ data4,
+                                                     const uint16_t *__restrict__ idx,
+                                                     const int64_t *__restrict__ indptr, int64_t n,
+                                                     int g, int gl, int cap,
+                                                     float *__restrict__ tval, uint8_t *__restrict__ trow,
+                                                     unsigned *__restrict__ thb32,
+                                                     uint16_t *__restrict__ tlens) {
+    extern __shared__ unsigned b_lds[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_id();
+    unsigned *cntw = b_lds;                       // [kBuildWaves][gl] counts, then counters
+    unsigned *start = cntw + kBuildWaves * gl;    // [gl + 1]
+    unsigned *wtot = start + gl + 1;              // [kBuildWaves] wave totals of the prefix
+    int *roff = (int *)(wtot + kBuildWaves);      // [kSubRows + 1]
+    float *stage_v = (float *)(roff + kSubRows + 1);  // [cap]
+    uint8_t *stage_r = (uint8_t *)(stage_v + cap);    // [cap]
+    const int64_t sb = blockIdx.x;
+    const int64_t r0 = sb * kSubRows;
+    const int nr = (int)(n - r0 < kSubRows ? n - r0 : kSubRows);
+    const int64_t s0 = indptr[r0];
+    for (int t = threadIdx.x; t <= nr; t += kBuildThreads) roff[t] = (int)(indptr[r0 + t] - s0);
+    for (int i = threadIdx.x; i < kBuildWaves * gl; i += kBuildThreads) cntw[i] = 0u;
+    __syncthreads();
+    const int E = roff[nr];
+    const int wr0 = kBuildRows * w < nr ? kBuildRows * w : nr;
+    const int wr1 = kBuildRows * (w + 1) < nr ? kBuildRows * (w + 1) : nr;
+    const int wa = roff[wr0], wb = roff[wr1];  // this wave's entries
+    unsigned *mycnt = cntw + w * gl;
+    for (int p = wa + lane; p < wb; p += 16 * kWave) {
+        unsigned c[16];
+#pragma unroll
+        for (int k = 0; k < 16; k++) c[k] = p + k * kWave < wb ? (unsigned)idx[s0 + p + k * kWave] : 0xffffffffu;
+#pragma unroll
+        for (int k = 0; k < 16; k++)
+            if (c[k] < (unsigned)g) atomicAdd(&mycnt[c[k]], 1u);
+    }
+    __syncthreads();
+    // Sub-block counts, their exclusive prefix, per-wave first positions.
+    unsigned base = 0;
+    for (int cb = 0; cb < gl; cb += kBuildThreads) {
+        const int c = cb + threadIdx.x;
+        unsigned tot = 0;
+        if (c < gl) {
+#pragma unroll
+            for (int v = 0; v < kBuildWaves; v++) tot += cntw[v * gl + c];
+            tlens[sb * gl + c] = (uint16_t)tot;
+        }
+        // Block-wide exclusive scan of tot (wave scans, then
+        // the wave totals).
+        unsigned incl = tot;
+#pragma unroll
+        for (int d = 1; d < kWave; d <<= 1) {
+            const unsigned t = __shfl_up(incl, d);
+            incl = lane >= d ? incl + t : incl;
+        }
+        __syncthreads();
+        if (lane == kWave - 1) wtot[w] = incl;
+        __syncthreads();
+        unsigned wbase = base, blk = 0;
+        for (int v = 0; v < kBuildWaves; v++) {
+            wbase += v < w ? wtot[v] : 0u;
+            blk += wtot[v];
+        }
+        __syncthreads();
+        if (c < gl) {
+            const unsigned st = wbase + incl - tot;
+            start[c] = st;
+            unsigned run = st;
+#pragma unroll
+            for (int v = 0; v < kBuildWaves; v++) {
+                const unsigned a = cntw[v * gl + c];
+                cntw[v * gl + c] = run;
+                run += a;
+            }
+        }
+        base += blk;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) start[gl] = base;
+    __syncthreads();
+    if (E == 0) return;
+    // Gene passes whose entries fit the stage.
+    int ga = 0;
+    while (ga < g) {
+        // gb: the largest gene boundary with start[gb] - start[ga] <= cap
+        // (at least one gene; a gene never exceeds 256 entries <= cap).
+        int lo = ga + 1, hi = g;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if ((int)(start[mid] - start[ga]) <= cap) lo = mid; else hi = mid - 1;
+        }
+        const int gb = lo;
+        const int pa = (int)start[ga], pb = (int)start[gb];
+        // Place this wave's entries of genes [ga, gb), rows in order, in
+        // batches of kB x 64 entries (lane l: positions b + l + 64 k) whose
+        // loads are issued two batches ahead (three rotating buffers).
+        if (wb > wa) {
+            constexpr int kB = 8, kStep = kB * kWave;
+            struct Batch {
+                unsigned c[kB];
+                float x[kB];
+            };
+            auto load_batch = [&](int b, Batch &B) {
+#pragma unroll
+                for (int k = 0; k < kB; k++) {
+                    int p = b + kWave * k + lane;
+                    p = p < wb ? p : wb - 1;
+                    B.c[k] = idx[s0 + p];
+                    B.x[k] = data4[s0 + p];
+                }
+            };
+            int r = wr0;
+            auto place = [&](int b, const Batch &bt, Batch &nxt) {
+                const int bn = b + 2 * kStep;
+                load_batch(bn < wb ? bn : b, nxt);
+#pragma unroll
+                for (int k = 0; k < kB; k++) {
+                    const int b0 = b + kWave * k;
+                    if (b0 >= wb) break;
+                    const int p = b0 + lane;
+                    const unsigned c = bt.c[k];
+                    const bool mine = p < wb && c >= (unsigned)ga && c < (unsigned)gb;
+                    const int last = (wb - b0 < kWave ? wb - b0 : kWave) - 1 + b0;
+                    while (roff[r + 1] <= b0) r++;
+                    for (int rr = r;; rr++) {
+                        const int ra = roff[rr], rb = roff[rr + 1];
+                        if (mine && p >= ra && p < rb) {
+                            const unsigned pos = mycnt[c];
+                            mycnt[c] = pos + 1u;
+                            stage_v[pos - pa] = bt.x[k];
+                            stage_r[pos - pa] = (uint8_t)tsort_row_slot(rr);
+                        }
+                        __builtin_amdgcn_wave_barrier();
+                        if (rb > last) break;
+                    }
+                }
+            };
+            Batch B0, B1, B2;
+            load_batch(wa, B0);
+            load_batch(wa + kStep < wb ? wa + kStep : wa, B1);
+            for (int b = wa;;) {
+                place(b, B0, B2);
+                if ((b += kStep) >= wb) break;
+                place(b, B1, B0);
+                if ((b += kStep) >= wb) break;
+                place(b, B2, B1);
+                if ((b += kStep) >= wb) break;
+            }
+        }
+        __syncthreads();
+        for (int i = threadIdx.x; i < pb - pa; i += kBuildThreads) {
+            tval[s0 + pa + i] = stage_v[i];
+            trow[s0 + pa + i] = stage_r[i];
+        }
+        for (int c = ga + threadIdx.x; c < gb; c += kBuildThreads) {
+            if (start[c + 1] > start[c]) {
+                const int64_t at = s0 + start[c];
+                atomicOr(thb32 + (at >> 5), 1u << (at & 31));
+            }
+        }
+        __syncthreads();
+        ga = gb;
+    }
+}
+
+// LDS of k_tsort_build for gl counter columns and a stage of cap entries.
+static size_t tsort_build_lds(int gl, int cap) {
+    return sizeof(unsigned) * (kBuildWaves * (size_t)gl + gl + 1 + kBuildWaves + kSubRows + 1) + 5 * (size_t)cap;
+}
+
+__global__ __launch_bounds__(256) void k_zero_words(unsigned *__restrict__ p, int64_t words) {
+    const int64_t stride = (int64_t)gridDim.x * 256;
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < words; i += stride) p[i] = 0u;
+}
+
+hipError_t launch_tsort_build(const float *data4, const uint16_t *idx16, const int64_t *indptr,
+                              int64_t n, int g, int gl, float *tval, uint8_t *trow,
+                              uint16_t *thb, int64_t thb_words, uint16_t *tlens,
+                              int64_t max_lds, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    const int64_t words32 = (thb_words + 1) / 2;
+    k_zero_words<<<(unsigned)std::min<int64_t>(ceil_div(words32, 256), 8192), 256, 0, s>>>(
+        (unsigned *)thb, words32);
+    // The largest stage that fits one workgroup per compute unit.
+    const size_t meta = tsort_build_lds(gl, 0);
+    const int64_t budget = std::min<int64_t>(max_lds, 160 * 1024);
+    int cap = (int)((budget - (int64_t)meta) / 5) & ~3;
+    if (cap < kSubRows) return hipErrorInvalidValue;
+    k_tsort_build<<<(unsigned)ceil_div(n, kSubRows), kBuildThreads, tsort_build_lds(gl, cap), s>>>(
+        data4, idx16, indptr, n, g, gl, cap, tval, trow, (unsigned *)thb, tlens);
+    return hipGetLastError();
 }
 
 // ================================================================ V side
@@ -1504,6 +1989,7 @@ static size_t capped_lds(size_t need, int wg, const Tables &T) {
 
 size_t lds_bytes_needed(int g, int m) {
     size_t need = spmv_t_lds_bytes(g);
+    if (g <= kMaxSortedGenes) need = std::max(need, spmv_t_sorted_lds_bytes(g));
     need = std::max(need, spmv_lds_bytes(g));
     need = std::max(need, sizeof(float) * 4 * (size_t)std::min(g, kGeneTile));  // colstats
     need = std::max(need, sizeof(double) * (size_t)m * kGemmCols);   // gemm
@@ -1524,8 +2010,8 @@ void launch_spmv_modes(const Tables &T, size_t lds_ldsv, size_t lds_global, hipS
 template <int R>
 void launch_spmv_t_tiles(const Tables &T, size_t lds, hipStream_t s) {
     const unsigned nblk = (unsigned)T.num_blocks;
-    const bool narrow = (T.modes & ((1 << kModeLds) | (1 << kModeGlobalV))) != 0;
-    const bool wide = (T.modes & (1 << kModeWide)) != 0;
+    const bool narrow = (T.tmodes & ((1 << kModeLds) | (1 << kModeGlobalV))) != 0;
+    const bool wide = (T.tmodes & (1 << kModeWide)) != 0;
     if (T.max_g <= kGeneTile) {   // one tile: no range test per element
         if (narrow) k_spmv_t<R, uint16_t, false><<<nblk, 256, lds, s>>>(T, 0);
         return;
@@ -1553,6 +2039,10 @@ static void launch_spmv_all(const Tables &T, int group, int wg, hipStream_t s) {
 // Block partials of X^T u for every problem (one launch per gene tile and
 // index width present).
 static void launch_spmv_t_all(const Tables &T, int group, int wg, hipStream_t s) {
+    if (T.tmodes & kSortedBit)
+        k_spmv_t_sorted<2><<<(unsigned)T.num_blocks, 256,
+                             spmv_t_sorted_lds_bytes(std::min(T.max_g, kMaxSortedGenes)), s>>>(T);
+    if (!(T.tmodes & ~kSortedBit)) return;
     const size_t lds = capped_lds(spmv_t_lds_bytes(T.max_g), wg, T);
     switch (group) {
         case 2: launch_spmv_t_tiles<2>(T, lds, s); break;

@@ -63,9 +63,19 @@ struct WsLayout {
         Ub, Sb, vw, vdot, vsq, indptr64, idx, data4, total;
     size_t idx_bytes;  // bytes per packed column index (2, or 4 when g > 65535)
     int64_t nnz4;  // nnz rounded up to whole quads of 4 elements
+    // Gene-sorted copy for X^T u (tsorted): values, row bytes, run-start
+    // bits (thb_words 16-bit words) and per-sub-block gene counts (row
+    // stride tgl).
+    bool tsorted;
+    size_t tval, trow, thb, tlens;
+    int64_t thb_words;
+    int tgl;
 };
 
-WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz) {
+// Whether a problem with g genes takes the gene-sorted X^T u.
+bool uses_sorted(int g, bool enabled) { return enabled && g <= bpca::kMaxSortedGenes; }
+
+WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz, bool sorted) {
     const int64_t nb = bpca::ceil_div(n, bpca::kRowBlock);
     const int64_t ng = bpca::ceil_div(nb, bpca::kGroup);
     const int64_t mm = m + (m & 1);
@@ -102,6 +112,19 @@ WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz) {
     L.idx_bytes = g > bpca::kMaxPacked16 ? 4 : 2;
     L.idx = take(L.idx_bytes * L.nnz4);
     L.data4 = take(sizeof(float) * L.nnz4);
+    L.tsorted = sorted;
+    if (sorted) {
+        // Lane chunks of kTChunk entries are loaded whole up to the chunk
+        // holding the last entry.
+        const int64_t tlen = (nnz + 2 * bpca::kTChunk + 63) / 64 * 64;
+        const int64_t nsub = bpca::ceil_div(n, bpca::kSubRows);
+        L.tgl = (g + 7) / 8 * 8;
+        L.thb_words = (tlen / 16 + 1) / 2 * 2;
+        L.tval = take(sizeof(float) * tlen);
+        L.trow = take(tlen);
+        L.thb = take(sizeof(uint16_t) * L.thb_words);
+        L.tlens = take(sizeof(uint16_t) * (size_t)nsub * L.tgl);
+    }
     L.total = off;
     return L;
 }
@@ -169,10 +192,10 @@ namespace {
 
 // Scratch bytes of an allocation-free solve: descriptor, block and group
 // lists, counters, workspace.
-uint64_t noalloc_bytes(int64_t n, int g, int m, int64_t nnz, bool widen) {
+uint64_t noalloc_bytes(int64_t n, int g, int m, int64_t nnz, bool widen, bool sorted) {
     const int64_t nb = bpca::ceil_div(n, bpca::kRowBlock), ng = bpca::ceil_div(nb, bpca::kGroup);
     return align_up(sizeof(bpca::Prob)) + align_up(sizeof(int2) * nb) + align_up(sizeof(int2) * ng) +
-           kAlign + ws_layout(n, g, m, widen, nnz).total;
+           kAlign + ws_layout(n, g, m, widen, nnz, sorted).total;
 }
 
 // With `device` current: record the device address of its scratch copy.
@@ -226,6 +249,7 @@ struct bpca_context {
     int spmv_group = 4, spmvt_group = 4;   // autotuned load-group sizes
     int spmv_wg = 0, spmvt_wg = 0;         // autotuned workgroups-per-CU caps
     int64_t lds_per_cu = 0;                // LDS bytes of one compute unit
+    bool sorted_xtu = true;                // gene-sorted X^T u for g <= kMaxSortedGenes
 };
 
 namespace {
@@ -369,6 +393,14 @@ bpca::Prob make_prob(const bpca_problem *pb, const WsLayout &L, char *base,
     d.vw = (double *)(base + L.vw);
     d.vdot = (double *)(base + L.vdot);
     d.vsq = (double *)(base + L.vsq);
+    d.tsorted = L.tsorted ? 1 : 0;
+    if (L.tsorted) {
+        d.tval = (const float *)(base + L.tval);
+        d.trow = (const uint8_t *)(base + L.trow);
+        d.thb = (const uint16_t *)(base + L.thb);
+        d.tlens = (const uint16_t *)(base + L.tlens);
+        d.tgl = L.tgl;
+    }
     d.phase = bpca::PH_SETUP;
     return d;
 }
@@ -411,11 +443,12 @@ void engine_drop_graph(bpca_engine *E) {
 int engine_rebuild_tables(bpca_engine *E) {
     bpca_context *ctx = E->ctx;
     std::vector<int2> blocks, groups;
-    int max_g = 2, max_m = 1, modes = 0;
+    int max_g = 2, max_m = 1, modes = 0, tmodes = 0;
     for (int s = 0; s < (int)E->slots.size(); s++) {
         const Slot &sl = E->slots[s];
         if (!sl.used) continue;
         modes |= 1 << sl.desc.mode;
+        tmodes |= sl.desc.tsorted ? bpca::kSortedBit : 1 << sl.desc.mode;
         for (int64_t b = 0; b < sl.desc.nb; b++) blocks.push_back(make_int2(s, (int)b));
         for (int64_t G = 0; G < sl.desc.ngrp; G++) groups.push_back(make_int2(s, (int)G));
         max_g = std::max(max_g, sl.desc.g);
@@ -448,6 +481,7 @@ int engine_rebuild_tables(bpca_engine *E) {
     E->T.max_g = max_g;
     E->T.max_m = max_m;
     E->T.modes = modes;
+    E->T.tmodes = tmodes;
     E->T.max_lds = (int64_t)ctx->max_lds;
     E->T.spmv_group = ctx->spmv_group;
     E->T.spmvt_group = ctx->spmvt_group;
@@ -671,6 +705,19 @@ int bpca_context_set_occupancy(bpca_context *ctx, int32_t spmv_wg,
     return BPCA_OK;
 }
 
+int bpca_context_set_sorted_xtu(bpca_context *ctx, int32_t enable) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    ctx->sorted_xtu = enable != 0;
+    return BPCA_OK;
+}
+
+int bpca_context_get_sorted_xtu(bpca_context *ctx, int32_t *enable) {
+    if (!ctx || !enable) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *enable = ctx->sorted_xtu ? 1 : 0;
+    return BPCA_OK;
+}
+
 int bpca_context_get_occupancy(bpca_context *ctx, int32_t *spmv_wg,
                                int32_t *spmvt_wg) {
     if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
@@ -755,7 +802,7 @@ uint64_t bpca_workspace_bytes(uint64_t num_cells, uint32_t num_genes,
     // Without the CSR's nnz this counts the packed copies at 7% density.
     const int64_t nnz = (int64_t)((double)num_cells * num_genes * 0.07);
     return ws_layout((int64_t)num_cells, (int)num_genes, (int)subspace_size,
-                     true, nnz).total;
+                     true, nnz, uses_sorted((int)num_genes, true)).total;
 }
 
 // ------------------------------------------------------------ engine ABI
@@ -831,7 +878,7 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
     HIP_TRY(hipStreamSynchronize(s));
     if (nnz < 0) return fail(BPCA_ERR_NONCANONICAL, "indptr[num_cells] is negative");
     const bool widen = pb->indptr_width == 4;
-    const WsLayout L = ws_layout(n, g, m, widen, nnz);
+    const WsLayout L = ws_layout(n, g, m, widen, nnz, uses_sorted(g, ctx->sorted_xtu));
     void *ws = nullptr;
     BPCA_TRY(pool_alloc(ctx, L.total, &ws));
     {
@@ -881,6 +928,9 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
     const std::vector<double> v0 = start_vector(pb->seed, g);
     hipError_t e = hipSuccess;
     if (widen) e = bpca::launch_widen_indptr((const int32_t *)pb->indptr, n + 1, (int64_t *)(base + L.indptr64), s);
+    if (e == hipSuccess && L.tsorted)
+        e = bpca::launch_tsort_build(d.data, (const uint16_t *)d.idx, d.indptr, n, g, L.tgl, (float *)d.tval,
+                                     (uint8_t *)d.trow, (uint16_t *)d.thb, L.thb_words, (uint16_t *)d.tlens, (int64_t)ctx->max_lds, s);
     if (e == hipSuccess) e = hipMemcpyAsync(d.V[0], v0.data(), sizeof(double) * g, hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipMemcpyAsync(E->d_probs + slot, &d, sizeof(d), hipMemcpyHostToDevice, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1169,7 +1219,7 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
         nnz = nnz32;
     }
     if (nnz < 0) return fail(BPCA_ERR_NONCANONICAL, "indptr[num_cells] is negative");
-    const uint64_t need = noalloc_bytes(n, g, m, nnz, indptr_width == 4);
+    const uint64_t need = noalloc_bytes(n, g, m, nnz, indptr_width == 4, uses_sorted(g, ctx->sorted_xtu));
     if (need > kScratchBytes)
         return fail(BPCA_ERR_INVALID_ARGUMENT,
                     "the allocation-free solve needs " + std::to_string(need) +
@@ -1190,7 +1240,7 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
     off += kAlign;
     char *ws = base + off;
     const bool widen = indptr_width == 4;
-    const WsLayout L = ws_layout(n, g, m, widen, nnz);
+    const WsLayout L = ws_layout(n, g, m, widen, nnz, uses_sorted(g, ctx->sorted_xtu));
     // The scratch keeps the last call's contents: clear the counters and the
     // quad padding past nnz of both packed copies.
     HIP_TRY(bpca::launch_zero(d_done, kAlign, s));
@@ -1223,6 +1273,9 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
     for (int64_t b = 0; b < nb; b++) blocks[b] = make_int2(0, (int)b);
     for (int64_t G = 0; G < ngrp; G++) groups[G] = make_int2(0, (int)G);
     if (widen) HIP_TRY(bpca::launch_widen_indptr((const int32_t *)d_indptr, n + 1, (int64_t *)(ws + L.indptr64), s));
+    if (L.tsorted)
+        HIP_TRY(bpca::launch_tsort_build(d.data, (const uint16_t *)d.idx, d.indptr, n, g, L.tgl, (float *)d.tval,
+                                         (uint8_t *)d.trow, (uint16_t *)d.thb, L.thb_words, (uint16_t *)d.tlens, (int64_t)ctx->max_lds, s));
     HIP_TRY(hipMemcpyAsync(d.V[0], v0.data(), sizeof(double) * g, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_prob, &d, sizeof(d), hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(d_blocks, blocks.data(), sizeof(int2) * nb, hipMemcpyHostToDevice, s));
@@ -1238,6 +1291,7 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
     T.max_g = std::max(g, 2);
     T.max_m = std::max(m, 1);
     T.modes = 1 << d.mode;
+    T.tmodes = d.tsorted ? bpca::kSortedBit : 1 << d.mode;
     T.max_lds = (int64_t)ctx->max_lds;
     T.spmv_group = ctx->spmv_group;
     T.spmvt_group = ctx->spmvt_group;

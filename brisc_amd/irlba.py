@@ -149,6 +149,17 @@ class Context:
             self._h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
 
+    def set_sorted_xtu(self, enable: bool) -> None:
+        """Compute X^T u of problems admitted from now on with at most 4096
+        genes from a gene-sorted copy of X (the default) or from the CSR.
+        The two orders of summation agree to rounding, not bit for bit."""
+        _lib.check(_lib.lib().bpca_context_set_sorted_xtu(self._h, int(bool(enable))))
+
+    def sorted_xtu(self) -> bool:
+        a = ctypes.c_int32(0)
+        _lib.check(_lib.lib().bpca_context_get_sorted_xtu(self._h, ctypes.byref(a)))
+        return bool(a.value)
+
     def set_occupancy(self, spmv_wg: int, spmvt_wg: int) -> None:
         """At most this many workgroups per compute unit for X v / X^T u
         (0 = no cap; speed only)."""

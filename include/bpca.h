@@ -117,6 +117,15 @@ int bpca_context_set_occupancy(bpca_context *ctx, int32_t spmv_wg,
                                int32_t spmvt_wg);
 int bpca_context_get_occupancy(bpca_context *ctx, int32_t *spmv_wg,
                                int32_t *spmvt_wg);
+/* Whether problems admitted from now on with num_genes <= 4096 compute
+ * X^T u from a gene-sorted copy of X made at admission (1, the default) or
+ * from the packed CSR (0).  The two sum each gene's block partial in
+ * different fixed orders, so results agree to rounding, not bit for bit; a
+ * problem keeps the choice it was admitted with.  The copy adds about 5
+ * bytes per nonzero to the workspace.  No counterpart in the reference (a
+ * data-layout knob of this library). */
+int bpca_context_set_sorted_xtu(bpca_context *ctx, int32_t enable);
+int bpca_context_get_sorted_xtu(bpca_context *ctx, int32_t *enable);
 /* Bytes of device workspace a solve of this shape allocates. */
 uint64_t bpca_workspace_bytes(uint64_t num_cells, uint32_t num_genes,
                               uint32_t k, uint32_t subspace_size);
