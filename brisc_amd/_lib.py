@@ -61,7 +61,9 @@ class SolveStats(ctypes.Structure):
                 ('seconds', ctypes.c_double),
                 ('svd_sweeps', ctypes.c_int32),
                 ('slot', ctypes.c_int32),
-                ('smax', ctypes.c_double)]
+                ('smax', ctypes.c_double),
+                ('reorth_passes', ctypes.c_int32),
+                ('reserved', ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -144,6 +146,8 @@ def lib() -> ctypes.CDLL:
         'bpca_context_set_occupancy': (i32, [p, i32, i32]),
         'bpca_context_set_sorted_xtu': (i32, [p, i32]),
         'bpca_context_get_sorted_xtu': (i32, [p, ctypes.POINTER(i32)]),
+        'bpca_context_set_reorth_passes': (i32, [p, i32]),
+        'bpca_context_get_reorth_passes': (i32, [p, ctypes.POINTER(i32)]),
         'bpca_context_get_occupancy': (i32, [p, ctypes.POINTER(ctypes.c_int32),
                                              ctypes.POINTER(ctypes.c_int32)]),
         'bpca_autotune': (i32, [p, ctypes.POINTER(Problem), i32, p]),

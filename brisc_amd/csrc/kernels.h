@@ -11,7 +11,8 @@
 //            loaded whole -- and indptr int64[n+1]
 //   U[0/1]   float64 column-major n x m (ld n): left Lanczos vectors
 //   V[0/1]   float64 column-major g x (m+1) (ld g): right Lanczos vectors
-//   u32      float32[n]: raw X v' of the current step
+//   u32      float32[2 (n + 64)]: raw X v' of the current and the previous step
+//   R[0/1]   float64 column-major g x (m+1) (ld g): X^T U (one-pass problems)
 //   vs       float32[g]: v_j / sd, the X v' input
 //   B        float64 column-major m x (m+1): projected matrix
 //   W, Jv    float64 m x m: Jacobi work / right singular vectors
@@ -70,6 +71,9 @@ constexpr int kMaxSortedGenes = 4096;  // X^T u from the gene-sorted copy up to 
 constexpr int kTChunk = 16;         // entries per lane in the gene-sorted X^T u
 constexpr int kTWindow = kWave * kTChunk;  // entries per wave step of that kernel
 constexpr int kSortedBit = 1 << 3;  // Tables::tmodes: some problem uses the gene-sorted X^T u
+constexpr int kReorthTwoBit = 1;    // Tables::rmodes: some problem reorthogonalises in two passes
+constexpr int kReorthOneBit = 2;    //                 some problem in one pass
+constexpr int64_t kOnePassMinCells = 65536;  // automatic one-pass choice: n >= max(8 g, this)
 
 // Index modes of a problem (Prob::mode): how its sparse products load.
 enum IndexMode : int32_t {
@@ -127,10 +131,23 @@ struct Prob {
     int32_t ubuf, vbuf, converged, nconv;
     int32_t svd_sweeps, pad0;
     int32_t tsorted, tgl;    // gene-sorted X^T u in use; tlens row stride (g rounded up to 8)
+    // One-pass reorthogonalisation (onepass != 0, n >= 8 g): column j of U is
+    // written by step j + 1's pass over U (or at the end of the sweep), and
+    // X^T U[:, j] is formed in gene space from X^T (u - mean) and R.
+    int32_t onepass;
+    int32_t jfirst;          // first step of the current sweep (0, then kc)
+    double *R[2];            // float64 g x (m+1) (ld g): X^T U, flipped with U
     double alpha, inv_alpha, rnorm, smax;
-    double umean;         // mean left in U[:, j] by the reorthogonalisation
+    double umean;         // two passes: mean left in U[:, j]; one pass: mean of u
     int64_t steps;
 };
+
+// The raw X v' of Lanczos step j: two buffers of n + kWave floats (a sink
+// slot for masked lanes at [n]), alternating with j, so that a one-pass
+// step still reads the previous step's vector.
+__device__ __forceinline__ float *u_of_step(const Prob &pr, int j) {
+    return pr.u32 + (int64_t)(j & 1) * (pr.n + kWave);
+}
 
 struct Tables {
     Prob *probs;
@@ -142,6 +159,7 @@ struct Tables {
     int max_g, max_m;
     int modes;            // bit set of the IndexModes of the problems
     int tmodes;           // X^T u: IndexModes of the CSR-path problems, kSortedBit
+    int rmodes;           // reorthogonalisation: kReorthTwoBit, kReorthOneBit
     int64_t max_lds;      // dynamic LDS bytes a workgroup may use
     int spmv_group;       // segments per load group in X v   (2, 4 or 8)
     int spmvt_group;      // segments per load group in X^T u (2, 4 or 8)

@@ -158,6 +158,7 @@ __global__ __launch_bounds__(256) void k_setup_finish(Tables T) {
     __syncthreads();
     if (threadIdx.x == 0) {
         pr.j = 0;
+        pr.jfirst = 0;
         pr.kc = 0;
         pr.iter = 0;
         pr.ubuf = 0;
@@ -419,15 +420,16 @@ __global__ __launch_bounds__(256) void k_spmv(Tables T) {
     const int nrows = (int)nrows64;
     const int64_t q0 = nrows > 0 ? stage_row_offsets(pr.indptr, r0, nrows, lane, off) : 0;
     __syncthreads();
+    float *uj = u_of_step(pr, pr.j);
     if (nrows == 0 || off[nrows] == off[0]) {
-        for (int q = lane; q < nrows; q += kWave) pr.u32[r0 + q] = 0.0f;
+        for (int q = lane; q < nrows; q += kWave) uj[r0 + q] = 0.0f;
         return;
     }
     gquad *data = (gquad *)pr.data + q0;
     typename IxQuad<IX>::gvec *indices = (typename IxQuad<IX>::gvec *)pr.idx + q0;
     gfloat_r *gv = (gfloat_r *)pr.vs;
-    gfloat_w *u = (gfloat_w *)(pr.u32 + r0);
-    gfloat_w *sink = (gfloat_w *)(pr.u32 + pr.n);
+    gfloat_w *u = (gfloat_w *)(uj + r0);
+    gfloat_w *sink = (gfloat_w *)(uj + pr.n);
     Cursor cs{0, uniform(off[0]), uniform(off[1])};
     SegGroup<R, IX> ga, gb;
     float carry = 0.0f;
@@ -459,14 +461,14 @@ __global__ __launch_bounds__(256) void k_reorth_partials(Tables T) {
     constexpr int R = kRowBlock / (2 * kWave);  // 8 row pairs per lane
     const int2 bp = T.blocks[blockIdx.x];
     Prob &pr = T.probs[bp.x];
-    if (pr.phase != PH_LANCZOS) return;
+    if (pr.phase != PH_LANCZOS || pr.onepass) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = wave_id();
     const int64_t n = pr.n, b = bp.y, r0 = b * kRowBlock;
     const int64_t nrows = n - r0 < kRowBlock ? n - r0 : kRowBlock;
     const bool vec = nrows == kRowBlock && (n & 1) == 0;
     const int j = pr.j;
-    const float *u32 = pr.u32;
+    const float *u32 = u_of_step(pr, j);
     const double *U = pr.U[pr.ubuf];
     double ua[R], ub[R];
 #pragma unroll
@@ -565,7 +567,7 @@ __global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
     __shared__ double tmp[4];
     const int2 bp = T.blocks[blockIdx.x];
     Prob &pr = T.probs[bp.x];
-    if (pr.phase != PH_LANCZOS) return;
+    if (pr.phase != PH_LANCZOS || pr.onepass) return;
     const int j = pr.j;
     const int64_t n = pr.n;
     for (int i = threadIdx.x; i <= j; i += 256) coef[i] = pr.red[i];
@@ -574,7 +576,7 @@ __global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
     const int64_t r0 = (int64_t)bp.y * kRowBlock;
     double *U = pr.U[pr.ubuf];
     double *Uj = U + (int64_t)j * n;
-    const float *u32 = pr.u32;
+    const float *u32 = u_of_step(pr, j);
     double acc = 0.0, sum = 0.0;
     const bool vec = r0 + kRowBlock <= n && (n & 1) == 0;
     static_assert(kRowBlock == 1024, "two 512-row halves per block");
@@ -626,23 +628,219 @@ __global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
     }
 }
 
+// One-pass reorthogonalisation (problems with Prob::onepass).  The two
+// passes above read U[:, :j] twice per Lanczos step: once for the dots h =
+// U^T u, once for u - U h.  Here step j reads it once and does two things
+// with each element it loads:
+//  * it completes the previous step's column, left pending by that step,
+//      U[:, j-1] = (u_{j-1} - mean_{j-1} - U[:, :j-1] h_{j-1}) / alpha_{j-1},
+//    whose coefficients are still in place (red: red[0] = the sum of u_{j-1},
+//    red[c] = U[:, c-1] . u_{j-1}; inv_alpha), and
+//  * it forms this step's partials: P[c] = U[:, c-1] . u_j for c = 1..j (the
+//    one for the column just completed from its values in registers), the
+//    block sum P[0] and the block's sum of squares sq[b] of u_j.
+// k_alpha then has alpha_j = |u_j - mean - U h|  from |u - mean|^2 - |h|^2
+// (U orthonormal), and X^T U[:, j] is formed in gene space (k_vside_w) from
+// X^T (u_j - mean) and R = X^T U[:, :j].  A column is pending when j >
+// jfirst (the first step of a sweep reads the restarted U, all of it
+// written); at the end of a sweep, kEnd, the last column is completed alone
+// (phase PH_RESTARTING or PH_FINAL) before the restart or final product.
+// The arithmetic per element is the two-pass one, in another order; only the
+// gene-space X^T U carries the previous columns' FP32 rounding forward, by a
+// factor |h| / alpha per step, about sqrt(g / n) -- hence n >= 8 g.
+//
+// Layout: wave w takes the columns c = w (mod 4) (wave 0: 4, 8, ...; it also
+// sums u), two per pass, each over the block's 1024 rows as 8 row pairs per
+// lane (lane l: rows 2 l + 128 t, + 1); each lane keeps the correction
+// U h of its 16 rows summed over its wave's columns in ascending order; the
+// four waves' corrections meet in LDS as (w0 + w1) + (w2 + w3); thread t
+// then completes rows (2 t, 2 t + 1) and (512 + 2 t, 513 + 2 t).
+template <bool kEnd>
+__global__ __launch_bounds__(256) void k_reorth_onepass(Tables T) {
+    constexpr int R = kRowBlock / (2 * kWave);  // 8 row pairs per lane
+    __shared__ double cbuf[4][kRowBlock];
+    __shared__ double tmp[4];
+    const int2 bp = T.blocks[blockIdx.x];
+    Prob &pr = T.probs[bp.x];
+    if (!pr.onepass) return;
+    if (kEnd ? (pr.phase != PH_RESTARTING && pr.phase != PH_FINAL) : pr.phase != PH_LANCZOS) return;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_id();
+    const int j = pr.j;
+    const bool pend = kEnd || j > pr.jfirst;
+    const int pc = kEnd ? j : j - 1;   // the pending column
+    const int nm = pend ? pc : j;      // columns read: U[:, :nm]
+    const int64_t n = pr.n, b = bp.y, r0 = b * kRowBlock, nb = pr.nb;
+    const int nrows = (int)(n - r0 < kRowBlock ? n - r0 : kRowBlock);
+    const bool vec = nrows == kRowBlock && (n & 1) == 0;
+    const double *U = pr.U[pr.ubuf];
+    const double *h = pr.red;
+    const float *ucur = u_of_step(pr, j);
+    double ua[R], ub[R], ca[R], cb[R];
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+        const int r = 2 * lane + t * 2 * kWave;
+        ua[t] = !kEnd && r < nrows ? (double)ucur[r0 + r] : 0.0;
+        ub[t] = !kEnd && r + 1 < nrows ? (double)ucur[r0 + r + 1] : 0.0;
+        ca[t] = 0.0;
+        cb[t] = 0.0;
+    }
+    int c0 = w;
+    if (w == 0) {
+        if (!kEnd) {
+            double s = 0.0, ss = 0.0;
+#pragma unroll
+            for (int t = 0; t < R; t++) {
+                s += ua[t];
+                s += ub[t];
+                ss = fma(ua[t], ua[t], ss);
+                ss = fma(ub[t], ub[t], ss);
+            }
+            s = wave_sum(s);
+            ss = wave_sum(ss);
+            if (lane == 0) {
+                pr.P[b] = s;
+                pr.sq[b] = ss;
+            }
+        }
+        c0 = 4;
+    }
+    if (vec) {
+        for (int c = c0; c <= nm; c += 8) {
+            const bool two = c + 4 <= nm;
+            const double *Uc = U + (int64_t)(c - 1) * n + r0 + 2 * lane;
+            const double *Ud = two ? Uc + 4 * n : Uc;
+            v2d x[R], y[R];
+#pragma unroll
+            for (int t = 0; t < R; t++) x[t] = __builtin_nontemporal_load((const v2d *)(Uc + t * 2 * kWave));
+            if (two) {
+#pragma unroll
+                for (int t = 0; t < R; t++) y[t] = __builtin_nontemporal_load((const v2d *)(Ud + t * 2 * kWave));
+            }
+            if (!kEnd) {
+                double acc = 0.0;
+#pragma unroll
+                for (int t = 0; t < R; t++) {
+                    acc = fma(x[t].x, ua[t], acc);
+                    acc = fma(x[t].y, ub[t], acc);
+                }
+                acc = wave_sum(acc);
+                if (lane == 0) pr.P[(int64_t)c * nb + b] = acc;
+            }
+            if (pend) {
+                const double hc = h[c];
+#pragma unroll
+                for (int t = 0; t < R; t++) {
+                    ca[t] = fma(x[t].x, hc, ca[t]);
+                    cb[t] = fma(x[t].y, hc, cb[t]);
+                }
+            }
+            if (two) {
+                if (!kEnd) {
+                    double acd = 0.0;
+#pragma unroll
+                    for (int t = 0; t < R; t++) {
+                        acd = fma(y[t].x, ua[t], acd);
+                        acd = fma(y[t].y, ub[t], acd);
+                    }
+                    acd = wave_sum(acd);
+                    if (lane == 0) pr.P[(int64_t)(c + 4) * nb + b] = acd;
+                }
+                if (pend) {
+                    const double hd = h[c + 4];
+#pragma unroll
+                    for (int t = 0; t < R; t++) {
+                        ca[t] = fma(y[t].x, hd, ca[t]);
+                        cb[t] = fma(y[t].y, hd, cb[t]);
+                    }
+                }
+            }
+        }
+    } else {
+        for (int c = c0; c <= nm; c += 4) {
+            const double *Uc = U + (int64_t)(c - 1) * n + r0;
+            const double hc = pend ? h[c] : 0.0;
+            double acc = 0.0;
+#pragma unroll
+            for (int t = 0; t < R; t++) {
+                const int r = 2 * lane + t * 2 * kWave;
+                const double x0 = r < nrows ? Uc[r] : 0.0;
+                const double x1 = r + 1 < nrows ? Uc[r + 1] : 0.0;
+                acc = fma(x0, ua[t], acc);
+                acc = fma(x1, ub[t], acc);
+                ca[t] = fma(x0, hc, ca[t]);
+                cb[t] = fma(x1, hc, cb[t]);
+            }
+            if (!kEnd) {
+                acc = wave_sum(acc);
+                if (lane == 0) pr.P[(int64_t)c * nb + b] = acc;
+            }
+        }
+    }
+    if (!pend) return;
+#pragma unroll
+    for (int t = 0; t < R; t++) {
+        const int r = 2 * lane + t * 2 * kWave;
+        cbuf[w][r] = ca[t];
+        cbuf[w][r + 1] = cb[t];
+    }
+    __syncthreads();
+    const double mean = h[0] / (double)n, ia = pr.inv_alpha;
+    const float *uprev = u_of_step(pr, pc);
+    double *Up = pr.U[pr.ubuf] + (int64_t)pc * n + r0;
+    double d = 0.0;
+#pragma unroll
+    for (int half = 0; half < 2; half++) {
+        const int r = 2 * threadIdx.x + 512 * half;
+        double x[2], uc[2];
+#pragma unroll
+        for (int e = 0; e < 2; e++) {
+            const int re = r + e;
+            const double cr = (cbuf[0][re] + cbuf[1][re]) + (cbuf[2][re] + cbuf[3][re]);
+            x[e] = re < nrows ? ((double)uprev[r0 + re] - mean - cr) * ia : 0.0;
+            uc[e] = !kEnd && re < nrows ? (double)ucur[r0 + re] : 0.0;
+        }
+        if (vec) {
+            *(v2d *)(Up + r) = v2d{x[0], x[1]};
+        } else {
+#pragma unroll
+            for (int e = 0; e < 2; e++)
+                if (r + e < nrows) Up[r + e] = x[e];
+        }
+        d = fma(x[0], uc[0], d);
+        d = fma(x[1], uc[1], d);
+    }
+    if (kEnd) return;
+    d = block_sum<4>(d, tmp);
+    if (threadIdx.x == 0) pr.P[(int64_t)j * nb + b] = d;
+}
+
 // mean = (sum of block sums) / n, removed from U[:, j] when k_spmv_t
 // normalises it; alpha = max(sqrt(sum of squares - n mean^2), 1e-8), the
-// norm of the centred column; B[j, j] = alpha.
+// norm of the centred column; B[j, j] = alpha.  One-pass problems: the sums
+// are u's (red[0], sq[b]), and alpha^2 = |u - mean|^2 - sum_c red[c]^2, the
+// squared norm of u - mean - U h for orthonormal U; mean is u's.
 __global__ __launch_bounds__(256) void k_alpha(Tables T) {
     __shared__ double tmp[4];
     Prob &pr = T.probs[blockIdx.x];
     if (pr.phase != PH_LANCZOS) return;
+    const bool one = pr.onepass != 0;
     double acc = 0.0, sum = 0.0;
     for (int64_t b = threadIdx.x; b < pr.nb; b += 256) {
         acc += pr.sq[b];
-        sum += pr.sq[pr.nb + b];
+        if (!one) sum += pr.sq[pr.nb + b];
     }
     acc = block_sum<4>(acc, tmp);
-    sum = block_sum<4>(sum, tmp);
+    if (!one) sum = block_sum<4>(sum, tmp);
     if (threadIdx.x == 0) {
+        if (one) sum = pr.red[0];
         const double mean = sum / (double)pr.n;
-        const double ss = acc - sum * mean;
+        double ss = acc - sum * mean;
+        if (one) {
+            double hh = 0.0;
+            for (int c = 1; c <= pr.j; c++) hh = fma(pr.red[c], pr.red[c], hh);
+            ss -= hh;
+        }
         double alpha = sqrt(ss > 0.0 ? ss : 0.0);
         if (alpha < 1e-8) alpha = 1e-8;
         pr.alpha = alpha;
@@ -687,7 +885,8 @@ __device__ __forceinline__ void spmv_t_reduce(const SegGroup<R, IX> &G,
 
 // One workgroup per data block; wave w owns a 256-row sub-block and a
 // private LDS accumulator of g floats.  u = (U[:, j] - mean) / alpha is
-// written back centred and normalised.  Per gene the sub-block sum is the FP32 fma chain over its rows
+// written back centred and normalised (one-pass problems: u = u_j - mean of
+// the step's raw vector, U untouched).  Per gene the sub-block sum is the FP32 fma chain over its rows
 // in order (a row's column indices are unique, so the lanes of one LDS update
 // touch distinct genes); sub-blocks combine as (w0+w1)+(w2+w3).  Loads are
 // the segment groups of k_spmv.
@@ -717,7 +916,11 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T, int c0) {
     const int nrows = (int)nrows64;
     double *Uj = pr.U[pr.ubuf] + (int64_t)pr.j * pr.n;
     const double inv_alpha = pr.inv_alpha, umean = pr.umean;
-    if (c0 == 0) {
+    if (pr.onepass) {
+        // u - mean of this step; U[:, j] is written by the next pass.
+        const float *uj = u_of_step(pr, pr.j);
+        for (int t = lane; t < nrows; t += kWave) uval[t] = (float)((double)uj[r0 + t] - umean);
+    } else if (c0 == 0) {
         for (int t = lane; t < nrows; t += kWave) {
             const double x = (Uj[r0 + t] - umean) * inv_alpha;
             Uj[r0 + t] = x;
@@ -922,7 +1125,7 @@ __host__ __device__ inline int tsort_lds_floats(int g) { return ((g + 4) & ~3) +
 size_t spmv_t_sorted_lds_bytes(int g) { return sizeof(float) * 4 * (size_t)tsort_lds_floats(g); }
 
 // One workgroup per data block, wave w on sub-block w; u = (U[:, j] - mean)
-// / alpha written back as in k_spmv_t.  Problems without the sorted copy
+// / alpha written back as in k_spmv_t (one pass: u_j - mean).  Problems without the sorted copy
 // are left to k_spmv_t.
 template <int R>
 __global__ __launch_bounds__(256) void k_spmv_t_sorted(Tables T) {
@@ -944,12 +1147,19 @@ __global__ __launch_bounds__(256) void k_spmv_t_sorted(Tables T) {
     const int nrows = (int)nrows64;
     double *Uj = pr.U[pr.ubuf] + (int64_t)pr.j * pr.n;
     const double inv_alpha = pr.inv_alpha, umean = pr.umean;
+    const bool one = pr.onepass != 0;
+    const float *uj = u_of_step(pr, pr.j);
     for (int t = lane; t < kSubRows; t += kWave) {
         float x = 0.f;
         if (t < nrows) {
-            const double d = (Uj[r0 + t] - umean) * inv_alpha;
-            Uj[r0 + t] = d;
-            x = (float)d;
+            if (one) {
+                // u - mean of this step; U[:, j] is written by the next pass.
+                x = (float)((double)uj[r0 + t] - umean);
+            } else {
+                const double d = (Uj[r0 + t] - umean) * inv_alpha;
+                Uj[r0 + t] = d;
+                x = (float)d;
+            }
         }
         uu[tsort_row_slot(t)] = x;
     }
@@ -1241,7 +1451,9 @@ hipError_t launch_tsort_build(const float *data4, const uint16_t *idx16, const i
 // Four short launches over genes (the V side is g long, so one workgroup per
 // problem would leave the step latency-bound):
 //   k_vside_w    w = (sum of the group partials in group order) / sd
-//                    - alpha V[:, j]                     (FP64, per gene)
+//                    - alpha V[:, j]                     (FP64, per gene;
+//                one pass: the sum is X^T (u - mean), corrected to X^T U[:, j]
+//                first)
 //   k_vside_dot  dot_c = V[:, c] . w, thread-strided over genes then
 //                block_sum<4>, one workgroup per column
 //   k_vside_upd  x = w - sum_c V[:, c] dot_c (c ascending); per-chunk sum
@@ -1260,7 +1472,17 @@ __global__ __launch_bounds__(kVChunk) void k_vside_w(Tables T) {
     if (i >= g) return;
     double acc = 0.0;
     for (int64_t G = 0; G < pr.ngrp; G++) acc += pr.grp[G * 2 * g + i];
-    const double *Vj = pr.V[pr.vbuf] + (int64_t)pr.j * g;
+    const int j = pr.j;
+    if (pr.onepass) {
+        // acc = X^T (u_j - mean); X^T U[:, j] = (acc - X^T U[:, :j] h) / alpha,
+        // h = red[1..j] (c ascending), kept as column j of R.
+        double *R = pr.R[pr.ubuf];
+        double corr = 0.0;
+        for (int c = 0; c < j; c++) corr = fma(R[(int64_t)c * g + i], pr.red[c + 1], corr);
+        acc = (acc - corr) * pr.inv_alpha;
+        R[(int64_t)j * g + i] = acc;
+    }
+    const double *Vj = pr.V[pr.vbuf] + (int64_t)j * g;
     pr.vw[i] = acc / pr.sd[i] - pr.alpha * Vj[i];
 }
 
@@ -1929,7 +2151,8 @@ hipError_t launch_gemm_mfma_test(const double *A, int64_t n, int m,
     return hipGetLastError();
 }
 
-// V <- V Vb[:, :kc], V[:, kc] = remainder, vs = remainder / sd.  Grid:
+// V <- V Vb[:, :kc], V[:, kc] = remainder, vs = remainder / sd (and, one
+// pass, R <- R Ub[:, :kc]).  Grid:
 // (problem, chunk of 64 genes, group of 4 columns); thread t computes gene
 // t % 64 of column 4 * group + t / 64, summing over l ascending.
 __global__ __launch_bounds__(256) void k_restart_v(Tables T) {
@@ -1949,6 +2172,13 @@ __global__ __launch_bounds__(256) void k_restart_v(Tables T) {
         double acc = 0.0;
         for (int l = 0; l < m; l++) acc = fma(V[(int64_t)l * g + i], Vb[l + (int64_t)c * m], acc);
         V2[(int64_t)c * g + i] = acc;
+        if (pr.onepass) {
+            // X^T U follows U: R <- R Ub[:, :kc].
+            const double *R = pr.R[pr.ubuf];
+            double ar = 0.0;
+            for (int l = 0; l < m; l++) ar = fma(R[(int64_t)l * g + i], pr.Ub[l + (int64_t)c * m], ar);
+            pr.R[pr.ubuf ^ 1][(int64_t)c * g + i] = ar;
+        }
     }
     if (cgrp == 0 && threadIdx.x < 64) {
         V2[(int64_t)kc * g + i] = pr.rem[i];
@@ -1963,6 +2193,7 @@ __global__ void k_restart_commit(Tables T) {
     pr.ubuf ^= 1;
     pr.vbuf ^= 1;
     pr.j = pr.kc;
+    pr.jfirst = pr.kc;
     pr.phase = PH_LANCZOS;
 }
 
@@ -2121,9 +2352,11 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     launch_spmv_all(T, T.spmv_group, T.spmv_wg, s);
     mark(0, 1);
     mark(2, 0);
-    k_reorth_partials<<<nblk, 256, 0, s>>>(T);
+    const bool two_pass = (T.rmodes & kReorthTwoBit) != 0, one_pass = (T.rmodes & kReorthOneBit) != 0;
+    if (two_pass) k_reorth_partials<<<nblk, 256, 0, s>>>(T);
+    if (one_pass) k_reorth_onepass<false><<<nblk, 256, 0, s>>>(T);
     k_reduce_columns<<<T.num_probs * (m + 1), 256, 0, s>>>(T);
-    k_reorth_apply<<<nblk, 256, sizeof(double) * (m + 1), s>>>(T);
+    if (two_pass) k_reorth_apply<<<nblk, 256, sizeof(double) * (m + 1), s>>>(T);
     k_alpha<<<T.num_probs, 256, 0, s>>>(T);
     mark(2, 1);
     mark(1, 0);
@@ -2149,6 +2382,9 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
         mark(4, 1);
     }
     mark(4, 0);
+    // One-pass problems whose sweep ended: their last column, before the
+    // restart or final product reads it.
+    if (one_pass) k_reorth_onepass<true><<<nblk, 256, 0, s>>>(T);
     const size_t mfma_lds = gemm_lds_bytes(m), stage_lds = gemm_stage_lds_bytes(m);
     const bool mfma = m <= 4 * kMfmaMaxSteps && (int64_t)mfma_lds <= T.max_lds;
     if (mfma && (int64_t)stage_lds <= T.max_lds)

@@ -59,7 +59,7 @@ inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 // Per-problem workspace: one allocation carved into these buffers.
 struct WsLayout {
-    size_t U0, U1, V0, V1, u32, P, red, sq, tpart, grp, B, sd, rem, vs, W, Jv,
+    size_t U0, U1, V0, V1, R0, R1, u32, P, red, sq, tpart, grp, B, sd, rem, vs, W, Jv,
         Ub, Sb, vw, vdot, vsq, indptr64, idx, data4, total;
     size_t idx_bytes;  // bytes per packed column index (2, or 4 when g > 65535)
     int64_t nnz4;  // nnz rounded up to whole quads of 4 elements
@@ -74,6 +74,16 @@ struct WsLayout {
 
 // Whether a problem with g genes takes the gene-sorted X^T u.
 bool uses_sorted(int g, bool enabled) { return enabled && g <= bpca::kMaxSortedGenes; }
+
+// Whether a problem reorthogonalises in one pass over U per Lanczos step
+// (kernels.hip, k_reorth_onepass).  Its gene-space X^T U carries rounding
+// forward by about sqrt(g / n) per step, so it needs n >= 8 g; passes = 0
+// (automatic) also needs n >= kOnePassMinCells, below which U is small and
+// the two-pass kernels cost little; passes = 1: whenever n >= 8 g; 2: never.
+bool uses_onepass(int64_t n, int g, int passes) {
+    if (passes == 2 || n < 8 * (int64_t)g) return false;
+    return passes == 1 || n >= bpca::kOnePassMinCells;
+}
 
 WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz, bool sorted) {
     const int64_t nb = bpca::ceil_div(n, bpca::kRowBlock);
@@ -90,7 +100,11 @@ WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz, bool sorted
     L.U1 = take(sizeof(double) * n * m);
     L.V0 = take(sizeof(double) * g * (m + 1));
     L.V1 = take(sizeof(double) * g * (m + 1));
-    L.u32 = take(sizeof(float) * (n + bpca::kWave));  // + a sink slot for masked lanes at [n]
+    L.R0 = take(sizeof(double) * g * (m + 1));
+    L.R1 = take(sizeof(double) * g * (m + 1));
+    // Two step vectors (current and previous), each with a sink slot for
+    // masked lanes at [n].
+    L.u32 = take(sizeof(float) * 2 * (n + bpca::kWave));
     L.P = take(sizeof(double) * (m + 1) * nb);
     L.red = take(sizeof(double) * (m + 1));
     L.sq = take(sizeof(double) * 2 * nb);
@@ -250,6 +264,7 @@ struct bpca_context {
     int spmv_wg = 0, spmvt_wg = 0;         // autotuned workgroups-per-CU caps
     int64_t lds_per_cu = 0;                // LDS bytes of one compute unit
     bool sorted_xtu = true;                // gene-sorted X^T u for g <= kMaxSortedGenes
+    int reorth_passes = 0;                 // 0 automatic, 1 one pass (n >= 8 g), 2 two passes
 };
 
 namespace {
@@ -353,7 +368,7 @@ void begin_call(bpca_context *ctx) {
 
 // The descriptor of problem `pb` whose workspace (layout L) starts at base.
 bpca::Prob make_prob(const bpca_problem *pb, const WsLayout &L, char *base,
-                     bool widen, int id) {
+                     bool widen, int id, int passes) {
     const int64_t n = (int64_t)pb->num_cells;
     bpca::Prob d{};
     d.data = (const float *)(base + L.data4);
@@ -376,6 +391,8 @@ bpca::Prob make_prob(const bpca_problem *pb, const WsLayout &L, char *base,
     d.U[1] = (double *)(base + L.U1);
     d.V[0] = (double *)(base + L.V0);
     d.V[1] = (double *)(base + L.V1);
+    d.R[0] = (double *)(base + L.R0);
+    d.R[1] = (double *)(base + L.R1);
     d.u32 = (float *)(base + L.u32);
     d.P = (double *)(base + L.P);
     d.red = (double *)(base + L.red);
@@ -393,6 +410,7 @@ bpca::Prob make_prob(const bpca_problem *pb, const WsLayout &L, char *base,
     d.vw = (double *)(base + L.vw);
     d.vdot = (double *)(base + L.vdot);
     d.vsq = (double *)(base + L.vsq);
+    d.onepass = uses_onepass(n, d.g, passes) ? 1 : 0;
     d.tsorted = L.tsorted ? 1 : 0;
     if (L.tsorted) {
         d.tval = (const float *)(base + L.tval);
@@ -443,12 +461,13 @@ void engine_drop_graph(bpca_engine *E) {
 int engine_rebuild_tables(bpca_engine *E) {
     bpca_context *ctx = E->ctx;
     std::vector<int2> blocks, groups;
-    int max_g = 2, max_m = 1, modes = 0, tmodes = 0;
+    int max_g = 2, max_m = 1, modes = 0, tmodes = 0, rmodes = 0;
     for (int s = 0; s < (int)E->slots.size(); s++) {
         const Slot &sl = E->slots[s];
         if (!sl.used) continue;
         modes |= 1 << sl.desc.mode;
         tmodes |= sl.desc.tsorted ? bpca::kSortedBit : 1 << sl.desc.mode;
+        rmodes |= sl.desc.onepass ? bpca::kReorthOneBit : bpca::kReorthTwoBit;
         for (int64_t b = 0; b < sl.desc.nb; b++) blocks.push_back(make_int2(s, (int)b));
         for (int64_t G = 0; G < sl.desc.ngrp; G++) groups.push_back(make_int2(s, (int)G));
         max_g = std::max(max_g, sl.desc.g);
@@ -482,6 +501,7 @@ int engine_rebuild_tables(bpca_engine *E) {
     E->T.max_m = max_m;
     E->T.modes = modes;
     E->T.tmodes = tmodes;
+    E->T.rmodes = rmodes;
     E->T.max_lds = (int64_t)ctx->max_lds;
     E->T.spmv_group = ctx->spmv_group;
     E->T.spmvt_group = ctx->spmvt_group;
@@ -718,6 +738,21 @@ int bpca_context_get_sorted_xtu(bpca_context *ctx, int32_t *enable) {
     return BPCA_OK;
 }
 
+int bpca_context_set_reorth_passes(bpca_context *ctx, int32_t passes) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (passes < 0 || passes > 2)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "passes must be 0 (automatic), 1 or 2");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    ctx->reorth_passes = passes;
+    return BPCA_OK;
+}
+
+int bpca_context_get_reorth_passes(bpca_context *ctx, int32_t *passes) {
+    if (!ctx || !passes) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL argument");
+    *passes = ctx->reorth_passes;
+    return BPCA_OK;
+}
+
 int bpca_context_get_occupancy(bpca_context *ctx, int32_t *spmv_wg,
                                int32_t *spmvt_wg) {
     if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
@@ -923,7 +958,7 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
         pool_free(ctx, ws);
         return rc;
     }
-    const bpca::Prob d = make_prob(pb, L, (char *)ws, widen, slot);
+    const bpca::Prob d = make_prob(pb, L, (char *)ws, widen, slot, ctx->reorth_passes);
     char *base = (char *)ws;
     const std::vector<double> v0 = start_vector(pb->seed, g);
     hipError_t e = hipSuccess;
@@ -1026,6 +1061,8 @@ int bpca_engine_collect(bpca_engine *E, int32_t max_out, int32_t *slots,
             st.svd_sweeps = cur[i].svd_sweeps;
             st.slot = i;
             st.smax = cur[i].smax;
+            st.reorth_passes = cur[i].onepass ? 1 : 2;
+            st.reserved = 0;
         }
         if (slots) slots[nout] = i;
         nout++;
@@ -1267,7 +1304,7 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
     pb.tolerance = tolerance;
     pb.seed = seed;
     pb.PCs = d_PCs;
-    const bpca::Prob d = make_prob(&pb, L, ws, widen, 0);
+    const bpca::Prob d = make_prob(&pb, L, ws, widen, 0, ctx->reorth_passes);
     const std::vector<double> v0 = start_vector(seed, g);
     std::vector<int2> blocks((size_t)nb), groups((size_t)ngrp);
     for (int64_t b = 0; b < nb; b++) blocks[b] = make_int2(0, (int)b);
@@ -1292,6 +1329,7 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
     T.max_m = std::max(m, 1);
     T.modes = 1 << d.mode;
     T.tmodes = d.tsorted ? bpca::kSortedBit : 1 << d.mode;
+    T.rmodes = d.onepass ? bpca::kReorthOneBit : bpca::kReorthTwoBit;
     T.max_lds = (int64_t)ctx->max_lds;
     T.spmv_group = ctx->spmv_group;
     T.spmvt_group = ctx->spmvt_group;
@@ -1331,6 +1369,8 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
         stats->svd_sweeps = fin.svd_sweeps;
         stats->slot = 0;
         stats->smax = fin.smax;
+        stats->reorth_passes = fin.onepass ? 1 : 2;
+        stats->reserved = 0;
     }
     return BPCA_OK;
 }

@@ -160,6 +160,18 @@ class Context:
         _lib.check(_lib.lib().bpca_context_get_sorted_xtu(self._h, ctypes.byref(a)))
         return bool(a.value)
 
+    def set_reorth_passes(self, passes: int) -> None:
+        """Reorthogonalisation of problems admitted from now on: 2 = two
+        passes over U per Lanczos step (dots, then the update), 1 = one pass
+        wherever num_cells >= 8 num_genes, 0 (default) = one pass where
+        num_cells >= max(8 num_genes, 65536).  Results agree to rounding."""
+        _lib.check(_lib.lib().bpca_context_set_reorth_passes(self._h, int(passes)))
+
+    def reorth_passes(self) -> int:
+        a = ctypes.c_int32(0)
+        _lib.check(_lib.lib().bpca_context_get_reorth_passes(self._h, ctypes.byref(a)))
+        return int(a.value)
+
     def set_occupancy(self, spmv_wg: int, spmvt_wg: int) -> None:
         """At most this many workgroups per compute unit for X v / X^T u
         (0 = no cap; speed only)."""

@@ -126,6 +126,17 @@ int bpca_context_get_occupancy(bpca_context *ctx, int32_t *spmv_wg,
  * data-layout knob of this library). */
 int bpca_context_set_sorted_xtu(bpca_context *ctx, int32_t enable);
 int bpca_context_get_sorted_xtu(bpca_context *ctx, int32_t *enable);
+/* How problems admitted from now on reorthogonalise each new left Lanczos
+ * vector (the reference's classical Gram-Schmidt, pca.pyx:997-1090):
+ * 2 = two passes over U per step (dots, then the update), as the reference;
+ * 1 = one pass per step wherever num_cells >= 8 num_genes (the update of a
+ * step is made in the next step's pass over U, and X^T U is carried in gene
+ * space); 0 (the default) = one pass where num_cells >= max(8 num_genes,
+ * 65536), else two.  The two agree to rounding, not bit for bit; a problem
+ * keeps the choice it was admitted with.  No counterpart in the reference
+ * (a memory-traffic knob of this library). */
+int bpca_context_set_reorth_passes(bpca_context *ctx, int32_t passes);
+int bpca_context_get_reorth_passes(bpca_context *ctx, int32_t *passes);
 /* Bytes of device workspace a solve of this shape allocates. */
 uint64_t bpca_workspace_bytes(uint64_t num_cells, uint32_t num_genes,
                               uint32_t k, uint32_t subspace_size);
@@ -140,6 +151,8 @@ typedef struct bpca_solve_stats {
     int32_t svd_sweeps;     /* Jacobi sweeps of the last small SVD */
     int32_t slot;           /* engine slot the problem ran in */
     double smax;            /* largest Ritz value seen (convergence scale) */
+    int32_t reorth_passes;  /* passes over U per Lanczos step (1 or 2) */
+    int32_t reserved;
 } bpca_solve_stats;
 
 /* ------------------------------------------------------- single solve */
