@@ -39,6 +39,16 @@ __device__ inline double wave_sum(double v) {
     return v;
 }
 
+// wave_sum with the lane exchanges of strides 16 and below done by DPP moves
+// and a swizzle (xor_lane_f64) instead of LDS permutes: the same pairs in the
+// same order, so the same bits.  Every lane of the wave must be active.
+__device__ inline double wave_sum_dpp(double v) {
+    v += __shfl_xor(v, 32);
+#pragma unroll
+    for (int off = 16; off > 0; off >>= 1) v += xor_lane_f64(v, off);
+    return v;
+}
+
 // Sum over a block of NW waves: xor butterfly inside each wave, then a
 // pairwise tree over the wave sums in wave order.  `red` holds NW doubles.
 template <int NW>
@@ -485,7 +495,7 @@ __global__ __launch_bounds__(256) void k_reorth_partials(Tables T) {
             acc += ua[t];
             acc += ub[t];
         }
-        acc = wave_sum(acc);
+        acc = wave_sum_dpp(acc);
         if (lane == 0) pr.P[b] = acc;
         c0 = 4;
     }
@@ -508,7 +518,7 @@ __global__ __launch_bounds__(256) void k_reorth_partials(Tables T) {
                 acc = fma(x[t].x, ua[t], acc);
                 acc = fma(x[t].y, ub[t], acc);
             }
-            acc = wave_sum(acc);
+            acc = wave_sum_dpp(acc);
             if (lane == 0) pr.P[(int64_t)c * pr.nb + b] = acc;
             if (two) {
 #pragma unroll
@@ -516,7 +526,7 @@ __global__ __launch_bounds__(256) void k_reorth_partials(Tables T) {
                     acd = fma(y[t].x, ua[t], acd);
                     acd = fma(y[t].y, ub[t], acd);
                 }
-                acd = wave_sum(acd);
+                acd = wave_sum_dpp(acd);
                 if (lane == 0) pr.P[(int64_t)(c + 4) * pr.nb + b] = acd;
             }
         }
@@ -533,7 +543,7 @@ __global__ __launch_bounds__(256) void k_reorth_partials(Tables T) {
             acc = fma(x0, ua[t], acc);
             acc = fma(x1, ub[t], acc);
         }
-        acc = wave_sum(acc);
+        acc = wave_sum_dpp(acc);
         if (lane == 0) pr.P[(int64_t)c * pr.nb + b] = acc;
     }
 }
@@ -643,8 +653,9 @@ __global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
 // (U orthonormal), and X^T U[:, j] is formed in gene space (k_vside_w) from
 // X^T (u_j - mean) and R = X^T U[:, :j].  A column is pending when j >
 // jfirst (the first step of a sweep reads the restarted U, all of it
-// written); at the end of a sweep, kEnd, the last column is completed alone
-// (phase PH_RESTARTING or PH_FINAL) before the restart or final product.
+// written).  The last column of a sweep is never completed by a pass: the
+// restart and final products take it from (u - mean) / alpha, written by
+// X^T u at that step, through their coefficients (restart_coef).
 // The arithmetic per element is the two-pass one, in another order; only the
 // gene-space X^T U carries the previous columns' FP32 rounding forward, by a
 // factor |h| / alpha per step, about sqrt(g / n) -- hence n >= 8 g.
@@ -655,7 +666,6 @@ __global__ __launch_bounds__(256) void k_reorth_apply(Tables T) {
 // U h of its 16 rows summed over its wave's columns in ascending order; the
 // four waves' corrections meet in LDS as (w0 + w1) + (w2 + w3); thread t
 // then completes rows (2 t, 2 t + 1) and (512 + 2 t, 513 + 2 t).
-template <bool kEnd>
 __global__ __launch_bounds__(256) void k_reorth_onepass(Tables T) {
     constexpr int R = kRowBlock / (2 * kWave);  // 8 row pairs per lane
     __shared__ double cbuf[4][kRowBlock];
@@ -663,12 +673,12 @@ __global__ __launch_bounds__(256) void k_reorth_onepass(Tables T) {
     const int2 bp = T.blocks[blockIdx.x];
     Prob &pr = T.probs[bp.x];
     if (!pr.onepass) return;
-    if (kEnd ? (pr.phase != PH_RESTARTING && pr.phase != PH_FINAL) : pr.phase != PH_LANCZOS) return;
+    if (pr.phase != PH_LANCZOS) return;
     const int lane = threadIdx.x & (kWave - 1);
     const int w = wave_id();
     const int j = pr.j;
-    const bool pend = kEnd || j > pr.jfirst;
-    const int pc = kEnd ? j : j - 1;   // the pending column
+    const bool pend = j > pr.jfirst;
+    const int pc = j - 1;              // the pending column
     const int nm = pend ? pc : j;      // columns read: U[:, :nm]
     const int64_t n = pr.n, b = bp.y, r0 = b * kRowBlock, nb = pr.nb;
     const int nrows = (int)(n - r0 < kRowBlock ? n - r0 : kRowBlock);
@@ -680,28 +690,26 @@ __global__ __launch_bounds__(256) void k_reorth_onepass(Tables T) {
 #pragma unroll
     for (int t = 0; t < R; t++) {
         const int r = 2 * lane + t * 2 * kWave;
-        ua[t] = !kEnd && r < nrows ? (double)ucur[r0 + r] : 0.0;
-        ub[t] = !kEnd && r + 1 < nrows ? (double)ucur[r0 + r + 1] : 0.0;
+        ua[t] = r < nrows ? (double)ucur[r0 + r] : 0.0;
+        ub[t] = r + 1 < nrows ? (double)ucur[r0 + r + 1] : 0.0;
         ca[t] = 0.0;
         cb[t] = 0.0;
     }
     int c0 = w;
     if (w == 0) {
-        if (!kEnd) {
-            double s = 0.0, ss = 0.0;
+        double s = 0.0, ss = 0.0;
 #pragma unroll
-            for (int t = 0; t < R; t++) {
-                s += ua[t];
-                s += ub[t];
-                ss = fma(ua[t], ua[t], ss);
-                ss = fma(ub[t], ub[t], ss);
-            }
-            s = wave_sum(s);
-            ss = wave_sum(ss);
-            if (lane == 0) {
-                pr.P[b] = s;
-                pr.sq[b] = ss;
-            }
+        for (int t = 0; t < R; t++) {
+            s += ua[t];
+            s += ub[t];
+            ss = fma(ua[t], ua[t], ss);
+            ss = fma(ub[t], ub[t], ss);
+        }
+        s = wave_sum_dpp(s);
+        ss = wave_sum_dpp(ss);
+        if (lane == 0) {
+            pr.P[b] = s;
+            pr.sq[b] = ss;
         }
         c0 = 4;
     }
@@ -717,16 +725,14 @@ __global__ __launch_bounds__(256) void k_reorth_onepass(Tables T) {
 #pragma unroll
                 for (int t = 0; t < R; t++) y[t] = __builtin_nontemporal_load((const v2d *)(Ud + t * 2 * kWave));
             }
-            if (!kEnd) {
-                double acc = 0.0;
+            double acc = 0.0;
 #pragma unroll
-                for (int t = 0; t < R; t++) {
-                    acc = fma(x[t].x, ua[t], acc);
-                    acc = fma(x[t].y, ub[t], acc);
-                }
-                acc = wave_sum(acc);
-                if (lane == 0) pr.P[(int64_t)c * nb + b] = acc;
+            for (int t = 0; t < R; t++) {
+                acc = fma(x[t].x, ua[t], acc);
+                acc = fma(x[t].y, ub[t], acc);
             }
+            acc = wave_sum_dpp(acc);
+            if (lane == 0) pr.P[(int64_t)c * nb + b] = acc;
             if (pend) {
                 const double hc = h[c];
 #pragma unroll
@@ -736,16 +742,14 @@ __global__ __launch_bounds__(256) void k_reorth_onepass(Tables T) {
                 }
             }
             if (two) {
-                if (!kEnd) {
-                    double acd = 0.0;
+                double acd = 0.0;
 #pragma unroll
-                    for (int t = 0; t < R; t++) {
-                        acd = fma(y[t].x, ua[t], acd);
-                        acd = fma(y[t].y, ub[t], acd);
-                    }
-                    acd = wave_sum(acd);
-                    if (lane == 0) pr.P[(int64_t)(c + 4) * nb + b] = acd;
+                for (int t = 0; t < R; t++) {
+                    acd = fma(y[t].x, ua[t], acd);
+                    acd = fma(y[t].y, ub[t], acd);
                 }
+                acd = wave_sum_dpp(acd);
+                if (lane == 0) pr.P[(int64_t)(c + 4) * nb + b] = acd;
                 if (pend) {
                     const double hd = h[c + 4];
 #pragma unroll
@@ -771,10 +775,8 @@ __global__ __launch_bounds__(256) void k_reorth_onepass(Tables T) {
                 ca[t] = fma(x0, hc, ca[t]);
                 cb[t] = fma(x1, hc, cb[t]);
             }
-            if (!kEnd) {
-                acc = wave_sum(acc);
-                if (lane == 0) pr.P[(int64_t)c * nb + b] = acc;
-            }
+            acc = wave_sum_dpp(acc);
+            if (lane == 0) pr.P[(int64_t)c * nb + b] = acc;
         }
     }
     if (!pend) return;
@@ -798,7 +800,7 @@ __global__ __launch_bounds__(256) void k_reorth_onepass(Tables T) {
             const int re = r + e;
             const double cr = (cbuf[0][re] + cbuf[1][re]) + (cbuf[2][re] + cbuf[3][re]);
             x[e] = re < nrows ? ((double)uprev[r0 + re] - mean - cr) * ia : 0.0;
-            uc[e] = !kEnd && re < nrows ? (double)ucur[r0 + re] : 0.0;
+            uc[e] = re < nrows ? (double)ucur[r0 + re] : 0.0;
         }
         if (vec) {
             *(v2d *)(Up + r) = v2d{x[0], x[1]};
@@ -810,7 +812,6 @@ __global__ __launch_bounds__(256) void k_reorth_onepass(Tables T) {
         d = fma(x[0], uc[0], d);
         d = fma(x[1], uc[1], d);
     }
-    if (kEnd) return;
     d = block_sum<4>(d, tmp);
     if (threadIdx.x == 0) pr.P[(int64_t)j * nb + b] = d;
 }
@@ -831,16 +832,19 @@ __global__ __launch_bounds__(256) void k_alpha(Tables T) {
         if (!one) sum += pr.sq[pr.nb + b];
     }
     acc = block_sum<4>(acc, tmp);
-    if (!one) sum = block_sum<4>(sum, tmp);
+    double hh = 0.0;
+    if (one) {
+        // |h|^2: thread t takes red[1 + t], red[257 + t], ...
+        for (int c = 1 + threadIdx.x; c <= pr.j; c += 256) hh = fma(pr.red[c], pr.red[c], hh);
+        hh = block_sum<4>(hh, tmp);
+    } else {
+        sum = block_sum<4>(sum, tmp);
+    }
     if (threadIdx.x == 0) {
         if (one) sum = pr.red[0];
         const double mean = sum / (double)pr.n;
         double ss = acc - sum * mean;
-        if (one) {
-            double hh = 0.0;
-            for (int c = 1; c <= pr.j; c++) hh = fma(pr.red[c], pr.red[c], hh);
-            ss -= hh;
-        }
+        if (one) ss -= hh;
         double alpha = sqrt(ss > 0.0 ? ss : 0.0);
         if (alpha < 1e-8) alpha = 1e-8;
         pr.alpha = alpha;
@@ -917,9 +921,16 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T, int c0) {
     double *Uj = pr.U[pr.ubuf] + (int64_t)pr.j * pr.n;
     const double inv_alpha = pr.inv_alpha, umean = pr.umean;
     if (pr.onepass) {
-        // u - mean of this step; U[:, j] is written by the next pass.
+        // u - mean of this step; U[:, j] is written by the next pass, except
+        // at the last step of a sweep, whose column the restart and final
+        // products form from (u - mean) / alpha (restart_coef).
         const float *uj = u_of_step(pr, pr.j);
-        for (int t = lane; t < nrows; t += kWave) uval[t] = (float)((double)uj[r0 + t] - umean);
+        const bool last = pr.j == pr.m - 1 && c0 == 0;
+        for (int t = lane; t < nrows; t += kWave) {
+            const double x = (double)uj[r0 + t] - umean;
+            uval[t] = (float)x;
+            if (last) Uj[r0 + t] = x * inv_alpha;
+        }
     } else if (c0 == 0) {
         for (int t = lane; t < nrows; t += kWave) {
             const double x = (Uj[r0 + t] - umean) * inv_alpha;
@@ -1148,13 +1159,18 @@ __global__ __launch_bounds__(256) void k_spmv_t_sorted(Tables T) {
     double *Uj = pr.U[pr.ubuf] + (int64_t)pr.j * pr.n;
     const double inv_alpha = pr.inv_alpha, umean = pr.umean;
     const bool one = pr.onepass != 0;
+    const bool last = pr.j == pr.m - 1;
     const float *uj = u_of_step(pr, pr.j);
     for (int t = lane; t < kSubRows; t += kWave) {
         float x = 0.f;
         if (t < nrows) {
             if (one) {
-                // u - mean of this step; U[:, j] is written by the next pass.
-                x = (float)((double)uj[r0 + t] - umean);
+                // u - mean of this step; U[:, j] is written by the next pass
+                // (at the last step of a sweep: (u - mean) / alpha, for
+                // restart_coef).
+                const double d = (double)uj[r0 + t] - umean;
+                x = (float)d;
+                if (last) Uj[r0 + t] = d * inv_alpha;
             } else {
                 const double d = (Uj[r0 + t] - umean) * inv_alpha;
                 Uj[r0 + t] = d;
@@ -1450,8 +1466,8 @@ hipError_t launch_tsort_build(const float *data4, const uint16_t *idx16, const i
 //
 // Four short launches over genes (the V side is g long, so one workgroup per
 // problem would leave the step latency-bound):
-//   k_vside_w    w = (sum of the group partials in group order) / sd
-//                    - alpha V[:, j]                     (FP64, per gene;
+//   k_vside_w    w = (sum of the group partials, in four ordered quarters)
+//                    / sd - alpha V[:, j]                (FP64, per gene;
 //                one pass: the sum is X^T (u - mean), corrected to X^T U[:, j]
 //                first)
 //   k_vside_dot  dot_c = V[:, c] . w, thread-strided over genes then
@@ -1463,24 +1479,49 @@ hipError_t launch_tsort_build(const float *data4, const uint16_t *idx16, const i
 
 constexpr int kVChunk = 256;  // genes per V-side workgroup
 
-__global__ __launch_bounds__(kVChunk) void k_vside_w(Tables T) {
-    const int chunks = (T.max_g + kVChunk - 1) / kVChunk;
+// k_vside_w: 64 genes per workgroup, lane l on gene 64 b + l; wave w sums
+// the quarter [w G / 4, (w + 1) G / 4) of the G group partials in order and,
+// one pass, the quarter of the correction's columns; the quarters meet as
+// (q0 + q1) + (q2 + q3).
+constexpr int kVwGenes = 64;
+
+__global__ __launch_bounds__(256) void k_vside_w(Tables T) {
+    __shared__ double part[2][4][kVwGenes];
+    const int chunks = (T.max_g + kVwGenes - 1) / kVwGenes;
     Prob &pr = T.probs[blockIdx.x / chunks];
     if (pr.phase != PH_LANCZOS) return;
     const int g = pr.g;
-    const int i = (blockIdx.x % chunks) * kVChunk + threadIdx.x;
-    if (i >= g) return;
-    double acc = 0.0;
-    for (int64_t G = 0; G < pr.ngrp; G++) acc += pr.grp[G * 2 * g + i];
+    const int lane = threadIdx.x & (kWave - 1), w = wave_id();
+    const int i = (blockIdx.x % chunks) * kVwGenes + lane;
+    if ((blockIdx.x % chunks) * kVwGenes >= g) return;
     const int j = pr.j;
-    if (pr.onepass) {
+    const bool one = pr.onepass != 0;
+    double s = 0.0, corr = 0.0;
+    if (i < g) {
+        const int64_t ng = pr.ngrp;
+        const int64_t G0 = ng * w / 4, G1 = ng * (w + 1) / 4;
+        const double *gp = pr.grp + i;
+#pragma unroll 4
+        for (int64_t G = G0; G < G1; G++) s += gp[G * 2 * g];
+        if (one) {
+            // X^T U[:, :j] h, h = red[1..j], c ascending within the quarter.
+            const double *R = pr.R[pr.ubuf] + i;
+            const int c0 = j * w / 4, c1 = j * (w + 1) / 4;
+#pragma unroll 4
+            for (int c = c0; c < c1; c++) corr = fma(R[(int64_t)c * g], pr.red[c + 1], corr);
+        }
+    }
+    part[0][w][lane] = s;
+    part[1][w][lane] = corr;
+    __syncthreads();
+    if (w != 0 || i >= g) return;
+    double acc = (part[0][0][lane] + part[0][1][lane]) + (part[0][2][lane] + part[0][3][lane]);
+    if (one) {
         // acc = X^T (u_j - mean); X^T U[:, j] = (acc - X^T U[:, :j] h) / alpha,
-        // h = red[1..j] (c ascending), kept as column j of R.
-        double *R = pr.R[pr.ubuf];
-        double corr = 0.0;
-        for (int c = 0; c < j; c++) corr = fma(R[(int64_t)c * g + i], pr.red[c + 1], corr);
-        acc = (acc - corr) * pr.inv_alpha;
-        R[(int64_t)j * g + i] = acc;
+        // kept as column j of R.
+        const double c = (part[1][0][lane] + part[1][1][lane]) + (part[1][2][lane] + part[1][3][lane]);
+        acc = (acc - c) * pr.inv_alpha;
+        pr.R[pr.ubuf][(int64_t)j * g + i] = acc;
     }
     const double *Vj = pr.V[pr.vbuf] + (int64_t)j * g;
     pr.vw[i] = acc / pr.sd[i] - pr.alpha * Vj[i];
@@ -1843,6 +1884,19 @@ size_t svd_lds_bytes(int m, bool resident) {
 
 // ================================================================ restart
 //
+// Coefficient (l, c) of the restart product U Ub[:, :kc] and of the final
+// product U (Ub[:, :k] Sb) (times Sb[c] there).  For a one-pass problem the
+// sweep's last column of U holds p = (u - mean) / alpha of its last step, not
+// the column itself, p - U[:, :m-1] h / alpha (h = red[1..m-1] of that
+// step): its second part goes into the other columns' coefficients,
+// Ub[l, c] - (h[l] / alpha) Ub[m-1, c], so no pass over U completes it.
+__device__ __forceinline__ double restart_coef(const Prob &pr, int l, int c) {
+    const int m = pr.m;
+    double v = pr.Ub[l + (int64_t)c * m];
+    if (pr.onepass && l < m - 1) v = fma(-pr.red[l + 1] * pr.inv_alpha, pr.Ub[(m - 1) + (int64_t)c * m], v);
+    return v;
+}
+//
 // out[r, c] = sum_l U[r, l] coef[l, c] with the sum over l ascending; one
 // thread per row, kGemmCols output columns per pass, the coefficient slab in
 // LDS.  Used for U <- U Ub[:, :kc] and for PCs = U (Ub[:, :k] Sb[:k]).
@@ -1866,7 +1920,7 @@ __global__ __launch_bounds__(256) void k_tall_gemm(Tables T) {
             const int l = i / kGemmCols, cc = i % kGemmCols;
             double v = 0.0;
             if (cc < nc) {
-                v = pr.Ub[l + (int64_t)(c0 + cc) * m];
+                v = restart_coef(pr, l, c0 + cc);
                 if (kFinal) v *= pr.Sb[c0 + cc];
             }
             cs[i] = v;
@@ -2074,7 +2128,7 @@ __global__ __launch_bounds__(kGemmThreads) void k_tall_gemm_mfma(Tables T) {
         const int l = idx / npad, c = idx % npad;
         double v = 0.0;
         if (l < m && c < kout) {
-            v = pr.Ub[l + (int64_t)c * m];
+            v = restart_coef(pr, l, c);
             if (kFinal) v *= pr.Sb[c];
         }
         cs[l * ldc + c] = v;
@@ -2354,7 +2408,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     mark(2, 0);
     const bool two_pass = (T.rmodes & kReorthTwoBit) != 0, one_pass = (T.rmodes & kReorthOneBit) != 0;
     if (two_pass) k_reorth_partials<<<nblk, 256, 0, s>>>(T);
-    if (one_pass) k_reorth_onepass<false><<<nblk, 256, 0, s>>>(T);
+    if (one_pass) k_reorth_onepass<<<nblk, 256, 0, s>>>(T);
     k_reduce_columns<<<T.num_probs * (m + 1), 256, 0, s>>>(T);
     if (two_pass) k_reorth_apply<<<nblk, 256, sizeof(double) * (m + 1), s>>>(T);
     k_alpha<<<T.num_probs, 256, 0, s>>>(T);
@@ -2365,7 +2419,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     mark(3, 0);
     k_reduce_groups<<<dim3((unsigned)ceil_div(g, 256), (unsigned)T.num_groups), 256, 0, s>>>(T, 0);
     const unsigned vchunks = (unsigned)ceil_div(g, kVChunk);
-    k_vside_w<<<T.num_probs * vchunks, kVChunk, 0, s>>>(T);
+    k_vside_w<<<T.num_probs * (unsigned)ceil_div(g, kVwGenes), 256, 0, s>>>(T);
     k_vside_dot<<<T.num_probs * (unsigned)m, 256, 0, s>>>(T);
     k_vside_upd<<<T.num_probs * vchunks, kVChunk, 0, s>>>(T);
     k_vside_fin<<<T.num_probs, 256, 0, s>>>(T);
@@ -2382,9 +2436,6 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
         mark(4, 1);
     }
     mark(4, 0);
-    // One-pass problems whose sweep ended: their last column, before the
-    // restart or final product reads it.
-    if (one_pass) k_reorth_onepass<true><<<nblk, 256, 0, s>>>(T);
     const size_t mfma_lds = gemm_lds_bytes(m), stage_lds = gemm_stage_lds_bytes(m);
     const bool mfma = m <= 4 * kMfmaMaxSteps && (int64_t)mfma_lds <= T.max_lds;
     if (mfma && (int64_t)stage_lds <= T.max_lds)
