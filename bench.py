@@ -35,6 +35,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+FP64_PEAK_TFLOPS = 78.6   # MI355X dense FP64 (matrix and vector), published
 
 
 def algorithmic_bytes(nnz: int, n: int, g: int) -> int:
@@ -582,13 +583,62 @@ def bench_copy(args):
           flush=True)
 
 
+def bench_fp64(args):
+    """The FP64 products of a restart (U Ub[:, :kc], kc = 99 and 92) and of
+    the final step (kout = 50) at the headline shape (10M cells, m = 100) on
+    device arrays, priced against the published FP64 matrix peak and against
+    the FP64 rates this device sustains with register operands only
+    (bpca_test_fp64_rate: v_mfma_f64_16x16x4f64 and v_fma_f64)."""
+    import ctypes
+
+    import torch
+    from brisc_amd import _lib, default_context
+
+    ctx = default_context(0)
+    L = _lib.lib()
+    rates = {}
+    for which, name in ((0, 'mfma_f64_16x16x4'), (1, 'valu_fma_f64')):
+        t = ctypes.c_double(0.0)
+        _lib.check(L.bpca_test_fp64_rate(ctx.handle, which, 5, ctypes.byref(t)))
+        rates[name] = round(t.value, 1)
+    log(f'# FP64 rates with register operands: {rates}')
+    n, m = 10_000_000, 100
+    A = torch.randn(n * m, dtype=torch.float64, device='cuda')
+    products = {}
+    for kout in (99, 92, 50):
+        C = torch.randn(m * kout, dtype=torch.float64, device='cuda')
+        O = torch.empty(n * kout, dtype=torch.float64, device='cuda')
+        call = lambda: _lib.check(L.bpca_test_gemm_device(ctx.handle, n, m, kout, A.data_ptr(),
+                                                          C.data_ptr(), O.data_ptr()))
+        call()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            call()
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        tf = 2.0 * n * m * kout / (ms * 1e-3) / 1e12
+        products[f'kout{kout}'] = {
+            'ms': round(ms, 3), 'tflops': round(tf, 1),
+            'frac_of_published_78.6': round(tf / FP64_PEAK_TFLOPS, 3),
+            'frac_of_measured_mfma': round(tf / rates['mfma_f64_16x16x4'], 3),
+            'hbm_bytes': 8 * n * m + 8 * n * kout}
+        log(f'# product n={n} m={m} kout={kout}: {products[f"kout{kout}"]}')
+        del C, O
+    print(json.dumps({'metric': 'FP64 restart / final products (10M x 100)',
+                      'unit': 'TFLOP/s', 'fp64_published_peak_tflops': FP64_PEAK_TFLOPS,
+                      'measured_register_rates_tflops': rates, 'products': products,
+                      'steps': args.steps, 'note': 'host-timed per product, synchronised'}),
+          flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument('--gpus', type=int, default=1)
     ap.add_argument('--steps', type=int, default=3)
     ap.add_argument('--warmup', type=int, default=1)
     ap.add_argument('--mode', choices=('single', 'batched', 'serve', 'copy', 'subset', 'knn',
-                                       'pipeline', 'normalize'),
+                                       'pipeline', 'normalize', 'fp64'),
                     default='single')
     ap.add_argument('--workload', default='cells10m')
     ap.add_argument('--k', type=int, default=50)
@@ -623,6 +673,8 @@ def main():
         return bench_serve(args)
     if args.mode == 'copy':
         return bench_copy(args)
+    if args.mode == 'fp64':
+        return bench_fp64(args)
     if args.mode == 'subset':
         return bench_subset(args)
     if args.mode == 'knn':

@@ -140,6 +140,7 @@ def lib() -> ctypes.CDLL:
         'bpca_test_small_svd': (i32, [p, i32, p, p, p, p, p]),
         'bpca_test_gemm': (i32, [p, i64, i32, i32, p, p, p]),
         'bpca_test_gemm_device': (i32, [p, i64, i32, i32, p, p, p]),
+        'bpca_test_fp64_rate': (i32, [p, i32, i32, ctypes.POINTER(ctypes.c_double)]),
         'bpca_context_set_kernel_config': (i32, [p, i32, i32]),
         'bpca_context_get_kernel_config': (i32, [p, ctypes.POINTER(ctypes.c_int32),
                                                  ctypes.POINTER(ctypes.c_int32)]),

@@ -132,8 +132,10 @@ struct Prob {
     int32_t svd_sweeps, pad0;
     int32_t tsorted, tgl;    // gene-sorted X^T u in use; tlens row stride (g rounded up to 8)
     // One-pass reorthogonalisation (onepass != 0, n >= 8 g): column j of U is
-    // written by step j + 1's pass over U (or at the end of the sweep), and
-    // X^T U[:, j] is formed in gene space from X^T (u - mean) and R.
+    // written by step j + 1's pass over U (the last column of a sweep holds
+    // (u - mean) / alpha instead, which the restart and final coefficients
+    // correct), and X^T U[:, j] is formed in gene space from X^T (u - mean)
+    // and R.
     int32_t onepass;
     int32_t jfirst;          // first step of the current sweep (0, then kc)
     double *R[2];            // float64 g x (m+1) (ld g): X^T U, flipped with U
@@ -231,6 +233,12 @@ hipError_t launch_svd_single(const double *B, int m, double *W, double *Jv,
 hipError_t launch_gemm_mfma_test(const double *A, int64_t n, int m,
                                  const double *C, int kout, double *out,
                                  hipStream_t s);
+// FP64 rate with register operands only (calibration): which = 0 MFMA
+// 16x16x4, 1 VALU fma; `blocks` workgroups of 256 threads, `iters` rounds
+// of 8 independent operations per wave; out: blocks x 256 doubles.
+// Returns the floating-point operations one launch performs in *flops.
+hipError_t launch_fp64_rate(int which, int blocks, int iters, double *out,
+                            double *flops, hipStream_t s);
 
 // QC x HVG subset of a CSR matrix (subset.hip).  Output row r is input row
 // rows[r] (rows = nullptr: all rows); input column c becomes col_map[c] or is

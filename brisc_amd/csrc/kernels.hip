@@ -2205,6 +2205,53 @@ hipError_t launch_gemm_mfma_test(const double *A, int64_t n, int m,
     return hipGetLastError();
 }
 
+// Calibration kernels: each wave runs `iters` rounds of 8 independent FP64
+// operations on registers (MFMA 16x16x4: 2048 flops each; VALU fma: 128
+// per wave), and stores one sum per lane so that nothing is dropped.
+__global__ __launch_bounds__(256) void k_fp64_rate_mfma(double *out, int iters) {
+    v4d acc[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) acc[t] = v4d{0.0, 0.0, 0.0, 0.0};
+    const double a = threadIdx.x * 1e-3, b = blockIdx.x * 1e-3;
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int t = 0; t < 8; t++) acc[t] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[t], 0, 0, 0);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < 8; t++) s += (acc[t][0] + acc[t][1]) + (acc[t][2] + acc[t][3]);
+    out[(int64_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+__global__ __launch_bounds__(256) void k_fp64_rate_valu(double *out, int iters) {
+    double a[8];
+#pragma unroll
+    for (int t = 0; t < 8; t++) a[t] = threadIdx.x * 1e-3 + t;
+    const double x = 1.0000001, y = 1e-9;
+    for (int it = 0; it < iters; it++) {
+#pragma unroll
+        for (int t = 0; t < 8; t++) a[t] = fma(a[t], x, y);
+    }
+    double s = 0.0;
+#pragma unroll
+    for (int t = 0; t < 8; t++) s += a[t];
+    out[(int64_t)blockIdx.x * 256 + threadIdx.x] = s;
+}
+
+hipError_t launch_fp64_rate(int which, int blocks, int iters, double *out,
+                            double *flops, hipStream_t s) {
+    if (blocks < 1 || iters < 1 || (which != 0 && which != 1)) return hipErrorInvalidValue;
+    const double waves = 4.0 * blocks;
+    if (which == 0) {
+        k_fp64_rate_mfma<<<blocks, 256, 0, s>>>(out, iters);
+        *flops = waves * iters * 8 * 2048.0;
+    } else {
+        k_fp64_rate_valu<<<blocks, 256, 0, s>>>(out, iters);
+        *flops = waves * iters * 8 * 128.0;
+    }
+    return hipGetLastError();
+}
+
 // V <- V Vb[:, :kc], V[:, kc] = remainder, vs = remainder / sd (and, one
 // pass, R <- R Ub[:, :kc]).  Grid:
 // (problem, chunk of 64 genes, group of 4 columns); thread t computes gene

@@ -620,6 +620,41 @@ int bpca_test_gemm_device(bpca_context *ctx, int64_t n, int32_t m,
     return BPCA_OK;
 }
 
+int bpca_test_fp64_rate(bpca_context *ctx, int32_t which, int32_t reps,
+                        double *tflops) {
+    if (!ctx || !tflops || reps < 1 || (which != 0 && which != 1))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "bad argument");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    int cus = 0;
+    HIP_TRY(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, ctx->device));
+    const int blocks = cus * 8;   // eight workgroups of four waves per compute unit
+    const int iters = which == 0 ? 2000 : 16000;
+    double *out = nullptr;
+    HIP_TRY(hipMalloc((void **)&out, sizeof(double) * (size_t)blocks * 256));
+    hipEvent_t a = nullptr, z = nullptr;
+    double flops = 0.0, best = 0.0;
+    hipError_t e = hipEventCreate(&a);
+    if (e == hipSuccess) e = hipEventCreate(&z);
+    // One untimed launch, then the best of `reps` timed ones.
+    if (e == hipSuccess) e = bpca::launch_fp64_rate(which, blocks, iters, out, &flops, ctx->stream);
+    for (int r = 0; r < reps && e == hipSuccess; r++) {
+        e = hipEventRecord(a, ctx->stream);
+        if (e == hipSuccess) e = bpca::launch_fp64_rate(which, blocks, iters, out, &flops, ctx->stream);
+        if (e == hipSuccess) e = hipEventRecord(z, ctx->stream);
+        if (e == hipSuccess) e = hipEventSynchronize(z);
+        float ms = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, a, z);
+        if (e == hipSuccess && ms > 0.0f) best = std::max(best, flops / (ms * 1e-3) / 1e12);
+    }
+    if (a) (void)hipEventDestroy(a);
+    if (z) (void)hipEventDestroy(z);
+    (void)hipFree(out);
+    if (e != hipSuccess) return hip_fail(e, "fp64 rate");
+    *tflops = best;
+    return BPCA_OK;
+}
+
 int bpca_context_create(int device, uint64_t hbm_budget_bytes,
                         bpca_context **out) {
     if (!out) return fail(BPCA_ERR_INVALID_ARGUMENT, "out is NULL");

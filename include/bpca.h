@@ -299,6 +299,14 @@ int bpca_test_gemm(bpca_context *ctx, int64_t n, int32_t m, int32_t kout,
 int bpca_test_gemm_device(bpca_context *ctx, int64_t n, int32_t m,
                           int32_t kout, const double *d_A, const double *d_C,
                           double *d_out);
+/* Calibration: the FP64 rate this device sustains on every compute unit with
+ * register operands only (which = 0: v_mfma_f64_16x16x4f64, the restart and
+ * final products' instruction; 1: v_fma_f64 on the vector ALUs), in
+ * TFLOP/s, timed with HIP events over `reps` launches.  No counterpart in
+ * the reference; bench.py prices the FP64 products against it beside the
+ * published peak. */
+int bpca_test_fp64_rate(bpca_context *ctx, int32_t which, int32_t reps,
+                        double *tflops);
 
 /* ------------------------------------------------ synthetic workloads */
 
