@@ -168,6 +168,7 @@ struct Tables {
     int spmv_wg;          // workgroups per CU at most for X v   (0 = no cap)
     int spmvt_wg;         // workgroups per CU at most for X^T u (0 = no cap)
     int64_t lds_per_cu;   // LDS bytes of one compute unit
+    int num_cus;          // compute units (0 = unknown)
 };
 
 // Optional timing hook around each class of launches (null when captured

@@ -140,7 +140,16 @@ __global__ __launch_bounds__(256) void k_reduce_groups(Tables T, int setup) {
     const int64_t b1 = b0 + kGroup < pr.nb ? b0 + kGroup : pr.nb;
     const float *src = pr.tpart + i;
     double acc = 0.0;
-    for (int64_t b = b0; b < b1; b++) acc += (double)src[b * 2 * g];
+    int64_t b = b0;
+    // Sixteen loads in flight, then their sum in block order.
+    for (; b + 16 <= b1; b += 16) {
+        float v[16];
+#pragma unroll
+        for (int t = 0; t < 16; t++) v[t] = src[(b + t) * 2 * g];
+#pragma unroll
+        for (int t = 0; t < 16; t++) acc += (double)v[t];
+    }
+    for (; b < b1; b++) acc += (double)src[b * 2 * g];
     pr.grp[(int64_t)gp.y * 2 * g + i] = acc;
 }
 
@@ -558,6 +567,7 @@ __global__ __launch_bounds__(256) void k_reduce_columns(Tables T) {
     if (pr.phase != PH_LANCZOS || c > pr.j) return;
     const double *p = pr.P + (int64_t)c * pr.nb;
     double acc = 0.0;
+#pragma unroll 8
     for (int64_t b = threadIdx.x; b < pr.nb; b += 256) acc += p[b];
     acc = block_sum<4>(acc, tmp);
     if (threadIdx.x == 0) pr.red[c] = acc;
@@ -827,6 +837,7 @@ __global__ __launch_bounds__(256) void k_alpha(Tables T) {
     if (pr.phase != PH_LANCZOS) return;
     const bool one = pr.onepass != 0;
     double acc = 0.0, sum = 0.0;
+#pragma unroll 8
     for (int64_t b = threadIdx.x; b < pr.nb; b += 256) {
         acc += pr.sq[b];
         if (!one) sum += pr.sq[pr.nb + b];
@@ -2115,47 +2126,72 @@ struct TileStaged {
 
 }  // namespace
 
+// Launched on one workgroup per compute unit (the slab and staging tiles
+// take most of a CU's LDS), each workgroup taking the block list's entries
+// blockIdx.x, + gridDim.x, ...: the slab is staged once per problem it meets
+// rather than once per block, a launch with no problem in its phase returns
+// after one look at the problem table, and the 16-row tiles of consecutive
+// blocks go round the waves without restarting at wave 0 (64 tiles per
+// block over 12 waves would leave four waves one tile behind per block).
 template <bool kFinal, bool kStage = false>
 __global__ __launch_bounds__(kGemmThreads) void k_tall_gemm_mfma(Tables T) {
     extern __shared__ double cs[];
-    const int2 bp = T.blocks[blockIdx.x];
-    Prob &pr = T.probs[bp.x];
-    if (pr.phase != (kFinal ? PH_FINAL : PH_RESTARTING)) return;
-    const int m = pr.m;
-    const int kout = kFinal ? pr.k : pr.kc;
-    const int mpad = (m + 3) & ~3, npad = (kout + 15) & ~15, ldc = slab_ld(npad);
-    for (int idx = threadIdx.x; idx < mpad * npad; idx += kGemmThreads) {
-        const int l = idx / npad, c = idx % npad;
-        double v = 0.0;
-        if (l < m && c < kout) {
-            v = restart_coef(pr, l, c);
-            if (kFinal) v *= pr.Sb[c];
-        }
-        cs[l * ldc + c] = v;
-    }
+    __shared__ int any;
+    const int want = kFinal ? PH_FINAL : PH_RESTARTING;
+    if (threadIdx.x == 0) any = 0;
     __syncthreads();
-    const int64_t n = pr.n;
-    const int64_t row0 = (int64_t)bp.y * kRowBlock;
-    const int64_t row_end = row0 + kRowBlock < n ? row0 + kRowBlock : n;
-    const double *A = pr.U[pr.ubuf];
+    for (int p = threadIdx.x; p < T.num_probs; p += kGemmThreads)
+        if (T.probs[p].phase == want) any = 1;
+    __syncthreads();
+    if (!any) return;
     const int nw = kGemmThreads / kWave;
-    if (kFinal) {
-        float *out = pr.PCs;
-        const int k = pr.k;
-        mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wave_id(), nw,
-                       by_element([&](int64_t row, int col, double v) {
-                           if (row < row_end && col < kout) out[row * k + col] = (float)v;
-                       }));
-    } else if (kStage) {
-        double *st = cs + mpad * ldc + wave_id() * kStageDoubles;
-        mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wave_id(), nw,
-                       TileStaged{pr.U[pr.ubuf ^ 1], n, row_end, kout, st});
-    } else {
-        double *out = pr.U[pr.ubuf ^ 1];
-        mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wave_id(), nw,
-                       by_element([&](int64_t row, int col, double v) {
-                           if (row < row_end && col < kout) out[(int64_t)col * n + row] = v;
-                       }));
+    int cur = -1, rot = 0;
+    for (int64_t item = blockIdx.x; item < T.num_blocks; item += gridDim.x) {
+        const int2 bp = T.blocks[item];
+        Prob &pr = T.probs[bp.x];
+        if (pr.phase != want) continue;
+        const int m = pr.m;
+        const int kout = kFinal ? pr.k : pr.kc;
+        const int mpad = (m + 3) & ~3, npad = (kout + 15) & ~15, ldc = slab_ld(npad);
+        if (bp.x != cur) {
+            __syncthreads();   // every wave is done with the previous slab
+            for (int idx = threadIdx.x; idx < mpad * npad; idx += kGemmThreads) {
+                const int l = idx / npad, c = idx % npad;
+                double v = 0.0;
+                if (l < m && c < kout) {
+                    v = restart_coef(pr, l, c);
+                    if (kFinal) v *= pr.Sb[c];
+                }
+                cs[l * ldc + c] = v;
+            }
+            __syncthreads();
+            cur = bp.x;
+        }
+        const int64_t n = pr.n;
+        const int64_t row0 = (int64_t)bp.y * kRowBlock;
+        const int64_t row_end = row0 + kRowBlock < n ? row0 + kRowBlock : n;
+        const int row_tiles = (int)((row_end - row0 + 15) / 16);
+        const int wv = (wave_id() - rot + nw) % nw;   // the wave's first tile of this block
+        rot = (rot + row_tiles) % nw;
+        const double *A = pr.U[pr.ubuf];
+        if (kFinal) {
+            float *out = pr.PCs;
+            const int k = pr.k;
+            mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wv, nw,
+                           by_element([&](int64_t row, int col, double v) {
+                               if (row < row_end && col < kout) out[row * k + col] = (float)v;
+                           }));
+        } else if (kStage) {
+            double *st = cs + mpad * ldc + wave_id() * kStageDoubles;
+            mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wv, nw,
+                           TileStaged{pr.U[pr.ubuf ^ 1], n, row_end, kout, st});
+        } else {
+            double *out = pr.U[pr.ubuf ^ 1];
+            mfma_row_tiles(A, n, m, cs, ldc, npad >> 4, row0, row_end, wv, nw,
+                           by_element([&](int64_t row, int col, double v) {
+                               if (row < row_end && col < kout) out[(int64_t)col * n + row] = v;
+                           }));
+        }
     }
 }
 
@@ -2485,10 +2521,12 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     mark(4, 0);
     const size_t mfma_lds = gemm_lds_bytes(m), stage_lds = gemm_stage_lds_bytes(m);
     const bool mfma = m <= 4 * kMfmaMaxSteps && (int64_t)mfma_lds <= T.max_lds;
+    // One workgroup per compute unit (the LDS request allows no second).
+    const unsigned ngemm = T.num_cus > 0 && (int64_t)T.num_cus < T.num_blocks ? (unsigned)T.num_cus : nblk;
     if (mfma && (int64_t)stage_lds <= T.max_lds)
-        k_tall_gemm_mfma<false, true><<<nblk, kGemmThreads, stage_lds, s>>>(T);
+        k_tall_gemm_mfma<false, true><<<ngemm, kGemmThreads, stage_lds, s>>>(T);
     else if (mfma)
-        k_tall_gemm_mfma<false><<<nblk, kGemmThreads, mfma_lds, s>>>(T);
+        k_tall_gemm_mfma<false><<<ngemm, kGemmThreads, mfma_lds, s>>>(T);
     else
         k_tall_gemm<false><<<nblk, 256, sizeof(double) * m * kGemmCols, s>>>(T);
     k_restart_v<<<T.num_probs * (unsigned)(ceil_div(g, 64) * ceil_div(m, 4)), 256, 0, s>>>(T);
@@ -2496,7 +2534,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     mark(4, 1);
     mark(5, 0);
     if (mfma)
-        k_tall_gemm_mfma<true><<<nblk, kGemmThreads, mfma_lds, s>>>(T);
+        k_tall_gemm_mfma<true><<<ngemm, kGemmThreads, mfma_lds, s>>>(T);
     else
         k_tall_gemm<true><<<nblk, 256, sizeof(double) * m * kGemmCols, s>>>(T);
     k_finish<<<T.num_probs, 64, 0, s>>>(T, done_counter);

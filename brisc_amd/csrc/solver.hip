@@ -263,6 +263,7 @@ struct bpca_context {
     int spmv_group = 4, spmvt_group = 4;   // autotuned load-group sizes
     int spmv_wg = 0, spmvt_wg = 0;         // autotuned workgroups-per-CU caps
     int64_t lds_per_cu = 0;                // LDS bytes of one compute unit
+    int num_cus = 0;                       // compute units of the device
     bool sorted_xtu = true;                // gene-sorted X^T u for g <= kMaxSortedGenes
     int reorth_passes = 0;                 // 0 automatic, 1 one pass (n >= 8 g), 2 two passes
 };
@@ -508,6 +509,7 @@ int engine_rebuild_tables(bpca_engine *E) {
     E->T.spmv_wg = ctx->spmv_wg;
     E->T.spmvt_wg = ctx->spmvt_wg;
     E->T.lds_per_cu = ctx->lds_per_cu;
+    E->T.num_cus = ctx->num_cus;
     engine_drop_graph(E);
     E->dirty = false;
     return BPCA_OK;
@@ -687,6 +689,9 @@ int bpca_context_create(int device, uint64_t hbm_budget_bytes,
     int lds_cu = 0;
     if (hipDeviceGetAttribute(&lds_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) == hipSuccess && lds_cu > 0)
         ctx->lds_per_cu = lds_cu;
+    int cus = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess && cus > 0)
+        ctx->num_cus = cus;
     const char *env = std::getenv("BPCA_NO_GRAPH");
     ctx->use_graph = !(env && env[0] == '1');
     const int rc = scratch_init(device);
@@ -1371,6 +1376,7 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
     T.spmv_wg = ctx->spmv_wg;
     T.spmvt_wg = ctx->spmvt_wg;
     T.lds_per_cu = ctx->lds_per_cu;
+    T.num_cus = ctx->num_cus;
     const auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(bpca::launch_setup(T, s));
     // Every sweep ends within m + 1 macro steps (its SVD, restart or final
