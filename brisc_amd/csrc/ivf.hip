@@ -582,11 +582,10 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
         T.nq = T.nc = n;
         T.st = st;
         T.ksteps = ksteps;
-        T.k = k;
         T.self = 1;
         T.out_idx = out_idx;
         T.out_dist = out_dist;
-        IVF_TRY(launch_knn_tiles(T, false, s));
+        IVF_TRY(launch_knn_tiles_k(T, false, k, s));
         IVF_TRY(hipStreamSynchronize(s));
         return 0;
     }
@@ -690,14 +689,13 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
     T.nq = T.nc = n;
     T.st = st;
     T.ksteps = ksteps;
-    T.k = k;
     T.self = 1;
     T.wg_off = wg_off.p;
     T.ent = ent.p;
     T.ids = order.p;
     T.out_idx = out_idx;
     T.out_dist = out_dist;
-    IVF_TRY(launch_knn_tiles(T, true, s));
+    IVF_TRY(launch_knn_tiles_k(T, true, k, s));
     IVF_TRY(hipStreamSynchronize(s));
     return 0;
 }

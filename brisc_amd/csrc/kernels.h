@@ -318,6 +318,14 @@ struct KnnTiles {
     // per query caps its threshold for the scan proper.  boot_rows >= k.
     const float *boot;
     int boot_rows;
+    // Optional, per output row (row q, or ids[q] for inverted-file
+    // launches): a floor (floor_d[row * floor_ld], floor_i[row * floor_ld]);
+    // only candidates after it in (distance, id) order are kept.  Searches
+    // for more than 64 neighbours run in passes of 64, each pass's floor the
+    // last entry of the pass before (launch_knn_tiles_k).
+    const float *floor_d;
+    const uint32_t *floor_i;
+    int floor_ld;
     uint32_t *out_idx;
     float *out_dist;
 };
@@ -325,6 +333,12 @@ struct KnnTiles {
 hipError_t launch_knn_pack(const float *X, const uint32_t *order, int64_t n,
                            int d, const float *norms, float *Xp, hipStream_t s);
 hipError_t launch_knn_tiles(const KnnTiles &T, bool ivf, hipStream_t s);
+// The same search for any k (T.k ignored; k >= 1): k <= 64 is one launch;
+// more runs ceil(k / 64) launches of at most 64 neighbours each, every one
+// after the first with the previous launch's last neighbour of each row as
+// its floor, into pass buffers copied to columns 64 p .. of out_idx /
+// out_dist (row stride k).  Synchronous when k > 64.
+hipError_t launch_knn_tiles_k(const KnnTiles &T, bool ivf, int k, hipStream_t s);
 
 // SingleCell.neighbors() with its inverted-file search (ivf.hip).
 struct IvfParams {

@@ -400,6 +400,17 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     int qid[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) qid[i] = q0 + 16 * w + 4 * g + i;
+    float fl_d[4] = {0.f, 0.f, 0.f, 0.f};
+    uint32_t fl_i[4] = {0u, 0u, 0u, 0u};
+    if (T.floor_d) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int q = qid[i] < nq ? qid[i] : nq - 1;
+            const size_t row = (size_t)(IVF ? T.ids[q] : (uint32_t)q) * T.floor_ld;
+            fl_d[i] = T.floor_d[row];
+            fl_i[i] = T.floor_i[row];
+        }
+    }
     float Ld[16];
     uint32_t Li[16];
 #pragma unroll
@@ -553,6 +564,20 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
 #pragma unroll
                     for (int i = 0; i < 4; i++)
                         acc[b][i] = pos + 16 * b + m == qid[i] ? __builtin_nanf("") : acc[b][i];
+            }
+        }
+        if (T.floor_d) {
+            // Passes after the first: a candidate not after its row's floor in
+            // (distance, id) order was returned by an earlier pass.
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint32_t idb = IVF ? cid[16 * b + m] : (uint32_t)(pos + 16 * b + m);
+#pragma unroll
+                for (int i = 0; i < 4; i++) {
+                    const float v = acc[b][i];
+                    const bool after = v > fl_d[i] || (v == fl_d[i] && idb > fl_i[i]);
+                    acc[b][i] = after ? v : __builtin_nanf("");
+                }
             }
         }
         if constexpr (ARGMIN) {
@@ -862,10 +887,53 @@ hipError_t launch_knn_tiles(const KnnTiles &T, bool ivf, hipStream_t s) {
     return hipGetLastError();
 }
 
+hipError_t launch_knn_tiles_k(const KnnTiles &T0, bool ivf, int k, hipStream_t s) {
+    if (k < 1) return hipErrorInvalidValue;
+    KnnTiles T = T0;
+    if (k <= kWave) {
+        T.k = k;
+        return launch_knn_tiles(T, ivf, s);
+    }
+    const size_t rows = (size_t)T.nq;
+    uint32_t *pi = nullptr;
+    float *pd = nullptr;
+    hipError_t e = hipMalloc((void **)&pi, sizeof(uint32_t) * 2 * rows * kWave);
+    if (e == hipSuccess) e = hipMalloc((void **)&pd, sizeof(float) * 2 * rows * kWave);
+    int prev_k = 0;
+    for (int p = 0; e == hipSuccess && kWave * p < k; p++) {
+        const int kp = k - kWave * p < kWave ? k - kWave * p : kWave;
+        uint32_t *oi = pi + (size_t)(p & 1) * rows * kWave;
+        float *od = pd + (size_t)(p & 1) * rows * kWave;
+        T.k = kp;
+        T.out_idx = oi;
+        T.out_dist = od;
+        if (p > 0) {
+            const size_t prev = (size_t)((p - 1) & 1) * rows * kWave;
+            T.floor_d = pd + prev + (prev_k - 1);
+            T.floor_i = pi + prev + (prev_k - 1);
+            T.floor_ld = prev_k;
+        }
+        e = launch_knn_tiles(T, ivf, s);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(T0.out_idx + kWave * p, sizeof(uint32_t) * k, oi, sizeof(uint32_t) * kp,
+                                 sizeof(uint32_t) * kp, rows, hipMemcpyDeviceToDevice, s);
+        if (e == hipSuccess)
+            e = hipMemcpy2DAsync(T0.out_dist + kWave * p, sizeof(float) * k, od, sizeof(float) * kp,
+                                 sizeof(float) * kp, rows, hipMemcpyDeviceToDevice, s);
+        prev_k = kp;
+    }
+    // The buffers are freed once the passes are done.
+    const hipError_t es = hipStreamSynchronize(s);
+    if (e == hipSuccess) e = es;
+    if (pi) (void)hipFree(pi);
+    if (pd) (void)hipFree(pd);
+    return e;
+}
+
 hipError_t launch_knn_mfma(const float *X, const float *norms, int64_t n,
                            int d, int k, float *Xp, uint32_t *out_idx,
                            float *out_dist, hipStream_t s) {
-    if (n <= 0 || d < 1 || d > kKnnMfmaMaxDims || k < 1 || k > kWave)
+    if (n <= 0 || d < 1 || d > kKnnMfmaMaxDims || k < 1 || k >= n)
         return hipErrorInvalidValue;
     hipError_t err = launch_knn_pack(X, nullptr, n, d, norms, Xp, s);
     if (err != hipSuccess) return err;
@@ -876,11 +944,10 @@ hipError_t launch_knn_mfma(const float *X, const float *norms, int64_t n,
     T.d = d;
     T.st = knn_mfma_stride(d);
     T.ksteps = knn_mfma_ksteps(d);
-    T.k = k;
     T.self = 1;
     T.out_idx = out_idx;
     T.out_dist = out_dist;
-    return launch_knn_tiles(T, false, s);
+    return launch_knn_tiles_k(T, false, k, s);
 }
 
 size_t knn_lds_bytes(int d, int kmax) {

@@ -1696,10 +1696,12 @@ int bpca_knn_exact(bpca_context *ctx, const float *d_X, int64_t num_cells,
     if (!d_X || !d_neighbors || !d_distances) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL device pointer");
     if (num_cells < 2) return fail(BPCA_ERR_INVALID_ARGUMENT, "there is only one cell, so nearest neighbors cannot be calculated");
     if (num_dims < 1 || num_dims > 256) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_dims must be in [1, 256]");
-    if (k < 1 || k > 64 || (int64_t)k >= num_cells)
-        return fail(BPCA_ERR_INVALID_ARGUMENT, "k must be >= 1, <= 64 and less than the number of cells");
+    if (k < 1 || (int64_t)k >= num_cells)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "k must be >= 1 and less than the number of cells");
     if (num_cells > (int64_t)UINT32_MAX) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells too large");
     const bool mfma = num_dims <= bpca::kKnnMfmaMaxDims;
+    if (!mfma && k > 64)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "more than 64 neighbours need num_dims <= " + std::to_string(bpca::kKnnMfmaMaxDims));
     const size_t lds = mfma ? bpca::knn_mfma_lds_bytes(num_dims)
                             : bpca::knn_lds_bytes(num_dims, bpca::knn_kmax(k));
     if ((int64_t)lds > (int64_t)ctx->max_lds)
@@ -1735,8 +1737,8 @@ int bpca_knn_ivf(bpca_context *ctx, const float *d_X, int64_t num_cells,
     if (num_cells > (int64_t)UINT32_MAX) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells too large");
     if (num_dims < 1 || num_dims > bpca::kKnnMfmaMaxDims)
         return fail(BPCA_ERR_INVALID_ARGUMENT, "num_dims must be in [1, " + std::to_string(bpca::kKnnMfmaMaxDims) + "]");
-    if (k < 1 || k > 64 || (int64_t)k >= num_cells)
-        return fail(BPCA_ERR_INVALID_ARGUMENT, "k must be >= 1, <= 64 and less than the number of cells");
+    if (k < 1 || (int64_t)k >= num_cells)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "k must be >= 1 and less than the number of cells");
     if (num_clusters < 1 || (int64_t)num_clusters > num_cells || num_clusters >= (1 << 26))
         return fail(BPCA_ERR_INVALID_ARGUMENT, "num_clusters must be >= 1 and <= the number of cells (and < 2^26)");
     if (num_clusters_searched < 1 || num_clusters_searched > num_clusters)

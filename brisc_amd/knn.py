@@ -13,7 +13,10 @@ from __future__ import annotations
 
 from . import _lib
 
-MAX_NEIGHBORS = 64
+# Neighbours per cell found in one pass of the tile engine; more run in
+# passes of this many (matrix-core path, num_dims <= IVF_MAX_DIMS), each
+# keeping only candidates after the previous pass's last.
+MAX_NEIGHBORS_PER_PASS = 64
 MAX_DIMS = 256
 
 
@@ -37,11 +40,14 @@ def knn_exact(X, num_neighbors: int, context=None):
     if n < 2:
         raise ValueError('there is only one cell, so nearest neighbors cannot '
                          'be calculated')
-    if not 1 <= k <= min(MAX_NEIGHBORS, n - 1):
+    if not 1 <= k <= n - 1:
         raise ValueError(f'num_neighbors is {k}, but must be between 1 and '
-                         f'min({MAX_NEIGHBORS}, number of cells - 1)')
+                         f'the number of cells - 1')
     if not 1 <= d <= MAX_DIMS:
         raise ValueError(f'X has {d} columns; at most {MAX_DIMS} are supported')
+    if d > IVF_MAX_DIMS and k > MAX_NEIGHBORS_PER_PASS:
+        raise ValueError(f'more than {MAX_NEIGHBORS_PER_PASS} neighbours need '
+                         f'at most {IVF_MAX_DIMS} columns')
     if n >= 2 ** 31:
         raise ValueError('at most 2**31 - 1 cells are supported')
     X = X.contiguous()
@@ -98,9 +104,9 @@ def knn_ivf(X, num_neighbors: int, num_clusters: int | None = None,
     if n < 2:
         raise ValueError('there is only one cell, so nearest neighbors cannot '
                          'be calculated')
-    if not 1 <= k <= min(MAX_NEIGHBORS, n - 1):
+    if not 1 <= k <= n - 1:
         raise ValueError(f'num_neighbors is {k}, but must be between 1 and '
-                         f'min({MAX_NEIGHBORS}, number of cells - 1)')
+                         f'the number of cells - 1')
     if not 1 <= d <= IVF_MAX_DIMS:
         raise ValueError(f'X has {d} columns; at most {IVF_MAX_DIMS} are '
                          f'supported')

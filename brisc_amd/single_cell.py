@@ -455,15 +455,17 @@ class SingleCell:
         the cells of its ``num_clusters_searched`` nearest clusters
         (``brisc_amd.knn.knn_ivf``).  When every cluster is searched, or
         more than 64 would be, the exhaustive search runs instead, which
-        returns what searching every cluster returns.  Limits on the GPU:
-        ``num_neighbors <= 64``; the PCs have at most 128 columns (256 with
-        every cluster searched); k-means|| initialisation
+        returns what searching every cluster returns.  More than 64
+        neighbours are found in passes of 64 (each keeping only candidates
+        after the previous pass's last distance and index).  Limits on the
+        GPU: the PCs have at most 128 columns (256 with every cluster
+        searched and at most 64 neighbours); k-means|| initialisation
         (``kmeans_barbar=True``) is not available.
         """
         import torch
 
-        from .knn import (IVF_MAX_DIMS, MAX_DIMS, MAX_NEIGHBORS, knn_exact,
-                          knn_ivf)
+        from .knn import (IVF_MAX_DIMS, MAX_DIMS, MAX_NEIGHBORS_PER_PASS,
+                          knn_exact, knn_ivf)
         check_type(neighbors_key, 'neighbors_key', str, 'a string')
         check_type(distances_key, 'distances_key', str, 'a string')
         check_type(overwrite, 'overwrite', bool, 'Boolean')
@@ -555,11 +557,9 @@ class SingleCell:
             num_clusters_searched = min(64, num_clusters)
         if chunk_size_kmeans is None:
             chunk_size_kmeans = min(4096, num_cells)
-        if num_neighbors > MAX_NEIGHBORS:
-            raise ValueError(f'num_neighbors is {num_neighbors:,}; at most '
-                             f'{MAX_NEIGHBORS} are supported on the GPU')
         exhaustive = num_clusters_searched == num_clusters
-        max_dims = MAX_DIMS if exhaustive else IVF_MAX_DIMS
+        max_dims = MAX_DIMS if exhaustive and \
+            num_neighbors <= MAX_NEIGHBORS_PER_PASS else IVF_MAX_DIMS
         if PCs.shape[1] > max_dims:
             raise ValueError(f'obsm[{PC_key!r}] has {PCs.shape[1]:,} columns; '
                              f'at most {max_dims} are supported on the GPU')

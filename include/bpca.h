@@ -421,7 +421,9 @@ int bpca_csr_transpose(bpca_context *ctx, const float *d_data,
  *   d_distances  float32 num_cells x k: squared Euclidean distances formed
  *                as ||q||^2 + (||c||^2 - 2 q.c) in FP32, as the reference
  * A cell is not its own neighbour; each row is in ascending distance, ties
- * in ascending index.  1 <= k <= min(64, num_cells - 1).  Synchronous. */
+ * in ascending index.  1 <= k <= num_cells - 1 (k <= 64 when num_dims >
+ * 128); more than 64 neighbours run in passes of 64, each pass keeping only
+ * candidates after the previous pass's last (distance, index).  Synchronous. */
 int bpca_knn_exact(bpca_context *ctx, const float *d_X, int64_t num_cells,
                    int32_t num_dims, int32_t k, uint32_t *d_neighbors,
                    float *d_distances);

@@ -306,6 +306,39 @@ def test_ivf_shapes_against_exact_search(n, d, k, centers):
     assert np.array_equal(dist[hit].view(np.uint32), exd[hit].view(np.uint32))
 
 
+def test_ivf_more_than_64_neighbours():
+    # 100 neighbours in two passes of the inverted-file scan (the same
+    # clusters searched as for 64): the first 64 columns equal the
+    # 64-neighbour search bit for bit; structure exact; recall against the
+    # exhaustive 100-neighbour search high, shared pairs' distances
+    # bit-identical to it.
+    from brisc_amd.knn import knn_exact, knn_ivf
+    n, d, k = 20000, 50, 100
+    X = helpers.knn_points(4242, n, d, 40, spread=0.7)
+    Xd = _dev(X)
+    nb, dist, info = knn_ivf(Xd, k)
+    nb64, dist64, info64 = knn_ivf(Xd, 64)
+    ex, exd = knn_exact(Xd, k)
+    assert info['num_clusters_searched'] == info64['num_clusters_searched']
+    assert not info['exhaustive']
+    nb, dist = nb.cpu().numpy().astype(np.int64), dist.cpu().numpy()
+    nb64, dist64 = nb64.cpu().numpy().astype(np.int64), dist64.cpu().numpy()
+    ex, exd = ex.cpu().numpy().astype(np.int64), exd.cpu().numpy()
+    assert np.array_equal(nb[:, :64], nb64)
+    assert np.array_equal(dist[:, :64].view(np.uint32), dist64.view(np.uint32))
+    assert nb.min() >= 0 and nb.max() < n
+    assert not (nb == np.arange(n)[:, None]).any()
+    for row in nb[:2000]:
+        assert np.unique(row).size == k
+    dd, ii = dist[:, 1:] - dist[:, :-1], nb[:, 1:] - nb[:, :-1]
+    assert ((dd > 0) | ((dd == 0) & (ii > 0))).all()
+    rec = _recall(nb, ex, k)
+    hit = nb == ex
+    print(f'n={n} d={d} k={k}: {info}; recall {rec:.5f}')
+    assert rec >= 0.95
+    assert np.array_equal(dist[hit].view(np.uint32), exd[hit].view(np.uint32))
+
+
 @pytest.mark.parametrize('d', [50, 20])
 def test_ivf_fewer_than_64_clusters_searches_each_cells_nearest_clusters(d):
     # 40 clusters, 6 searched: the ranking's first tile (the clusters nearest

@@ -95,6 +95,23 @@ def test_exact_knn_matches_fp64_oracle(n, d, k):
           f'same neighbour set {same_set:.4f}')
 
 
+@pytest.mark.parametrize('n,d,k', [(3000, 30, 100), (2500, 50, 150), (1000, 7, 200), (700, 3, 65)])
+def test_exact_knn_more_than_64_neighbours(n, d, k):
+    # Passes of 64: the first 64 columns are the 64-neighbour search's,
+    # bit for bit; the whole row against the FP64 oracle as above (with
+    # lattice-like duplicates in the 3-dimensional case).
+    X = _clustered(n * 3 + d, n, d)
+    if d == 3:
+        X = np.round(X * 2.0) / 2.0
+    nb, dist = _knn(X, k)
+    nb64, dist64 = _knn(X, 64)
+    assert np.array_equal(nb[:, :64], nb64)
+    assert np.array_equal(dist[:, :64].view(np.uint32), dist64.view(np.uint32))
+    same_order, same_set = check_knn(X, nb, dist, k)
+    print(f'n={n} d={d} k={k}: rows identical to the oracle {same_order:.4f}, '
+          f'same neighbour set {same_set:.4f}')
+
+
 def test_exact_knn_duplicates_and_ties():
     # Exact duplicates: distance 0 up to rounding of the norm identity; each
     # duplicate group member lists the other members first.
