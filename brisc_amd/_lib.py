@@ -137,6 +137,14 @@ def lib() -> ctypes.CDLL:
         'bpca_kmeans': (i32, [p, p, i64, i32, i32, i32, ctypes.c_float, i64,
                               ctypes.c_uint64, p, p, p, p]),
         'bpca_kmeans_init_cells': (i32, [i64, i32, ctypes.c_uint64, p]),
+        'bpca_knn_ivf_ex': (i32, [p, p, i64, i32, i32, i32, i32, i32,
+                                  ctypes.c_float, i64, ctypes.c_uint64, i32, i32,
+                                  ctypes.c_float, p, p, p]),
+        'bpca_kmeans_ex': (i32, [p, p, i64, i32, i32, i32, ctypes.c_float, i64,
+                                 ctypes.c_uint64, i32, i32, ctypes.c_float,
+                                 p, p, p, p]),
+        'bpca_kmeans_barbar_init': (i32, [p, p, i64, i32, i32, i32, ctypes.c_float,
+                                          i64, ctypes.c_uint64, p, p]),
         'bpca_test_small_svd': (i32, [p, i32, p, p, p, p, p]),
         'bpca_test_gemm': (i32, [p, i64, i32, i32, p, p, p]),
         'bpca_test_gemm_device': (i32, [p, i64, i32, i32, p, p, p]),

@@ -29,6 +29,8 @@
 // centroids may be assigned differently; everything else is exact integer
 // work or follows the reference's FP32 operation order.
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_select.hpp>
+#include <rocprim/iterator/counting_iterator.hpp>
 
 #include <algorithm>
 #include <cmath>
@@ -309,6 +311,229 @@ __global__ __launch_bounds__(256) void k_ivf_entries(const uint32_t *__restrict_
     }
 }
 
+// ---------------------------------------------------------------------------
+// k-means|| initialisation (kmeans_barbar_init, brisc/kmeans.pyx:41-360; the
+// parallel version 362-720 selects the same cells in the same order).
+// Selection rounds: each cell's squared distance to its nearest candidate
+// (min_distances), the cost (chunk sums in cell order, then the chunk sums
+// in order), and a draw per cell from srand(seed + cell * round) taking it
+// with probability l * distance / cost.  Candidate updates rank the round's
+// new candidates with the tile engine (k = 1, partial distances, ties to the
+// lower index).  Then a weighted k-means++ pass over the candidates: per
+// draw one launch updates every candidate's weighted distance and its
+// 256-candidate halving-tree sum, and a one-workgroup launch adds the tree
+// sums in order and walks the reference's sequential probability chain.
+
+// Distances to the first random cell, dimensions in order, the first square
+// a product and each later one fused (kmeans.pyx:84-97).
+__global__ __launch_bounds__(256) void k_kb_first(const float *__restrict__ X, int64_t n,
+                                                  int d, int64_t rc,
+                                                  float *__restrict__ md) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float *x = X + i * d, *r = X + rc * d;
+    float diff = x[0] - r[0];
+    float sum = diff * diff;
+    for (int j = 1; j < d; j++) {
+        diff = x[j] - r[j];
+        sum = fmaf(diff, diff, sum);
+    }
+    md[i] = sum;
+}
+
+// Sum of each chunk of `chunk` cells, cells in order (kmeans.pyx:215-224).
+__global__ __launch_bounds__(256) void k_kb_chunk_sums(const float *__restrict__ md,
+                                                       int64_t n, int64_t chunk,
+                                                       int64_t nchunks,
+                                                       float *__restrict__ out) {
+    const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (c >= nchunks) return;
+    const int64_t i0 = c * chunk, i1 = i0 + chunk < n ? i0 + chunk : n;
+    float sum = 0.0f;
+#pragma unroll 16
+    for (int64_t i = i0; i < i1; i++) sum += md[i];
+    out[c] = sum;
+}
+
+// One selection round's draws (kmeans.pyx:107-112, 227-240): cell i draws
+// random_uniform from srand(seed + i * round), the product in 32 bits; a
+// taken cell's distance becomes 0.
+__global__ __launch_bounds__(256) void k_kb_sample(float *__restrict__ md, int64_t n,
+                                                   uint64_t seed, uint32_t round,
+                                                   float l_over_cost,
+                                                   uint8_t *__restrict__ flags) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    Pcg32 g(seed + (uint64_t)((uint32_t)i * round));
+    const float u = g.uniform();
+    const bool take = l_over_cost * md[i] >= u;
+    flags[i] = take ? 1 : 0;
+    if (take) md[i] = 0.0f;
+}
+
+// Each new candidate is its own cell's nearest (kmeans.pyx:127-130,
+// 247-252); a cell taken twice keeps its later position, as the serial
+// loop's last write does.
+__global__ __launch_bounds__(256) void k_kb_set_best(const uint32_t *__restrict__ sel,
+                                                     int64_t p0, int64_t p1,
+                                                     uint32_t *__restrict__ best) {
+    const int64_t p = p0 + (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (p < p1) atomicMax(&best[sel[p]], (uint32_t)p);
+}
+
+// Candidate update (kmeans.pyx:196-214): the round's best partial distance
+// ||c||^2 - 2 x.c against min_distance - ||x||^2, strictly smaller wins;
+// min_distance is stored back with ||x||^2 added.
+__global__ __launch_bounds__(256) void k_kb_update(float *__restrict__ md,
+                                                   const float *__restrict__ norms,
+                                                   uint32_t *__restrict__ best,
+                                                   const float *__restrict__ pd,
+                                                   const uint32_t *__restrict__ pj,
+                                                   uint32_t prev, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i >= n) return;
+    const float xn = norms[i];
+    float m = md[i] - xn;
+    const float v = pd[i];
+    if (v < m) {
+        m = v;
+        best[i] = prev + pj[i];
+    }
+    md[i] = m + xn;
+}
+
+// Candidate weights: cells nearest each candidate (kmeans.pyx:276-280).
+__global__ __launch_bounds__(256) void k_kb_weights(const uint32_t *__restrict__ best,
+                                                    int64_t n, uint32_t *__restrict__ w) {
+    const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (i < n) atomicAdd(&w[best[i]], 1u);
+}
+
+// Candidate coordinates by dimension: candT[j S + i] = X[sel[i], j].
+__global__ __launch_bounds__(256) void k_kb_gather_t(const float *__restrict__ X, int d,
+                                                     const uint32_t *__restrict__ sel,
+                                                     int64_t S, float *__restrict__ candT) {
+    const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    if (e >= S * d) return;
+    const int64_t j = e / S, i = e - j * S;
+    candT[e] = X[(int64_t)sel[i] * d + j];
+}
+
+// k-means++ update (kmeans.pyx:286-300, 331-348): candidate i's weighted
+// squared distance to the centroid at candidate position *pos, kept if
+// smaller (first: taken as is), and this workgroup's halving-tree sum of
+// the 256 results.
+__global__ __launch_bounds__(256) void k_kpp_update(const float *__restrict__ candT,
+                                                    int64_t S, int d,
+                                                    const uint32_t *__restrict__ w,
+                                                    float *__restrict__ m2,
+                                                    const uint32_t *__restrict__ pos,
+                                                    int first, float *__restrict__ partials) {
+    __shared__ float red[256];
+    const int tid = threadIdx.x;
+    const int64_t i = (int64_t)blockIdx.x * 256 + tid;
+    const int64_t c = *pos;
+    float m = 0.0f;
+    if (i < S) {
+        float diff = candT[i] - candT[c];
+        float dist = diff * diff;
+        for (int j = 1; j < d; j++) {
+            diff = candT[(int64_t)j * S + i] - candT[(int64_t)j * S + c];
+            dist = fmaf(diff, diff, dist);
+        }
+        dist = dist * (float)w[i];
+        if (first) {
+            m = dist;
+        } else {
+            m = m2[i];
+            if (dist < m) m = dist;
+        }
+        m2[i] = m;
+    }
+    red[tid] = m;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        if (tid < h) red[tid] += red[tid + h];
+        __syncthreads();
+    }
+    if (tid == 0) partials[blockIdx.x] = red[0];
+}
+
+constexpr int kKppSeg = 4096;  // candidates staged in LDS per segment
+
+// One k-means++ draw (kmeans.pyx:307-321): cost = the tree sums in order,
+// probability = u, then probability -= m2[i] / cost (times 1 / cost, fused)
+// for i in order until it goes negative; the last candidate if it never
+// does.  The chosen candidate's m2 becomes 0.  The four waves stage the
+// candidates in LDS; the first walks the chain, 64 candidates at a time
+// (the chain never increases, so a block whose end stays non-negative holds
+// no crossing).
+__global__ __launch_bounds__(256) void k_kpp_draw(float *__restrict__ m2, int64_t S,
+                                                  const float *__restrict__ partials,
+                                                  int G, float u, uint32_t *__restrict__ pos,
+                                                  const uint32_t *__restrict__ sel,
+                                                  uint32_t *__restrict__ cell_out) {
+    __shared__ __align__(16) float seg[kKppSeg];
+    __shared__ float sh_p, sh_inv;
+    __shared__ int64_t sh_pick;
+    const int tid = threadIdx.x;
+    if (tid == 0) {
+        float cost = 0.0f;
+        for (int b = 0; b < G; b++) cost += partials[b];
+        sh_inv = 1.0f / cost;
+        sh_p = u;
+        sh_pick = -1;
+    }
+    __syncthreads();
+    const float inv = sh_inv;
+    for (int64_t s0 = 0; s0 < S; s0 += kKppSeg) {
+        const int len = (int)(S - s0 < kKppSeg ? S - s0 : kKppSeg);
+        for (int k = tid; k < kKppSeg; k += 256) seg[k] = k < len ? m2[s0 + k] : 0.0f;
+        __syncthreads();
+        if (tid < kWave) {
+            float p = sh_p;
+            int64_t pick = -1;
+            const int nblk = (len + 63) / 64;
+            for (int b = 0; b < nblk; b++) {
+                const float4 *v = (const float4 *)(seg + 64 * b);
+                float q = p;
+#pragma unroll
+                for (int t = 0; t < 16; t++) {
+                    const float4 a = v[t];
+                    q = fmaf(-a.x, inv, q);
+                    q = fmaf(-a.y, inv, q);
+                    q = fmaf(-a.z, inv, q);
+                    q = fmaf(-a.w, inv, q);
+                }
+                if (q < 0.0f) {
+                    q = p;
+                    for (int t = 0; t < 64; t++) {
+                        q = fmaf(-seg[64 * b + t], inv, q);
+                        if (q < 0.0f) {
+                            pick = s0 + 64 * b + t;
+                            break;
+                        }
+                    }
+                    break;
+                }
+                p = q;
+            }
+            if (tid == 0) {
+                sh_p = p;
+                sh_pick = pick;
+            }
+        }
+        __syncthreads();
+        if (sh_pick >= 0) break;
+    }
+    if (tid == 0) {
+        const int64_t pick = sh_pick >= 0 ? sh_pick : S - 1;
+        m2[pick] = 0.0f;
+        *pos = (uint32_t)pick;
+        *cell_out = sel[pick];
+    }
+}
+
 template <typename T>
 struct Buf {
     T *p = nullptr;
@@ -357,6 +582,182 @@ std::vector<uint32_t> kmeans_random_init_cells(int64_t n, int num_clusters, uint
     return out;
 }
 
+int kmeans_barbar_init_cells(const float *X, const float *norms_in, const float *Xp_in,
+                             const IvfParams &P, std::vector<uint32_t> *cells,
+                             int64_t *candidates, std::string *msg,
+                             hipError_t *herr, hipStream_t s) {
+    const int64_t n = P.n;
+    const int d = P.d, C = P.num_clusters;
+    const int st = knn_mfma_stride(d);
+    const int64_t pad = kKnnPadRows;
+    const int64_t chunk = P.chunk, nchunks = ceil_div(n, chunk);
+    const float l = P.oversampling * (float)(uint32_t)C;
+    Buf<float> norms_own, Xp_own;
+    const float *norms = norms_in, *Xp = Xp_in;
+    if (!norms || !Xp) {
+        IVF_TRY(norms_own.alloc(n + pad));
+        IVF_TRY(Xp_own.alloc((n + pad) * st));
+        IVF_TRY(launch_row_norms(X, n, d, norms_own.p, s));
+        IVF_TRY(launch_knn_pack(X, nullptr, n, d, norms_own.p, Xp_own.p, s));
+        norms = norms_own.p;
+        Xp = Xp_own.p;
+    }
+    Buf<float> md, pd, csum;
+    Buf<uint32_t> best, sel, fresh, pj, nsel;
+    Buf<uint8_t> flags;
+    IVF_TRY(md.alloc(n));
+    IVF_TRY(pd.alloc(n));
+    IVF_TRY(csum.alloc(nchunks));
+    IVF_TRY(best.alloc(n));
+    IVF_TRY(sel.alloc(n));
+    IVF_TRY(fresh.alloc(n));
+    IVF_TRY(pj.alloc(n));
+    IVF_TRY(nsel.alloc(1));
+    IVF_TRY(flags.alloc(n));
+    IVF_TRY(hipMemsetAsync(best.p, 0, sizeof(uint32_t) * n, s));
+    size_t sel_bytes = 0;
+    IVF_TRY(rocprim::select(nullptr, sel_bytes, rocprim::counting_iterator<uint32_t>(0), flags.p,
+                            fresh.p, nsel.p, (size_t)n, s));
+    Buf<char> sel_tmp;
+    IVF_TRY(sel_tmp.alloc(sel_bytes));
+    std::vector<float> h_csum((size_t)nchunks);
+    // The cost: chunk sums in cell order, then the chunk sums in order.
+    auto cost_of = [&](float *out) -> hipError_t {
+        k_kb_chunk_sums<<<grid_of(nchunks), 256, 0, s>>>(md.p, n, chunk, nchunks, csum.p);
+        hipError_t e = hipGetLastError();
+        if (e == hipSuccess) e = hipMemcpyAsync(h_csum.data(), csum.p, sizeof(float) * nchunks, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) return e;
+        float c = 0.0f;
+        for (int64_t i = 0; i < nchunks; i++) c += h_csum[i];
+        *out = c;
+        return hipSuccess;
+    };
+    int64_t S = 0;
+    // A round's draws; the taken cells (in cell order) are appended to sel.
+    auto sample = [&](uint32_t round, float l_over_cost, int64_t *taken) -> int {
+        k_kb_sample<<<grid_of(n), 256, 0, s>>>(md.p, n, P.seed, round, l_over_cost, flags.p);
+        IVF_TRY(hipGetLastError());
+        size_t bytes = sel_bytes;
+        IVF_TRY(rocprim::select(sel_tmp.p, bytes, rocprim::counting_iterator<uint32_t>(0), flags.p,
+                                fresh.p, nsel.p, (size_t)n, s));
+        uint32_t h_count = 0;
+        IVF_TRY(hipMemcpyAsync(&h_count, nsel.p, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+        IVF_TRY(hipStreamSynchronize(s));
+        if (S + (int64_t)h_count > n) {
+            *msg = "k-means||: more candidates than cells";
+            return 1;
+        }
+        if (h_count)
+            IVF_TRY(hipMemcpyAsync(sel.p + S, fresh.p, sizeof(uint32_t) * h_count, hipMemcpyDeviceToDevice, s));
+        *taken = h_count;
+        return 0;
+    };
+
+    // The first candidate and every cell's distance to it.
+    Pcg32 g0(P.seed - 1);
+    const uint32_t rc = g0.bounded((uint32_t)n);
+    IVF_TRY(hipMemcpyAsync(sel.p, &rc, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    S = 1;
+    k_kb_first<<<grid_of(n), 256, 0, s>>>(X, n, d, rc, md.p);
+    IVF_TRY(hipGetLastError());
+    float cost = 0.0f;
+    IVF_TRY(cost_of(&cost));
+    if (cost == 0.0f) {
+        *msg = "all cells have the same principal component loadings";
+        return 1;
+    }
+    int64_t taken = 0;
+    if (int r = sample(1, l / cost, &taken)) return r;
+    int64_t prev = 1;
+    S += taken;
+    int64_t fresh_n = taken;
+    if (fresh_n) {
+        k_kb_set_best<<<grid_of(fresh_n), 256, 0, s>>>(sel.p, prev, S, best.p);
+        IVF_TRY(hipGetLastError());
+    }
+
+    Buf<float> candp;
+    KnnTiles E{};
+    E.Q = Xp;
+    E.qnorm = 0;
+    E.d = d;
+    E.nq = n;
+    E.st = st;
+    E.ksteps = knn_mfma_ksteps(d);
+    E.k = 1;
+    E.self = 0;
+    E.out_idx = pj.p;
+    E.out_dist = pd.p;
+    uint32_t round = 1;
+    for (;;) {
+        if (fresh_n > 0) {
+            if ((int64_t)candp.count < (fresh_n + pad) * st) {
+                candp.reset();
+                IVF_TRY(candp.alloc((fresh_n + pad) * st));
+            }
+            IVF_TRY(launch_knn_pack(X, sel.p + prev, fresh_n, d, norms, candp.p, s));
+            E.C = candp.p;
+            E.nc = fresh_n;
+            IVF_TRY(launch_knn_tiles(E, false, s));
+            k_kb_update<<<grid_of(n), 256, 0, s>>>(md.p, norms, best.p, pd.p, pj.p, (uint32_t)prev, n);
+            IVF_TRY(hipGetLastError());
+            IVF_TRY(cost_of(&cost));
+        }
+        if (int r = sample(round, l / cost, &taken)) return r;
+        prev = S;
+        S += taken;
+        fresh_n = taken;
+        if (fresh_n) {
+            k_kb_set_best<<<grid_of(fresh_n), 256, 0, s>>>(sel.p, prev, S, best.p);
+            IVF_TRY(hipGetLastError());
+        }
+        if (fresh_n == 0 && cost == 0.0f) {
+            *msg = "num_clusters (" + std::to_string(C) +
+                   ") is greater than the number of cells with distinct principal component "
+                   "loadings; decrease num_clusters";
+            return 1;
+        }
+        round++;
+        if ((round == (uint32_t)P.init_iterations && S >= C) || S == n) break;
+    }
+    *candidates = S;
+
+    // Weighted k-means++ over the S candidates.
+    Buf<uint32_t> w, pos, cells_d;
+    Buf<float> candT, m2, parts;
+    IVF_TRY(w.alloc(S));
+    IVF_TRY(hipMemsetAsync(w.p, 0, sizeof(uint32_t) * S, s));
+    k_kb_weights<<<grid_of(n), 256, 0, s>>>(best.p, n, w.p);
+    IVF_TRY(hipGetLastError());
+    Pcg32 g2(P.seed - 2);
+    const uint32_t p0 = g2.bounded((uint32_t)S);
+    uint32_t first_cell = 0;
+    IVF_TRY(hipMemcpyAsync(&first_cell, sel.p + p0, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+    IVF_TRY(hipStreamSynchronize(s));
+    cells->assign(C, 0);
+    (*cells)[0] = first_cell;
+    if (C == 1) return 0;
+    const int64_t G = ceil_div(S, 256);
+    IVF_TRY(candT.alloc(S * d));
+    IVF_TRY(m2.alloc(S));
+    IVF_TRY(parts.alloc(G));
+    IVF_TRY(pos.alloc(1));
+    IVF_TRY(cells_d.alloc(C));
+    k_kb_gather_t<<<grid_of(S * d), 256, 0, s>>>(X, d, sel.p, S, candT.p);
+    IVF_TRY(hipGetLastError());
+    IVF_TRY(hipMemcpyAsync(pos.p, &p0, sizeof(uint32_t), hipMemcpyHostToDevice, s));
+    for (int c = 1; c < C; c++) {
+        const float u = g2.uniform();
+        k_kpp_update<<<(unsigned)G, 256, 0, s>>>(candT.p, S, d, w.p, m2.p, pos.p, c == 1, parts.p);
+        k_kpp_draw<<<1, 256, 0, s>>>(m2.p, S, parts.p, (int)G, u, pos.p, sel.p, cells_d.p + c);
+    }
+    IVF_TRY(hipGetLastError());
+    IVF_TRY(hipMemcpyAsync(cells->data() + 1, cells_d.p + 1, sizeof(uint32_t) * (C - 1), hipMemcpyDeviceToHost, s));
+    IVF_TRY(hipStreamSynchronize(s));
+    return 0;
+}
+
 int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
                 float *out_dist, IvfInfo *info, std::string *msg,
                 hipError_t *herr, hipStream_t s) {
@@ -375,7 +776,14 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
     IVF_TRY(launch_knn_pack(X, nullptr, n, d, norms.p, Xp.p, s));
 
     // 1. k-means.
-    std::vector<uint32_t> init = kmeans_random_init_cells(n, C, P.seed);
+    std::vector<uint32_t> init;
+    info->candidates = 0;
+    if (P.barbar) {
+        if (int r = kmeans_barbar_init_cells(X, norms.p, Xp.p, P, &init, &info->candidates, msg, herr, s))
+            return r;
+    } else {
+        init = kmeans_random_init_cells(n, C, P.seed);
+    }
     Buf<uint32_t> rows, labels, counts, order, order_tmp, labels_sorted;
     Buf<float> cent, centp, cnorm, mind, sums;
     Buf<int64_t> off;

@@ -355,6 +355,10 @@ struct IvfParams {
     uint32_t *kmeans_labels = nullptr;
     float *kmeans_centroids = nullptr;
     uint32_t *kmeans_counts = nullptr;
+    // k-means|| initialisation (kmeans_barbar=True) instead of random cells.
+    int barbar = 0;
+    int init_iterations = 5;
+    float oversampling = 1.0f;
 };
 struct IvfInfo {
     int clusters;       // non-empty clusters
@@ -362,6 +366,7 @@ struct IvfInfo {
     int iterations;     // k-means iterations run
     int exhaustive;     // every cluster searched: the exhaustive search ran
     int64_t entries;    // (workgroup, cluster) pairs scanned
+    int64_t candidates; // k-means||: candidates selected (0 otherwise)
 };
 // 0: done; 1: invalid input (*msg); 2: HIP error (*herr).  Synchronous.
 int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
@@ -369,6 +374,17 @@ int run_knn_ivf(const float *X, const IvfParams &P, uint32_t *out_idx,
                 hipError_t *herr, hipStream_t s);
 std::vector<uint32_t> kmeans_random_init_cells(int64_t n, int num_clusters,
                                                uint64_t seed);
+// k-means|| initialisation (brisc/kmeans.pyx:41-360) of P.num_clusters
+// cells from P's n x d cells X (device), with P.init_iterations,
+// P.oversampling, P.chunk and P.seed: the chosen cells go to *cells (host),
+// the number of candidates selected to *candidates.  norms and Xp: X's
+// squared norms (launch_row_norms) and padded copy (launch_knn_pack), or
+// null to have them made here.  0: done; 1: the reference's input errors
+// (*msg); 2: HIP error (*herr).  Synchronous.
+int kmeans_barbar_init_cells(const float *X, const float *norms, const float *Xp,
+                             const IvfParams &P, std::vector<uint32_t> *cells,
+                             int64_t *candidates, std::string *msg,
+                             hipError_t *herr, hipStream_t s);
 
 // Transpose of a sparse matrix stored by rows (transpose.hip): input
 // num_rows x num_cols with int32 or int64 offsets / indices (widths 4 or 8),

@@ -42,6 +42,15 @@ int hip_fail(hipError_t e, const char *what) {
                 std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// The k-means|| arguments (brisc/single_cell.py:12389-12401), or null.
+const char *barbar_arg_error(int32_t barbar, int32_t init_iterations, float oversampling) {
+    if (barbar != 0 && barbar != 1) return "kmeans_barbar must be 0 or 1";
+    if (init_iterations < 1) return "num_init_iterations must be a positive integer";
+    if (!(oversampling > 0.0f) || !std::isfinite(oversampling))
+        return "oversampling_factor must be a positive number";
+    return nullptr;
+}
+
 #define HIP_TRY(expr)                                      \
     do {                                                   \
         hipError_t _e = (expr);                            \
@@ -1731,6 +1740,22 @@ int bpca_knn_ivf(bpca_context *ctx, const float *d_X, int64_t num_cells,
                  float kmeans_tolerance, int64_t chunk_size_kmeans,
                  uint64_t seed, uint32_t *d_neighbors, float *d_distances,
                  int64_t *info) {
+    int64_t info6[6];
+    const int rc = bpca_knn_ivf_ex(ctx, d_X, num_cells, num_dims, k, num_clusters,
+                                   num_clusters_searched, num_kmeans_iterations,
+                                   kmeans_tolerance, chunk_size_kmeans, seed, 0, 5, 1.0f,
+                                   d_neighbors, d_distances, info ? info6 : nullptr);
+    if (rc == BPCA_OK) std::copy(info6, info6 + 5, info);
+    return rc;
+}
+
+int bpca_knn_ivf_ex(bpca_context *ctx, const float *d_X, int64_t num_cells,
+                    int32_t num_dims, int32_t k, int32_t num_clusters,
+                    int32_t num_clusters_searched, int32_t num_kmeans_iterations,
+                    float kmeans_tolerance, int64_t chunk_size_kmeans,
+                    uint64_t seed, int32_t kmeans_barbar,
+                    int32_t num_init_iterations, float oversampling_factor,
+                    uint32_t *d_neighbors, float *d_distances, int64_t *info) {
     if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (!d_X || !d_neighbors || !d_distances || !info) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL pointer");
     if (num_cells < 2) return fail(BPCA_ERR_INVALID_ARGUMENT, "there is only one cell, so nearest neighbors cannot be calculated");
@@ -1749,11 +1774,16 @@ int bpca_knn_ivf(bpca_context *ctx, const float *d_X, int64_t num_cells,
         return fail(BPCA_ERR_INVALID_ARGUMENT, "chunk_size_kmeans must be >= 1 and <= the number of cells");
     if ((int64_t)bpca::knn_mfma_lds_bytes(num_dims) > (int64_t)ctx->max_lds)
         return fail(BPCA_ERR_INVALID_ARGUMENT, "num_dims needs more LDS than the device has");
+    if (const char *e = barbar_arg_error(kmeans_barbar, num_init_iterations, oversampling_factor))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, e);
     std::lock_guard<std::recursive_mutex> guard(ctx->lock);
     DeviceGuard dg(ctx->device);
     begin_call(ctx);
     bpca::IvfParams P{num_cells, num_dims, k, num_clusters, num_clusters_searched,
                       num_kmeans_iterations, kmeans_tolerance, chunk_size_kmeans, seed};
+    P.barbar = kmeans_barbar;
+    P.init_iterations = num_init_iterations;
+    P.oversampling = oversampling_factor;
     bpca::IvfInfo I{};
     std::string msg;
     hipError_t herr = hipSuccess;
@@ -1765,6 +1795,7 @@ int bpca_knn_ivf(bpca_context *ctx, const float *d_X, int64_t num_cells,
     info[2] = I.iterations;
     info[3] = I.exhaustive;
     info[4] = I.entries;
+    info[5] = I.candidates;
     return BPCA_OK;
 }
 
@@ -1774,6 +1805,18 @@ int bpca_kmeans(bpca_context *ctx, const float *d_X, int64_t num_cells,
                 int64_t chunk_size_kmeans, uint64_t seed, uint32_t *d_labels,
                 float *d_centroids, uint32_t *d_counts,
                 int32_t *iterations_run) {
+    return bpca_kmeans_ex(ctx, d_X, num_cells, num_dims, num_clusters, num_kmeans_iterations,
+                          kmeans_tolerance, chunk_size_kmeans, seed, 0, 5, 1.0f, d_labels,
+                          d_centroids, d_counts, iterations_run);
+}
+
+int bpca_kmeans_ex(bpca_context *ctx, const float *d_X, int64_t num_cells,
+                   int32_t num_dims, int32_t num_clusters,
+                   int32_t num_kmeans_iterations, float kmeans_tolerance,
+                   int64_t chunk_size_kmeans, uint64_t seed, int32_t kmeans_barbar,
+                   int32_t num_init_iterations, float oversampling_factor,
+                   uint32_t *d_labels, float *d_centroids, uint32_t *d_counts,
+                   int32_t *iterations_run) {
     if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (!d_X || !d_labels || !d_centroids || !d_counts || !iterations_run)
         return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL pointer");
@@ -1786,11 +1829,16 @@ int bpca_kmeans(bpca_context *ctx, const float *d_X, int64_t num_cells,
     if (!(kmeans_tolerance > 0.0f)) return fail(BPCA_ERR_INVALID_ARGUMENT, "kmeans_tolerance must be > 0");
     if (chunk_size_kmeans < 1 || chunk_size_kmeans > num_cells)
         return fail(BPCA_ERR_INVALID_ARGUMENT, "chunk_size_kmeans must be >= 1 and <= the number of cells");
+    if (const char *e = barbar_arg_error(kmeans_barbar, num_init_iterations, oversampling_factor))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, e);
     std::lock_guard<std::recursive_mutex> guard(ctx->lock);
     DeviceGuard dg(ctx->device);
     begin_call(ctx);
     bpca::IvfParams P{num_cells, num_dims, 1, num_clusters, 1,
                       num_kmeans_iterations, kmeans_tolerance, chunk_size_kmeans, seed};
+    P.barbar = kmeans_barbar;
+    P.init_iterations = num_init_iterations;
+    P.oversampling = oversampling_factor;
     P.kmeans_labels = d_labels;
     P.kmeans_centroids = d_centroids;
     P.kmeans_counts = d_counts;
@@ -1801,6 +1849,40 @@ int bpca_kmeans(bpca_context *ctx, const float *d_X, int64_t num_cells,
     if (rc == 1) return fail(BPCA_ERR_INVALID_ARGUMENT, msg);
     if (rc == 2) return hip_fail(herr, "run_knn_ivf");
     *iterations_run = I.iterations;
+    return BPCA_OK;
+}
+
+int bpca_kmeans_barbar_init(bpca_context *ctx, const float *d_X, int64_t num_cells,
+                            int32_t num_dims, int32_t num_clusters,
+                            int32_t num_init_iterations, float oversampling_factor,
+                            int64_t chunk_size_kmeans, uint64_t seed, uint32_t *cells,
+                            int64_t *num_candidates) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    if (!d_X || !cells || !num_candidates) return fail(BPCA_ERR_INVALID_ARGUMENT, "NULL pointer");
+    if (num_cells < 1 || num_cells > (int64_t)UINT32_MAX) return fail(BPCA_ERR_INVALID_ARGUMENT, "num_cells out of range");
+    if (num_dims < 1 || num_dims > bpca::kKnnMfmaMaxDims)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "num_dims must be in [1, " + std::to_string(bpca::kKnnMfmaMaxDims) + "]");
+    if (num_clusters < 1 || (int64_t)num_clusters > num_cells)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "num_clusters must be >= 1 and <= the number of cells");
+    if (chunk_size_kmeans < 1 || chunk_size_kmeans > num_cells)
+        return fail(BPCA_ERR_INVALID_ARGUMENT, "chunk_size_kmeans must be >= 1 and <= the number of cells");
+    if (const char *e = barbar_arg_error(1, num_init_iterations, oversampling_factor))
+        return fail(BPCA_ERR_INVALID_ARGUMENT, e);
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    DeviceGuard dg(ctx->device);
+    begin_call(ctx);
+    bpca::IvfParams P{num_cells, num_dims, 1, num_clusters, 1, 1, 1.0f, chunk_size_kmeans, seed};
+    P.barbar = 1;
+    P.init_iterations = num_init_iterations;
+    P.oversampling = oversampling_factor;
+    std::vector<uint32_t> v;
+    std::string msg;
+    hipError_t herr = hipSuccess;
+    const int rc = bpca::kmeans_barbar_init_cells(d_X, nullptr, nullptr, P, &v, num_candidates, &msg, &herr,
+                                                  ctx->stream);
+    if (rc == 1) return fail(BPCA_ERR_INVALID_ARGUMENT, msg);
+    if (rc == 2) return hip_fail(herr, "kmeans_barbar_init_cells");
+    std::copy(v.begin(), v.end(), cells);
     return BPCA_OK;
 }
 

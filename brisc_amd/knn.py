@@ -71,14 +71,27 @@ def default_num_clusters(num_cells: int) -> int:
                                     num_cells / 100))))
 
 
+def _barbar_args(kmeans_barbar, num_init_iterations, oversampling_factor):
+    if not isinstance(kmeans_barbar, bool):
+        raise TypeError('kmeans_barbar must be Boolean')
+    if int(num_init_iterations) < 1:
+        raise ValueError('num_init_iterations must be a positive integer')
+    if not float(oversampling_factor) > 0:
+        raise ValueError('oversampling_factor must be a positive number')
+    return int(kmeans_barbar), int(num_init_iterations), \
+        float(oversampling_factor)
+
+
 def knn_ivf(X, num_neighbors: int, num_clusters: int | None = None,
             num_clusters_searched: int | None = None,
             num_kmeans_iterations: int = 2, kmeans_tolerance: float = 1e-2,
             chunk_size_kmeans: int | None = None, seed: int = 0,
-            context=None):
+            context=None, kmeans_barbar: bool = False,
+            num_init_iterations: int = 5, oversampling_factor: float = 1.0):
     """``SingleCell.neighbors()``'s default search on the device
-    (``bpca_knn_ivf``): k-means with random initialisation, then each cell
-    searches the cells of its ``num_clusters_searched`` nearest clusters.
+    (``bpca_knn_ivf_ex``): k-means (random initial cells, or k-means||
+    with ``kmeans_barbar=True``), then each cell searches the cells of its
+    ``num_clusters_searched`` nearest clusters.
 
     ``X``: float32 device tensor, ``num_cells x num_dims`` (``num_dims <=
     128``).  Defaults as the reference's: ``num_clusters`` =
@@ -87,7 +100,8 @@ def knn_ivf(X, num_neighbors: int, num_clusters: int | None = None,
     Returns ``(neighbors int32, distances float32, info)`` with ``info`` a
     dict: non-empty clusters, clusters searched, k-means iterations run,
     whether the exhaustive search ran (every cluster searched), and the
-    number of (workgroup, cluster) pairs scanned.
+    number of (workgroup, cluster) pairs scanned, and the k-means||
+    candidates selected (0 with random initial cells).
     """
     import ctypes
 
@@ -95,6 +109,8 @@ def knn_ivf(X, num_neighbors: int, num_clusters: int | None = None,
     import torch
 
     from .irlba import default_context
+    barbar = _barbar_args(kmeans_barbar, num_init_iterations,
+                          oversampling_factor)
     if not isinstance(X, torch.Tensor) or X.device.type != 'cuda':
         raise TypeError('X must be a device tensor')
     if X.dtype != torch.float32 or X.dim() != 2:
@@ -119,16 +135,17 @@ def knn_ivf(X, num_neighbors: int, num_clusters: int | None = None,
     ctx = context or default_context(X.device.index or 0)
     neighbors = torch.empty((n, k), dtype=torch.int32, device=X.device)
     distances = torch.empty((n, k), dtype=torch.float32, device=X.device)
-    info = np.zeros(5, np.int64)
-    _lib.check(_lib.lib().bpca_knn_ivf(
+    info = np.zeros(6, np.int64)
+    _lib.check(_lib.lib().bpca_knn_ivf_ex(
         ctx.handle, X.data_ptr(), n, d, k, C, searched,
         int(num_kmeans_iterations), ctypes.c_float(kmeans_tolerance), chunk,
-        ctypes.c_uint64(int(seed) & 0xffffffffffffffff),
+        ctypes.c_uint64(int(seed) & 0xffffffffffffffff), barbar[0],
+        barbar[1], ctypes.c_float(barbar[2]),
         neighbors.data_ptr(), distances.data_ptr(), info.ctypes.data))
     return neighbors, distances, dict(
         num_clusters=int(info[0]), num_clusters_searched=int(info[1]),
         kmeans_iterations=int(info[2]), exhaustive=bool(info[3]),
-        entries=int(info[4]))
+        entries=int(info[4]), kmeans_candidates=int(info[5]))
 
 
 def kmeans_init_cells(num_cells: int, num_clusters: int, seed: int = 0):
@@ -145,12 +162,46 @@ def kmeans_init_cells(num_cells: int, num_clusters: int, seed: int = 0):
     return out
 
 
+def kmeans_barbar_init(X, num_clusters: int, num_init_iterations: int = 5,
+                       oversampling_factor: float = 1.0,
+                       chunk_size: int | None = None, seed: int = 0,
+                       context=None):
+    """The cells k-means|| initialisation takes as centroids
+    (``brisc/kmeans.pyx:41-360``) from a float32 device tensor ``X``
+    (``num_dims <= 128``), through ``bpca_kmeans_barbar_init``.  Returns
+    ``(cells uint32 (num_clusters,), candidates selected)``."""
+    import ctypes
+
+    import numpy as np
+    import torch
+
+    from .irlba import default_context
+    _barbar_args(True, num_init_iterations, oversampling_factor)
+    if not isinstance(X, torch.Tensor) or X.device.type != 'cuda' or \
+            X.dtype != torch.float32 or X.dim() != 2:
+        raise TypeError('X must be a 2-D float32 device tensor')
+    n, d = X.shape
+    X = X.contiguous()
+    ctx = context or default_context(X.device.index or 0)
+    cells = np.empty(int(num_clusters), np.uint32)
+    S = ctypes.c_int64(0)
+    _lib.check(_lib.lib().bpca_kmeans_barbar_init(
+        ctx.handle, X.data_ptr(), n, d, int(num_clusters),
+        int(num_init_iterations), ctypes.c_float(oversampling_factor),
+        min(4096, n) if chunk_size is None else int(chunk_size),
+        ctypes.c_uint64(int(seed) & 0xffffffffffffffff), cells.ctypes.data,
+        ctypes.byref(S)))
+    return cells, int(S.value)
+
+
 def kmeans(X, num_clusters: int, num_kmeans_iterations: int = 2,
            kmeans_tolerance: float = 1e-2, chunk_size: int | None = None,
-           seed: int = 0, context=None):
-    """``kmeans()`` with random initialisation (``brisc/kmeans.pyx:889-1100``)
-    on a float32 device tensor ``X`` (``num_dims <= 128``), through
-    ``bpca_kmeans``.  Returns ``(labels int32 (n,), centroids float32
+           seed: int = 0, context=None, kmeans_barbar: bool = False,
+           num_init_iterations: int = 5, oversampling_factor: float = 1.0):
+    """``kmeans()`` (``brisc/kmeans.pyx:889-1100``; random initial cells, or
+    k-means|| with ``kmeans_barbar=True``) on a float32 device tensor ``X``
+    (``num_dims <= 128``), through ``bpca_kmeans_ex``.  Returns ``(labels
+    int32 (n,), centroids float32
     (num_clusters, d), counts int32 (num_clusters,), iterations run)`` --
     the labels and counts of a final E-step against the final centroids,
     as the reference leaves them."""
@@ -159,6 +210,8 @@ def kmeans(X, num_clusters: int, num_kmeans_iterations: int = 2,
     import torch
 
     from .irlba import default_context
+    barbar = _barbar_args(kmeans_barbar, num_init_iterations,
+                          oversampling_factor)
     if not isinstance(X, torch.Tensor) or X.device.type != 'cuda' or \
             X.dtype != torch.float32 or X.dim() != 2:
         raise TypeError('X must be a 2-D float32 device tensor')
@@ -170,10 +223,11 @@ def kmeans(X, num_clusters: int, num_kmeans_iterations: int = 2,
                        device=X.device)
     counts = torch.empty(int(num_clusters), dtype=torch.int32, device=X.device)
     run = ctypes.c_int32(0)
-    _lib.check(_lib.lib().bpca_kmeans(
+    _lib.check(_lib.lib().bpca_kmeans_ex(
         ctx.handle, X.data_ptr(), n, d, int(num_clusters),
         int(num_kmeans_iterations), ctypes.c_float(kmeans_tolerance),
         min(4096, n) if chunk_size is None else int(chunk_size),
-        ctypes.c_uint64(int(seed) & 0xffffffffffffffff), labels.data_ptr(),
+        ctypes.c_uint64(int(seed) & 0xffffffffffffffff), barbar[0],
+        barbar[1], ctypes.c_float(barbar[2]), labels.data_ptr(),
         cent.data_ptr(), counts.data_ptr(), ctypes.byref(run)))
     return labels, cent, counts, int(run.value)

@@ -450,8 +450,9 @@ class SingleCell:
         cells failing QC get ``UINT32_MAX`` and NaN.
 
         The search is the reference's: k-means (random initialisation drawn
-        with the reference's generator, ``num_kmeans_iterations`` Lloyd
-        iterations) into ``num_clusters`` clusters, then each cell searches
+        with the reference's generator, or k-means|| with
+        ``kmeans_barbar=True``; ``num_kmeans_iterations`` Lloyd iterations)
+        into ``num_clusters`` clusters, then each cell searches
         the cells of its ``num_clusters_searched`` nearest clusters
         (``brisc_amd.knn.knn_ivf``).  When every cluster is searched, or
         more than 64 would be, the exhaustive search runs instead, which
@@ -459,8 +460,7 @@ class SingleCell:
         neighbours are found in passes of 64 (each keeping only candidates
         after the previous pass's last distance and index).  Limits on the
         GPU: the PCs have at most 128 columns (256 with every cluster
-        searched and at most 64 neighbours); k-means|| initialisation
-        (``kmeans_barbar=True``) is not available.
+        searched and at most 64 neighbours).
         """
         import torch
 
@@ -563,9 +563,6 @@ class SingleCell:
         if PCs.shape[1] > max_dims:
             raise ValueError(f'obsm[{PC_key!r}] has {PCs.shape[1]:,} columns; '
                              f'at most {max_dims} are supported on the GPU')
-        if kmeans_barbar and not exhaustive:
-            raise ValueError('kmeans_barbar=True (k-means|| initialisation) '
-                             'is not supported on the GPU')
         X = torch.from_numpy(np.ascontiguousarray(PCs)).to('cuda')
         if exhaustive:
             nb, dist = knn_exact(X, num_neighbors)
@@ -575,7 +572,10 @@ class SingleCell:
                 num_clusters_searched=num_clusters_searched,
                 num_kmeans_iterations=num_kmeans_iterations,
                 kmeans_tolerance=kmeans_tolerance,
-                chunk_size_kmeans=chunk_size_kmeans, seed=seed)
+                chunk_size_kmeans=chunk_size_kmeans, seed=seed,
+                kmeans_barbar=kmeans_barbar,
+                num_init_iterations=num_init_iterations,
+                oversampling_factor=oversampling_factor)
             if verbose:
                 print(f'neighbors: {info["num_clusters"]:,} clusters, '
                       f'{info["num_clusters_searched"]} searched per cell, '

@@ -449,6 +449,21 @@ int bpca_knn_ivf(bpca_context *ctx, const float *d_X, int64_t num_cells,
                  uint64_t seed, uint32_t *d_neighbors, float *d_distances,
                  int64_t *info);
 
+/* bpca_knn_ivf with the initialisation choice of SingleCell.neighbors()
+ * (brisc/single_cell.py:12151-12155, 12532-12542): kmeans_barbar = 1 starts
+ * k-means from k-means|| (brisc/kmeans.pyx:41-720; see
+ * bpca_kmeans_barbar_init) with num_init_iterations (>= 1) and
+ * oversampling_factor (> 0); 0 draws random cells (the two other arguments
+ * unused).  info is int64[6]: bpca_knn_ivf's five, then the k-means||
+ * candidates selected (0 for random cells).  Synchronous. */
+int bpca_knn_ivf_ex(bpca_context *ctx, const float *d_X, int64_t num_cells,
+                    int32_t num_dims, int32_t k, int32_t num_clusters,
+                    int32_t num_clusters_searched, int32_t num_kmeans_iterations,
+                    float kmeans_tolerance, int64_t chunk_size_kmeans,
+                    uint64_t seed, int32_t kmeans_barbar,
+                    int32_t num_init_iterations, float oversampling_factor,
+                    uint32_t *d_neighbors, float *d_distances, int64_t *info);
+
 /* kmeans() with random initialisation (brisc/kmeans.pyx:889-1432), the
  * first stage of bpca_knn_ivf, on its own: d_labels (uint32 num_cells,
  * from the final E-step against the final centroids), d_centroids (float32
@@ -462,6 +477,38 @@ int bpca_kmeans(bpca_context *ctx, const float *d_X, int64_t num_cells,
                 int64_t chunk_size_kmeans, uint64_t seed, uint32_t *d_labels,
                 float *d_centroids, uint32_t *d_counts,
                 int32_t *iterations_run);
+
+/* bpca_kmeans with kmeans()'s kmeans_barbar, num_init_iterations and
+ * oversampling_factor arguments (brisc/kmeans.pyx:889-903, 938-948),
+ * meaning as in bpca_knn_ivf_ex.  Synchronous. */
+int bpca_kmeans_ex(bpca_context *ctx, const float *d_X, int64_t num_cells,
+                   int32_t num_dims, int32_t num_clusters,
+                   int32_t num_kmeans_iterations, float kmeans_tolerance,
+                   int64_t chunk_size_kmeans, uint64_t seed, int32_t kmeans_barbar,
+                   int32_t num_init_iterations, float oversampling_factor,
+                   uint32_t *d_labels, float *d_centroids, uint32_t *d_counts,
+                   int32_t *iterations_run);
+
+/* The cells k-means|| initialisation takes as centroids
+ * (kmeans_barbar_init, brisc/kmeans.pyx:41-360; kmeans_barbar_init_parallel,
+ * 362-720, takes the same), from d_X (float32 num_cells x num_dims,
+ * num_dims <= 128): rounds of draws from srand(seed + cell * round), each
+ * cell taken with probability oversampling_factor * num_clusters *
+ * distance / cost, until num_init_iterations rounds and at least
+ * num_clusters candidates (or every cell); then weighted k-means++ over the
+ * candidates.  Distances by explicit differences, fused, dimensions in
+ * order; the rounds' candidate updates by partial distances on the matrix
+ * cores; costs as chunk_size_kmeans chunk sums in order (k-means++: sums of
+ * 256-candidate halving trees in order); the k-means++ draw the reference's
+ * fused sequential chain.  cells: host uint32[num_clusters] out;
+ * *num_candidates: candidates selected.  The reference's errors (every
+ * cell at one point; fewer distinct cells than clusters) are
+ * BPCA_ERR_INVALID_ARGUMENT with its message.  Synchronous. */
+int bpca_kmeans_barbar_init(bpca_context *ctx, const float *d_X, int64_t num_cells,
+                            int32_t num_dims, int32_t num_clusters,
+                            int32_t num_init_iterations, float oversampling_factor,
+                            int64_t chunk_size_kmeans, uint64_t seed, uint32_t *cells,
+                            int64_t *num_candidates);
 
 /* The IRLBA start vector for `seed`: the num_genes FP32 normal draws that
  * brisc/pca.pyx:801-806 puts in V[:, 0] (srand and random_normal of

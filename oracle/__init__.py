@@ -24,6 +24,9 @@ Contents:
   (``brisc/kmeans.pyx:889-1432``: given initial cells, E-step by partial
   distance with ties to the lower index, M-step means, empty clusters moved
   to the farthest cells, a final E-step), in FP64.
+* ``kmeans_barbar_init`` -- k-means|| initialisation
+  (``brisc/kmeans.pyx:41-360``, ``kmeans_barbar.c``) in the operation order
+  the device path defines: the cells taken as initial centroids.
 * ``knn_fp64`` -- brute-force k nearest neighbours in FP64 (squared
   Euclidean, self excluded, ties by index): what ``knn_self``
   (``brisc/knn.pyx:13-516``) returns when every cluster is searched, up to the
@@ -69,6 +72,10 @@ def lib() -> ctypes.CDLL:
         L.irlba_port.argtypes = [p, p, p, u64, u32, u32, u32, ctypes.c_float,
                                  u32, u64, u32, u64, p, p, p, p, p]
         L.irlba_port.restype = ctypes.c_int
+        L.kmeans_barbar_init.argtypes = [p, i64, ctypes.c_int32, ctypes.c_int32,
+                                         ctypes.c_int32, ctypes.c_float, i64,
+                                         u64, p, p]
+        L.kmeans_barbar_init.restype = ctypes.c_int
         _LIB = L
     return _LIB
 
@@ -344,3 +351,34 @@ def kmeans_fp64(X: np.ndarray, init_cells, iterations: int):
         cent = sums
     labels = e_step(cent)
     return labels, cent, np.bincount(labels, minlength=C)
+
+
+def kmeans_barbar_init(X: np.ndarray, num_clusters: int,
+                       num_init_iterations: int = 5,
+                       oversampling_factor: float = 1.0,
+                       chunk_size: int | None = None, seed: int = 0):
+    """The cells k-means|| initialisation takes as initial centroids
+    (``brisc/kmeans.pyx:41-360``), and the number of candidates it selected
+    before the weighted k-means++ pass.  Raises ValueError where the
+    reference does: every cell at one point, or fewer distinct cells than
+    clusters."""
+    X = np.ascontiguousarray(X, np.float32)
+    n, d = X.shape
+    chunk = min(4096, n) if chunk_size is None else chunk_size
+    cells = np.empty(num_clusters, np.uint32)
+    S = ctypes.c_int64(0)
+    rc = lib().kmeans_barbar_init(_ptr(X), n, d, num_clusters,
+                                  num_init_iterations,
+                                  float(oversampling_factor), chunk,
+                                  seed & (2 ** 64 - 1), _ptr(cells),
+                                  ctypes.addressof(S))
+    if rc == 1:
+        raise ValueError('all cells have the same principal component '
+                         'loadings')
+    if rc == 2:
+        raise ValueError(f'num_clusters ({num_clusters:,}) is greater than '
+                         f'the number of cells with distinct principal '
+                         f'component loadings; decrease num_clusters')
+    if rc != 0:
+        raise MemoryError('kmeans_barbar_init: out of memory')
+    return cells, int(S.value)

@@ -11,6 +11,7 @@ Run in the development container only (it reads the reference sources):
     python tests/golden/make_ref_vectors.py --rng       (adds the start-vector draws)
     python tests/golden/make_ref_vectors.py --calibrate (times the reference against the port)
     python tests/golden/make_ref_vectors.py --normalize (adds the normalisation fixtures)
+    python tests/golden/make_ref_vectors.py --kmeans-barbar (adds the k-means|| fixtures)
 
 It cythonizes ``brisc/cyutils.pyx``, ``brisc/pca.pyx``,
 ``brisc/normalize.pyx``, ``brisc/kmeans.pyx`` and ``brisc/knn.pyx`` from the
@@ -240,9 +241,11 @@ def ref_neighbors(kmeans_mod, knn_mod, X, k, num_clusters_searched=None,
 
 
 def ref_kmeans(kmeans_mod, X, num_clusters, iterations, seed=0,
-               num_threads=8):
-    """kmeans() with random initialisation, as neighbors() calls it:
-    labels (last E-step), centroids, counts and iterations run."""
+               num_threads=8, kmeans_barbar=False, num_init_iterations=5,
+               oversampling_factor=1.0, chunk_size=None):
+    """kmeans() as neighbors() calls it (random initialisation unless
+    ``kmeans_barbar``): labels (last E-step), centroids, counts and
+    iterations run."""
     from threadpoolctl import threadpool_limits
     n, d = X.shape
     labels = np.zeros(n, np.uint32)
@@ -253,11 +256,98 @@ def ref_kmeans(kmeans_mod, X, num_clusters, iterations, seed=0,
         it = kmeans_mod.kmeans(
             X=X, cluster_labels=labels, centroids=cent, centroids_new=cent_new,
             num_cells_per_cluster=counts, min_distances=np.zeros(n, np.float32),
-            cell_norms=np.zeros(n, np.float32), kmeans_barbar=False,
-            num_init_iterations=5, num_kmeans_iterations=iterations,
-            tolerance=1e-2, oversampling_factor=1, seed=seed,
-            chunk_size=min(4096, n), is_mac=False, num_threads=num_threads)
+            cell_norms=np.zeros(n, np.float32), kmeans_barbar=kmeans_barbar,
+            num_init_iterations=num_init_iterations,
+            num_kmeans_iterations=iterations, tolerance=1e-2,
+            oversampling_factor=oversampling_factor, seed=seed,
+            chunk_size=min(4096, n) if chunk_size is None else chunk_size,
+            is_mac=False, num_threads=num_threads)
     return labels, (cent_new if it & 1 else cent), counts, it
+
+
+# k-means|| inputs on integer lattices (every distance and cost exact in FP32):
+# (name, seed, cells, dims, centres, clusters, init iterations, oversampling,
+#  k-means seed, chunk sizes)
+KMEANS_BARBAR = [
+    ('kb_a', 41, 3000, 8, 12, 40, 5, 1.0, 0, (3000, 512)),
+    ('kb_b', 42, 20000, 10, 30, 120, 3, 2.0, 5, (4096, 1000)),
+    ('kb_c', 43, 60, 3, 4, 10, 5, 1.0, 7, (60, 7)),
+    ('kb_d', 44, 5000, 6, 8, 1, 5, 0.5, 3, (4096,)),
+    ('kb_e', 45, 8000, 16, 40, 300, 2, 1.5, 11, (4096, 333)),
+    ('kb_f', 46, 100000, 6, 50, 1000, 5, 2.0, 13, (4096,)),
+]
+
+
+def kmeans_barbar_fixtures():
+    """kmeans() with k-means|| initialisation (kmeans.pyx:41-720) and one
+    Lloyd iteration on integer-lattice points, serial (one thread) and
+    parallel (several chunks, eight threads): the centroids after the
+    iteration (means of the cells nearest each initial centroid, which pins
+    the initial centroids), the last E-step's labels and counts.  The inputs
+    are regenerated by helpers.lattice_points (SHA-256 recorded)."""
+    kmeans_mod, _ = import_knn()
+    arrays, meta = {}, []
+    for (name, seed, n, d, centres, C, iters, over, kseed, chunks) in KMEANS_BARBAR:
+        lattice = (2, 1.0) if n > 50000 else (4, 1.5)
+        X = helpers.lattice_points(seed, n, d, centres, *lattice)
+        assert np.abs(X).max() < 128
+        runs = []
+        for chunk in chunks:
+            for threads in (1, 8):
+                t0 = time.perf_counter()
+                out = ref_kmeans(kmeans_mod, X, C, 1, seed=kseed,
+                                 num_threads=threads, kmeans_barbar=True,
+                                 num_init_iterations=iters,
+                                 oversampling_factor=over, chunk_size=chunk)
+                runs.append((chunk, threads, time.perf_counter() - t0, out))
+        l0, c0, n0, _ = runs[0][3]
+        for chunk, threads, _, (l, c, cnt, it) in runs:
+            assert it == 1
+            assert np.array_equal(c.view(np.uint32), c0.view(np.uint32)), \
+                (name, chunk, threads, 'centroids differ')
+            assert np.array_equal(l, l0) and np.array_equal(cnt, n0)
+        import hashlib
+        arrays[f'{name}_centroids'] = c0
+        arrays[f'{name}_labels'] = l0.astype(np.uint16)
+        arrays[f'{name}_counts'] = n0
+        meta.append(dict(name=name, seed=seed, num_cells=n, num_dims=d,
+                         centres=centres, lattice=list(lattice),
+                         X_sha256=hashlib.sha256(X.tobytes()).hexdigest(),
+                         num_clusters=C,
+                         num_init_iterations=iters, oversampling_factor=over,
+                         kmeans_seed=kseed, chunk_sizes=list(chunks),
+                         seconds=[round(r[2], 4) for r in runs]))
+        print(name, [round(r[2], 3) for r in runs], flush=True)
+    # Every cell at one point: the reference's error.
+    Xs = np.ones((50, 4), np.float32)
+    try:
+        ref_kmeans(kmeans_mod, Xs, 5, 1, kmeans_barbar=True)
+        same_error = None
+    except ValueError as e:
+        same_error = str(e)
+    # Five distinct points, 40 copies each: the reference's outcome (error
+    # message or centroids after one iteration) for a few cluster counts.
+    Xd = np.ascontiguousarray(np.repeat(
+        np.array([[0, 0], [3, 1], [-2, 4], [5, -3], [1, 1]], np.float32),
+        40, axis=0))
+    arrays['kb_dup_X'] = Xd.astype(np.int8)
+    dup = {}
+    for C in (1, 4, 5, 8):
+        try:
+            _, c, _, _ = ref_kmeans(kmeans_mod, Xd, C, 1, kmeans_barbar=True)
+            arrays[f'kb_dup_{C}_centroids'] = c
+            dup[str(C)] = 'centroids'
+        except ValueError as e:
+            dup[str(C)] = str(e)
+    np.savez_compressed(os.path.join(HERE, 'ref_kmeans_barbar.npz'), **arrays)
+    path = os.path.join(HERE, 'ref_vectors.json')
+    allmeta = json.load(open(path))
+    allmeta['kmeans_barbar'] = meta
+    allmeta['kmeans_barbar_same_point_error'] = same_error
+    allmeta['kmeans_barbar_duplicates'] = dup
+    print(dup)
+    json.dump(allmeta, open(path, 'w'), indent=1)
+    print(json.dumps(meta, indent=1), same_error)
 
 
 def knn_fixtures():
@@ -604,6 +694,8 @@ def main():
                     help='only time the reference against the oracle port (50k cells)')
     ap.add_argument('--normalize', action='store_true',
                     help='only add the normalisation fixtures (ref_normalize.npz)')
+    ap.add_argument('--kmeans-barbar', action='store_true',
+                    help='only add the k-means|| fixtures (ref_kmeans_barbar.npz)')
     args = ap.parse_args()
     build_reference(args.reference)
     pca_mod, norm_mod = import_reference()
@@ -613,6 +705,8 @@ def main():
         return calibrate_port(pca_mod, norm_mod)
     if args.normalize:
         return normalize_fixture(norm_mod)
+    if args.kmeans_barbar:
+        return kmeans_barbar_fixtures()
     if args.cells1m:
         return cells1m(pca_mod, norm_mod)
     if args.knn:

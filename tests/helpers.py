@@ -75,6 +75,19 @@ def knn_points(seed, n, d, centers, spread=0.5):
     return synth.pc_mixture(seed, n, d, centers, spread)
 
 
+def lattice_points(seed, n, d, centres, half_width=4, spread=1.5):
+    """Integer-valued FP32 points: ``centres`` random lattice centres in
+    [-half_width, half_width]^d, each point a centre plus rounded Gaussian
+    noise.  Squared distances, norms and dot products of such points are
+    exact in FP32 in any summation order while their sums stay below 2^24,
+    so k-means initialisations on them can be compared bit for bit."""
+    rng = np.random.default_rng(seed)
+    cen = rng.integers(-half_width, half_width + 1, (centres, d))
+    X = cen[rng.integers(0, centres, n)] + \
+        np.round(rng.standard_normal((n, d)) * spread)
+    return np.ascontiguousarray(X.astype(np.float32))
+
+
 def knn_sets_agree(X, nb_a, nb_b, rtol, rows=None):
     """Whether two neighbour lists of the same points (row i answering query
     ``rows[i]``, default i) differ only by near-ties: every neighbour in one

@@ -325,3 +325,156 @@ def test_kmeans_init_draw_matches_reference(golden):
     other = kmeans_init_cells(len(X), C, 1).astype(np.int64)
     lo, _, _ = oracle.kmeans_fp64(X, other, 1)
     assert (lo == ref['knn_c_kmeans1_labels']).mean() < 0.5
+
+
+# ----------------------------------------------------------- k-means||
+
+def _kb_fixtures():
+    import json
+    here = os.path.join(os.path.dirname(__file__), 'golden')
+    meta = json.load(open(os.path.join(here, 'ref_vectors.json')))
+    return meta['kmeans_barbar'], meta['kmeans_barbar_same_point_error'], \
+        np.load(os.path.join(here, 'ref_kmeans_barbar.npz'))
+
+
+def _kb_points(m):
+    import hashlib
+    X = helpers.lattice_points(m['seed'], m['num_cells'], m['num_dims'],
+                               m['centres'], *m['lattice'])
+    assert hashlib.sha256(X.tobytes()).hexdigest() == m['X_sha256']
+    return X
+
+
+def one_lloyd_step_fp32(X, cells):
+    """Centroids after one Lloyd iteration from the given initial cells, as
+    ``kmeans()`` computes them on integer-lattice points (where every
+    distance and sum is exact): nearest initial centroid (lower index on
+    ties), exact sums, times the reciprocal count rounded to FP32."""
+    X64 = X.astype(np.float64)
+    cent = X64[np.asarray(cells, np.int64)]
+    lab = ((cent * cent).sum(1)[None, :] - 2.0 * X64 @ cent.T).argmin(1)
+    C = len(cells)
+    cnt = np.bincount(lab, minlength=C)
+    sums = np.zeros((C, X.shape[1]))
+    np.add.at(sums, lab, X64)
+    assert (cnt > 0).all()
+    return sums.astype(np.float32) * \
+        (1.0 / cnt).astype(np.float32)[:, None]
+
+
+def test_kmeans_barbar_oracle_matches_reference():
+    # On integer lattices the reference's k-means|| is exact except for its
+    # fused k-means++ probability chain: the centroids after one Lloyd
+    # iteration from the oracle's cells must equal the reference's bit for
+    # bit, for every chunk size it was run with (serial and parallel
+    # versions, which the generator checked agree).  kb_f is large enough
+    # that the chain's rounding decides some draws.
+    meta, _, ref = _kb_fixtures()
+    for m in meta:
+        name = m['name']
+        X = _kb_points(m)
+        for chunk in m['chunk_sizes']:
+            cells, S = oracle.kmeans_barbar_init(
+                X, m['num_clusters'], m['num_init_iterations'],
+                m['oversampling_factor'], chunk, m['kmeans_seed'])
+            assert len(np.unique(cells)) == m['num_clusters']
+            assert S >= m['num_clusters']
+            c = one_lloyd_step_fp32(X, cells)
+            assert np.array_equal(c.view(np.uint32),
+                                  ref[f'{name}_centroids'].view(np.uint32)), \
+                (name, chunk)
+
+
+def test_kmeans_barbar_oracle_other_seed_differs():
+    meta, _, ref = _kb_fixtures()
+    m = next(m for m in meta if m['name'] == 'kb_a')
+    X = _kb_points(m)
+    cells, _ = oracle.kmeans_barbar_init(
+        X, m['num_clusters'], m['num_init_iterations'],
+        m['oversampling_factor'], None, m['kmeans_seed'] + 1)
+    c = one_lloyd_step_fp32(X, cells)
+    same = (c.view(np.uint32) == ref['kb_a_centroids'].view(np.uint32)).all(1)
+    assert same.mean() < 0.5
+
+
+def test_kmeans_barbar_oracle_special_cases():
+    # Every cell at one point, and five distinct points (40 copies each)
+    # with 1, 4, 5 and 8 clusters: the reference's error or its centroids
+    # after one iteration (recorded by the fixture generator).  With five
+    # clusters the reference stalls before its fifth round and raises, as
+    # the oracle must.
+    _, same_point_error, ref = _kb_fixtures()
+    with pytest.raises(ValueError, match='same principal component'):
+        oracle.kmeans_barbar_init(np.ones((50, 4), np.float32), 5)
+    assert 'same principal component' in same_point_error
+    import json
+    meta = json.load(open(os.path.join(os.path.dirname(__file__), 'golden',
+                                       'ref_vectors.json')))
+    X = ref['kb_dup_X'].astype(np.float32)
+    for C, outcome in meta['kmeans_barbar_duplicates'].items():
+        C = int(C)
+        if outcome == 'centroids':
+            cells, _ = oracle.kmeans_barbar_init(X, C)
+            c = one_lloyd_step_fp32(X, cells)
+            assert np.array_equal(c.view(np.uint32),
+                                  ref[f'kb_dup_{C}_centroids'].view(np.uint32)), C
+        else:
+            with pytest.raises(ValueError) as e:
+                oracle.kmeans_barbar_init(X, C)
+            assert str(e.value) == outcome, C
+
+
+def test_kmeans_barbar_first_rounds_closed_form():
+    # Two rounds by hand on small integer points (every distance exact): the
+    # first candidate is randint(n) from srand(seed - 1); round one takes
+    # each cell whose uniform from srand(seed + i) lies at or below
+    # l d_i / cost; round two (seeds seed + i * 1, the same uniforms) does
+    # the same with each distance lowered to the nearest new candidate and
+    # taken cells at 0; with num_init_iterations = 2 the selection then
+    # stops.  With one cluster the output is candidate randint(S) from
+    # srand(seed - 2).
+    def pcg(state):
+        x = state
+        state = (x * 6364136223846793005 + 1442695040888963407) % 2 ** 64
+        s = ((x ^ (x >> 18)) >> 27) & 0xffffffff
+        rot = x >> 59
+        return ((s >> rot) | (s << ((-rot) & 31))) & 0xffffffff, state
+
+    def srand(seed):
+        _, st = pcg((seed + 1442695040888963407) % 2 ** 64)
+        return st
+
+    def bounded(bound, st):
+        r, st = pcg(st)
+        m = r * bound
+        lo = m & 0xffffffff
+        if lo < bound:
+            thr = (2 ** 32 - bound) % bound
+            while lo < thr:
+                r, st = pcg(st)
+                m = r * bound
+                lo = m & 0xffffffff
+        return m >> 32, st
+
+    rng = np.random.default_rng(3)
+    X = rng.integers(-3, 4, (40, 3)).astype(np.float32)
+    seed = 17
+    l = np.float32(2.0)
+    u = np.array([np.float32(pcg(srand(seed + i))[0]) / np.float32(2 ** 32)
+                  for i in range(40)], np.float32)
+    first, _ = bounded(40, srand((seed - 1) % 2 ** 64))
+    d = ((X - X[first]) ** 2).sum(1).astype(np.float32)
+    cand = [first]
+    for rnd in range(2):
+        if rnd:
+            for c in new:
+                d = np.minimum(d, ((X - X[c]) ** 2).sum(1).astype(np.float32))
+        cost = np.float32(d.sum())
+        new = [i for i in range(40) if np.float32(l / cost) * d[i] >= u[i]]
+        d[new] = 0
+        cand += new
+    cells, S = oracle.kmeans_barbar_init(X, 1, num_init_iterations=2,
+                                         oversampling_factor=2.0, seed=seed)
+    assert S == len(cand)
+    pick, _ = bounded(S, srand((seed - 2) % 2 ** 64))
+    assert cells[0] == cand[pick]
