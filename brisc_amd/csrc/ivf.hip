@@ -437,6 +437,7 @@ __global__ __launch_bounds__(256) void k_kpp_update(const float *__restrict__ ca
     if (i < S) {
         float diff = candT[i] - candT[c];
         float dist = diff * diff;
+#pragma unroll 8
         for (int j = 1; j < d; j++) {
             diff = candT[(int64_t)j * S + i] - candT[(int64_t)j * S + c];
             dist = fmaf(diff, diff, dist);
@@ -464,70 +465,116 @@ constexpr int kKppSeg = 4096;  // candidates staged in LDS per segment
 // One k-means++ draw (kmeans.pyx:307-321): cost = the tree sums in order,
 // probability = u, then probability -= m2[i] / cost (times 1 / cost, fused)
 // for i in order until it goes negative; the last candidate if it never
-// does.  The chosen candidate's m2 becomes 0.  The four waves stage the
-// candidates in LDS; the first walks the chain, 64 candidates at a time
-// (the chain never increases, so a block whose end stays non-negative holds
-// no crossing).
-__global__ __launch_bounds__(256) void k_kpp_draw(float *__restrict__ m2, int64_t S,
+// does.  The chosen candidate's m2 becomes 0.  The first wave walks the
+// chain over a segment of candidates in LDS, 64 at a time (the chain never
+// increases, so a block whose end stays non-negative holds no crossing),
+// while the other three stage the next segment.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(1, 1))) void k_kpp_draw(float *__restrict__ m2, int64_t S,
                                                   const float *__restrict__ partials,
                                                   int G, float u, uint32_t *__restrict__ pos,
                                                   const uint32_t *__restrict__ sel,
                                                   uint32_t *__restrict__ cell_out) {
-    __shared__ __align__(16) float seg[kKppSeg];
+    __shared__ __align__(16) float seg[2][kKppSeg];
     __shared__ float sh_p, sh_inv;
-    __shared__ int64_t sh_pick;
+    __shared__ int64_t sh_pick[2];  // by segment parity: read after a barrier, rewritten two later
     const int tid = threadIdx.x;
+    const int wv = tid >> 6, lane = tid & (kWave - 1);
+    // The tree sums, 1024 at a time through LDS (loaded by every thread),
+    // added in order by the first wave.
+    float cost = 0.0f;
+    for (int b0 = 0; b0 < G; b0 += 1024) {
+        for (int k = tid; k < 1024; k += 256) seg[0][k] = b0 + k < G ? partials[b0 + k] : 0.0f;
+        __syncthreads();
+        if (wv == 0) {
+            const float4 *v = (const float4 *)seg[0];
+            const int nv = (G - b0 < 1024 ? G - b0 + 3 : 1024) / 4;
+            for (int t = 0; t < nv; t++) {
+                const float4 a = v[t];
+                cost += a.x;
+                cost += a.y;
+                cost += a.z;
+                cost += a.w;
+            }
+        }
+        __syncthreads();
+    }
     if (tid == 0) {
-        float cost = 0.0f;
-        for (int b = 0; b < G; b++) cost += partials[b];
         sh_inv = 1.0f / cost;
         sh_p = u;
-        sh_pick = -1;
     }
+    const int nseg = (int)((S + kKppSeg - 1) / kKppSeg);
+    auto stage = [&](int sidx, int t0, int nt) {
+        const int64_t s0 = (int64_t)sidx * kKppSeg;
+        const int len = (int)(S - s0 < kKppSeg ? S - s0 : kKppSeg);
+        float *dst = seg[sidx & 1];
+        for (int k = t0; k < kKppSeg; k += nt) dst[k] = k < len ? m2[s0 + k] : 0.0f;
+    };
+    stage(0, tid, 256);
     __syncthreads();
     const float inv = sh_inv;
-    for (int64_t s0 = 0; s0 < S; s0 += kKppSeg) {
-        const int len = (int)(S - s0 < kKppSeg ? S - s0 : kKppSeg);
-        for (int k = tid; k < kKppSeg; k += 256) seg[k] = k < len ? m2[s0 + k] : 0.0f;
-        __syncthreads();
-        if (tid < kWave) {
+    int64_t chosen = -1;
+    for (int sidx = 0; sidx < nseg; sidx++) {
+        if (wv == 0) {
+            const int64_t s0 = (int64_t)sidx * kKppSeg;
+            const int len = (int)(S - s0 < kKppSeg ? S - s0 : kKppSeg);
+            const float *cur = seg[sidx & 1];
+            const float4 *v = (const float4 *)cur;
             float p = sh_p;
             int64_t pick = -1;
             const int nblk = (len + 63) / 64;
-            for (int b = 0; b < nblk; b++) {
-                const float4 *v = (const float4 *)(seg + 64 * b);
+            // Two register sets in turn: block b + 1's loads are in flight
+            // while block b's chain runs.  Returns true at a crossing.
+            auto walk = [&](const float4 *x, int b) -> bool {
                 float q = p;
 #pragma unroll
                 for (int t = 0; t < 16; t++) {
-                    const float4 a = v[t];
-                    q = fmaf(-a.x, inv, q);
-                    q = fmaf(-a.y, inv, q);
-                    q = fmaf(-a.z, inv, q);
-                    q = fmaf(-a.w, inv, q);
+                    q = fmaf(-x[t].x, inv, q);
+                    q = fmaf(-x[t].y, inv, q);
+                    q = fmaf(-x[t].z, inv, q);
+                    q = fmaf(-x[t].w, inv, q);
                 }
                 if (q < 0.0f) {
                     q = p;
                     for (int t = 0; t < 64; t++) {
-                        q = fmaf(-seg[64 * b + t], inv, q);
+                        q = fmaf(-cur[64 * b + t], inv, q);
                         if (q < 0.0f) {
-                            pick = s0 + 64 * b + t;
+                            pick = (int64_t)sidx * kKppSeg + 64 * b + t;
                             break;
                         }
                     }
-                    break;
+                    return true;
                 }
                 p = q;
+                return false;
+            };
+            float4 A[16], B[16];
+#pragma unroll
+            for (int t = 0; t < 16; t++) A[t] = v[t];
+            for (int b = 0; b < nblk; b += 2) {
+                if (b + 1 < nblk) {
+#pragma unroll
+                    for (int t = 0; t < 16; t++) B[t] = v[16 * (b + 1) + t];
+                }
+                if (walk(A, b) || b + 1 >= nblk) break;
+                if (b + 2 < nblk) {
+#pragma unroll
+                    for (int t = 0; t < 16; t++) A[t] = v[16 * (b + 2) + t];
+                }
+                if (walk(B, b + 1)) break;
             }
-            if (tid == 0) {
+            if (lane == 0) {
                 sh_p = p;
-                sh_pick = pick;
+                sh_pick[sidx & 1] = pick;
             }
+        } else if (sidx + 1 < nseg) {
+            stage(sidx + 1, tid - kWave, 256 - kWave);
         }
         __syncthreads();
-        if (sh_pick >= 0) break;
+        chosen = sh_pick[sidx & 1];
+        if (chosen >= 0) break;
     }
     if (tid == 0) {
-        const int64_t pick = sh_pick >= 0 ? sh_pick : S - 1;
+        const int64_t pick = chosen >= 0 ? chosen : S - 1;
         m2[pick] = 0.0f;
         *pos = (uint32_t)pick;
         *cell_out = sel[pick];

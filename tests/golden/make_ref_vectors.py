@@ -12,6 +12,7 @@ Run in the development container only (it reads the reference sources):
     python tests/golden/make_ref_vectors.py --calibrate (times the reference against the port)
     python tests/golden/make_ref_vectors.py --normalize (adds the normalisation fixtures)
     python tests/golden/make_ref_vectors.py --kmeans-barbar (adds the k-means|| fixtures)
+    python tests/golden/make_ref_vectors.py --knn-large-barbar (times the 1M-cell k-means|| search)
 
 It cythonizes ``brisc/cyutils.pyx``, ``brisc/pca.pyx``,
 ``brisc/normalize.pyx``, ``brisc/kmeans.pyx`` and ``brisc/knn.pyx`` from the
@@ -176,7 +177,8 @@ def import_knn():
 
 
 def ref_neighbors(kmeans_mod, knn_mod, X, k, num_clusters_searched=None,
-                  num_threads=8, seed=0, num_clusters=None):
+                  num_threads=8, seed=0, num_clusters=None,
+                  kmeans_barbar=False):
     """The steps of SingleCell.neighbors() (single_cell.py:12452-12650) with
     its defaults, on PCs ``X``; ``num_clusters_searched='all'`` searches every
     cluster (the exhaustive search).  Returns neighbours (uint32), squared
@@ -195,7 +197,8 @@ def ref_neighbors(kmeans_mod, knn_mod, X, k, num_clusters_searched=None,
         it = kmeans_mod.kmeans(
             X=X, cluster_labels=labels, centroids=cent, centroids_new=cent_new,
             num_cells_per_cluster=counts, min_distances=min_dist,
-            cell_norms=norms, kmeans_barbar=False, num_init_iterations=5,
+            cell_norms=norms, kmeans_barbar=kmeans_barbar,
+            num_init_iterations=5,
             num_kmeans_iterations=2, tolerance=1e-2, oversampling_factor=1,
             seed=seed, chunk_size=min(4096, n), is_mac=False,
             num_threads=num_threads)
@@ -489,6 +492,33 @@ def knn_large():
     print(json.dumps(meta['knn_large'], indent=1))
 
 
+def knn_large_barbar():
+    """The reference's neighbour search with k-means|| initialisation
+    (kmeans_barbar=True) on the same 1M-cell PC mixture: seconds on this
+    host's 8 threads and the recall of a row sample against the exact
+    neighbours (FP64)."""
+    kmeans_mod, knn_mod = import_knn()
+    name, seed, n, d, centers, k, spread = KNN_LARGE
+    X = synth.pc_mixture(seed, n, d, centers, spread)
+    t0 = time.perf_counter()
+    nb, dist, C = ref_neighbors(kmeans_mod, knn_mod, X, k, kmeans_barbar=True)
+    secs = time.perf_counter() - t0
+    rows = np.random.Generator(np.random.PCG64(3)).choice(
+        n, KNN_ROW_SAMPLE, replace=False)
+    rows.sort()
+    exact_i, _ = oracle.knn_fp64(X, k, rows=rows)
+    recall = float(np.mean([len(set(a) & set(b)) / k for a, b in
+                            zip(nb[rows].tolist(), exact_i.tolist())]))
+    path = os.path.join(HERE, 'ref_vectors.json')
+    with open(path) as f:
+        meta = json.load(f)
+    meta['knn_large']['seconds_kmeans_barbar_8_threads'] = round(secs, 3)
+    meta['knn_large']['recall_kmeans_barbar_rows'] = round(recall, 5)
+    with open(path, 'w') as f:
+        json.dump(meta, f, indent=1)
+    print(json.dumps(meta['knn_large'], indent=1))
+
+
 def pipeline1m(pca_mod, norm_mod):
     """The reference's normalize -> irlba -> neighbors chain on the 1M-cell
     workload (SingleCell.pca() + SingleCell.neighbors() defaults, all genes,
@@ -694,6 +724,8 @@ def main():
                     help='only time the reference against the oracle port (50k cells)')
     ap.add_argument('--normalize', action='store_true',
                     help='only add the normalisation fixtures (ref_normalize.npz)')
+    ap.add_argument('--knn-large-barbar', action='store_true',
+                    help='only time the 1M-cell search with k-means|| initialisation')
     ap.add_argument('--kmeans-barbar', action='store_true',
                     help='only add the k-means|| fixtures (ref_kmeans_barbar.npz)')
     args = ap.parse_args()
@@ -707,6 +739,8 @@ def main():
         return normalize_fixture(norm_mod)
     if args.kmeans_barbar:
         return kmeans_barbar_fixtures()
+    if args.knn_large_barbar:
+        return knn_large_barbar()
     if args.cells1m:
         return cells1m(pca_mod, norm_mod)
     if args.knn:
