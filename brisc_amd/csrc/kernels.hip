@@ -1697,16 +1697,22 @@ __device__ void jacobi_svd(double *W, double *Jv, int m, int mm, int *sweeps,
     if (threadIdx.x == 0) *sweeps = sweep + 1;
 }
 
-// Same rotations with W and Jv in LDS and one 16-lane group per pair (up to
-// blockDim/16 pairs at once, all m/2 pairs of a round for m <= 128 with 1024
-// threads).  Partial sums over a pair's rows run lane-strided, then the xor
-// butterfly 8,4,2,1 inside the group (DPP moves within the 16-lane row).
+// Same rotations with W and Jv in LDS (mm <= 128) and one 32-lane group per
+// pair, up to blockDim/32 pairs at once.  A lane holds its (at most 4) rows of
+// the pair's W columns in registers: one batch of LDS loads, the partial sums
+// lane-strided, then the xor butterfly 16,8,4,2,1 (a swizzle, then DPP moves
+// within the 16-lane row); a rotating pair then loads its Jv columns and
+// stores all four.  Rows past the end enter the sums as exact zeros.  A 32-lane
+// group's 256 contiguous bytes span every LDS bank once, so the two groups of
+// a wave never conflict (16-lane groups of columns 100 doubles apart did: the
+// Jacobi took 1.65x as long).
 __device__ void jacobi_svd_lds(double *W, double *Jv, int m, int mm,
                                int *sweeps, int *rot_count) {
-    const int sub = threadIdx.x & 15;
-    const int grp = threadIdx.x >> 4;
-    const int ngrp = blockDim.x >> 4;
+    const int sub = threadIdx.x & 31;
+    const int grp = threadIdx.x >> 5;
+    const int ngrp = blockDim.x >> 5;
     const double tol = 2.220446049250313e-16 * sqrt((double)m);
+    constexpr int kR = 4;   // rows per lane: mm <= 128 (larger mm: jacobi_svd)
     int sweep = 0;
     for (; sweep < 60; sweep++) {
         if (threadIdx.x == 0) *rot_count = 0;
@@ -1721,15 +1727,23 @@ __device__ void jacobi_svd_lds(double *W, double *Jv, int m, int mm,
                     q = t;
                 }
                 double *wp = W + p * m, *wq = W + q * m;
+                double *jp = Jv + p * mm, *jq = Jv + q * mm;
                 double a = 0.0, b = 0.0, c = 0.0;
-                for (int i = sub; i < m; i += 16) {
-                    const double x = wp[i], y = wq[i];
-                    a = fma(x, x, a);
-                    b = fma(y, y, b);
-                    c = fma(x, y, c);
+                double x[kR], y[kR], jx[kR], jy[kR];
+#pragma unroll
+                for (int t = 0; t < kR; t++) {
+                    const int i = sub + 32 * t;
+                    x[t] = i < m ? wp[i] : 0.0;
+                    y[t] = i < m ? wq[i] : 0.0;
                 }
 #pragma unroll
-                for (int off = 8; off > 0; off >>= 1) {
+                for (int t = 0; t < kR; t++) {
+                    a = fma(x[t], x[t], a);
+                    b = fma(y[t], y[t], b);
+                    c = fma(x[t], y[t], c);
+                }
+#pragma unroll
+                for (int off = 16; off > 0; off >>= 1) {
                     a += xor_lane_f64(a, off);
                     b += xor_lane_f64(b, off);
                     c += xor_lane_f64(c, off);
@@ -1740,16 +1754,23 @@ __device__ void jacobi_svd_lds(double *W, double *Jv, int m, int mm,
                                      (fabs(zeta) + sqrt(1.0 + zeta * zeta));
                     const double cs = 1.0 / sqrt(1.0 + t * t);
                     const double sn = cs * t;
-                    for (int i = sub; i < m; i += 16) {
-                        const double x = wp[i], y = wq[i];
-                        wp[i] = cs * x - sn * y;
-                        wq[i] = sn * x + cs * y;
+#pragma unroll
+                    for (int u = 0; u < kR; u++) {
+                        const int i = sub + 32 * u;
+                        jx[u] = i < mm ? jp[i] : 0.0;
+                        jy[u] = i < mm ? jq[i] : 0.0;
                     }
-                    double *jp = Jv + p * mm, *jq = Jv + q * mm;
-                    for (int i = sub; i < mm; i += 16) {
-                        const double x = jp[i], y = jq[i];
-                        jp[i] = cs * x - sn * y;
-                        jq[i] = sn * x + cs * y;
+#pragma unroll
+                    for (int u = 0; u < kR; u++) {
+                        const int i = sub + 32 * u;
+                        if (i < m) {
+                            wp[i] = cs * x[u] - sn * y[u];
+                            wq[i] = sn * x[u] + cs * y[u];
+                        }
+                        if (i < mm) {
+                            jp[i] = cs * jx[u] - sn * jy[u];
+                            jq[i] = sn * jx[u] + cs * jy[u];
+                        }
                     }
                     if (sub == 0) atomicAdd(rot_count, 1);
                 }
@@ -1778,18 +1799,18 @@ __device__ void svd_sort_outputs(const double *W, const double *Jv, int m,
         a = wave_sum(a);
         if (lane == 0) s[c] = sqrt(a);
     }
+    for (int c = threadIdx.x; c < m; c += blockDim.x) perm[c] = c;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        for (int c = 0; c < m; c++) perm[c] = c;
-        for (int a = 1; a < m; a++) {
-            const int v = perm[a];
-            int b = a - 1;
-            while (b >= 0 && (s[perm[b]] < s[v])) {
-                perm[b + 1] = perm[b];
-                b--;
-            }
-            perm[b + 1] = v;
+    // Stable descending order: column c goes to position (# larger norms) +
+    // (# equal norms at lower indices).
+    for (int c = threadIdx.x; c < m; c += blockDim.x) {
+        const double v = s[c];
+        int r = 0;
+        for (int j = 0; j < m; j++) {
+            const double w = s[j];
+            r += (w > v) || (w == v && j < c);
         }
+        perm[r] = c;
     }
     __syncthreads();
     for (int idx = threadIdx.x; idx < m * m; idx += blockDim.x) {
@@ -1835,17 +1856,25 @@ __global__ __launch_bounds__(1024) void k_svd(Tables T) {
     // Right vectors go after the Jacobi work area of Jv in global memory.
     double *Vb = pr.Jv + (int64_t)mm * mm;
     svd_sort_outputs(W, Jv, m, mm, s, perm, pr.Sb, pr.Ub, Vb);
+    if (threadIdx.x == 0) sh_int[2] = 0;
+    __syncthreads();
+    {
+        // Residual test of the k leading triplets, one per thread.
+        const double rnorm = pr.rnorm;
+        const double smax = pr.Sb[0] > pr.smax ? pr.Sb[0] : pr.smax;
+        int mine = 0;
+        for (int i = threadIdx.x; i < k; i += blockDim.x) {
+            const double res = rnorm * pr.Ub[(m - 1) + (int64_t)i * m];
+            mine += fabs(res) < (double)pr.tol * smax;
+        }
+        if (mine) atomicAdd(&sh_int[2], mine);
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        const double rnorm = pr.rnorm;
         double smax = pr.smax;
         smax = pr.Sb[0] > smax ? pr.Sb[0] : smax;
         pr.smax = smax;
-        int nconv = 0;
-        for (int i = 0; i < k; i++) {
-            const double res = rnorm * pr.Ub[(m - 1) + (int64_t)i * m];
-            nconv += fabs(res) < (double)pr.tol * smax;
-        }
+        const int nconv = sh_int[2];
         pr.nconv = nconv;
         pr.svd_sweeps = sh_int[0];
         int kc = pr.kc;
@@ -2466,7 +2495,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     };
     // Kernel classes follow enum bpca_kernel_class in bpca.h.
     auto svd = [&](hipStream_t st) {
-        if ((int64_t)svd_lds_bytes(m, true) <= T.max_lds)
+        if ((int64_t)svd_lds_bytes(m, true) <= T.max_lds && m <= 128)
             k_svd<true><<<T.num_probs, 1024, svd_lds_bytes(m, true), st>>>(T);
         else
             k_svd<false><<<T.num_probs, 1024, svd_lds_bytes(m, false), st>>>(T);
@@ -2668,7 +2697,7 @@ hipError_t launch_svd_single(const double *B, int m, double *W, double *Jv,
                              int64_t max_lds, hipStream_t s) {
     const int mm = m + (m & 1);
     double *Vb = Jv + (int64_t)mm * mm;
-    if ((int64_t)svd_lds_bytes(m, true) <= max_lds)
+    if ((int64_t)svd_lds_bytes(m, true) <= max_lds && m <= 128)
         k_svd_single<true><<<1, 1024, svd_lds_bytes(m, true), s>>>(B, m, W, Jv, Ub, Sb, Vb, sweeps);
     else
         k_svd_single<false><<<1, 1024, svd_lds_bytes(m, false), s>>>(B, m, W, Jv, Ub, Sb, Vb, sweeps);
