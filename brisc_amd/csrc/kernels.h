@@ -278,7 +278,7 @@ int knn_kmax(int k);
 // Matrix-core path for num_dims <= kKnnMfmaMaxDims: Xp is scratch of
 // (n + kKnnPadRows) * knn_mfma_stride(d) floats (the padded copy of X);
 // norms (n floats) from launch_row_norms.
-constexpr int kKnnMfmaMaxDims = 128;
+constexpr int kKnnMfmaMaxDims = 256;
 constexpr int kKnnPadRows = 64;
 constexpr int64_t kKnnMaxRows = (1ll << 31) - 1 - kKnnPadRows;  // 32-bit row positions
 int knn_mfma_ksteps(int d);   // compiled k-steps covering ceil((d + 2) / 4): x, 1, ||x||^2

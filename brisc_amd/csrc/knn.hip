@@ -191,6 +191,13 @@ namespace {
 
 constexpr int kTile = 64;  // cells per tile; queries per workgroup
 constexpr int kBuf = 64;   // buffered candidates per query row
+constexpr int kKsWidest = 65;     // k-steps of 256 dimensions: ceil((256 + 2) / 4)
+constexpr int kKsRotBuf = 33;     // widest k-steps whose ranking keeps 64-slot rows
+// Row buffer slots of a launch: 64 for the ranking (k = 64 there: half the
+// merges) while its tiles leave room, 32 otherwise.
+__host__ __device__ constexpr int rot_buffers(bool rot, int ks) {
+    return rot && ks <= kKsRotBuf ? kBuf : 32;
+}
 typedef float knn_f4 __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ bool key_lt(float da, uint32_t ia, float db, uint32_t ib) {
@@ -329,8 +336,8 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
     // The queries' operands are held in registers: the queries pass through
     // candidate buffer 1 on the way (LDS: two tiles, ids, row buffers of 32
     // slots, 64 for the ranking, where k = 64).
-    static_assert(KS >= 1 && KS <= 33, "k-steps");
-    constexpr int kB = ROT ? kBuf : 32;
+    static_assert(KS >= 1 && KS <= kKsWidest, "k-steps");
+    constexpr int kB = rot_buffers(ROT, KS);
     const int st = T.st;
     const int chunks = st >> 2;                   // 64 rows x st floats
     float *const cs0 = lds_f;                     // 2 x (64 x st) candidates
@@ -821,7 +828,7 @@ __global__ __launch_bounds__(256) void k_knn_mfma(const KnnTiles T) {
 // The tile kernels are compiled for these k-step counts; a width runs with
 // the smallest that covers its ceil((d + 2) / 4) columns of four (the extra
 // columns are zero in every padded row, so they add nothing).
-static constexpr int kKsBuckets[] = {2, 3, 4, 5, 6, 8, 10, 13, 16, 20, 26, 33};
+static constexpr int kKsBuckets[] = {2, 3, 4, 5, 6, 8, 10, 13, 16, 20, 26, 33, 40, 48, 56, 65};
 
 int knn_mfma_ksteps(int d) {
     const int need = (d + 2 + 3) / 4;
@@ -837,7 +844,7 @@ int knn_mfma_stride(int d) { return 4 * knn_mfma_ksteps(d) + 4; }
 // the ranking.
 static size_t tile_lds_bytes(int st, bool buffers, bool rot) {
     const size_t tiles = 2 * (size_t)kTile * st + 2 * kTile;
-    const size_t bufs = buffers ? 2 * (size_t)kTile * (rot ? kBuf : 32) : 0;
+    const size_t bufs = buffers ? 2 * (size_t)kTile * rot_buffers(rot, (st - 4) / 4) : 0;
     return sizeof(float) * (tiles + bufs);
 }
 
@@ -882,6 +889,10 @@ hipError_t launch_knn_tiles(const KnnTiles &T, bool ivf, hipStream_t s) {
         case 20: launch_ks<20>(T, ivf, grid, s); break;
         case 26: launch_ks<26>(T, ivf, grid, s); break;
         case 33: launch_ks<33>(T, ivf, grid, s); break;
+        case 40: launch_ks<40>(T, ivf, grid, s); break;
+        case 48: launch_ks<48>(T, ivf, grid, s); break;
+        case 56: launch_ks<56>(T, ivf, grid, s); break;
+        case 65: launch_ks<65>(T, ivf, grid, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -60,7 +60,7 @@ def knn_exact(X, num_neighbors: int, context=None):
     return neighbors, distances
 
 
-IVF_MAX_DIMS = 128
+IVF_MAX_DIMS = 256
 
 
 def default_num_clusters(num_cells: int) -> int:
@@ -94,7 +94,7 @@ def knn_ivf(X, num_neighbors: int, num_clusters: int | None = None,
     ``num_clusters_searched`` nearest clusters.
 
     ``X``: float32 device tensor, ``num_cells x num_dims`` (``num_dims <=
-    128``).  Defaults as the reference's: ``num_clusters`` =
+    256``).  Defaults as the reference's: ``num_clusters`` =
     ``ceil(min(4 sqrt(n), n / 100))``, ``num_clusters_searched`` =
     ``min(64, num_clusters)``, ``chunk_size_kmeans`` = ``min(4096, n)``.
     Returns ``(neighbors int32, distances float32, info)`` with ``info`` a
@@ -168,7 +168,7 @@ def kmeans_barbar_init(X, num_clusters: int, num_init_iterations: int = 5,
                        context=None):
     """The cells k-means|| initialisation takes as centroids
     (``brisc/kmeans.pyx:41-360``) from a float32 device tensor ``X``
-    (``num_dims <= 128``), through ``bpca_kmeans_barbar_init``.  Returns
+    (``num_dims <= 256``), through ``bpca_kmeans_barbar_init``.  Returns
     ``(cells uint32 (num_clusters,), candidates selected)``."""
     import ctypes
 
@@ -200,7 +200,7 @@ def kmeans(X, num_clusters: int, num_kmeans_iterations: int = 2,
            num_init_iterations: int = 5, oversampling_factor: float = 1.0):
     """``kmeans()`` (``brisc/kmeans.pyx:889-1100``; random initial cells, or
     k-means|| with ``kmeans_barbar=True``) on a float32 device tensor ``X``
-    (``num_dims <= 128``), through ``bpca_kmeans_ex``.  Returns ``(labels
+    (``num_dims <= 256``), through ``bpca_kmeans_ex``.  Returns ``(labels
     int32 (n,), centroids float32
     (num_clusters, d), counts int32 (num_clusters,), iterations run)`` --
     the labels and counts of a final E-step against the final centroids,

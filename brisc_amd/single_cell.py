@@ -459,8 +459,7 @@ class SingleCell:
         returns what searching every cluster returns.  More than 64
         neighbours are found in passes of 64 (each keeping only candidates
         after the previous pass's last distance and index).  Limits on the
-        GPU: the PCs have at most 128 columns (256 with every cluster
-        searched and at most 64 neighbours).
+        GPU: the PCs have at most 256 columns.
         """
         import torch
 

@@ -421,8 +421,8 @@ int bpca_csr_transpose(bpca_context *ctx, const float *d_data,
  *   d_distances  float32 num_cells x k: squared Euclidean distances formed
  *                as ||q||^2 + (||c||^2 - 2 q.c) in FP32, as the reference
  * A cell is not its own neighbour; each row is in ascending distance, ties
- * in ascending index.  1 <= k <= num_cells - 1 (k <= 64 when num_dims >
- * 128); more than 64 neighbours run in passes of 64, each pass keeping only
+ * in ascending index.  1 <= k <= num_cells - 1; more than 64 neighbours
+ * run in passes of 64, each pass keeping only
  * candidates after the previous pass's last (distance, index).  Synchronous. */
 int bpca_knn_exact(bpca_context *ctx, const float *d_X, int64_t num_cells,
                    int32_t num_dims, int32_t k, uint32_t *d_neighbors,
@@ -431,7 +431,7 @@ int bpca_knn_exact(bpca_context *ctx, const float *d_X, int64_t num_cells,
 /* SingleCell.neighbors() with its default inverted-file search
  * (brisc/single_cell.py:12139-12665: kmeans() of brisc/kmeans.pyx:889-1100
  * with random initialisation, then knn_self of brisc/knn.pyx:13-516).
- *   d_X            float32 num_cells x num_dims PCs (num_dims <= 128)
+ *   d_X            float32 num_cells x num_dims PCs (num_dims <= 256)
  *   num_clusters   k-means clusters, 1 <= num_clusters <= min(num_cells, 2^26)
  *   num_clusters_searched  nearest clusters searched per cell (raised, as
  *                  the reference does, until k + 1 candidates are certain)
@@ -492,7 +492,7 @@ int bpca_kmeans_ex(bpca_context *ctx, const float *d_X, int64_t num_cells,
 /* The cells k-means|| initialisation takes as centroids
  * (kmeans_barbar_init, brisc/kmeans.pyx:41-360; kmeans_barbar_init_parallel,
  * 362-720, takes the same), from d_X (float32 num_cells x num_dims,
- * num_dims <= 128): rounds of draws from srand(seed + cell * round), each
+ * num_dims <= 256): rounds of draws from srand(seed + cell * round), each
  * cell taken with probability oversampling_factor * num_clusters *
  * distance / cost, until num_init_iterations rounds and at least
  * num_clusters candidates (or every cell); then weighted k-means++ over the

@@ -278,10 +278,12 @@ def test_ivf_matches_reference_with_few_clusters_searched(name):
 
 @pytest.mark.parametrize('n,d,k,centers', [
     (20000, 128, 64, 40), (20000, 7, 5, 30), (5000, 1, 3, 10),
-    (30001, 50, 20, 200), (20000, 50, 64, 40), (20000, 48, 33, 40)])
+    (30001, 50, 20, 200), (20000, 50, 64, 40), (20000, 48, 33, 40),
+    (12000, 150, 30, 30), (20000, 200, 20, 40), (20000, 256, 64, 40)])
 def test_ivf_shapes_against_exact_search(n, d, k, centers):
-    # Widths at the edges of the tile engine (1 and 128 dimensions, the
-    # 64-entry lists), 13-step tiles (48 and 50 dimensions) with more
+    # Widths at the edges of the tile engine (1, 128 and 256 dimensions, the
+    # 64-entry lists; past 132 dimensions the ranking keeps 32-slot rows),
+    # 13-step tiles (48 and 50 dimensions) with more
     # neighbours than the 32-slot row buffers hold, and a ragged
     # cell count, in the inverted-file mode:
     # structure exact, recall against the exhaustive search high, distances
@@ -387,3 +389,20 @@ def test_ivf_fewer_than_64_clusters_searches_each_cells_nearest_clusters(d):
         assert np.abs(dist[q] - got).max() <= 2e-5 * (norms[q] + norms.max()), q
     print(f'd={d}: {info}; rows checked {checked} of {rows.size}')
     assert checked >= 0.9 * rows.size
+
+
+@pytest.mark.parametrize('d', [150, 256])
+def test_kmeans_wide_pcs_against_fp64_oracle(d):
+    # k-means on PCs wider than 128 columns (the wide tile widths' argmin
+    # E-steps): labels of a final E-step against the FP64 restatement from
+    # the same initial cells, at least 99.9% equal.
+    import oracle
+    from brisc_amd.knn import kmeans, kmeans_init_cells
+    n, C = 8000, 60
+    X = helpers.knn_points(77 + d, n, d, 30, spread=0.7)
+    labels, cent, counts, it = kmeans(_dev(X), C, 2)
+    cells = kmeans_init_cells(n, C, 0).astype(np.int64)
+    l2, c2, n2 = oracle.kmeans_fp64(X, cells, 2)
+    same = float((labels.cpu().numpy() == l2).mean())
+    print(f'd={d}: k-means labels equal to the FP64 oracle {same:.5f}')
+    assert it == 2 and same >= 0.999

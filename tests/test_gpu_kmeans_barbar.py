@@ -110,7 +110,8 @@ def test_duplicate_points_and_errors_as_reference():
 
 
 @pytest.mark.parametrize('seed,n,d,centres', [(51, 20000, 30, 40),
-                                              (52, 6000, 50, 25)])
+                                              (52, 6000, 50, 25),
+                                              (54, 5000, 200, 20)])
 def test_real_valued_init_matches_oracle(seed, n, d, centres):
     from brisc_amd.knn import default_num_clusters, kmeans_barbar_init
     X = helpers.knn_points(seed, n, d, centres)

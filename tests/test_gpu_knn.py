@@ -84,7 +84,7 @@ def check_knn(X, nb, dist, k):
     (2, 1, 1), (65, 3, 1), (257, 50, 20), (1000, 50, 20), (3000, 50, 64),
     (4099, 256, 16), (5000, 10, 32), (700, 7, 33), (1000, 60, 5),
     (1500, 61, 64), (2000, 128, 20), (999, 129, 7), (20000, 50, 20),
-    (3000, 47, 20), (2500, 49, 64)])
+    (3000, 47, 20), (2500, 49, 64), (3000, 200, 20), (1200, 161, 64)])
 def test_exact_knn_matches_fp64_oracle(n, d, k):
     X = _clustered(n * 7 + d, n, d)
     nb, dist = _knn(X, k)
@@ -95,7 +95,8 @@ def test_exact_knn_matches_fp64_oracle(n, d, k):
           f'same neighbour set {same_set:.4f}')
 
 
-@pytest.mark.parametrize('n,d,k', [(3000, 30, 100), (2500, 50, 150), (1000, 7, 200), (700, 3, 65)])
+@pytest.mark.parametrize('n,d,k', [(3000, 30, 100), (2500, 50, 150), (1000, 7, 200), (700, 3, 65),
+                                   (2000, 160, 100), (1500, 256, 65)])
 def test_exact_knn_more_than_64_neighbours(n, d, k):
     # Passes of 64: the first 64 columns are the 64-neighbour search's,
     # bit for bit; the whole row against the FP64 oracle as above (with

@@ -28,13 +28,13 @@ def _sc(n=50, d=8, **obsm):
     (dict(num_kmeans_iterations=0), ValueError, 'num_kmeans_iterations'),
     (dict(chunk_size_search=51), ValueError, 'chunk_size_search is 51'),
     (dict(num_neighbors=2.0), TypeError, 'num_neighbors'),
-    (dict(num_neighbors=49), ValueError, 'at most 128 are supported'),
+    (dict(num_neighbors=49), ValueError, 'at most 256 are supported'),
 ])
 def test_neighbors_argument_errors(kwargs, error, fragment):
     sc = _sc(n=50)
     if kwargs.get('num_neighbors') == 49:
-        # more than 64 neighbours run on the matrix-core path: <= 128 PCs
-        sc = _sc(n=100, d=200)
+        # the matrix-core tiles take at most 256 PCs
+        sc = _sc(n=100, d=300)
         kwargs = dict(num_neighbors=65, num_clusters=1)
     with pytest.raises(error, match=fragment):
         sc.neighbors(**kwargs)
