@@ -63,7 +63,7 @@ class SolveStats(ctypes.Structure):
                 ('slot', ctypes.c_int32),
                 ('smax', ctypes.c_double),
                 ('reorth_passes', ctypes.c_int32),
-                ('reserved', ctypes.c_int32)]
+                ('pair_products', ctypes.c_int32)]
 
     def as_dict(self) -> dict:
         return {name: getattr(self, name) for name, _ in self._fields_}
@@ -156,6 +156,8 @@ def lib() -> ctypes.CDLL:
         'bpca_context_set_sorted_xtu': (i32, [p, i32]),
         'bpca_context_get_sorted_xtu': (i32, [p, ctypes.POINTER(i32)]),
         'bpca_context_set_reorth_passes': (i32, [p, i32]),
+        'bpca_context_set_pair_products': (i32, [p, i32]),
+        'bpca_context_get_pair_products': (i32, [p, ctypes.POINTER(i32)]),
         'bpca_context_get_reorth_passes': (i32, [p, ctypes.POINTER(i32)]),
         'bpca_context_get_occupancy': (i32, [p, ctypes.POINTER(ctypes.c_int32),
                                              ctypes.POINTER(ctypes.c_int32)]),

@@ -71,6 +71,8 @@ constexpr int kMaxSortedGenes = 4096;  // X^T u from the gene-sorted copy up to 
 constexpr int kTChunk = 16;         // entries per lane in the gene-sorted X^T u
 constexpr int kTWindow = kWave * kTChunk;  // entries per wave step of that kernel
 constexpr int kSortedBit = 1 << 3;  // Tables::tmodes: some problem uses the gene-sorted X^T u
+constexpr int kPairBit = 1 << 4;    // Tables::tmodes: some problem forms X v and X^T u in one pass
+constexpr int kMaxPairGenes = 4096; // X v and X^T u in one pass up to this g (LDS)
 constexpr int kReorthTwoBit = 1;    // Tables::rmodes: some problem reorthogonalises in two passes
 constexpr int kReorthOneBit = 2;    //                 some problem in one pass
 constexpr int64_t kOnePassMinCells = 65536;  // automatic one-pass choice: n >= max(8 g, this)
@@ -139,6 +141,13 @@ struct Prob {
     int32_t onepass;
     int32_t jfirst;          // first step of the current sweep (0, then kc)
     double *R[2];            // float64 g x (m+1) (ld g): X^T U, flipped with U
+    // X v and X^T u from one read of X (pair != 0: one-pass problems with
+    // 16-bit indices and g <= kMaxPairGenes).  X^T is applied to
+    // u - vmean, vmean = colsum . vs / n being the mean of X vs known before
+    // the pass; k_vside_w moves the result to X^T (u - mean of u).
+    int32_t pair, pad1;
+    double *colsum;          // float64 g: column sums of X
+    double vmean;            // colsum . vs / n for the current vs
     double alpha, inv_alpha, rnorm, smax;
     double umean;         // two passes: mean left in U[:, j]; one pass: mean of u
     int64_t steps;
@@ -160,7 +169,8 @@ struct Tables {
     int64_t num_groups;
     int max_g, max_m;
     int modes;            // bit set of the IndexModes of the problems
-    int tmodes;           // X^T u: IndexModes of the CSR-path problems, kSortedBit
+    int vmodes;           // X v: IndexModes of the problems without the one-pass pair
+    int tmodes;           // X^T u: IndexModes of the CSR-path problems, kSortedBit, kPairBit
     int rmodes;           // reorthogonalisation: kReorthTwoBit, kReorthOneBit
     int64_t max_lds;      // dynamic LDS bytes a workgroup may use
     int spmv_group;       // segments per load group in X v   (2, 4 or 8)

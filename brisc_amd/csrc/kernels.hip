@@ -81,6 +81,15 @@ __device__ inline void sub_block(int64_t n, int64_t lb, int w, int64_t &first,
     count = count < 0 ? 0 : (count > kSubRows ? kSubRows : count);
 }
 
+// colsum . vs / n, the mean of X vs, for the problem's current vs: thread t
+// takes genes t, t + 256, ... in order, then block_sum<4>.  Every thread of
+// a 256-thread block calls it after vs is written.
+__device__ inline double vs_mean(const Prob &pr, double *tmp) {
+    double a = 0.0;
+    for (int i = threadIdx.x; i < pr.g; i += 256) a = fma(pr.colsum[i], (double)pr.vs[i], a);
+    return block_sum<4>(a, tmp) / (double)pr.n;
+}
+
 }  // namespace
 
 // ================================================================ set-up
@@ -156,6 +165,7 @@ __global__ __launch_bounds__(256) void k_reduce_groups(Tables T, int setup) {
 // sd = max(sqrt((sum x^2 - (sum x)^2/n)/(n-1)), 1e-8) in FP64; vs = V0/sd;
 // B = 0; state reset; phase -> Lanczos.
 __global__ __launch_bounds__(256) void k_setup_finish(Tables T) {
+    __shared__ double tmp[4];
     Prob &pr = T.probs[blockIdx.x];
     if (pr.phase != PH_SETUP) return;
     const int g = pr.g, m = pr.m;
@@ -171,11 +181,14 @@ __global__ __launch_bounds__(256) void k_setup_finish(Tables T) {
         double sd = sqrt(var > 0.0 ? var : 0.0);
         sd = sd < 1e-8 ? 1e-8 : sd;
         pr.sd[i] = sd;
+        pr.colsum[i] = s;
         pr.vs[i] = (float)(V0[i] / sd);
     }
     for (int i = threadIdx.x; i < m * (m + 1); i += 256) pr.B[i] = 0.0;
     __syncthreads();
+    const double vm = vs_mean(pr, tmp);
     if (threadIdx.x == 0) {
+        pr.vmean = vm;
         pr.j = 0;
         pr.jfirst = 0;
         pr.kc = 0;
@@ -335,11 +348,65 @@ __device__ __forceinline__ void form_group(const int *off, int nrows,
 }
 
 // Partial sums continue across a row's segments (`carry` holds the open
-// row's); the rows that end in the group are reduced together (below).  The
-// rows of a group's slots are consecutive, so slot q's row is the first
+// row's); the rows that end in the group are reduced together
+// (store_row_totals).  Branch-free: every lane gathers all four v entries
+// (a quad's indices are all below g, padding included) and elements outside
+// the segment enter the chain as 0 * v.
+template <int R, typename IX, bool GV>
+__device__ __forceinline__ void spmv_reduce(const SegGroup<R, IX> &G,
+                                            const float *lds_v, gfloat_r *gv,
+                                            int lane, float &carry,
+                                            gfloat_w *u, gfloat_w *sink);
+
+// The rows of a group's slots are consecutive, so slot q's row is the first
 // slot's row plus the finished slots before q.  Every lane stores -- lanes
 // without a finished row into the sink slot -- so the store is not behind a
 // branch and the waits for loads stay fixed counts.
+//
+// Halving butterfly: at offsets 32, 16, ... (log2 R steps) each lane keeps
+// half of its rows and adds its partner's share of them, so after those
+// steps lane l holds a partial of row l / (64 / R); the remaining offsets
+// finish each row within its 64 / R lanes.  Every row is summed in the
+// xor-butterfly tree 32, 16, 8, 4, 2, 1 (floating addition is commutative,
+// so the bits are those of a plain per-row butterfly) with R - 1 + 6 -
+// log2 R exchanges instead of 6 R.  kDpp: exchanges at offsets up to 16 by
+// DPP moves and a swizzle (xor_lane) instead of LDS permutes, the same pairs.
+//
+// kPair: each finished row's total also goes to the wave's LDS table uv as
+// (float)(total - vmean), the centred value X^T u takes (uv[kSubRows] is the
+// sink slot of that store).
+template <int R, typename IX, bool kPair, bool kDpp>
+__device__ __forceinline__ void store_row_totals(float (&done)[R], const SegGroup<R, IX> &G,
+                                                 int lane, gfloat_w *u, gfloat_w *sink,
+                                                 float *uv, double vmean) {
+    auto xchg = [&](float v, int off) -> float {
+        if (kDpp && off <= 16) return __int_as_float(xor_lane(__float_as_int(v), off));
+        return __shfl_xor(v, off);
+    };
+    constexpr int kLanesPerRow = kWave / R;
+    int off2 = 32;
+#pragma unroll
+    for (int half = R / 2; half >= 1; half >>= 1, off2 >>= 1) {
+        const bool upper = (lane & off2) != 0;
+#pragma unroll
+        for (int i = 0; i < half; i++) {
+            const float keep = upper ? done[i + half] : done[i];
+            const float send = upper ? done[i] : done[i + half];
+            done[i] = keep + xchg(send, off2);
+        }
+    }
+#pragma unroll
+    for (; off2 > 0; off2 >>= 1) done[0] += xchg(done[0], off2);
+    unsigned finmask = 0;
+#pragma unroll
+    for (int q = 0; q < R; q++) finmask |= (unsigned)G.fin[q] << q;
+    const int q = lane / kLanesPerRow;
+    const bool wr = (lane % kLanesPerRow) == 0 && ((finmask >> q) & 1u);
+    const int row = G.row[0] + __popc(finmask & ((1u << q) - 1u));
+    *(wr ? u + row : sink) = done[0];
+    if (kPair) uv[wr ? row : kSubRows] = (float)((double)done[0] - vmean);
+}
+
 template <int R, typename IX, bool GV>
 __device__ __forceinline__ void spmv_reduce(const SegGroup<R, IX> &G,
                                             const float *lds_v, gfloat_r *gv,
@@ -349,9 +416,6 @@ __device__ __forceinline__ void spmv_reduce(const SegGroup<R, IX> &G,
     float acc = carry;
 #pragma unroll
     for (int q = 0; q < R; q++) {
-        // Branch-free: every lane gathers all four v entries (a quad's
-        // indices are all below g, padding included) and elements outside
-        // the segment enter the chain as 0 * v.
         const int a0 = 4 * (G.q0[q] + lane);
         float vv[4];
 #pragma unroll
@@ -365,34 +429,7 @@ __device__ __forceinline__ void spmv_reduce(const SegGroup<R, IX> &G,
         acc = G.fin[q] ? 0.0f : acc;
     }
     carry = acc;
-    // Halving butterfly: at offsets 32, 16, ... (log2 R steps) each lane
-    // keeps half of its rows and adds its partner's share of them, so after
-    // those steps lane l holds a partial of row l / (64 / R); the remaining
-    // offsets finish each row within its 64 / R lanes.  Every row is summed
-    // in the xor-butterfly tree 32, 16, 8, 4, 2, 1 (floating addition is
-    // commutative, so the bits are those of a plain per-row butterfly) with
-    // R - 1 + 6 - log2 R exchanges instead of 6 R.
-    constexpr int kLanesPerRow = kWave / R;
-    int off2 = 32;
-#pragma unroll
-    for (int half = R / 2; half >= 1; half >>= 1, off2 >>= 1) {
-        const bool upper = (lane & off2) != 0;
-#pragma unroll
-        for (int i = 0; i < half; i++) {
-            const float keep = upper ? done[i + half] : done[i];
-            const float send = upper ? done[i] : done[i + half];
-            done[i] = keep + __shfl_xor(send, off2);
-        }
-    }
-#pragma unroll
-    for (; off2 > 0; off2 >>= 1) done[0] += __shfl_xor(done[0], off2);
-    unsigned finmask = 0;
-#pragma unroll
-    for (int q = 0; q < R; q++) finmask |= (unsigned)G.fin[q] << q;
-    const int q = lane / kLanesPerRow;
-    const bool wr = (lane % kLanesPerRow) == 0 && ((finmask >> q) & 1u);
-    const int row = G.row[0] + __popc(finmask & ((1u << q) - 1u));
-    *(wr ? u + row : sink) = done[0];
+    store_row_totals<R, IX, false, false>(done, G, lane, u, sink, nullptr, 0.0);
 }
 
 }  // namespace
@@ -427,7 +464,7 @@ __global__ __launch_bounds__(256) void k_spmv(Tables T) {
     extern __shared__ float lds_v[];
     const int2 bp = T.blocks[blockIdx.x];
     Prob &pr = T.probs[bp.x];
-    if (pr.phase != PH_LANCZOS || pr.mode != MODE) return;
+    if (pr.phase != PH_LANCZOS || pr.mode != MODE || pr.pair) return;
     const int g = pr.g;
     if (!GV)
         for (int i = threadIdx.x; i < g; i += 256) lds_v[i] = pr.vs[i];
@@ -915,7 +952,7 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T, int c0) {
     extern __shared__ float lds[];
     const int2 bp = T.blocks[blockIdx.x];
     Prob &pr = T.probs[bp.x];
-    if (pr.phase != PH_LANCZOS || pr.tsorted || (pr.mode == kModeWide) != (sizeof(IX) == 4)) return;
+    if (pr.phase != PH_LANCZOS || pr.tsorted || pr.pair || (pr.mode == kModeWide) != (sizeof(IX) == 4)) return;
     const int g = pr.g;
     if (c0 >= g) return;
     const int tw = g - c0 < kGeneTile ? g - c0 : kGeneTile;
@@ -974,6 +1011,282 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T, int c0) {
     float *out = pr.tpart + (int64_t)bp.y * 2 * g + c0;
     for (int i = threadIdx.x; i < tw; i += 256)
         out[i] = (lds[i] + lds[gs + i]) + (lds[2 * gs + i] + lds[3 * gs + i]);
+}
+
+// ====================================== X v and X^T u from one read of X
+//
+// One-pass problems with 16-bit indices and g <= kMaxPairGenes (Prob::pair)
+// form both sparse products of a Lanczos step in one pass over X.  The
+// X^T u of the one-pass scheme is applied to the raw step vector less its
+// mean, X^T (u - mean) (k_vside_w turns it into X^T U[:, j] in gene space),
+// so nothing in it waits for the reorthogonalisation: the mean of u = X vs
+// is colsum . vs / n before the pass (vmean), and X^T (u - vmean) differs
+// from X^T (u - mean) by (mean - vmean) colsum, which k_vside_w adds in
+// FP64.  Per row the kernel does what k_spmv and k_spmv_t do, on the same
+// loaded quads: the row total u (k_spmv's order of summation) goes to u32
+// and, centred, to a 256-entry LDS table; then every element of the row adds
+// x * (u - vmean) to the wave's FP32 accumulator of its gene, rows in order
+// (k_spmv_t's order of summation for every gene).
+//
+// A row's X^T part needs its total, known once its last segment is reduced.
+// The X^T part of a group runs right after its X v part, so it has every row
+// that ends in the group; a row left open at the group's end has, if it has
+// at most two segments (up to 253 elements at least), one segment here and
+// its last one in the first slot of the next group: that segment is kept in
+// registers (`Deferred`) and applied first in the next group.  Rows of three
+// or more segments are skipped by the streamed path and, when their last
+// segment is reached, re-read whole from memory (the loads hit the cache
+// that just delivered them) and applied in place, so every gene still sees
+// its rows in order.
+
+namespace {
+
+// Segment groups of the pair kernel: form_group's stream, with two changes.
+// (1) A row of two segments is never started in a group's last slot: that
+// slot is left empty (no elements; its loads re-read the row's first quad)
+// and the row starts the next group, so every row of at most two segments
+// lies within one group and its total is known when the group's X^T part
+// runs.  (2) Per slot, whether its row has three or more segments (`lng`)
+// and the row's bounds, for the re-read of such rows.
+template <int R>
+struct PairGroup {
+    SegGroup<R, uint16_t> s;
+    bool lng[R];
+    int rs[R], re[R];
+};
+
+struct PairCursor {
+    Cursor c;
+    int start;   // first position of row c.r
+};
+
+template <int R>
+__device__ __forceinline__ void form_group_pair(const int *off, int nrows, gquad *data,
+                                                IxQuad<uint16_t>::gvec *indices, int lane,
+                                                PairCursor &pc, PairGroup<R> &P) {
+    Cursor &cs = pc.c;
+    SegGroup<R, uint16_t> &G = P.s;
+    int W[R + 1];  // ends of rows cs.r + 1 ... cs.r + R + 1
+#pragma unroll
+    for (int i = 0; i <= R; i++) {
+        const int r = cs.r + 2 + i;
+        W[i] = uniform(off[r < nrows ? r : nrows]);
+    }
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        const int q0 = cs.p >> 2;
+        const int wend = 4 * (q0 + kWave);
+        const bool lng = cs.e > pc.start && ((cs.e - 1) >> 2) - (pc.start >> 2) >= 2 * kWave;
+        const bool hold = q == R - 1 && cs.p == pc.start && wend < cs.e && !lng && cs.r < nrows;
+        const int se = hold ? cs.p : (wend < cs.e ? wend : cs.e);
+        const bool fin = !hold && wend >= cs.e;
+        const int lastq = hold ? q0 : (se > 0 ? se - 1 : 0) >> 2;
+        G.b[q] = cs.p;
+        G.e[q] = se;
+        G.q0[q] = q0;
+        G.row[q] = cs.r;
+        G.fin[q] = fin && cs.r < nrows;
+        P.lng[q] = lng;
+        P.rs[q] = pc.start;
+        P.re[q] = cs.e;
+        const int qi = q0 + lane < lastq ? q0 + lane : lastq;
+        G.c[q].d = __builtin_nontemporal_load(data + qi);
+        G.c[q].ix = __builtin_nontemporal_load(indices + qi);
+        if (fin) {
+            cs.r = cs.r < nrows ? cs.r + 1 : nrows;
+            cs.p = cs.e;
+            pc.start = cs.e;
+            cs.e = W[0];
+#pragma unroll
+            for (int i = 0; i < R; i++) W[i] = W[i + 1];
+        } else if (!hold) {
+            cs.p = wend;
+        }
+    }
+}
+
+// The gene of each element of a group's quads, or g for elements outside
+// their segment: lds_v[g] is 0, so such an element enters X v's chain as
+// x * 0, and X^T u sends it to the accumulator's spare slot acc[g].  One
+// unsigned compare per element: position a0 + k lies in [b, e) iff
+// a0 - b + k < e - b.
+template <int R>
+__device__ __forceinline__ void pair_genes(const SegGroup<R, uint16_t> &G, int lane, int g,
+                                           int (&ci)[R][4]) {
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        const unsigned t = (unsigned)(4 * (G.q0[q] + lane) - G.b[q]);
+        const unsigned len = (unsigned)(G.e[q] - G.b[q]);
+#pragma unroll
+        for (int k = 0; k < 4; k++) ci[q][k] = t + k < len ? quad_index(G.c[q], k) : g;
+    }
+}
+
+// X v part of a group: k_spmv's chain (the same products in the same
+// order; an element outside its segment adds x * 0 where k_spmv adds 0 * v),
+// the row totals to u; returns, per lane, the centred value
+// (float)(total - vmean) of the row whose total the lane holds.
+template <int R>
+__device__ __forceinline__ float pair_xv(const SegGroup<R, uint16_t> &G, const int (&ci)[R][4],
+                                         const float *lds_v, int lane, float &carry,
+                                         gfloat_w *u, gfloat_w *sink, double vmean) {
+    float done[R];
+    float acc = carry;
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        float vv[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) vv[k] = lds_v[ci[q][k]];
+#pragma unroll
+        for (int k = 0; k < 4; k++) acc = fmaf(quad_value(G.c[q], k), vv[k], acc);
+        done[q] = acc;
+        acc = G.fin[q] ? 0.0f : acc;
+    }
+    carry = acc;
+    store_row_totals<R, uint16_t, false, true>(done, G, lane, u, sink, nullptr, 0.0);
+    return (float)((double)done[0] - vmean);
+}
+
+// acc[gene] += x * ur for the four elements of one lane's quad (genes from
+// pair_genes; a row's genes are distinct, so one update's live lanes touch
+// distinct genes).
+__device__ __forceinline__ void pair_xt_quad(const f32x4 &d, const int (&ci)[4], float ur,
+                                             float *acc) {
+    float x[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) x[k] = acc[ci[k]];
+    acc[ci[0]] = fmaf(d.x, ur, x[0]);
+    acc[ci[1]] = fmaf(d.y, ur, x[1]);
+    acc[ci[2]] = fmaf(d.z, ur, x[2]);
+    acc[ci[3]] = fmaf(d.w, ur, x[3]);
+}
+
+// The X^T part of a row of three or more segments, elements [rs, re)
+// (positions relative to the sub-block's first quad), re-read window by
+// window.
+__device__ __noinline__ void pair_xt_long_row(gquad *data, IxQuad<uint16_t>::gvec *indices,
+                                              int rs, int re, float ur, int lane,
+                                              int g, float *acc) {
+    const int lastq = (re - 1) >> 2;
+    for (int q0 = rs >> 2; q0 <= lastq; q0 += kWave) {
+        const int qi = q0 + lane < lastq ? q0 + lane : lastq;
+        Quad<uint16_t> c;
+        c.d = data[qi];
+        c.ix = indices[qi];
+        const unsigned t = (unsigned)(4 * (q0 + lane) - rs);
+        const unsigned len = (unsigned)(re - rs);
+        int ci[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) ci[k] = t + k < len ? quad_index(c, k) : g;
+        pair_xt_quad(c.d, ci, ur, acc);
+    }
+}
+
+// The X^T part of group P, after its X v part: the row ending at slot f
+// has its centred total in lane f kWave / R of uc (store_row_totals' lane
+// for slot f); a slot that does not finish its row belongs to a row ending
+// in the next slot (two segments), to a row of three or more segments
+// (applied whole at its last slot), or is an empty last slot.
+template <int R>
+__device__ __forceinline__ void pair_xt(const PairGroup<R> &P, const int (&ci)[R][4], float uc,
+                                        gquad *data, IxQuad<uint16_t>::gvec *indices,
+                                        int lane, int g, float *acc) {
+    const SegGroup<R, uint16_t> &G = P.s;
+    constexpr int kLanesPerRow = kWave / R;
+    const int ucb = __float_as_int(uc);
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        const int f = G.fin[q] || q == R - 1 ? q : q + 1;   // the slot ending the row
+        const float ur = __int_as_float(__builtin_amdgcn_readlane(ucb, f * kLanesPerRow));
+        if (P.lng[q]) {
+            if (G.fin[q]) pair_xt_long_row(data, indices, P.rs[q], P.re[q], ur, lane, g, acc);
+        } else {
+            pair_xt_quad(G.c[q].d, ci[q], ur, acc);
+        }
+    }
+}
+
+}  // namespace
+
+// LDS of k_spmv_pair: vs (g floats and a zero), 4 wave accumulators of
+// g + 1 floats (a spare slot each), 4 slices of kSubRows + 1 offsets.
+size_t spmv_pair_lds_bytes(int g) {
+    return sizeof(float) * ((size_t)g + 1 + 4 * ((size_t)g + 1)) +
+           sizeof(int) * 4 * (size_t)(kSubRows + 1);
+}
+
+// One workgroup per data block, wave w on its 256-row sub-block; loads
+// non-temporal (X is read once per step).  The block's partials of
+// X^T (u - vmean) go to tpart as in k_spmv_t.
+template <int R>
+__global__ __launch_bounds__(256) void k_spmv_pair(Tables T) {
+    extern __shared__ float lds[];
+    const int2 bp = T.blocks[blockIdx.x];
+    Prob &pr = T.probs[bp.x];
+    if (pr.phase != PH_LANCZOS || !pr.pair) return;
+    const int g = pr.g, gs = g + 1;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_id();
+    float *lds_v = lds;
+    float *accs = lds + gs;
+    float *acc = accs + (size_t)w * gs;
+    int *off = (int *)(accs + 4 * (size_t)gs) + w * (kSubRows + 1);
+    for (int i = threadIdx.x; i < gs; i += 256) lds_v[i] = i < g ? pr.vs[i] : 0.0f;
+    for (int i = lane; i < gs; i += kWave) acc[i] = 0.0f;
+    int64_t r0, nrows64;
+    sub_block(pr.n, bp.y, w, r0, nrows64);
+    const int nrows = (int)nrows64;
+    const int64_t q0 = nrows > 0 ? stage_row_offsets(pr.indptr, r0, nrows, lane, off) : 0;
+    __syncthreads();
+    float *uj = u_of_step(pr, pr.j);
+    if (nrows > 0 && off[nrows] > off[0]) {
+        const double vmean = pr.vmean;
+        gquad *data = (gquad *)pr.data + q0;
+        IxQuad<uint16_t>::gvec *indices = (IxQuad<uint16_t>::gvec *)pr.idx + q0;
+        gfloat_w *u = (gfloat_w *)(uj + r0);
+        gfloat_w *sink = (gfloat_w *)(uj + pr.n);
+        PairCursor pc{{0, uniform(off[0]), uniform(off[1])}, uniform(off[0])};
+        PairGroup<R> ga, gb;
+        float carry = 0.0f;
+        int ci[R][4];
+        form_group_pair<R>(off, nrows, data, indices, lane, pc, ga);
+        for (;;) {
+            form_group_pair<R>(off, nrows, data, indices, lane, pc, gb);
+            __builtin_amdgcn_sched_barrier(0);
+            pair_genes<R>(ga.s, lane, g, ci);
+            float uc = pair_xv<R>(ga.s, ci, lds_v, lane, carry, u, sink, vmean);
+            pair_xt<R>(ga, ci, uc, data, indices, lane, g, acc);
+            const bool b_last = pc.c.r >= nrows;  // gb finished the last row
+            form_group_pair<R>(off, nrows, data, indices, lane, pc, ga);
+            __builtin_amdgcn_sched_barrier(0);
+            pair_genes<R>(gb.s, lane, g, ci);
+            uc = pair_xv<R>(gb.s, ci, lds_v, lane, carry, u, sink, vmean);
+            pair_xt<R>(gb, ci, uc, data, indices, lane, g, acc);
+            if (b_last) break;
+        }
+    } else {
+        for (int q = lane; q < nrows; q += kWave) uj[r0 + q] = 0.0f;
+    }
+    __syncthreads();
+    float *out = pr.tpart + (int64_t)bp.y * 2 * g;
+    for (int i = threadIdx.x; i < g; i += 256)
+        out[i] = (accs[i] + accs[gs + i]) + (accs[2 * gs + i] + accs[3 * gs + i]);
+}
+
+// Last step of a sweep, pair problems: U[:, m-1] = (u - mean) / alpha (what
+// k_spmv_t writes for other one-pass problems; the restart and final
+// coefficients add the rest of the column).
+__global__ __launch_bounds__(256) void k_pair_last_column(Tables T) {
+    const int2 bp = T.blocks[blockIdx.x];
+    Prob &pr = T.probs[bp.x];
+    if (pr.phase != PH_LANCZOS || !pr.pair || pr.j != pr.m - 1) return;
+    const int64_t r0 = (int64_t)bp.y * kRowBlock;
+    const int64_t nrows = pr.n - r0 < kRowBlock ? pr.n - r0 : kRowBlock;
+    const float *uj = u_of_step(pr, pr.j);
+    double *Uj = pr.U[pr.ubuf] + (int64_t)pr.j * pr.n;
+    const double umean = pr.umean, inv_alpha = pr.inv_alpha;
+    for (int64_t t = threadIdx.x; t < nrows; t += 256)
+        Uj[r0 + t] = ((double)uj[r0 + t] - umean) * inv_alpha;
 }
 
 // ================================================== X^T u, gene-sorted copy
@@ -1528,8 +1841,10 @@ __global__ __launch_bounds__(256) void k_vside_w(Tables T) {
     if (w != 0 || i >= g) return;
     double acc = (part[0][0][lane] + part[0][1][lane]) + (part[0][2][lane] + part[0][3][lane]);
     if (one) {
-        // acc = X^T (u_j - mean); X^T U[:, j] = (acc - X^T U[:, :j] h) / alpha,
-        // kept as column j of R.
+        // acc = X^T (u_j - mean) (pair problems: X^T (u_j - vmean), moved
+        // here by (vmean - mean) colsum); X^T U[:, j] = (acc - X^T U[:, :j] h)
+        // / alpha, kept as column j of R.
+        if (pr.pair) acc = fma(pr.vmean - pr.umean, pr.colsum[i], acc);
         const double c = (part[1][0][lane] + part[1][1][lane]) + (part[1][2][lane] + part[1][3][lane]);
         acc = (acc - c) * pr.inv_alpha;
         pr.R[pr.ubuf][(int64_t)j * g + i] = acc;
@@ -1581,6 +1896,7 @@ __global__ __launch_bounds__(kVChunk) void k_vside_upd(Tables T) {
 
 __global__ __launch_bounds__(256) void k_vside_fin(Tables T) {
     __shared__ double sbeta;
+    __shared__ double tmp[4];
     Prob &pr = T.probs[blockIdx.x];
     if (pr.phase != PH_LANCZOS) return;
     const int g = pr.g, m = pr.m, j = pr.j;
@@ -1602,7 +1918,9 @@ __global__ __launch_bounds__(256) void k_vside_fin(Tables T) {
             pr.vs[i] = (float)(x / pr.sd[i]);
         }
         __syncthreads();
+        const double vm = pr.pair ? vs_mean(pr, tmp) : 0.0;
         if (threadIdx.x == 0) {
+            pr.vmean = vm;
             pr.B[(int64_t)j + (int64_t)(j + 1) * m] = bclip;
             pr.j = j + 1;
             pr.steps += 1;
@@ -2353,9 +2671,13 @@ __global__ __launch_bounds__(256) void k_restart_v(Tables T) {
 }
 
 // Flip the U and V buffers and continue the Lanczos sweep at j = kc.
-__global__ void k_restart_commit(Tables T) {
+__global__ __launch_bounds__(256) void k_restart_commit(Tables T) {
+    __shared__ double tmp[4];
     Prob &pr = T.probs[blockIdx.x];
-    if (threadIdx.x != 0 || pr.phase != PH_RESTARTING) return;
+    if (pr.phase != PH_RESTARTING) return;
+    const double vm = pr.pair ? vs_mean(pr, tmp) : 0.0;  // vs from k_restart_v
+    if (threadIdx.x != 0) return;
+    pr.vmean = vm;
     pr.ubuf ^= 1;
     pr.vbuf ^= 1;
     pr.j = pr.kc;
@@ -2387,6 +2709,7 @@ static size_t capped_lds(size_t need, int wg, const Tables &T) {
 size_t lds_bytes_needed(int g, int m) {
     size_t need = spmv_t_lds_bytes(g);
     if (g <= kMaxSortedGenes) need = std::max(need, spmv_t_sorted_lds_bytes(g));
+    if (g <= kMaxPairGenes) need = std::max(need, spmv_pair_lds_bytes(g));
     need = std::max(need, spmv_lds_bytes(g));
     need = std::max(need, sizeof(float) * 4 * (size_t)std::min(g, kGeneTile));  // colstats
     need = std::max(need, sizeof(double) * (size_t)m * kGemmCols);   // gemm
@@ -2399,9 +2722,9 @@ namespace {
 template <int R>
 void launch_spmv_modes(const Tables &T, size_t lds_ldsv, size_t lds_global, hipStream_t s) {
     const unsigned nblk = (unsigned)T.num_blocks;
-    if (T.modes & (1 << kModeLds)) k_spmv<R, kModeLds><<<nblk, 256, lds_ldsv, s>>>(T);
-    if (T.modes & (1 << kModeGlobalV)) k_spmv<R, kModeGlobalV><<<nblk, 256, lds_global, s>>>(T);
-    if (T.modes & (1 << kModeWide)) k_spmv<R, kModeWide><<<nblk, 256, lds_global, s>>>(T);
+    if (T.vmodes & (1 << kModeLds)) k_spmv<R, kModeLds><<<nblk, 256, lds_ldsv, s>>>(T);
+    if (T.vmodes & (1 << kModeGlobalV)) k_spmv<R, kModeGlobalV><<<nblk, 256, lds_global, s>>>(T);
+    if (T.vmodes & (1 << kModeWide)) k_spmv<R, kModeWide><<<nblk, 256, lds_global, s>>>(T);
 }
 
 template <int R>
@@ -2423,7 +2746,17 @@ void launch_spmv_t_tiles(const Tables &T, size_t lds, hipStream_t s) {
 
 // X v for every problem of the table (one launch per index mode present),
 // `group` segments per load group, at most `wg` workgroups per CU.
+// Problems with the one-pass pair (kPairBit) get both products here, in
+// k_spmv_pair (groups of 2 or 4 segments; 8 runs as 4).
 static void launch_spmv_all(const Tables &T, int group, int wg, hipStream_t s) {
+    if (T.tmodes & kPairBit) {
+        const size_t lp = capped_lds(spmv_pair_lds_bytes(std::min(T.max_g, kMaxPairGenes)), wg, T);
+        if (group == 2)
+            k_spmv_pair<2><<<(unsigned)T.num_blocks, 256, lp, s>>>(T);
+        else
+            k_spmv_pair<4><<<(unsigned)T.num_blocks, 256, lp, s>>>(T);
+    }
+    if (!T.vmodes) return;
     const size_t lv = capped_lds(spmv_lds_bytes(std::min(T.max_g, kMaxLdsV)), wg, T);
     const size_t lg = capped_lds(spmv_lds_bytes(kMaxLdsV + 1), wg, T);
     switch (group) {
@@ -2439,7 +2772,7 @@ static void launch_spmv_t_all(const Tables &T, int group, int wg, hipStream_t s)
     if (T.tmodes & kSortedBit)
         k_spmv_t_sorted<2><<<(unsigned)T.num_blocks, 256,
                              spmv_t_sorted_lds_bytes(std::min(T.max_g, kMaxSortedGenes)), s>>>(T);
-    if (!(T.tmodes & ~kSortedBit)) return;
+    if (!(T.tmodes & ~(kSortedBit | kPairBit))) return;
     const size_t lds = capped_lds(spmv_t_lds_bytes(T.max_g), wg, T);
     switch (group) {
         case 2: launch_spmv_t_tiles<2>(T, lds, s); break;
@@ -2524,6 +2857,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     k_reduce_columns<<<T.num_probs * (m + 1), 256, 0, s>>>(T);
     if (two_pass) k_reorth_apply<<<nblk, 256, sizeof(double) * (m + 1), s>>>(T);
     k_alpha<<<T.num_probs, 256, 0, s>>>(T);
+    if (T.tmodes & kPairBit) k_pair_last_column<<<nblk, 256, 0, s>>>(T);
     mark(2, 1);
     mark(1, 0);
     launch_spmv_t_all(T, T.spmvt_group, T.spmvt_wg, s);
@@ -2559,7 +2893,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
     else
         k_tall_gemm<false><<<nblk, 256, sizeof(double) * m * kGemmCols, s>>>(T);
     k_restart_v<<<T.num_probs * (unsigned)(ceil_div(g, 64) * ceil_div(m, 4)), 256, 0, s>>>(T);
-    k_restart_commit<<<T.num_probs, 64, 0, s>>>(T);
+    k_restart_commit<<<T.num_probs, 256, 0, s>>>(T);
     mark(4, 1);
     mark(5, 0);
     if (mfma)

@@ -68,7 +68,7 @@ inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 
 // Per-problem workspace: one allocation carved into these buffers.
 struct WsLayout {
-    size_t U0, U1, V0, V1, R0, R1, u32, P, red, sq, tpart, grp, B, sd, rem, vs, W, Jv,
+    size_t U0, U1, V0, V1, R0, R1, u32, P, red, sq, tpart, grp, B, sd, colsum, rem, vs, W, Jv,
         Ub, Sb, vw, vdot, vsq, indptr64, idx, data4, total;
     size_t idx_bytes;  // bytes per packed column index (2, or 4 when g > 65535)
     int64_t nnz4;  // nnz rounded up to whole quads of 4 elements
@@ -92,6 +92,13 @@ bool uses_sorted(int g, bool enabled) { return enabled && g <= bpca::kMaxSortedG
 bool uses_onepass(int64_t n, int g, int passes) {
     if (passes == 2 || n < 8 * (int64_t)g) return false;
     return passes == 1 || n >= bpca::kOnePassMinCells;
+}
+
+// Whether a one-pass problem forms X v and X^T u in one read of X
+// (kernels.hip, k_spmv_pair): 16-bit indices with v in LDS and g <=
+// kMaxPairGenes.  Such a problem needs no gene-sorted copy.
+bool uses_pair(int64_t n, int g, int passes, bool enabled) {
+    return enabled && g <= bpca::kMaxPairGenes && uses_onepass(n, g, passes);
 }
 
 WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz, bool sorted) {
@@ -121,6 +128,7 @@ WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz, bool sorted
     L.grp = take(sizeof(double) * ng * 2 * g);
     L.B = take(sizeof(double) * m * (m + 1));
     L.sd = take(sizeof(double) * g);
+    L.colsum = take(sizeof(double) * g);
     L.rem = take(sizeof(double) * g);
     L.vs = take(sizeof(float) * g);
     L.W = take(sizeof(double) * m * mm);
@@ -274,10 +282,17 @@ struct bpca_context {
     int64_t lds_per_cu = 0;                // LDS bytes of one compute unit
     int num_cus = 0;                       // compute units of the device
     bool sorted_xtu = true;                // gene-sorted X^T u for g <= kMaxSortedGenes
+    bool pair_products = true;             // X v and X^T u in one pass (one-pass problems, g <= kMaxPairGenes)
     int reorth_passes = 0;                 // 0 automatic, 1 one pass (n >= 8 g), 2 two passes
 };
 
 namespace {
+
+// Whether a problem of this context takes the gene-sorted copy for X^T u:
+// enabled, g <= kMaxSortedGenes, and X^T u not formed in the pass of X v.
+bool layout_sorted(const bpca_context *ctx, int64_t n, int g) {
+    return uses_sorted(g, ctx->sorted_xtu) && !uses_pair(n, g, ctx->reorth_passes, ctx->pair_products);
+}
 
 int pool_alloc(bpca_context *ctx, size_t bytes, void **out) {
     bytes = align_up(bytes);
@@ -378,7 +393,7 @@ void begin_call(bpca_context *ctx) {
 
 // The descriptor of problem `pb` whose workspace (layout L) starts at base.
 bpca::Prob make_prob(const bpca_problem *pb, const WsLayout &L, char *base,
-                     bool widen, int id, int passes) {
+                     bool widen, int id, int passes, bool pair) {
     const int64_t n = (int64_t)pb->num_cells;
     bpca::Prob d{};
     d.data = (const float *)(base + L.data4);
@@ -411,6 +426,7 @@ bpca::Prob make_prob(const bpca_problem *pb, const WsLayout &L, char *base,
     d.grp = (double *)(base + L.grp);
     d.B = (double *)(base + L.B);
     d.sd = (double *)(base + L.sd);
+    d.colsum = (double *)(base + L.colsum);
     d.rem = (double *)(base + L.rem);
     d.vs = (float *)(base + L.vs);
     d.W = (double *)(base + L.W);
@@ -421,6 +437,7 @@ bpca::Prob make_prob(const bpca_problem *pb, const WsLayout &L, char *base,
     d.vdot = (double *)(base + L.vdot);
     d.vsq = (double *)(base + L.vsq);
     d.onepass = uses_onepass(n, d.g, passes) ? 1 : 0;
+    d.pair = uses_pair(n, d.g, passes, pair) ? 1 : 0;
     d.tsorted = L.tsorted ? 1 : 0;
     if (L.tsorted) {
         d.tval = (const float *)(base + L.tval);
@@ -471,12 +488,13 @@ void engine_drop_graph(bpca_engine *E) {
 int engine_rebuild_tables(bpca_engine *E) {
     bpca_context *ctx = E->ctx;
     std::vector<int2> blocks, groups;
-    int max_g = 2, max_m = 1, modes = 0, tmodes = 0, rmodes = 0;
+    int max_g = 2, max_m = 1, modes = 0, vmodes = 0, tmodes = 0, rmodes = 0;
     for (int s = 0; s < (int)E->slots.size(); s++) {
         const Slot &sl = E->slots[s];
         if (!sl.used) continue;
         modes |= 1 << sl.desc.mode;
-        tmodes |= sl.desc.tsorted ? bpca::kSortedBit : 1 << sl.desc.mode;
+        if (!sl.desc.pair) vmodes |= 1 << sl.desc.mode;
+        tmodes |= sl.desc.pair ? bpca::kPairBit : sl.desc.tsorted ? bpca::kSortedBit : 1 << sl.desc.mode;
         rmodes |= sl.desc.onepass ? bpca::kReorthOneBit : bpca::kReorthTwoBit;
         for (int64_t b = 0; b < sl.desc.nb; b++) blocks.push_back(make_int2(s, (int)b));
         for (int64_t G = 0; G < sl.desc.ngrp; G++) groups.push_back(make_int2(s, (int)G));
@@ -510,6 +528,7 @@ int engine_rebuild_tables(bpca_engine *E) {
     E->T.max_g = max_g;
     E->T.max_m = max_m;
     E->T.modes = modes;
+    E->T.vmodes = vmodes;
     E->T.tmodes = tmodes;
     E->T.rmodes = rmodes;
     E->T.max_lds = (int64_t)ctx->max_lds;
@@ -787,6 +806,19 @@ int bpca_context_get_sorted_xtu(bpca_context *ctx, int32_t *enable) {
     return BPCA_OK;
 }
 
+int bpca_context_set_pair_products(bpca_context *ctx, int32_t enable) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    ctx->pair_products = enable != 0;
+    return BPCA_OK;
+}
+
+int bpca_context_get_pair_products(bpca_context *ctx, int32_t *enable) {
+    if (!ctx || !enable) return fail(BPCA_ERR_INVALID_ARGUMENT, "bad argument");
+    *enable = ctx->pair_products ? 1 : 0;
+    return BPCA_OK;
+}
+
 int bpca_context_set_reorth_passes(bpca_context *ctx, int32_t passes) {
     if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     if (passes < 0 || passes > 2)
@@ -962,7 +994,7 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
     HIP_TRY(hipStreamSynchronize(s));
     if (nnz < 0) return fail(BPCA_ERR_NONCANONICAL, "indptr[num_cells] is negative");
     const bool widen = pb->indptr_width == 4;
-    const WsLayout L = ws_layout(n, g, m, widen, nnz, uses_sorted(g, ctx->sorted_xtu));
+    const WsLayout L = ws_layout(n, g, m, widen, nnz, layout_sorted(ctx, n, g));
     void *ws = nullptr;
     BPCA_TRY(pool_alloc(ctx, L.total, &ws));
     {
@@ -1007,7 +1039,7 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
         pool_free(ctx, ws);
         return rc;
     }
-    const bpca::Prob d = make_prob(pb, L, (char *)ws, widen, slot, ctx->reorth_passes);
+    const bpca::Prob d = make_prob(pb, L, (char *)ws, widen, slot, ctx->reorth_passes, ctx->pair_products);
     char *base = (char *)ws;
     const std::vector<double> v0 = start_vector(pb->seed, g);
     hipError_t e = hipSuccess;
@@ -1111,7 +1143,7 @@ int bpca_engine_collect(bpca_engine *E, int32_t max_out, int32_t *slots,
             st.slot = i;
             st.smax = cur[i].smax;
             st.reorth_passes = cur[i].onepass ? 1 : 2;
-            st.reserved = 0;
+            st.pair_products = cur[i].pair;
         }
         if (slots) slots[nout] = i;
         nout++;
@@ -1305,7 +1337,7 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
         nnz = nnz32;
     }
     if (nnz < 0) return fail(BPCA_ERR_NONCANONICAL, "indptr[num_cells] is negative");
-    const uint64_t need = noalloc_bytes(n, g, m, nnz, indptr_width == 4, uses_sorted(g, ctx->sorted_xtu));
+    const uint64_t need = noalloc_bytes(n, g, m, nnz, indptr_width == 4, layout_sorted(ctx, n, g));
     if (need > kScratchBytes)
         return fail(BPCA_ERR_INVALID_ARGUMENT,
                     "the allocation-free solve needs " + std::to_string(need) +
@@ -1326,7 +1358,7 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
     off += kAlign;
     char *ws = base + off;
     const bool widen = indptr_width == 4;
-    const WsLayout L = ws_layout(n, g, m, widen, nnz, uses_sorted(g, ctx->sorted_xtu));
+    const WsLayout L = ws_layout(n, g, m, widen, nnz, layout_sorted(ctx, n, g));
     // The scratch keeps the last call's contents: clear the counters and the
     // quad padding past nnz of both packed copies.
     HIP_TRY(bpca::launch_zero(d_done, kAlign, s));
@@ -1353,7 +1385,7 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
     pb.tolerance = tolerance;
     pb.seed = seed;
     pb.PCs = d_PCs;
-    const bpca::Prob d = make_prob(&pb, L, ws, widen, 0, ctx->reorth_passes);
+    const bpca::Prob d = make_prob(&pb, L, ws, widen, 0, ctx->reorth_passes, ctx->pair_products);
     const std::vector<double> v0 = start_vector(seed, g);
     std::vector<int2> blocks((size_t)nb), groups((size_t)ngrp);
     for (int64_t b = 0; b < nb; b++) blocks[b] = make_int2(0, (int)b);
@@ -1377,7 +1409,8 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
     T.max_g = std::max(g, 2);
     T.max_m = std::max(m, 1);
     T.modes = 1 << d.mode;
-    T.tmodes = d.tsorted ? bpca::kSortedBit : 1 << d.mode;
+    T.vmodes = d.pair ? 0 : 1 << d.mode;
+    T.tmodes = d.pair ? bpca::kPairBit : d.tsorted ? bpca::kSortedBit : 1 << d.mode;
     T.rmodes = d.onepass ? bpca::kReorthOneBit : bpca::kReorthTwoBit;
     T.max_lds = (int64_t)ctx->max_lds;
     T.spmv_group = ctx->spmv_group;
@@ -1420,7 +1453,7 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
         stats->slot = 0;
         stats->smax = fin.smax;
         stats->reorth_passes = fin.onepass ? 1 : 2;
-        stats->reserved = 0;
+        stats->pair_products = fin.pair;
     }
     return BPCA_OK;
 }

@@ -172,6 +172,18 @@ class Context:
         _lib.check(_lib.lib().bpca_context_get_reorth_passes(self._h, ctypes.byref(a)))
         return int(a.value)
 
+    def set_pair_products(self, enable: bool) -> None:
+        """Form X v and X^T u of a Lanczos step from one read of X for
+        problems admitted from now on that reorthogonalise in one pass and
+        have at most 4096 genes (the default), or in two passes.  Results
+        agree to rounding."""
+        _lib.check(_lib.lib().bpca_context_set_pair_products(self._h, int(bool(enable))))
+
+    def pair_products(self) -> bool:
+        a = ctypes.c_int32(0)
+        _lib.check(_lib.lib().bpca_context_get_pair_products(self._h, ctypes.byref(a)))
+        return bool(a.value)
+
     def set_occupancy(self, spmv_wg: int, spmvt_wg: int) -> None:
         """At most this many workgroups per compute unit for X v / X^T u
         (0 = no cap; speed only)."""

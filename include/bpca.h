@@ -137,6 +137,17 @@ int bpca_context_get_sorted_xtu(bpca_context *ctx, int32_t *enable);
  * (a memory-traffic knob of this library). */
 int bpca_context_set_reorth_passes(bpca_context *ctx, int32_t passes);
 int bpca_context_get_reorth_passes(bpca_context *ctx, int32_t *passes);
+/* Whether problems admitted from now on that reorthogonalise in one pass,
+ * with num_genes <= 4096, form X v and X^T u of a Lanczos step from one read
+ * of X (1, the default; no gene-sorted copy is made for them) or in two
+ * passes (0).  The one-pass scheme applies X^T to the raw step vector less
+ * its mean, which the pass knows beforehand from the column sums of X, so
+ * the two products need not wait for the reorthogonalisation in between.
+ * Results agree with the two-pass products to rounding, not bit for bit; a
+ * problem keeps the choice it was admitted with.  No counterpart in the
+ * reference (a memory-traffic knob of this library). */
+int bpca_context_set_pair_products(bpca_context *ctx, int32_t enable);
+int bpca_context_get_pair_products(bpca_context *ctx, int32_t *enable);
 /* Bytes of device workspace a solve of this shape allocates. */
 uint64_t bpca_workspace_bytes(uint64_t num_cells, uint32_t num_genes,
                               uint32_t k, uint32_t subspace_size);
@@ -152,7 +163,7 @@ typedef struct bpca_solve_stats {
     int32_t slot;           /* engine slot the problem ran in */
     double smax;            /* largest Ritz value seen (convergence scale) */
     int32_t reorth_passes;  /* passes over U per Lanczos step (1 or 2) */
-    int32_t reserved;
+    int32_t pair_products;  /* 1: X v and X^T u formed in one read of X */
 } bpca_solve_stats;
 
 /* ------------------------------------------------------- single solve */
