@@ -145,7 +145,12 @@ struct Prob {
     // 16-bit indices and g <= kMaxPairGenes).  X^T is applied to
     // u - vmean, vmean = colsum . vs / n being the mean of X vs known before
     // the pass; k_vside_w moves the result to X^T (u - mean of u).
-    int32_t pair, pad1;
+    int32_t pair;
+    // Pair problems: X is packed with every row padded to whole quads
+    // (padding: value 0, gene g) after a zero quad at position 0, and each
+    // packed index holds gene << idx_shift (2: the byte offset of the gene's
+    // FP32 slot in LDS).  0 for other problems.
+    int32_t idx_shift;
     double *colsum;          // float64 g: column sums of X
     double vmean;            // colsum . vs / n for the current vs
     double alpha, inv_alpha, rnorm, smax;
@@ -213,10 +218,23 @@ size_t lds_bytes_needed(int g, int m);
 // both copies must hold nnz rounded up to 4 elements, the tail zeroed by the
 // caller.  Sets *flag when indptr does not start at 0, decreases or exceeds
 // nnz, or a row's indices are not strictly increasing in [0, g).
+// check_only: validate without writing data4 / idx.
 hipError_t launch_validate_pack(const float *data, const int32_t *indices,
                                 const void *indptr, int indptr_width, int64_t n,
                                 int64_t nnz, int g, float *data4, void *idx,
-                                int *flag, hipStream_t s);
+                                int *flag, hipStream_t s, bool check_only = false);
+// The packed copy of a pair problem's CSR (X v and X^T u in one pass; the
+// CSR must have passed launch_validate_pack with check_only): quad 0 of
+// data4 / idx holds zeros of gene g, and row r occupies positions
+// [poff[r], poff[r + 1]) (poff[0] = 4), its elements then zeros of gene g up
+// to a multiple of 4; idx holds gene << 2 (g <= 16383).  poff: n + 1 int64;
+// scratch: ceil(n / 1024) + 1 int64.  data4 and idx must hold
+// pad_packed_bound(nnz, n) elements.
+int64_t pad_packed_bound(int64_t nnz, int64_t n);
+hipError_t launch_pack_padded(const float *data, const int32_t *indices,
+                              const void *indptr, int indptr_width, int64_t n,
+                              int g, int64_t *poff, int64_t *scratch,
+                              float *data4, uint16_t *idx, hipStream_t s);
 // The gene-sorted copy of one problem's packed CSR (g <= kMaxSortedGenes,
 // 16-bit indices, int64 row offsets): per 256-row sub-block the entry counts
 // of each gene (tlens, row stride gl), then its entries ordered by (gene,

@@ -123,8 +123,9 @@ __global__ __launch_bounds__(256) void k_colstats(Tables T, int c0) {
     for (int64_t rr = 0; rr < nrows; rr++) {
         const int64_t b = indptr[r0 + rr], e = indptr[r0 + rr + 1];
         for (int64_t p = b + lane; p < e; p += kWave) {
-            const int c = (int)indices[p] - c0;
-            if (!kTiled || (c >= 0 && c < tw)) {
+            // Pair problems: indices hold gene << 2 and padding has gene g.
+            const int c = ((int)indices[p] >> pr.idx_shift) - c0;
+            if ((unsigned)c < (unsigned)tw) {
                 const float x = data[p];
                 acc[c] = kSquare ? fmaf(x, x, acc[c]) : acc[c] + x;
             }
@@ -372,13 +373,9 @@ __device__ __forceinline__ void spmv_reduce(const SegGroup<R, IX> &G,
 // log2 R exchanges instead of 6 R.  kDpp: exchanges at offsets up to 16 by
 // DPP moves and a swizzle (xor_lane) instead of LDS permutes, the same pairs.
 //
-// kPair: each finished row's total also goes to the wave's LDS table uv as
-// (float)(total - vmean), the centred value X^T u takes (uv[kSubRows] is the
-// sink slot of that store).
-template <int R, typename IX, bool kPair, bool kDpp>
-__device__ __forceinline__ void store_row_totals(float (&done)[R], const SegGroup<R, IX> &G,
-                                                 int lane, gfloat_w *u, gfloat_w *sink,
-                                                 float *uv, double vmean) {
+template <int R, bool kDpp>
+__device__ __forceinline__ void store_row_totals(float (&done)[R], const bool (&fin)[R], int row0,
+                                                 int lane, gfloat_w *u, gfloat_w *sink) {
     auto xchg = [&](float v, int off) -> float {
         if (kDpp && off <= 16) return __int_as_float(xor_lane(__float_as_int(v), off));
         return __shfl_xor(v, off);
@@ -399,12 +396,11 @@ __device__ __forceinline__ void store_row_totals(float (&done)[R], const SegGrou
     for (; off2 > 0; off2 >>= 1) done[0] += xchg(done[0], off2);
     unsigned finmask = 0;
 #pragma unroll
-    for (int q = 0; q < R; q++) finmask |= (unsigned)G.fin[q] << q;
+    for (int q = 0; q < R; q++) finmask |= (unsigned)fin[q] << q;
     const int q = lane / kLanesPerRow;
     const bool wr = (lane % kLanesPerRow) == 0 && ((finmask >> q) & 1u);
-    const int row = G.row[0] + __popc(finmask & ((1u << q) - 1u));
+    const int row = row0 + __popc(finmask & ((1u << q) - 1u));
     *(wr ? u + row : sink) = done[0];
-    if (kPair) uv[wr ? row : kSubRows] = (float)((double)done[0] - vmean);
 }
 
 template <int R, typename IX, bool GV>
@@ -429,7 +425,7 @@ __device__ __forceinline__ void spmv_reduce(const SegGroup<R, IX> &G,
         acc = G.fin[q] ? 0.0f : acc;
     }
     carry = acc;
-    store_row_totals<R, IX, false, false>(done, G, lane, u, sink, nullptr, 0.0);
+    store_row_totals<R, false>(done, G.fin, G.row[0], lane, u, sink);
 }
 
 }  // namespace
@@ -1041,16 +1037,22 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T, int c0) {
 
 namespace {
 
-// Segment groups of the pair kernel: form_group's stream, with two changes.
-// (1) A row of two segments is never started in a group's last slot: that
-// slot is left empty (no elements; its loads re-read the row's first quad)
+// Segment groups of the pair kernel.  A pair problem's rows start on whole
+// quads and are padded to whole quads (zeros of gene g), so a segment is a
+// run of whole quads: lane l loads quad q0 + l of its segment, or, past the
+// segment's end, the zero quad at position 0 -- no element of a loaded quad
+// needs a range test.  Two more changes from form_group: (1) a row of two
+// segments is never started in a group's last slot: that slot is left empty
 // and the row starts the next group, so every row of at most two segments
 // lies within one group and its total is known when the group's X^T part
-// runs.  (2) Per slot, whether its row has three or more segments (`lng`)
-// and the row's bounds, for the re-read of such rows.
+// runs; (2) per slot, whether its row has three or more segments (`lng`) and
+// the row's bounds, for the re-read of such rows.
 template <int R>
 struct PairGroup {
-    SegGroup<R, uint16_t> s;
+    f32x4 d[R];   // values
+    u32x2 ix[R];  // packed indices (gene << 2, 16 bits each)
+    int row[R];   // row of the segment (nrows past the last row)
+    bool fin[R];  // the segment ends row `row`
     bool lng[R];
     int rs[R], re[R];
 };
@@ -1062,10 +1064,9 @@ struct PairCursor {
 
 template <int R>
 __device__ __forceinline__ void form_group_pair(const int *off, int nrows, gquad *data,
-                                                IxQuad<uint16_t>::gvec *indices, int lane,
-                                                PairCursor &pc, PairGroup<R> &P) {
+                                                IxQuad<uint16_t>::gvec *indices, int zq,
+                                                int lane, PairCursor &pc, PairGroup<R> &P) {
     Cursor &cs = pc.c;
-    SegGroup<R, uint16_t> &G = P.s;
     int W[R + 1];  // ends of rows cs.r + 1 ... cs.r + R + 1
 #pragma unroll
     for (int i = 0; i <= R; i++) {
@@ -1080,18 +1081,14 @@ __device__ __forceinline__ void form_group_pair(const int *off, int nrows, gquad
         const bool hold = q == R - 1 && cs.p == pc.start && wend < cs.e && !lng && cs.r < nrows;
         const int se = hold ? cs.p : (wend < cs.e ? wend : cs.e);
         const bool fin = !hold && wend >= cs.e;
-        const int lastq = hold ? q0 : (se > 0 ? se - 1 : 0) >> 2;
-        G.b[q] = cs.p;
-        G.e[q] = se;
-        G.q0[q] = q0;
-        G.row[q] = cs.r;
-        G.fin[q] = fin && cs.r < nrows;
+        P.row[q] = cs.r;
+        P.fin[q] = fin && cs.r < nrows;
         P.lng[q] = lng;
         P.rs[q] = pc.start;
         P.re[q] = cs.e;
-        const int qi = q0 + lane < lastq ? q0 + lane : lastq;
-        G.c[q].d = __builtin_nontemporal_load(data + qi);
-        G.c[q].ix = __builtin_nontemporal_load(indices + qi);
+        const int qi = lane < ((se - cs.p) >> 2) ? q0 + lane : zq;
+        P.d[q] = __builtin_nontemporal_load(data + qi);
+        P.ix[q] = __builtin_nontemporal_load(indices + qi);
         if (fin) {
             cs.r = cs.r < nrows ? cs.r + 1 : nrows;
             cs.p = cs.e;
@@ -1105,103 +1102,93 @@ __device__ __forceinline__ void form_group_pair(const int *off, int nrows, gquad
     }
 }
 
-// The gene of each element of a group's quads, or g for elements outside
-// their segment: lds_v[g] is 0, so such an element enters X v's chain as
-// x * 0, and X^T u sends it to the accumulator's spare slot acc[g].  One
-// unsigned compare per element: position a0 + k lies in [b, e) iff
-// a0 - b + k < e - b.
-template <int R>
-__device__ __forceinline__ void pair_genes(const SegGroup<R, uint16_t> &G, int lane, int g,
-                                           int (&ci)[R][4]) {
-#pragma unroll
-    for (int q = 0; q < R; q++) {
-        const unsigned t = (unsigned)(4 * (G.q0[q] + lane) - G.b[q]);
-        const unsigned len = (unsigned)(G.e[q] - G.b[q]);
-#pragma unroll
-        for (int k = 0; k < 4; k++) ci[q][k] = t + k < len ? quad_index(G.c[q], k) : g;
-    }
+// LDS byte offset of element k's gene slot (padding and the zero quad: g).
+__device__ __forceinline__ int pair_off(const u32x2 &ix, int k) {
+    const unsigned h = k < 2 ? ix.x : ix.y;
+    return (int)((k & 1) ? (h >> 16) : (h & 0xffffu));
 }
 
-// X v part of a group: k_spmv's chain (the same products in the same
-// order; an element outside its segment adds x * 0 where k_spmv adds 0 * v),
-// the row totals to u; returns, per lane, the centred value
-// (float)(total - vmean) of the row whose total the lane holds.
+__device__ __forceinline__ float lds_at(const float *base, int byte_off) {
+    return *(const float *)((const char *)base + byte_off);
+}
+
+__device__ __forceinline__ float f4(const f32x4 &d, int k) {
+    return k == 0 ? d.x : k == 1 ? d.y : k == 2 ? d.z : d.w;
+}
+
+// X v part of a group: per row the chain of k_spmv's order over its quads
+// (padding adds 0 * v[g] = 0, v[g] being 0), the row totals to u; returns,
+// per lane, the centred value (float)(total - vmean) of the row whose total
+// the lane holds (store_row_totals' lane for the slot ending it).
 template <int R>
-__device__ __forceinline__ float pair_xv(const SegGroup<R, uint16_t> &G, const int (&ci)[R][4],
-                                         const float *lds_v, int lane, float &carry,
-                                         gfloat_w *u, gfloat_w *sink, double vmean) {
+__device__ __forceinline__ float pair_xv(const PairGroup<R> &P, const float *lds_v, int lane,
+                                         float &carry, gfloat_w *u, gfloat_w *sink,
+                                         double vmean) {
     float done[R];
     float acc = carry;
 #pragma unroll
     for (int q = 0; q < R; q++) {
         float vv[4];
 #pragma unroll
-        for (int k = 0; k < 4; k++) vv[k] = lds_v[ci[q][k]];
+        for (int k = 0; k < 4; k++) vv[k] = lds_at(lds_v, pair_off(P.ix[q], k));
 #pragma unroll
-        for (int k = 0; k < 4; k++) acc = fmaf(quad_value(G.c[q], k), vv[k], acc);
+        for (int k = 0; k < 4; k++) acc = fmaf(f4(P.d[q], k), vv[k], acc);
         done[q] = acc;
-        acc = G.fin[q] ? 0.0f : acc;
+        acc = P.fin[q] ? 0.0f : acc;
     }
     carry = acc;
-    store_row_totals<R, uint16_t, false, true>(done, G, lane, u, sink, nullptr, 0.0);
+    store_row_totals<R, true>(done, P.fin, P.row[0], lane, u, sink);
     return (float)((double)done[0] - vmean);
 }
 
-// acc[gene] += x * ur for the four elements of one lane's quad (genes from
-// pair_genes; a row's genes are distinct, so one update's live lanes touch
-// distinct genes).
-__device__ __forceinline__ void pair_xt_quad(const f32x4 &d, const int (&ci)[4], float ur,
+// acc[gene] += x * ur for the four elements of one lane's quad: a row's
+// genes are distinct, so one update's live lanes touch distinct genes;
+// padding updates the spare slot acc[g].
+__device__ __forceinline__ void pair_xt_quad(const f32x4 &d, const u32x2 &ix, float ur,
                                              float *acc) {
     float x[4];
+    int o[4];
 #pragma unroll
-    for (int k = 0; k < 4; k++) x[k] = acc[ci[k]];
-    acc[ci[0]] = fmaf(d.x, ur, x[0]);
-    acc[ci[1]] = fmaf(d.y, ur, x[1]);
-    acc[ci[2]] = fmaf(d.z, ur, x[2]);
-    acc[ci[3]] = fmaf(d.w, ur, x[3]);
+    for (int k = 0; k < 4; k++) {
+        o[k] = pair_off(ix, k);
+        x[k] = lds_at(acc, o[k]);
+    }
+#pragma unroll
+    for (int k = 0; k < 4; k++) *(float *)((char *)acc + o[k]) = fmaf(f4(d, k), ur, x[k]);
 }
 
-// The X^T part of a row of three or more segments, elements [rs, re)
-// (positions relative to the sub-block's first quad), re-read window by
-// window.
+// The X^T part of a row of three or more segments, positions [rs, re)
+// (relative to the sub-block's first quad), re-read window by window.
 __device__ __noinline__ void pair_xt_long_row(gquad *data, IxQuad<uint16_t>::gvec *indices,
-                                              int rs, int re, float ur, int lane,
-                                              int g, float *acc) {
-    const int lastq = (re - 1) >> 2;
-    for (int q0 = rs >> 2; q0 <= lastq; q0 += kWave) {
-        const int qi = q0 + lane < lastq ? q0 + lane : lastq;
-        Quad<uint16_t> c;
-        c.d = data[qi];
-        c.ix = indices[qi];
-        const unsigned t = (unsigned)(4 * (q0 + lane) - rs);
-        const unsigned len = (unsigned)(re - rs);
-        int ci[4];
-#pragma unroll
-        for (int k = 0; k < 4; k++) ci[k] = t + k < len ? quad_index(c, k) : g;
-        pair_xt_quad(c.d, ci, ur, acc);
+                                              int zq, int rs, int re, float ur, int lane,
+                                              float *acc) {
+    for (int p = rs; p < re; p += 4 * kWave) {
+        const int qi = p + 4 * lane < re ? (p >> 2) + lane : zq;
+        const f32x4 d = data[qi];
+        const u32x2 ix = indices[qi];
+        pair_xt_quad(d, ix, ur, acc);
     }
 }
 
 // The X^T part of group P, after its X v part: the row ending at slot f
-// has its centred total in lane f kWave / R of uc (store_row_totals' lane
-// for slot f); a slot that does not finish its row belongs to a row ending
-// in the next slot (two segments), to a row of three or more segments
-// (applied whole at its last slot), or is an empty last slot.
+// has its centred total in lane f kWave / R of uc; a slot that does not
+// finish its row belongs to a row ending in the next slot (two segments), to
+// a row of three or more segments (applied whole at its last slot), or is an
+// empty last slot.
 template <int R>
-__device__ __forceinline__ void pair_xt(const PairGroup<R> &P, const int (&ci)[R][4], float uc,
-                                        gquad *data, IxQuad<uint16_t>::gvec *indices,
-                                        int lane, int g, float *acc) {
-    const SegGroup<R, uint16_t> &G = P.s;
+__device__ __forceinline__ void pair_xt(const PairGroup<R> &P, float uc, gquad *data,
+                                        IxQuad<uint16_t>::gvec *indices, int zq, int lane,
+                                        float *acc) {
     constexpr int kLanesPerRow = kWave / R;
     const int ucb = __float_as_int(uc);
 #pragma unroll
     for (int q = 0; q < R; q++) {
-        const int f = G.fin[q] || q == R - 1 ? q : q + 1;   // the slot ending the row
+        const int f = P.fin[q] || q == R - 1 ? q : q + 1;   // the slot ending the row
         const float ur = __int_as_float(__builtin_amdgcn_readlane(ucb, f * kLanesPerRow));
         if (P.lng[q]) {
-            if (G.fin[q]) pair_xt_long_row(data, indices, P.rs[q], P.re[q], ur, lane, g, acc);
+            if (P.fin[q]) pair_xt_long_row(data, indices, zq, P.rs[q], P.re[q], ur, lane, acc);
         } else {
-            pair_xt_quad(G.c[q].d, ci[q], ur, acc);
+            pair_xt_quad(P.d[q], P.ix[q], ur, acc);
         }
     }
 }
@@ -1217,7 +1204,8 @@ size_t spmv_pair_lds_bytes(int g) {
 
 // One workgroup per data block, wave w on its 256-row sub-block; loads
 // non-temporal (X is read once per step).  The block's partials of
-// X^T (u - vmean) go to tpart as in k_spmv_t.
+// X^T (u - vmean) go to tpart as in k_spmv_t.  vs sits at LDS offset 0, so
+// an element's packed index is its v slot's address.
 template <int R>
 __global__ __launch_bounds__(256) void k_spmv_pair(Tables T) {
     extern __shared__ float lds[];
@@ -1243,25 +1231,23 @@ __global__ __launch_bounds__(256) void k_spmv_pair(Tables T) {
         const double vmean = pr.vmean;
         gquad *data = (gquad *)pr.data + q0;
         IxQuad<uint16_t>::gvec *indices = (IxQuad<uint16_t>::gvec *)pr.idx + q0;
+        const int zq = (int)-q0;   // the zero quad, relative to the sub-block's first
         gfloat_w *u = (gfloat_w *)(uj + r0);
         gfloat_w *sink = (gfloat_w *)(uj + pr.n);
         PairCursor pc{{0, uniform(off[0]), uniform(off[1])}, uniform(off[0])};
         PairGroup<R> ga, gb;
         float carry = 0.0f;
-        int ci[R][4];
-        form_group_pair<R>(off, nrows, data, indices, lane, pc, ga);
+        form_group_pair<R>(off, nrows, data, indices, zq, lane, pc, ga);
         for (;;) {
-            form_group_pair<R>(off, nrows, data, indices, lane, pc, gb);
+            form_group_pair<R>(off, nrows, data, indices, zq, lane, pc, gb);
             __builtin_amdgcn_sched_barrier(0);
-            pair_genes<R>(ga.s, lane, g, ci);
-            float uc = pair_xv<R>(ga.s, ci, lds_v, lane, carry, u, sink, vmean);
-            pair_xt<R>(ga, ci, uc, data, indices, lane, g, acc);
+            float uc = pair_xv<R>(ga, lds_v, lane, carry, u, sink, vmean);
+            pair_xt<R>(ga, uc, data, indices, zq, lane, acc);
             const bool b_last = pc.c.r >= nrows;  // gb finished the last row
-            form_group_pair<R>(off, nrows, data, indices, lane, pc, ga);
+            form_group_pair<R>(off, nrows, data, indices, zq, lane, pc, ga);
             __builtin_amdgcn_sched_barrier(0);
-            pair_genes<R>(gb.s, lane, g, ci);
-            uc = pair_xv<R>(gb.s, ci, lds_v, lane, carry, u, sink, vmean);
-            pair_xt<R>(gb, ci, uc, data, indices, lane, g, acc);
+            uc = pair_xv<R>(gb, lds_v, lane, carry, u, sink, vmean);
+            pair_xt<R>(gb, uc, data, indices, zq, lane, acc);
             if (b_last) break;
         }
     } else {
@@ -2907,7 +2893,7 @@ hipError_t launch_macro_step(const Tables &T, int *done_counter,
 
 // ------------------------------------------------------- validation etc.
 
-template <typename IP, typename IX>
+template <typename IP, typename IX, bool kWrite = true>
 __global__ __launch_bounds__(256) void k_validate_pack(
     const float *__restrict__ data, const int32_t *__restrict__ indices,
     const IP *__restrict__ indptr, int64_t n, int64_t nnz, int g,
@@ -2924,8 +2910,10 @@ __global__ __launch_bounds__(256) void k_validate_pack(
             const int32_t c = indices[p];
             const int32_t prev = p > b ? indices[p - 1] : -1;
             if (c < 0 || c >= g || c <= prev) bad = 1;
-            idx[p] = (IX)c;
-            data4[p] = data[p];
+            if (kWrite) {
+                idx[p] = (IX)c;
+                data4[p] = data[p];
+            }
         }
     }
     if (bad) atomicOr(flag, 1);
@@ -2934,9 +2922,16 @@ __global__ __launch_bounds__(256) void k_validate_pack(
 hipError_t launch_validate_pack(const float *data, const int32_t *indices,
                                 const void *indptr, int indptr_width, int64_t n,
                                 int64_t nnz, int g, float *data4, void *idx,
-                                int *flag, hipStream_t s) {
+                                int *flag, hipStream_t s, bool check_only) {
     dim3 grid((unsigned)ceil_div(n, 4));
     const bool wide = g > kMaxPacked16;
+    if (check_only) {
+        if (indptr_width == 8)
+            k_validate_pack<int64_t, uint16_t, false><<<grid, 256, 0, s>>>(data, indices, (const int64_t *)indptr, n, nnz, g, nullptr, nullptr, flag);
+        else
+            k_validate_pack<int32_t, uint16_t, false><<<grid, 256, 0, s>>>(data, indices, (const int32_t *)indptr, n, nnz, g, nullptr, nullptr, flag);
+        return hipGetLastError();
+    }
     if (indptr_width == 8 && wide)
         k_validate_pack<int64_t, uint32_t><<<grid, 256, 0, s>>>(data, indices, (const int64_t *)indptr, n, nnz, g, data4, (uint32_t *)idx, flag);
     else if (indptr_width == 8)
@@ -2945,6 +2940,141 @@ hipError_t launch_validate_pack(const float *data, const int32_t *indices,
         k_validate_pack<int32_t, uint32_t><<<grid, 256, 0, s>>>(data, indices, (const int32_t *)indptr, n, nnz, g, data4, (uint32_t *)idx, flag);
     else
         k_validate_pack<int32_t, uint16_t><<<grid, 256, 0, s>>>(data, indices, (const int32_t *)indptr, n, nnz, g, data4, (uint16_t *)idx, flag);
+    return hipGetLastError();
+}
+
+// ------------------------------------------ packed copy padded to quads
+
+namespace {
+
+// Exclusive prefix sum over a 256-thread block of int64 values, in thread
+// order; tmp holds 4 int64.  Returns the block total in *total.
+__device__ inline int64_t block_exclusive_scan(int64_t v, int64_t *tmp, int64_t *total) {
+    const int lane = threadIdx.x & (kWave - 1), w = threadIdx.x / kWave;
+    int64_t x = v;
+#pragma unroll
+    for (int off = 1; off < kWave; off <<= 1) {
+        const int64_t y = __shfl_up(x, off);
+        if (lane >= off) x += y;
+    }
+    if (lane == kWave - 1) tmp[w] = x;
+    __syncthreads();
+    int64_t before = 0;
+    for (int i = 0; i < w; i++) before += tmp[i];
+    *total = tmp[0] + tmp[1] + tmp[2] + tmp[3];
+    __syncthreads();
+    return before + x - v;
+}
+
+template <typename IP>
+__device__ __forceinline__ int64_t padded_len(const IP *indptr, int64_t r) {
+    return (((int64_t)indptr[r + 1] - (int64_t)indptr[r]) + 3) & ~(int64_t)3;
+}
+
+}  // namespace
+
+// Sum of the padded lengths of the 1024 rows of block b.
+template <typename IP>
+__global__ __launch_bounds__(256) void k_pad_block_sums(const IP *__restrict__ indptr, int64_t n,
+                                                        int64_t *__restrict__ bsum) {
+    __shared__ int64_t tmp[4];
+    int64_t v = 0;
+    for (int t = 0; t < 4; t++) {
+        const int64_t r = (int64_t)blockIdx.x * kRowBlock + 4 * threadIdx.x + t;
+        if (r < n) v += padded_len(indptr, r);
+    }
+    int64_t total;
+    (void)block_exclusive_scan(v, tmp, &total);
+    if (threadIdx.x == 0) bsum[blockIdx.x] = total;
+}
+
+// bsum[0..nb) -> exclusive prefix sums starting at 4 (the zero quad), in
+// place; bsum[nb] = the end of the last row.
+__global__ __launch_bounds__(256) void k_pad_scan_blocks(int64_t *__restrict__ bsum, int64_t nb) {
+    __shared__ int64_t tmp[4];
+    int64_t carry = 4;
+    for (int64_t b0 = 0; b0 < nb; b0 += 256) {
+        const int64_t b = b0 + threadIdx.x;
+        const int64_t v = b < nb ? bsum[b] : 0;
+        int64_t total;
+        const int64_t x = block_exclusive_scan(v, tmp, &total);
+        if (b < nb) bsum[b] = carry + x;
+        carry += total;
+    }
+    if (threadIdx.x == 0) bsum[nb] = carry;
+}
+
+// poff[r] for the rows of block b (4 consecutive rows a thread), and
+// poff[n].
+template <typename IP>
+__global__ __launch_bounds__(256) void k_pad_offsets(const IP *__restrict__ indptr, int64_t n,
+                                                     const int64_t *__restrict__ boff,
+                                                     int64_t *__restrict__ poff) {
+    __shared__ int64_t tmp[4];
+    int64_t len[4], v = 0;
+    for (int t = 0; t < 4; t++) {
+        const int64_t r = (int64_t)blockIdx.x * kRowBlock + 4 * threadIdx.x + t;
+        len[t] = r < n ? padded_len(indptr, r) : 0;
+        v += len[t];
+    }
+    int64_t total;
+    int64_t x = boff[blockIdx.x] + block_exclusive_scan(v, tmp, &total);
+    for (int t = 0; t < 4; t++) {
+        const int64_t r = (int64_t)blockIdx.x * kRowBlock + 4 * threadIdx.x + t;
+        if (r < n) poff[r] = x;
+        x += len[t];
+    }
+    if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) poff[n] = boff[gridDim.x];
+}
+
+// One wave per row: the row's elements at poff[r], indices as gene << 2,
+// then zeros of gene g to the next multiple of 4; block 0 also writes the
+// zero quad at position 0.
+template <typename IP>
+__global__ __launch_bounds__(256) void k_pack_padded(const float *__restrict__ data,
+                                                     const int32_t *__restrict__ indices,
+                                                     const IP *__restrict__ indptr, int64_t n, int g,
+                                                     const int64_t *__restrict__ poff,
+                                                     float *__restrict__ data4,
+                                                     uint16_t *__restrict__ idx) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const uint16_t pad = (uint16_t)(g << 2);
+    if (blockIdx.x == 0 && threadIdx.x < 4) {
+        data4[threadIdx.x] = 0.0f;
+        idx[threadIdx.x] = pad;
+    }
+    const int64_t row = (int64_t)blockIdx.x * 4 + wave_id();
+    if (row >= n) return;
+    const int64_t b = indptr[row], len = (int64_t)indptr[row + 1] - b;
+    const int64_t o = poff[row], plen = poff[row + 1] - o;
+    for (int64_t t = lane; t < plen; t += kWave) {
+        const bool in = t < len;
+        data4[o + t] = in ? data[b + t] : 0.0f;
+        idx[o + t] = in ? (uint16_t)(indices[b + t] << 2) : pad;
+    }
+}
+
+int64_t pad_packed_bound(int64_t nnz, int64_t n) { return (4 + nnz + 3 * n + 3) / 4 * 4; }
+
+hipError_t launch_pack_padded(const float *data, const int32_t *indices,
+                              const void *indptr, int indptr_width, int64_t n,
+                              int g, int64_t *poff, int64_t *scratch,
+                              float *data4, uint16_t *idx, hipStream_t s) {
+    if (n <= 0 || (g << 2) > 0xffff) return hipErrorInvalidValue;
+    const int64_t nb = ceil_div(n, kRowBlock);
+    if (indptr_width == 8) {
+        const int64_t *ip = (const int64_t *)indptr;
+        k_pad_block_sums<<<(unsigned)nb, 256, 0, s>>>(ip, n, scratch);
+        k_pad_scan_blocks<<<1, 256, 0, s>>>(scratch, nb);
+        k_pad_offsets<<<(unsigned)nb, 256, 0, s>>>(ip, n, scratch, poff);
+        k_pack_padded<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(data, indices, ip, n, g, poff, data4, idx);
+    } else {
+        const int32_t *ip = (const int32_t *)indptr;
+        k_pad_block_sums<<<(unsigned)nb, 256, 0, s>>>(ip, n, scratch);
+        k_pad_scan_blocks<<<1, 256, 0, s>>>(scratch, nb);
+        k_pad_offsets<<<(unsigned)nb, 256, 0, s>>>(ip, n, scratch, poff);
+        k_pack_padded<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(data, indices, ip, n, g, poff, data4, idx);
+    }
     return hipGetLastError();
 }
 

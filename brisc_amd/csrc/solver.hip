@@ -101,7 +101,7 @@ bool uses_pair(int64_t n, int g, int passes, bool enabled) {
     return enabled && g <= bpca::kMaxPairGenes && uses_onepass(n, g, passes);
 }
 
-WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz, bool sorted) {
+WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz, bool sorted, bool pair) {
     const int64_t nb = bpca::ceil_div(n, bpca::kRowBlock);
     const int64_t ng = bpca::ceil_div(nb, bpca::kGroup);
     const int64_t mm = m + (m & 1);
@@ -138,8 +138,9 @@ WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz, bool sorted
     L.vw = take(sizeof(double) * g);
     L.vdot = take(sizeof(double) * (m + 1));
     L.vsq = take(sizeof(double) * bpca::ceil_div(g, 256));
-    L.indptr64 = widen ? take(sizeof(int64_t) * (n + 1)) : 0;
-    L.nnz4 = (nnz + 3) / 4 * 4;
+    // Pair problems: the padded row offsets; others: the widened caller's.
+    L.indptr64 = widen || pair ? take(sizeof(int64_t) * (n + 1)) : 0;
+    L.nnz4 = pair ? bpca::pad_packed_bound(nnz, n) : (nnz + 3) / 4 * 4;
     L.idx_bytes = g > bpca::kMaxPacked16 ? 4 : 2;
     L.idx = take(L.idx_bytes * L.nnz4);
     L.data4 = take(sizeof(float) * L.nnz4);
@@ -223,10 +224,10 @@ namespace {
 
 // Scratch bytes of an allocation-free solve: descriptor, block and group
 // lists, counters, workspace.
-uint64_t noalloc_bytes(int64_t n, int g, int m, int64_t nnz, bool widen, bool sorted) {
+uint64_t noalloc_bytes(int64_t n, int g, int m, int64_t nnz, bool widen, bool sorted, bool pair) {
     const int64_t nb = bpca::ceil_div(n, bpca::kRowBlock), ng = bpca::ceil_div(nb, bpca::kGroup);
     return align_up(sizeof(bpca::Prob)) + align_up(sizeof(int2) * nb) + align_up(sizeof(int2) * ng) +
-           kAlign + ws_layout(n, g, m, widen, nnz, sorted).total;
+           kAlign + ws_layout(n, g, m, widen, nnz, sorted, pair).total;
 }
 
 // With `device` current: record the device address of its scratch copy.
@@ -290,8 +291,15 @@ namespace {
 
 // Whether a problem of this context takes the gene-sorted copy for X^T u:
 // enabled, g <= kMaxSortedGenes, and X^T u not formed in the pass of X v.
-bool layout_sorted(const bpca_context *ctx, int64_t n, int g) {
-    return uses_sorted(g, ctx->sorted_xtu) && !uses_pair(n, g, ctx->reorth_passes, ctx->pair_products);
+// Whether a problem of this context forms X v and X^T u in one pass (its
+// padded packed copy must stay below 2^31 quads).
+bool problem_pair(const bpca_context *ctx, int64_t n, int g, int64_t nnz) {
+    return uses_pair(n, g, ctx->reorth_passes, ctx->pair_products) &&
+           bpca::pad_packed_bound(nnz, n) / 4 < (int64_t)INT32_MAX;
+}
+
+bool layout_sorted(const bpca_context *ctx, int64_t n, int g, int64_t nnz) {
+    return uses_sorted(g, ctx->sorted_xtu) && !problem_pair(ctx, n, g, nnz);
 }
 
 int pool_alloc(bpca_context *ctx, size_t bytes, void **out) {
@@ -394,11 +402,13 @@ void begin_call(bpca_context *ctx) {
 // The descriptor of problem `pb` whose workspace (layout L) starts at base.
 bpca::Prob make_prob(const bpca_problem *pb, const WsLayout &L, char *base,
                      bool widen, int id, int passes, bool pair) {
+    // pair: the problem forms X v and X^T u in one pass (problem_pair), and L
+    // holds its padded packed copy.
     const int64_t n = (int64_t)pb->num_cells;
     bpca::Prob d{};
     d.data = (const float *)(base + L.data4);
     d.idx = (const void *)(base + L.idx);
-    d.indptr = widen ? (const int64_t *)(base + L.indptr64) : (const int64_t *)pb->indptr;
+    d.indptr = widen || pair ? (const int64_t *)(base + L.indptr64) : (const int64_t *)pb->indptr;
     d.PCs = pb->PCs;
     d.n = n;
     d.nb = bpca::ceil_div(n, bpca::kRowBlock);
@@ -437,7 +447,8 @@ bpca::Prob make_prob(const bpca_problem *pb, const WsLayout &L, char *base,
     d.vdot = (double *)(base + L.vdot);
     d.vsq = (double *)(base + L.vsq);
     d.onepass = uses_onepass(n, d.g, passes) ? 1 : 0;
-    d.pair = uses_pair(n, d.g, passes, pair) ? 1 : 0;
+    d.pair = pair ? 1 : 0;
+    d.idx_shift = pair ? 2 : 0;
     d.tsorted = L.tsorted ? 1 : 0;
     if (L.tsorted) {
         d.tval = (const float *)(base + L.tval);
@@ -917,8 +928,11 @@ uint64_t bpca_workspace_bytes(uint64_t num_cells, uint32_t num_genes,
     (void)k;
     // Without the CSR's nnz this counts the packed copies at 7% density.
     const int64_t nnz = (int64_t)((double)num_cells * num_genes * 0.07);
+    // Default context settings: one-pass problems with few genes take the
+    // pair products, other problems with few genes the gene-sorted copy.
+    const bool pair = uses_pair((int64_t)num_cells, (int)num_genes, 0, true);
     return ws_layout((int64_t)num_cells, (int)num_genes, (int)subspace_size,
-                     true, nnz, uses_sorted((int)num_genes, true)).total;
+                     true, nnz, uses_sorted((int)num_genes, true) && !pair, pair).total;
 }
 
 // ------------------------------------------------------------ engine ABI
@@ -994,7 +1008,8 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
     HIP_TRY(hipStreamSynchronize(s));
     if (nnz < 0) return fail(BPCA_ERR_NONCANONICAL, "indptr[num_cells] is negative");
     const bool widen = pb->indptr_width == 4;
-    const WsLayout L = ws_layout(n, g, m, widen, nnz, layout_sorted(ctx, n, g));
+    const bool pair = problem_pair(ctx, n, g, nnz);
+    const WsLayout L = ws_layout(n, g, m, widen, nnz, layout_sorted(ctx, n, g, nnz), pair);
     void *ws = nullptr;
     BPCA_TRY(pool_alloc(ctx, L.total, &ws));
     {
@@ -1003,17 +1018,18 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
         hipError_t e = hipMalloc((void **)&flag, sizeof(int));
         int h_flag = 0;
         if (e == hipSuccess) e = hipMemsetAsync(flag, 0, sizeof(int), s);
-        // Zero the quad padding past nnz in both packed copies.
-        if (e == hipSuccess && L.nnz4 > nnz)
+        // Zero the quad padding past nnz in both packed copies (pair
+        // problems: the padded copy is written after the check).
+        if (e == hipSuccess && L.nnz4 > nnz && !pair)
             e = hipMemsetAsync((char *)ws + L.idx + L.idx_bytes * nnz, 0,
                                L.idx_bytes * (L.nnz4 - nnz), s);
-        if (e == hipSuccess && L.nnz4 > nnz)
+        if (e == hipSuccess && L.nnz4 > nnz && !pair)
             e = hipMemsetAsync((char *)ws + L.data4 + sizeof(float) * nnz, 0,
                                sizeof(float) * (L.nnz4 - nnz), s);
         if (e == hipSuccess)
             e = bpca::launch_validate_pack(pb->data, pb->indices, pb->indptr, pb->indptr_width, n, nnz, g,
                                            (float *)((char *)ws + L.data4),
-                                           (char *)ws + L.idx, flag, s);
+                                           (char *)ws + L.idx, flag, s, pair);
         if (e == hipSuccess) e = hipMemcpyAsync(&h_flag, flag, sizeof(int), hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipStreamSynchronize(s);
         if (flag) (void)hipFree(flag);
@@ -1039,11 +1055,16 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
         pool_free(ctx, ws);
         return rc;
     }
-    const bpca::Prob d = make_prob(pb, L, (char *)ws, widen, slot, ctx->reorth_passes, ctx->pair_products);
+    const bpca::Prob d = make_prob(pb, L, (char *)ws, widen, slot, ctx->reorth_passes, pair);
     char *base = (char *)ws;
     const std::vector<double> v0 = start_vector(pb->seed, g);
     hipError_t e = hipSuccess;
-    if (widen) e = bpca::launch_widen_indptr((const int32_t *)pb->indptr, n + 1, (int64_t *)(base + L.indptr64), s);
+    if (pair)   // tpart (at least nb + 1 int64) is free until set-up
+        e = bpca::launch_pack_padded(pb->data, pb->indices, pb->indptr, pb->indptr_width, n, g,
+                                     (int64_t *)(base + L.indptr64), (int64_t *)(base + L.tpart),
+                                     (float *)(base + L.data4), (uint16_t *)(base + L.idx), s);
+    else if (widen)
+        e = bpca::launch_widen_indptr((const int32_t *)pb->indptr, n + 1, (int64_t *)(base + L.indptr64), s);
     if (e == hipSuccess && L.tsorted)
         e = bpca::launch_tsort_build(d.data, (const uint16_t *)d.idx, d.indptr, n, g, L.tgl, (float *)d.tval,
                                      (uint8_t *)d.trow, (uint16_t *)d.thb, L.thb_words, (uint16_t *)d.tlens, (int64_t)ctx->max_lds, s);
@@ -1337,7 +1358,8 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
         nnz = nnz32;
     }
     if (nnz < 0) return fail(BPCA_ERR_NONCANONICAL, "indptr[num_cells] is negative");
-    const uint64_t need = noalloc_bytes(n, g, m, nnz, indptr_width == 4, layout_sorted(ctx, n, g));
+    const bool pair = problem_pair(ctx, n, g, nnz);
+    const uint64_t need = noalloc_bytes(n, g, m, nnz, indptr_width == 4, layout_sorted(ctx, n, g, nnz), pair);
     if (need > kScratchBytes)
         return fail(BPCA_ERR_INVALID_ARGUMENT,
                     "the allocation-free solve needs " + std::to_string(need) +
@@ -1358,14 +1380,17 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
     off += kAlign;
     char *ws = base + off;
     const bool widen = indptr_width == 4;
-    const WsLayout L = ws_layout(n, g, m, widen, nnz, layout_sorted(ctx, n, g));
+    const WsLayout L = ws_layout(n, g, m, widen, nnz, layout_sorted(ctx, n, g, nnz), pair);
     // The scratch keeps the last call's contents: clear the counters and the
-    // quad padding past nnz of both packed copies.
+    // quad padding past nnz of both packed copies (pair problems: the padded
+    // copy is written whole after the check).
     HIP_TRY(bpca::launch_zero(d_done, kAlign, s));
-    HIP_TRY(bpca::launch_zero(ws + L.idx + L.idx_bytes * nnz, L.idx_bytes * (L.nnz4 - nnz), s));
-    HIP_TRY(bpca::launch_zero(ws + L.data4 + sizeof(float) * nnz, sizeof(float) * (L.nnz4 - nnz), s));
+    if (!pair) {
+        HIP_TRY(bpca::launch_zero(ws + L.idx + L.idx_bytes * nnz, L.idx_bytes * (L.nnz4 - nnz), s));
+        HIP_TRY(bpca::launch_zero(ws + L.data4 + sizeof(float) * nnz, sizeof(float) * (L.nnz4 - nnz), s));
+    }
     HIP_TRY(bpca::launch_validate_pack(d_data, d_indices, d_indptr, indptr_width, n, nnz, g,
-                                       (float *)(ws + L.data4), ws + L.idx, d_flag, s));
+                                       (float *)(ws + L.data4), ws + L.idx, d_flag, s, pair));
     int h_flag = 0;
     HIP_TRY(hipMemcpyAsync(&h_flag, d_flag, sizeof(int), hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -1385,12 +1410,17 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
     pb.tolerance = tolerance;
     pb.seed = seed;
     pb.PCs = d_PCs;
-    const bpca::Prob d = make_prob(&pb, L, ws, widen, 0, ctx->reorth_passes, ctx->pair_products);
+    const bpca::Prob d = make_prob(&pb, L, ws, widen, 0, ctx->reorth_passes, pair);
     const std::vector<double> v0 = start_vector(seed, g);
     std::vector<int2> blocks((size_t)nb), groups((size_t)ngrp);
     for (int64_t b = 0; b < nb; b++) blocks[b] = make_int2(0, (int)b);
     for (int64_t G = 0; G < ngrp; G++) groups[G] = make_int2(0, (int)G);
-    if (widen) HIP_TRY(bpca::launch_widen_indptr((const int32_t *)d_indptr, n + 1, (int64_t *)(ws + L.indptr64), s));
+    if (pair)
+        HIP_TRY(bpca::launch_pack_padded(d_data, d_indices, d_indptr, indptr_width, n, g,
+                                         (int64_t *)(ws + L.indptr64), (int64_t *)(ws + L.tpart),
+                                         (float *)(ws + L.data4), (uint16_t *)(ws + L.idx), s));
+    else if (widen)
+        HIP_TRY(bpca::launch_widen_indptr((const int32_t *)d_indptr, n + 1, (int64_t *)(ws + L.indptr64), s));
     if (L.tsorted)
         HIP_TRY(bpca::launch_tsort_build(d.data, (const uint16_t *)d.idx, d.indptr, n, g, L.tgl, (float *)d.tval,
                                          (uint8_t *)d.trow, (uint16_t *)d.thb, L.thb_words, (uint16_t *)d.tlens, (int64_t)ctx->max_lds, s));

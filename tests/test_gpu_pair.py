@@ -10,7 +10,7 @@ order; the products agree with the separate ones to rounding, not bit for
 bit.  Cases: a 2000-gene problem with a partial last data block, whole
 blocks with few genes, and rows of three or more 256-element segments (the
 path that re-reads a long row when its last segment is reached), mixed with
-short rows.  The pair path is bitwise reproducible run to run and alone
+short rows, and empty rows (single and in a run).  The pair path is bitwise reproducible run to run and alone
 versus in a batch beside a problem that takes the separate products.
 
 Tolerances: singular values within 1e-6 of the oracle and 2e-6 between the
@@ -42,9 +42,19 @@ def _dev(*arrays):
     return [torch.from_numpy(np.ascontiguousarray(a)).cuda() for a in arrays]
 
 
-def _workload(seed, n, g, dense_every=0):
+def _workload(seed, n, g, dense_every=0, empty_every=0):
     w = synth.make_workload(seed, n, num_types=8, num_genes=g, target_density=0.07)
     data, indices, indptr = oracle.generate_csr(w)
+    if empty_every:
+        # Every empty_every-th row emptied, and a run of 300 empty rows.
+        keep = np.ones(n, bool)
+        keep[::empty_every] = False
+        keep[1000:1300] = False
+        lens = np.diff(indptr) * keep
+        mask = np.repeat(keep, np.diff(indptr))
+        data, indices = data[mask], indices[mask]
+        indptr = np.zeros(n + 1, np.int32)
+        indptr[1:] = np.cumsum(lens)
     if not dense_every:
         return data, indices, indptr
     # Every dense_every-th row replaced by a row holding 40-100% of the genes
@@ -67,15 +77,16 @@ def _workload(seed, n, g, dense_every=0):
     return np.concatenate(rows_d), np.concatenate(rows_i), indptr
 
 
-@pytest.mark.parametrize('n,g,k,m,dense_every', [
-    (70001, 2000, 20, 50, 0),
-    (65536, 400, 12, 40, 0),
-    (70000, 3000, 15, 40, 37),
+@pytest.mark.parametrize('n,g,k,m,dense_every,empty_every', [
+    (70001, 2000, 20, 50, 0, 0),
+    (65536, 400, 12, 40, 0, 0),
+    (70000, 3000, 15, 40, 37, 0),
+    (66001, 1000, 10, 40, 0, 13),
 ])
-def test_pair_against_separate_products_and_oracle(n, g, k, m, dense_every):
-    data, indices, indptr = _workload(n + g, n, g, dense_every)
+def test_pair_against_separate_products_and_oracle(n, g, k, m, dense_every, empty_every):
+    data, indices, indptr = _workload(n + g, n, g, dense_every, empty_every)
     print(f'case pair products n={n} g={g} k={k} m={m} dense_every={dense_every} '
-          f'nnz={len(data)}', flush=True)
+          f'empty_every={empty_every} nnz={len(data)}', flush=True)
     d, i, p = _dev(data, indices, indptr)
     ctx1, ctx2 = Context(0), Context(0)
     ctx2.set_pair_products(False)
