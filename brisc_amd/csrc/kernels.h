@@ -73,6 +73,7 @@ constexpr int kTWindow = kWave * kTChunk;  // entries per wave step of that kern
 constexpr int kSortedBit = 1 << 3;  // Tables::tmodes: some problem uses the gene-sorted X^T u
 constexpr int kPairBit = 1 << 4;    // Tables::tmodes: some problem forms X v and X^T u in one pass
 constexpr int kMaxPairGenes = 4096; // X v and X^T u in one pass up to this g (LDS)
+constexpr int kPairGroup = 4;       // segments per load group of the pair kernel
 constexpr int kReorthTwoBit = 1;    // Tables::rmodes: some problem reorthogonalises in two passes
 constexpr int kReorthOneBit = 2;    //                 some problem in one pass
 constexpr int64_t kOnePassMinCells = 65536;  // automatic one-pass choice: n >= max(8 g, this)
@@ -147,10 +148,16 @@ struct Prob {
     // the pass; k_vside_w moves the result to X^T (u - mean of u).
     int32_t pair;
     // Pair problems: X is packed with every row padded to whole quads
-    // (padding: value 0, gene g) after a zero quad at position 0, and each
-    // packed index holds gene << idx_shift (2: the byte offset of the gene's
-    // FP32 slot in LDS).  0 for other problems.
+    // (padding: value 0, gene g) and a zero quad after the last row of each
+    // 256-row sub-block, and each packed index holds gene << idx_shift (2:
+    // the byte offset of the gene's FP32 slot in LDS).  0 for other
+    // problems.
     int32_t idx_shift;
+    // Pair problems: the segment table of each 256-row sub-block (built at
+    // admission, launch_pair_segments): sub-block s's segments are
+    // segs[segbase[s] .. segbase[s + 1]), a multiple of kPairGroup.
+    const uint16_t *segs;
+    const int64_t *segbase;
     double *colsum;          // float64 g: column sums of X
     double vmean;            // colsum . vs / n for the current vs
     double alpha, inv_alpha, rnorm, smax;
@@ -224,13 +231,24 @@ hipError_t launch_validate_pack(const float *data, const int32_t *indices,
                                 int64_t nnz, int g, float *data4, void *idx,
                                 int *flag, hipStream_t s, bool check_only = false);
 // The packed copy of a pair problem's CSR (X v and X^T u in one pass; the
-// CSR must have passed launch_validate_pack with check_only): quad 0 of
-// data4 / idx holds zeros of gene g, and row r occupies positions
-// [poff[r], poff[r + 1]) (poff[0] = 4), its elements then zeros of gene g up
-// to a multiple of 4; idx holds gene << 2 (g <= 16383).  poff: n + 1 int64;
+// CSR must have passed launch_validate_pack with check_only): row r
+// occupies positions [poff[r], poff[r + 1]) (poff[0] = 0), its elements then
+// zeros of gene g up to a multiple of 4 and, after the last row of each
+// 256-row sub-block, one more quad of them (the sub-block's zero quad); idx
+// holds gene << 2 (g <= 16383).  poff: n + 1 int64;
 // scratch: ceil(n / 1024) + 1 int64.  data4 and idx must hold
 // pad_packed_bound(nnz, n) elements.
 int64_t pad_packed_bound(int64_t nnz, int64_t n);
+// The pair kernel's segment table for a padded packed copy (row offsets
+// poff, n rows): per 256-row sub-block, its rows cut into segments of at
+// most 64 quads, one uint16 each (bits 0-6: quads, bit 7: the segment ends
+// its row, bit 8: its row has three or more segments), with an empty
+// segment before any two-segment row that would start a group's last slot,
+// and empty segments to a multiple of kPairGroup.  segbase: nsub + 1 int64
+// (nsub = ceil(n / 256)); segs: pair_segments_bound(n, poff[n]) entries.
+int64_t pair_segments_bound(int64_t n, int64_t padded_total);
+hipError_t launch_pair_segments(const int64_t *poff, int64_t n, int64_t *segbase,
+                                uint16_t *segs, hipStream_t s);
 hipError_t launch_pack_padded(const float *data, const int32_t *indices,
                               const void *indptr, int indptr_width, int64_t n,
                               int g, int64_t *poff, int64_t *scratch,

@@ -276,6 +276,19 @@ __device__ inline int64_t stage_row_offsets(const int64_t *__restrict__ indptr,
 
 __device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
+__device__ __forceinline__ uint64_t uniform64(uint64_t v) {
+    const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+    const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+    return ((uint64_t)hi << 32) | lo;
+}
+
+__device__ __forceinline__ int64_t uniform64(int64_t v) { return (int64_t)uniform64((uint64_t)v); }
+
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p) {
+    return (T *)(uintptr_t)uniform64((uint64_t)(uintptr_t)p);
+}
+
 // A wave walks its sub-block as a stream of segments: a row is cut into
 // windows of 64 quads (an empty row is one empty segment).  Segments are
 // taken R at a time; the next group's loads are issued before the current
@@ -1039,66 +1052,47 @@ namespace {
 
 // Segment groups of the pair kernel.  A pair problem's rows start on whole
 // quads and are padded to whole quads (zeros of gene g), so a segment is a
-// run of whole quads: lane l loads quad q0 + l of its segment, or, past the
-// segment's end, the zero quad at position 0 -- no element of a loaded quad
-// needs a range test.  Two more changes from form_group: (1) a row of two
-// segments is never started in a group's last slot: that slot is left empty
-// and the row starts the next group, so every row of at most two segments
-// lies within one group and its total is known when the group's X^T part
-// runs; (2) per slot, whether its row has three or more segments (`lng`) and
-// the row's bounds, for the re-read of such rows.
+// run of at most 64 whole quads: lane l loads quad q0 + l of its segment,
+// or, past the segment's end, the zero quad after the sub-block's last row
+// -- no element of a loaded quad needs a range test.  The segments come from the table built
+// at admission (launch_pair_segments): four 16-bit descriptors per group,
+// one scalar load, and the segment's first quad and row follow from the
+// quads and row ends before it.  A row of two segments never starts in a
+// group's last slot (the table puts an empty segment there), so every row of
+// at most two segments lies within one group and its total is known when
+// the group's X^T part runs; rows of three or more segments are flagged.
 template <int R>
 struct PairGroup {
     f32x4 d[R];   // values
     u32x2 ix[R];  // packed indices (gene << 2, 16 bits each)
     int row[R];   // row of the segment (nrows past the last row)
     bool fin[R];  // the segment ends row `row`
-    bool lng[R];
-    int rs[R], re[R];
+    bool lng[R];  // row `row` has three or more segments
 };
 
 struct PairCursor {
-    Cursor c;
-    int start;   // first position of row c.r
+    int q;     // first quad of the next segment (relative to the sub-block)
+    int row;   // row of the next segment
 };
 
+typedef const uint64_t __attribute__((address_space(4))) cu64;
+
 template <int R>
-__device__ __forceinline__ void form_group_pair(const int *off, int nrows, gquad *data,
+__device__ __forceinline__ void form_group_pair(uint64_t desc, gquad *data,
                                                 IxQuad<uint16_t>::gvec *indices, int zq,
                                                 int lane, PairCursor &pc, PairGroup<R> &P) {
-    Cursor &cs = pc.c;
-    int W[R + 1];  // ends of rows cs.r + 1 ... cs.r + R + 1
-#pragma unroll
-    for (int i = 0; i <= R; i++) {
-        const int r = cs.r + 2 + i;
-        W[i] = uniform(off[r < nrows ? r : nrows]);
-    }
 #pragma unroll
     for (int q = 0; q < R; q++) {
-        const int q0 = cs.p >> 2;
-        const int wend = 4 * (q0 + kWave);
-        const bool lng = cs.e > pc.start && ((cs.e - 1) >> 2) - (pc.start >> 2) >= 2 * kWave;
-        const bool hold = q == R - 1 && cs.p == pc.start && wend < cs.e && !lng && cs.r < nrows;
-        const int se = hold ? cs.p : (wend < cs.e ? wend : cs.e);
-        const bool fin = !hold && wend >= cs.e;
-        P.row[q] = cs.r;
-        P.fin[q] = fin && cs.r < nrows;
-        P.lng[q] = lng;
-        P.rs[q] = pc.start;
-        P.re[q] = cs.e;
-        const int qi = lane < ((se - cs.p) >> 2) ? q0 + lane : zq;
+        const unsigned dq = (unsigned)(desc >> (16 * q)) & 0xffffu;
+        const int nq = (int)(dq & 127u);
+        P.row[q] = pc.row;
+        P.fin[q] = (dq & 128u) != 0;
+        P.lng[q] = (dq & 256u) != 0;
+        const unsigned qi = (unsigned)(lane < nq ? pc.q + lane : zq);
         P.d[q] = __builtin_nontemporal_load(data + qi);
         P.ix[q] = __builtin_nontemporal_load(indices + qi);
-        if (fin) {
-            cs.r = cs.r < nrows ? cs.r + 1 : nrows;
-            cs.p = cs.e;
-            pc.start = cs.e;
-            cs.e = W[0];
-#pragma unroll
-            for (int i = 0; i < R; i++) W[i] = W[i + 1];
-        } else if (!hold) {
-            cs.p = wend;
-        }
+        pc.q += nq;
+        pc.row += P.fin[q] ? 1 : 0;
     }
 }
 
@@ -1163,7 +1157,7 @@ __device__ __noinline__ void pair_xt_long_row(gquad *data, IxQuad<uint16_t>::gve
                                               int zq, int rs, int re, float ur, int lane,
                                               float *acc) {
     for (int p = rs; p < re; p += 4 * kWave) {
-        const int qi = p + 4 * lane < re ? (p >> 2) + lane : zq;
+        const unsigned qi = (unsigned)(p + 4 * lane < re ? (p >> 2) + lane : zq);
         const f32x4 d = data[qi];
         const u32x2 ix = indices[qi];
         pair_xt_quad(d, ix, ur, acc);
@@ -1176,9 +1170,9 @@ __device__ __noinline__ void pair_xt_long_row(gquad *data, IxQuad<uint16_t>::gve
 // a row of three or more segments (applied whole at its last slot), or is an
 // empty last slot.
 template <int R>
-__device__ __forceinline__ void pair_xt(const PairGroup<R> &P, float uc, gquad *data,
-                                        IxQuad<uint16_t>::gvec *indices, int zq, int lane,
-                                        float *acc) {
+__device__ __forceinline__ void pair_xt(const PairGroup<R> &P, float uc, const int *off,
+                                        gquad *data, IxQuad<uint16_t>::gvec *indices, int zq,
+                                        int lane, float *acc) {
     constexpr int kLanesPerRow = kWave / R;
     const int ucb = __float_as_int(uc);
 #pragma unroll
@@ -1186,7 +1180,9 @@ __device__ __forceinline__ void pair_xt(const PairGroup<R> &P, float uc, gquad *
         const int f = P.fin[q] || q == R - 1 ? q : q + 1;   // the slot ending the row
         const float ur = __int_as_float(__builtin_amdgcn_readlane(ucb, f * kLanesPerRow));
         if (P.lng[q]) {
-            if (P.fin[q]) pair_xt_long_row(data, indices, zq, P.rs[q], P.re[q], ur, lane, acc);
+            if (P.fin[q])
+                pair_xt_long_row(data, indices, zq, uniform(off[P.row[q]]), uniform(off[P.row[q] + 1]),
+                                 ur, lane, acc);
         } else {
             pair_xt_quad(P.d[q], P.ix[q], ur, acc);
         }
@@ -1227,31 +1223,44 @@ __global__ __launch_bounds__(256) void k_spmv_pair(Tables T) {
     const int64_t q0 = nrows > 0 ? stage_row_offsets(pr.indptr, r0, nrows, lane, off) : 0;
     __syncthreads();
     float *uj = u_of_step(pr, pr.j);
-    if (nrows > 0 && off[nrows] > off[0]) {
+    if (nrows > 0) {
         const double vmean = pr.vmean;
-        gquad *data = (gquad *)pr.data + q0;
-        IxQuad<uint16_t>::gvec *indices = (IxQuad<uint16_t>::gvec *)pr.idx + q0;
-        const int zq = (int)-q0;   // the zero quad, relative to the sub-block's first
+        // Wave-uniform bases (scalar registers), so that every load is a
+        // scalar base plus a 32-bit lane offset.
+        gquad *data = uniform_ptr((gquad *)pr.data + q0);
+        IxQuad<uint16_t>::gvec *indices = uniform_ptr((IxQuad<uint16_t>::gvec *)pr.idx + q0);
+        const int zq = uniform(off[nrows] >> 2) - 1;   // the sub-block's zero quad
         gfloat_w *u = (gfloat_w *)(uj + r0);
         gfloat_w *sink = (gfloat_w *)(uj + pr.n);
-        PairCursor pc{{0, uniform(off[0]), uniform(off[1])}, uniform(off[0])};
+        const int64_t sub = (int64_t)bp.y * (kRowBlock / kSubRows) + w;
+        const int64_t sb = uniform64(pr.segbase[sub]);
+        const int ngroups = uniform((int)((pr.segbase[sub + 1] - sb) / R));
+        // R = 4 descriptors a group, read by scalar loads (constant address
+        // space, uniform address), each one group ahead of its use.
+        cu64 *sd = (cu64 *)uniform_ptr((const uint64_t *)(pr.segs + sb));
+        auto desc = [&](int gi) -> uint64_t {
+            const uint64_t v = sd[gi < ngroups ? gi : ngroups - 1];
+            return gi < ngroups ? v : 0ull;
+        };
+        PairCursor pc{0, 0};
         PairGroup<R> ga, gb;
         float carry = 0.0f;
-        form_group_pair<R>(off, nrows, data, indices, zq, lane, pc, ga);
-        for (;;) {
-            form_group_pair<R>(off, nrows, data, indices, zq, lane, pc, gb);
+        // Groups past the table are empty (no quads, no row ends).
+        uint64_t dn = desc(1);
+        form_group_pair<R>(desc(0), data, indices, zq, lane, pc, ga);
+        for (int gi = 1;; gi += 2) {
+            form_group_pair<R>(dn, data, indices, zq, lane, pc, gb);
+            dn = desc(gi + 1);
             __builtin_amdgcn_sched_barrier(0);
             float uc = pair_xv<R>(ga, lds_v, lane, carry, u, sink, vmean);
-            pair_xt<R>(ga, uc, data, indices, zq, lane, acc);
-            const bool b_last = pc.c.r >= nrows;  // gb finished the last row
-            form_group_pair<R>(off, nrows, data, indices, zq, lane, pc, ga);
+            pair_xt<R>(ga, uc, off, data, indices, zq, lane, acc);
+            form_group_pair<R>(dn, data, indices, zq, lane, pc, ga);
+            dn = desc(gi + 2);
             __builtin_amdgcn_sched_barrier(0);
             uc = pair_xv<R>(gb, lds_v, lane, carry, u, sink, vmean);
-            pair_xt<R>(gb, uc, data, indices, zq, lane, acc);
-            if (b_last) break;
+            pair_xt<R>(gb, uc, off, data, indices, zq, lane, acc);
+            if (gi + 1 >= ngroups) break;
         }
-    } else {
-        for (int q = lane; q < nrows; q += kWave) uj[r0 + q] = 0.0f;
     }
     __syncthreads();
     float *out = pr.tpart + (int64_t)bp.y * 2 * g;
@@ -2733,14 +2742,11 @@ void launch_spmv_t_tiles(const Tables &T, size_t lds, hipStream_t s) {
 // X v for every problem of the table (one launch per index mode present),
 // `group` segments per load group, at most `wg` workgroups per CU.
 // Problems with the one-pass pair (kPairBit) get both products here, in
-// k_spmv_pair (groups of 2 or 4 segments; 8 runs as 4).
+// k_spmv_pair (groups of kPairGroup segments, the table's).
 static void launch_spmv_all(const Tables &T, int group, int wg, hipStream_t s) {
     if (T.tmodes & kPairBit) {
         const size_t lp = capped_lds(spmv_pair_lds_bytes(std::min(T.max_g, kMaxPairGenes)), wg, T);
-        if (group == 2)
-            k_spmv_pair<2><<<(unsigned)T.num_blocks, 256, lp, s>>>(T);
-        else
-            k_spmv_pair<4><<<(unsigned)T.num_blocks, 256, lp, s>>>(T);
+        k_spmv_pair<kPairGroup><<<(unsigned)T.num_blocks, 256, lp, s>>>(T);
     }
     if (!T.vmodes) return;
     const size_t lv = capped_lds(spmv_lds_bytes(std::min(T.max_g, kMaxLdsV)), wg, T);
@@ -2966,9 +2972,12 @@ __device__ inline int64_t block_exclusive_scan(int64_t v, int64_t *tmp, int64_t 
     return before + x - v;
 }
 
+// Row r's length rounded up to whole quads, plus the sub-block's zero quad
+// after the last row of each 256-row sub-block.
 template <typename IP>
-__device__ __forceinline__ int64_t padded_len(const IP *indptr, int64_t r) {
-    return (((int64_t)indptr[r + 1] - (int64_t)indptr[r]) + 3) & ~(int64_t)3;
+__device__ __forceinline__ int64_t padded_len(const IP *indptr, int64_t r, int64_t n) {
+    const int64_t q = (((int64_t)indptr[r + 1] - (int64_t)indptr[r]) + 3) & ~(int64_t)3;
+    return q + ((r % kSubRows == kSubRows - 1 || r == n - 1) ? 4 : 0);
 }
 
 }  // namespace
@@ -2981,18 +2990,19 @@ __global__ __launch_bounds__(256) void k_pad_block_sums(const IP *__restrict__ i
     int64_t v = 0;
     for (int t = 0; t < 4; t++) {
         const int64_t r = (int64_t)blockIdx.x * kRowBlock + 4 * threadIdx.x + t;
-        if (r < n) v += padded_len(indptr, r);
+        if (r < n) v += padded_len(indptr, r, n);
     }
     int64_t total;
     (void)block_exclusive_scan(v, tmp, &total);
     if (threadIdx.x == 0) bsum[blockIdx.x] = total;
 }
 
-// bsum[0..nb) -> exclusive prefix sums starting at 4 (the zero quad), in
-// place; bsum[nb] = the end of the last row.
-__global__ __launch_bounds__(256) void k_pad_scan_blocks(int64_t *__restrict__ bsum, int64_t nb) {
+// bsum[0..nb) -> exclusive prefix sums starting at `start`, in place;
+// bsum[nb] = the total plus start.
+__global__ __launch_bounds__(256) void k_scan_in_place(int64_t *__restrict__ bsum, int64_t nb,
+                                                       int64_t start) {
     __shared__ int64_t tmp[4];
-    int64_t carry = 4;
+    int64_t carry = start;
     for (int64_t b0 = 0; b0 < nb; b0 += 256) {
         const int64_t b = b0 + threadIdx.x;
         const int64_t v = b < nb ? bsum[b] : 0;
@@ -3014,7 +3024,7 @@ __global__ __launch_bounds__(256) void k_pad_offsets(const IP *__restrict__ indp
     int64_t len[4], v = 0;
     for (int t = 0; t < 4; t++) {
         const int64_t r = (int64_t)blockIdx.x * kRowBlock + 4 * threadIdx.x + t;
-        len[t] = r < n ? padded_len(indptr, r) : 0;
+        len[t] = r < n ? padded_len(indptr, r, n) : 0;
         v += len[t];
     }
     int64_t total;
@@ -3028,8 +3038,8 @@ __global__ __launch_bounds__(256) void k_pad_offsets(const IP *__restrict__ indp
 }
 
 // One wave per row: the row's elements at poff[r], indices as gene << 2,
-// then zeros of gene g to the next multiple of 4; block 0 also writes the
-// zero quad at position 0.
+// then zeros of gene g to poff[r + 1] (the next multiple of 4, and the
+// sub-block's zero quad after its last row).
 template <typename IP>
 __global__ __launch_bounds__(256) void k_pack_padded(const float *__restrict__ data,
                                                      const int32_t *__restrict__ indices,
@@ -3039,10 +3049,6 @@ __global__ __launch_bounds__(256) void k_pack_padded(const float *__restrict__ d
                                                      uint16_t *__restrict__ idx) {
     const int lane = threadIdx.x & (kWave - 1);
     const uint16_t pad = (uint16_t)(g << 2);
-    if (blockIdx.x == 0 && threadIdx.x < 4) {
-        data4[threadIdx.x] = 0.0f;
-        idx[threadIdx.x] = pad;
-    }
     const int64_t row = (int64_t)blockIdx.x * 4 + wave_id();
     if (row >= n) return;
     const int64_t b = indptr[row], len = (int64_t)indptr[row + 1] - b;
@@ -3054,7 +3060,9 @@ __global__ __launch_bounds__(256) void k_pack_padded(const float *__restrict__ d
     }
 }
 
-int64_t pad_packed_bound(int64_t nnz, int64_t n) { return (4 + nnz + 3 * n + 3) / 4 * 4; }
+int64_t pad_packed_bound(int64_t nnz, int64_t n) {
+    return (nnz + 3 * n + 4 * ceil_div(n, kSubRows) + 3) / 4 * 4;
+}
 
 hipError_t launch_pack_padded(const float *data, const int32_t *indices,
                               const void *indptr, int indptr_width, int64_t n,
@@ -3065,16 +3073,71 @@ hipError_t launch_pack_padded(const float *data, const int32_t *indices,
     if (indptr_width == 8) {
         const int64_t *ip = (const int64_t *)indptr;
         k_pad_block_sums<<<(unsigned)nb, 256, 0, s>>>(ip, n, scratch);
-        k_pad_scan_blocks<<<1, 256, 0, s>>>(scratch, nb);
+        k_scan_in_place<<<1, 256, 0, s>>>(scratch, nb, 0);
         k_pad_offsets<<<(unsigned)nb, 256, 0, s>>>(ip, n, scratch, poff);
         k_pack_padded<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(data, indices, ip, n, g, poff, data4, idx);
     } else {
         const int32_t *ip = (const int32_t *)indptr;
         k_pad_block_sums<<<(unsigned)nb, 256, 0, s>>>(ip, n, scratch);
-        k_pad_scan_blocks<<<1, 256, 0, s>>>(scratch, nb);
+        k_scan_in_place<<<1, 256, 0, s>>>(scratch, nb, 0);
         k_pad_offsets<<<(unsigned)nb, 256, 0, s>>>(ip, n, scratch, poff);
         k_pack_padded<<<(unsigned)ceil_div(n, 4), 256, 0, s>>>(data, indices, ip, n, g, poff, data4, idx);
     }
+    return hipGetLastError();
+}
+
+// One thread per 256-row sub-block walks its rows (the pair kernel's rule:
+// a row of two segments is not started in a group's last slot) and counts
+// its segments (kWrite = false: into segbase[sub]) or writes them from
+// segbase[sub].
+template <bool kWrite>
+__global__ __launch_bounds__(256) void k_pair_segments(const int64_t *__restrict__ poff, int64_t n,
+                                                       int64_t *__restrict__ segbase,
+                                                       uint16_t *__restrict__ segs) {
+    const int64_t sub = (int64_t)blockIdx.x * 256 + threadIdx.x;
+    const int64_t nsub = (n + kSubRows - 1) / kSubRows;
+    if (sub >= nsub) return;
+    const int64_t r0 = sub * kSubRows;
+    const int nrows = (int)(n - r0 < kSubRows ? n - r0 : kSubRows);
+    const int64_t base = kWrite ? segbase[sub] : 0;
+    int64_t cnt = 0;
+    int slot = 0;
+    auto emit = [&](unsigned d) {
+        if (kWrite) segs[base + cnt] = (uint16_t)d;
+        cnt++;
+        slot = (slot + 1) & (kPairGroup - 1);
+    };
+    for (int r = 0; r < nrows; r++) {
+        // Quads of the row (the sub-block's zero quad follows its last row).
+        const int64_t nq = ((poff[r0 + r + 1] - poff[r0 + r]) >> 2) - (r == nrows - 1 ? 1 : 0);
+        const bool lng = nq > 2 * kWave;                              // three or more segments
+        if (nq > kWave && !lng && slot == kPairGroup - 1) emit(0);     // empty last slot
+        int64_t left = nq;
+        do {
+            const int64_t take = left < kWave ? left : kWave;
+            left -= take;
+            emit((unsigned)take | (left == 0 ? 128u : 0u) | (lng ? 256u : 0u));
+        } while (left > 0);
+    }
+    while (slot != 0) emit(0);
+    if (!kWrite) segbase[sub] = cnt;
+}
+
+int64_t pair_segments_bound(int64_t n, int64_t padded_total) {
+    // A row of q quads has ceil(q / 64) segments (one if empty), at most
+    // 1 + q / 64; empty slots before two-segment rows number at most q / 64
+    // over the rows; each sub-block adds at most kPairGroup - 1.
+    return n + 2 * (padded_total / 4 / kWave) + (kPairGroup - 1) * ceil_div(n, kSubRows) + kPairGroup;
+}
+
+hipError_t launch_pair_segments(const int64_t *poff, int64_t n, int64_t *segbase,
+                                uint16_t *segs, hipStream_t s) {
+    if (n <= 0) return hipErrorInvalidValue;
+    const int64_t nsub = ceil_div(n, kSubRows);
+    const unsigned grid = (unsigned)ceil_div(nsub, 256);
+    k_pair_segments<false><<<grid, 256, 0, s>>>(poff, n, segbase, segs);
+    k_scan_in_place<<<1, 256, 0, s>>>(segbase, nsub, 0);
+    k_pair_segments<true><<<grid, 256, 0, s>>>(poff, n, segbase, segs);
     return hipGetLastError();
 }
 

@@ -69,7 +69,7 @@ inline size_t align_up(size_t x) { return (x + kAlign - 1) / kAlign * kAlign; }
 // Per-problem workspace: one allocation carved into these buffers.
 struct WsLayout {
     size_t U0, U1, V0, V1, R0, R1, u32, P, red, sq, tpart, grp, B, sd, colsum, rem, vs, W, Jv,
-        Ub, Sb, vw, vdot, vsq, indptr64, idx, data4, total;
+        Ub, Sb, vw, vdot, vsq, indptr64, idx, data4, segbase, segs, total;
     size_t idx_bytes;  // bytes per packed column index (2, or 4 when g > 65535)
     int64_t nnz4;  // nnz rounded up to whole quads of 4 elements
     // Gene-sorted copy for X^T u (tsorted): values, row bytes, run-start
@@ -144,6 +144,11 @@ WsLayout ws_layout(int64_t n, int g, int m, bool widen, int64_t nnz, bool sorted
     L.idx_bytes = g > bpca::kMaxPacked16 ? 4 : 2;
     L.idx = take(L.idx_bytes * L.nnz4);
     L.data4 = take(sizeof(float) * L.nnz4);
+    if (pair) {
+        // The pair kernel's segment table (bound from the padded size).
+        L.segbase = take(sizeof(int64_t) * (bpca::ceil_div(n, bpca::kSubRows) + 1));
+        L.segs = take(sizeof(uint16_t) * bpca::pair_segments_bound(n, L.nnz4));
+    }
     L.tsorted = sorted;
     if (sorted) {
         // Lane chunks of kTChunk entries are loaded whole up to the chunk
@@ -449,6 +454,10 @@ bpca::Prob make_prob(const bpca_problem *pb, const WsLayout &L, char *base,
     d.onepass = uses_onepass(n, d.g, passes) ? 1 : 0;
     d.pair = pair ? 1 : 0;
     d.idx_shift = pair ? 2 : 0;
+    if (pair) {
+        d.segs = (const uint16_t *)(base + L.segs);
+        d.segbase = (const int64_t *)(base + L.segbase);
+    }
     d.tsorted = L.tsorted ? 1 : 0;
     if (L.tsorted) {
         d.tval = (const float *)(base + L.tval);
@@ -1063,7 +1072,10 @@ int bpca_engine_add(bpca_engine *E, const bpca_problem *pb, int32_t *slot_out) {
         e = bpca::launch_pack_padded(pb->data, pb->indices, pb->indptr, pb->indptr_width, n, g,
                                      (int64_t *)(base + L.indptr64), (int64_t *)(base + L.tpart),
                                      (float *)(base + L.data4), (uint16_t *)(base + L.idx), s);
-    else if (widen)
+    if (pair && e == hipSuccess)
+        e = bpca::launch_pair_segments((const int64_t *)(base + L.indptr64), n, (int64_t *)(base + L.segbase),
+                                       (uint16_t *)(base + L.segs), s);
+    if (!pair && widen)
         e = bpca::launch_widen_indptr((const int32_t *)pb->indptr, n + 1, (int64_t *)(base + L.indptr64), s);
     if (e == hipSuccess && L.tsorted)
         e = bpca::launch_tsort_build(d.data, (const uint16_t *)d.idx, d.indptr, n, g, L.tgl, (float *)d.tval,
@@ -1419,7 +1431,10 @@ int bpca_irlba_device_noalloc(bpca_context *ctx, const float *d_data,
         HIP_TRY(bpca::launch_pack_padded(d_data, d_indices, d_indptr, indptr_width, n, g,
                                          (int64_t *)(ws + L.indptr64), (int64_t *)(ws + L.tpart),
                                          (float *)(ws + L.data4), (uint16_t *)(ws + L.idx), s));
-    else if (widen)
+    if (pair)
+        HIP_TRY(bpca::launch_pair_segments((const int64_t *)(ws + L.indptr64), n, (int64_t *)(ws + L.segbase),
+                                           (uint16_t *)(ws + L.segs), s));
+    if (!pair && widen)
         HIP_TRY(bpca::launch_widen_indptr((const int32_t *)d_indptr, n + 1, (int64_t *)(ws + L.indptr64), s));
     if (L.tsorted)
         HIP_TRY(bpca::launch_tsort_build(d.data, (const uint16_t *)d.idx, d.indptr, n, g, L.tgl, (float *)d.tval,
