@@ -102,7 +102,7 @@ __device__ inline double vs_mean(const Prob &pr, double *tmp) {
 // the four sub-blocks combine as (w0 + w1) + (w2 + w3), whatever the tile.
 // IX is the width of the packed column indices (problems of the other
 // width return at once).
-template <bool kSquare, typename IX, bool kTiled>
+template <bool kSquare, typename IX>
 __global__ __launch_bounds__(256) void k_colstats(Tables T, int c0) {
     extern __shared__ float lds[];
     const int2 bp = T.blocks[blockIdx.x];
@@ -111,30 +111,32 @@ __global__ __launch_bounds__(256) void k_colstats(Tables T, int c0) {
     const int g = pr.g;
     if (c0 >= g) return;
     const int tw = g - c0 < kGeneTile ? g - c0 : kGeneTile;
+    const int gs = tw + 1;   // accumulator stride: a spare slot at [tw]
     const int lane = threadIdx.x & (kWave - 1);
     const int w = wave_id();
-    float *acc = lds + (size_t)w * tw;
-    for (int i = lane; i < tw; i += kWave) acc[i] = 0.0f;
+    float *acc = lds + (size_t)w * gs;
+    for (int i = lane; i < gs; i += kWave) acc[i] = 0.0f;
     int64_t r0, nrows;
     sub_block(pr.n, bp.y, w, r0, nrows);
     const int64_t *indptr = pr.indptr;
     const IX *indices = (const IX *)pr.idx;
     const float *data = pr.data;
+    const int shift = pr.idx_shift;
     for (int64_t rr = 0; rr < nrows; rr++) {
         const int64_t b = indptr[r0 + rr], e = indptr[r0 + rr + 1];
         for (int64_t p = b + lane; p < e; p += kWave) {
-            // Pair problems: indices hold gene << 2 and padding has gene g.
-            const int c = ((int)indices[p] >> pr.idx_shift) - c0;
-            if ((unsigned)c < (unsigned)tw) {
-                const float x = data[p];
-                acc[c] = kSquare ? fmaf(x, x, acc[c]) : acc[c] + x;
-            }
+            // Genes outside the tile (and pair problems' padding, gene g;
+            // their indices hold gene << 2) update the spare slot.
+            int c = ((int)indices[p] >> shift) - c0;
+            c = (unsigned)c < (unsigned)tw ? c : tw;
+            const float x = data[p];
+            acc[c] = kSquare ? fmaf(x, x, acc[c]) : acc[c] + x;
         }
     }
     __syncthreads();
     float *out = pr.tpart + (int64_t)bp.y * 2 * g + (kSquare ? g : 0) + c0;
     for (int i = threadIdx.x; i < tw; i += 256)
-        out[i] = (lds[i] + lds[tw + i]) + (lds[2 * tw + i] + lds[3 * tw + i]);
+        out[i] = (lds[i] + lds[gs + i]) + (lds[2 * gs + i] + lds[3 * gs + i]);
 }
 
 // grp[G*2g + i] = FP64 sum over the data blocks of group G, in block order.
@@ -1110,25 +1112,35 @@ __device__ __forceinline__ float f4(const f32x4 &d, int k) {
     return k == 0 ? d.x : k == 1 ? d.y : k == 2 ? d.z : d.w;
 }
 
-// X v part of a group: per row the chain of k_spmv's order over its quads
-// (padding adds 0 * v[g] = 0, v[g] being 0), the row totals to u; returns,
-// per lane, the centred value (float)(total - vmean) of the row whose total
-// the lane holds (store_row_totals' lane for the slot ending it).
+// X v part of a group: per lane and segment the FMA chain over the quad's
+// four elements in order (padding adds 0 * v[g] = 0, v[g] being 0); a row's
+// lane sum is its segments' chains added in order (one segment: the chain
+// itself), then the lanes meet in store_row_totals' butterfly and the row
+// totals go to u.  The segments' chains are independent of each other, so
+// they overlap.  Returns, per lane, the centred value (float)(total - vmean)
+// of the row whose total the lane holds (store_row_totals' lane for the
+// slot ending it).
 template <int R>
 __device__ __forceinline__ float pair_xv(const PairGroup<R> &P, const float *lds_v, int lane,
                                          float &carry, gfloat_w *u, gfloat_w *sink,
                                          double vmean) {
-    float done[R];
-    float acc = carry;
+    float part[R];
 #pragma unroll
     for (int q = 0; q < R; q++) {
         float vv[4];
 #pragma unroll
         for (int k = 0; k < 4; k++) vv[k] = lds_at(lds_v, pair_off(P.ix[q], k));
+        float a = f4(P.d[q], 0) * vv[0];
 #pragma unroll
-        for (int k = 0; k < 4; k++) acc = fmaf(f4(P.d[q], k), vv[k], acc);
-        done[q] = acc;
-        acc = P.fin[q] ? 0.0f : acc;
+        for (int k = 1; k < 4; k++) a = fmaf(f4(P.d[q], k), vv[k], a);
+        part[q] = a;
+    }
+    float done[R];
+    float acc = carry;
+#pragma unroll
+    for (int q = 0; q < R; q++) {
+        done[q] = acc + part[q];
+        acc = P.fin[q] ? 0.0f : done[q];
     }
     carry = acc;
     store_row_totals<R, true>(done, P.fin, P.row[0], lane, u, sink);
@@ -1167,26 +1179,28 @@ __device__ __noinline__ void pair_xt_long_row(gquad *data, IxQuad<uint16_t>::gve
 // The X^T part of group P, after its X v part: the row ending at slot f
 // has its centred total in lane f kWave / R of uc; a slot that does not
 // finish its row belongs to a row ending in the next slot (two segments), to
-// a row of three or more segments (applied whole at its last slot), or is an
-// empty last slot.
+// a row of three or more segments, or is an empty last slot.  The group's
+// segments go in slot order, a row of three or more segments adding x * 0
+// there; such a row is then applied whole, re-read, after the group's other
+// rows (a fixed order, set by the data).
 template <int R>
 __device__ __forceinline__ void pair_xt(const PairGroup<R> &P, float uc, const int *off,
                                         gquad *data, IxQuad<uint16_t>::gvec *indices, int zq,
                                         int lane, float *acc) {
     constexpr int kLanesPerRow = kWave / R;
     const int ucb = __float_as_int(uc);
+    float ur[R];
 #pragma unroll
     for (int q = 0; q < R; q++) {
         const int f = P.fin[q] || q == R - 1 ? q : q + 1;   // the slot ending the row
-        const float ur = __int_as_float(__builtin_amdgcn_readlane(ucb, f * kLanesPerRow));
-        if (P.lng[q]) {
-            if (P.fin[q])
-                pair_xt_long_row(data, indices, zq, uniform(off[P.row[q]]), uniform(off[P.row[q] + 1]),
-                                 ur, lane, acc);
-        } else {
-            pair_xt_quad(P.d[q], P.ix[q], ur, acc);
-        }
+        ur[q] = __int_as_float(__builtin_amdgcn_readlane(ucb, f * kLanesPerRow));
+        pair_xt_quad(P.d[q], P.ix[q], P.lng[q] ? 0.0f : ur[q], acc);
     }
+#pragma unroll
+    for (int q = 0; q < R; q++)
+        if (P.lng[q] && P.fin[q])
+            pair_xt_long_row(data, indices, zq, uniform(off[P.row[q]]), uniform(off[P.row[q] + 1]),
+                             ur[q], lane, acc);
 }
 
 }  // namespace
@@ -2706,7 +2720,7 @@ size_t lds_bytes_needed(int g, int m) {
     if (g <= kMaxSortedGenes) need = std::max(need, spmv_t_sorted_lds_bytes(g));
     if (g <= kMaxPairGenes) need = std::max(need, spmv_pair_lds_bytes(g));
     need = std::max(need, spmv_lds_bytes(g));
-    need = std::max(need, sizeof(float) * 4 * (size_t)std::min(g, kGeneTile));  // colstats
+    need = std::max(need, sizeof(float) * 4 * ((size_t)std::min(g, kGeneTile) + 1));  // colstats
     need = std::max(need, sizeof(double) * (size_t)m * kGemmCols);   // gemm
     need = std::max(need, sizeof(double) * (size_t)m + sizeof(int) * (size_t)m);
     return need;
@@ -2784,22 +2798,22 @@ hipError_t launch_spmv_variant(const Tables &T, int which, int group, int wg,
 
 hipError_t launch_setup(const Tables &T, hipStream_t s) {
     if (T.num_probs == 0) return hipSuccess;
-    const size_t lds4g = sizeof(float) * 4 * (size_t)std::min(T.max_g, kGeneTile);
+    const size_t lds4g = sizeof(float) * 4 * ((size_t)std::min(T.max_g, kGeneTile) + 1);
     const unsigned nblk = (unsigned)T.num_blocks;
     const bool narrow = (T.modes & ((1 << kModeLds) | (1 << kModeGlobalV))) != 0;
     const bool wide = (T.modes & (1 << kModeWide)) != 0;
     if (T.max_g <= kGeneTile) {   // one tile: no range test per element
-        k_colstats<false, uint16_t, false><<<nblk, 256, lds4g, s>>>(T, 0);
-        k_colstats<true, uint16_t, false><<<nblk, 256, lds4g, s>>>(T, 0);
+        k_colstats<false, uint16_t><<<nblk, 256, lds4g, s>>>(T, 0);
+        k_colstats<true, uint16_t><<<nblk, 256, lds4g, s>>>(T, 0);
     } else {
         for (int c0 = 0; c0 < T.max_g; c0 += kGeneTile) {
             if (narrow) {
-                k_colstats<false, uint16_t, true><<<nblk, 256, lds4g, s>>>(T, c0);
-                k_colstats<true, uint16_t, true><<<nblk, 256, lds4g, s>>>(T, c0);
+                k_colstats<false, uint16_t><<<nblk, 256, lds4g, s>>>(T, c0);
+                k_colstats<true, uint16_t><<<nblk, 256, lds4g, s>>>(T, c0);
             }
             if (wide) {
-                k_colstats<false, uint32_t, true><<<nblk, 256, lds4g, s>>>(T, c0);
-                k_colstats<true, uint32_t, true><<<nblk, 256, lds4g, s>>>(T, c0);
+                k_colstats<false, uint32_t><<<nblk, 256, lds4g, s>>>(T, c0);
+                k_colstats<true, uint32_t><<<nblk, 256, lds4g, s>>>(T, c0);
             }
         }
     }
