@@ -1091,8 +1091,8 @@ __device__ __forceinline__ void form_group_pair(uint64_t desc, gquad *data,
         P.fin[q] = (dq & 128u) != 0;
         P.lng[q] = (dq & 256u) != 0;
         const unsigned qi = (unsigned)(lane < nq ? pc.q + lane : zq);
-        P.d[q] = __builtin_nontemporal_load(data + qi);
-        P.ix[q] = __builtin_nontemporal_load(indices + qi);
+        P.d[q] = data[qi];
+        P.ix[q] = indices[qi];
         pc.q += nq;
         pc.row += P.fin[q] ? 1 : 0;
     }
