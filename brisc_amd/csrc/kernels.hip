@@ -1259,21 +1259,32 @@ __global__ __launch_bounds__(256) void k_spmv_pair(Tables T) {
         PairCursor pc{0, 0};
         PairGroup<R> ga, gb;
         float carry = 0.0f;
-        // Groups past the table are empty (no quads, no row ends).
-        uint64_t dn = desc(1);
+        // Three groups rotate: while one is reduced, the loads of the next
+        // two are in flight.  Groups past the table are empty (no quads, no
+        // row ends); the loop ends after reducing the table's last group.
+        PairGroup<R> gc;
+        uint64_t dn = desc(2);
         form_group_pair<R>(desc(0), data, indices, zq, lane, pc, ga);
-        for (int gi = 1;; gi += 2) {
-            form_group_pair<R>(dn, data, indices, zq, lane, pc, gb);
+        form_group_pair<R>(desc(1), data, indices, zq, lane, pc, gb);
+        for (int gi = 2;; gi += 3) {
+            form_group_pair<R>(dn, data, indices, zq, lane, pc, gc);
             dn = desc(gi + 1);
             __builtin_amdgcn_sched_barrier(0);
             float uc = pair_xv<R>(ga, lds_v, lane, carry, u, sink, vmean);
             pair_xt<R>(ga, uc, off, data, indices, zq, lane, acc);
+            if (gi - 1 >= ngroups) break;        // ga was group gi - 2
             form_group_pair<R>(dn, data, indices, zq, lane, pc, ga);
             dn = desc(gi + 2);
             __builtin_amdgcn_sched_barrier(0);
             uc = pair_xv<R>(gb, lds_v, lane, carry, u, sink, vmean);
             pair_xt<R>(gb, uc, off, data, indices, zq, lane, acc);
-            if (gi + 1 >= ngroups) break;
+            if (gi >= ngroups) break;            // gb was group gi - 1
+            form_group_pair<R>(dn, data, indices, zq, lane, pc, gb);
+            dn = desc(gi + 3);
+            __builtin_amdgcn_sched_barrier(0);
+            uc = pair_xv<R>(gc, lds_v, lane, carry, u, sink, vmean);
+            pair_xt<R>(gc, uc, off, data, indices, zq, lane, acc);
+            if (gi + 1 >= ngroups) break;        // gc was group gi
         }
     }
     __syncthreads();
