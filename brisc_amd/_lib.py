@@ -115,6 +115,7 @@ def lib() -> ctypes.CDLL:
                                  i32, p, p]),
         'bpca_test_log1pf': (i32, [p, p, i64, p]),
         'bpca_context_set_profiling': (i32, [p, i32]),
+        'bpca_context_set_profiled_classes': (i32, [p, ctypes.c_uint32]),
         'bpca_context_kernel_times': (i32, [p, p, p]),
         'bpca_context_reset_kernel_times': (i32, [p]),
         'bpca_irlba_f32': (i32, [p, p, p, p, p, p, i32, u64, u32, u32, u32,

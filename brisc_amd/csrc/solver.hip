@@ -275,6 +275,7 @@ struct bpca_context {
     // Kernel timing: with profiling on, macro steps are launched directly
     // with events around each kernel class on the engine's stream.
     bool profiling = false;
+    uint32_t prof_classes = ~0u;   // classes timed while profiling (bit per class)
     std::vector<hipEvent_t> events;
     size_t next_event = 0;
     struct Pending { int kind, a, b; };
@@ -375,6 +376,7 @@ int prof_record(bpca_context *ctx) {
 
 void prof_mark(void *user, int kind, int end) {
     bpca_context *ctx = (bpca_context *)user;
+    if (!((ctx->prof_classes >> kind) & 1u)) return;
     if (!end) {
         ctx->open_event[kind] = prof_record(ctx);
     } else {
@@ -908,6 +910,13 @@ int bpca_context_set_profiling(bpca_context *ctx, int enable) {
     if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
     std::lock_guard<std::recursive_mutex> guard(ctx->lock);
     ctx->profiling = enable != 0;
+    return BPCA_OK;
+}
+
+int bpca_context_set_profiled_classes(bpca_context *ctx, uint32_t mask) {
+    if (!ctx) return fail(BPCA_ERR_INVALID_ARGUMENT, "ctx is NULL");
+    std::lock_guard<std::recursive_mutex> guard(ctx->lock);
+    ctx->prof_classes = mask;
     return BPCA_OK;
 }
 

@@ -108,8 +108,15 @@ class Context:
             _lib.check(_lib.lib().bpca_context_set_stream(self._h, stream))
         return self._h
 
-    def set_profiling(self, enable: bool = True) -> None:
-        """Record HIP events around each kernel class of later solves."""
+    def set_profiling(self, enable: bool = True, classes=None) -> None:
+        """Record HIP events around each kernel class of later solves, or
+        only around the named classes (names of _lib.KERNEL_CLASSES)."""
+        mask = 0xFFFFFFFF
+        if classes is not None:
+            mask = 0
+            for c in classes:
+                mask |= 1 << _lib.KERNEL_CLASSES.index(c)
+        _lib.check(_lib.lib().bpca_context_set_profiled_classes(self._h, mask))
         _lib.check(_lib.lib().bpca_context_set_profiling(self._h,
                                                          int(enable)))
 

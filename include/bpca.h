@@ -90,6 +90,11 @@ enum bpca_kernel_class {
     BPCA_NUM_KERNEL_CLASSES = 7
 };
 int bpca_context_set_profiling(bpca_context *ctx, int enable);
+/* The classes timed while profiling is on: bit c for class c (default all).
+ * Each timed class adds two event records per macro step, and each record
+ * waits for the launches before it to finish, so timing fewer classes
+ * perturbs the solve less. */
+int bpca_context_set_profiled_classes(bpca_context *ctx, uint32_t mask);
 /* Accumulated milliseconds and launch counts per class since the last reset
  * (arrays of BPCA_NUM_KERNEL_CLASSES entries). */
 int bpca_context_kernel_times(bpca_context *ctx, double *total_ms,
