@@ -44,12 +44,19 @@ def _device_csr(seed, n, types, wide):
                torch.from_numpy(indptr.astype(np.int64 if wide else np.int32)).cuda())
 
 
-def test_noalloc_matches_allocating_solve_twice():
+@pytest.mark.parametrize('passes', [0, 1])
+def test_noalloc_matches_allocating_solve_twice(passes):
     import torch
     assert torch.cuda.is_available()
     ctx = Context(0)
+    # passes = 1: one pass over U per step for n >= 8 g, so the 2000-gene
+    # problems below also take X v and X^T u in one read of X (the padded
+    # copy and segment table live in the scratch too).
+    ctx.set_reorth_passes(passes)
     # First a larger problem, then a smaller one with other genes' counts.
-    for seed, n, k, m, wide in [(501, 20_000, 20, 60, False), (502, 6_000, 12, 40, True)]:
+    cases = [(501, 20_000, 20, 60, False), (502, 6_000, 12, 40, True)] if passes == 0 else \
+        [(503, 20_000, 20, 60, False), (504, 17_000, 12, 40, True)]
+    for seed, n, k, m, wide in cases:
         w, (d, i, p) = _device_csr(seed, n, 7, wide)
         print(f'case noalloc seed={seed} n={n} k={k} m={m} wide={wide}', flush=True)
         rc, P0, sv0, st0, used0 = _solve(True, ctx, d, i, p, n, w.num_genes, k, m)
@@ -59,6 +66,7 @@ def test_noalloc_matches_allocating_solve_twice():
         print('noalloc', st0.as_dict(), 'allocating', st1.as_dict(),
               'device bytes taken by the call: noalloc', used0, 'allocating', used1, flush=True)
         assert st0.converged == st1.converged == 1
+        assert st0.pair_products == st1.pair_products == (1 if passes == 1 else 0)
         assert st0.restarts == st1.restarts and st0.lanczos_steps == st1.lanczos_steps
         assert np.array_equal(P0.cpu().numpy().view(np.uint32), P1.cpu().numpy().view(np.uint32))
         assert np.array_equal(sv0, sv1)
