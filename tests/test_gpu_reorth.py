@@ -8,7 +8,8 @@ different orders, so they agree to rounding, not bit for bit; both are
 checked against the FP64 oracle (``oracle.fp64_pca``, the gene Gram).
 Cases: an odd cell count whose last data block is partial (the element
 path of the one-pass kernel), whole 1024-row blocks (its 16-byte path),
-the CSR X^T u path (``set_sorted_xtu(False)``), n = 8 g exactly, and a
+the CSR X^T u path (``set_sorted_xtu(False)``), n = 8 g exactly (all with
+X v and X^T u in separate passes, ``set_pair_products(False)``), and a
 problem with too few cells (n < 8 g) that must stay on two passes.  A
 one-pass problem is bitwise reproducible run to run and alone versus in a
 batch beside a two-pass problem.
@@ -62,6 +63,8 @@ def test_one_pass_against_two_passes_and_oracle(n, g, k, m, sorted_xtu):
     ctx1, ctx2 = Context(0), Context(0)
     for c in (ctx1, ctx2):
         c.set_sorted_xtu(sorted_xtu)
+        # The separate products (the pair products are tests/test_gpu_pair.py's).
+        c.set_pair_products(False)
     ctx1.set_reorth_passes(1)
     ctx2.set_reorth_passes(2)
     assert ctx1.reorth_passes() == 1 and ctx2.reorth_passes() == 2
