@@ -107,7 +107,7 @@ __global__ __launch_bounds__(256) void k_colstats(Tables T, int c0) {
     extern __shared__ float lds[];
     const int2 bp = T.blocks[blockIdx.x];
     Prob &pr = T.probs[bp.x];
-    if (pr.phase != PH_SETUP || (pr.mode == kModeWide) != (sizeof(IX) == 4)) return;
+    if (pr.phase != PH_SETUP || pr.pair || (pr.mode == kModeWide) != (sizeof(IX) == 4)) return;
     const int g = pr.g;
     if (c0 >= g) return;
     const int tw = g - c0 < kGeneTile ? g - c0 : kGeneTile;
@@ -137,6 +137,56 @@ __global__ __launch_bounds__(256) void k_colstats(Tables T, int c0) {
     float *out = pr.tpart + (int64_t)bp.y * 2 * g + (kSquare ? g : 0) + c0;
     for (int i = threadIdx.x; i < tw; i += 256)
         out[i] = (lds[i] + lds[gs + i]) + (lds[2 * gs + i] + lds[3 * gs + i]);
+}
+
+// Column sums of x and x^2 of a pair problem's data block in one read of its
+// padded copy (rows of whole quads, padding of gene g): wave w walks its
+// sub-block's rows in order, a quad a lane, and adds each element to the
+// (sum, sum of squares) pair of its gene in a private LDS vector (float2 a
+// gene, the spare pair at [g]); per gene each sum is the FP32 chain over
+// the sub-block's rows in order, and the four sub-blocks combine as
+// (w0 + w1) + (w2 + w3).
+__global__ __launch_bounds__(256) void k_colstats_pair(Tables T) {
+    extern __shared__ float lds[];
+    const int2 bp = T.blocks[blockIdx.x];
+    Prob &pr = T.probs[bp.x];
+    if (pr.phase != PH_SETUP || !pr.pair) return;
+    const int g = pr.g, gs = g + 1;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int w = wave_id();
+    float2 *acc = (float2 *)lds + (size_t)w * gs;
+    for (int i = lane; i < gs; i += kWave) acc[i] = make_float2(0.0f, 0.0f);
+    int64_t r0, nrows;
+    sub_block(pr.n, bp.y, w, r0, nrows);
+    const int64_t *indptr = pr.indptr;
+    const float4 *data = (const float4 *)pr.data;
+    const uint2 *idx = (const uint2 *)pr.idx;
+    for (int64_t rr = 0; rr < nrows; rr++) {
+        const int64_t b = indptr[r0 + rr] >> 2, e = indptr[r0 + rr + 1] >> 2;
+        for (int64_t q = b + lane; q < e; q += kWave) {
+            const float4 d = data[q];
+            const uint2 ix = idx[q];
+            const float x[4] = {d.x, d.y, d.z, d.w};
+            int c[4];
+            float2 a[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const unsigned h = k < 2 ? ix.x : ix.y;
+                c[k] = (int)(((k & 1) ? (h >> 16) : (h & 0xffffu)) >> 2);
+                a[k] = acc[c[k]];
+            }
+#pragma unroll
+            for (int k = 0; k < 4; k++) acc[c[k]] = make_float2(a[k].x + x[k], fmaf(x[k], x[k], a[k].y));
+        }
+    }
+    __syncthreads();
+    const float2 *A = (const float2 *)lds;
+    float *out = pr.tpart + (int64_t)bp.y * 2 * g;
+    for (int i = threadIdx.x; i < g; i += 256) {
+        const float2 a0 = A[i], a1 = A[gs + i], a2 = A[2 * gs + i], a3 = A[3 * gs + i];
+        out[i] = (a0.x + a1.x) + (a2.x + a3.x);
+        out[g + i] = (a0.y + a1.y) + (a2.y + a3.y);
+    }
 }
 
 // grp[G*2g + i] = FP64 sum over the data blocks of group G, in block order.
@@ -2730,6 +2780,7 @@ size_t lds_bytes_needed(int g, int m) {
     size_t need = spmv_t_lds_bytes(g);
     if (g <= kMaxSortedGenes) need = std::max(need, spmv_t_sorted_lds_bytes(g));
     if (g <= kMaxPairGenes) need = std::max(need, spmv_pair_lds_bytes(g));
+    if (g <= kMaxPairGenes) need = std::max(need, sizeof(float) * 8 * ((size_t)g + 1));  // pair colstats
     need = std::max(need, spmv_lds_bytes(g));
     need = std::max(need, sizeof(float) * 4 * ((size_t)std::min(g, kGeneTile) + 1));  // colstats
     need = std::max(need, sizeof(double) * (size_t)m * kGemmCols);   // gemm
@@ -2813,6 +2864,8 @@ hipError_t launch_setup(const Tables &T, hipStream_t s) {
     const unsigned nblk = (unsigned)T.num_blocks;
     const bool narrow = (T.modes & ((1 << kModeLds) | (1 << kModeGlobalV))) != 0;
     const bool wide = (T.modes & (1 << kModeWide)) != 0;
+    if (T.tmodes & kPairBit)
+        k_colstats_pair<<<nblk, 256, sizeof(float) * 8 * ((size_t)std::min(T.max_g, kMaxPairGenes) + 1), s>>>(T);
     if (T.max_g <= kGeneTile) {   // one tile: no range test per element
         k_colstats<false, uint16_t><<<nblk, 256, lds4g, s>>>(T, 0);
         k_colstats<true, uint16_t><<<nblk, 256, lds4g, s>>>(T, 0);
@@ -3062,9 +3115,11 @@ __global__ __launch_bounds__(256) void k_pad_offsets(const IP *__restrict__ indp
     if (blockIdx.x == gridDim.x - 1 && threadIdx.x == 0) poff[n] = boff[gridDim.x];
 }
 
-// One wave per row: the row's elements at poff[r], indices as gene << 2,
-// then zeros of gene g to poff[r + 1] (the next multiple of 4, and the
-// sub-block's zero quad after its last row).
+// One wave per row, one output quad a lane: the row's elements at poff[r]
+// (a multiple of 4), indices as gene << 2, then zeros of gene g to
+// poff[r + 1] (the next multiple of 4, and the sub-block's zero quad after
+// its last row); each lane writes its quad with one 16-byte and one 8-byte
+// store.
 template <typename IP>
 __global__ __launch_bounds__(256) void k_pack_padded(const float *__restrict__ data,
                                                      const int32_t *__restrict__ indices,
@@ -3073,15 +3128,24 @@ __global__ __launch_bounds__(256) void k_pack_padded(const float *__restrict__ d
                                                      float *__restrict__ data4,
                                                      uint16_t *__restrict__ idx) {
     const int lane = threadIdx.x & (kWave - 1);
-    const uint16_t pad = (uint16_t)(g << 2);
+    const unsigned pad = (unsigned)(g << 2);
     const int64_t row = (int64_t)blockIdx.x * 4 + wave_id();
     if (row >= n) return;
     const int64_t b = indptr[row], len = (int64_t)indptr[row + 1] - b;
     const int64_t o = poff[row], plen = poff[row + 1] - o;
-    for (int64_t t = lane; t < plen; t += kWave) {
-        const bool in = t < len;
-        data4[o + t] = in ? data[b + t] : 0.0f;
-        idx[o + t] = in ? (uint16_t)(indices[b + t] << 2) : pad;
+    f32x4 *dq = (f32x4 *)(data4 + o);
+    u32x2 *iq = (u32x2 *)(idx + o);
+    for (int64_t t = 4 * lane; t < plen; t += 4 * kWave) {
+        float x[4];
+        unsigned c[4];
+#pragma unroll
+        for (int k = 0; k < 4; k++) {
+            const bool in = t + k < len;
+            x[k] = in ? data[b + t + k] : 0.0f;
+            c[k] = in ? (unsigned)indices[b + t + k] << 2 : pad;
+        }
+        dq[t >> 2] = f32x4{x[0], x[1], x[2], x[3]};
+        iq[t >> 2] = u32x2{c[0] | (c[1] << 16), c[2] | (c[3] << 16)};
     }
 }
 
