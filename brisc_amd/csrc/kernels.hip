@@ -1234,9 +1234,9 @@ __device__ __noinline__ void pair_xt_long_row(gquad *data, IxQuad<uint16_t>::gve
 // there; such a row is then applied whole, re-read, after the group's other
 // rows (a fixed order, set by the data).
 template <int R>
-__device__ __forceinline__ void pair_xt(const PairGroup<R> &P, float uc, const int *off,
-                                        gquad *data, IxQuad<uint16_t>::gvec *indices, int zq,
-                                        int lane, float *acc) {
+__device__ __forceinline__ void pair_xt(const PairGroup<R> &P, float uc, const int64_t *rowpos,
+                                        int64_t pos0, gquad *data, IxQuad<uint16_t>::gvec *indices,
+                                        int zq, int lane, float *acc) {
     constexpr int kLanesPerRow = kWave / R;
     const int ucb = __float_as_int(uc);
     float ur[R];
@@ -1249,17 +1249,17 @@ __device__ __forceinline__ void pair_xt(const PairGroup<R> &P, float uc, const i
 #pragma unroll
     for (int q = 0; q < R; q++)
         if (P.lng[q] && P.fin[q])
-            pair_xt_long_row(data, indices, zq, uniform(off[P.row[q]]), uniform(off[P.row[q] + 1]),
-                             ur[q], lane, acc);
+            pair_xt_long_row(data, indices, zq, uniform((int)(rowpos[P.row[q]] - pos0)),
+                             uniform((int)(rowpos[P.row[q] + 1] - pos0)), ur[q], lane, acc);
 }
 
 }  // namespace
 
-// LDS of k_spmv_pair: vs (g floats and a zero), 4 wave accumulators of
-// g + 1 floats (a spare slot each), 4 slices of kSubRows + 1 offsets.
+// LDS of k_spmv_pair: vs (g floats and a zero) and 4 wave accumulators of
+// g + 1 floats (a spare slot each): 40 KB at g = 2000, four workgroups per
+// compute unit.
 size_t spmv_pair_lds_bytes(int g) {
-    return sizeof(float) * ((size_t)g + 1 + 4 * ((size_t)g + 1)) +
-           sizeof(int) * 4 * (size_t)(kSubRows + 1);
+    return sizeof(float) * ((size_t)g + 1 + 4 * ((size_t)g + 1));
 }
 
 // One workgroup per data block, wave w on its 256-row sub-block; loads
@@ -1278,13 +1278,18 @@ __global__ __launch_bounds__(256) void k_spmv_pair(Tables T) {
     float *lds_v = lds;
     float *accs = lds + gs;
     float *acc = accs + (size_t)w * gs;
-    int *off = (int *)(accs + 4 * (size_t)gs) + w * (kSubRows + 1);
     for (int i = threadIdx.x; i < gs; i += 256) lds_v[i] = i < g ? pr.vs[i] : 0.0f;
     for (int i = lane; i < gs; i += kWave) acc[i] = 0.0f;
     int64_t r0, nrows64;
     sub_block(pr.n, bp.y, w, r0, nrows64);
     const int nrows = (int)nrows64;
-    const int64_t q0 = nrows > 0 ? stage_row_offsets(pr.indptr, r0, nrows, lane, off) : 0;
+    // The sub-block's rows start on whole quads: positions below are
+    // relative to its first (the padded row offsets are read directly; only
+    // the zero quad's place and the rows of three or more segments need
+    // them).
+    const int64_t *rowpos = pr.indptr + r0;
+    const int64_t pos0 = nrows > 0 ? uniform64(rowpos[0]) : 0;
+    const int64_t q0 = pos0 >> 2;
     __syncthreads();
     float *uj = u_of_step(pr, pr.j);
     if (nrows > 0) {
@@ -1293,7 +1298,7 @@ __global__ __launch_bounds__(256) void k_spmv_pair(Tables T) {
         // scalar base plus a 32-bit lane offset.
         gquad *data = uniform_ptr((gquad *)pr.data + q0);
         IxQuad<uint16_t>::gvec *indices = uniform_ptr((IxQuad<uint16_t>::gvec *)pr.idx + q0);
-        const int zq = uniform(off[nrows] >> 2) - 1;   // the sub-block's zero quad
+        const int zq = (int)((uniform64(rowpos[nrows]) - pos0) >> 2) - 1;   // the sub-block's zero quad
         gfloat_w *u = (gfloat_w *)(uj + r0);
         gfloat_w *sink = (gfloat_w *)(uj + pr.n);
         const int64_t sub = (int64_t)bp.y * (kRowBlock / kSubRows) + w;
@@ -1321,19 +1326,19 @@ __global__ __launch_bounds__(256) void k_spmv_pair(Tables T) {
             dn = desc(gi + 1);
             __builtin_amdgcn_sched_barrier(0);
             float uc = pair_xv<R>(ga, lds_v, lane, carry, u, sink, vmean);
-            pair_xt<R>(ga, uc, off, data, indices, zq, lane, acc);
+            pair_xt<R>(ga, uc, rowpos, pos0, data, indices, zq, lane, acc);
             if (gi - 1 >= ngroups) break;        // ga was group gi - 2
             form_group_pair<R>(dn, data, indices, zq, lane, pc, ga);
             dn = desc(gi + 2);
             __builtin_amdgcn_sched_barrier(0);
             uc = pair_xv<R>(gb, lds_v, lane, carry, u, sink, vmean);
-            pair_xt<R>(gb, uc, off, data, indices, zq, lane, acc);
+            pair_xt<R>(gb, uc, rowpos, pos0, data, indices, zq, lane, acc);
             if (gi >= ngroups) break;            // gb was group gi - 1
             form_group_pair<R>(dn, data, indices, zq, lane, pc, gb);
             dn = desc(gi + 3);
             __builtin_amdgcn_sched_barrier(0);
             uc = pair_xv<R>(gc, lds_v, lane, carry, u, sink, vmean);
-            pair_xt<R>(gc, uc, off, data, indices, zq, lane, acc);
+            pair_xt<R>(gc, uc, rowpos, pos0, data, indices, zq, lane, acc);
             if (gi + 1 >= ngroups) break;        // gc was group gi
         }
     }
