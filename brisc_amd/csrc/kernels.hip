@@ -1083,22 +1083,17 @@ __global__ __launch_bounds__(256) void k_spmv_t(Tables T, int c0) {
 // so nothing in it waits for the reorthogonalisation: the mean of u = X vs
 // is colsum . vs / n before the pass (vmean), and X^T (u - vmean) differs
 // from X^T (u - mean) by (mean - vmean) colsum, which k_vside_w adds in
-// FP64.  Per row the kernel does what k_spmv and k_spmv_t do, on the same
-// loaded quads: the row total u (k_spmv's order of summation) goes to u32
-// and, centred, to a 256-entry LDS table; then every element of the row adds
-// x * (u - vmean) to the wave's FP32 accumulator of its gene, rows in order
-// (k_spmv_t's order of summation for every gene).
+// FP64.  Per row the kernel forms the row total u (to u32) and then adds
+// x * (u - vmean) for every element of the row to the wave's FP32
+// accumulator of its gene, on the same loaded quads.
 //
-// A row's X^T part needs its total, known once its last segment is reduced.
-// The X^T part of a group runs right after its X v part, so it has every row
-// that ends in the group; a row left open at the group's end has, if it has
-// at most two segments (up to 253 elements at least), one segment here and
-// its last one in the first slot of the next group: that segment is kept in
-// registers (`Deferred`) and applied first in the next group.  Rows of three
-// or more segments are skipped by the streamed path and, when their last
-// segment is reached, re-read whole from memory (the loads hit the cache
-// that just delivered them) and applied in place, so every gene still sees
-// its rows in order.
+// A row's X^T part needs its total, known once its last segment is reduced:
+// the X^T part of a group of four segments runs right after its X v part,
+// and the segment table never lets a row of two segments cross a group
+// boundary, so every such row is complete when its group's X^T part runs.
+// Rows of three or more segments add x * 0 in the stream and are re-read
+// whole from memory after their group (the loads hit the cache that just
+// delivered them).
 
 namespace {
 
